@@ -1,0 +1,2043 @@
+// avbd_oracle.cpp — CPU restatement of the reference AVBD rigid-body step.
+//
+// TEST INFRASTRUCTURE.  This is the checker the HIP step is compared with;
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it.  The product library (ornis_amd/csrc) neither includes nor links
+// anything under oracle/.
+//
+// Restated from the reference's Rust source, read as text (it cannot be
+// built here: no Rust toolchain).  Scope: Box / Sphere / HalfSpace shapes,
+// Ball joints, static / dynamic / kinematic bodies, sleep, TOI, contact
+// manifolds with persistent duals, rolling/torsion friction.
+//
+//   step (accumulator)      crates/physics/src/avbd/mod.rs:1212-1235
+//   step_inner              crates/physics/src/avbd/mod.rs:846-1143
+//   generate_pairs / merge  crates/physics/src/avbd/rows.rs:767-943
+//   discover_pair           crates/physics/src/avbd/rows.rs:432-756
+//   solve_body / solve_6x6  crates/physics/src/avbd/assembly.rs:25-903
+//   dual_update             crates/physics/src/avbd/joints_ext.rs:464-919
+//   sleep / joint wake      crates/physics/src/avbd/sleep.rs:1-175
+//   shape_distance et al.   crates/physics/src/collision/distance.rs:63-891
+//   box SAT / features      crates/physics/src/collision/distance_box.rs:21-147
+//
+// Two orders:
+//   ORDER_REFERENCE  the reference exactly: reverse-handle Gauss-Seidel sweep,
+//                    history-ordered pair vector (new pairs appended,
+//                    unseen pairs swap-removed), sequential TOI clamp.
+//   ORDER_COLOUR     the GPU step's order: the same arithmetic, with the pair
+//                    vector sorted by (a, b) after every merge, the sweep run
+//                    colour by colour of a deterministic greedy colouring,
+//                    and every TOI cast taken against the step-top poses
+//                    before any clamp is applied.
+// Within one body solve both orders sum the 6x6 rows in pair-vector order,
+// exactly as the reference does.
+//
+// Discovery and TOI candidate search use a uniform grid above a small body
+// count; the exact reference prefilters run on every candidate, so the
+// discovered set (and everything downstream) is identical to the O(n^2)
+// scan, which stays selectable for tests.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <unordered_map>
+#include <vector>
+
+#include "oracle_math.hpp"
+
+namespace orc {
+
+enum Shape : int { SH_SPHERE = 0, SH_BOX = 1, SH_HALFSPACE = 2 };
+enum BodyType : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
+enum Order : int { ORDER_REFERENCE = 0, ORDER_COLOUR = 1 };
+
+// Reference constants, crates/physics/src/avbd/mod.rs:154-247, :1205-1210;
+// crates/physics/src/constants.rs; crates/gpu-abi/src/lib.rs:52.
+#define CF(x) T(x##f)
+
+template <class T>
+struct K {
+  static T ALPHA() { return CF(0.99); }
+  static T GAMMA() { return CF(0.999); }
+  static T BETA() { return CF(10000.0); }
+  static T MARGIN() { return CF(0.01); }
+  static T PEN_MIN() { return CF(1.0); }
+  static T PEN_MAX() { return CF(1.0e10); }
+  static T PEN_INIT() { return CF(1.0); }
+  static T JOINT_PEN_INIT() { return CF(1.0); }
+  static T GEN_MARGIN() { return CF(0.05); }
+  static T EXPAND_SLOP() { return CF(0.005); }
+  static T POINT_MATCH() { return CF(0.03); }
+  static T LAMBDA_MATCH() { return CF(0.05); }
+  static T C_EPS() { return CF(1e-7); }
+  static T STICK() { return CF(0.00001); }
+  static T SLEEP_LIN() { return CF(0.15); }
+  static T SLEEP_ANG() { return CF(0.15); }
+  static T HALF() { return CF(0.5); }
+  static T SLEEP_TIME() { return CF(0.5); }
+  static T WAKE_IMPACT() { return CF(0.5); }
+  static T WAKE_PEN() { return CF(0.01); }
+  static T ROLL_PEN() { return CF(100.0); }
+  static T MIN_TOI_LEN() { return CF(1e-9); }
+  static T SPIN_REST() { return CF(1e-9); }
+  static T CCD_BOUNCE() { return CF(1.0); }
+  static T DT() { return T(1.0f / 60.0f); }
+  static T NEAR_ZERO() { return CF(1e-9); }
+  static T AXIS_REST_LEN2() { return CF(1e-16); }
+  static T DEGENERATE() { return CF(1e-12); }
+  static T SHAPE_TOUCH() { return CF(1e-3); }
+  static T CCD_GATE() { return CF(0.5); }
+  static T JOINT_REST() { return CF(0.005); }
+  static T TANGENT_DOT() { return CF(0.9); }
+  static T FACE_AXIS_EPS() { return CF(1e-6); }
+  static T SPHERE_I() { return CF(0.4); }
+  static T BOX_I_DIV() { return CF(12.0); }
+};
+static const int MAX_POINTS = 8;
+static const int MAX_SUBSTEPS = 4;
+
+template <class T>
+struct Body {
+  V3<T> pos;
+  Q<T> rot;
+  V3<T> vel, angvel, torque;
+  T mass, inv_mass;
+  V3<T> inertia;
+  T restitution, friction, rolling, torsion;
+  int shape;
+  V3<T> sp;  // sphere: (r, 0, 0); box: half extents; half-space: unit normal
+  uint32_t layer, mask;
+  bool trigger;
+  int type;
+};
+
+template <class T>
+struct Point {
+  V3<T> ra, rb;
+  T lam[3], pen[3];
+  bool stuck;
+  T roll[3];
+};
+
+template <class T>
+struct Pair {
+  int a, b;
+  V3<T> n;
+  T mu[2];
+  T gap;
+  std::vector<Point<T>> pts;
+};
+
+template <class T>
+struct Joint {
+  int a, b;
+  V3<T> la, lb;
+  T lam_l[3], pen_l[3];
+};
+
+template <class T>
+struct Dist {
+  T dist;
+  V3<T> pa, pb;
+};
+
+template <class T>
+struct SRef {
+  int shape;
+  V3<T> sp;
+  V3<T> pos;
+  Q<T> rot;
+};
+
+// ---------------------------------------------------------------------------
+// Shape helpers (crates/physics/src/collision/shape.rs, body.rs, rows.rs)
+// ---------------------------------------------------------------------------
+
+// Shape::inertia for the dynamic arms (shape.rs:713-767): sphere
+// 0.4*m*r^2, box m/12*(y^2+z^2) per axis over full side lengths.
+template <class T>
+V3<T> shape_inertia(int shape, V3<T> sp, T mass) {
+  if (!(mass > T(0)) || !std::isfinite(mass)) return V3<T>{T(0), T(0), T(0)};
+  if (shape == SH_SPHERE) {
+    T v = K<T>::SPHERE_I() * mass * sp.x * sp.x;
+    return V3<T>{v, v, v};
+  }
+  if (shape == SH_BOX) {
+    V3<T> s = sp * T(2);
+    T x = s.x * s.x, y = s.y * s.y, z = s.z * s.z;
+    T k = mass / K<T>::BOX_I_DIV();
+    return V3<T>{k * (y + z), k * (z + x), k * (x + y)};
+  }
+  return V3<T>{T(0), T(0), T(0)};
+}
+
+// bound_radius (rows.rs:253-292).
+template <class T>
+T bound_radius(int shape, V3<T> sp) {
+  if (shape == SH_SPHERE) return sp.x;
+  if (shape == SH_BOX) return length(sp);
+  return std::numeric_limits<T>::infinity();
+}
+
+// shape_min_dimension (rows.rs:373-403).
+template <class T>
+T shape_min_dim(int shape, V3<T> sp) {
+  if (shape == SH_SPHERE) return sp.x;
+  if (shape == SH_BOX) return min_element(sp);
+  return std::numeric_limits<T>::infinity();
+}
+
+// tangent_basis (crates/physics/src/math.rs:152-160).
+template <class T>
+void tangent_basis(V3<T> n, V3<T>* t1, V3<T>* t2) {
+  V3<T> axis = std::fabs(n.x) < K<T>::TANGENT_DOT() ? V3<T>{T(1), T(0), T(0)} : V3<T>{T(0), T(1), T(0)};
+  *t1 = normalize_or(cross(n, axis), V3<T>{T(0), T(0), T(1)});
+  *t2 = cross(*t1, n);
+}
+
+// quat_mul (rows.rs:91-98).
+template <class T>
+Q<T> quat_mul(Q<T> a, Q<T> b) {
+  return Q<T>{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+
+// quat_integrate (rows.rs:110-119).
+template <class T>
+Q<T> quat_integrate(Q<T> q, V3<T> v, int mm) {
+  T theta = length(v);
+  if (theta < K<T>::NEAR_ZERO()) return q;
+  T s, c;
+  sincos_t(K<T>::HALF() * theta, &s, &c, mm);
+  T k = s / theta;
+  Q<T> dq = quat_mul(Q<T>{v.x * k, v.y * k, v.z * k, c}, q);
+  return qnormalize(dq);
+}
+
+// quat_diff_vec (rows.rs:128-146).
+template <class T>
+V3<T> quat_diff_vec(Q<T> a, Q<T> b, int mm) {
+  Q<T> r = quat_mul(a, Q<T>{-b.x, -b.y, -b.z, b.w});
+  V3<T> xyz{r.x, r.y, r.z};
+  if (r.w < T(0)) xyz = -xyz;
+  T s = fmin_(length(xyz), T(1));
+  if (s < K<T>::NEAR_ZERO()) return V3<T>{T(0), T(0), T(0)};
+  return xyz * (T(2) * asin_t(s, mm) / s);
+}
+
+// world_inertia (rows.rs:194-207).
+template <class T>
+void world_inertia(V3<T> inertia, Q<T> rot, T m[3][3]) {
+  M3c<T> r = mat3_from_quat(rot);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      T ca0 = r.c[0][a], ca1 = r.c[1][a], ca2 = r.c[2][a];
+      T cb0 = r.c[0][b], cb1 = r.c[1][b], cb2 = r.c[2][b];
+      m[a][b] = ca0 * inertia.x * cb0 + ca1 * inertia.y * cb1 + ca2 * inertia.z * cb2;
+    }
+}
+
+// inverse_symmetric (rows.rs:211-238).
+template <class T>
+void inverse_symmetric(const T m[3][3], V3<T> diag, T out[3][3]) {
+  T a = m[0][0], b = m[0][1], c = m[0][2];
+  T d = m[1][1], e = m[1][2], f = m[2][2];
+  T det = a * (d * f - e * e) - b * (b * f - c * e) + c * (b * e - c * d);
+  if (std::fabs(det) < K<T>::DEGENERATE()) {
+    T ix = diag.x > T(0) ? T(1) / diag.x : T(0);
+    T iy = diag.y > T(0) ? T(1) / diag.y : T(0);
+    T iz = diag.z > T(0) ? T(1) / diag.z : T(0);
+    T z = T(0);
+    T r[3][3] = {{ix, z, z}, {z, iy, z}, {z, z, iz}};
+    std::memcpy(out, r, sizeof(r));
+    return;
+  }
+  T id = T(1) / det;
+  out[0][0] = (d * f - e * e) * id;
+  out[0][1] = (c * e - b * f) * id;
+  out[0][2] = (b * e - c * d) * id;
+  out[1][0] = (c * e - b * f) * id;
+  out[1][1] = (a * f - c * c) * id;
+  out[1][2] = (b * c - a * e) * id;
+  out[2][0] = (b * e - c * d) * id;
+  out[2][1] = (b * c - a * e) * id;
+  out[2][2] = (a * d - b * b) * id;
+}
+
+template <class T>
+V3<T> mat3_vec(const T m[3][3], V3<T> v) {
+  return V3<T>{m[0][0] * v.x + m[0][1] * v.y + m[0][2] * v.z, m[1][0] * v.x + m[1][1] * v.y + m[1][2] * v.z,
+               m[2][0] * v.x + m[2][1] * v.y + m[2][2] * v.z};
+}
+
+// ---------------------------------------------------------------------------
+// Distance queries (crates/physics/src/collision/distance.rs,
+// distance_box.rs, gjk.rs:68-79)
+// ---------------------------------------------------------------------------
+
+template <class T>
+V3<T> seg_pt_dummy(V3<T> a) {
+  return a;
+}
+
+// seg_seg_closest (distance.rs:75-113).
+template <class T>
+void seg_seg_closest(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1, V3<T>* x, V3<T>* y) {
+  const T EPS = CF(1e-12);
+  V3<T> d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+  T a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  if (a < EPS && e < EPS) {
+    *x = a0;
+    *y = b0;
+    return;
+  }
+  if (a < EPS) {
+    T t = clampf(f / e, T(0), T(1));
+    *x = a0;
+    *y = b0 + d2 * t;
+    return;
+  }
+  T c = dot(d1, r);
+  if (e < EPS) {
+    T t = clampf(-c / a, T(0), T(1));
+    *x = a0 + d1 * t;
+    *y = b0;
+    return;
+  }
+  T b = dot(d1, d2);
+  T denom = a * e - b * b;
+  T s = std::fabs(denom) > EPS ? clampf((b * f - c * e) / denom, T(0), T(1)) : T(0);
+  T t = e > EPS ? (b * s + f) / e : T(0);
+  if (!(t >= T(0) && t <= T(1))) {
+    t = clampf(t, T(0), T(1));
+    s = clampf((b * t - c) / a, T(0), T(1));
+  }
+  *x = a0 + d1 * s;
+  *y = b0 + d2 * t;
+}
+
+// point_obb_closest (distance.rs:117-138).
+template <class T>
+V3<T> point_obb_closest(V3<T> p, V3<T> pos, V3<T> half, Q<T> rot) {
+  V3<T> l = rotate(conj(rot), p - pos);
+  V3<T> cl = vclamp(l, -half, half);
+  V3<T> q;
+  if (cl == l) {
+    T dx = half.x - std::fabs(l.x), dy = half.y - std::fabs(l.y), dz = half.z - std::fabs(l.z);
+    q = l;
+    if (dx <= dy && dx <= dz)
+      q.x = std::copysign(half.x, l.x);
+    else if (dy <= dz)
+      q.y = std::copysign(half.y, l.y);
+    else
+      q.z = std::copysign(half.z, l.z);
+  } else {
+    q = cl;
+  }
+  return pos + rotate(rot, q);
+}
+
+// obb_corners (distance.rs:142-156): bit2 = x, bit1 = y, bit0 = z, 0 = +.
+template <class T>
+void obb_corners(V3<T> pos, V3<T> half, Q<T> rot, V3<T> out[8]) {
+  V3<T> x = rotate(rot, V3<T>{T(1) * half.x, T(0) * half.x, T(0) * half.x});
+  V3<T> y = rotate(rot, V3<T>{T(0) * half.y, T(1) * half.y, T(0) * half.y});
+  V3<T> z = rotate(rot, V3<T>{T(0) * half.z, T(0) * half.z, T(1) * half.z});
+  out[0] = ((pos + x) + y) + z;
+  out[1] = ((pos + x) + y) - z;
+  out[2] = ((pos + x) - y) + z;
+  out[3] = ((pos + x) - y) - z;
+  out[4] = ((pos - x) + y) + z;
+  out[5] = ((pos - x) + y) - z;
+  out[6] = ((pos - x) - y) + z;
+  out[7] = ((pos - x) - y) - z;
+}
+
+static const int OBB_EDGES[12][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}, {0, 2}, {1, 3},
+                                     {4, 6}, {5, 7}, {0, 1}, {2, 3}, {4, 5}, {6, 7}};
+
+// sphere_sphere (distance.rs:180-189).
+template <class T>
+Dist<T> sphere_sphere(const SRef<T>& a, T ra, const SRef<T>& b, T rb) {
+  V3<T> d = b.pos - a.pos;
+  T cd = length(d);
+  V3<T> n = normalize_or(d, V3<T>{T(1), T(0), T(0)});
+  return Dist<T>{cd - ra - rb, a.pos + n * ra, b.pos - n * rb};
+}
+
+// sphere_obb (distance.rs:192-216).
+template <class T>
+Dist<T> sphere_obb(const SRef<T>& a, T r, const SRef<T>& b, V3<T> half) {
+  V3<T> pb = point_obb_closest(a.pos, b.pos, half, b.rot);
+  V3<T> dv = a.pos - pb;
+  T core = length(dv);
+  V3<T> local = rotate(conj(b.rot), a.pos - b.pos);
+  V3<T> al = vabs(local);
+  bool inside = al.x <= half.x && al.y <= half.y && al.z <= half.z;
+  V3<T> n;
+  if (inside) {
+    V3<T> cl = half - al;
+    V3<T> axis;
+    if (cl.x <= cl.y && cl.x <= cl.z) {
+      T s = signum(local.x);
+      axis = V3<T>{T(1) * s, T(0) * s, T(0) * s};
+    } else if (cl.y <= cl.z) {
+      T s = signum(local.y);
+      axis = V3<T>{T(0) * s, T(1) * s, T(0) * s};
+    } else {
+      T s = signum(local.z);
+      axis = V3<T>{T(0) * s, T(0) * s, T(1) * s};
+    }
+    n = normalize_or(pb - a.pos, rotate(b.rot, axis));
+  } else {
+    n = normalize_or(dv, V3<T>{T(1), T(0), T(0)});
+  }
+  return Dist<T>{inside ? -core - r : core - r, a.pos - n * r, pb};
+}
+
+// gjk::support, box/sphere arms (gjk.rs:68-79).
+template <class T>
+V3<T> support(const SRef<T>& s, V3<T> dir) {
+  if (s.shape == SH_SPHERE) return s.pos + normalize_or(dir, V3<T>{T(1), T(0), T(0)}) * s.sp.x;
+  V3<T> local = rotate(conj(s.rot), dir);
+  V3<T> sg{local.x >= T(0) ? T(1) : T(-1), local.y >= T(0) ? T(1) : T(-1), local.z >= T(0) ? T(1) : T(-1)};
+  return s.pos + rotate(s.rot, cmul(s.sp, sg));
+}
+
+// halfspace_distance (distance.rs:607-641).
+template <class T>
+Dist<T> halfspace_distance(const SRef<T>& hs, const SRef<T>& other, bool first) {
+  const T INF = std::numeric_limits<T>::infinity();
+  Dist<T> none{INF, first ? hs.pos : other.pos, first ? other.pos : hs.pos};
+  if (other.shape == SH_HALFSPACE) return none;
+  V3<T> n = rotate(hs.rot, hs.sp);
+  if (!finite3(n)) return none;
+  V3<T> s = support(other, -n);
+  T dist = dot(s - hs.pos, n);
+  Dist<T> d{dist, s - n * dist, s};
+  if (first) return d;
+  return Dist<T>{d.dist, d.pb, d.pa};
+}
+
+// separating_axis (distance_box.rs:21-52).
+template <class T>
+T separating_axis(const SRef<T>& a, V3<T> ha, const SRef<T>& b, V3<T> hb, V3<T>* normal_out) {
+  M3c<T> ra = mat3_from_quat(a.rot), rb = mat3_from_quat(b.rot);
+  V3<T> axes[15];
+  const V3<T> Z{T(0), T(0), T(0)};
+  for (int i = 0; i < 3; ++i) {
+    axes[i] = ra.c[i];
+    axes[3 + i] = rb.c[i];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      V3<T> c = cross(ra.c[i], rb.c[j]);
+      axes[6 + 3 * i + j] = length_sq(c) > CF(1e-12) ? normalize(c) : Z;
+    }
+  V3<T> delta = b.pos - a.pos;
+  T best = -std::numeric_limits<T>::infinity();
+  V3<T> bn{T(0), T(1), T(0)};
+  for (int k = 0; k < 15; ++k) {
+    V3<T> n = axes[k];
+    if (n == Z) continue;
+    T ea = ha.x * std::fabs(dot(n, ra.c[0])) + ha.y * std::fabs(dot(n, ra.c[1])) + ha.z * std::fabs(dot(n, ra.c[2]));
+    T eb = hb.x * std::fabs(dot(n, rb.c[0])) + hb.y * std::fabs(dot(n, rb.c[1])) + hb.z * std::fabs(dot(n, rb.c[2]));
+    T proj = dot(delta, n);
+    T gap = std::fabs(proj) - ea - eb;
+    if (gap > best) {
+      best = gap;
+      bn = proj >= T(0) ? n : -n;
+    }
+  }
+  if (normal_out) *normal_out = bn;
+  return best;
+}
+
+// support_face (distance_box.rs:81-98).
+template <class T>
+void support_face(const SRef<T>& body, V3<T> half, V3<T> dir, SRef<T>* out, V3<T>* ext) {
+  V3<T> local = rotate(conj(body.rot), dir);
+  V3<T> off{T(0), T(0), T(0)};
+  V3<T> e = half;
+  for (int i = 0; i < 3; ++i) {
+    if (std::fabs(local[i]) > K<T>::FACE_AXIS_EPS()) {
+      off[i] = signum(local[i]) * half[i];
+      e[i] = T(0);
+    }
+  }
+  *out = body;
+  out->pos = body.pos + rotate(body.rot, off);
+  *ext = e;
+}
+
+// feature_distance (distance_box.rs:100-133).
+template <class T>
+Dist<T> feature_distance(const SRef<T>& a, V3<T> ha, const SRef<T>& b, V3<T> hb) {
+  V3<T> ca[8], cb[8];
+  obb_corners(a.pos, ha, a.rot, ca);
+  obb_corners(b.pos, hb, b.rot, cb);
+  Dist<T> res{std::numeric_limits<T>::infinity(), a.pos, b.pos};
+  T best = std::numeric_limits<T>::infinity();
+  auto consider = [&](V3<T> x, V3<T> y) {
+    T sq = length_sq(y - x);
+    if (sq < best) {
+      best = sq;
+      res = Dist<T>{std::sqrt(sq), x, y};
+    }
+  };
+  for (int i = 0; i < 8; ++i) consider(ca[i], point_obb_closest(ca[i], b.pos, hb, b.rot));
+  for (int i = 0; i < 8; ++i) consider(point_obb_closest(cb[i], a.pos, ha, a.rot), cb[i]);
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < 12; ++j) {
+      V3<T> x, y;
+      seg_seg_closest(ca[OBB_EDGES[i][0]], ca[OBB_EDGES[i][1]], cb[OBB_EDGES[j][0]], cb[OBB_EDGES[j][1]], &x, &y);
+      consider(x, y);
+    }
+  return res;
+}
+
+// box_box (distance_box.rs:135-147); returns the SAT gap through *sat.
+template <class T>
+Dist<T> box_box(const SRef<T>& a, V3<T> ha, const SRef<T>& b, V3<T> hb, T* sat) {
+  V3<T> normal;
+  T gap = separating_axis(a, ha, b, hb, &normal);
+  *sat = gap;
+  if (gap > T(0)) return feature_distance(a, ha, b, hb);
+  SRef<T> fa, fb;
+  V3<T> ea, eb;
+  support_face(a, ha, normal, &fa, &ea);
+  support_face(b, hb, -normal, &fb, &eb);
+  Dist<T> r = feature_distance(fa, ea, fb, eb);
+  r.dist = gap;
+  return r;
+}
+
+// shape_distance (distance.rs:654-823) for box/sphere/half-space; *sat
+// receives the box-box SAT gap (box_box_signed_gap, distance_box.rs:54-79).
+template <class T>
+Dist<T> shape_distance(const SRef<T>& a, const SRef<T>& b, T* sat) {
+  *sat = T(0);
+  if (a.shape == SH_HALFSPACE) return halfspace_distance(a, b, true);
+  if (b.shape == SH_HALFSPACE) return halfspace_distance(b, a, false);
+  if (a.shape == SH_SPHERE && b.shape == SH_SPHERE) return sphere_sphere(a, a.sp.x, b, b.sp.x);
+  if (a.shape == SH_SPHERE && b.shape == SH_BOX) return sphere_obb(a, a.sp.x, b, b.sp);
+  if (a.shape == SH_BOX && b.shape == SH_SPHERE) {
+    Dist<T> d = sphere_obb(b, b.sp.x, a, a.sp);
+    return Dist<T>{d.dist, d.pb, d.pa};
+  }
+  return box_box(a, a.sp, b, b.sp, sat);
+}
+
+// ---------------------------------------------------------------------------
+// Uniform grid for candidate search (acceleration only; exact tests follow).
+// ---------------------------------------------------------------------------
+struct Grid {
+  double cell = 1.0;
+  std::unordered_map<uint64_t, std::vector<int>> cells;
+  static uint64_t key(int64_t x, int64_t y, int64_t z) {
+    return (uint64_t(x + (1 << 20)) << 42) | (uint64_t(y + (1 << 20)) << 21) | uint64_t(z + (1 << 20));
+  }
+  int64_t ci(double v) const { return (int64_t)std::floor(v / cell); }
+};
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+struct Stats {
+  int pairs, points, colours, max_colour, sleeping, breakdowns, toi_hits;
+};
+
+template <class T>
+struct Engine {
+  V3<T> gravity;
+  std::vector<Body<T>> bodies;
+  std::vector<Pair<T>> pairs;
+  std::vector<Joint<T>> joints;
+  std::vector<V3<T>> pos0, inertial, prev_pos;
+  std::vector<Q<T>> rot0, inertial_rot, prev_rot;
+  std::vector<T> sleep_timer;
+  std::vector<uint8_t> asleep;
+  T time_debt = T(0);
+  int iters = 10;
+  int order = ORDER_REFERENCE;
+  int math = MATH_LIBM;
+  bool brute = false;
+  std::vector<int> colour;  // per body, -1 = not swept
+  int ncolours = 0;
+  Stats stats{};
+  // per-step incidence (pair indices in vector order; joint indices)
+  std::vector<std::vector<int>> inc_pairs, inc_joints;
+
+  bool solvable(int h) const { return bodies[h].type == BT_DYNAMIC && bodies[h].inv_mass > T(0); }
+  T eff_inv_mass(int h) const { return bodies[h].type == BT_DYNAMIC ? bodies[h].inv_mass : T(0); }
+  SRef<T> sref(int h) const { return SRef<T>{bodies[h].shape, bodies[h].sp, bodies[h].pos, bodies[h].rot}; }
+
+  int add_body(const Body<T>& b) {
+    bodies.push_back(b);
+    pos0.push_back(V3<T>{T(0), T(0), T(0)});
+    rot0.push_back(qident<T>());
+    inertial.push_back(V3<T>{T(0), T(0), T(0)});
+    inertial_rot.push_back(qident<T>());
+    prev_pos.push_back(b.pos);
+    prev_rot.push_back(b.rot);
+    sleep_timer.push_back(T(0));
+    asleep.push_back(0);
+    colour.push_back(-1);
+    return (int)bodies.size() - 1;
+  }
+
+  // add_joint, Ball arm (mod.rs:1308-1510; joint.rs:1273-1279).
+  int add_ball_joint(int a, int b, V3<T> la, V3<T> lb) {
+    wake_body(a);
+    wake_body(b);
+    if (a == b || a < 0 || b < 0 || a >= (int)bodies.size() || b >= (int)bodies.size()) return -1;
+    Joint<T> j;
+    j.a = a;
+    j.b = b;
+    j.la = la;
+    j.lb = lb;
+    for (int k = 0; k < 3; ++k) {
+      j.lam_l[k] = T(0);
+      j.pen_l[k] = K<T>::JOINT_PEN_INIT();
+    }
+    joints.push_back(j);
+    return (int)joints.size() - 1;
+  }
+
+  // sleep_body / wake_body (mod.rs:822-842; body.rs:403-418).
+  void sleep_body(int h) {
+    if (bodies[h].type != BT_DYNAMIC) return;
+    Body<T>& b = bodies[h];
+    b.vel = V3<T>{T(0), T(0), T(0)};
+    b.angvel = V3<T>{T(0), T(0), T(0)};
+    b.inv_mass = T(0);
+    b.inertia = V3<T>{T(0), T(0), T(0)};
+    asleep[h] = 1;
+  }
+  void wake_body(int h) {
+    if (h < 0 || h >= (int)asleep.size() || !asleep[h]) return;
+    asleep[h] = 0;
+    sleep_timer[h] = T(0);
+    Body<T>& b = bodies[h];
+    if (b.type == BT_DYNAMIC) {
+      b.inv_mass = T(1) / b.mass;
+      b.inertia = shape_inertia(b.shape, b.sp, b.mass);
+    }
+  }
+
+  static bool pair_allowed(const Body<T>& a, const Body<T>& b) {
+    return (a.layer & b.mask) != 0 && (b.layer & a.mask) != 0;
+  }
+  bool ball_jointed(int ia, int ib) const {
+    for (const auto& j : joints)
+      if ((j.a == ia && j.b == ib) || (j.a == ib && j.b == ia)) return true;
+    return false;
+  }
+
+  // ---- joints helpers (sleep.rs) ----
+  T joint_rest_error(const Joint<T>& j) const {
+    const Body<T>& a = bodies[j.a];
+    const Body<T>& b = bodies[j.b];
+    V3<T> delta = (b.pos + rotate(b.rot, j.lb)) - (a.pos + rotate(a.rot, j.la));
+    return length(delta);
+  }
+  bool joint_driven(const Joint<T>& j) const {
+    int m[2] = {j.a, j.b};
+    for (int h : m) {
+      const Body<T>& b = bodies[h];
+      if (b.type != BT_DYNAMIC && (b.pos != prev_pos[h] || b.rot != prev_rot[h])) return true;
+    }
+    return false;
+  }
+  void propagate_joint_flags(std::vector<uint8_t>& flags, uint8_t value) const {
+    for (;;) {
+      bool changed = false;
+      for (const auto& j : joints) {
+        int m[4] = {j.a, j.b, j.a, j.b};
+        bool any = false;
+        for (int h : m)
+          if (bodies[h].type == BT_DYNAMIC && flags[h] == value) any = true;
+        if (any)
+          for (int h : m)
+            if (bodies[h].type == BT_DYNAMIC && flags[h] != value) {
+              flags[h] = value;
+              changed = true;
+            }
+      }
+      if (!changed) break;
+    }
+  }
+  void wake_joint_motion() {
+    size_t n = bodies.size();
+    std::vector<uint8_t> awake(n);
+    const V3<T> Z{T(0), T(0), T(0)};
+    for (size_t h = 0; h < n; ++h) {
+      const Body<T>& b = bodies[h];
+      awake[h] = b.type == BT_DYNAMIC && (!asleep[h] || b.vel != Z || b.angvel != Z || b.torque != Z ||
+                                          b.pos != prev_pos[h] || b.rot != prev_rot[h]);
+    }
+    if (joints.empty()) {
+      // propagate is a no-op without joints
+    }
+    for (const auto& j : joints) {
+      if (joint_driven(j) || joint_rest_error(j) > K<T>::JOINT_REST()) {
+        int m[4] = {j.a, j.b, j.a, j.b};
+        for (int h : m)
+          if (bodies[h].type == BT_DYNAMIC) awake[h] = 1;
+      }
+    }
+    propagate_joint_flags(awake, 1);
+    for (size_t h = 0; h < n; ++h)
+      if (awake[h] && asleep[h]) wake_body((int)h);
+  }
+  std::vector<uint8_t> joint_sleep_ready() const {
+    std::vector<uint8_t> ready(bodies.size(), 1);
+    for (const auto& j : joints) {
+      if (joint_driven(j) || joint_rest_error(j) > K<T>::JOINT_REST()) {
+        ready[j.a] = 0;
+        ready[j.b] = 0;
+      }
+    }
+    propagate_joint_flags(ready, 0);
+    return ready;
+  }
+
+  // ---- TOI clamp: cast_shape (distance.rs:841-891) ----
+  struct Hit {
+    T t;
+    V3<T> point, normal;
+    int handle;
+    bool valid;
+  };
+  Hit cast_one(const SRef<T>& mover, V3<T> dir, T len, int handle, const SRef<T>& target) const {
+    Hit hit{T(0), {}, {}, handle, false};
+    T t = T(0);
+    for (int it = 0; it < 24; ++it) {
+      SRef<T> m = mover;
+      m.pos = mover.pos + dir * t;
+      T sat;
+      Dist<T> d = shape_distance(m, target, &sat);
+      if (d.dist <= K<T>::SHAPE_TOUCH()) {
+        if (t > T(0)) {
+          V3<T> nn = normalize_or(d.pa - d.pb, -dir);
+          hit = Hit{t, d.pb, nn, handle, true};
+        }
+        break;
+      }
+      t += d.dist - K<T>::SHAPE_TOUCH() * K<T>::HALF();
+      if (t >= len) break;
+    }
+    return hit;
+  }
+  // cast against candidate targets (ascending index); first minimum wins.
+  Hit cast_shape(int h, V3<T> delta, const std::vector<int>& targets) const {
+    Hit best{T(0), {}, {}, -1, false};
+    T len = length(delta);
+    if (len < K<T>::NEAR_ZERO()) return best;
+    V3<T> dir = delta / len;
+    SRef<T> mover = sref(h);
+    for (int o : targets) {
+      Hit hh = cast_one(mover, dir, len, o, sref(o));
+      if (hh.valid && (!best.valid || hh.t < best.t)) best = hh;
+    }
+    return best;
+  }
+  bool toi_target_ok(int h, int o, const std::vector<int>& joined) const {
+    if (o == h || bodies[o].trigger || !pair_allowed(bodies[h], bodies[o])) return false;
+    for (int jn : joined)
+      if (jn == o) return false;
+    return true;
+  }
+  std::vector<int> joined_of(int h) const {
+    std::vector<int> out;
+    for (const auto& j : joints) {
+      if (j.a == h)
+        out.push_back(j.b);
+      else if (j.b == h)
+        out.push_back(j.a);
+    }
+    return out;
+  }
+  bool toi_mover(int h) const {
+    if (!solvable(h) || asleep[h] || bodies[h].trigger) return false;
+    V3<T> disp = bodies[h].vel * K<T>::DT();
+    return !(length(disp) <= K<T>::CCD_GATE() * shape_min_dim(bodies[h].shape, bodies[h].sp));
+  }
+  // Candidate TOI targets of mover h from a grid of the current poses.
+  void toi_candidates(int h, const Grid* g, const std::vector<int>& large, std::vector<int>& out) const {
+    out.clear();
+    size_t n = bodies.size();
+    if (!g) {
+      for (size_t o = 0; o < n; ++o) out.push_back((int)o);
+      return;
+    }
+    const Body<T>& b = bodies[h];
+    V3<T> disp = b.vel * K<T>::DT();
+    double r = (double)bound_radius(b.shape, b.sp) + g->cell * 0.5 + 0.01;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      double p0 = (double)b.pos[k], p1 = (double)(b.pos[k] + disp[k]);
+      lo[k] = std::min(p0, p1) - r;
+      hi[k] = std::max(p0, p1) + r;
+    }
+    for (int64_t x = g->ci(lo[0]); x <= g->ci(hi[0]); ++x)
+      for (int64_t y = g->ci(lo[1]); y <= g->ci(hi[1]); ++y)
+        for (int64_t z = g->ci(lo[2]); z <= g->ci(hi[2]); ++z) {
+          auto it = g->cells.find(Grid::key(x, y, z));
+          if (it == g->cells.end()) continue;
+          for (int o : it->second) out.push_back(o);
+        }
+    for (int o : large) out.push_back(o);
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+  }
+  // Bodies split into grid members (finite bound radius below a cutoff)
+  // and "large" ones tested against everything.
+  void build_grid(Grid& g, std::vector<int>& large, double extra) const {
+    size_t n = bodies.size();
+    std::vector<double> rs;
+    for (size_t h = 0; h < n; ++h) {
+      double r = (double)bound_radius(bodies[h].shape, bodies[h].sp);
+      if (std::isfinite(r)) rs.push_back(r);
+    }
+    double cut = 1.0;
+    if (!rs.empty()) {
+      std::vector<double> s = rs;
+      std::nth_element(s.begin(), s.begin() + s.size() / 2, s.end());
+      cut = std::max(4.0 * s[s.size() / 2], 1e-3);
+    }
+    double rmax = 0.0;
+    for (double r : rs)
+      if (r <= cut) rmax = std::max(rmax, r);
+    g.cell = (2.0 * rmax + 0.05 + extra) * 1.01 + 1e-6;
+    g.cells.clear();
+    large.clear();
+    for (size_t h = 0; h < n; ++h) {
+      double r = (double)bound_radius(bodies[h].shape, bodies[h].sp);
+      if (!std::isfinite(r) || r > cut) {
+        large.push_back((int)h);
+        continue;
+      }
+      const Body<T>& b = bodies[h];
+      g.cells[Grid::key(g.ci(b.pos.x), g.ci(b.pos.y), g.ci(b.pos.z))].push_back((int)h);
+    }
+  }
+  void grid_move(Grid& g, int h, V3<T> oldp) const {
+    uint64_t k0 = Grid::key(g.ci(oldp.x), g.ci(oldp.y), g.ci(oldp.z));
+    const Body<T>& b = bodies[h];
+    uint64_t k1 = Grid::key(g.ci(b.pos.x), g.ci(b.pos.y), g.ci(b.pos.z));
+    if (k0 == k1) return;
+    auto it = g.cells.find(k0);
+    if (it != g.cells.end()) {
+      auto& v = it->second;
+      v.erase(std::remove(v.begin(), v.end(), h), v.end());
+    }
+    g.cells[k1].push_back(h);
+  }
+
+  void apply_toi_hit(int h, const Hit& hit) {
+    Body<T>& b = bodies[h];
+    V3<T> disp = b.vel * K<T>::DT();
+    T len = fmax_(length(disp), K<T>::MIN_TOI_LEN());
+    T frac = clampf(hit.t / len, T(0), T(1));
+    T e = fmin_(bodies[h].restitution, bodies[hit.handle].restitution);
+    b.pos = b.pos + disp * frac;
+    T vn = dot(b.vel, hit.normal);
+    if (vn < T(0)) {
+      T bounce = vn < -K<T>::CCD_BOUNCE() ? T(1) + e : T(1);
+      b.vel = b.vel - hit.normal * (bounce * vn);
+    }
+    wake_body(hit.handle);
+    sleep_timer[h] = T(0);
+    stats.toi_hits++;
+  }
+
+  // mod.rs:882-978.
+  void toi_clamp() {
+    int n = (int)bodies.size();
+    bool use_grid = !brute && n > 256;
+    Grid g;
+    std::vector<int> large, cand;
+    if (use_grid) build_grid(g, large, 0.0);
+    if (order == ORDER_REFERENCE) {
+      for (int h = 0; h < n; ++h) {
+        if (!toi_mover(h)) continue;
+        std::vector<int> joined = joined_of(h);
+        toi_candidates(h, use_grid ? &g : nullptr, large, cand);
+        std::vector<int> tg;
+        for (int o : cand)
+          if (toi_target_ok(h, o, joined)) tg.push_back(o);
+        Hit hit = cast_shape(h, bodies[h].vel * K<T>::DT(), tg);
+        if (hit.valid) {
+          V3<T> oldp = bodies[h].pos;
+          apply_toi_hit(h, hit);
+          if (use_grid && std::isfinite((double)bound_radius(bodies[h].shape, bodies[h].sp)) &&
+              std::find(large.begin(), large.end(), h) == large.end())
+            grid_move(g, h, oldp);
+        }
+      }
+    } else {
+      std::vector<Hit> hits(n, Hit{T(0), {}, {}, -1, false});
+      for (int h = 0; h < n; ++h) {
+        if (!toi_mover(h)) continue;
+        std::vector<int> joined = joined_of(h);
+        toi_candidates(h, use_grid ? &g : nullptr, large, cand);
+        std::vector<int> tg;
+        for (int o : cand)
+          if (toi_target_ok(h, o, joined)) tg.push_back(o);
+        hits[h] = cast_shape(h, bodies[h].vel * K<T>::DT(), tg);
+      }
+      for (int h = 0; h < n; ++h)
+        if (hits[h].valid) apply_toi_hit(h, hits[h]);
+    }
+  }
+
+  // ---- discovery (rows.rs:432-756) ----
+  struct Disc {
+    int ia, ib;
+    bool trigger, exists, wake;
+    V3<T> normal;
+    T mu[2];
+    T signed_;
+    std::vector<std::pair<V3<T>, V3<T>>> fresh;
+  };
+
+  const Pair<T>* find_pair(int ia, int ib, const std::unordered_map<uint64_t, int>* index) const {
+    if (index) {
+      auto it = index->find((uint64_t(ia) << 32) | uint32_t(ib));
+      return it == index->end() ? nullptr : &pairs[it->second];
+    }
+    for (const auto& p : pairs)
+      if (p.a == ia && p.b == ib) return &p;
+    return nullptr;
+  }
+
+  bool discover_pair(int ia, int ib, const std::unordered_map<uint64_t, int>* index, Disc& out) const {
+    const Body<T>& a = bodies[ia];
+    const Body<T>& b = bodies[ib];
+    if (!pair_allowed(a, b)) return false;
+    if (!a.trigger && !b.trigger && !joints.empty() && ball_jointed(ia, ib)) return false;
+    T ra = bound_radius(a.shape, a.sp);
+    T rb = bound_radius(b.shape, b.sp);
+    if (length(a.pos - b.pos) > ra + rb + K<T>::GEN_MARGIN()) return false;
+    T sat;
+    SRef<T> sa = sref(ia), sb = sref(ib);
+    Dist<T> d = shape_distance(sa, sb, &sat);
+    T sgn = (a.shape == SH_BOX && b.shape == SH_BOX) ? sat : d.dist;
+    out.ia = ia;
+    out.ib = ib;
+    out.fresh.clear();
+    out.signed_ = sgn;
+    if (a.trigger || b.trigger) {
+      out.trigger = true;
+      out.exists = false;
+      out.wake = false;
+      out.normal = V3<T>{T(0), T(1), T(0)};
+      out.mu[0] = out.mu[1] = T(0);
+      return true;
+    }
+    out.trigger = false;
+    const Pair<T>* old = find_pair(ia, ib, index);
+    // orient_contact_normal (rows.rs:303-333)
+    V3<T> normal;
+    if (a.shape == SH_HALFSPACE) {
+      normal = -rotate(a.rot, a.sp);
+    } else if (b.shape == SH_HALFSPACE) {
+      normal = rotate(b.rot, b.sp);
+    } else {
+      normal = d.pa - d.pb;
+      if (d.dist < T(0)) normal = -normal;
+      if (length_sq(normal) < K<T>::AXIS_REST_LEN2()) normal = a.pos - b.pos;
+      normal = normalize_or(normal, V3<T>{T(0), T(1), T(0)});
+      if (old) {
+        if (dot(normal, old->n) < T(0)) normal = -normal;
+      } else if (dot(normal, a.pos - b.pos) < T(0)) {
+        normal = -normal;
+      }
+    }
+    bool exists = old != nullptr;
+    T approach = fmax_(dot(b.vel - a.vel, normal), T(0));
+    T reach = K<T>::GEN_MARGIN() + approach * K<T>::DT();
+    if (sgn > reach && !exists) return false;
+    bool separated = sgn > reach;
+    // friction_frame, isotropic arm (rows.rs:32-52)
+    T mu1 = fmax_(a.friction, b.friction);
+    out.mu[0] = fmax_(mu1, T(0));
+    out.mu[1] = fmax_(mu1, T(0));
+    T patch = fmin_(ra, rb) + K<T>::MARGIN();
+    V3<T> pp = (d.pa + d.pb) * K<T>::HALF();
+    V3<T> pa_c = a.pos - normal * dot(a.pos - pp, normal);
+    V3<T> pb_c = b.pos - normal * dot(b.pos - pp, normal);
+    auto plane = [&](const Body<T>& body, V3<T> witness, T sign) -> T {
+      if (body.shape == SH_BOX) {
+        T extent = dot(vabs(rotate(conj(body.rot), normal)), body.sp);
+        return dot(normal, body.pos) - sign * extent;
+      }
+      return dot(normal, witness);
+    };
+    T plane_a = plane(a, d.pa, T(1));
+    T plane_b = plane(b, d.pb, T(-1));
+    auto& fresh = out.fresh;
+    Q<T> inv_a = conj(a.rot), inv_b = conj(b.rot);
+    auto dup = [&](V3<T> ra_l, V3<T> rb_l) {
+      for (auto& e : fresh)
+        if (length(e.first - ra_l) < K<T>::POINT_MATCH() && length(e.second - rb_l) < K<T>::POINT_MATCH())
+          return true;
+      return false;
+    };
+    if (!separated) {
+      for (int side = 0; side < 2; ++side) {
+        int h = side == 0 ? ia : ib;
+        V3<T> witness = side == 0 ? d.pa : d.pb;
+        T sign = side == 0 ? T(1) : T(-1);
+        V3<T> pc = side == 0 ? pa_c : pb_c;
+        const Body<T>& body = bodies[h];
+        if (body.shape != SH_BOX) continue;
+        V3<T> hh = body.sp;
+        for (int cx = 0; cx < 2; ++cx)
+          for (int cy = 0; cy < 2; ++cy)
+            for (int cz = 0; cz < 2; ++cz) {
+              if ((int)fresh.size() >= MAX_POINTS) goto next_side;
+              {
+                T sx = cx ? T(1) : T(-1), sy = cy ? T(1) : T(-1), sz = cz ? T(1) : T(-1);
+                V3<T> corner{sx * hh.x, sy * hh.y, sz * hh.z};
+                V3<T> world = body.pos + rotate(body.rot, corner);
+                T along = dot(world - witness, sign * normal);
+                if (std::fabs(along) > K<T>::EXPAND_SLOP() + fmax_(-sgn, T(0))) continue;
+                V3<T> rel_c = world - pc;
+                V3<T> tang = rel_c - normal * dot(rel_c, normal);
+                if (length(tang) > patch) continue;
+                V3<T> pa, pb;
+                if (side == 0) {
+                  pa = world;
+                  pb = world + normal * (plane_b - dot(normal, world));
+                } else {
+                  pa = world + normal * (plane_a - dot(normal, world));
+                  pb = world;
+                }
+                V3<T> ra_l = rotate(inv_a, pa - a.pos);
+                V3<T> rb_l = rotate(inv_b, pb - b.pos);
+                if (dup(ra_l, rb_l)) continue;
+                fresh.push_back({ra_l, rb_l});
+              }
+            }
+      next_side:;
+      }
+    }
+    if (!separated && fresh.size() >= 2 && (int)fresh.size() < MAX_POINTS) {
+      V3<T> mid = ra < rb ? pa_c : (rb < ra ? pb_c : (pa_c + pb_c) * K<T>::HALF());
+      V3<T> pa = mid + normal * (plane_a - dot(normal, mid));
+      V3<T> pb = mid + normal * (plane_b - dot(normal, mid));
+      V3<T> ra_l = rotate(inv_a, pa - a.pos);
+      V3<T> rb_l = rotate(inv_b, pb - b.pos);
+      if (!dup(ra_l, rb_l)) fresh.push_back({ra_l, rb_l});
+    }
+    if (!separated && fresh.size() < 3) {
+      V3<T> ra_l = rotate(inv_a, d.pa - a.pos);
+      V3<T> rb_l = rotate(inv_b, d.pb - b.pos);
+      if (!dup(ra_l, rb_l)) fresh.push_back({ra_l, rb_l});
+    }
+    bool wake;
+    if (exists) {
+      wake = (asleep[ia] || asleep[ib]) && sgn < -K<T>::WAKE_PEN();
+    } else {
+      bool impact = sgn < -K<T>::WAKE_PEN();
+      if (!impact) {
+        int ss[2] = {ia, ib}, oo[2] = {ib, ia};
+        T sg[2] = {T(1), T(-1)};
+        for (int k = 0; k < 2; ++k) {
+          if (asleep[ss[k]]) {
+            T app = dot(bodies[oo[k]].vel - bodies[ss[k]].vel, sg[k] * normal);
+            if (app > K<T>::WAKE_IMPACT()) {
+              impact = true;
+              break;
+            }
+          }
+        }
+      }
+      wake = impact;
+    }
+    out.exists = exists;
+    out.normal = normal;
+    out.wake = wake;
+    return true;
+  }
+
+  // candidate (ia < ib) pairs: every pair (brute) or grid neighbours + large.
+  void candidate_pairs(std::vector<std::pair<int, int>>& out) const {
+    int n = (int)bodies.size();
+    out.clear();
+    if (brute || n <= 256) {
+      for (int ia = 0; ia < n; ++ia)
+        for (int ib = ia + 1; ib < n; ++ib) out.push_back({ia, ib});
+      return;
+    }
+    Grid g;
+    std::vector<int> large;
+    build_grid(g, large, 0.0);
+    std::vector<uint8_t> is_large(n, 0);
+    for (int h : large) is_large[h] = 1;
+    for (int ia = 0; ia < n; ++ia) {
+      if (is_large[ia]) {
+        for (int ib = ia + 1; ib < n; ++ib) out.push_back({ia, ib});
+        continue;
+      }
+      const Body<T>& b = bodies[ia];
+      int64_t cx = g.ci(b.pos.x), cy = g.ci(b.pos.y), cz = g.ci(b.pos.z);
+      std::vector<int> nb;
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dz = -1; dz <= 1; ++dz) {
+            auto it = g.cells.find(Grid::key(cx + dx, cy + dy, cz + dz));
+            if (it == g.cells.end()) continue;
+            for (int o : it->second)
+              if (o > ia) nb.push_back(o);
+          }
+      for (int o : large)
+        if (o > ia) nb.push_back(o);
+      std::sort(nb.begin(), nb.end());
+      for (int o : nb) out.push_back({ia, o});
+    }
+  }
+
+  // generate_pairs (rows.rs:767-943).
+  void generate_pairs() {
+    std::unordered_map<uint64_t, int> index;
+    bool use_index = !brute;
+    if (use_index)
+      for (size_t i = 0; i < pairs.size(); ++i) index[(uint64_t(pairs[i].a) << 32) | uint32_t(pairs[i].b)] = (int)i;
+    std::vector<std::pair<int, int>> cand;
+    candidate_pairs(cand);
+    std::vector<Disc> disc;
+    Disc d;
+    for (auto& c : cand)
+      if (discover_pair(c.first, c.second, use_index ? &index : nullptr, d)) disc.push_back(d);
+    // candidates are generated in (ia, ib) order already; sort defensively.
+    std::stable_sort(disc.begin(), disc.end(),
+                     [](const Disc& x, const Disc& y) { return x.ia != y.ia ? x.ia < y.ia : x.ib < y.ib; });
+    std::vector<uint8_t> seen(pairs.size(), 0);
+    std::vector<std::pair<int, int>> wake;
+    for (auto& dd : disc) {
+      int ia = dd.ia, ib = dd.ib;
+      if (dd.trigger) continue;
+      if (dd.exists) {
+        int idx = -1;
+        if (use_index) {
+          auto it = index.find((uint64_t(ia) << 32) | uint32_t(ib));
+          if (it != index.end()) idx = it->second;
+        } else {
+          for (size_t i = 0; i < pairs.size(); ++i)
+            if (pairs[i].a == ia && pairs[i].b == ib) {
+              idx = (int)i;
+              break;
+            }
+        }
+        if (idx < 0) continue;
+        seen[idx] = 1;
+        Pair<T>& pair = pairs[idx];
+        pair.n = dd.normal;
+        pair.mu[0] = dd.mu[0];
+        pair.mu[1] = dd.mu[1];
+        pair.gap = dd.signed_;
+        if (dd.wake) wake.push_back({ia, ib});
+        std::vector<Point<T>> next;
+        bool touching = dd.signed_ <= T(0);
+        bool smooth = bodies[ia].shape == SH_SPHERE || bodies[ib].shape == SH_SPHERE;
+        for (auto& f : dd.fresh) {
+          const Point<T>* old = nullptr;
+          for (auto& p : pair.pts)
+            if (length(p.ra - f.first) < K<T>::LAMBDA_MATCH()) {
+              old = &p;
+              break;
+            }
+          Point<T> np;
+          if (old) {
+            bool keep = old->stuck && touching && !smooth;
+            np.ra = keep ? old->ra : f.first;
+            np.rb = keep ? old->rb : f.second;
+            for (int k = 0; k < 3; ++k) {
+              np.lam[k] = old->lam[k];
+              np.pen[k] = old->pen[k];
+              np.roll[k] = old->roll[k];
+            }
+            np.stuck = old->stuck;
+          } else {
+            np.ra = f.first;
+            np.rb = f.second;
+            for (int k = 0; k < 3; ++k) {
+              np.lam[k] = T(0);
+              np.pen[k] = K<T>::PEN_INIT();
+              np.roll[k] = T(0);
+            }
+            np.stuck = true;
+          }
+          next.push_back(np);
+        }
+        pair.pts = std::move(next);
+      } else {
+        seen.push_back(1);
+        if (dd.wake) wake.push_back({ia, ib});
+        Pair<T> p;
+        p.a = ia;
+        p.b = ib;
+        p.n = dd.normal;
+        p.mu[0] = dd.mu[0];
+        p.mu[1] = dd.mu[1];
+        p.gap = dd.signed_;
+        for (auto& f : dd.fresh) {
+          Point<T> np;
+          np.ra = f.first;
+          np.rb = f.second;
+          for (int k = 0; k < 3; ++k) {
+            np.lam[k] = T(0);
+            np.pen[k] = K<T>::PEN_INIT();
+            np.roll[k] = T(0);
+          }
+          np.stuck = true;
+          p.pts.push_back(np);
+        }
+        pairs.push_back(std::move(p));
+      }
+    }
+    for (int idx = (int)seen.size() - 1; idx >= 0; --idx) {
+      if (!seen[idx]) {
+        if (idx != (int)pairs.size() - 1) pairs[idx] = std::move(pairs.back());
+        pairs.pop_back();
+      }
+    }
+    for (auto& w : wake) {
+      wake_body(w.first);
+      wake_body(w.second);
+    }
+    if (order == ORDER_COLOUR) {
+      std::sort(pairs.begin(), pairs.end(),
+                [](const Pair<T>& x, const Pair<T>& y) { return x.a != y.a ? x.a < y.a : x.b < y.b; });
+    }
+  }
+
+  // ---- row helpers (rows.rs:947-1111; joints_ext.rs:21-32) ----
+  T gap_c0(const Pair<T>& p, const Point<T>& pt) const {
+    const Body<T>& a = bodies[p.a];
+    const Body<T>& b = bodies[p.b];
+    if (a.shape == SH_SPHERE || b.shape == SH_SPHERE) return p.gap + K<T>::MARGIN();
+    V3<T> pa = pos0[p.a] + rotate(rot0[p.a], pt.ra);
+    V3<T> pb = pos0[p.b] + rotate(rot0[p.b], pt.rb);
+    return dot(p.n, pa - pb) + K<T>::MARGIN();
+  }
+  void contact_levers(const Pair<T>& p, const Point<T>& pt, V3<T>* ra_w, V3<T>* rb_w) const {
+    const Body<T>& a = bodies[p.a];
+    const Body<T>& b = bodies[p.b];
+    *ra_w = a.shape == SH_SPHERE ? (-p.n) * a.sp.x : rotate(a.rot, pt.ra);
+    *rb_w = b.shape == SH_SPHERE ? p.n * b.sp.x : rotate(b.rot, pt.rb);
+  }
+  void friction_levers(const Pair<T>& p, const Point<T>& pt, V3<T>* ra_f, V3<T>* rb_f) const {
+    V3<T> ra_w, rb_w;
+    contact_levers(p, pt, &ra_w, &rb_w);
+    const Body<T>& a = bodies[p.a];
+    const Body<T>& b = bodies[p.b];
+    T sep = dot((a.pos + ra_w) - (b.pos + rb_w), p.n);
+    *ra_f = ra_w - p.n * (sep * K<T>::HALF());
+    *rb_f = rb_w + p.n * (sep * K<T>::HALF());
+  }
+  T row_c_levers(const Pair<T>& p, V3<T> u, T c0, V3<T> ra_w, V3<T> rb_w) const {
+    const Body<T>& a = bodies[p.a];
+    const Body<T>& b = bodies[p.b];
+    V3<T> d_a = (a.pos - pos0[p.a]) + u * dot(cross(ra_w, u), quat_diff_vec(a.rot, rot0[p.a], math));
+    V3<T> d_b = (b.pos - pos0[p.b]) + u * dot(cross(rb_w, u), quat_diff_vec(b.rot, rot0[p.b], math));
+    return c0 * (T(1) - K<T>::ALPHA()) + dot(u, d_a - d_b);
+  }
+  static void contact_force(T cn, const T pens[3], const T lam[3], T ct0, T ct1, const T mu[2], T f[3]) {
+    T fn_c = fmin_(pens[0] * cn + lam[0], T(0));
+    T r0 = pens[1] * ct0 + lam[1], r1 = pens[2] * ct1 + lam[2];
+    T b1 = std::fabs(fn_c) * mu[0];
+    T b2 = std::fabs(fn_c) * mu[1];
+    if (b1 <= T(0) || b2 <= T(0)) {
+      f[0] = fn_c;
+      f[1] = b1 <= T(0) ? T(0) : clampf(r0, -b1, b1);
+      f[2] = b2 <= T(0) ? T(0) : clampf(r1, -b2, b2);
+      return;
+    }
+    T s = (r0 / b1) * (r0 / b1) + (r1 / b2) * (r1 / b2);
+    T k = s > T(1) ? T(1) / std::sqrt(s) : T(1);
+    f[0] = fn_c;
+    f[1] = r0 * k;
+    f[2] = r1 * k;
+  }
+  static bool row_live(T c, T f) { return std::fabs(c) >= K<T>::C_EPS() || f != T(0); }
+  static void stamp_row(T lhs[6][6], T rhs[6], V3<T> axis, T pen, T f, V3<T> r, T sign) {
+    V3<T> nn = sign * axis;
+    V3<T> t = cross(r, nn);
+    for (int x = 0; x < 3; ++x)
+      for (int y = 0; y < 3; ++y) {
+        lhs[x][y] += pen * (nn[x] * nn[y]);
+        lhs[3 + x][3 + y] += pen * (t[x] * t[y]);
+        lhs[x][3 + y] += pen * (nn[x] * t[y]);
+        lhs[3 + x][y] += pen * (nn[y] * t[x]);
+      }
+    rhs[0] += f * nn.x;
+    rhs[1] += f * nn.y;
+    rhs[2] += f * nn.z;
+    rhs[3] += f * t.x;
+    rhs[4] += f * t.y;
+    rhs[5] += f * t.z;
+  }
+
+  // solve_6x6 (assembly.rs:25-69).
+  static bool solve_6x6(const T lhs[6][6], const T rhs[6], T x[6]) {
+    T l[6][6] = {};
+    T d[6] = {};
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j <= i; ++j) {
+        T s = lhs[i][j];
+        for (int k = 0; k < j; ++k) s -= l[i][k] * d[k] * l[j][k];
+        if (i == j) {
+          if (s <= K<T>::DEGENERATE()) return false;
+          d[i] = s;
+          l[i][i] = T(1);
+        } else {
+          l[i][j] = s / d[j];
+        }
+      }
+    T y[6];
+    for (int i = 0; i < 6; ++i) {
+      T s = rhs[i];
+      for (int k = 0; k < i; ++k) s -= l[i][k] * y[k];
+      y[i] = s;
+    }
+    T z[6];
+    for (int i = 0; i < 6; ++i) z[i] = y[i] / d[i];
+    for (int i = 5; i >= 0; --i) {
+      T s = z[i];
+      for (int k = i + 1; k < 6; ++k) s -= l[k][i] * x[k];
+      x[i] = s;
+    }
+    return true;
+  }
+
+  // solve_body (assembly.rs:73-903), contact + Ball arms.
+  void solve_body(int h) {
+    const T DT = K<T>::DT();
+    const T DT2 = DT * DT;
+    Body<T>& body = bodies[h];
+    T m_dt2 = T(1) / eff_inv_mass(h) / DT2;
+    T iw[3][3];
+    world_inertia(body.inertia, body.rot, iw);
+    T lhs[6][6] = {};
+    T rhs[6] = {};
+    for (int i = 0; i < 3; ++i) lhs[i][i] = m_dt2;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) lhs[3 + a][3 + b] = iw[a][b] / DT2;
+    V3<T> rl = m_dt2 * (body.pos - inertial[h]);
+    rhs[0] = rl.x;
+    rhs[1] = rl.y;
+    rhs[2] = rl.z;
+    V3<T> ra = mat3_vec(iw, quat_diff_vec(body.rot, inertial_rot[h], math)) / DT2;
+    rhs[3] = ra.x;
+    rhs[4] = ra.y;
+    rhs[5] = ra.z;
+
+    for (int pi : inc_pairs[h]) {
+      const Pair<T>& p = pairs[pi];
+      bool is_a = p.a == h;
+      T sign = is_a ? T(1) : T(-1);
+      bool touching = p.gap <= T(0);
+      for (const Point<T>& pt : p.pts) {
+        T c0 = touching ? gap_c0(p, pt) : T(0);
+        V3<T> r_up, r_lo;
+        contact_levers(p, pt, &r_up, &r_lo);
+        T cn = row_c_levers(p, p.n, c0, r_up, r_lo);
+        V3<T> r_side = is_a ? r_up : r_lo;
+        if (!touching) {
+          V3<T> a_vel = bodies[p.a].vel, b_vel = bodies[p.b].vel;
+          T approach = fmax_(dot(b_vel - a_vel, p.n), T(0));
+          if (p.gap - approach * DT > T(0)) continue;
+          if (std::fabs(cn) >= K<T>::C_EPS()) {
+            T npts = T(std::max<size_t>(p.pts.size(), 1));
+            T ka = eff_inv_mass(p.a), kb = eff_inv_mass(p.b);
+            T m_red = ka + kb > K<T>::NEAR_ZERO() ? T(1) / (ka + kb) : T(0);
+            if (m_red > T(0)) {
+              T pen_d = fmin_(T(2) * m_red / DT2 / npts, K<T>::PEN_MAX());
+              T f_d = fmin_(pen_d * cn, T(0));
+              stamp_row(lhs, rhs, p.n, pen_d, f_d, r_side, sign);
+            }
+          }
+          continue;
+        }
+        V3<T> t1, t2u;
+        tangent_basis(p.n, &t1, &t2u);
+        V3<T> t2 = cross(t1, p.n);
+        V3<T> ra_f, rb_f;
+        friction_levers(p, pt, &ra_f, &rb_f);
+        V3<T> r_side_f = is_a ? ra_f : rb_f;
+        T ct1 = row_c_levers(p, t1, T(0), ra_f, rb_f);
+        T ct2 = row_c_levers(p, t2, T(0), ra_f, rb_f);
+        T f[3];
+        contact_force(cn, pt.pen, pt.lam, ct1, ct2, p.mu, f);
+        V3<T> axes[3] = {p.n, t1, t2};
+        T cvs[3] = {cn, ct1, ct2};
+        V3<T> rs[3] = {r_side, r_side_f, r_side_f};
+        for (int row = 0; row < 3; ++row) {
+          if (!row_live(cvs[row], f[row])) continue;
+          stamp_row(lhs, rhs, axes[row], pt.pen[row], f[row], rs[row], sign);
+        }
+        T mu_roll = fmax_(bodies[p.a].rolling, bodies[p.b].rolling);
+        T mu_spin = fmax_(bodies[p.a].torsion, bodies[p.b].torsion);
+        if (mu_roll > T(0) || mu_spin > T(0)) {
+          V3<T> drot_a = quat_diff_vec(bodies[p.a].rot, rot0[p.a], math);
+          V3<T> drot_b = quat_diff_vec(bodies[p.b].rot, rot0[p.b], math);
+          V3<T> wrel = drot_a - drot_b;
+          V3<T> rax[3] = {t1, t2, p.n};
+          T mur[3] = {mu_roll, mu_roll, mu_spin};
+          for (int li = 0; li < 3; ++li) {
+            if (mur[li] <= T(0)) continue;
+            T c = dot(wrel, rax[li]);
+            T cap = mur[li] * std::fabs(f[0]);
+            T fr = clampf(K<T>::ROLL_PEN() * c + pt.roll[li], -cap, cap);
+            V3<T> g = is_a ? rax[li] : -rax[li];
+            for (int x = 0; x < 3; ++x)
+              for (int y = 0; y < 3; ++y) lhs[3 + x][3 + y] += K<T>::ROLL_PEN() * (g[x] * g[y]);
+            rhs[3] += fr * g.x;
+            rhs[4] += fr * g.y;
+            rhs[5] += fr * g.z;
+          }
+        }
+      }
+    }
+
+    for (int ji : inc_joints[h]) {
+      const Joint<T>& j = joints[ji];
+      bool is_a = j.a == h;
+      T sign = is_a ? T(1) : T(-1);
+      const Body<T>& a = bodies[j.a];
+      const Body<T>& b = bodies[j.b];
+      V3<T> pa = a.pos + rotate(a.rot, j.la);
+      V3<T> pb = b.pos + rotate(b.rot, j.lb);
+      V3<T> pa0 = pos0[j.a] + rotate(rot0[j.a], j.la);
+      V3<T> pb0 = pos0[j.b] + rotate(rot0[j.b], j.lb);
+      V3<T> live = pa - pb;
+      V3<T> c0v = pa0 - pb0;
+      V3<T> fl{T(0), T(0), T(0)};
+      const V3<T> Z{T(0), T(0), T(0)};
+      for (int k = 0; k < 3; ++k) {
+        V3<T> axis{T(0), T(0), T(0)};
+        axis[k] = T(1);
+        T c = live[k] - K<T>::ALPHA() * c0v[k];
+        T f = j.pen_l[k] * c + j.lam_l[k];
+        if (!row_live(c, f)) continue;
+        V3<T> r_side = is_a ? rotate(a.rot, j.la + Z) : rotate(b.rot, j.lb);
+        stamp_row(lhs, rhs, axis, j.pen_l[k], f, r_side, sign);
+        fl[k] = f;
+      }
+      // geometric stiffness (assembly.rs:589-618; rows.rs:69-88)
+      V3<T> r = is_a ? rotate(a.rot, j.la + Z) : -rotate(b.rot, j.lb);
+      T hm[3][3] = {};
+      for (int k = 0; k < 3; ++k) {
+        T g[3][3] = {};
+        g[0][0] = -r[k];
+        g[1][1] = -r[k];
+        g[2][2] = -r[k];
+        g[0][k] += r[0];
+        g[1][k] += r[1];
+        g[2][k] += r[2];
+        for (int x = 0; x < 3; ++x)
+          for (int y = 0; y < 3; ++y) hm[x][y] += g[x][y] * fl[k];
+      }
+      V3<T> hd{length(V3<T>{hm[0][0], hm[1][0], hm[2][0]}), length(V3<T>{hm[0][1], hm[1][1], hm[2][1]}),
+               length(V3<T>{hm[0][2], hm[1][2], hm[2][2]})};
+      lhs[3][3] += hd.x;
+      lhs[4][4] += hd.y;
+      lhs[5][5] += hd.z;
+    }
+
+    T neg[6] = {-rhs[0], -rhs[1], -rhs[2], -rhs[3], -rhs[4], -rhs[5]};
+    T dx[6];
+    if (solve_6x6(lhs, neg, dx)) {
+      body.pos = body.pos + V3<T>{dx[0], dx[1], dx[2]};
+      body.rot = quat_integrate(body.rot, V3<T>{dx[3], dx[4], dx[5]}, math);
+    } else {
+      stats.breakdowns++;
+    }
+  }
+
+  // dual_update (joints_ext.rs:464-598), contact + Ball arms.
+  void dual_update() {
+    for (auto& p : pairs) {
+      V3<T> t1, t2u;
+      tangent_basis(p.n, &t1, &t2u);
+      V3<T> t2 = cross(t1, p.n);
+      bool touching = p.gap <= T(0);
+      for (auto& pt : p.pts) {
+        T c0 = touching ? gap_c0(p, pt) : T(0);
+        V3<T> r_up, r_lo;
+        contact_levers(p, pt, &r_up, &r_lo);
+        T cn = row_c_levers(p, p.n, c0, r_up, r_lo);
+        V3<T> ra_f, rb_f;
+        friction_levers(p, pt, &ra_f, &rb_f);
+        T ct1 = row_c_levers(p, t1, T(0), ra_f, rb_f);
+        T ct2 = row_c_levers(p, t2, T(0), ra_f, rb_f);
+        T pen[3] = {pt.pen[0], pt.pen[1], pt.pen[2]};
+        T lam[3] = {pt.lam[0], pt.lam[1], pt.lam[2]};
+        T f[3];
+        contact_force(cn, pen, lam, ct1, ct2, p.mu, f);
+        {
+          T bnd = std::fabs(f[0]) * fmax_(p.mu[0], p.mu[1]);
+          T u0 = pen[1] * ct1 + lam[1], u1 = pen[2] * ct2 + lam[2];
+          T fs = std::sqrt(u0 * u0 + u1 * u1);
+          pt.stuck = fs <= bnd && std::sqrt(ct1 * ct1 + ct2 * ct2) < K<T>::STICK();
+        }
+        if (touching && std::fabs(cn) >= K<T>::C_EPS()) {
+          pt.lam[0] = f[0];
+          if (f[0] < T(0)) pt.pen[0] = fmin_(pt.pen[0] + K<T>::BETA() * std::fabs(cn), K<T>::PEN_MAX());
+        }
+        T b1 = std::fabs(f[0]) * p.mu[0];
+        T b2 = std::fabs(f[0]) * p.mu[1];
+        T u0 = pen[1] * ct1 + lam[1], u1 = pen[2] * ct2 + lam[2];
+        T s_ell = (b1 > T(0) && b2 > T(0)) ? (u0 / b1) * (u0 / b1) + (u1 / b2) * (u1 / b2)
+                                           : std::numeric_limits<T>::infinity();
+        if (touching) {
+          pt.lam[1] = f[1];
+          pt.lam[2] = f[2];
+          if (s_ell <= T(1)) {
+            if (std::fabs(ct1) >= K<T>::C_EPS()) pt.pen[1] = fmin_(pt.pen[1] + K<T>::BETA() * std::fabs(ct1), K<T>::PEN_MAX());
+            if (std::fabs(ct2) >= K<T>::C_EPS()) pt.pen[2] = fmin_(pt.pen[2] + K<T>::BETA() * std::fabs(ct2), K<T>::PEN_MAX());
+          }
+        }
+        T mu_roll = fmax_(bodies[p.a].rolling, bodies[p.b].rolling);
+        T mu_spin = fmax_(bodies[p.a].torsion, bodies[p.b].torsion);
+        if (touching && (mu_roll > T(0) || mu_spin > T(0))) {
+          V3<T> drot_a = quat_diff_vec(bodies[p.a].rot, rot0[p.a], math);
+          V3<T> drot_b = quat_diff_vec(bodies[p.b].rot, rot0[p.b], math);
+          V3<T> wrel = drot_a - drot_b;
+          T lam_n = std::fabs(pt.lam[0]);
+          T rl[3] = {pt.roll[0], pt.roll[1], pt.roll[2]};
+          V3<T> rax[3] = {t1, t2, p.n};
+          T mur[3] = {mu_roll, mu_roll, mu_spin};
+          for (int li = 0; li < 3; ++li) {
+            if (mur[li] <= T(0)) continue;
+            T c = dot(wrel, rax[li]);
+            T cap = mur[li] * lam_n;
+            pt.roll[li] = clampf(K<T>::ROLL_PEN() * c + rl[li], -cap, cap);
+          }
+        }
+      }
+    }
+    for (auto& j : joints) {
+      const Body<T>& a = bodies[j.a];
+      const Body<T>& b = bodies[j.b];
+      V3<T> pa = a.pos + rotate(a.rot, j.la);
+      V3<T> pb = b.pos + rotate(b.rot, j.lb);
+      V3<T> pa0 = pos0[j.a] + rotate(rot0[j.a], j.la);
+      V3<T> pb0 = pos0[j.b] + rotate(rot0[j.b], j.lb);
+      V3<T> live = pa - pb, c0v = pa0 - pb0;
+      for (int k = 0; k < 3; ++k) {
+        T c = live[k] - K<T>::ALPHA() * c0v[k];
+        if (std::fabs(c) >= K<T>::C_EPS()) {
+          j.lam_l[k] += j.pen_l[k] * c;
+          j.pen_l[k] = fmin_(j.pen_l[k] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+        }
+      }
+    }
+  }
+
+  // Deterministic greedy colouring of the awake dynamic bodies: vertices
+  // in descending (hash(h), then ascending h) priority take the smallest
+  // colour unused by already-coloured neighbours; edges are pairs holding
+  // at least one point, and joints, between two swept bodies.
+  static uint32_t prio_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+  }
+  void colour_bodies() {
+    int n = (int)bodies.size();
+    std::vector<uint8_t> vert(n);
+    for (int h = 0; h < n; ++h) vert[h] = solvable(h) && !asleep[h];
+    std::vector<std::vector<int>> adj(n);
+    for (auto& p : pairs)
+      if (!p.pts.empty() && vert[p.a] && vert[p.b]) {
+        adj[p.a].push_back(p.b);
+        adj[p.b].push_back(p.a);
+      }
+    for (auto& j : joints)
+      if (vert[j.a] && vert[j.b]) {
+        adj[j.a].push_back(j.b);
+        adj[j.b].push_back(j.a);
+      }
+    std::vector<int> orderv;
+    for (int h = 0; h < n; ++h)
+      if (vert[h]) orderv.push_back(h);
+    std::sort(orderv.begin(), orderv.end(), [](int x, int y) {
+      uint32_t hx = prio_hash((uint32_t)x), hy = prio_hash((uint32_t)y);
+      return hx != hy ? hx > hy : x < y;
+    });
+    std::fill(colour.begin(), colour.end(), -1);
+    ncolours = 0;
+    std::vector<int> used;
+    for (int h : orderv) {
+      used.clear();
+      for (int o : adj[h])
+        if (colour[o] >= 0) used.push_back(colour[o]);
+      std::sort(used.begin(), used.end());
+      int c = 0;
+      for (int u : used) {
+        if (u == c)
+          ++c;
+        else if (u > c)
+          break;
+      }
+      colour[h] = c;
+      ncolours = std::max(ncolours, c + 1);
+    }
+  }
+
+  void build_incidence() {
+    size_t n = bodies.size();
+    inc_pairs.assign(n, {});
+    inc_joints.assign(n, {});
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      inc_pairs[pairs[i].a].push_back((int)i);
+      inc_pairs[pairs[i].b].push_back((int)i);
+    }
+    for (size_t i = 0; i < joints.size(); ++i) {
+      inc_joints[joints[i].a].push_back((int)i);
+      inc_joints[joints[i].b].push_back((int)i);
+    }
+  }
+
+  // step_inner (mod.rs:846-1143), state-changing phases (events omitted).
+  void step_inner() {
+    const T DT = K<T>::DT();
+    int n = (int)bodies.size();
+    stats = Stats{};
+    wake_joint_motion();
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h)) continue;
+      Body<T>& b = bodies[h];
+      T iw[3][3], iwi[3][3];
+      world_inertia(b.inertia, b.rot, iw);
+      V3<T> dg{b.inertia.x > T(0) ? b.inertia.x : T(0), b.inertia.y > T(0) ? b.inertia.y : T(0),
+               b.inertia.z > T(0) ? b.inertia.z : T(0)};
+      inverse_symmetric(iw, dg, iwi);
+      V3<T> torque = b.torque;
+      b.torque = V3<T>{T(0), T(0), T(0)};
+      b.angvel = b.angvel + mat3_vec(iwi, torque) * DT;
+    }
+    toi_clamp();
+    generate_pairs();
+    wake_joint_motion();
+    for (int h = 0; h < n; ++h)
+      if (asleep[h]) {
+        const Body<T>& b = bodies[h];
+        if (length(b.vel) > K<T>::SLEEP_LIN() || length(b.angvel) > K<T>::SLEEP_ANG()) wake_body(h);
+      }
+    for (int h = 0; h < n; ++h) {
+      if (bodies[h].type == BT_KINEMATIC) {
+        pos0[h] = prev_pos[h];
+        rot0[h] = prev_rot[h];
+      } else {
+        pos0[h] = bodies[h].pos;
+        rot0[h] = bodies[h].rot;
+      }
+      if (!solvable(h)) {
+        inertial[h] = bodies[h].pos;
+        inertial_rot[h] = bodies[h].rot;
+        continue;
+      }
+      const Body<T>& b = bodies[h];
+      inertial[h] = (b.pos + b.vel * DT) + gravity * (DT * DT);
+      inertial_rot[h] = quat_integrate(b.rot, b.angvel * DT, math);
+    }
+    const T AG = K<T>::ALPHA() * K<T>::GAMMA();
+    for (auto& p : pairs)
+      for (auto& pt : p.pts) {
+        for (int k = 0; k < 3; ++k) {
+          pt.lam[k] *= AG;
+          pt.roll[k] *= AG;
+        }
+        for (int k = 0; k < 3; ++k) pt.pen[k] = clampf(pt.pen[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
+      }
+    for (auto& j : joints)
+      for (int k = 0; k < 3; ++k) {
+        j.lam_l[k] *= AG;
+        j.pen_l[k] = clampf(j.pen_l[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
+      }
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h) || asleep[h]) continue;
+      bodies[h].pos = inertial[h];
+      bodies[h].rot = inertial_rot[h];
+    }
+    build_incidence();
+    if (order == ORDER_COLOUR) {
+      colour_bodies();
+      std::vector<std::vector<int>> cls(ncolours);
+      for (int h = 0; h < n; ++h)
+        if (colour[h] >= 0) cls[colour[h]].push_back(h);
+      for (int it = 0; it < iters; ++it) {
+        for (int c = 0; c < ncolours; ++c)
+          for (int h : cls[c]) solve_body(h);
+        dual_update();
+      }
+      stats.colours = ncolours;
+      for (auto& c : cls) stats.max_colour = std::max<int>(stats.max_colour, (int)c.size());
+    } else {
+      std::fill(colour.begin(), colour.end(), -1);
+      ncolours = 0;
+      for (int it = 0; it < iters; ++it) {
+        for (int h = n - 1; h >= 0; --h)
+          if (solvable(h) && !asleep[h]) solve_body(h);
+        dual_update();
+      }
+    }
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h) || asleep[h]) continue;
+      Body<T>& b = bodies[h];
+      b.vel = (b.pos - pos0[h]) / DT;
+      V3<T> spin = quat_diff_vec(b.rot, rot0[h], math);
+      b.angvel = length(spin) < K<T>::SPIN_REST() ? V3<T>{T(0), T(0), T(0)} : spin / DT;
+      b.rot = qnormalize(b.rot);
+    }
+    std::vector<uint8_t> supported(n, 0);
+    for (auto& p : pairs)
+      if (p.gap <= T(0) && !p.pts.empty()) {
+        supported[p.a] = 1;
+        supported[p.b] = 1;
+      }
+    for (auto& j : joints) {
+      supported[j.a] = 1;
+      supported[j.b] = 1;
+    }
+    std::vector<uint8_t> ready = joint_sleep_ready();
+    std::vector<uint8_t> freeze = asleep;
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h) || asleep[h]) continue;
+      if (!supported[h] || !ready[h]) {
+        sleep_timer[h] = T(0);
+        continue;
+      }
+      const Body<T>& b = bodies[h];
+      if (length(b.vel) < K<T>::SLEEP_LIN() && length(b.angvel) < K<T>::SLEEP_ANG()) {
+        sleep_timer[h] += DT;
+        freeze[h] = sleep_timer[h] >= K<T>::SLEEP_TIME();
+      } else {
+        sleep_timer[h] = T(0);
+      }
+    }
+    propagate_joint_flags(freeze, 0);
+    for (int h = 0; h < n; ++h)
+      if (freeze[h] && !asleep[h]) sleep_body(h);
+    for (int h = 0; h < n; ++h) {
+      prev_pos[h] = bodies[h].pos;
+      prev_rot[h] = bodies[h].rot;
+    }
+    stats.pairs = (int)pairs.size();
+    for (auto& p : pairs) stats.points += (int)p.pts.size();
+    for (int h = 0; h < n; ++h) stats.sleeping += asleep[h] ? 1 : 0;
+  }
+
+  // PhysicsEngine::step (mod.rs:1212-1235).
+  void step(T dt) {
+    if (!std::isfinite(dt) || dt <= T(0)) return;
+    const T DT = K<T>::DT();
+    time_debt = fmin_(time_debt + dt, DT * T(float(MAX_SUBSTEPS)));
+    int k = 0;
+    while (time_debt >= DT && k < MAX_SUBSTEPS) {
+      time_debt -= DT;
+      ++k;
+      step_inner();
+    }
+  }
+};
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C API (ctypes) — float in/out for both precisions.
+// ---------------------------------------------------------------------------
+using namespace orc;
+
+struct OracleCtx {
+  int prec;
+  Engine<float>* f = nullptr;
+  Engine<double>* d = nullptr;
+};
+
+template <class T>
+static Engine<T>* mk(const float* g, int order, int math, int iters, int brute) {
+  Engine<T>* e = new Engine<T>();
+  e->gravity = V3<T>{T(g[0]), T(g[1]), T(g[2])};
+  e->order = order;
+  e->math = math;
+  e->iters = iters;
+  e->brute = brute != 0;
+  return e;
+}
+
+#define DISPATCH(ctx, expr)     \
+  do {                          \
+    if ((ctx)->prec == 64) {    \
+      auto* E = (ctx)->d;       \
+      expr;                     \
+    } else {                    \
+      auto* E = (ctx)->f;       \
+      expr;                     \
+    }                           \
+  } while (0)
+
+template <class T>
+static void add_bodies(Engine<T>* E, int n, const int* shape, const float* sp, const float* pos, const float* quat,
+                       const float* vel, const float* angvel, const float* mass, const float* friction,
+                       const float* restitution, const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger,
+                       const int* type, const float* rolling, const float* torsion) {
+  for (int i = 0; i < n; ++i) {
+    Body<T> b;
+    b.shape = shape[i];
+    b.sp = V3<T>{T(sp[3 * i]), T(sp[3 * i + 1]), T(sp[3 * i + 2])};
+    b.pos = V3<T>{T(pos[3 * i]), T(pos[3 * i + 1]), T(pos[3 * i + 2])};
+    b.rot = Q<T>{T(quat[4 * i]), T(quat[4 * i + 1]), T(quat[4 * i + 2]), T(quat[4 * i + 3])};
+    b.vel = V3<T>{T(vel[3 * i]), T(vel[3 * i + 1]), T(vel[3 * i + 2])};
+    b.angvel = V3<T>{T(angvel[3 * i]), T(angvel[3 * i + 1]), T(angvel[3 * i + 2])};
+    b.torque = V3<T>{T(0), T(0), T(0)};
+    T m = T(mass[i]);
+    b.type = type[i];
+    if (b.type == BT_DYNAMIC && !(m > T(0))) b.type = BT_STATIC;
+    if (b.type == BT_DYNAMIC) {
+      b.mass = m;
+      b.inv_mass = T(1) / m;
+      b.inertia = shape_inertia(b.shape, b.sp, m);
+    } else {
+      b.mass = T(0);
+      b.inv_mass = T(0);
+      b.inertia = V3<T>{T(0), T(0), T(0)};
+    }
+    b.friction = T(friction[i]);
+    b.restitution = T(restitution[i]);
+    b.rolling = rolling ? T(rolling[i]) : T(0);
+    b.torsion = torsion ? T(torsion[i]) : T(0);
+    b.layer = layer[i];
+    b.mask = mask[i];
+    b.trigger = trigger[i] != 0;
+    E->add_body(b);
+  }
+}
+
+extern "C" {
+
+void* orc_create(int prec, const float* gravity, int order, int math, int iters, int brute) {
+  OracleCtx* c = new OracleCtx();
+  c->prec = prec == 64 ? 64 : 32;
+  if (c->prec == 64)
+    c->d = mk<double>(gravity, order, math, iters, brute);
+  else
+    c->f = mk<float>(gravity, order, math, iters, brute);
+  return c;
+}
+
+void orc_destroy(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  delete c->f;
+  delete c->d;
+  delete c;
+}
+
+void orc_add_bodies(void* h, int n, const int* shape, const float* sp, const float* pos, const float* quat,
+                    const float* vel, const float* angvel, const float* mass, const float* friction,
+                    const float* restitution, const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger,
+                    const int* type, const float* rolling, const float* torsion) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, add_bodies(E, n, shape, sp, pos, quat, vel, angvel, mass, friction, restitution, layer, mask, trigger,
+                         type, rolling, torsion));
+}
+
+int orc_add_ball_joint(void* h, int a, int b, const float* la, const float* lb) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = -1;
+  if (c->prec == 64)
+    r = c->d->add_ball_joint(a, b, V3<double>{la[0], la[1], la[2]}, V3<double>{lb[0], lb[1], lb[2]});
+  else
+    r = c->f->add_ball_joint(a, b, V3<float>{la[0], la[1], la[2]}, V3<float>{lb[0], lb[1], lb[2]});
+  return r;
+}
+
+void orc_step(void* h, float dt, int nsteps) {
+  OracleCtx* c = (OracleCtx*)h;
+  for (int i = 0; i < nsteps; ++i) DISPATCH(c, E->step(decltype(E->time_debt)(dt)));
+}
+
+void orc_step_inner(void* h, int nsteps) {
+  OracleCtx* c = (OracleCtx*)h;
+  for (int i = 0; i < nsteps; ++i) DISPATCH(c, E->step_inner());
+}
+
+int orc_body_count(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  int n = 0;
+  DISPATCH(c, n = (int)E->bodies.size());
+  return n;
+}
+
+// out: n x 13 floats (pos3, quat4, vel3, angvel3), double precision cast.
+void orc_get_state(void* h, float* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      auto& b = E->bodies[i];
+      float* o = out + 13 * i;
+      o[0] = (float)b.pos.x; o[1] = (float)b.pos.y; o[2] = (float)b.pos.z;
+      o[3] = (float)b.rot.x; o[4] = (float)b.rot.y; o[5] = (float)b.rot.z; o[6] = (float)b.rot.w;
+      o[7] = (float)b.vel.x; o[8] = (float)b.vel.y; o[9] = (float)b.vel.z;
+      o[10] = (float)b.angvel.x; o[11] = (float)b.angvel.y; o[12] = (float)b.angvel.z;
+    }
+  });
+}
+
+void orc_get_state_f64(void* h, double* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      auto& b = E->bodies[i];
+      double* o = out + 13 * i;
+      o[0] = b.pos.x; o[1] = b.pos.y; o[2] = b.pos.z;
+      o[3] = b.rot.x; o[4] = b.rot.y; o[5] = b.rot.z; o[6] = b.rot.w;
+      o[7] = b.vel.x; o[8] = b.vel.y; o[9] = b.vel.z;
+      o[10] = b.angvel.x; o[11] = b.angvel.y; o[12] = b.angvel.z;
+    }
+  });
+}
+
+int orc_pair_count(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  int n = 0;
+  DISPATCH(c, n = (int)E->pairs.size());
+  return n;
+}
+
+// pairs in vector order: ab[2*i], ab[2*i+1]; npts[i]; gap[i]; normal[3*i].
+void orc_get_pairs(void* h, uint32_t* ab, uint32_t* npts, float* gap, float* normal) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->pairs.size(); ++i) {
+      auto& p = E->pairs[i];
+      if (ab) { ab[2 * i] = (uint32_t)p.a; ab[2 * i + 1] = (uint32_t)p.b; }
+      if (npts) npts[i] = (uint32_t)p.pts.size();
+      if (gap) gap[i] = (float)p.gap;
+      if (normal) { normal[3 * i] = (float)p.n.x; normal[3 * i + 1] = (float)p.n.y; normal[3 * i + 2] = (float)p.n.z; }
+    }
+  });
+}
+
+// points in pair-vector order, 16 floats each:
+// ra3 rb3 lam3 pen3 roll3 stuck.
+void orc_get_points(void* h, float* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    size_t k = 0;
+    for (auto& p : E->pairs)
+      for (auto& pt : p.pts) {
+        float* o = out + 16 * k++;
+        o[0] = (float)pt.ra.x; o[1] = (float)pt.ra.y; o[2] = (float)pt.ra.z;
+        o[3] = (float)pt.rb.x; o[4] = (float)pt.rb.y; o[5] = (float)pt.rb.z;
+        for (int q = 0; q < 3; ++q) { o[6 + q] = (float)pt.lam[q]; o[9 + q] = (float)pt.pen[q]; o[12 + q] = (float)pt.roll[q]; }
+        o[15] = pt.stuck ? 1.0f : 0.0f;
+      }
+  });
+}
+
+void orc_get_colours(void* h, int32_t* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->colour.size(); ++i) out[i] = E->colour[i];
+  });
+}
+
+void orc_get_sleep(void* h, uint8_t* asleep, float* timer) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      if (asleep) asleep[i] = E->asleep[i];
+      if (timer) timer[i] = (float)E->sleep_timer[i];
+    }
+  });
+}
+
+// joints: 6 floats each: lam_l3 pen_l3.
+void orc_get_joints(void* h, float* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->joints.size(); ++i)
+      for (int k = 0; k < 3; ++k) {
+        out[6 * i + k] = (float)E->joints[i].lam_l[k];
+        out[6 * i + 3 + k] = (float)E->joints[i].pen_l[k];
+      }
+  });
+}
+
+// stats: pairs, points, colours, max_colour, sleeping, breakdowns, toi_hits
+void orc_get_stats(void* h, int32_t* out) {
+  OracleCtx* c = (OracleCtx*)h;
+  Stats s{};
+  DISPATCH(c, s = E->stats);
+  out[0] = s.pairs; out[1] = s.points; out[2] = s.colours; out[3] = s.max_colour;
+  out[4] = s.sleeping; out[5] = s.breakdowns; out[6] = s.toi_hits;
+}
+
+// Unit hooks for analytic pins.
+int orc_solve_6x6_f32(const float* lhs36, const float* rhs6, float* x6) {
+  float l[6][6], r[6];
+  std::memcpy(l, lhs36, sizeof(l));
+  std::memcpy(r, rhs6, sizeof(r));
+  return Engine<float>::solve_6x6(l, r, x6) ? 1 : 0;
+}
+
+void orc_stamp_row_f32(float* lhs36, float* rhs6, const float* axis, float pen, float f, const float* r, float sign) {
+  float l[6][6], rh[6];
+  std::memcpy(l, lhs36, sizeof(l));
+  std::memcpy(rh, rhs6, sizeof(rh));
+  Engine<float>::stamp_row(l, rh, V3<float>{axis[0], axis[1], axis[2]}, pen, f, V3<float>{r[0], r[1], r[2]}, sign);
+  std::memcpy(lhs36, l, sizeof(l));
+  std::memcpy(rhs6, rh, sizeof(rh));
+}
+
+void orc_contact_force_f32(float cn, const float* pens, const float* lam, float ct0, float ct1, const float* mu,
+                           float* f) {
+  Engine<float>::contact_force(cn, pens, lam, ct0, ct1, mu, f);
+}
+
+void orc_quat_integrate_f32(const float* q, const float* v, int math, float* out) {
+  Q<float> r = quat_integrate(Q<float>{q[0], q[1], q[2], q[3]}, V3<float>{v[0], v[1], v[2]}, math);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
+
+void orc_quat_diff_vec_f32(const float* a, const float* b, int math, float* out) {
+  V3<float> r = quat_diff_vec(Q<float>{a[0], a[1], a[2], a[3]}, Q<float>{b[0], b[1], b[2], b[3]}, math);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+void orc_tangent_basis_f32(const float* n, float* t1, float* t2) {
+  V3<float> a, b;
+  tangent_basis(V3<float>{n[0], n[1], n[2]}, &a, &b);
+  t1[0] = a.x; t1[1] = a.y; t1[2] = a.z;
+  t2[0] = b.x; t2[1] = b.y; t2[2] = b.z;
+}
+
+// shape_distance between two placed shapes: out = dist, pa3, pb3, sat.
+void orc_shape_distance_f32(int sa, const float* pa_sp, const float* pa_pos, const float* pa_q, int sb,
+                            const float* pb_sp, const float* pb_pos, const float* pb_q, float* out) {
+  SRef<float> a{sa, {pa_sp[0], pa_sp[1], pa_sp[2]}, {pa_pos[0], pa_pos[1], pa_pos[2]}, {pa_q[0], pa_q[1], pa_q[2], pa_q[3]}};
+  SRef<float> b{sb, {pb_sp[0], pb_sp[1], pb_sp[2]}, {pb_pos[0], pb_pos[1], pb_pos[2]}, {pb_q[0], pb_q[1], pb_q[2], pb_q[3]}};
+  float sat;
+  Dist<float> d = shape_distance(a, b, &sat);
+  out[0] = d.dist;
+  out[1] = d.pa.x; out[2] = d.pa.y; out[3] = d.pa.z;
+  out[4] = d.pb.x; out[5] = d.pb.y; out[6] = d.pb.z;
+  out[7] = sat;
+}
+
+void orc_fmath_sincos(float x, float* s, float* c) { oavbd_sincosf(x, s, c); }
+float orc_fmath_asin(float x) { return oavbd_asinf(x); }
+
+}  // extern "C"

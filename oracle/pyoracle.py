@@ -1,0 +1,177 @@
+"""ctypes front end of the CPU oracle (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module.  It drives oracle/liboavbd_oracle.so, the C++ restatement of
+the reference AVBD engine (crates/physics/src/avbd/*.rs).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboavbd_oracle.so")
+_lib = None
+
+ORDER_REFERENCE = 0
+ORDER_COLOUR = 1
+MATH_LIBM = 0
+MATH_PORTABLE = 1
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_add_bodies.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                     _f32p, _f32p, _f32p, _u32p, _u32p, _u8p, _i32p, _f32p, _f32p]
+        L.orc_add_ball_joint.restype = ctypes.c_int
+        L.orc_add_ball_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]
+        L.orc_step_inner.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_body_count.restype = ctypes.c_int
+        L.orc_body_count.argtypes = [ctypes.c_void_p]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, _f32p]
+        L.orc_get_state_f64.argtypes = [ctypes.c_void_p, _f64p]
+        L.orc_pair_count.restype = ctypes.c_int
+        L.orc_pair_count.argtypes = [ctypes.c_void_p]
+        L.orc_get_pairs.argtypes = [ctypes.c_void_p, _u32p, _u32p, _f32p, _f32p]
+        L.orc_get_points.argtypes = [ctypes.c_void_p, _f32p]
+        L.orc_get_colours.argtypes = [ctypes.c_void_p, _i32p]
+        L.orc_get_sleep.argtypes = [ctypes.c_void_p, _u8p, _f32p]
+        L.orc_get_joints.argtypes = [ctypes.c_void_p, _f32p]
+        L.orc_get_stats.argtypes = [ctypes.c_void_p, _i32p]
+        L.orc_solve_6x6_f32.restype = ctypes.c_int
+        L.orc_solve_6x6_f32.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_stamp_row_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float]
+        L.orc_contact_force_f32.argtypes = [ctypes.c_float, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p, _f32p]
+        L.orc_quat_integrate_f32.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p]
+        L.orc_quat_diff_vec_f32.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p]
+        L.orc_tangent_basis_f32.argtypes = [_f32p, _f32p, _f32p]
+        L.orc_shape_distance_f32.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
+        L.orc_fmath_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
+        L.orc_fmath_asin.restype = ctypes.c_float
+        L.orc_fmath_asin.argtypes = [ctypes.c_float]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class OracleEngine:
+    """CPU AVBD engine: ORDER_REFERENCE (the reference's sweep) or
+    ORDER_COLOUR (the GPU step's order), float32 or float64."""
+
+    def __init__(self, scene, order=ORDER_REFERENCE, math=MATH_LIBM, precision=32, iters=10, brute=False):
+        self._L = lib()
+        g = f32(scene.get("gravity", (0.0, -9.81, 0.0)))
+        self.h = self._L.orc_create(int(precision), _p(g, _f32p), int(order), int(math), int(iters), int(brute))
+        self.n = 0
+        self.add_bodies(scene)
+        for (a, b, la, lb) in scene.get("joints", []):
+            r = self._L.orc_add_ball_joint(self.h, int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p))
+            if r < 0:
+                raise ValueError("bad joint")
+
+    def add_bodies(self, s):
+        n = len(s["shape"])
+        shape = np.ascontiguousarray(s["shape"], dtype=np.int32)
+        arrs = [f32(s[k]) for k in ("shape_param", "pos", "quat", "vel", "angvel", "mass", "friction", "restitution")]
+        layer = np.ascontiguousarray(s.get("layer", np.ones(n)), dtype=np.uint32)
+        mask = np.ascontiguousarray(s.get("mask", np.full(n, 0xFFFFFFFF)), dtype=np.uint32)
+        trig = np.ascontiguousarray(s.get("trigger", np.zeros(n)), dtype=np.uint8)
+        btype = np.ascontiguousarray(s["body_type"], dtype=np.int32)
+        roll = f32(s.get("rolling", np.zeros(n)))
+        tors = f32(s.get("torsion", np.zeros(n)))
+        self._L.orc_add_bodies(self.h, n, _p(shape, _i32p), *[_p(a, _f32p) for a in arrs], _p(layer, _u32p),
+                               _p(mask, _u32p), _p(trig, _u8p), _p(btype, _i32p), _p(roll, _f32p), _p(tors, _f32p))
+        self.n += n
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self._L.orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, dt=1.0 / 60.0, nsteps=1):
+        self._L.orc_step(self.h, ctypes.c_float(dt), int(nsteps))
+
+    def step_inner(self, nsteps=1):
+        self._L.orc_step_inner(self.h, int(nsteps))
+
+    def state(self):
+        out = np.zeros((self.n, 13), np.float32)
+        self._L.orc_get_state(self.h, _p(out, _f32p))
+        return out
+
+    def state_f64(self):
+        out = np.zeros((self.n, 13), np.float64)
+        self._L.orc_get_state_f64(self.h, _p(out, _f64p))
+        return out
+
+    def pairs(self):
+        m = self._L.orc_pair_count(self.h)
+        ab = np.zeros((m, 2), np.uint32)
+        npts = np.zeros(m, np.uint32)
+        gap = np.zeros(m, np.float32)
+        nrm = np.zeros((m, 3), np.float32)
+        self._L.orc_get_pairs(self.h, _p(ab, _u32p), _p(npts, _u32p), _p(gap, _f32p), _p(nrm, _f32p))
+        return ab, npts, gap, nrm
+
+    def points(self):
+        _, npts, _, _ = self.pairs()
+        out = np.zeros((int(npts.sum()), 16), np.float32)
+        self._L.orc_get_points(self.h, _p(out, _f32p))
+        return out
+
+    def colours(self):
+        out = np.zeros(self.n, np.int32)
+        self._L.orc_get_colours(self.h, _p(out, _i32p))
+        return out
+
+    def sleep(self):
+        a = np.zeros(self.n, np.uint8)
+        t = np.zeros(self.n, np.float32)
+        self._L.orc_get_sleep(self.h, _p(a, _u8p), _p(t, _f32p))
+        return a, t
+
+    def joints(self, nj):
+        out = np.zeros((nj, 6), np.float32)
+        self._L.orc_get_joints(self.h, _p(out, _f32p))
+        return out
+
+    def stats(self):
+        out = np.zeros(7, np.int32)
+        self._L.orc_get_stats(self.h, _p(out, _i32p))
+        return dict(zip(["pairs", "points", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits"],
+                        out.tolist()))

@@ -1,0 +1,173 @@
+"""Scene construction for the AVBD step: body arrays in the layout the C ABI
+(`include/oavbd.h`) takes, plus seeded generators for the benchmark
+workloads.
+
+Material defaults mirror the reference constructors
+(crates/physics/src/body.rs:20-32): boxes restitution 0.3 / friction 0.5,
+spheres 0.5 / 0.3, half-space planes 0.3 / 0.6
+(`RigidBody::try_new_halfspace`, body.rs:873-889).  Mass 0 means static
+(`RigidBody::build`, body.rs:346-376).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SHAPE_SPHERE = 0
+SHAPE_BOX = 1
+SHAPE_HALFSPACE = 2
+
+BODY_STATIC = 0
+BODY_DYNAMIC = 1
+BODY_KINEMATIC = 2
+
+_FIELDS3 = ("shape_param", "pos", "vel", "angvel")
+
+
+class SceneBuilder:
+    """Accumulates bodies and ball joints; `build()` returns numpy arrays."""
+
+    def __init__(self, gravity=(0.0, -9.81, 0.0)):
+        self.gravity = tuple(float(g) for g in gravity)
+        self.rows = []
+        self.joints = []
+
+    def _add(self, shape, sp, pos, mass, restitution, friction, quat=(0.0, 0.0, 0.0, 1.0),
+             vel=(0.0, 0.0, 0.0), angvel=(0.0, 0.0, 0.0), body_type=None):
+        if body_type is None:
+            body_type = BODY_DYNAMIC if mass > 0 else BODY_STATIC
+        self.rows.append(dict(shape=shape, shape_param=sp, pos=pos, quat=quat, vel=vel, angvel=angvel,
+                              mass=mass, friction=friction, restitution=restitution, body_type=body_type))
+        return len(self.rows) - 1
+
+    def box(self, pos, half, mass, **kw):
+        kw.setdefault("restitution", 0.3)
+        kw.setdefault("friction", 0.5)
+        return self._add(SHAPE_BOX, tuple(half), tuple(pos), float(mass), **kw)
+
+    def sphere(self, pos, radius, mass, **kw):
+        kw.setdefault("restitution", 0.5)
+        kw.setdefault("friction", 0.3)
+        return self._add(SHAPE_SPHERE, (float(radius), 0.0, 0.0), tuple(pos), float(mass), **kw)
+
+    def halfspace(self, pos, normal, **kw):
+        kw.setdefault("restitution", 0.3)
+        kw.setdefault("friction", 0.6)
+        n = np.asarray(normal, np.float32)
+        n = n * (np.float32(1.0) / np.float32(np.sqrt(np.float32(np.dot(n, n)))))
+        return self._add(SHAPE_HALFSPACE, tuple(float(x) for x in n), tuple(pos), 0.0, **kw)
+
+    def ball_joint(self, a, b, la, lb):
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb)))
+        return len(self.joints) - 1
+
+    def build(self):
+        n = len(self.rows)
+        s = {"gravity": np.asarray(self.gravity, np.float32)}
+        s["shape"] = np.array([r["shape"] for r in self.rows], np.int32)
+        for k in _FIELDS3:
+            s[k] = np.array([r[k] for r in self.rows], np.float32).reshape(n, 3)
+        s["quat"] = np.array([r["quat"] for r in self.rows], np.float32).reshape(n, 4)
+        for k in ("mass", "friction", "restitution"):
+            s[k] = np.array([r[k] for r in self.rows], np.float32)
+        s["body_type"] = np.array([r["body_type"] for r in self.rows], np.int32)
+        s["layer"] = np.ones(n, np.uint32)
+        s["mask"] = np.full(n, 0xFFFFFFFF, np.uint32)
+        s["trigger"] = np.zeros(n, np.uint8)
+        s["rolling"] = np.zeros(n, np.float32)
+        s["torsion"] = np.zeros(n, np.float32)
+        s["joints"] = list(self.joints)
+        return s
+
+
+def concat(scenes):
+    """Concatenate scenes into one body set (joint indices shifted)."""
+    out = {"gravity": scenes[0]["gravity"]}
+    keys = [k for k in scenes[0] if k not in ("gravity", "joints")]
+    for k in keys:
+        out[k] = np.concatenate([s[k] for s in scenes])
+    joints = []
+    off = 0
+    for s in scenes:
+        for (a, b, la, lb) in s.get("joints", []):
+            joints.append((a + off, b + off, la, lb))
+        off += len(s["shape"])
+    out["joints"] = joints
+    return out
+
+
+def _uniform_quats(rng, n):
+    """Shoemake's uniform random unit quaternions (x, y, z, w)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * math.pi * u2), a * np.cos(2 * math.pi * u2),
+                  b * np.sin(2 * math.pi * u3), b * np.cos(2 * math.pi * u3)], axis=1)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(np.float32)
+
+
+def box_pyramid(base=20, rows=20, edge=1.0, density=1000.0, friction=0.5):
+    """Box pyramid: row r holds base - r unit cubes resting on a static plane
+    at y = 0 (the plain CPU-path workload: 210 cubes for base = rows = 20)."""
+    sb = SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=friction, restitution=0.0)
+    h = edge * 0.5
+    mass = float(np.float32(density) * np.float32(edge) ** 3)
+    for r in range(rows):
+        cnt = base - r
+        if cnt <= 0:
+            break
+        x0 = -0.5 * (cnt - 1) * edge
+        for i in range(cnt):
+            sb.box((x0 + i * edge, h + r * edge, 0.0), (h, h, h), mass, friction=friction, restitution=0.0)
+    return sb.build()
+
+
+def box_drop_lattice(nx=50, nz=50, ny=40, spacing=2.0, jitter=0.1, hmin=0.25, hmax=0.75,
+                     density=1000.0, mu=(0.3, 0.8), container=110.0, seed=0xA5BD2026):
+    """Random boxes in an nx x nz column lattice, ny layers high, dropped into
+    a walled container on a static plane (100,000 boxes at the defaults)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = nx * nz * ny
+    ix, iz, iy = np.meshgrid(np.arange(nx), np.arange(nz), np.arange(ny), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    pos = np.stack([-(nx - 1) * spacing * 0.5 + spacing * ix,
+                    2.0 + spacing * iy,
+                    -(nz - 1) * spacing * 0.5 + spacing * iz], axis=1)
+    pos = pos + rng.uniform(-jitter, jitter, size=(n, 3))
+    half = rng.uniform(hmin, hmax, size=(n, 3)).astype(np.float32)
+    fr = rng.uniform(mu[0], mu[1], size=n).astype(np.float32)
+    quat = _uniform_quats(rng, n)
+    vol = np.float32(8.0) * half[:, 0] * half[:, 1] * half[:, 2]
+    mass = (np.float32(density) * vol).astype(np.float32)
+    sb = SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=float(mu[1]), restitution=0.0)
+    c = container * 0.5
+    wall_h, wall_t = 20.0, 1.0
+    for p, hh in [((c + wall_t, wall_h, 0.0), (wall_t, wall_h, c + 2 * wall_t)),
+                  ((-c - wall_t, wall_h, 0.0), (wall_t, wall_h, c + 2 * wall_t)),
+                  ((0.0, wall_h, c + wall_t), (c, wall_h, wall_t)),
+                  ((0.0, wall_h, -c - wall_t), (c, wall_h, wall_t))]:
+        sb.box(p, hh, 0.0, friction=float(mu[1]), restitution=0.0)
+    statics = sb.build()
+    dyn = {
+        "gravity": statics["gravity"],
+        "shape": np.full(n, SHAPE_BOX, np.int32),
+        "shape_param": half,
+        "pos": pos.astype(np.float32),
+        "quat": quat,
+        "vel": np.zeros((n, 3), np.float32),
+        "angvel": np.zeros((n, 3), np.float32),
+        "mass": mass,
+        "friction": fr,
+        "restitution": np.zeros(n, np.float32),
+        "body_type": np.full(n, BODY_DYNAMIC, np.int32),
+        "layer": np.ones(n, np.uint32),
+        "mask": np.full(n, 0xFFFFFFFF, np.uint32),
+        "trigger": np.zeros(n, np.uint8),
+        "rolling": np.zeros(n, np.float32),
+        "torsion": np.zeros(n, np.float32),
+        "joints": [],
+    }
+    return concat([statics, dyn])
