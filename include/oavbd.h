@@ -1,0 +1,153 @@
+/*
+ * oavbd.h — C ABI of the MI355X AVBD rigid-body step (liboavbd.so).
+ *
+ * This is the device boundary behind the reference's physics-solver seam:
+ * a Rust `GpuAvbdDispatch` implementation (crates/physics/src/gpu/avbd.rs:
+ * 146-151, consulted by AvbdEngine::step, crates/physics/src/avbd/mod.rs:
+ * 1220-1223) would hold one `oavbd_ctx*` and call these entry points
+ * through `extern "C"`.  Plain pointers and sizes only; no C++ exceptions
+ * cross this boundary; every entry point returns an oavbd_status.
+ *
+ * Threading: a context is used by one host thread at a time (the
+ * reference's `step(&mut self)`, crates/physics/src/engine.rs:24);
+ * separate contexts may run concurrently.  All device work of a context
+ * goes to its own HIP stream.  Host arrays passed in are borrowed for the
+ * duration of the call only.
+ *
+ * Bodies are dense indices 0..n-1 (the reference's BodyHandle,
+ * crates/physics/src/body.rs).  Shapes: 0 sphere (param[0] = radius),
+ * 1 box (param = half extents), 2 half-space (param = outward unit normal,
+ * static only).  Body types: 0 static, 1 dynamic, 2 kinematic.  Mass 0
+ * makes a dynamic body static (RigidBody::build, body.rs:346-376).
+ *
+ * Scenes: every body carries a scene id; bodies of different scenes never
+ * collide, so one context steps thousands of independent scenes in one
+ * launch sequence.  Within a scene the result equals stepping that scene
+ * alone.
+ */
+#ifndef OAVBD_H
+#define OAVBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum oavbd_status {
+  OAVBD_OK = 0,
+  OAVBD_EINVAL = -1,    /* bad argument (null pointer, index out of range)  */
+  OAVBD_ECAPACITY = -2, /* more bodies / joints than the context holds      */
+  OAVBD_EDEVICE = -3,   /* HIP runtime error; see oavbd_last_error           */
+  OAVBD_ENOMEM = -4     /* device allocation failed                          */
+} oavbd_status;
+
+typedef struct oavbd_ctx oavbd_ctx;
+
+/* Solver parameters.  Defaults (oavbd_default_params) equal the reference
+ * constants (crates/physics/src/avbd/mod.rs:154-247). */
+typedef struct oavbd_params {
+  float gravity[3]; /* world acceleration, default (0, -9.81, 0)            */
+  uint32_t iters;   /* primal/dual iterations per step, default 10 (ITERS)  */
+} oavbd_params;
+
+/* Per-step counters (observability; the reference's sleeping_count,
+ * crates/physics/src/avbd/mod.rs:788-790, plus solver sizes). */
+typedef struct oavbd_stats {
+  uint32_t pairs;       /* contact pairs after the merge                     */
+  uint32_t points;      /* contact points                                    */
+  uint32_t candidates;  /* broadphase candidate pairs                        */
+  uint32_t colours;     /* colour classes swept                              */
+  uint32_t max_colour;  /* bodies in the largest colour                      */
+  uint32_t sleeping;    /* dynamic bodies frozen by sleep                    */
+  uint32_t breakdowns;  /* 6x6 LDL breakdowns (body kept its pose)           */
+  uint32_t toi_hits;    /* TOI clamps applied                                */
+  uint32_t colour_rounds; /* Jones-Plassmann rounds of the colouring         */
+} oavbd_stats;
+
+void oavbd_default_params(oavbd_params* p);
+
+/* Create a context able to hold max_bodies bodies and max_joints joints on
+ * HIP device `device`.  Replaces: AvbdEngine::new (mod.rs:1183-1204) plus
+ * the device set-up of WgpuAvbdSolver::new (crates/physics/src/gpu/avbd.rs:
+ * 502-632). */
+int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_joints, int device,
+                 oavbd_ctx** out);
+void oavbd_destroy(oavbd_ctx* ctx);
+
+/* Replace all bodies (and drop joints, pairs and warm starts).  Arrays of n
+ * entries; vectors are packed xyz (3 floats) or xyzw (4 floats).  Optional
+ * arrays may be NULL: layer (default 1), mask (default all), trigger (0),
+ * rolling/torsion friction (0), scene (0).  Replaces: repeated
+ * PhysicsEngine::add_body (mod.rs:1236-1239) and WgpuAvbdSolver::upload
+ * (crates/physics/src/gpu/avbd.rs:645-656). */
+int oavbd_set_bodies(oavbd_ctx* ctx, uint32_t n, const int32_t* shape, const float* shape_param3,
+                     const float* pos3, const float* quat4, const float* vel3, const float* angvel3,
+                     const float* mass, const float* friction, const float* restitution,
+                     const int32_t* body_type, const uint32_t* layer, const uint32_t* mask,
+                     const uint8_t* trigger, const float* rolling, const float* torsion,
+                     const uint32_t* scene);
+
+/* Ball joint between bodies a and b with body-local anchors.  Replaces:
+ * PhysicsEngine::add_joint with JointKind::Ball (mod.rs:1308-1510,
+ * crates/physics/src/joint.rs:1273-1279).  Returns the joint index through
+ * *out_index (may be NULL).  EINVAL for a == b or an out-of-range body. */
+int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                         uint32_t* out_index);
+
+/* Advance by dt seconds through the reference's fixed-step accumulator
+ * (1/60 s substeps, at most 4 per call; non-finite or non-positive dt is
+ * a no-op).  Replaces: PhysicsEngine::step for AVBD (mod.rs:1212-1235). */
+int oavbd_step(oavbd_ctx* ctx, float dt);
+
+/* Run exactly nsteps fixed 1/60 s substeps (step_inner, mod.rs:846-1143). */
+int oavbd_step_n(oavbd_ctx* ctx, uint32_t nsteps);
+
+/* Copy the body state to host: n x 13 floats (pos3, quat4, vel3, angvel3),
+ * the layout of the reference's snapshot render (crates/physics/tests/
+ * avbd_engine.rs:1605-1630).  Replaces: get_body reads / try_download
+ * (crates/physics/src/gpu/avbd.rs:712-758). */
+int oavbd_get_state(oavbd_ctx* ctx, float* out13, uint32_t n);
+
+/* Overwrite the body state from host (same layout as oavbd_get_state);
+ * the reference's get_body_mut pose/velocity edits. */
+int oavbd_set_state(oavbd_ctx* ctx, const float* in13, uint32_t n);
+
+/* Contact pairs in canonical (a, b) order: ab[2i], ab[2i+1]; *n receives
+ * the pair count (up to cap entries are written; ab may be NULL to query). */
+int oavbd_get_pairs(oavbd_ctx* ctx, uint32_t* ab, uint32_t cap, uint32_t* n);
+
+/* Colour of every body in the last step's sweep (-1: not swept). */
+int oavbd_get_colours(oavbd_ctx* ctx, int32_t* colour, uint32_t n);
+
+/* Sleep flags and timers (reference sleep_body / sleep_timer). */
+int oavbd_get_sleep(oavbd_ctx* ctx, uint8_t* asleep, float* timer, uint32_t n);
+
+/* Contact points in pair order, 16 floats each: ra3 rb3 lam3 pen3 roll3
+ * stuck; *n receives the point count. */
+int oavbd_get_points(oavbd_ctx* ctx, float* out16, uint32_t cap, uint32_t* n);
+
+/* Ball-joint duals: 6 floats each (lam3, pen3). */
+int oavbd_get_joints(oavbd_ctx* ctx, float* out6, uint32_t n);
+
+int oavbd_get_stats(oavbd_ctx* ctx, oavbd_stats* out);
+
+/* Per-kernel-class device time of the last step in milliseconds, measured
+ * with HIP events on the context stream: [0] setup (TOI, broadphase,
+ * narrowphase, merge, colouring), [1] primal sweep, [2] dual update,
+ * [3] finish (BDF1, sleep), [4] whole step.  Timing is recorded only while
+ * enabled with oavbd_set_timing(ctx, 1). */
+int oavbd_set_timing(oavbd_ctx* ctx, int enable);
+int oavbd_get_timing(oavbd_ctx* ctx, float* ms5);
+
+/* Block until all work of the context has finished. */
+int oavbd_sync(oavbd_ctx* ctx);
+
+/* Last error message of the context (never NULL). */
+const char* oavbd_last_error(oavbd_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OAVBD_H */

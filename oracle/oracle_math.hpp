@@ -60,9 +60,11 @@ template <class T> inline V3<T> normalize_or(V3<T> a, V3<T> fb) {
 }
 template <class T> inline V3<T> normalize(V3<T> a) { return a * (T(1) / length(a)); }
 template <class T> inline V3<T> vabs(V3<T> a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
-// Rust f32::max / f32::min (IEEE maxNum/minNum).
-template <class T> inline T fmax_(T a, T b) { return std::fmax(a, b); }
-template <class T> inline T fmin_(T a, T b) { return std::fmin(a, b); }
+// f32::max / f32::min as x86-64 libm evaluates them (fmaxf/fminf): the
+// first operand when strictly larger (smaller) or when the second is NaN,
+// else the second, so (+0, -0) yields -0 and (-0, +0) yields +0.
+template <class T> inline T fmax_(T a, T b) { return (a > b || b != b) ? a : b; }
+template <class T> inline T fmin_(T a, T b) { return (a < b || b != b) ? a : b; }
 template <class T> inline V3<T> vmax(V3<T> a, V3<T> b) { return {fmax_(a.x, b.x), fmax_(a.y, b.y), fmax_(a.z, b.z)}; }
 template <class T> inline V3<T> vmin(V3<T> a, V3<T> b) { return {fmin_(a.x, b.x), fmin_(a.y, b.y), fmin_(a.z, b.z)}; }
 // glam Vec3::clamp = max then min.

@@ -1,0 +1,249 @@
+"""Python mirror of the AVBD engine interface over the MI355X C ABI.
+
+`AvbdEngine` follows the reference's `AvbdEngine` / `PhysicsEngine` seam
+(crates/physics/src/avbd/mod.rs:424-464, crates/physics/src/engine.rs:24-133):
+bodies are dense handles, `step(dt)` runs the fixed-step accumulator
+(mod.rs:1212-1235), ball joints are added between handles
+(mod.rs:1308-1510), and body state reads back as position, orientation,
+velocity and angular velocity.  Every step runs on the GPU through
+liboavbd.so; there is no CPU path — a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboavbd.so")
+
+OK, EINVAL, ECAPACITY, EDEVICE, ENOMEM = 0, -1, -2, -3, -4
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class OavbdParams(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_float * 3), ("iters", ctypes.c_uint32)]
+
+
+class OavbdStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint32) for k in (
+        "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
+        "colour_rounds")]
+
+
+# Every entry point of include/oavbd.h with its ctypes signature.
+SIGNATURES = {
+    "oavbd_default_params": (None, [ctypes.POINTER(OavbdParams)]),
+    "oavbd_create": (ctypes.c_int, [ctypes.POINTER(OavbdParams), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_void_p)]),
+    "oavbd_destroy": (None, [ctypes.c_void_p]),
+    "oavbd_set_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                        _f32p, _f32p, _f32p, _i32p, _u32p, _u32p, _u8p, _f32p, _f32p, _u32p]),
+    "oavbd_add_ball_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                            _u32p]),
+    "oavbd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
+    "oavbd_step_n": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "oavbd_get_state": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
+    "oavbd_set_state": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
+    "oavbd_get_pairs": (ctypes.c_int, [ctypes.c_void_p, _u32p, ctypes.c_uint32, _u32p]),
+    "oavbd_get_points": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32, _u32p]),
+    "oavbd_get_colours": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
+    "oavbd_get_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
+    "oavbd_get_joints": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
+    "oavbd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdStats)]),
+    "oavbd_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "oavbd_get_timing": (ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    "oavbd_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load liboavbd.so and bind every C entry point (no device work)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"liboavbd.so not built ({path}); run __graft_entry__.build()")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class AvbdError(RuntimeError):
+    pass
+
+
+class AvbdEngine:
+    """GPU AVBD engine (one context on one HIP device)."""
+
+    def __init__(self, gravity=(0.0, -9.81, 0.0), iters=10, max_bodies=None, max_joints=None, device=0,
+                 scene=None):
+        self._L = load_library()
+        self._gravity = tuple(float(g) for g in gravity)
+        self._iters = int(iters)
+        self._device = int(device)
+        self._ctx = ctypes.c_void_p()
+        self._max_bodies = max_bodies
+        self._max_joints = max_joints
+        self.n = 0
+        self.njoints = 0
+        if scene is not None:
+            self.load(scene)
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self._L.oavbd_last_error(self._ctx) if self._ctx else b""
+            raise AvbdError(f"oavbd error {rc}: {msg.decode(errors='replace') if msg else ''}")
+
+    def _ensure_ctx(self, n, nj):
+        if self._ctx:
+            return
+        prm = OavbdParams()
+        self._L.oavbd_default_params(ctypes.byref(prm))
+        for k in range(3):
+            prm.gravity[k] = self._gravity[k]
+        prm.iters = self._iters
+        mb = self._max_bodies or max(n, 1)
+        mj = self._max_joints or max(nj, 64)
+        self._check(self._L.oavbd_create(ctypes.byref(prm), mb, mj, self._device, ctypes.byref(self._ctx)))
+
+    def load(self, s, scene_ids=None):
+        """Replace all bodies and joints with a scene (see ornis_amd.scenes)."""
+        n = len(s["shape"])
+        joints = s.get("joints", [])
+        if "gravity" in s:
+            self._gravity = tuple(float(g) for g in s["gravity"])
+        self._ensure_ctx(n, len(joints))
+        c = np.ascontiguousarray
+        shape = c(s["shape"], dtype=np.int32)
+        f = {k: c(s[k], dtype=np.float32) for k in ("shape_param", "pos", "quat", "vel", "angvel", "mass",
+                                                    "friction", "restitution")}
+        btype = c(s["body_type"], dtype=np.int32)
+        layer = c(s.get("layer", np.ones(n)), dtype=np.uint32)
+        mask = c(s.get("mask", np.full(n, 0xFFFFFFFF)), dtype=np.uint32)
+        trig = c(s.get("trigger", np.zeros(n)), dtype=np.uint8)
+        roll = c(s.get("rolling", np.zeros(n)), dtype=np.float32)
+        tors = c(s.get("torsion", np.zeros(n)), dtype=np.float32)
+        sid = s.get("scene_id") if scene_ids is None else scene_ids
+        sid = None if sid is None else c(sid, dtype=np.uint32)
+        self._keep = (shape, f, btype, layer, mask, trig, roll, tors, sid)
+        self._check(self._L.oavbd_set_bodies(
+            self._ctx, n, _p(shape, _i32p), _p(f["shape_param"], _f32p), _p(f["pos"], _f32p), _p(f["quat"], _f32p),
+            _p(f["vel"], _f32p), _p(f["angvel"], _f32p), _p(f["mass"], _f32p), _p(f["friction"], _f32p),
+            _p(f["restitution"], _f32p), _p(btype, _i32p), _p(layer, _u32p), _p(mask, _u32p), _p(trig, _u8p),
+            _p(roll, _f32p), _p(tors, _f32p), _p(sid, _u32p)))
+        self.n = n
+        self.njoints = 0
+        for (a, b, la, lb) in joints:
+            self.add_ball_joint(a, b, la, lb)
+
+    def add_ball_joint(self, a, b, la, lb):
+        la = np.ascontiguousarray(la, dtype=np.float32)
+        lb = np.ascontiguousarray(lb, dtype=np.float32)
+        idx = ctypes.c_uint32()
+        self._check(self._L.oavbd_add_ball_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
+                                                 ctypes.byref(idx)))
+        self.njoints += 1
+        return idx.value
+
+    # PhysicsEngine::step
+    def step(self, dt=1.0 / 60.0):
+        self._check(self._L.oavbd_step(self._ctx, ctypes.c_float(dt)))
+
+    def step_n(self, nsteps=1):
+        self._check(self._L.oavbd_step_n(self._ctx, int(nsteps)))
+
+    def sync(self):
+        self._check(self._L.oavbd_sync(self._ctx))
+
+    def state(self):
+        out = np.zeros((self.n, 13), np.float32)
+        self._check(self._L.oavbd_get_state(self._ctx, _p(out, _f32p), self.n))
+        return out
+
+    def set_state(self, st):
+        st = np.ascontiguousarray(st, dtype=np.float32)
+        self._check(self._L.oavbd_set_state(self._ctx, _p(st, _f32p), st.shape[0]))
+
+    def pairs(self):
+        m = ctypes.c_uint32()
+        self._check(self._L.oavbd_get_pairs(self._ctx, None, 0, ctypes.byref(m)))
+        ab = np.zeros((m.value, 2), np.uint32)
+        self._check(self._L.oavbd_get_pairs(self._ctx, _p(ab, _u32p), m.value, ctypes.byref(m)))
+        return ab
+
+    def points(self):
+        m = ctypes.c_uint32()
+        self._check(self._L.oavbd_get_points(self._ctx, None, 0, ctypes.byref(m)))
+        out = np.zeros((m.value, 16), np.float32)
+        self._check(self._L.oavbd_get_points(self._ctx, _p(out, _f32p), m.value, ctypes.byref(m)))
+        return out
+
+    def colours(self):
+        out = np.zeros(self.n, np.int32)
+        self._check(self._L.oavbd_get_colours(self._ctx, _p(out, _i32p), self.n))
+        return out
+
+    def sleep(self):
+        a = np.zeros(self.n, np.uint8)
+        t = np.zeros(self.n, np.float32)
+        self._check(self._L.oavbd_get_sleep(self._ctx, _p(a, _u8p), _p(t, _f32p), self.n))
+        return a, t
+
+    def joints(self):
+        out = np.zeros((self.njoints, 6), np.float32)
+        self._check(self._L.oavbd_get_joints(self._ctx, _p(out, _f32p), self.njoints))
+        return out
+
+    def stats(self):
+        s = OavbdStats()
+        self._check(self._L.oavbd_get_stats(self._ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in OavbdStats._fields_}
+
+    def set_timing(self, on=True):
+        self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))
+
+    def timing(self):
+        out = np.zeros(5, np.float32)
+        self._check(self._L.oavbd_get_timing(self._ctx, _p(out, _f32p)))
+        return dict(zip(["setup", "primal", "dual", "finish", "step"], out.tolist()))
+
+    @property
+    def sleeping_count(self):
+        return int(self.sleep()[0].sum())
+
+    @property
+    def body_count(self):
+        return self.n
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.oavbd_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["AvbdEngine", "AvbdError", "load_library", "SIGNATURES", "scenes"]

@@ -1,0 +1,1447 @@
+// avbd_kernels.hpp — the HIP kernels of one AVBD step on gfx950.
+//
+// Pipeline of a fixed 1/60 s substep (step_inner, crates/physics/src/avbd/
+// mod.rs:846-1143), every phase on the device:
+//   k_begin            torque -> angular velocity, TOI mover flags
+//   k_grid_*           hash grid of the bodies' bounding spheres
+//   k_toi_cast/apply   linear TOI clamp against the step-top poses
+//   k_bp_*             candidate pairs, canonically ordered by (a, b)
+//   k_narrow           discover_pair: prefilters, signed gap, normal,
+//                      manifold points, wake vote (rows.rs:432-756)
+//   k_merge            persistent-dual merge into the new pair table
+//                      (rows.rs:795-931) fused with the warm-start decay
+//   k_inertial etc.    inertial targets and warm-started positions
+//   k_inc_* / k_jp_*   per-body incidence lists and the greedy colouring
+//   k_primal_wave      one wavefront per body: rows staged in LDS, summed
+//                      per matrix element in row order, 6x6 LDL^T in FP32
+//   k_dual / joints    dual / penalty update, one thread per point
+//   k_bdf1 / sleep     velocities, renormalisation, sleep bookkeeping
+#pragma once
+
+#include "avbd_geom.hpp"
+#include "avbd_state.hpp"
+
+#define TPB 256
+
+__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------------------
+// body helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ f3 shape_inertia_dev(int kind, f3 p, float mass) {
+  if (!(mass > 0.0f) || !isfinite(mass)) return zero3();
+  if (kind == SH_SPHERE) {
+    float v = 0.4f * mass * p.x * p.x;
+    return f3{v, v, v};
+  }
+  if (kind == SH_BOX) {
+    f3 s = mul(p, 2.0f);
+    float x = s.x * s.x, y = s.y * s.y, z = s.z * s.z;
+    float k = mass / 12.0f;
+    return f3{k * (y + z), k * (z + x), k * (x + y)};
+  }
+  return zero3();
+}
+
+// wake_body (crates/physics/src/avbd/mod.rs:833-842; body.rs:410-418).
+__device__ __forceinline__ void wake_body_dev(const Dev& d, uint32_t h) {
+  if (!d.b.asleep[h]) return;
+  d.b.asleep[h] = 0;
+  d.b.timer[h] = 0.0f;
+  if (d.b.type[h] == BT_DYNAMIC) {
+    float4 in = d.b.inertia[h];
+    float4 sh = d.b.shape[h];
+    f3 I = shape_inertia_dev(d.b.kind[h], xyz(sh), in.w);
+    d.b.inertia[h] = pk(I, in.w);
+    float4 p = d.b.pos[h];
+    p.w = 1.0f / in.w;
+    d.b.pos[h] = p;
+  }
+}
+
+__device__ __forceinline__ shp body_shape(const Dev& d, uint32_t h) {
+  float4 sh = d.b.shape[h];
+  return shp{d.b.kind[h], xyz(sh), xyz(d.b.pos[h]), asq(d.b.rot[h])};
+}
+
+__device__ __forceinline__ bool pair_allowed_dev(const Dev& d, uint32_t a, uint32_t b) {
+  return (d.b.layer[a] & d.b.mask[b]) != 0u && (d.b.layer[b] & d.b.mask[a]) != 0u;
+}
+
+__device__ __forceinline__ bool jointed_dev(const Dev& d, uint32_t a, uint32_t b) {
+  for (uint32_t k = d.b.jp_off[a]; k < d.b.jp_off[a + 1]; ++k)
+    if (d.b.jp[k] == b) return true;
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// phase 1: torque and TOI mover flags (mod.rs:855-871, :882-895)
+// ---------------------------------------------------------------------------
+__global__ void k_begin(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  float4 p = d.b.pos[h];
+  bool solvable = d.b.type[h] == BT_DYNAMIC && p.w > 0.0f;
+  uint8_t fast = 0;
+  if (solvable) {
+    float4 in = d.b.inertia[h];
+    f3 I = xyz(in);
+    qf q = asq(d.b.rot[h]);
+    float iw[3][3], iwi[3][3];
+    world_inertia(I, q, iw);
+    f3 dg{I.x > 0.0f ? I.x : 0.0f, I.y > 0.0f ? I.y : 0.0f, I.z > 0.0f ? I.z : 0.0f};
+    inverse_symmetric(iw, dg, iwi);
+    float4 tq = d.b.torque[h];
+    d.b.torque[h] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 w = d.b.angvel[h];
+    f3 nw = add(xyz(w), mul(mat3_vec(iwi, xyz(tq)), K_DT));
+    d.b.angvel[h] = pk(nw, w.w);
+    if (!d.b.asleep[h] && !d.b.trigger[h]) {
+      f3 disp = mul(xyz(d.b.vel[h]), K_DT);
+      float4 sh = d.b.shape[h];
+      if (!(len3(disp) <= K_CCD_GATE * shape_min_dim(d.b.kind[h], xyz(sh)))) fast = 1;
+    }
+  }
+  d.fast[h] = fast;
+  if (fast) atomicAdd(&d.counters[CTR_FAST], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// hash grid over finite, not-large bodies (acceleration structure only:
+// every candidate it yields is re-tested exactly)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int cell_of(float x, float cell) { return (int)floorf(x / cell); }
+__device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t nb) {
+  uint32_t h = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u;
+  h ^= h >> 13;
+  h *= 0x5bd1e995u;
+  h ^= h >> 15;
+  return h & (nb - 1u);
+}
+
+__global__ void k_grid_clear(Dev d) {
+  uint32_t i = gid();
+  if (i >= d.g.nbuckets) return;
+  d.g.cnt[i] = 0u;
+  d.g.cursor[i] = 0u;
+}
+
+__global__ void k_grid_count(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.g.is_large[h]) {
+    d.g.bucket[h] = 0xffffffffu;
+    return;
+  }
+  float4 p = d.b.pos[h];
+  uint32_t bk = bucket_of(cell_of(p.x, d.g.cell), cell_of(p.y, d.g.cell), cell_of(p.z, d.g.cell), d.g.nbuckets);
+  d.g.bucket[h] = bk;
+  atomicAdd(&d.g.cnt[bk], 1u);
+}
+
+__global__ void k_grid_fill(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  uint32_t bk = d.g.bucket[h];
+  if (bk == 0xffffffffu) return;
+  uint32_t slot = atomicAdd(&d.g.cursor[bk], 1u);
+  d.g.items[d.g.start[bk] + slot] = h;
+}
+
+// Distinct buckets of the cells in [lo, hi] (at most cap); returns count.
+__device__ __forceinline__ int gather_buckets(const Dev& d, int lx, int ly, int lz, int hx, int hy, int hz,
+                                              uint32_t* out, int cap) {
+  int m = 0;
+  for (int x = lx; x <= hx; ++x)
+    for (int y = ly; y <= hy; ++y)
+      for (int z = lz; z <= hz; ++z) {
+        uint32_t bk = bucket_of(x, y, z, d.g.nbuckets);
+        bool dup = false;
+        for (int i = 0; i < m; ++i)
+          if (out[i] == bk) dup = true;
+        if (!dup && m < cap) out[m++] = bk;
+      }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// TOI clamp (mod.rs:882-978; cast_shape, distance.rs:841-891).  Every cast
+// reads the step-top poses; clamps are applied afterwards.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool toi_target_ok(const Dev& d, uint32_t h, uint32_t o) {
+  if (o == h || d.b.trigger[o] || d.b.scene[o] != d.b.scene[h] || !pair_allowed_dev(d, h, o)) return false;
+  return !jointed_dev(d, h, o);
+}
+
+// Conservative advancement of mover h against target o; t < 0 if no hit.
+__device__ float cast_one(const Dev& d, const shp& mover, f3 dir, float len, const shp& target, f3* normal) {
+  float t = 0.0f;
+  for (int it = 0; it < 24; ++it) {
+    shp m = mover;
+    m.pos = add(mover.pos, mul(dir, t));
+    float sat;
+    dist_t r = shape_distance(m, target, &sat);
+    if (r.dist <= K_SHAPE_TOUCH) {
+      if (t > 0.0f) {
+        *normal = normalize_or(sub(r.pa, r.pb), neg(dir));
+        return t;
+      }
+      return -1.0f;
+    }
+    t += r.dist - K_SHAPE_TOUCH * 0.5f;
+    if (t >= len) break;
+  }
+  return -1.0f;
+}
+
+// Can the bounding spheres come within the touch distance along the sweep?
+__device__ __forceinline__ bool sweep_may_touch(f3 p0, f3 dir, float len, float rm, f3 po, float ro) {
+  f3 w = sub(po, p0);
+  float s = dot3(w, dir);
+  s = s < 0.0f ? 0.0f : (s > len ? len : s);
+  f3 c = sub(w, mul(dir, s));
+  float reach = rm + ro + K_SHAPE_TOUCH;
+  return len3(c) <= reach * 1.001f + 1e-4f;
+}
+
+__global__ void k_toi_cast(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  d.toi_hit[h] = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+  if (!d.fast[h]) return;
+  f3 delta = mul(xyz(d.b.vel[h]), K_DT);
+  float len = len3(delta);
+  if (len < K_NEAR_ZERO) return;
+  f3 dir = divs(delta, len);
+  shp mover = body_shape(d, h);
+  float rm = d.b.shape[h].w;
+  float best_t = 0.0f;
+  uint32_t best_o = 0xffffffffu;
+  f3 best_n = zero3();
+  auto consider = [&](uint32_t o) {
+    if (!toi_target_ok(d, h, o)) return;
+    float ro = d.b.shape[o].w;
+    if (isfinite(ro) && !sweep_may_touch(mover.pos, dir, len, rm, xyz(d.b.pos[o]), ro)) return;
+    shp target = body_shape(d, o);
+    f3 nrm;
+    float t = cast_one(d, mover, dir, len, target, &nrm);
+    if (t < 0.0f) return;
+    if (best_o == 0xffffffffu || t < best_t || (t == best_t && o < best_o)) {
+      best_t = t;
+      best_o = o;
+      best_n = nrm;
+    }
+  };
+  // grid members near the swept bounding sphere
+  float r = rm + d.g.cell * 0.5f + 0.01f;
+  f3 p0 = mover.pos, p1 = add(mover.pos, delta);
+  int lx = cell_of(fminf(p0.x, p1.x) - r, d.g.cell), hx = cell_of(fmaxf(p0.x, p1.x) + r, d.g.cell);
+  int ly = cell_of(fminf(p0.y, p1.y) - r, d.g.cell), hy = cell_of(fmaxf(p0.y, p1.y) + r, d.g.cell);
+  int lz = cell_of(fminf(p0.z, p1.z) - r, d.g.cell), hz = cell_of(fmaxf(p0.z, p1.z) + r, d.g.cell);
+  long span = (long)(hx - lx + 1) * (long)(hy - ly + 1) * (long)(hz - lz + 1);
+  if (span <= 64) {
+    uint32_t bks[64];
+    int nbk = gather_buckets(d, lx, ly, lz, hx, hy, hz, bks, 64);
+    for (int i = 0; i < nbk; ++i) {
+      uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
+      for (uint32_t k = 0; k < c0; ++k) consider(d.g.items[s0 + k]);
+    }
+  } else {
+    for (uint32_t o = 0; o < d.n; ++o)
+      if (!d.g.is_large[o]) consider(o);
+  }
+  for (uint32_t k = 0; k < d.g.nlarge; ++k) consider(d.g.large[k]);
+  if (best_o != 0xffffffffu) {
+    d.toi_hit[h] = pk(best_n, best_t);
+    d.toi_tgt[h] = best_o;
+  }
+}
+
+__global__ void k_toi_apply(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  float4 hit = d.toi_hit[h];
+  if (!(hit.w >= 0.0f) || !d.fast[h]) return;
+  uint32_t o = d.toi_tgt[h];
+  f3 nrm = xyz(hit);
+  float4 v4 = d.b.vel[h];
+  f3 vel = xyz(v4);
+  f3 disp = mul(vel, K_DT);
+  float len = fmax_(len3(disp), K_MIN_TOI_LEN);
+  float frac = clampf_(hit.w / len, 0.0f, 1.0f);
+  float e = fmin_(d.b.mat[h].y, d.b.mat[o].y);
+  float4 p = d.b.pos[h];
+  d.b.pos[h] = pk(add(xyz(p), mul(disp, frac)), p.w);
+  float vn = dot3(vel, nrm);
+  if (vn < 0.0f) {
+    float bounce = vn < -K_CCD_BOUNCE ? 1.0f + e : 1.0f;
+    d.b.vel[h] = pk(sub(vel, mul(nrm, bounce * vn)), v4.w);
+  }
+  d.b.timer[h] = 0.0f;
+  d.b.flag_a[o] = 1;  // wake the target
+  atomicAdd(&d.counters[CTR_TOIHITS], 1u);
+}
+
+// Apply pending wakes (flag_a) and clear them.
+__global__ void k_apply_wake(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.b.flag_a[h]) {
+    d.b.flag_a[h] = 0;
+    wake_body_dev(d, h);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// broadphase: candidates (a < b) passing the reference prefilters
+// (rows.rs:432-479): scene, layer/mask, pin-joint no-collide, bounding
+// spheres within GEN_MARGIN.  Output is grouped by a and sorted by b.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool prefilter(const Dev& d, uint32_t a, uint32_t b) {
+  if (d.b.scene[a] != d.b.scene[b]) return false;
+  if (!pair_allowed_dev(d, a, b)) return false;
+  if (!d.b.trigger[a] && !d.b.trigger[b] && d.b.jp_off[a] != d.b.jp_off[a + 1] && jointed_dev(d, a, b)) return false;
+  float ra = d.b.shape[a].w, rb = d.b.shape[b].w;
+  return !(len3(sub(xyz(d.b.pos[a]), xyz(d.b.pos[b]))) > ra + rb + K_GEN_MARGIN);
+}
+
+template <bool FILL>
+__device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, uint2* out) {
+  float4 p = d.b.pos[a];
+  int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
+  uint32_t bks[27];
+  int nbk = gather_buckets(d, cx - 1, cy - 1, cz - 1, cx + 1, cy + 1, cz + 1, bks, 27);
+  uint32_t m = 0;
+  for (int i = 0; i < nbk; ++i) {
+    uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
+    for (uint32_t k = 0; k < c0; ++k) {
+      uint32_t o = d.g.items[s0 + k];
+      if (o > a && prefilter(d, a, o)) {
+        if (FILL) out[m] = make_uint2(a, o);
+        ++m;
+      }
+    }
+  }
+  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
+    uint32_t o = d.g.large[k];
+    if (o > a && prefilter(d, a, o)) {
+      if (FILL) out[m] = make_uint2(a, o);
+      ++m;
+    }
+  }
+  return m;
+}
+
+__global__ void k_bp_count(Dev d, uint32_t* cnt) {
+  uint32_t a = gid();
+  if (a >= d.n) return;
+  cnt[a] = d.g.is_large[a] ? 0u : bp_small<false>(d, a, nullptr);
+}
+
+// Large bodies pair with every later body passing the prefilter.
+__global__ void k_bp_large_flag(Dev d) {
+  uint32_t b = gid();
+  if (b >= d.n) return;
+  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
+    uint32_t L = d.g.large[k];
+    d.lflag[(size_t)k * d.n + b] = (L < b && prefilter(d, L, b)) ? 1u : 0u;
+  }
+}
+
+__global__ void k_bp_large_count(Dev d, uint32_t* cnt) {
+  uint32_t k = gid();
+  if (k >= d.g.nlarge) return;
+  cnt[d.g.large[k]] = d.lrank[(size_t)k * (d.n + 1) + d.n];
+}
+
+__global__ void k_bp_fill(Dev d) {
+  uint32_t a = gid();
+  if (a >= d.n || d.g.is_large[a]) return;
+  uint2* out = d.C.ab + d.cand_off[a];
+  uint32_t m = bp_small<true>(d, a, out);
+  for (uint32_t i = 1; i < m; ++i) {  // insertion sort by b
+    uint2 v = out[i];
+    uint32_t j = i;
+    while (j > 0 && out[j - 1].y > v.y) {
+      out[j] = out[j - 1];
+      --j;
+    }
+    out[j] = v;
+  }
+}
+
+__global__ void k_bp_large_fill(Dev d) {
+  uint32_t b = gid();
+  if (b >= d.n) return;
+  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
+    if (!d.lflag[(size_t)k * d.n + b]) continue;
+    uint32_t L = d.g.large[k];
+    d.C.ab[d.cand_off[L] + d.lrank[(size_t)k * (d.n + 1) + b]] = make_uint2(L, b);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// narrowphase: discover_pair (rows.rs:432-756) for every candidate
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_old_pair(const Dev& d, uint32_t a, uint32_t b) {
+  uint64_t key = ((uint64_t)a << 32) | b;
+  int lo = 0, hi = (int)d.npairs_old;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    uint2 ab = d.Pold.ab[mid];
+    uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
+    if (k < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  if (lo < (int)d.npairs_old) {
+    uint2 ab = d.Pold.ab[lo];
+    if (ab.x == a && ab.y == b) return lo;
+  }
+  return -1;
+}
+
+__global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
+  uint32_t c = gid();
+  if (c >= ncand) return;
+  d.C.acc[c] = 0u;
+  d.C.npts[c] = 0u;
+  uint2 ab = d.C.ab[c];
+  uint32_t ia = ab.x, ib = ab.y;
+  if (d.b.trigger[ia] || d.b.trigger[ib]) return;  // trigger overlaps form no pair
+  shp A = body_shape(d, ia), B = body_shape(d, ib);
+  int old = find_old_pair(d, ia, ib);
+  f3 va = xyz(d.b.vel[ia]), vb = xyz(d.b.vel[ib]);
+  dist_t dd;
+  float sgn;
+  if (A.kind == SH_BOX && B.kind == SH_BOX) {
+    f3 sn;
+    float sat = separating_axis(A, A.p, B, B.p, &sn);
+    if (old < 0) {
+      // Creation gate bound: reach <= GEN_MARGIN + |vb - va| dt (|normal| = 1).
+      float reach_hi = K_GEN_MARGIN + len3(sub(vb, va)) * K_DT * 1.0001f + 1e-6f;
+      if (sat > reach_hi) return;
+    }
+    dd = box_box_from_sat(A, A.p, B, B.p, sat, sn);
+    sgn = sat;
+  } else {
+    float sat;
+    dd = shape_distance(A, B, &sat);
+    sgn = dd.dist;
+  }
+  // orient_contact_normal (rows.rs:303-333)
+  f3 normal;
+  if (A.kind == SH_HALFSPACE) {
+    normal = neg(qrot(A.rot, A.p));
+  } else if (B.kind == SH_HALFSPACE) {
+    normal = qrot(B.rot, B.p);
+  } else {
+    normal = sub(dd.pa, dd.pb);
+    if (dd.dist < 0.0f) normal = neg(normal);
+    if (len2(normal) < K_AXIS_REST_LEN2) normal = sub(A.pos, B.pos);
+    normal = normalize_or(normal, f3{0.0f, 1.0f, 0.0f});
+    if (old >= 0) {
+      if (dot3(normal, xyz(d.Pold.n[old])) < 0.0f) normal = neg(normal);
+    } else if (dot3(normal, sub(A.pos, B.pos)) < 0.0f) {
+      normal = neg(normal);
+    }
+  }
+  float approach = fmax_(dot3(sub(vb, va), normal), 0.0f);
+  float reach = K_GEN_MARGIN + approach * K_DT;
+  if (sgn > reach && old < 0) return;
+  bool separated = sgn > reach;
+  float4 ma = d.b.mat[ia], mb = d.b.mat[ib];
+  float mu1 = fmax_(ma.x, mb.x);
+  mu1 = fmax_(mu1, 0.0f);
+  float ra = d.b.shape[ia].w, rb = d.b.shape[ib].w;
+  float patch = fmin_(ra, rb) + K_MARGIN;
+  f3 pp = mul(add(dd.pa, dd.pb), 0.5f);
+  f3 pa_c = sub(A.pos, mul(normal, dot3(sub(A.pos, pp), normal)));
+  f3 pb_c = sub(B.pos, mul(normal, dot3(sub(B.pos, pp), normal)));
+  float plane_a = A.kind == SH_BOX ? dot3(normal, A.pos) - 1.0f * dot3(abs3(qrot(qconj(A.rot), normal)), A.p)
+                                   : dot3(normal, dd.pa);
+  float plane_b = B.kind == SH_BOX ? dot3(normal, B.pos) - (-1.0f) * dot3(abs3(qrot(qconj(B.rot), normal)), B.p)
+                                   : dot3(normal, dd.pb);
+  qf inv_a = qconj(A.rot), inv_b = qconj(B.rot);
+  f3 fra[K_MAX_POINTS], frb[K_MAX_POINTS];
+  int nf = 0;
+  auto dup = [&](f3 x, f3 y) {
+    for (int i = 0; i < nf; ++i)
+      if (len3(sub(fra[i], x)) < K_POINT_MATCH && len3(sub(frb[i], y)) < K_POINT_MATCH) return true;
+    return false;
+  };
+  if (!separated) {
+    for (int side = 0; side < 2; ++side) {
+      const shp& body = side == 0 ? A : B;
+      if (body.kind != SH_BOX) continue;
+      f3 witness = side == 0 ? dd.pa : dd.pb;
+      float sign = side == 0 ? 1.0f : -1.0f;
+      f3 pc = side == 0 ? pa_c : pb_c;
+      f3 sn = smul(sign, normal);
+      float slop = K_EXPAND_SLOP + fmax_(-sgn, 0.0f);
+      for (int ci = 0; ci < 8 && nf < K_MAX_POINTS; ++ci) {
+        float sx = (ci & 4) ? 1.0f : -1.0f, sy = (ci & 2) ? 1.0f : -1.0f, sz = (ci & 1) ? 1.0f : -1.0f;
+        f3 corner{sx * body.p.x, sy * body.p.y, sz * body.p.z};
+        f3 world = add(body.pos, qrot(body.rot, corner));
+        float along = dot3(sub(world, witness), sn);
+        if (fabsf(along) > slop) continue;
+        f3 rel = sub(world, pc);
+        f3 tang = sub(rel, mul(normal, dot3(rel, normal)));
+        if (len3(tang) > patch) continue;
+        f3 pa, pb;
+        if (side == 0) {
+          pa = world;
+          pb = add(world, mul(normal, plane_b - dot3(normal, world)));
+        } else {
+          pa = add(world, mul(normal, plane_a - dot3(normal, world)));
+          pb = world;
+        }
+        f3 ral = qrot(inv_a, sub(pa, A.pos));
+        f3 rbl = qrot(inv_b, sub(pb, B.pos));
+        if (dup(ral, rbl)) continue;
+        fra[nf] = ral;
+        frb[nf] = rbl;
+        ++nf;
+      }
+    }
+    if (nf >= 2 && nf < K_MAX_POINTS) {
+      f3 mid = ra < rb ? pa_c : (rb < ra ? pb_c : mul(add(pa_c, pb_c), 0.5f));
+      f3 pa = add(mid, mul(normal, plane_a - dot3(normal, mid)));
+      f3 pb = add(mid, mul(normal, plane_b - dot3(normal, mid)));
+      f3 ral = qrot(inv_a, sub(pa, A.pos));
+      f3 rbl = qrot(inv_b, sub(pb, B.pos));
+      if (!dup(ral, rbl)) {
+        fra[nf] = ral;
+        frb[nf] = rbl;
+        ++nf;
+      }
+    }
+    if (nf < 3) {
+      f3 ral = qrot(inv_a, sub(dd.pa, A.pos));
+      f3 rbl = qrot(inv_b, sub(dd.pb, B.pos));
+      if (!dup(ral, rbl)) {
+        fra[nf] = ral;
+        frb[nf] = rbl;
+        ++nf;
+      }
+    }
+  }
+  bool wake;
+  bool sa = d.b.asleep[ia] != 0, sb = d.b.asleep[ib] != 0;
+  if (old >= 0) {
+    wake = (sa || sb) && sgn < -K_WAKE_PEN;
+  } else {
+    wake = sgn < -K_WAKE_PEN;
+    if (!wake && sa) {
+      if (dot3(sub(vb, va), smul(1.0f, normal)) > K_WAKE_IMPACT) wake = true;
+    }
+    if (!wake && sb) {
+      if (dot3(sub(va, vb), smul(-1.0f, normal)) > K_WAKE_IMPACT) wake = true;
+    }
+  }
+  d.C.acc[c] = 1u;
+  d.C.npts[c] = (uint32_t)nf;
+  d.C.disc0[c] = pk(normal, sgn);
+  d.C.disc1[c] = make_float4(mu1, mu1, __int_as_float(old), wake ? 1.0f : 0.0f);
+  for (int k = 0; k < nf; ++k) {
+    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fra[k].x, fra[k].y, fra[k].z, frb[k].x);
+    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(frb[k].y, frb[k].z, 0.0f, 0.0f);
+  }
+}
+
+// Merge (rows.rs:795-931) fused with the warm-start decay (mod.rs:1019-1040).
+__global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
+  uint32_t c = gid();
+  if (c >= ncand || !d.C.acc[c]) return;
+  uint32_t p = d.acc_off[c];
+  uint32_t start = d.pt_off[c];
+  uint32_t cnt = d.C.npts[c];
+  uint2 ab = d.C.ab[c];
+  float4 d0 = d.C.disc0[c], d1 = d.C.disc1[c];
+  int old = __float_as_int(d1.z);
+  float4 ma = d.b.mat[ab.x], mb = d.b.mat[ab.y];
+  d.P.ab[p] = ab;
+  d.P.n[p] = d0;
+  d.P.mu[p] = make_float4(d1.x, d1.y, fmax_(ma.z, mb.z), fmax_(ma.w, mb.w));
+  d.P.pts[p] = make_uint2(start, cnt);
+  if (d1.w != 0.0f) {
+    d.b.flag_a[ab.x] = 1;
+    d.b.flag_a[ab.y] = 1;
+  }
+  bool touching = d0.w <= 0.0f;
+  bool smooth = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
+  const float AG = K_ALPHA * K_GAMMA;
+  uint2 op = old >= 0 ? d.Pold.pts[old] : make_uint2(0u, 0u);
+  for (uint32_t k = 0; k < cnt; ++k) {
+    float4 f0 = d.C.fresh[(size_t)c * 16 + 2 * k];
+    float4 f1 = d.C.fresh[(size_t)c * 16 + 2 * k + 1];
+    f3 ral{f0.x, f0.y, f0.z}, rbl{f0.w, f1.x, f1.y};
+    int m = -1;
+    for (uint32_t j = 0; j < op.y; ++j)
+      if (len3(sub(xyz(d.Qold.ra[op.x + j]), ral)) < K_LAMBDA_MATCH) {
+        m = (int)(op.x + j);
+        break;
+      }
+    uint32_t q = start + k;
+    if (m >= 0) {
+      float4 ora = d.Qold.ra[m], orb = d.Qold.rb[m];
+      bool stuck = orb.w != 0.0f;
+      bool keep = stuck && touching && !smooth;
+      f3 ra = keep ? xyz(ora) : ral;
+      f3 rb = keep ? xyz(orb) : rbl;
+      float4 l = d.Qold.lam[m], pe = d.Qold.pen[m], ro = d.Qold.roll[m];
+      d.Q.ra[q] = pk(ra, 0.0f);
+      d.Q.rb[q] = pk(rb, stuck ? 1.0f : 0.0f);
+      d.Q.lam[q] = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+      d.Q.pen[q] = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
+                               clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+      d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
+    } else {
+      float z = 0.0f * AG;
+      float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
+      d.Q.ra[q] = pk(ral, 0.0f);
+      d.Q.rb[q] = pk(rbl, 1.0f);
+      d.Q.lam[q] = make_float4(z, z, z, 0.0f);
+      d.Q.pen[q] = make_float4(p1, p1, p1, 0.0f);
+      d.Q.roll[q] = make_float4(z, z, z, 0.0f);
+    }
+    d.Q.pair[q] = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// joint-component propagation (sleep.rs:155-175) over static labels
+// ---------------------------------------------------------------------------
+__global__ void k_prop_mark(Dev d, uint8_t* flags, uint8_t value) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.b.type[h] == BT_DYNAMIC && flags[h] == value) d.b.comp_any[d.b.comp[h]] = 1;
+}
+__global__ void k_prop_apply(Dev d, uint8_t* flags, uint8_t value) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.b.type[h] == BT_DYNAMIC && d.b.comp_any[d.b.comp[h]]) flags[h] = value;
+}
+__global__ void k_prop_clear(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  d.b.comp_any[h] = 0;
+}
+
+// Ball joint: driven (a non-dynamic member teleported) or off its rest pose.
+__device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
+  uint2 ab = d.J.ab[j];
+  uint32_t m[2] = {ab.x, ab.y};
+  for (int i = 0; i < 2; ++i) {
+    uint32_t h = m[i];
+    if (d.b.type[h] != BT_DYNAMIC) {
+      float4 p = d.b.pos[h], pp = d.b.prev_pos[h];
+      float4 q = d.b.rot[h], pq = d.b.prev_rot[h];
+      if (!(p.x == pp.x && p.y == pp.y && p.z == pp.z) || !(q.x == pq.x && q.y == pq.y && q.z == pq.z && q.w == pq.w))
+        return true;
+    }
+  }
+  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(asq(d.b.rot[ab.x]), xyz(d.J.la[j])));
+  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(asq(d.b.rot[ab.y]), xyz(d.J.lb[j])));
+  return len3(sub(pb, pa)) > K_JOINT_REST;
+}
+
+// wake_joint_motion (sleep.rs:107-137): per-body part.
+__global__ void k_awake_init(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  uint8_t aw = 0;
+  if (d.b.type[h] == BT_DYNAMIC) {
+    float4 v = d.b.vel[h], w = d.b.angvel[h], t = d.b.torque[h];
+    float4 p = d.b.pos[h], pp = d.b.prev_pos[h], q = d.b.rot[h], pq = d.b.prev_rot[h];
+    bool nz = v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || w.x != 0.0f || w.y != 0.0f || w.z != 0.0f ||
+              t.x != 0.0f || t.y != 0.0f || t.z != 0.0f;
+    bool moved = p.x != pp.x || p.y != pp.y || p.z != pp.z || q.x != pq.x || q.y != pq.y || q.z != pq.z ||
+                 q.w != pq.w;
+    aw = (!d.b.asleep[h] || nz || moved) ? 1 : 0;
+  }
+  d.b.flag_b[h] = aw;
+}
+__global__ void k_awake_joints(Dev d) {
+  uint32_t j = gid();
+  if (j >= d.njoints) return;
+  if (joint_unsettled(d, j)) {
+    uint2 ab = d.J.ab[j];
+    if (d.b.type[ab.x] == BT_DYNAMIC) d.b.flag_b[ab.x] = 1;
+    if (d.b.type[ab.y] == BT_DYNAMIC) d.b.flag_b[ab.y] = 1;
+  }
+}
+__global__ void k_awake_apply(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.b.flag_b[h] && d.b.asleep[h]) wake_body_dev(d, h);
+}
+
+// External wake (mod.rs:984-992).
+__global__ void k_external_wake(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n || !d.b.asleep[h]) return;
+  if (len3(xyz(d.b.vel[h])) > K_SLEEP_LIN || len3(xyz(d.b.angvel[h])) > K_SLEEP_ANG) wake_body_dev(d, h);
+}
+
+// ---------------------------------------------------------------------------
+// inertial targets + warm-started positions (mod.rs:993-1049)
+// ---------------------------------------------------------------------------
+__global__ void k_inertial(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  float4 p = d.b.pos[h];
+  float4 q = d.b.rot[h];
+  float4 p0, q0;
+  if (d.b.type[h] == BT_KINEMATIC) {
+    p0 = d.b.prev_pos[h];
+    q0 = d.b.prev_rot[h];
+  } else {
+    p0 = p;
+    q0 = q;
+  }
+  float srad = d.b.kind[h] == SH_SPHERE ? d.b.shape[h].x : -1.0f;
+  d.b.pos0[h] = make_float4(p0.x, p0.y, p0.z, srad);
+  d.b.rot0[h] = q0;
+  bool solvable = d.b.type[h] == BT_DYNAMIC && p.w > 0.0f;
+  if (!solvable) {
+    d.b.inertial[h] = p;
+    d.b.inertial_rot[h] = q;
+  } else {
+    f3 g{d.gravity[0], d.gravity[1], d.gravity[2]};
+    f3 y = add(add(xyz(p), mul(xyz(d.b.vel[h]), K_DT)), mul(g, K_DT * K_DT));
+    qf yq = quat_integrate(asq(q), mul(xyz(d.b.angvel[h]), K_DT));
+    d.b.inertial[h] = pk(y, 0.0f);
+    d.b.inertial_rot[h] = pkq(yq);
+    if (!d.b.asleep[h]) {
+      p = pk(y, p.w);
+      q = pkq(yq);
+      d.b.pos[h] = p;
+      d.b.rot[h] = q;
+    }
+  }
+  f3 dr = quat_diff_vec(asq(q), asq(q0));
+  d.b.drot[h] = pk(dr, 0.0f);
+}
+
+__global__ void k_joint_prep(Dev d) {
+  uint32_t j = gid();
+  if (j >= d.njoints) return;
+  const float AG = K_ALPHA * K_GAMMA;
+  float4 l = d.J.lam[j], pe = d.J.pen[j];
+  d.J.lam[j] = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+  d.J.pen[j] = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
+                           clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+  uint2 ab = d.J.ab[j];
+  f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(asq(d.b.rot0[ab.x]), xyz(d.J.la[j])));
+  f3 pb0 = add(xyz(d.b.pos0[ab.y]), qrot(asq(d.b.rot0[ab.y]), xyz(d.J.lb[j])));
+  d.J.c0v[j] = pk(sub(pa0, pb0), 0.0f);
+}
+
+// Per-pair step constants: friction frame, separated-damper stiffness
+// (assembly.rs:129-174), step-start violation c0 per point (joints_ext.rs:21-32).
+__global__ void k_pair_prep(Dev d) {
+  uint32_t p = gid();
+  if (p >= d.npairs) return;
+  uint2 ab = d.P.ab[p];
+  float4 n4 = d.P.n[p];
+  f3 n = xyz(n4);
+  float gap = n4.w;
+  f3 t1, t2u;
+  tangent_basis(n, &t1, &t2u);
+  f3 t2 = cross3(t1, n);
+  uint2 pts = d.P.pts[p];
+  float damp = 0.0f;
+  if (!(gap <= 0.0f)) {
+    float approach = fmax_(dot3(sub(xyz(d.b.vel[ab.y]), xyz(d.b.vel[ab.x])), n), 0.0f);
+    if (!(gap - approach * K_DT > 0.0f)) {
+      float npts = (float)(pts.y > 1u ? pts.y : 1u);
+      float ka = d.b.type[ab.x] == BT_DYNAMIC ? d.b.pos[ab.x].w : 0.0f;
+      float kb = d.b.type[ab.y] == BT_DYNAMIC ? d.b.pos[ab.y].w : 0.0f;
+      float m_red = ka + kb > K_NEAR_ZERO ? 1.0f / (ka + kb) : 0.0f;
+      if (m_red > 0.0f) damp = fmin_(2.0f * m_red / (K_DT * K_DT) / npts, K_PEN_MAX);
+    }
+  }
+  d.P.t1[p] = pk(t1, damp);
+  d.P.t2[p] = pk(t2, 0.0f);
+  bool sphere = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
+  float4 pa0 = d.b.pos0[ab.x], pb0 = d.b.pos0[ab.y];
+  qf qa0 = asq(d.b.rot0[ab.x]), qb0 = asq(d.b.rot0[ab.y]);
+  for (uint32_t k = 0; k < pts.y; ++k) {
+    uint32_t q = pts.x + k;
+    float4 ra = d.Q.ra[q];
+    float c0 = 0.0f;
+    if (gap <= 0.0f) {
+      if (sphere) {
+        c0 = gap + K_MARGIN;
+      } else {
+        f3 pa = add(xyz(pa0), qrot(qa0, xyz(ra)));
+        f3 pb = add(xyz(pb0), qrot(qb0, xyz(d.Q.rb[q])));
+        c0 = dot3(n, sub(pa, pb)) + K_MARGIN;
+      }
+    }
+    ra.w = c0;
+    d.Q.ra[q] = ra;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-body incidence (point-sides in pair order) and colouring
+// ---------------------------------------------------------------------------
+__global__ void k_inc_count(Dev d) {
+  uint32_t p = gid();
+  if (p >= d.npairs) return;
+  uint2 ab = d.P.ab[p];
+  uint32_t c = d.P.pts[p].y;
+  atomicAdd(&d.ptcnt[ab.x], c);
+  atomicAdd(&d.ptcnt[ab.y], c);
+  atomicAdd(&d.bcnt[ab.y], 1u);
+}
+__global__ void k_inc_bfill(Dev d) {
+  uint32_t p = gid();
+  if (p >= d.npairs) return;
+  uint32_t b = d.P.ab[p].y;
+  uint32_t s = atomicAdd(&d.bcur[b], 1u);
+  d.blist[d.boff[b] + s] = p;
+}
+// Body h's pairs in ascending pair index: pairs where h is b (a < h), then
+// the contiguous run where h is a.
+__global__ void k_inc_build(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  uint32_t b0 = d.boff[h], b1 = d.boff[h + 1];
+  for (uint32_t i = b0 + 1; i < b1; ++i) {
+    uint32_t v = d.blist[i];
+    uint32_t j = i;
+    while (j > b0 && d.blist[j - 1] > v) {
+      d.blist[j] = d.blist[j - 1];
+      --j;
+    }
+    d.blist[j] = v;
+  }
+  uint32_t w = d.inc_off[h];
+  for (uint32_t i = b0; i < b1; ++i) {
+    uint2 pts = d.P.pts[d.blist[i]];
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = ((pts.x + k) << 1) | 1u;
+  }
+  uint32_t a0 = d.acc_off[d.cand_off[h]], a1 = d.acc_off[d.cand_off[h + 1]];
+  for (uint32_t p = a0; p < a1; ++p) {
+    uint2 pts = d.P.pts[p];
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = (pts.x + k) << 1;
+  }
+}
+
+// Jones-Plassmann round: an uncoloured swept body whose uncoloured
+// neighbours all have lower priority takes the smallest colour unused by
+// its neighbours.  Equal to the sequential greedy colouring in priority
+// order.  Priority: larger hash, then smaller scene-local index.
+__device__ __forceinline__ bool swept(const Dev& d, uint32_t h) {
+  return d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f && !d.b.asleep[h];
+}
+__device__ __forceinline__ bool higher(const Dev& d, uint32_t o, uint32_t h) {
+  uint32_t po = d.b.prio[o], ph = d.b.prio[h];
+  return po > ph || (po == ph && d.b.local[o] < d.b.local[h]);
+}
+__device__ __forceinline__ uint32_t partner_of_item(const Dev& d, uint32_t it) {
+  uint32_t q = it >> 1;
+  uint2 ab = d.P.ab[d.Q.pair[q]];
+  return (it & 1u) ? ab.x : ab.y;
+}
+
+// col_round[h] packs (round << 16) | colour in one word (written by a
+// single store), -1 while uncoloured; a neighbour coloured in the current
+// round still counts as uncoloured.
+__global__ void k_colour_init(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  d.b.colour[h] = -1;
+  d.b.col_round[h] = -1;
+}
+
+__device__ __forceinline__ int colour_seen(const Dev& d, uint32_t o, int round) {
+  int v = __atomic_load_n(&d.b.col_round[o], __ATOMIC_RELAXED);
+  if (v < 0 || (v >> 16) == round) return -1;
+  return v & 0xffff;
+}
+
+__global__ void k_colour_round(Dev d, int round) {
+  uint32_t h = gid();
+  if (h >= d.n || !swept(d, h) || d.b.col_round[h] >= 0) return;
+  for (uint32_t k = d.inc_off[h]; k < d.inc_off[h + 1]; ++k) {
+    uint32_t o = partner_of_item(d, d.inc[k]);
+    if (swept(d, o) && colour_seen(d, o, round) < 0 && higher(d, o, h)) {
+      atomicAdd(&d.counters[CTR_REMAIN], 1u);
+      return;
+    }
+  }
+  for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
+    uint32_t e = d.b.ji[k];
+    uint2 ab = d.J.ab[e >> 1];
+    uint32_t o = (e & 1u) ? ab.x : ab.y;
+    if (swept(d, o) && colour_seen(d, o, round) < 0 && higher(d, o, h)) {
+      atomicAdd(&d.counters[CTR_REMAIN], 1u);
+      return;
+    }
+  }
+  for (int base = 0;; base += 64) {
+    uint64_t used = 0;
+    for (uint32_t k = d.inc_off[h]; k < d.inc_off[h + 1]; ++k) {
+      uint32_t o = partner_of_item(d, d.inc[k]);
+      int co = swept(d, o) ? colour_seen(d, o, round) : -1;
+      if (co >= base && co < base + 64) used |= 1ull << (co - base);
+    }
+    for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
+      uint32_t e = d.b.ji[k];
+      uint2 ab = d.J.ab[e >> 1];
+      uint32_t o = (e & 1u) ? ab.x : ab.y;
+      int co = swept(d, o) ? colour_seen(d, o, round) : -1;
+      if (co >= base && co < base + 64) used |= 1ull << (co - base);
+    }
+    if (~used != 0ull) {
+      int c = base + __builtin_ctzll(~used);
+      __atomic_store_n(&d.b.col_round[h], (round << 16) | c, __ATOMIC_RELAXED);
+      atomicMax(&d.counters[CTR_MAXCOL], (uint32_t)(c + 1));
+      atomicAdd(&d.colour_cnt[c], 1u);
+      return;
+    }
+  }
+}
+
+__global__ void k_colour_fill(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  int v = d.b.col_round[h];
+  int c = v < 0 ? -1 : (v & 0xffff);
+  d.b.colour[h] = c;
+  if (c < 0) return;
+  uint32_t s = atomicAdd(&d.colour_cur[c], 1u);
+  d.colour_list[d.colour_off[c] + s] = h;
+}
+
+// ---------------------------------------------------------------------------
+// primal: solve_body (assembly.rs:73-903) for the bodies of one colour
+// ---------------------------------------------------------------------------
+
+// A constraint row as the accumulation sees it.  kind 0: full stamp
+// (stamp_row: axis nn, angular lever t, penalty, force); kind 1: angular
+// couple (rolling/torsion: axis g, ROLL_PEN, force); kind 2: diagonal
+// add to the angular block (geometric stiffness).
+struct Row {
+  float nn[3];
+  float t[3];
+  float pen;
+  float f;
+  int kind;
+};
+
+__device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, f3 lever, float sign) {
+  f3 nn = smul(sign, axis);
+  f3 t = cross3(lever, nn);
+  r.nn[0] = nn.x; r.nn[1] = nn.y; r.nn[2] = nn.z;
+  r.t[0] = t.x; r.t[1] = t.y; r.t[2] = t.z;
+  r.pen = pen;
+  r.f = f;
+  r.kind = 0;
+}
+
+// Lower-triangle element e (0..20: row-major i >= j) or rhs (21..26)
+// contribution of a row, in the exact product form of stamp_row.
+__device__ __forceinline__ bool row_elem(const Row& r, int i, int j, int e, float* v) {
+  if (r.kind == 0) {
+    if (e >= 21) {
+      int k = e - 21;
+      *v = k < 3 ? r.f * r.nn[k] : r.f * r.t[k - 3];
+      return true;
+    }
+    if (i < 3)
+      *v = r.pen * (r.nn[i] * r.nn[j]);
+    else if (j >= 3)
+      *v = r.pen * (r.t[i - 3] * r.t[j - 3]);
+    else
+      *v = r.pen * (r.nn[j] * r.t[i - 3]);
+    return true;
+  }
+  if (r.kind == 1) {
+    if (e >= 21) {
+      int k = e - 21;
+      if (k < 3) return false;
+      *v = r.f * r.nn[k - 3];
+      return true;
+    }
+    if (i < 3 || j < 3) return false;
+    *v = r.pen * (r.nn[i - 3] * r.nn[j - 3]);
+    return true;
+  }
+  if (e < 21 && i >= 3 && i == j) {
+    *v = r.nn[i - 3];
+    return true;
+  }
+  return false;
+}
+
+// Rows of one contact point-side of body h (in the reference's row order).
+// Returns the number of rows written to out (<= 6).
+__device__ int point_rows(const Dev& d, uint32_t it, Row* out) {
+  uint32_t q = it >> 1;
+  bool is_a = (it & 1u) == 0u;
+  float sign = is_a ? 1.0f : -1.0f;
+  uint32_t p = d.Q.pair[q];
+  uint2 ab = d.P.ab[p];
+  float4 n4 = d.P.n[p];
+  f3 n = xyz(n4);
+  float gap = n4.w;
+  bool touching = gap <= 0.0f;
+  float4 ra4 = d.Q.ra[q];
+  float c0 = touching ? ra4.w : 0.0f;
+  float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+  float4 a0 = d.b.pos0[ab.x], b0 = d.b.pos0[ab.y];
+  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  f3 dra = xyz(d.b.drot[ab.x]), drb = xyz(d.b.drot[ab.y]);
+  f3 dpa = sub(xyz(pa4), xyz(a0)), dpb = sub(xyz(pb4), xyz(b0));
+  // contact_levers (rows.rs:963-973)
+  f3 ra_w = a0.w >= 0.0f ? mul(neg(n), a0.w) : qrot(qa, xyz(ra4));
+  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(d.Q.rb[q]));
+  // row_c_levers (rows.rs:1006-1025)
+  auto row_c = [&](f3 u, float cc, f3 la, f3 lb) {
+    f3 d_a = add(dpa, mul(u, dot3(cross3(la, u), dra)));
+    f3 d_b = add(dpb, mul(u, dot3(cross3(lb, u), drb)));
+    return cc * (1.0f - K_ALPHA) + dot3(u, sub(d_a, d_b));
+  };
+  float cn = row_c(n, c0, ra_w, rb_w);
+  f3 r_side = is_a ? ra_w : rb_w;
+  float4 t14 = d.P.t1[p];
+  if (!touching) {
+    float damp = t14.w;
+    if (damp > 0.0f && fabsf(cn) >= K_C_EPS) {
+      float f_d = fmin_(damp * cn, 0.0f);
+      make_stamp(out[0], n, damp, f_d, r_side, sign);
+      return 1;
+    }
+    return 0;
+  }
+  f3 t1 = xyz(t14), t2 = xyz(d.P.t2[p]);
+  // friction_levers (rows.rs:987-996)
+  float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
+  f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
+  f3 rb_f = add(rb_w, mul(n, sep * 0.5f));
+  float ct1 = row_c(t1, 0.0f, ra_f, rb_f);
+  float ct2 = row_c(t2, 0.0f, ra_f, rb_f);
+  float4 pen = d.Q.pen[q], lam = d.Q.lam[q];
+  float4 mu = d.P.mu[p];
+  // contact_force (rows.rs:1032-1060)
+  float fn_c = fmin_(pen.x * cn + lam.x, 0.0f);
+  float r0 = pen.y * ct1 + lam.y, r1 = pen.z * ct2 + lam.z;
+  float b1 = fabsf(fn_c) * mu.x, b2 = fabsf(fn_c) * mu.y;
+  float f1, f2;
+  if (b1 <= 0.0f || b2 <= 0.0f) {
+    f1 = b1 <= 0.0f ? 0.0f : clampf_(r0, -b1, b1);
+    f2 = b2 <= 0.0f ? 0.0f : clampf_(r1, -b2, b2);
+  } else {
+    float s = (r0 / b1) * (r0 / b1) + (r1 / b2) * (r1 / b2);
+    float k = s > 1.0f ? 1.0f / sqrtf(s) : 1.0f;
+    f1 = r0 * k;
+    f2 = r1 * k;
+  }
+  int m = 0;
+  f3 r_side_f = is_a ? ra_f : rb_f;
+  if (fabsf(cn) >= K_C_EPS || fn_c != 0.0f) make_stamp(out[m++], n, pen.x, fn_c, r_side, sign);
+  if (fabsf(ct1) >= K_C_EPS || f1 != 0.0f) make_stamp(out[m++], t1, pen.y, f1, r_side_f, sign);
+  if (fabsf(ct2) >= K_C_EPS || f2 != 0.0f) make_stamp(out[m++], t2, pen.z, f2, r_side_f, sign);
+  if (mu.z > 0.0f || mu.w > 0.0f) {
+    f3 wrel = sub(dra, drb);
+    float4 rl = d.Q.roll[q];
+    f3 ax[3] = {t1, t2, n};
+    float mur[3] = {mu.z, mu.z, mu.w};
+    float rlv[3] = {rl.x, rl.y, rl.z};
+    for (int li = 0; li < 3; ++li) {
+      if (mur[li] <= 0.0f) continue;
+      float c = dot3(wrel, ax[li]);
+      float cap = mur[li] * fabsf(fn_c);
+      float fr = clampf_(K_ROLL_PEN * c + rlv[li], -cap, cap);
+      f3 g = is_a ? ax[li] : neg(ax[li]);
+      Row& r = out[m++];
+      r.nn[0] = g.x; r.nn[1] = g.y; r.nn[2] = g.z;
+      r.pen = K_ROLL_PEN;
+      r.f = fr;
+      r.kind = 1;
+    }
+  }
+  return m;
+}
+
+// Rows of one Ball-joint side (assembly.rs:279-310, :589-618).
+__device__ int joint_rows(const Dev& d, uint32_t e, Row* out) {
+  uint32_t j = e >> 1;
+  bool is_a = (e & 1u) == 0u;
+  float sign = is_a ? 1.0f : -1.0f;
+  uint2 ab = d.J.ab[j];
+  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  f3 la = xyz(d.J.la[j]), lb = xyz(d.J.lb[j]);
+  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(qa, la));
+  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, lb));
+  f3 live = sub(pa, pb);
+  f3 c0v = xyz(d.J.c0v[j]);
+  float4 lam = d.J.lam[j], pen = d.J.pen[j];
+  float lamv[3] = {lam.x, lam.y, lam.z}, penv[3] = {pen.x, pen.y, pen.z};
+  f3 r_a = qrot(qa, xyz(d.J.la0[j]));
+  f3 r_b = qrot(qb, lb);
+  float fl[3] = {0.0f, 0.0f, 0.0f};
+  int m = 0;
+  for (int k = 0; k < 3; ++k) {
+    float c = comp(live, k) - K_ALPHA * comp(c0v, k);
+    float f = penv[k] * c + lamv[k];
+    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+    f3 axis = zero3();
+    setc(axis, k, 1.0f);
+    make_stamp(out[m++], axis, penv[k], f, is_a ? r_a : r_b, sign);
+    fl[k] = f;
+  }
+  f3 r = is_a ? r_a : neg(r_b);
+  float hm[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+  for (int k = 0; k < 3; ++k) {
+    float g[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    float rk = comp(r, k);
+    g[0][0] = -rk;
+    g[1][1] = -rk;
+    g[2][2] = -rk;
+    g[0][k] += r.x;
+    g[1][k] += r.y;
+    g[2][k] += r.z;
+    for (int x = 0; x < 3; ++x)
+      for (int y = 0; y < 3; ++y) hm[x][y] += g[x][y] * fl[k];
+  }
+  Row& gr = out[m++];
+  gr.nn[0] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
+  gr.nn[1] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
+  gr.nn[2] = len3(f3{hm[0][2], hm[1][2], hm[2][2]});
+  gr.kind = 2;
+  return m;
+}
+
+// solve_6x6 (assembly.rs:25-69) on the 21-entry lower triangle.
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+__device__ bool solve_6x6_dev(const float* L21, const float* rhs, float* x) {
+  float l[6][6];
+  float dg[6];
+  for (int i = 0; i < 6; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      float s = L21[tri(i, j)];
+      for (int k = 0; k < j; ++k) s -= l[i][k] * dg[k] * l[j][k];
+      if (i == j) {
+        if (s <= K_DEGENERATE) return false;
+        dg[i] = s;
+        l[i][i] = 1.0f;
+      } else {
+        l[i][j] = s / dg[j];
+      }
+    }
+  }
+  float y[6];
+  for (int i = 0; i < 6; ++i) {
+    float s = rhs[i];
+    for (int k = 0; k < i; ++k) s -= l[i][k] * y[k];
+    y[i] = s;
+  }
+  float z[6];
+  for (int i = 0; i < 6; ++i) z[i] = y[i] / dg[i];
+  for (int i = 5; i >= 0; --i) {
+    float s = z[i];
+    for (int k = i + 1; k < 6; ++k) s -= l[k][i] * x[k];
+    x[i] = s;
+  }
+  return true;
+}
+
+// Inertial block and rhs of body h (assembly.rs:74-96): acc[0..20] lower
+// triangle, acc[21..26] rhs.
+__device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, int i, int j) {
+  float4 p = d.b.pos[h];
+  float m_dt2 = 1.0f / p.w / (K_DT * K_DT);
+  qf q = asq(d.b.rot[h]);
+  float iw[3][3];
+  world_inertia(xyz(d.b.inertia[h]), q, iw);
+  if (e < 21) {
+    if (i < 3) return i == j ? m_dt2 : 0.0f;
+    if (j < 3) return 0.0f;
+    return iw[i - 3][j - 3] / (K_DT * K_DT);
+  }
+  int k = e - 21;
+  if (k < 3) {
+    f3 rl = smul(m_dt2, sub(xyz(p), xyz(d.b.inertial[h])));
+    return comp(rl, k);
+  }
+  f3 dv = quat_diff_vec(q, asq(d.b.inertial_rot[h]));
+  f3 ra = divs(mat3_vec(iw, dv), K_DT * K_DT);
+  return comp(ra, k - 3);
+}
+
+#define WAVE 64
+#define ROWS_PER_ITEM 6
+#define WAVES_PER_BLOCK 1
+
+// One wavefront per body.  Lanes evaluate up to 64 items (point-sides,
+// then joint-sides) at a time into LDS; lanes 0..26 then add the staged
+// rows to their matrix element strictly in item-then-row order, which is
+// the reference's sequential summation order; lane 0 factors and solves.
+__global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK) k_primal_wave(Dev d, const uint32_t* __restrict__ list,
+                                                                        uint32_t count) {
+  __shared__ Row rows[WAVES_PER_BLOCK][WAVE * ROWS_PER_ITEM];
+  __shared__ int nrows[WAVES_PER_BLOCK][WAVE];
+  __shared__ float acc_s[WAVES_PER_BLOCK][27];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int w = threadIdx.x / WAVE;
+  uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
+  if (slot >= count) return;  // whole wave exits together
+  uint32_t h = list[slot];
+  // element of this lane
+  int ei = 0, ej = 0;
+  if (lane < 21) {
+    int i = 0;
+    while (tri(i + 1, 0) <= lane) ++i;
+    ei = i;
+    ej = lane - tri(i, 0);
+  }
+  float acc = 0.0f;
+  if (lane < 27) acc = inertial_elem(d, h, lane, ei, ej);
+  uint32_t i0 = d.inc_off[h], np = d.inc_off[h + 1] - i0;
+  uint32_t j0 = d.b.ji_off[h], nj = d.b.ji_off[h + 1] - j0;
+  uint32_t total = np + nj;
+  for (uint32_t base = 0; base < total; base += WAVE) {
+    uint32_t k = base + lane;
+    int m = 0;
+    if (k < np)
+      m = point_rows(d, d.inc[i0 + k], &rows[w][lane * ROWS_PER_ITEM]);
+    else if (k < total)
+      m = joint_rows(d, d.b.ji[j0 + (k - np)], &rows[w][lane * ROWS_PER_ITEM]);
+    nrows[w][lane] = m;
+    __syncthreads();
+    if (lane < 27) {
+      uint32_t lim = total - base < (uint32_t)WAVE ? total - base : (uint32_t)WAVE;
+      for (uint32_t it = 0; it < lim; ++it) {
+        int mm = nrows[w][it];
+        for (int r = 0; r < mm; ++r) {
+          float v;
+          if (row_elem(rows[w][it * ROWS_PER_ITEM + r], ei, ej, lane, &v)) acc += v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (lane < 27) acc_s[w][lane] = acc;
+  __syncthreads();
+  if (lane == 0) {
+    float L21[21], rhs[6], dx[6];
+    for (int e = 0; e < 21; ++e) L21[e] = acc_s[w][e];
+    for (int k = 0; k < 6; ++k) rhs[k] = -acc_s[w][21 + k];
+    if (solve_6x6_dev(L21, rhs, dx)) {
+      float4 p = d.b.pos[h];
+      d.b.pos[h] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
+      qf q = quat_integrate(asq(d.b.rot[h]), f3{dx[3], dx[4], dx[5]});
+      d.b.rot[h] = pkq(q);
+      d.b.drot[h] = pk(quat_diff_vec(q, asq(d.b.rot0[h])), 0.0f);
+    } else {
+      atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// dual update (joints_ext.rs:464-598), one thread per point / joint
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB) k_dual(Dev d) {
+  uint32_t q = gid();
+  if (q >= d.npts) return;
+  uint32_t p = d.Q.pair[q];
+  uint2 ab = d.P.ab[p];
+  float4 n4 = d.P.n[p];
+  f3 n = xyz(n4);
+  bool touching = n4.w <= 0.0f;
+  float4 ra4 = d.Q.ra[q], rb4 = d.Q.rb[q];
+  float c0 = touching ? ra4.w : 0.0f;
+  float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+  float4 a0 = d.b.pos0[ab.x], b0 = d.b.pos0[ab.y];
+  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  f3 dra = xyz(d.b.drot[ab.x]), drb = xyz(d.b.drot[ab.y]);
+  f3 dpa = sub(xyz(pa4), xyz(a0)), dpb = sub(xyz(pb4), xyz(b0));
+  f3 ra_w = a0.w >= 0.0f ? mul(neg(n), a0.w) : qrot(qa, xyz(ra4));
+  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(rb4));
+  auto row_c = [&](f3 u, float cc, f3 la, f3 lb) {
+    f3 d_a = add(dpa, mul(u, dot3(cross3(la, u), dra)));
+    f3 d_b = add(dpb, mul(u, dot3(cross3(lb, u), drb)));
+    return cc * (1.0f - K_ALPHA) + dot3(u, sub(d_a, d_b));
+  };
+  float cn = row_c(n, c0, ra_w, rb_w);
+  f3 t1 = xyz(d.P.t1[p]), t2 = xyz(d.P.t2[p]);
+  float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
+  f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
+  f3 rb_f = add(rb_w, mul(n, sep * 0.5f));
+  float ct1 = row_c(t1, 0.0f, ra_f, rb_f);
+  float ct2 = row_c(t2, 0.0f, ra_f, rb_f);
+  float4 pen = d.Q.pen[q], lam = d.Q.lam[q];
+  float4 mu = d.P.mu[p];
+  float fn_c = fmin_(pen.x * cn + lam.x, 0.0f);
+  float r0 = pen.y * ct1 + lam.y, r1 = pen.z * ct2 + lam.z;
+  float b1 = fabsf(fn_c) * mu.x, b2 = fabsf(fn_c) * mu.y;
+  float f1, f2;
+  if (b1 <= 0.0f || b2 <= 0.0f) {
+    f1 = b1 <= 0.0f ? 0.0f : clampf_(r0, -b1, b1);
+    f2 = b2 <= 0.0f ? 0.0f : clampf_(r1, -b2, b2);
+  } else {
+    float s = (r0 / b1) * (r0 / b1) + (r1 / b2) * (r1 / b2);
+    float k = s > 1.0f ? 1.0f / sqrtf(s) : 1.0f;
+    f1 = r0 * k;
+    f2 = r1 * k;
+  }
+  {
+    float bnd = fabsf(fn_c) * fmax_(mu.x, mu.y);
+    float fs = sqrtf(r0 * r0 + r1 * r1);
+    bool stuck = fs <= bnd && sqrtf(ct1 * ct1 + ct2 * ct2) < K_STICK;
+    rb4.w = stuck ? 1.0f : 0.0f;
+    d.Q.rb[q] = rb4;
+  }
+  float4 nl = lam, npn = pen;
+  if (touching && fabsf(cn) >= K_C_EPS) {
+    nl.x = fn_c;
+    if (fn_c < 0.0f) npn.x = fmin_(pen.x + K_BETA * fabsf(cn), K_PEN_MAX);
+  }
+  float bb1 = fabsf(fn_c) * mu.x, bb2 = fabsf(fn_c) * mu.y;
+  float s_ell = (bb1 > 0.0f && bb2 > 0.0f) ? (r0 / bb1) * (r0 / bb1) + (r1 / bb2) * (r1 / bb2) : __builtin_inff();
+  if (touching) {
+    nl.y = f1;
+    nl.z = f2;
+    if (s_ell <= 1.0f) {
+      if (fabsf(ct1) >= K_C_EPS) npn.y = fmin_(pen.y + K_BETA * fabsf(ct1), K_PEN_MAX);
+      if (fabsf(ct2) >= K_C_EPS) npn.z = fmin_(pen.z + K_BETA * fabsf(ct2), K_PEN_MAX);
+    }
+  }
+  d.Q.lam[q] = nl;
+  d.Q.pen[q] = npn;
+  if (touching && (mu.z > 0.0f || mu.w > 0.0f)) {
+    f3 wrel = sub(dra, drb);
+    float lam_n = fabsf(nl.x);
+    float4 rl = d.Q.roll[q];
+    f3 ax[3] = {t1, t2, n};
+    float mur[3] = {mu.z, mu.z, mu.w};
+    float rlv[3] = {rl.x, rl.y, rl.z};
+    for (int li = 0; li < 3; ++li) {
+      if (mur[li] <= 0.0f) continue;
+      float c = dot3(wrel, ax[li]);
+      float cap = mur[li] * lam_n;
+      rlv[li] = clampf_(K_ROLL_PEN * c + rlv[li], -cap, cap);
+    }
+    d.Q.roll[q] = make_float4(rlv[0], rlv[1], rlv[2], 0.0f);
+  }
+}
+
+__global__ void k_dual_joint(Dev d) {
+  uint32_t j = gid();
+  if (j >= d.njoints) return;
+  uint2 ab = d.J.ab[j];
+  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(asq(d.b.rot[ab.x]), xyz(d.J.la[j])));
+  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(asq(d.b.rot[ab.y]), xyz(d.J.lb[j])));
+  f3 live = sub(pa, pb);
+  f3 c0v = xyz(d.J.c0v[j]);
+  float4 lam = d.J.lam[j], pen = d.J.pen[j];
+  float lv[3] = {lam.x, lam.y, lam.z}, pv[3] = {pen.x, pen.y, pen.z};
+  for (int k = 0; k < 3; ++k) {
+    float c = comp(live, k) - K_ALPHA * comp(c0v, k);
+    if (fabsf(c) >= K_C_EPS) {
+      lv[k] += pv[k] * c;
+      pv[k] = fmin_(pv[k] + K_BETA * fabsf(c), K_PEN_MAX);
+    }
+  }
+  d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], 0.0f);
+  d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// finish: BDF1 (mod.rs:1062-1078) and sleep (mod.rs:1088-1122)
+// ---------------------------------------------------------------------------
+__global__ void k_bdf1(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  float4 p = d.b.pos[h];
+  d.b.supported[h] = 0;
+  if (!(d.b.type[h] == BT_DYNAMIC && p.w > 0.0f) || d.b.asleep[h]) return;
+  f3 v = divs(sub(xyz(p), xyz(d.b.pos0[h])), K_DT);
+  qf q = asq(d.b.rot[h]);
+  f3 spin = quat_diff_vec(q, asq(d.b.rot0[h]));
+  f3 w = len3(spin) < K_SPIN_REST ? zero3() : divs(spin, K_DT);
+  float4 v4 = d.b.vel[h], w4 = d.b.angvel[h];
+  d.b.vel[h] = pk(v, v4.w);
+  d.b.angvel[h] = pk(w, w4.w);
+  d.b.rot[h] = pkq(qnormalize(q));
+}
+
+__global__ void k_supported(Dev d) {
+  uint32_t p = gid();
+  if (p < d.npairs) {
+    if (d.P.n[p].w <= 0.0f && d.P.pts[p].y > 0u) {
+      uint2 ab = d.P.ab[p];
+      d.b.supported[ab.x] = 1;
+      d.b.supported[ab.y] = 1;
+    }
+  }
+  if (p < d.njoints) {
+    uint2 ab = d.J.ab[p];
+    d.b.supported[ab.x] = 1;
+    d.b.supported[ab.y] = 1;
+  }
+}
+
+// joint_sleep_ready (sleep.rs:139-150): flag_b = ready.
+__global__ void k_ready_init(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  d.b.flag_b[h] = 1;
+}
+__global__ void k_ready_joints(Dev d) {
+  uint32_t j = gid();
+  if (j >= d.njoints) return;
+  if (joint_unsettled(d, j)) {
+    uint2 ab = d.J.ab[j];
+    d.b.flag_b[ab.x] = 0;
+    d.b.flag_b[ab.y] = 0;
+  }
+}
+
+// Sleep timers; flag_a = freeze.
+__global__ void k_sleep_timer(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  uint8_t asl = d.b.asleep[h];
+  d.b.flag_a[h] = asl;
+  if (!(d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f) || asl) return;
+  if (!d.b.supported[h] || !d.b.flag_b[h]) {
+    d.b.timer[h] = 0.0f;
+    return;
+  }
+  if (len3(xyz(d.b.vel[h])) < K_SLEEP_LIN && len3(xyz(d.b.angvel[h])) < K_SLEEP_ANG) {
+    float t = d.b.timer[h] + K_DT;
+    d.b.timer[h] = t;
+    d.b.flag_a[h] = t >= K_SLEEP_TIME ? 1 : 0;
+  } else {
+    d.b.timer[h] = 0.0f;
+  }
+}
+
+// sleep_body (mod.rs:822-828) where frozen; refresh the driver baseline
+// (mod.rs:1139-1142); count sleepers.
+__global__ void k_sleep_apply(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  if (d.b.flag_a[h] && !d.b.asleep[h] && d.b.type[h] == BT_DYNAMIC) {
+    float4 v = d.b.vel[h], w = d.b.angvel[h], in = d.b.inertia[h], p = d.b.pos[h];
+    d.b.vel[h] = make_float4(0.0f, 0.0f, 0.0f, v.w);
+    d.b.angvel[h] = make_float4(0.0f, 0.0f, 0.0f, w.w);
+    d.b.inertia[h] = make_float4(0.0f, 0.0f, 0.0f, in.w);
+    d.b.pos[h] = make_float4(p.x, p.y, p.z, 0.0f);
+    d.b.asleep[h] = 1;
+  }
+  d.b.flag_a[h] = 0;
+  if (d.b.asleep[h]) atomicAdd(&d.counters[CTR_SLEEPING], 1u);
+  d.b.prev_pos[h] = d.b.pos[h];
+  d.b.prev_rot[h] = d.b.rot[h];
+}

@@ -1,0 +1,182 @@
+// avbd_state.hpp — device-resident state of one AVBD context (SoA in HBM).
+//
+// Bodies, the canonical pair table (sorted by (a, b)), contact points,
+// Ball joints and the per-step scratch of the pipeline.  All arrays live on
+// one GPU; a context's kernels read and write them through this struct,
+// passed by value to every launch.
+#pragma once
+
+#include <stdint.h>
+
+#include "avbd_math.hpp"
+
+// Step constants (crates/physics/src/avbd/mod.rs:154-247, :1205).
+#define K_DT (1.0f / 60.0f)
+#define K_ALPHA 0.99f
+#define K_GAMMA 0.999f
+#define K_BETA 10000.0f
+#define K_MARGIN 0.01f
+#define K_PEN_MIN 1.0f
+#define K_PEN_MAX 1.0e10f
+#define K_PEN_INIT 1.0f
+#define K_GEN_MARGIN 0.05f
+#define K_EXPAND_SLOP 0.005f
+#define K_POINT_MATCH 0.03f
+#define K_LAMBDA_MATCH 0.05f
+#define K_C_EPS 1e-7f
+#define K_STICK 0.00001f
+#define K_SLEEP_LIN 0.15f
+#define K_SLEEP_ANG 0.15f
+#define K_SLEEP_TIME 0.5f
+#define K_WAKE_IMPACT 0.5f
+#define K_WAKE_PEN 0.01f
+#define K_ROLL_PEN 100.0f
+#define K_MIN_TOI_LEN 1e-9f
+#define K_SPIN_REST 1e-9f
+#define K_CCD_BOUNCE 1.0f
+#define K_NEAR_ZERO 1e-9f
+#define K_AXIS_REST_LEN2 1e-16f
+#define K_DEGENERATE 1e-12f
+#define K_SHAPE_TOUCH 1e-3f
+#define K_CCD_GATE 0.5f
+#define K_JOINT_REST 0.005f
+#define K_MAX_POINTS 8
+
+enum { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
+
+struct Bodies {
+  float4* pos;       // xyz, w = effective inverse mass (0 unless dynamic & awake)
+  float4* rot;       // quaternion xyzw
+  float4* vel;       // xyz
+  float4* angvel;    // xyz
+  float4* torque;    // xyz
+  float4* inertia;   // body-frame diagonal xyz (0 while asleep), w = mass
+  float4* shape;     // param xyz, w = bound radius
+  float4* mat;       // friction, restitution, rolling, torsion
+  int* kind;         // shape kind
+  int* type;         // body type
+  uint32_t* layer;
+  uint32_t* mask;
+  uint32_t* scene;
+  uint32_t* prio;    // colouring priority hash of the scene-local index
+  uint32_t* local;   // scene-local index (priority tie-break)
+  uint8_t* trigger;
+  float4* pos0;      // step-start position, w = sphere radius (or -1)
+  float4* rot0;
+  float4* inertial;
+  float4* inertial_rot;
+  float4* drot;      // quat_diff_vec(rot, rot0), kept current
+  float4* prev_pos;
+  float4* prev_rot;
+  uint8_t* asleep;
+  float* timer;
+  int* colour;
+  int* col_round;
+  uint8_t* flag_a;   // scratch flags (wake / awake / ready / freeze)
+  uint8_t* flag_b;
+  uint8_t* supported;
+  uint32_t* comp;    // joint component label (static)
+  uint8_t* comp_any;
+  uint32_t* jp_off;  // joint-partner CSR (static): partners for no-collide
+  uint32_t* jp;
+  uint32_t* ji_off;  // joint incidence CSR (static): (joint << 1) | side
+  uint32_t* ji;
+};
+
+struct Pairs {
+  uint2* ab;         // body indices, a < b
+  float4* n;         // normal xyz (B -> A), w = signed gap
+  float4* t1;        // friction tangent t1, w = damper stiffness (separated pairs)
+  float4* t2;        // t2 = t1 x n
+  float4* mu;        // mu1, mu2, rolling, torsion coefficients
+  uint2* pts;        // first point, point count
+};
+
+struct Points {
+  float4* ra;        // anchor on A (body frame), w = step-start violation c0
+  float4* rb;        // anchor on B (body frame), w = stuck (0/1)
+  float4* lam;       // duals xyz
+  float4* pen;       // penalties xyz
+  float4* roll;      // rolling/torsion duals xyz
+  uint32_t* pair;    // owning pair
+};
+
+struct Joints {
+  uint2* ab;
+  float4* la;        // local anchor on A
+  float4* lb;        // local anchor on B
+  float4* la0;       // la + 0 (the lever form the reference stamps with)
+  float4* lam;       // lam_l xyz
+  float4* pen;       // pen_l xyz
+  float4* c0v;       // step-start anchor separation
+};
+
+// Broadphase grid + candidate scratch.
+struct Grid {
+  uint32_t* cnt;     // per bucket
+  uint32_t* start;   // per bucket (exclusive scan)
+  uint32_t* cursor;  // per bucket fill cursor
+  uint32_t* items;   // body ids bucketed
+  uint32_t* bucket;  // per body bucket id (0xffffffff = large / not gridded)
+  uint32_t nbuckets; // power of two
+  float cell;
+  uint32_t* large;   // large body ids
+  uint32_t nlarge;
+  uint8_t* is_large;
+};
+
+struct Cands {
+  uint2* ab;
+  uint32_t* acc;     // accepted (becomes a pair)
+  uint32_t* npts;    // fresh point count of accepted candidates
+  float4* disc0;     // normal xyz, signed gap
+  float4* disc1;     // mu1, mu2, old pair index (as int bits), wake vote
+  float4* fresh;     // 8 x (ra xyz, rb.x), 8 x (rb.y, rb.z, -, -) per candidate
+};
+
+struct Dev {
+  Bodies b;
+  uint32_t n;
+  Pairs P;           // current pair table
+  Pairs Pold;        // previous step's table (merge source)
+  Points Q;
+  Points Qold;
+  uint32_t npairs, npts, npairs_old, npts_old;
+  Joints J;
+  uint32_t njoints;
+  Grid g;
+  Cands C;
+  float gravity[3];
+  uint32_t* counters;  // device counters (see CTR_*)
+  // per-step incidence
+  uint32_t* inc_off;   // n+1
+  uint32_t* inc;       // (point << 1) | side, in pair order per body
+  uint32_t* bcnt;      // b-side pair counts
+  uint32_t* boff;      // n+1
+  uint32_t* bcur;
+  uint32_t* blist;
+  uint32_t* cand_off;  // n+1 candidate offsets per first body
+  uint32_t* acc_off;   // ncand+1 exclusive scan of accepted
+  uint32_t* pt_off;    // ncand+1 exclusive scan of fresh points
+  uint32_t* ptcnt;     // per body point-side counts
+  uint32_t* lflag;     // nlarge x n flags of large-body candidates
+  uint32_t* lrank;     // nlarge x (n+1) scans
+  uint32_t* colour_cnt;
+  uint32_t* colour_off;
+  uint32_t* colour_cur;
+  uint32_t* colour_list;
+  float4* toi_hit;     // t, normal xyz per body (t < 0: none)
+  uint32_t* toi_tgt;
+  uint8_t* fast;
+};
+
+enum {
+  CTR_FAST = 0,
+  CTR_REMAIN = 1,
+  CTR_MAXCOL = 2,
+  CTR_SLEEPING = 3,
+  CTR_BREAKDOWN = 4,
+  CTR_TOIHITS = 5,
+  CTR_ANYASLEEP = 6,
+  CTR_COUNT = 8
+};

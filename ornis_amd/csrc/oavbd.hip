@@ -1,0 +1,890 @@
+// oavbd.hip — liboavbd.so: C ABI, device memory and step orchestration of
+// the MI355X AVBD rigid-body step (declarations: include/oavbd.h).
+//
+// One context = one HIP stream on one GPU.  All solver state stays in HBM
+// between steps; a step is a fixed sequence of kernel launches
+// (avbd_kernels.hpp) with a handful of small device->host count reads that
+// size the next launches (candidate pairs, pairs/points, colour classes).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/oavbd.h"
+#include "avbd_kernels.hpp"
+#include "avbd_scan.hpp"
+
+namespace {
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIPC(expr)                                                                                   \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)};         \
+  } while (0)
+
+inline uint32_t blocks(uint64_t n, uint32_t tpb = TPB) { return (uint32_t)((n + tpb - 1) / tpb); }
+
+template <class T>
+void dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) throw HipError{std::string("hipMalloc: ") + hipGetErrorString(e)};
+}
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// lowbias32 integer hash (C. Wellons): colouring priority of a local index.
+inline uint32_t prio_hash(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+struct HostJoint {
+  uint32_t a, b;
+  float la[3], lb[3];
+};
+
+}  // namespace
+
+struct oavbd_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  oavbd_params prm{};
+  uint32_t max_bodies = 0, max_joints = 0;
+  Dev d{};
+  size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0;
+  uint32_t ncand = 0;
+  uint32_t* cand_cnt = nullptr;  // n + 1
+  uint32_t* scan_scratch = nullptr;
+  uint32_t* counters_h = nullptr;  // pinned
+  std::vector<HostJoint> joints;
+  bool joints_dirty = false;
+  std::vector<int32_t> kind_h, type_h;
+  float time_debt = 0.0f;
+  uint32_t sleeping = 0;
+  oavbd_stats stats{};
+  std::vector<uint32_t> colour_cnt_h, colour_off_h;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  float ms[5] = {};
+  std::string err;
+};
+
+static int fail(oavbd_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define GUARD(c, ...)                                       \
+  do {                                                      \
+    try {                                                   \
+      __VA_ARGS__;                                          \
+    } catch (const HipError& e_) {                          \
+      return fail((c), OAVBD_EDEVICE, e_.msg);              \
+    } catch (const std::bad_alloc&) {                       \
+      return fail((c), OAVBD_ENOMEM, "host allocation failed"); \
+    }                                                       \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// memory
+// ---------------------------------------------------------------------------
+static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
+  Bodies& b = c->d.b;
+  dalloc(&b.pos, n); dalloc(&b.rot, n); dalloc(&b.vel, n); dalloc(&b.angvel, n); dalloc(&b.torque, n);
+  dalloc(&b.inertia, n); dalloc(&b.shape, n); dalloc(&b.mat, n); dalloc(&b.kind, n); dalloc(&b.type, n);
+  dalloc(&b.layer, n); dalloc(&b.mask, n); dalloc(&b.scene, n); dalloc(&b.prio, n); dalloc(&b.local, n);
+  dalloc(&b.trigger, n); dalloc(&b.pos0, n); dalloc(&b.rot0, n); dalloc(&b.inertial, n);
+  dalloc(&b.inertial_rot, n); dalloc(&b.drot, n); dalloc(&b.prev_pos, n); dalloc(&b.prev_rot, n);
+  dalloc(&b.asleep, n); dalloc(&b.timer, n); dalloc(&b.colour, n); dalloc(&b.col_round, n);
+  dalloc(&b.flag_a, n); dalloc(&b.flag_b, n); dalloc(&b.supported, n); dalloc(&b.comp, n);
+  dalloc(&b.comp_any, n); dalloc(&b.jp_off, n + 1); dalloc(&b.ji_off, n + 1);
+  b.jp = nullptr;
+  b.ji = nullptr;
+  Dev& d = c->d;
+  dalloc(&d.counters, CTR_COUNT);
+  dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
+  dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
+  dalloc(&c->cand_cnt, n + 1);
+  dalloc(&d.colour_list, n); dalloc(&d.colour_cnt, 65536); dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
+  dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
+  uint32_t nb = 1;
+  while (nb < 2 * n) nb <<= 1;
+  d.g.nbuckets = nb;
+  dalloc(&d.g.cnt, nb); dalloc(&d.g.start, nb + 1); dalloc(&d.g.cursor, nb);
+  dalloc(&d.g.items, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
+  d.g.large = nullptr;
+  d.lflag = nullptr;
+  d.lrank = nullptr;
+  dalloc(&c->scan_scratch, ScanScratch::need((size_t)1 << 28));
+  d.blist = nullptr;
+  d.inc = nullptr;
+  d.acc_off = nullptr;
+  d.pt_off = nullptr;
+}
+
+static void free_pairs(Pairs& P) {
+  dfree(P.ab); dfree(P.n); dfree(P.t1); dfree(P.t2); dfree(P.mu); dfree(P.pts);
+}
+static void alloc_pairs(Pairs& P, size_t cap) {
+  dalloc(&P.ab, cap); dalloc(&P.n, cap); dalloc(&P.t1, cap); dalloc(&P.t2, cap); dalloc(&P.mu, cap); dalloc(&P.pts, cap);
+}
+static void free_points(Points& Q) {
+  dfree(Q.ra); dfree(Q.rb); dfree(Q.lam); dfree(Q.pen); dfree(Q.roll); dfree(Q.pair);
+}
+static void alloc_points(Points& Q, size_t cap) {
+  dalloc(&Q.ra, cap); dalloc(&Q.rb, cap); dalloc(&Q.lam, cap); dalloc(&Q.pen, cap); dalloc(&Q.roll, cap);
+  dalloc(&Q.pair, cap);
+}
+
+static size_t grow(size_t need) {
+  size_t c = 1024;
+  while (c < need) c = c + c / 2;
+  return c;
+}
+
+static void ensure_cand(oavbd_ctx* c, size_t need) {
+  if (need <= c->cap_cand && c->d.C.ab) return;
+  size_t cap = grow(need);
+  Cands& C = c->d.C;
+  dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
+  dfree(c->d.acc_off); dfree(c->d.pt_off);
+  dalloc(&C.ab, cap); dalloc(&C.acc, cap); dalloc(&C.npts, cap); dalloc(&C.disc0, cap); dalloc(&C.disc1, cap);
+  dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
+  if (cap + 1 > ((size_t)1 << 28)) throw HipError{"candidate count exceeds the scan capacity"};
+  c->cap_cand = cap;
+}
+
+static void ensure_pairs(oavbd_ctx* c, size_t need) {
+  if (need <= c->cap_pairs && c->d.P.ab) return;
+  free_pairs(c->d.P);
+  c->cap_pairs = grow(need);
+  alloc_pairs(c->d.P, c->cap_pairs);
+}
+
+static void ensure_points(oavbd_ctx* c, size_t need) {
+  if (need <= c->cap_pts && c->d.Q.ra) return;
+  free_points(c->d.Q);
+  c->cap_pts = grow(need);
+  alloc_points(c->d.Q, c->cap_pts);
+}
+
+// Per-step incidence scratch, sized for the current pair / point counts.
+static void ensure_incidence(oavbd_ctx* c, size_t npairs, size_t npts) {
+  if (npairs > c->cap_blist || !c->d.blist) {
+    dfree(c->d.blist);
+    c->cap_blist = grow(npairs);
+    dalloc(&c->d.blist, c->cap_blist);
+  }
+  if (2 * npts > c->cap_inc || !c->d.inc) {
+    dfree(c->d.inc);
+    c->cap_inc = grow(2 * npts);
+    dalloc(&c->d.inc, c->cap_inc);
+  }
+}
+
+static void free_all(oavbd_ctx* c) {
+  Bodies& b = c->d.b;
+  dfree(b.pos); dfree(b.rot); dfree(b.vel); dfree(b.angvel); dfree(b.torque); dfree(b.inertia); dfree(b.shape);
+  dfree(b.mat); dfree(b.kind); dfree(b.type); dfree(b.layer); dfree(b.mask); dfree(b.scene); dfree(b.prio);
+  dfree(b.local); dfree(b.trigger); dfree(b.pos0); dfree(b.rot0); dfree(b.inertial); dfree(b.inertial_rot);
+  dfree(b.drot); dfree(b.prev_pos); dfree(b.prev_rot); dfree(b.asleep); dfree(b.timer); dfree(b.colour);
+  dfree(b.col_round); dfree(b.flag_a); dfree(b.flag_b); dfree(b.supported); dfree(b.comp); dfree(b.comp_any);
+  dfree(b.jp_off); dfree(b.jp); dfree(b.ji_off); dfree(b.ji);
+  Dev& d = c->d;
+  free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
+  dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
+  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.bucket); dfree(d.g.large);
+  dfree(d.g.is_large);
+  Cands& C = d.C;
+  dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
+  dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
+  dfree(d.cand_off); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
+  dfree(d.colour_cnt); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
+  dfree(d.toi_tgt); dfree(d.fast); dfree(c->cand_cnt); dfree(c->scan_scratch);
+}
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = gid();
+  if (i < n) p[i] = 0u;
+}
+__global__ void k_zero_u8(uint8_t* p, uint32_t n) {
+  uint32_t i = gid();
+  if (i < n) p[i] = 0;
+}
+
+static void zero_u32(oavbd_ctx* c, uint32_t* p, uint32_t n) {
+  if (n) k_zero_u32<<<blocks(n), TPB, 0, c->st>>>(p, n);
+}
+
+static void read_counters(oavbd_ctx* c) {
+  HIPC(hipMemcpyAsync(c->counters_h, c->d.counters, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+}
+
+static uint32_t read_u32(oavbd_ctx* c, const uint32_t* dptr) {
+  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+  return c->counters_h[CTR_COUNT];
+}
+
+// ---------------------------------------------------------------------------
+// static joint structures: partner CSR (no-collide), incidence CSR (rows,
+// colouring), dynamic-joint components (sleep/wake propagation)
+// ---------------------------------------------------------------------------
+static void upload_joints(oavbd_ctx* c) {
+  Dev& d = c->d;
+  uint32_t n = d.n, nj = (uint32_t)c->joints.size();
+  dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
+  dfree(d.b.jp); dfree(d.b.ji);
+  dalloc(&d.J.ab, nj); dalloc(&d.J.la, nj); dalloc(&d.J.lb, nj); dalloc(&d.J.la0, nj); dalloc(&d.J.lam, nj);
+  dalloc(&d.J.pen, nj); dalloc(&d.J.c0v, nj);
+  std::vector<uint2> ab(nj);
+  std::vector<float4> la(nj), lb(nj), la0(nj), lam(nj), pen(nj);
+  std::vector<uint32_t> jp_cnt(n + 1, 0), ji_cnt(n + 1, 0);
+  for (uint32_t j = 0; j < nj; ++j) {
+    const HostJoint& h = c->joints[j];
+    ab[j] = make_uint2(h.a, h.b);
+    la[j] = make_float4(h.la[0], h.la[1], h.la[2], 0.0f);
+    lb[j] = make_float4(h.lb[0], h.lb[1], h.lb[2], 0.0f);
+    // the reference stamps A's lever as la + Vec3::ZERO (assembly.rs:297-304)
+    la0[j] = make_float4(h.la[0] + 0.0f, h.la[1] + 0.0f, h.la[2] + 0.0f, 0.0f);
+    lam[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pen[j] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    jp_cnt[h.a]++;
+    jp_cnt[h.b]++;
+    ji_cnt[h.a]++;
+    ji_cnt[h.b]++;
+  }
+  std::vector<uint32_t> jp_off(n + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) jp_off[i + 1] = jp_off[i] + jp_cnt[i];
+  std::vector<uint32_t> jp(jp_off[n]), ji(jp_off[n]), cur(n, 0);
+  for (uint32_t j = 0; j < nj; ++j) {  // joint order per body = ascending joint index
+    const HostJoint& h = c->joints[j];
+    jp[jp_off[h.a] + cur[h.a]] = h.b;
+    ji[jp_off[h.a] + cur[h.a]++] = (j << 1) | 0u;
+    jp[jp_off[h.b] + cur[h.b]] = h.a;
+    ji[jp_off[h.b] + cur[h.b]++] = (j << 1) | 1u;
+  }
+  // components of the graph of joints between two dynamic bodies
+  std::vector<uint32_t> parent(n);
+  std::iota(parent.begin(), parent.end(), 0u);
+  auto find = [&](uint32_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (uint32_t j = 0; j < nj; ++j) {
+    const HostJoint& h = c->joints[j];
+    if (c->type_h[h.a] == BT_DYNAMIC && c->type_h[h.b] == BT_DYNAMIC) {
+      uint32_t x = find(h.a), y = find(h.b);
+      if (x != y) parent[std::max(x, y)] = std::min(x, y);
+    }
+  }
+  std::vector<uint32_t> comp(n);
+  for (uint32_t i = 0; i < n; ++i) comp[i] = find(i);
+  dalloc(&d.b.jp, jp.size());
+  dalloc(&d.b.ji, ji.size());
+  HIPC(hipMemcpy(d.J.ab, ab.data(), nj * sizeof(uint2), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.J.la, la.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.J.lb, lb.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.J.la0, la0.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.J.lam, lam.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.J.pen, pen.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.b.jp_off, jp_off.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d.b.ji_off, jp_off.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (!jp.empty()) {
+    HIPC(hipMemcpy(d.b.jp, jp.data(), jp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.ji, ji.data(), ji.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  HIPC(hipMemcpy(d.b.comp, comp.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  d.njoints = nj;
+  c->joints_dirty = false;
+}
+
+// ---------------------------------------------------------------------------
+// step phases
+// ---------------------------------------------------------------------------
+static void propagate(oavbd_ctx* c, uint8_t* flags, uint8_t value) {
+  uint32_t n = c->d.n;
+  k_prop_clear<<<blocks(n), TPB, 0, c->st>>>(c->d);
+  k_prop_mark<<<blocks(n), TPB, 0, c->st>>>(c->d, flags, value);
+  k_prop_apply<<<blocks(n), TPB, 0, c->st>>>(c->d, flags, value);
+}
+
+static void wake_joint_motion(oavbd_ctx* c) {
+  uint32_t n = c->d.n;
+  k_awake_init<<<blocks(n), TPB, 0, c->st>>>(c->d);
+  if (c->d.njoints) {
+    k_awake_joints<<<blocks(c->d.njoints), TPB, 0, c->st>>>(c->d);
+    propagate(c, c->d.b.flag_b, 1);
+  }
+  k_awake_apply<<<blocks(n), TPB, 0, c->st>>>(c->d);
+}
+
+static void build_grid(oavbd_ctx* c) {
+  Dev& d = c->d;
+  k_grid_clear<<<blocks(d.g.nbuckets), TPB, 0, c->st>>>(d);
+  k_grid_count<<<blocks(d.n), TPB, 0, c->st>>>(d);
+  scan_excl(d.g.cnt, d.g.start, d.g.nbuckets, c->st, c->scan_scratch);
+  k_grid_fill<<<blocks(d.n), TPB, 0, c->st>>>(d);
+}
+
+static void broadphase(oavbd_ctx* c) {
+  Dev& d = c->d;
+  uint32_t n = d.n;
+  k_bp_count<<<blocks(n), TPB, 0, c->st>>>(d, c->cand_cnt);
+  if (d.g.nlarge) {
+    k_bp_large_flag<<<blocks(n), TPB, 0, c->st>>>(d);
+    for (uint32_t k = 0; k < d.g.nlarge; ++k)
+      scan_excl(d.lflag + (size_t)k * n, d.lrank + (size_t)k * (n + 1), n, c->st, c->scan_scratch);
+    k_bp_large_count<<<blocks(d.g.nlarge), TPB, 0, c->st>>>(d, c->cand_cnt);
+  }
+  scan_excl(c->cand_cnt, d.cand_off, n, c->st, c->scan_scratch);
+  c->ncand = read_u32(c, d.cand_off + n);
+  ensure_cand(c, c->ncand);
+  k_bp_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.g.nlarge) k_bp_large_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+}
+
+static void narrow_and_merge(oavbd_ctx* c) {
+  Dev& d = c->d;
+  std::swap(d.P, d.Pold);
+  std::swap(d.Q, d.Qold);
+  std::swap(c->cap_pairs, c->cap_pairs_old);
+  std::swap(c->cap_pts, c->cap_pts_old);
+  d.npairs_old = d.npairs;
+  d.npts_old = d.npts;
+  uint32_t nc = c->ncand;
+  if (nc) k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+  scan_excl(d.C.acc, d.acc_off, nc, c->st, c->scan_scratch);
+  scan_excl(d.C.npts, d.pt_off, nc, c->st, c->scan_scratch);
+  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT + 1, d.pt_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+  d.npairs = c->counters_h[CTR_COUNT];
+  d.npts = c->counters_h[CTR_COUNT + 1];
+  ensure_pairs(c, d.npairs);
+  ensure_points(c, d.npts);
+  ensure_incidence(c, d.npairs, d.npts);
+  if (nc) k_merge<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+  k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d);
+}
+
+static void incidence_and_colour(oavbd_ctx* c) {
+  Dev& d = c->d;
+  uint32_t n = d.n;
+  zero_u32(c, d.ptcnt, n + 1);
+  zero_u32(c, d.bcnt, n + 1);
+  zero_u32(c, d.bcur, n + 1);
+  if (d.npairs) k_inc_count<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  scan_excl(d.ptcnt, d.inc_off, n, c->st, c->scan_scratch);
+  scan_excl(d.bcnt, d.boff, n, c->st, c->scan_scratch);
+  if (d.npairs) k_inc_bfill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
+  // Jones-Plassmann colouring
+  k_colour_init<<<blocks(n), TPB, 0, c->st>>>(d);
+  zero_u32(c, d.colour_cnt, 65536);
+  zero_u32(c, d.counters + CTR_MAXCOL, 1);
+  int round = 0;
+  for (;; ++round) {
+    zero_u32(c, d.counters + CTR_REMAIN, 1);
+    k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, round);
+    if (read_u32(c, d.counters + CTR_REMAIN) == 0) break;
+    if (round > 30000) throw HipError{"colouring did not converge"};
+  }
+  c->stats.colour_rounds = (uint32_t)round + 1;
+  uint32_t ncol = read_u32(c, d.counters + CTR_MAXCOL);
+  c->colour_cnt_h.assign(ncol, 0);
+  c->colour_off_h.assign(ncol + 1, 0);
+  if (ncol) {
+    HIPC(hipMemcpy(c->colour_cnt_h.data(), d.colour_cnt, ncol * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < ncol; ++k) c->colour_off_h[k + 1] = c->colour_off_h[k] + c->colour_cnt_h[k];
+    HIPC(hipMemcpyAsync(d.colour_off, c->colour_off_h.data(), (ncol + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                        c->st));
+  }
+  zero_u32(c, d.colour_cur, std::max<uint32_t>(ncol, 1));
+  k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+  c->stats.colours = ncol;
+  c->stats.max_colour = 0;
+  for (uint32_t k = 0; k < ncol; ++k) c->stats.max_colour = std::max(c->stats.max_colour, c->colour_cnt_h[k]);
+}
+
+static void record(oavbd_ctx* c, int i) {
+  if (c->timing) HIPC(hipEventRecord(c->ev[i], c->st));
+}
+
+static void step_inner(oavbd_ctx* c) {
+  Dev& d = c->d;
+  uint32_t n = d.n;
+  if (c->joints_dirty) upload_joints(c);
+  record(c, 0);
+  zero_u32(c, d.counters, CTR_COUNT);
+  if (c->sleeping) wake_joint_motion(c);
+  k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
+  uint32_t nfast = read_u32(c, d.counters + CTR_FAST);
+  build_grid(c);
+  if (nfast) {
+    k_toi_cast<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+    build_grid(c);
+  }
+  broadphase(c);
+  narrow_and_merge(c);
+  if (c->sleeping) {
+    wake_joint_motion(c);
+    k_external_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+  }
+  k_inertial<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.njoints) k_joint_prep<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
+  if (d.npairs) k_pair_prep<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  incidence_and_colour(c);
+  record(c, 1);
+  const uint32_t ncol = c->stats.colours;
+  float primal_ms = 0.0f, dual_ms = 0.0f;
+  for (uint32_t it = 0; it < c->prm.iters; ++it) {
+    for (uint32_t k = 0; k < ncol; ++k) {
+      uint32_t cnt = c->colour_cnt_h[k];
+      if (cnt) k_primal_wave<<<blocks(cnt, WAVES_PER_BLOCK), WAVE * WAVES_PER_BLOCK, 0, c->st>>>(
+          d, d.colour_list + c->colour_off_h[k], cnt);
+    }
+    if (c->timing && it == 0) record(c, 2);
+    if (d.npts) k_dual<<<blocks(d.npts), TPB, 0, c->st>>>(d);
+    if (d.njoints) k_dual_joint<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
+    if (c->timing && it == 0) {
+      record(c, 3);
+      HIPC(hipEventSynchronize(c->ev[3]));
+      float a = 0, b = 0;
+      HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+      HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+      primal_ms = a * c->prm.iters;
+      dual_ms = b * c->prm.iters;
+    }
+  }
+  record(c, 3);
+  k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
+  uint32_t ns = std::max(d.npairs, d.njoints);
+  if (ns) k_supported<<<blocks(ns), TPB, 0, c->st>>>(d);
+  k_ready_init<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.njoints) {
+    k_ready_joints<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
+    propagate(c, d.b.flag_b, 0);
+  }
+  k_sleep_timer<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.njoints) propagate(c, d.b.flag_a, 0);
+  k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
+  record(c, 4);
+  read_counters(c);
+  c->sleeping = c->counters_h[CTR_SLEEPING];
+  c->stats.sleeping = c->sleeping;
+  c->stats.breakdowns = c->counters_h[CTR_BREAKDOWN];
+  c->stats.toi_hits = c->counters_h[CTR_TOIHITS];
+  c->stats.pairs = d.npairs;
+  c->stats.points = d.npts;
+  c->stats.candidates = c->ncand;
+  if (c->timing) {
+    float setup = 0, fin = 0, all = 0;
+    HIPC(hipEventElapsedTime(&setup, c->ev[0], c->ev[1]));
+    HIPC(hipEventElapsedTime(&fin, c->ev[3], c->ev[4]));
+    HIPC(hipEventElapsedTime(&all, c->ev[0], c->ev[4]));
+    c->ms[0] = setup;
+    c->ms[1] = primal_ms;
+    c->ms[2] = dual_ms;
+    c->ms[3] = fin;
+    c->ms[4] = all;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void oavbd_default_params(oavbd_params* p) {
+  if (!p) return;
+  p->gravity[0] = 0.0f;
+  p->gravity[1] = -9.81f;
+  p->gravity[2] = 0.0f;
+  p->iters = 10;
+}
+
+int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_joints, int device, oavbd_ctx** out) {
+  if (!out || max_bodies == 0) return OAVBD_EINVAL;
+  *out = nullptr;
+  oavbd_ctx* c = new (std::nothrow) oavbd_ctx();
+  if (!c) return OAVBD_ENOMEM;
+  if (params)
+    c->prm = *params;
+  else
+    oavbd_default_params(&c->prm);
+  c->max_bodies = max_bodies;
+  c->max_joints = max_joints;
+  c->device = device;
+  try {
+    HIPC(hipSetDevice(device));
+    HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    HIPC(hipHostMalloc((void**)&c->counters_h, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+    alloc_bodies(c, max_bodies);
+  } catch (const HipError& e) {
+    std::string m = e.msg;
+    oavbd_destroy(c);
+    std::fprintf(stderr, "oavbd_create: %s\n", m.c_str());
+    return OAVBD_EDEVICE;
+  }
+  for (int k = 0; k < 3; ++k) c->d.gravity[k] = c->prm.gravity[k];
+  *out = c;
+  return OAVBD_OK;
+}
+
+void oavbd_destroy(oavbd_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  free_all(c);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->counters_h) (void)hipHostFree(c->counters_h);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float* sp3, const float* pos3,
+                     const float* quat4, const float* vel3, const float* angvel3, const float* mass,
+                     const float* friction, const float* restitution, const int32_t* body_type,
+                     const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger, const float* rolling,
+                     const float* torsion, const uint32_t* scene) {
+  if (!c) return OAVBD_EINVAL;
+  if (n > c->max_bodies) return fail(c, OAVBD_ECAPACITY, "more bodies than max_bodies");
+  if (!shape || !sp3 || !pos3 || !quat4 || !vel3 || !angvel3 || !mass || !friction || !restitution || !body_type)
+    return fail(c, OAVBD_EINVAL, "null body array");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (shape[i] < 0 || shape[i] > 2) return fail(c, OAVBD_EINVAL, "bad shape kind");
+    if (body_type[i] < 0 || body_type[i] > 2) return fail(c, OAVBD_EINVAL, "bad body type");
+    if (shape[i] == SH_HALFSPACE && body_type[i] == BT_DYNAMIC && mass[i] > 0.0f)
+      return fail(c, OAVBD_EINVAL, "dynamic half-space");
+  }
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    Dev& d = c->d;
+    d.n = n;
+    std::vector<float4> pos(n), rot(n), vel(n), ang(n), zero(n), inert(n), shp4(n), mat(n);
+    std::vector<uint32_t> lay(n), msk(n), scn(n), prio(n), loc(n);
+    std::vector<uint8_t> trig(n), isl(n, 0);
+    c->kind_h.assign(shape, shape + n);
+    c->type_h.assign(n, 0);
+    std::vector<float> brad(n);
+    std::vector<uint32_t> per_scene_count;
+    for (uint32_t i = 0; i < n; ++i) {
+      int t = body_type[i];
+      float m = mass[i];
+      if (t == BT_DYNAMIC && !(m > 0.0f)) t = BT_STATIC;  // RigidBody::build
+      c->type_h[i] = t;
+      f3 p{sp3[3 * i], sp3[3 * i + 1], sp3[3 * i + 2]};
+      float inv = 0.0f, mm = 0.0f;
+      f3 I = zero3();
+      if (t == BT_DYNAMIC) {
+        mm = m;
+        inv = 1.0f / m;
+        if (shape[i] == SH_SPHERE) {
+          float v = 0.4f * m * p.x * p.x;
+          I = f3{v, v, v};
+        } else if (shape[i] == SH_BOX) {
+          f3 s2 = mul(p, 2.0f);
+          float x = s2.x * s2.x, y = s2.y * s2.y, z = s2.z * s2.z;
+          float k = m / 12.0f;
+          I = f3{k * (y + z), k * (z + x), k * (x + y)};
+        }
+      }
+      float br = shape[i] == SH_SPHERE ? p.x : (shape[i] == SH_BOX ? sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) : INFINITY);
+      brad[i] = br;
+      pos[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], inv);
+      rot[i] = make_float4(quat4[4 * i], quat4[4 * i + 1], quat4[4 * i + 2], quat4[4 * i + 3]);
+      vel[i] = make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.0f);
+      ang[i] = make_float4(angvel3[3 * i], angvel3[3 * i + 1], angvel3[3 * i + 2], 0.0f);
+      zero[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      inert[i] = make_float4(I.x, I.y, I.z, mm);
+      shp4[i] = make_float4(p.x, p.y, p.z, br);
+      mat[i] = make_float4(friction[i], restitution[i], rolling ? rolling[i] : 0.0f, torsion ? torsion[i] : 0.0f);
+      lay[i] = layer ? layer[i] : 1u;
+      msk[i] = mask ? mask[i] : 0xffffffffu;
+      scn[i] = scene ? scene[i] : 0u;
+      trig[i] = trigger ? trigger[i] : 0;
+      if (scn[i] >= per_scene_count.size()) per_scene_count.resize(scn[i] + 1, 0);
+      loc[i] = per_scene_count[scn[i]]++;
+      prio[i] = prio_hash(loc[i]);
+    }
+    // grid geometry: bodies with unbounded or large spheres (over 4x the
+    // median bound) are tested against everything instead of gridded.
+    std::vector<float> fin;
+    for (float r : brad)
+      if (std::isfinite(r)) fin.push_back(r);
+    float cut = 1.0f;
+    if (!fin.empty()) {
+      std::vector<float> s = fin;
+      std::nth_element(s.begin(), s.begin() + s.size() / 2, s.end());
+      cut = std::max(4.0f * s[s.size() / 2], 1e-3f);
+    }
+    float rmax = 0.0f;
+    for (float r : fin)
+      if (r <= cut) rmax = std::max(rmax, r);
+    std::vector<uint32_t> large;
+    for (uint32_t i = 0; i < n; ++i)
+      if (!std::isfinite(brad[i]) || brad[i] > cut) {
+        large.push_back(i);
+        isl[i] = 1;
+      }
+    d.g.cell = (2.0f * rmax + 0.05f) * 1.01f + 1e-6f;
+    d.g.nlarge = (uint32_t)large.size();
+    dfree(d.g.large);
+    dalloc(&d.g.large, large.size());
+    if (!large.empty()) HIPC(hipMemcpy(d.g.large, large.data(), large.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    dfree(d.lflag);
+    dfree(d.lrank);
+    dalloc(&d.lflag, (size_t)d.g.nlarge * n);
+    dalloc(&d.lrank, (size_t)d.g.nlarge * (n + 1));
+    auto up4 = [&](float4* dst, const std::vector<float4>& v) {
+      HIPC(hipMemcpy(dst, v.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    };
+    up4(d.b.pos, pos); up4(d.b.rot, rot); up4(d.b.vel, vel); up4(d.b.angvel, ang); up4(d.b.torque, zero);
+    up4(d.b.inertia, inert); up4(d.b.shape, shp4); up4(d.b.mat, mat); up4(d.b.prev_pos, pos); up4(d.b.prev_rot, rot);
+    up4(d.b.pos0, pos); up4(d.b.rot0, rot); up4(d.b.drot, zero);
+    HIPC(hipMemcpy(d.b.kind, shape, n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.type, c->type_h.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.layer, lay.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.mask, msk.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.scene, scn.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.prio, prio.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.local, loc.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.b.trigger, trig.data(), n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.g.is_large, isl.data(), n, hipMemcpyHostToDevice));
+    HIPC(hipMemset(d.b.asleep, 0, n));
+    HIPC(hipMemset(d.b.timer, 0, n * sizeof(float)));
+    HIPC(hipMemset(d.b.flag_a, 0, n));
+    HIPC(hipMemset(d.b.flag_b, 0, n));
+    HIPC(hipMemset(d.b.colour, 0xff, n * sizeof(int32_t)));
+    d.npairs = d.npts = d.npairs_old = d.npts_old = 0;
+    c->joints.clear();
+    upload_joints(c);
+    c->sleeping = 0;
+    c->time_debt = 0.0f;
+    c->stats = oavbd_stats{};
+    ensure_pairs(c, 1024);
+    ensure_points(c, 4096);
+    ensure_incidence(c, 1024, 4096);
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                         uint32_t* out_index) {
+  if (!c || !la || !lb) return OAVBD_EINVAL;
+  if (a == b || a >= c->d.n || b >= c->d.n) return fail(c, OAVBD_EINVAL, "bad joint bodies");
+  if (c->joints.size() >= c->max_joints) return fail(c, OAVBD_ECAPACITY, "more joints than max_joints");
+  HostJoint j{a, b, {la[0], la[1], la[2]}, {lb[0], lb[1], lb[2]}};
+  c->joints.push_back(j);
+  c->joints_dirty = true;
+  // the reference wakes both members on add_joint (mod.rs:1317-1318)
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    uint8_t one = 1;
+    HIPC(hipMemcpy(c->d.b.flag_a + a, &one, 1, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d.b.flag_a + b, &one, 1, hipMemcpyHostToDevice));
+    k_apply_wake<<<blocks(c->d.n), TPB, 0, c->st>>>(c->d);
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  if (out_index) *out_index = (uint32_t)c->joints.size() - 1;
+  return OAVBD_OK;
+}
+
+int oavbd_step_n(oavbd_ctx* c, uint32_t nsteps) {
+  if (!c) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    for (uint32_t s = 0; s < nsteps; ++s) step_inner(c);
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_step(oavbd_ctx* c, float dt) {
+  if (!c) return OAVBD_EINVAL;
+  if (!std::isfinite(dt) || dt <= 0.0f) return OAVBD_OK;
+  const float DT = 1.0f / 60.0f;
+  c->time_debt = fmin_(c->time_debt + dt, DT * 4.0f);
+  uint32_t k = 0;
+  while (c->time_debt >= DT && k < 4) {
+    c->time_debt -= DT;
+    ++k;
+  }
+  return k ? oavbd_step_n(c, k) : OAVBD_OK;
+}
+
+int oavbd_sync(oavbd_ctx* c) {
+  if (!c) return OAVBD_EINVAL;
+  GUARD(c, HIPC(hipStreamSynchronize(c->st)));
+  return OAVBD_OK;
+}
+
+int oavbd_get_state(oavbd_ctx* c, float* out13, uint32_t n) {
+  if (!c || !out13 || n > c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    std::vector<float4> p(n), q(n), v(n), w(n);
+    HIPC(hipMemcpyAsync(p.data(), c->d.b.pos, n * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(q.data(), c->d.b.rot, n * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(v.data(), c->d.b.vel, n * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(w.data(), c->d.b.angvel, n * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    for (uint32_t i = 0; i < n; ++i) {
+      float* o = out13 + 13 * (size_t)i;
+      o[0] = p[i].x; o[1] = p[i].y; o[2] = p[i].z;
+      o[3] = q[i].x; o[4] = q[i].y; o[5] = q[i].z; o[6] = q[i].w;
+      o[7] = v[i].x; o[8] = v[i].y; o[9] = v[i].z;
+      o[10] = w[i].x; o[11] = w[i].y; o[12] = w[i].z;
+    }
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_set_state(oavbd_ctx* c, const float* in13, uint32_t n) {
+  if (!c || !in13 || n > c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    std::vector<float4> p(n), q(n), v(n), w(n);
+    HIPC(hipMemcpy(p.data(), c->d.b.pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      const float* s = in13 + 13 * (size_t)i;
+      p[i] = make_float4(s[0], s[1], s[2], p[i].w);
+      q[i] = make_float4(s[3], s[4], s[5], s[6]);
+      v[i] = make_float4(s[7], s[8], s[9], 0.0f);
+      w[i] = make_float4(s[10], s[11], s[12], 0.0f);
+    }
+    HIPC(hipMemcpy(c->d.b.pos, p.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d.b.rot, q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d.b.vel, v.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d.b.angvel, w.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_pairs(oavbd_ctx* c, uint32_t* ab, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return OAVBD_EINVAL;
+  *n = c->d.npairs;
+  if (!ab) return OAVBD_OK;
+  uint32_t m = std::min(cap, c->d.npairs);
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    if (m) HIPC(hipMemcpy(ab, c->d.P.ab, m * sizeof(uint2), hipMemcpyDeviceToHost));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_points(oavbd_ctx* c, float* out16, uint32_t cap, uint32_t* n) {
+  if (!c || !n) return OAVBD_EINVAL;
+  *n = c->d.npts;
+  if (!out16) return OAVBD_OK;
+  uint32_t m = std::min(cap, c->d.npts);
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    std::vector<float4> ra(m), rb(m), lam(m), pen(m), roll(m);
+    if (m) {
+      HIPC(hipMemcpy(ra.data(), c->d.Q.ra, m * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(rb.data(), c->d.Q.rb, m * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(lam.data(), c->d.Q.lam, m * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(pen.data(), c->d.Q.pen, m * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(roll.data(), c->d.Q.roll, m * sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < m; ++i) {
+      float* o = out16 + 16 * (size_t)i;
+      o[0] = ra[i].x; o[1] = ra[i].y; o[2] = ra[i].z;
+      o[3] = rb[i].x; o[4] = rb[i].y; o[5] = rb[i].z;
+      o[6] = lam[i].x; o[7] = lam[i].y; o[8] = lam[i].z;
+      o[9] = pen[i].x; o[10] = pen[i].y; o[11] = pen[i].z;
+      o[12] = roll[i].x; o[13] = roll[i].y; o[14] = roll[i].z;
+      o[15] = rb[i].w;
+    }
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_colours(oavbd_ctx* c, int32_t* colour, uint32_t n) {
+  if (!c || !colour || n > c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipMemcpy(colour, c->d.b.colour, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_sleep(oavbd_ctx* c, uint8_t* asleep, float* timer, uint32_t n) {
+  if (!c || n > c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    if (asleep) HIPC(hipMemcpy(asleep, c->d.b.asleep, n, hipMemcpyDeviceToHost));
+    if (timer) HIPC(hipMemcpy(timer, c->d.b.timer, n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_joints(oavbd_ctx* c, float* out6, uint32_t n) {
+  if (!c || !out6 || n > c->d.njoints) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    std::vector<float4> lam(n), pen(n);
+    if (n) {
+      HIPC(hipMemcpy(lam.data(), c->d.J.lam, n * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(pen.data(), c->d.J.pen, n * sizeof(float4), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      out6[6 * i] = lam[i].x; out6[6 * i + 1] = lam[i].y; out6[6 * i + 2] = lam[i].z;
+      out6[6 * i + 3] = pen[i].x; out6[6 * i + 4] = pen[i].y; out6[6 * i + 5] = pen[i].z;
+    }
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
+  if (!c || !out) return OAVBD_EINVAL;
+  *out = c->stats;
+  return OAVBD_OK;
+}
+
+int oavbd_set_timing(oavbd_ctx* c, int enable) {
+  if (!c) return OAVBD_EINVAL;
+  c->timing = enable != 0;
+  return OAVBD_OK;
+}
+
+int oavbd_get_timing(oavbd_ctx* c, float* ms5) {
+  if (!c || !ms5) return OAVBD_EINVAL;
+  for (int k = 0; k < 5; ++k) ms5[k] = c->ms[k];
+  return OAVBD_OK;
+}
+
+const char* oavbd_last_error(oavbd_ctx* c) {
+  if (!c) return "null context";
+  return c->err.c_str();
+}
+
+}  // extern "C"

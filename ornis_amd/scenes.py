@@ -171,3 +171,81 @@ def box_drop_lattice(nx=50, nz=50, ny=40, spacing=2.0, jitter=0.1, hmin=0.25, hm
         "joints": [],
     }
     return concat([statics, dyn])
+
+
+def chains(n_chains=2000, links=32, grid=45, spacing=None, height=20.0, half=(0.05, 0.2, 0.05),
+           n_spheres=5000, rmin=0.2, rmax=1.0, density=1000.0, friction=0.5, seed=0xA5BD2026):
+    """Chains of 0.1 x 0.4 x 0.1 m boxes hanging along -y, consecutive links
+    joined by ball joints at their shared end, roots on a grid at `height`,
+    falling onto a plane scattered with static spheres (64,000 bodies and
+    62,000 joints at the defaults)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 3))
+    hx, hy, hz = half
+    link_len = 2.0 * hy
+    if spacing is None:
+        spacing = 2.0 * link_len
+    extent = spacing * (grid - 1)
+    sb = SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=friction, restitution=0.0)
+    mass = float(np.float32(density) * np.float32(8.0 * hx * hy * hz))
+    for c in range(n_chains):
+        gx, gz = c % grid, c // grid
+        x0 = -0.5 * extent + spacing * gx
+        z0 = -0.5 * extent + spacing * gz
+        prev = None
+        for k in range(links):
+            y = height - k * link_len
+            h = sb.box((x0, y, z0), half, mass, friction=friction, restitution=0.0)
+            if prev is not None:
+                sb.ball_joint(prev, h, (0.0, -hy, 0.0), (0.0, hy, 0.0))
+            prev = h
+    base = sb.build()
+    if n_spheres:
+        r = rng.uniform(rmin, rmax, size=n_spheres).astype(np.float32)
+        span = 0.5 * extent + 2.0
+        px = rng.uniform(-span, span, size=n_spheres)
+        pz = rng.uniform(-span, span, size=n_spheres)
+        sp = SceneBuilder()
+        for i in range(n_spheres):
+            sp.sphere((float(px[i]), float(r[i]), float(pz[i])), float(r[i]), 0.0, friction=friction,
+                      restitution=0.0)
+        return concat([base, sp.build()])
+    return base
+
+
+def box_drop_scene(n, rng, half_lo=0.2, half_hi=0.6, mu=(0.2, 0.9), floor=10.0, density=1000.0, height=1.0):
+    """One small scene: n random boxes dropped onto a static floor box
+    `floor` metres wide (used in batches)."""
+    sb = SceneBuilder()
+    sb.box((0.0, -0.5, 0.0), (floor * 0.5, 0.5, floor * 0.5), 0.0, friction=float(mu[1]), restitution=0.0)
+    statics = sb.build()
+    side = max(1, int(math.ceil(n ** (1.0 / 3.0))))
+    pitch = 2.0 * half_hi + 0.1
+    idx = np.arange(n)
+    ix, iz, iy = idx % side, (idx // side) % side, idx // (side * side)
+    pos = np.stack([(ix - (side - 1) * 0.5) * pitch, height + half_hi + iy * pitch,
+                    (iz - (side - 1) * 0.5) * pitch], axis=1)
+    half = rng.uniform(half_lo, half_hi, size=(n, 3)).astype(np.float32)
+    fr = rng.uniform(mu[0], mu[1], size=n).astype(np.float32)
+    quat = _uniform_quats(rng, n)
+    mass = (np.float32(density) * np.float32(8.0) * half[:, 0] * half[:, 1] * half[:, 2]).astype(np.float32)
+    dyn = {
+        "gravity": statics["gravity"], "shape": np.full(n, SHAPE_BOX, np.int32), "shape_param": half,
+        "pos": pos.astype(np.float32), "quat": quat, "vel": np.zeros((n, 3), np.float32),
+        "angvel": np.zeros((n, 3), np.float32), "mass": mass, "friction": fr,
+        "restitution": np.zeros(n, np.float32), "body_type": np.full(n, BODY_DYNAMIC, np.int32),
+        "layer": np.ones(n, np.uint32), "mask": np.full(n, 0xFFFFFFFF, np.uint32),
+        "trigger": np.zeros(n, np.uint8), "rolling": np.zeros(n, np.float32),
+        "torsion": np.zeros(n, np.float32), "joints": []}
+    return concat([statics, dyn])
+
+
+def batched_drops(n_scenes=4096, nmin=8, nmax=512, seed=0xA5BD2026):
+    """Independent box-drop scenes with log-uniform body counts in
+    [nmin, nmax]; returns (list of scenes, concatenated scene with scene_id)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 4))
+    counts = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=n_scenes))).astype(int)
+    parts = [box_drop_scene(int(k), rng) for k in counts]
+    allsc = concat(parts)
+    allsc["scene_id"] = np.concatenate([np.full(len(p["shape"]), i, np.uint32) for i, p in enumerate(parts)])
+    return parts, allsc

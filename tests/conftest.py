@@ -1,0 +1,20 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built liboavbd.so")
+
+
+@pytest.fixture(scope="session")
+def gpu_available():
+    import torch  # noqa: F401  (device discovery only)
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return True
