@@ -127,6 +127,11 @@ int oavbd_get_sleep(oavbd_ctx* ctx, uint8_t* asleep, float* timer, uint32_t n);
  * stuck; *n receives the point count. */
 int oavbd_get_points(oavbd_ctx* ctx, float* out16, uint32_t cap, uint32_t* n);
 
+/* TOI clamps of the last step per body: distance travelled to the hit
+ * (t < 0: no clamp) and the body hit (the data of the reference's CCD
+ * ContactEventKind::Hit, crates/physics/src/avbd/mod.rs:919-937). */
+int oavbd_get_toi(oavbd_ctx* ctx, float* t, int32_t* target, uint32_t n);
+
 /* Ball-joint duals: 6 floats each (lam3, pen3). */
 int oavbd_get_joints(oavbd_ctx* ctx, float* out6, uint32_t n);
 

@@ -569,6 +569,9 @@ struct Engine {
   Stats stats{};
   // per-step incidence (pair indices in vector order; joint indices)
   std::vector<std::vector<int>> inc_pairs, inc_joints;
+  // last step's TOI clamps: distance travelled to the hit (-1: none), target
+  std::vector<T> toi_t;
+  std::vector<int> toi_tgt;
 
   bool solvable(int h) const { return bodies[h].type == BT_DYNAMIC && bodies[h].inv_mass > T(0); }
   T eff_inv_mass(int h) const { return bodies[h].type == BT_DYNAMIC ? bodies[h].inv_mass : T(0); }
@@ -852,11 +855,15 @@ struct Engine {
     wake_body(hit.handle);
     sleep_timer[h] = T(0);
     stats.toi_hits++;
+    toi_t[h] = hit.t;
+    toi_tgt[h] = hit.handle;
   }
 
   // mod.rs:882-978.
   void toi_clamp() {
     int n = (int)bodies.size();
+    toi_t.assign(n, T(-1));
+    toi_tgt.assign(n, -1);
     bool use_grid = !brute && n > 256;
     Grid g;
     std::vector<int> large, cand;
@@ -1983,6 +1990,17 @@ void orc_get_stats(void* h, int32_t* out) {
   DISPATCH(c, s = E->stats);
   out[0] = s.pairs; out[1] = s.points; out[2] = s.colours; out[3] = s.max_colour;
   out[4] = s.sleeping; out[5] = s.breakdowns; out[6] = s.toi_hits;
+}
+
+// Last step's TOI clamps per body: t (-1 = none) and target index.
+void orc_get_toi(void* h, float* t, int32_t* target) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      t[i] = i < E->toi_t.size() ? (float)E->toi_t[i] : -1.0f;
+      target[i] = i < E->toi_tgt.size() ? E->toi_tgt[i] : -1;
+    }
+  });
 }
 
 // Unit hooks for analytic pins.

@@ -63,6 +63,7 @@ def lib():
         L.orc_get_sleep.argtypes = [ctypes.c_void_p, _u8p, _f32p]
         L.orc_get_joints.argtypes = [ctypes.c_void_p, _f32p]
         L.orc_get_stats.argtypes = [ctypes.c_void_p, _i32p]
+        L.orc_get_toi.argtypes = [ctypes.c_void_p, _f32p, _i32p]
         L.orc_solve_6x6_f32.restype = ctypes.c_int
         L.orc_solve_6x6_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_stamp_row_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float]
@@ -169,6 +170,12 @@ class OracleEngine:
         out = np.zeros((nj, 6), np.float32)
         self._L.orc_get_joints(self.h, _p(out, _f32p))
         return out
+
+    def toi(self):
+        t = np.zeros(self.n, np.float32)
+        tg = np.zeros(self.n, np.int32)
+        self._L.orc_get_toi(self.h, _p(t, _f32p), _p(tg, _i32p))
+        return t, tg
 
     def stats(self):
         out = np.zeros(7, np.int32)

@@ -56,6 +56,7 @@ SIGNATURES = {
     "oavbd_get_points": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32, _u32p]),
     "oavbd_get_colours": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_get_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
+    "oavbd_get_toi": (ctypes.c_int, [ctypes.c_void_p, _f32p, _i32p, ctypes.c_uint32]),
     "oavbd_get_joints": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
     "oavbd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdStats)]),
     "oavbd_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -207,6 +208,13 @@ class AvbdEngine:
         t = np.zeros(self.n, np.float32)
         self._check(self._L.oavbd_get_sleep(self._ctx, _p(a, _u8p), _p(t, _f32p), self.n))
         return a, t
+
+    def toi(self):
+        """Last step's TOI clamps: (t per body, -1 = none; target body)."""
+        t = np.zeros(self.n, np.float32)
+        tg = np.zeros(self.n, np.int32)
+        self._check(self._L.oavbd_get_toi(self._ctx, _p(t, _f32p), _p(tg, _i32p), self.n))
+        return t, tg
 
     def joints(self):
         out = np.zeros((self.njoints, 6), np.float32)

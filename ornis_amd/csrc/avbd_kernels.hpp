@@ -208,7 +208,7 @@ __device__ __forceinline__ bool sweep_may_touch(f3 p0, f3 dir, float len, float 
 __global__ void k_toi_cast(Dev d) {
   uint32_t h = gid();
   if (h >= d.n) return;
-  d.toi_hit[h] = make_float4(-1.0f, 0.0f, 0.0f, 0.0f);
+  d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
   if (!d.fast[h]) return;
   f3 delta = mul(xyz(d.b.vel[h]), K_DT);
   float len = len3(delta);
@@ -256,6 +256,11 @@ __global__ void k_toi_cast(Dev d) {
     d.toi_hit[h] = pk(best_n, best_t);
     d.toi_tgt[h] = best_o;
   }
+}
+
+__global__ void k_toi_clear(Dev d) {
+  uint32_t h = gid();
+  if (h < d.n) d.toi_hit[h] = make_float4(-1.0f, 0.0f, 0.0f, -1.0f);
 }
 
 __global__ void k_toi_apply(Dev d) {

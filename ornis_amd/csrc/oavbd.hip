@@ -449,6 +449,8 @@ static void step_inner(oavbd_ctx* c) {
     k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
     k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
     build_grid(c);
+  } else {
+    k_toi_clear<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   broadphase(c);
   narrow_and_merge(c);
@@ -843,6 +845,25 @@ int oavbd_get_sleep(oavbd_ctx* c, uint8_t* asleep, float* timer, uint32_t n) {
     HIPC(hipSetDevice(c->device));
     if (asleep) HIPC(hipMemcpy(asleep, c->d.b.asleep, n, hipMemcpyDeviceToHost));
     if (timer) HIPC(hipMemcpy(timer, c->d.b.timer, n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_toi(oavbd_ctx* c, float* t, int32_t* target, uint32_t n) {
+  if (!c || !t || !target || n > c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    std::vector<float4> hit(n);
+    std::vector<uint32_t> tg(n);
+    if (n) {
+      HIPC(hipMemcpy(hit.data(), c->d.toi_hit, n * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(tg.data(), c->d.toi_tgt, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+      bool hitp = hit[i].w >= 0.0f;
+      t[i] = hitp ? hit[i].w : -1.0f;
+      target[i] = hitp ? (int32_t)tg[i] : -1;
+    }
   });
   return OAVBD_OK;
 }

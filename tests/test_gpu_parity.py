@@ -32,6 +32,10 @@ def assert_same(g, o, step):
     assert pg.shape == po_.shape and np.array_equal(pg, po_), f"pair list differs at step {step}"
     cg, co = g.colours(), o.colours()
     assert np.array_equal(cg, co), f"colouring differs at step {step}"
+    tg, ttg = g.toi()
+    to, tto = o.toi()
+    assert np.array_equal(tg.view(np.uint32), to.view(np.uint32)) and np.array_equal(ttg, tto), \
+        f"TOI clamps differ at step {step}"
     sg, so = g.state(), o.state()
     bad = np.where((sg.view(np.uint32) != so.view(np.uint32)).any(axis=1))[0]
     assert bad.size == 0, f"step {step}: {bad.size} bodies differ, first {bad[:5]}, " \
