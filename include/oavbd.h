@@ -63,7 +63,31 @@ typedef struct oavbd_stats {
   uint32_t breakdowns;  /* 6x6 LDL breakdowns (body kept its pose)           */
   uint32_t toi_hits;    /* TOI clamps applied                                */
   uint32_t colour_rounds; /* Jones-Plassmann rounds of the colouring         */
+  uint32_t solved;      /* bodies swept by the primal solve                  */
+  uint32_t point_sides; /* contact point-sides of swept bodies               */
+  uint32_t pair_sides;  /* contact pair-sides (with points) of swept bodies  */
+  uint32_t joint_sides; /* joint sides of swept bodies                       */
 } oavbd_stats;
+
+/* Kernel time and algorithmic bytes accumulated since the last reset
+ * while timing is enabled (oavbd_set_timing).  Every primal-solve and
+ * dual-update launch is bracketed by HIP events on the context stream.
+ * Algorithmic bytes count each datum a launch must read or write once, at
+ * its stored size:
+ *   primal, per iteration: 168 B per swept body + 72 B per point-side +
+ *     144 B per pair-side + 168 B per joint-side;
+ *   dual, per iteration: 116 B per point + 208 B per pair + 264 B per joint. */
+typedef struct oavbd_kernel_times {
+  double primal_ms;      /* summed primal launch durations                 */
+  double dual_ms;        /* summed dual launch durations                   */
+  double primal_bytes;   /* summed algorithmic bytes of those launches     */
+  double dual_bytes;
+  uint64_t primal_launches;
+  uint64_t dual_launches;
+  uint64_t steps;
+} oavbd_kernel_times;
+
+int oavbd_get_kernel_times(oavbd_ctx* ctx, oavbd_kernel_times* out, int reset);
 
 void oavbd_default_params(oavbd_params* p);
 

@@ -35,7 +35,13 @@ class OavbdParams(ctypes.Structure):
 class OavbdStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in (
         "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
-        "colour_rounds")]
+        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides")]
+
+
+class OavbdKernelTimes(ctypes.Structure):
+    _fields_ = [("primal_ms", ctypes.c_double), ("dual_ms", ctypes.c_double), ("primal_bytes", ctypes.c_double),
+                ("dual_bytes", ctypes.c_double), ("primal_launches", ctypes.c_uint64),
+                ("dual_launches", ctypes.c_uint64), ("steps", ctypes.c_uint64)]
 
 
 # Every entry point of include/oavbd.h with its ctypes signature.
@@ -60,6 +66,7 @@ SIGNATURES = {
     "oavbd_get_joints": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
     "oavbd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdStats)]),
     "oavbd_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "oavbd_get_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdKernelTimes), ctypes.c_int]),
     "oavbd_get_timing": (ctypes.c_int, [ctypes.c_void_p, _f32p]),
     "oavbd_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
@@ -233,6 +240,11 @@ class AvbdEngine:
         out = np.zeros(5, np.float32)
         self._check(self._L.oavbd_get_timing(self._ctx, _p(out, _f32p)))
         return dict(zip(["setup", "primal", "dual", "finish", "step"], out.tolist()))
+
+    def kernel_times(self, reset=False):
+        k = OavbdKernelTimes()
+        self._check(self._L.oavbd_get_kernel_times(self._ctx, ctypes.byref(k), 1 if reset else 0))
+        return {f: getattr(k, f) for f, _ in OavbdKernelTimes._fields_}
 
     @property
     def sleeping_count(self):

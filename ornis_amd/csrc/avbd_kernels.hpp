@@ -921,6 +921,14 @@ __global__ void k_colour_fill(Dev d) {
   int c = v < 0 ? -1 : (v & 0xffff);
   d.b.colour[h] = c;
   if (c < 0) return;
+  // sizes of the swept work (algorithmic-byte accounting)
+  uint32_t ps = 0;
+  for (uint32_t i = d.boff[h]; i < d.boff[h + 1]; ++i) ps += d.P.pts[d.blist[i]].y > 0u ? 1u : 0u;
+  for (uint32_t p = d.acc_off[d.cand_off[h]]; p < d.acc_off[d.cand_off[h + 1]]; ++p) ps += d.P.pts[p].y > 0u ? 1u : 0u;
+  atomicAdd(&d.counters[CTR_SOLVED], 1u);
+  atomicAdd(&d.counters[CTR_PTSIDES], d.inc_off[h + 1] - d.inc_off[h]);
+  atomicAdd(&d.counters[CTR_PAIRSIDES], ps);
+  atomicAdd(&d.counters[CTR_JSIDES], d.b.ji_off[h + 1] - d.b.ji_off[h]);
   uint32_t s = atomicAdd(&d.colour_cur[c], 1u);
   d.colour_list[d.colour_off[c] + s] = h;
 }

@@ -177,6 +177,9 @@ enum {
   CTR_SLEEPING = 3,
   CTR_BREAKDOWN = 4,
   CTR_TOIHITS = 5,
-  CTR_ANYASLEEP = 6,
-  CTR_COUNT = 8
+  CTR_SOLVED = 6,
+  CTR_PTSIDES = 7,
+  CTR_PAIRSIDES = 8,
+  CTR_JSIDES = 9,
+  CTR_COUNT = 12
 };

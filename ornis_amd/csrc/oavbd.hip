@@ -83,6 +83,8 @@ struct oavbd_ctx {
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   bool timing = false;
+  std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
+  oavbd_kernel_times kt{};
   hipEvent_t ev[5] = {};
   float ms[5] = {};
   std::string err;
@@ -464,25 +466,32 @@ static void step_inner(oavbd_ctx* c) {
   incidence_and_colour(c);
   record(c, 1);
   const uint32_t ncol = c->stats.colours;
-  float primal_ms = 0.0f, dual_ms = 0.0f;
+  // In timing mode every primal / dual launch is bracketed by pool events;
+  // they are resolved after the step's final synchronisation.
+  size_t np_ev = 0, nd_ev = 0;
+  auto pool_event = [&](std::vector<hipEvent_t>& pool, size_t i) {
+    while (pool.size() <= i) {
+      hipEvent_t e;
+      HIPC(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[i];
+  };
   for (uint32_t it = 0; it < c->prm.iters; ++it) {
     for (uint32_t k = 0; k < ncol; ++k) {
       uint32_t cnt = c->colour_cnt_h[k];
-      if (cnt) k_primal_wave<<<blocks(cnt, WAVES_PER_BLOCK), WAVE * WAVES_PER_BLOCK, 0, c->st>>>(
+      if (!cnt) continue;
+      if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev), c->st));
+      k_primal_wave<<<blocks(cnt, WAVES_PER_BLOCK), WAVE * WAVES_PER_BLOCK, 0, c->st>>>(
           d, d.colour_list + c->colour_off_h[k], cnt);
+      if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
+      np_ev += 2;
     }
-    if (c->timing && it == 0) record(c, 2);
+    if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev), c->st));
     if (d.npts) k_dual<<<blocks(d.npts), TPB, 0, c->st>>>(d);
     if (d.njoints) k_dual_joint<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-    if (c->timing && it == 0) {
-      record(c, 3);
-      HIPC(hipEventSynchronize(c->ev[3]));
-      float a = 0, b = 0;
-      HIPC(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
-      HIPC(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
-      primal_ms = a * c->prm.iters;
-      dual_ms = b * c->prm.iters;
-    }
+    if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev + 1), c->st));
+    nd_ev += 2;
   }
   record(c, 3);
   k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
@@ -505,16 +514,43 @@ static void step_inner(oavbd_ctx* c) {
   c->stats.pairs = d.npairs;
   c->stats.points = d.npts;
   c->stats.candidates = c->ncand;
+  c->stats.solved = c->counters_h[CTR_SOLVED];
+  c->stats.point_sides = c->counters_h[CTR_PTSIDES];
+  c->stats.pair_sides = c->counters_h[CTR_PAIRSIDES];
+  c->stats.joint_sides = c->counters_h[CTR_JSIDES];
   if (c->timing) {
-    float setup = 0, fin = 0, all = 0;
+    float setup = 0, iter = 0, fin = 0, all = 0;
     HIPC(hipEventElapsedTime(&setup, c->ev[0], c->ev[1]));
+    HIPC(hipEventElapsedTime(&iter, c->ev[1], c->ev[3]));
     HIPC(hipEventElapsedTime(&fin, c->ev[3], c->ev[4]));
     HIPC(hipEventElapsedTime(&all, c->ev[0], c->ev[4]));
+    double pm = 0.0, dm = 0.0;
+    for (size_t i = 0; i < np_ev; i += 2) {
+      float t = 0;
+      HIPC(hipEventElapsedTime(&t, c->pev[i], c->pev[i + 1]));
+      pm += t;
+    }
+    for (size_t i = 0; i < nd_ev; i += 2) {
+      float t = 0;
+      HIPC(hipEventElapsedTime(&t, c->dev_ev[i], c->dev_ev[i + 1]));
+      dm += t;
+    }
+    const oavbd_stats& s = c->stats;
+    double iters = c->prm.iters;
+    c->kt.primal_ms += pm;
+    c->kt.dual_ms += dm;
+    c->kt.primal_launches += np_ev / 2;
+    c->kt.dual_launches += nd_ev / 2;
+    c->kt.primal_bytes += iters * (168.0 * s.solved + 72.0 * s.point_sides + 144.0 * s.pair_sides +
+                                   168.0 * s.joint_sides);
+    c->kt.dual_bytes += iters * (116.0 * s.points + 208.0 * s.pairs + 264.0 * d.njoints);
+    c->kt.steps += 1;
     c->ms[0] = setup;
-    c->ms[1] = primal_ms;
-    c->ms[2] = dual_ms;
+    c->ms[1] = (float)pm;
+    c->ms[2] = (float)dm;
     c->ms[3] = fin;
     c->ms[4] = all;
+    (void)iter;
   }
 }
 
@@ -567,6 +603,8 @@ void oavbd_destroy(oavbd_ctx* c) {
   free_all(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->pev) (void)hipEventDestroy(e);
+  for (auto& e : c->dev_ev) (void)hipEventDestroy(e);
   if (c->counters_h) (void)hipHostFree(c->counters_h);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -888,6 +926,13 @@ int oavbd_get_joints(oavbd_ctx* c, float* out6, uint32_t n) {
 int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   if (!c || !out) return OAVBD_EINVAL;
   *out = c->stats;
+  return OAVBD_OK;
+}
+
+int oavbd_get_kernel_times(oavbd_ctx* c, oavbd_kernel_times* out, int reset) {
+  if (!c || !out) return OAVBD_EINVAL;
+  *out = c->kt;
+  if (reset) c->kt = oavbd_kernel_times{};
   return OAVBD_OK;
 }
 
