@@ -56,6 +56,10 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--progress", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0, help="primal lanes per body (0: library default)")
+    ap.add_argument("--waves", type=int, default=1, help="primal wavefronts per workgroup")
+    ap.add_argument("--sweep-shapes", type=int, default=0,
+                    help="after warmup, time this many steps per primal launch shape and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the 100k-box scene fits one MI355X and multi-GPU "
@@ -74,6 +78,19 @@ def main():
             print(f"warmup {s} {eng.stats()}", file=sys.stderr, flush=True)
     eng.sync()
     eng.set_timing(True)
+    if args.lanes:
+        eng.set_primal_shape(args.lanes, args.waves)
+    if args.sweep_shapes:
+        for shape in ((16, 1), (16, 4), (32, 1), (32, 2), (32, 4), (32, 8), (64, 1), (64, 4)):
+            eng.set_primal_shape(*shape)
+            eng.kernel_times(reset=True)
+            eng.step_n(args.sweep_shapes)
+            kt = eng.kernel_times()
+            print(json.dumps({"lanes": shape[0], "waves": shape[1],
+                              "primal_ms_per_step": kt["primal_ms"] / kt["steps"],
+                              "avg_launch_us": 1e3 * kt["primal_ms"] / kt["primal_launches"],
+                              "GBps": kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9}), flush=True)
+        return
     eng.kernel_times(reset=True)
     t0 = time.perf_counter()
     chunk = 10
@@ -114,7 +131,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_primal_wave",
+                     "kernel": "k_primal",
                      "bytes_per_launch": kt["primal_bytes"] / max(kt["primal_launches"], 1),
                      "avg_launch_us": 1e3 * kt["primal_ms"] / max(kt["primal_launches"], 1)},
         "cpu_baseline": cpu,

@@ -169,6 +169,12 @@ int oavbd_get_stats(oavbd_ctx* ctx, oavbd_stats* out);
 int oavbd_set_timing(oavbd_ctx* ctx, int enable);
 int oavbd_get_timing(oavbd_ctx* ctx, float* ms5);
 
+/* Launch shape of the primal solve: `lanes` lanes cooperate on one body
+ * (8, 16, 32 or 64) and a workgroup holds `waves` wavefronts (1, 2, 4 or
+ * 8).  Results are bitwise independent of the shape; only speed changes.
+ * Default 32 lanes, 1 wavefront. */
+int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
+
 /* Block until all work of the context has finished. */
 int oavbd_sync(oavbd_ctx* ctx);
 

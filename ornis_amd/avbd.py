@@ -68,6 +68,7 @@ SIGNATURES = {
     "oavbd_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "oavbd_get_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdKernelTimes), ctypes.c_int]),
     "oavbd_get_timing": (ctypes.c_int, [ctypes.c_void_p, _f32p]),
+    "oavbd_set_primal_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
@@ -232,6 +233,11 @@ class AvbdEngine:
         s = OavbdStats()
         self._check(self._L.oavbd_get_stats(self._ctx, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in OavbdStats._fields_}
+
+    def set_primal_shape(self, lanes=32, waves=4):
+        """Primal launch shape: lanes per body, wavefronts per workgroup.
+        Bitwise neutral."""
+        self._check(self._L.oavbd_set_primal_shape(self._ctx, int(lanes), int(waves)))
 
     def set_timing(self, on=True):
         self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))

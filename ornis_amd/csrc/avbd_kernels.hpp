@@ -4,7 +4,8 @@
 // mod.rs:846-1143), every phase on the device:
 //   k_begin            torque -> angular velocity, TOI mover flags
 //   k_grid_*           hash grid of the bodies' bounding spheres
-//   k_toi_cast/apply   linear TOI clamp against the step-top poses
+//   k_toi_*           linear TOI clamp against the step-top poses, one
+//                      thread per (mover, target) cast
 //   k_bp_*             candidate pairs, canonically ordered by (a, b)
 //   k_narrow           discover_pair: prefilters, signed gap, normal,
 //                      manifold points, wake vote (rows.rs:432-756)
@@ -12,7 +13,7 @@
 //                      (rows.rs:795-931) fused with the warm-start decay
 //   k_inertial etc.    inertial targets and warm-started positions
 //   k_inc_* / k_jp_*   per-body incidence lists and the greedy colouring
-//   k_primal_wave      one wavefront per body: rows staged in LDS, summed
+//   k_primal<L>        L lanes per body: rows staged in LDS, summed
 //                      per matrix element in row order, 6x6 LDL^T in FP32
 //   k_dual / joints    dual / penalty update, one thread per point
 //   k_bdf1 / sleep     velocities, renormalisation, sleep bookkeeping
@@ -205,42 +206,34 @@ __device__ __forceinline__ bool sweep_may_touch(f3 p0, f3 dir, float len, float 
   return len3(c) <= reach * 1.001f + 1e-4f;
 }
 
-__global__ void k_toi_cast(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
-  d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
-  if (!d.fast[h]) return;
+// Candidate targets of fast mover h: grid members near its swept bounding
+// sphere, then the large bodies, after the target filters and a bounding-
+// sphere sweep test (a target outside it cannot come within the touch
+// distance, so it cannot produce a hit).  Order is irrelevant: the hit is
+// the minimum of (t, target index) over all targets.
+template <bool FILL>
+__device__ uint32_t toi_targets(const Dev& d, uint32_t h, uint2* out) {
   f3 delta = mul(xyz(d.b.vel[h]), K_DT);
   float len = len3(delta);
-  if (len < K_NEAR_ZERO) return;
+  if (len < K_NEAR_ZERO) return 0;
   f3 dir = divs(delta, len);
-  shp mover = body_shape(d, h);
+  f3 p0 = xyz(d.b.pos[h]);
   float rm = d.b.shape[h].w;
-  float best_t = 0.0f;
-  uint32_t best_o = 0xffffffffu;
-  f3 best_n = zero3();
+  uint32_t m = 0;
   auto consider = [&](uint32_t o) {
     if (!toi_target_ok(d, h, o)) return;
     float ro = d.b.shape[o].w;
-    if (isfinite(ro) && !sweep_may_touch(mover.pos, dir, len, rm, xyz(d.b.pos[o]), ro)) return;
-    shp target = body_shape(d, o);
-    f3 nrm;
-    float t = cast_one(d, mover, dir, len, target, &nrm);
-    if (t < 0.0f) return;
-    if (best_o == 0xffffffffu || t < best_t || (t == best_t && o < best_o)) {
-      best_t = t;
-      best_o = o;
-      best_n = nrm;
-    }
+    if (isfinite(ro) && !sweep_may_touch(p0, dir, len, rm, xyz(d.b.pos[o]), ro)) return;
+    if (FILL) out[m] = make_uint2(h, o);
+    ++m;
   };
-  // grid members near the swept bounding sphere
   float r = rm + d.g.cell * 0.5f + 0.01f;
-  f3 p0 = mover.pos, p1 = add(mover.pos, delta);
+  f3 p1 = add(p0, delta);
   int lx = cell_of(fminf(p0.x, p1.x) - r, d.g.cell), hx = cell_of(fmaxf(p0.x, p1.x) + r, d.g.cell);
   int ly = cell_of(fminf(p0.y, p1.y) - r, d.g.cell), hy = cell_of(fmaxf(p0.y, p1.y) + r, d.g.cell);
   int lz = cell_of(fminf(p0.z, p1.z) - r, d.g.cell), hz = cell_of(fmaxf(p0.z, p1.z) + r, d.g.cell);
   long span = (long)(hx - lx + 1) * (long)(hy - ly + 1) * (long)(hz - lz + 1);
-  if (span <= 64) {
+  if (span > 0 && span <= 64) {
     uint32_t bks[64];
     int nbk = gather_buckets(d, lx, ly, lz, hx, hy, hz, bks, 64);
     for (int i = 0; i < nbk; ++i) {
@@ -252,10 +245,54 @@ __global__ void k_toi_cast(Dev d) {
       if (!d.g.is_large[o]) consider(o);
   }
   for (uint32_t k = 0; k < d.g.nlarge; ++k) consider(d.g.large[k]);
-  if (best_o != 0xffffffffu) {
-    d.toi_hit[h] = pk(best_n, best_t);
-    d.toi_tgt[h] = best_o;
-  }
+  return m;
+}
+
+__global__ void k_toi_count(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
+  d.toi_best[h] = ~0ull;
+  d.toi_cnt[h] = d.fast[h] ? toi_targets<false>(d, h, nullptr) : 0u;
+}
+
+__global__ void k_toi_fill(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n || !d.fast[h]) return;
+  toi_targets<true>(d, h, d.toi_pairs + d.toi_off[h]);
+}
+
+__device__ __forceinline__ float cast_pair(const Dev& d, uint32_t h, uint32_t o, f3* nrm) {
+  f3 delta = mul(xyz(d.b.vel[h]), K_DT);
+  float len = len3(delta);
+  f3 dir = divs(delta, len);
+  return cast_one(d, body_shape(d, h), dir, len, body_shape(d, o), nrm);
+}
+
+// One (mover, target) cast per thread; the mover keeps the smallest
+// (t, target) key: t > 0, so its IEEE bits order like the value.
+__global__ void __launch_bounds__(TPB) k_toi_pair(Dev d, uint32_t npairs) {
+  uint32_t i = gid();
+  if (i >= npairs) return;
+  uint2 ho = d.toi_pairs[i];
+  f3 nrm;
+  float t = cast_pair(d, ho.x, ho.y, &nrm);
+  if (t < 0.0f) return;
+  unsigned long long key = ((unsigned long long)__float_as_uint(t) << 32) | ho.y;
+  atomicMin(&d.toi_best[ho.x], key);
+}
+
+// The winning cast again, for its contact normal (same inputs, same bits).
+__global__ void k_toi_finalize(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  unsigned long long key = d.toi_best[h];
+  if (key == ~0ull) return;
+  uint32_t o = (uint32_t)(key & 0xffffffffu);
+  f3 nrm;
+  float t = cast_pair(d, h, o, &nrm);
+  d.toi_hit[h] = pk(nrm, t);
+  d.toi_tgt[h] = o;
 }
 
 __global__ void k_toi_clear(Dev d) {
@@ -828,13 +865,16 @@ __global__ void k_inc_build(Dev d) {
   }
   uint32_t w = d.inc_off[h];
   for (uint32_t i = b0; i < b1; ++i) {
-    uint2 pts = d.P.pts[d.blist[i]];
-    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = ((pts.x + k) << 1) | 1u;
+    uint32_t p = d.blist[i];
+    uint2 pts = d.P.pts[p];
+    uint2 ab = d.P.ab[p];
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = make_uint4((pts.x + k) | 0x80000000u, p, ab.x, ab.y);
   }
   uint32_t a0 = d.acc_off[d.cand_off[h]], a1 = d.acc_off[d.cand_off[h + 1]];
   for (uint32_t p = a0; p < a1; ++p) {
     uint2 pts = d.P.pts[p];
-    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = (pts.x + k) << 1;
+    uint2 ab = d.P.ab[p];
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = make_uint4(pts.x + k, p, ab.x, ab.y);
   }
 }
 
@@ -849,10 +889,8 @@ __device__ __forceinline__ bool higher(const Dev& d, uint32_t o, uint32_t h) {
   uint32_t po = d.b.prio[o], ph = d.b.prio[h];
   return po > ph || (po == ph && d.b.local[o] < d.b.local[h]);
 }
-__device__ __forceinline__ uint32_t partner_of_item(const Dev& d, uint32_t it) {
-  uint32_t q = it >> 1;
-  uint2 ab = d.P.ab[d.Q.pair[q]];
-  return (it & 1u) ? ab.x : ab.y;
+__device__ __forceinline__ uint32_t partner_of_item(const Dev& d, uint4 it) {
+  return (it.x >> 31) ? it.z : it.w;
 }
 
 // col_round[h] packs (round << 16) | colour in one word (written by a
@@ -937,58 +975,54 @@ __global__ void k_colour_fill(Dev d) {
 // primal: solve_body (assembly.rs:73-903) for the bodies of one colour
 // ---------------------------------------------------------------------------
 
-// A constraint row as the accumulation sees it.  kind 0: full stamp
-// (stamp_row: axis nn, angular lever t, penalty, force); kind 1: angular
-// couple (rolling/torsion: axis g, ROLL_PEN, force); kind 2: diagonal
-// add to the angular block (geometric stiffness).
+// A constraint row as the accumulation sees it: a full stamp (stamp_row:
+// axis nn, angular lever t, penalty, force), an angular couple
+// (rolling/torsion: axis g, ROLL_PEN, force) or a diagonal add to the
+// angular block (geometric stiffness).
+//
+// x holds the row's gradient (linear part, angular part).  The kind is
+// carried by the sign of pen (penalties are >= 1, damper stiffnesses > 0):
+//   pen > 0  stamp_row: x = (nn, t = r x nn); lower-triangle entry (i, j)
+//            receives pen * (x[i] * x[j]), rhs k receives f * x[k];
+//   pen < 0  angular couple: x = (0, g); only angular entries, stiffness -pen;
+//   pen == 0 geometric stiffness: x[3..5] added to the angular diagonal.
 struct Row {
-  float nn[3];
-  float t[3];
+  float x[6];
   float pen;
   float f;
-  int kind;
 };
 
 __device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, f3 lever, float sign) {
   f3 nn = smul(sign, axis);
   f3 t = cross3(lever, nn);
-  r.nn[0] = nn.x; r.nn[1] = nn.y; r.nn[2] = nn.z;
-  r.t[0] = t.x; r.t[1] = t.y; r.t[2] = t.z;
+  r.x[0] = nn.x; r.x[1] = nn.y; r.x[2] = nn.z;
+  r.x[3] = t.x; r.x[4] = t.y; r.x[5] = t.z;
   r.pen = pen;
   r.f = f;
-  r.kind = 0;
 }
 
-// Lower-triangle element e (0..20: row-major i >= j) or rhs (21..26)
-// contribution of a row, in the exact product form of stamp_row.
+// Contribution of a row to matrix element e: lower-triangle entry (i, j)
+// for e < 21 (row-major, i >= j), rhs entry e - 21 otherwise.  Products are
+// formed exactly as stamp_row forms them (pen * (a * b), f * a).  The branch
+// on the row kind is uniform across the lanes summing one row.
 __device__ __forceinline__ bool row_elem(const Row& r, int i, int j, int e, float* v) {
-  if (r.kind == 0) {
-    if (e >= 21) {
-      int k = e - 21;
-      *v = k < 3 ? r.f * r.nn[k] : r.f * r.t[k - 3];
-      return true;
-    }
-    if (i < 3)
-      *v = r.pen * (r.nn[i] * r.nn[j]);
-    else if (j >= 3)
-      *v = r.pen * (r.t[i - 3] * r.t[j - 3]);
-    else
-      *v = r.pen * (r.nn[j] * r.t[i - 3]);
+  const float pen = r.pen;
+  if (pen > 0.0f) {
+    *v = e >= 21 ? r.f * r.x[e - 21] : pen * (r.x[i] * r.x[j]);
     return true;
   }
-  if (r.kind == 1) {
+  if (pen < 0.0f) {
     if (e >= 21) {
-      int k = e - 21;
-      if (k < 3) return false;
-      *v = r.f * r.nn[k - 3];
+      if (e < 24) return false;
+      *v = r.f * r.x[e - 21];
       return true;
     }
-    if (i < 3 || j < 3) return false;
-    *v = r.pen * (r.nn[i - 3] * r.nn[j - 3]);
+    if (j < 3) return false;
+    *v = (-pen) * (r.x[i] * r.x[j]);
     return true;
   }
   if (e < 21 && i >= 3 && i == j) {
-    *v = r.nn[i - 3];
+    *v = r.x[i];
     return true;
   }
   return false;
@@ -996,12 +1030,12 @@ __device__ __forceinline__ bool row_elem(const Row& r, int i, int j, int e, floa
 
 // Rows of one contact point-side of body h (in the reference's row order).
 // Returns the number of rows written to out (<= 6).
-__device__ int point_rows(const Dev& d, uint32_t it, Row* out) {
-  uint32_t q = it >> 1;
-  bool is_a = (it & 1u) == 0u;
+__device__ int point_rows(const Dev& d, uint4 it, Row* out) {
+  uint32_t q = it.x & 0x7fffffffu;
+  bool is_a = (it.x >> 31) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
-  uint32_t p = d.Q.pair[q];
-  uint2 ab = d.P.ab[p];
+  uint32_t p = it.y;
+  uint2 ab = make_uint2(it.z, it.w);
   float4 n4 = d.P.n[p];
   f3 n = xyz(n4);
   float gap = n4.w;
@@ -1075,10 +1109,10 @@ __device__ int point_rows(const Dev& d, uint32_t it, Row* out) {
       float fr = clampf_(K_ROLL_PEN * c + rlv[li], -cap, cap);
       f3 g = is_a ? ax[li] : neg(ax[li]);
       Row& r = out[m++];
-      r.nn[0] = g.x; r.nn[1] = g.y; r.nn[2] = g.z;
-      r.pen = K_ROLL_PEN;
+      r.x[0] = 0.0f; r.x[1] = 0.0f; r.x[2] = 0.0f;
+      r.x[3] = g.x; r.x[4] = g.y; r.x[5] = g.z;
+      r.pen = -K_ROLL_PEN;
       r.f = fr;
-      r.kind = 1;
     }
   }
   return m;
@@ -1126,10 +1160,12 @@ __device__ int joint_rows(const Dev& d, uint32_t e, Row* out) {
       for (int y = 0; y < 3; ++y) hm[x][y] += g[x][y] * fl[k];
   }
   Row& gr = out[m++];
-  gr.nn[0] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
-  gr.nn[1] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
-  gr.nn[2] = len3(f3{hm[0][2], hm[1][2], hm[2][2]});
-  gr.kind = 2;
+  gr.x[0] = 0.0f; gr.x[1] = 0.0f; gr.x[2] = 0.0f;
+  gr.x[3] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
+  gr.x[4] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
+  gr.x[5] = len3(f3{hm[0][2], hm[1][2], hm[2][2]});
+  gr.pen = 0.0f;
+  gr.f = 0.0f;
   return m;
 }
 
@@ -1191,72 +1227,100 @@ __device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, 
 }
 
 #define WAVE 64
-#define ROWS_PER_ITEM 6
-#define WAVES_PER_BLOCK 1
 
-// One wavefront per body.  Lanes evaluate up to 64 items (point-sides,
-// then joint-sides) at a time into LDS; lanes 0..26 then add the staged
-// rows to their matrix element strictly in item-then-row order, which is
-// the reference's sequential summation order; lane 0 factors and solves.
-__global__ void __launch_bounds__(WAVE* WAVES_PER_BLOCK) k_primal_wave(Dev d, const uint32_t* __restrict__ list,
-                                                                        uint32_t count) {
-  __shared__ Row rows[WAVES_PER_BLOCK][WAVE * ROWS_PER_ITEM];
-  __shared__ int nrows[WAVES_PER_BLOCK][WAVE];
-  __shared__ float acc_s[WAVES_PER_BLOCK][27];
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int w = threadIdx.x / WAVE;
-  uint32_t slot = blockIdx.x * WAVES_PER_BLOCK + w;
-  if (slot >= count) return;  // whole wave exits together
-  uint32_t h = list[slot];
-  // element of this lane
-  int ei = 0, ej = 0;
-  if (lane < 21) {
-    int i = 0;
-    while (tri(i + 1, 0) <= lane) ++i;
-    ei = i;
-    ej = lane - tri(i, 0);
+// Primal solve of one colour class, wavefront form.  A workgroup of W
+// wavefronts holds W * 64 / L bodies, L lanes per body.  Per chunk, the L
+// lanes of a body each evaluate one item (point-sides in pair order, then
+// joint-sides) into LDS; lane j of the body then owns matrix elements j,
+// j + L, ... (21 lower-triangle entries + 6 rhs) and adds every staged row
+// to them in item-then-row order — the reference's sequential order, so the
+// result does not depend on L or W.  The workgroup's systems are then
+// factored and solved by the first wavefront, one lane per body; the other
+// wavefronts retire early.  R = rows per item.
+template <int L, int W, int R>
+__global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __restrict__ list, uint32_t count) {
+  constexpr int G = WAVE / L;                 // bodies per wavefront
+  constexpr int NB = G * W;                   // bodies per workgroup
+  constexpr int NE = (27 + L - 1) / L;        // elements per lane
+  __shared__ Row rows[NB][L * R];
+  __shared__ uint8_t nrows[NB][L];
+  __shared__ float sums[NB][28];
+  __shared__ uint32_t chunks_s[NB];
+  const int tid = threadIdx.x;
+  const int grp = tid / L, sub = tid % L;     // grp: body of this lane in the workgroup
+  const uint32_t slot = blockIdx.x * NB + grp;
+  const bool valid = slot < count;
+  const uint32_t h = valid ? list[slot] : 0u;
+  uint32_t i0 = 0, np = 0, j0 = 0, nj = 0;
+  if (valid) {
+    i0 = d.inc_off[h];
+    np = d.inc_off[h + 1] - i0;
+    j0 = d.b.ji_off[h];
+    nj = d.b.ji_off[h + 1] - j0;
   }
-  float acc = 0.0f;
-  if (lane < 27) acc = inertial_elem(d, h, lane, ei, ej);
-  uint32_t i0 = d.inc_off[h], np = d.inc_off[h + 1] - i0;
-  uint32_t j0 = d.b.ji_off[h], nj = d.b.ji_off[h + 1] - j0;
-  uint32_t total = np + nj;
-  for (uint32_t base = 0; base < total; base += WAVE) {
-    uint32_t k = base + lane;
+  const uint32_t total = np + nj;
+  float acc[NE];
+  int ei[NE], ej[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    int e = sub + k * L;
+    int i = 0;
+    if (e < 21)
+      while (tri(i + 1, 0) <= e) ++i;
+    ei[k] = i;
+    ej[k] = e < 21 ? e - tri(i, 0) : 0;
+    acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, ei[k], ej[k]) : 0.0f;
+  }
+  if (sub == 0) chunks_s[grp] = (total + L - 1) / L;
+  __syncthreads();
+  uint32_t nch = 0;
+  for (int g = 0; g < NB; ++g) nch = chunks_s[g] > nch ? chunks_s[g] : nch;
+  for (uint32_t c = 0; c < nch; ++c) {
+    uint32_t k = c * L + sub;
     int m = 0;
+    Row* out = &rows[grp][sub * R];
     if (k < np)
-      m = point_rows(d, d.inc[i0 + k], &rows[w][lane * ROWS_PER_ITEM]);
+      m = point_rows(d, d.inc[i0 + k], out);
     else if (k < total)
-      m = joint_rows(d, d.b.ji[j0 + (k - np)], &rows[w][lane * ROWS_PER_ITEM]);
-    nrows[w][lane] = m;
+      m = joint_rows(d, d.b.ji[j0 + (k - np)], out);
+    nrows[grp][sub] = (uint8_t)m;
     __syncthreads();
-    if (lane < 27) {
-      uint32_t lim = total - base < (uint32_t)WAVE ? total - base : (uint32_t)WAVE;
-      for (uint32_t it = 0; it < lim; ++it) {
-        int mm = nrows[w][it];
-        for (int r = 0; r < mm; ++r) {
+    const uint32_t lim = total > c * L ? (total - c * L < (uint32_t)L ? total - c * L : (uint32_t)L) : 0u;
+    for (uint32_t it = 0; it < lim; ++it) {
+      int mm = nrows[grp][it];
+      for (int r = 0; r < mm; ++r) {
+        const Row& row = rows[grp][it * R + r];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          int e = sub + q * L;
           float v;
-          if (row_elem(rows[w][it * ROWS_PER_ITEM + r], ei, ej, lane, &v)) acc += v;
+          if (e < 27 && row_elem(row, ei[q], ej[q], e, &v)) acc[q] += v;
         }
       }
     }
     __syncthreads();
   }
-  if (lane < 27) acc_s[w][lane] = acc;
+#pragma unroll
+  for (int q = 0; q < NE; ++q) {
+    int e = sub + q * L;
+    if (e < 27) sums[grp][e] = acc[q];
+  }
   __syncthreads();
-  if (lane == 0) {
-    float L21[21], rhs[6], dx[6];
-    for (int e = 0; e < 21; ++e) L21[e] = acc_s[w][e];
-    for (int k = 0; k < 6; ++k) rhs[k] = -acc_s[w][21 + k];
-    if (solve_6x6_dev(L21, rhs, dx)) {
-      float4 p = d.b.pos[h];
-      d.b.pos[h] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
-      qf q = quat_integrate(asq(d.b.rot[h]), f3{dx[3], dx[4], dx[5]});
-      d.b.rot[h] = pkq(q);
-      d.b.drot[h] = pk(quat_diff_vec(q, asq(d.b.rot0[h])), 0.0f);
-    } else {
-      atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
-    }
+  if (tid >= NB) return;
+  const uint32_t s2 = blockIdx.x * NB + tid;
+  if (s2 >= count) return;
+  const uint32_t hb = list[s2];
+  float L21[21], rhs[6], dx[6];
+  for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
+  for (int k = 0; k < 6; ++k) rhs[k] = -sums[tid][21 + k];
+  if (solve_6x6_dev(L21, rhs, dx)) {
+    float4 p = d.b.pos[hb];
+    d.b.pos[hb] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
+    qf q = quat_integrate(asq(d.b.rot[hb]), f3{dx[3], dx[4], dx[5]});
+    d.b.rot[hb] = pkq(q);
+    d.b.drot[hb] = pk(quat_diff_vec(q, asq(d.b.rot0[hb])), 0.0f);
+  } else {
+    atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
   }
 }
 

@@ -150,7 +150,7 @@ struct Dev {
   uint32_t* counters;  // device counters (see CTR_*)
   // per-step incidence
   uint32_t* inc_off;   // n+1
-  uint32_t* inc;       // (point << 1) | side, in pair order per body
+  uint4* inc;          // items per body in pair order: (point | side << 31, pair, a, b)
   uint32_t* bcnt;      // b-side pair counts
   uint32_t* boff;      // n+1
   uint32_t* bcur;
@@ -165,8 +165,12 @@ struct Dev {
   uint32_t* colour_off;
   uint32_t* colour_cur;
   uint32_t* colour_list;
-  float4* toi_hit;     // t, normal xyz per body (t < 0: none)
+  float4* toi_hit;     // normal xyz, w = travel to the hit (< 0: none)
   uint32_t* toi_tgt;
+  uint32_t* toi_cnt;   // n+1 cast candidates per mover
+  uint32_t* toi_off;   // n+1
+  uint2* toi_pairs;    // (mover, target) casts
+  unsigned long long* toi_best;  // per mover min (t bits << 32 | target)
   uint8_t* fast;
 };
 

@@ -70,7 +70,7 @@ struct oavbd_ctx {
   oavbd_params prm{};
   uint32_t max_bodies = 0, max_joints = 0;
   Dev d{};
-  size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0;
+  size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0, cap_toi = 0;
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
   uint32_t* scan_scratch = nullptr;
@@ -82,6 +82,9 @@ struct oavbd_ctx {
   uint32_t sleeping = 0;
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
+  int lanes = 32;   // primal launch shape: lanes per body
+  int waves = 1;    // primal launch shape: wavefronts per workgroup
+  bool rows6 = false;  // rolling/torsion rows present: 6 rows per item
   bool timing = false;
   std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
   oavbd_kernel_times kt{};
@@ -128,6 +131,8 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&c->cand_cnt, n + 1);
   dalloc(&d.colour_list, n); dalloc(&d.colour_cnt, 65536); dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
+  dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
+  d.toi_pairs = nullptr;
   uint32_t nb = 1;
   while (nb < 2 * n) nb <<= 1;
   d.g.nbuckets = nb;
@@ -221,7 +226,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.colour_cnt); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
-  dfree(d.toi_tgt); dfree(d.fast); dfree(c->cand_cnt); dfree(c->scan_scratch);
+  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(c->cand_cnt); dfree(c->scan_scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,6 +437,36 @@ static void incidence_and_colour(oavbd_ctx* c) {
   for (uint32_t k = 0; k < ncol; ++k) c->stats.max_colour = std::max(c->stats.max_colour, c->colour_cnt_h[k]);
 }
 
+template <int L, int W>
+static void launch_lw(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+  constexpr int NB = (WAVE / L) * W;
+  if (c->rows6)
+    k_primal<L, W, 6><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+  else
+    k_primal<L, W, 4><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+}
+
+template <int L>
+static void launch_l(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+  switch (c->waves) {
+    case 1: return launch_lw<L, 1>(c, list, cnt);
+    case 2: return launch_lw<L, 2>(c, list, cnt);
+    case 8: return launch_lw<L, 8>(c, list, cnt);
+    default: return launch_lw<L, 4>(c, list, cnt);
+  }
+}
+
+// Primal launch of one colour class: `lanes` lanes per body, `waves`
+// wavefronts per workgroup (the configured launch shape).
+static void launch_primal(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+  switch (c->lanes) {
+    case 8: return launch_l<8>(c, list, cnt);
+    case 16: return launch_l<16>(c, list, cnt);
+    case 64: return launch_l<64>(c, list, cnt);
+    default: return launch_l<32>(c, list, cnt);
+  }
+}
+
 static void record(oavbd_ctx* c, int i) {
   if (c->timing) HIPC(hipEventRecord(c->ev[i], c->st));
 }
@@ -447,7 +482,19 @@ static void step_inner(oavbd_ctx* c) {
   uint32_t nfast = read_u32(c, d.counters + CTR_FAST);
   build_grid(c);
   if (nfast) {
-    k_toi_cast<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_toi_count<<<blocks(n), TPB, 0, c->st>>>(d);
+    scan_excl(d.toi_cnt, d.toi_off, n, c->st, c->scan_scratch);
+    uint32_t ncast = read_u32(c, d.toi_off + n);
+    if (ncast > c->cap_toi) {
+      dfree(d.toi_pairs);
+      c->cap_toi = grow(ncast);
+      dalloc(&d.toi_pairs, c->cap_toi);
+    }
+    if (ncast) {
+      k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+      k_toi_pair<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
+    }
+    k_toi_finalize<<<blocks(n), TPB, 0, c->st>>>(d);
     k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
     k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
     build_grid(c);
@@ -482,8 +529,7 @@ static void step_inner(oavbd_ctx* c) {
       uint32_t cnt = c->colour_cnt_h[k];
       if (!cnt) continue;
       if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev), c->st));
-      k_primal_wave<<<blocks(cnt, WAVES_PER_BLOCK), WAVE * WAVES_PER_BLOCK, 0, c->st>>>(
-          d, d.colour_list + c->colour_off_h[k], cnt);
+      launch_primal(c, d.colour_list + c->colour_off_h[k], cnt);
       if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
       np_ev += 2;
     }
@@ -696,6 +742,9 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
         isl[i] = 1;
       }
     d.g.cell = (2.0f * rmax + 0.05f) * 1.01f + 1e-6f;
+    c->rows6 = false;
+    for (uint32_t i = 0; i < n; ++i)
+      if (mat[i].z > 0.0f || mat[i].w > 0.0f) c->rows6 = true;
     d.g.nlarge = (uint32_t)large.size();
     dfree(d.g.large);
     dalloc(&d.g.large, large.size());
@@ -933,6 +982,15 @@ int oavbd_get_kernel_times(oavbd_ctx* c, oavbd_kernel_times* out, int reset) {
   if (!c || !out) return OAVBD_EINVAL;
   *out = c->kt;
   if (reset) c->kt = oavbd_kernel_times{};
+  return OAVBD_OK;
+}
+
+int oavbd_set_primal_shape(oavbd_ctx* c, uint32_t lanes, uint32_t waves) {
+  if (!c) return OAVBD_EINVAL;
+  if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return OAVBD_EINVAL;
+  if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return OAVBD_EINVAL;
+  c->lanes = (int)lanes;
+  c->waves = (int)waves;
   return OAVBD_OK;
 }
 

@@ -123,3 +123,16 @@ def test_gpu_run_to_run_bitwise(gpu_available):
         g.step_n(60)
         outs.append(g.state())
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+@pytest.mark.parametrize("shape", [(8, 1), (16, 2), (32, 1), (32, 8), (64, 4)])
+def test_launch_shape_does_not_change_bits(gpu_available, shape):
+    """The primal launch shape (lanes per body, wavefronts per workgroup)
+    is a pure speed knob."""
+    s = scenes.box_drop_scene(150, np.random.Generator(np.random.PCG64(17)))
+    ref = AvbdEngine(scene=s)
+    ref.step_n(40)
+    g = AvbdEngine(scene=s)
+    g.set_primal_shape(*shape)
+    g.step_n(40)
+    assert np.array_equal(ref.state().view(np.uint32), g.state().view(np.uint32))
