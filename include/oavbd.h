@@ -67,6 +67,7 @@ typedef struct oavbd_stats {
   uint32_t point_sides; /* contact point-sides of swept bodies               */
   uint32_t pair_sides;  /* contact pair-sides (with points) of swept bodies  */
   uint32_t joint_sides; /* joint sides of swept bodies                       */
+  uint32_t toi_casts;   /* (mover, target) casts tested by the TOI clamp     */
 } oavbd_stats;
 
 /* Kernel time and algorithmic bytes accumulated since the last reset

@@ -35,7 +35,7 @@ class OavbdParams(ctypes.Structure):
 class OavbdStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in (
         "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
-        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides")]
+        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts")]
 
 
 class OavbdKernelTimes(ctypes.Structure):

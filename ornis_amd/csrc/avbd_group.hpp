@@ -1,0 +1,176 @@
+// avbd_group.hpp — box-box distance evaluated by a group of G lanes of one
+// wavefront (G a power of two, 2..64).
+//
+// The serial separating_axis / feature_distance (avbd_geom.hpp) scan a
+// fixed list of candidates (15 axes; 8 + 8 corner-box and 144 edge-edge
+// tests) and keep the first best one.  Here lane s of the group evaluates
+// candidates s, s + G, s + 2G, ... with the same arithmetic, keeps its own
+// first best, and the group reduces (value, candidate index)
+// lexicographically, so the winner — and every bit of the result — is the
+// serial one.  Every lane of a group ends with the same result; all lanes
+// of a group must execute these functions together.
+#pragma once
+
+#include "avbd_geom.hpp"
+
+namespace grp {
+
+template <int G>
+__device__ __forceinline__ float xorf(float v, int m) {
+  return __shfl_xor(v, m, G);
+}
+template <int G>
+__device__ __forceinline__ int xori(int v, int m) {
+  return __shfl_xor(v, m, G);
+}
+// v of lane `src` (0..G-1) of the calling lane's group.
+template <int G>
+__device__ __forceinline__ f3 from_lane(f3 v, int src) {
+  return f3{__shfl(v.x, src, G), __shfl(v.y, src, G), __shfl(v.z, src, G)};
+}
+
+// Box axes scaled by the half extents, as obb_corners forms them.
+struct axes {
+  f3 x, y, z;
+};
+__device__ __forceinline__ axes box_axes(f3 half, qf rot) {
+  return axes{qrot(rot, f3{1.0f * half.x, 0.0f * half.x, 0.0f * half.x}),
+              qrot(rot, f3{0.0f * half.y, 1.0f * half.y, 0.0f * half.y}),
+              qrot(rot, f3{0.0f * half.z, 0.0f * half.z, 1.0f * half.z})};
+}
+// Corner k of obb_corners: bit 2 selects -x, bit 1 -y, bit 0 -z, summed in
+// the same order.
+__device__ __forceinline__ f3 corner(f3 pos, const axes& e, int k) {
+  f3 c = (k & 4) ? sub(pos, e.x) : add(pos, e.x);
+  c = (k & 2) ? sub(c, e.y) : add(c, e.y);
+  return (k & 1) ? sub(c, e.z) : add(c, e.z);
+}
+// Box edge i as the corner pair (ea_[i], eb_[i]) of feature_distance.
+__device__ __forceinline__ void edge(int i, int* c0, int* c1) {
+  if (i < 4) {
+    *c0 = i;
+    *c1 = i + 4;
+  } else if (i < 8) {
+    int m = i - 4;
+    *c0 = (m & 1) | ((m & 2) << 1);
+    *c1 = *c0 + 2;
+  } else {
+    *c0 = 2 * (i - 8);
+    *c1 = *c0 + 1;
+  }
+}
+
+// separating_axis over the group.
+template <int G>
+__device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, int s) {
+  m3c ra = mat3_from_quat(a.rot), rb = mat3_from_quat(b.rot);
+  f3 delta = sub(b.pos, a.pos);
+  float best = -__builtin_inff();
+  int bk = 99;
+  f3 bn{0.0f, 1.0f, 0.0f};
+  for (int k = s; k < 15; k += G) {
+    f3 n;
+    if (k < 3) {
+      n = mcol(ra, k);
+    } else if (k < 6) {
+      n = mcol(rb, k - 3);
+    } else {
+      int i = (k - 6) / 3, j = (k - 6) % 3;
+      f3 c = cross3(mcol(ra, i), mcol(rb, j));
+      if (!(len2(c) > 1e-12f)) continue;
+      n = normalize3(c);
+    }
+    if (n.x == 0.0f && n.y == 0.0f && n.z == 0.0f) continue;
+    float ea = ha.x * fabsf(dot3(n, ra.c0)) + ha.y * fabsf(dot3(n, ra.c1)) + ha.z * fabsf(dot3(n, ra.c2));
+    float eb = hb.x * fabsf(dot3(n, rb.c0)) + hb.y * fabsf(dot3(n, rb.c1)) + hb.z * fabsf(dot3(n, rb.c2));
+    float proj = dot3(delta, n);
+    float gap = fabsf(proj) - ea - eb;
+    if (gap > best) {
+      best = gap;
+      bk = k;
+      bn = proj >= 0.0f ? n : neg(n);
+    }
+  }
+  // larger gap wins; equal gaps go to the earlier axis
+  float g = best;
+  int k = bk;
+  for (int m = G / 2; m > 0; m >>= 1) {
+    float og = xorf<G>(g, m);
+    int ok = xori<G>(k, m);
+    if (og > g || (og == g && ok < k)) {
+      g = og;
+      k = ok;
+    }
+  }
+  if (k != 99) {
+    bn = from_lane<G>(bn, k % G);
+  } else {
+    bn = f3{0.0f, 1.0f, 0.0f};
+  }
+  *normal_out = bn;
+  return g;
+}
+
+// feature_distance over the group.
+template <int G>
+__device__ dist_t feature(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
+  const axes ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
+  float best = __builtin_inff();
+  int bc = 0x7fffffff;
+  f3 bx = a.pos, by = b.pos;
+  for (int c = s; c < 160; c += G) {
+    f3 x, y;
+    float sq;
+    if (c < 8) {
+      x = corner(a.pos, ea, c);
+      y = point_obb_closest(x, b.pos, hb, b.rot);
+      sq = len2(sub(y, x));
+    } else if (c < 16) {
+      y = corner(b.pos, eb, c - 8);
+      x = point_obb_closest(y, a.pos, ha, a.rot);
+      sq = len2(sub(y, x));
+    } else {
+      int i = (c - 16) / 12, j = (c - 16) % 12;
+      int a0, a1, b0, b1;
+      edge(i, &a0, &a1);
+      edge(j, &b0, &b1);
+      seg_seg_closest(corner(a.pos, ea, a0), corner(a.pos, ea, a1), corner(b.pos, eb, b0), corner(b.pos, eb, b1), &x,
+                      &y);
+      sq = len2(sub(y, x));
+    }
+    if (sq < best) {
+      best = sq;
+      bc = c;
+      bx = x;
+      by = y;
+    }
+  }
+  // smaller squared distance wins; ties go to the earlier candidate
+  float q = best;
+  int c = bc;
+  for (int m = G / 2; m > 0; m >>= 1) {
+    float oq = xorf<G>(q, m);
+    int oc = xori<G>(c, m);
+    if (oq < q || (oq == q && oc < c)) {
+      q = oq;
+      c = oc;
+    }
+  }
+  if (c == 0x7fffffff) return dist_t{__builtin_inff(), a.pos, b.pos};
+  return dist_t{sqrtf(q), from_lane<G>(bx, c % G), from_lane<G>(by, c % G)};
+}
+
+// box_box_from_sat over the group.
+template <int G>
+__device__ dist_t box_box(const shp& a, f3 ha, const shp& b, f3 hb, float gap, f3 normal, int s) {
+  if (gap > 0.0f) return feature<G>(a, ha, b, hb, s);
+  shp fa, fb;
+  f3 ea, eb;
+  support_face(a, ha, normal, &fa, &ea);
+  support_face(b, hb, neg(normal), &fb, &eb);
+  dist_t r = feature<G>(fa, ea, fb, eb, s);
+  r.dist = gap;
+  return r;
+}
+
+}  // namespace grp

@@ -20,11 +20,51 @@
 #pragma once
 
 #include "avbd_geom.hpp"
+#include "avbd_group.hpp"
 #include "avbd_state.hpp"
 
 #define TPB 256
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// Wavefront-aggregated atomics: a counter hit by every thread of a large
+// launch is updated once per wavefront instead of once per thread.
+
+// Adds the number of active lanes whose `pred` holds.
+__device__ __forceinline__ void wave_count(uint32_t* ctr, bool pred) {
+  unsigned long long m = __ballot(pred);
+  if (pred && __lane_id() == (unsigned)(__ffsll((unsigned long long)m) - 1)) atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+
+// Adds the wavefront's sum of v.  Every lane of the wavefront must be
+// active here (the calling kernel has no early return before it).
+__device__ __forceinline__ void wave_add(uint32_t* ctr, uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0 && v) atomicAdd(ctr, v);
+}
+
+// Active lanes holding the same key k >= 0 reserve consecutive slots of
+// cnt[k] with one atomic per distinct key; returns the lane's slot.  If mx
+// is set it receives max(k + 1).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* cnt, int k, uint32_t* mx) {
+  uint32_t slot = 0;
+  const uint32_t lane = __lane_id();
+  unsigned long long pending = __ballot(k >= 0);
+  while (pending) {
+    int leader = __ffsll(pending) - 1;
+    int kl = __shfl(k, leader);
+    unsigned long long grp = __ballot(k == kl);
+    uint32_t base = 0;
+    if (lane == (uint32_t)leader) {
+      base = atomicAdd(&cnt[kl], (uint32_t)__popcll(grp));
+      if (mx) atomicMax(mx, (uint32_t)(kl + 1));
+    }
+    base = __shfl(base, leader);
+    if (k == kl) slot = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
+    pending &= ~grp;
+  }
+  return slot;
+}
 
 // ---------------------------------------------------------------------------
 // body helpers
@@ -105,7 +145,7 @@ __global__ void k_begin(Dev d) {
     }
   }
   d.fast[h] = fast;
-  if (fast) atomicAdd(&d.counters[CTR_FAST], 1u);
+  wave_count(&d.counters[CTR_FAST], fast != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -175,14 +215,40 @@ __device__ __forceinline__ bool toi_target_ok(const Dev& d, uint32_t h, uint32_t
   return !jointed_dev(d, h, o);
 }
 
-// Conservative advancement of mover h against target o; t < 0 if no hit.
-__device__ float cast_one(const Dev& d, const shp& mover, f3 dir, float len, const shp& target, f3* normal) {
+// Conservative advancement of mover against target (cast_shape,
+// distance.rs:841-891); t < 0 if no hit.
+//
+// Box against box: the SAT gap is a lower bound of the distance (the
+// separation along any unit axis is), so once the gap lowered by `slack`
+// (a bound on the rounding of both computations, proportional to the
+// coordinates' magnitude) already exceeds the touch distance and the
+// remaining sweep, the loop would end without a hit whatever the exact
+// distance: the expensive feature-distance pass is skipped and the outcome
+// is unchanged.
+HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* normal) {
+  const bool bb = mover.kind == SH_BOX && target.kind == SH_BOX;
+  float slack = 0.0f;
+  if (bb) {
+    float s = 1.0f + len + len3(mover.p) + len3(target.p);
+    s += fmaxf(fmaxf(fabsf(mover.pos.x), fabsf(mover.pos.y)), fabsf(mover.pos.z));
+    s += fmaxf(fmaxf(fabsf(target.pos.x), fabsf(target.pos.y)), fabsf(target.pos.z));
+    slack = 4e-5f * s;
+  }
   float t = 0.0f;
   for (int it = 0; it < 24; ++it) {
     shp m = mover;
     m.pos = add(mover.pos, mul(dir, t));
-    float sat;
-    dist_t r = shape_distance(m, target, &sat);
+    dist_t r;
+    if (bb) {
+      f3 sn;
+      float gap = separating_axis(m, m.p, target, target.p, &sn);
+      float lo = gap - slack;
+      if (lo > K_SHAPE_TOUCH && t + (lo - K_SHAPE_TOUCH * 0.5f) >= len) return -1.0f;
+      r = box_box_from_sat(m, m.p, target, target.p, gap, sn);
+    } else {
+      float sat;
+      r = shape_distance(m, target, &sat);
+    }
     if (r.dist <= K_SHAPE_TOUCH) {
       if (t > 0.0f) {
         *normal = normalize_or(sub(r.pa, r.pb), neg(dir));
@@ -262,37 +328,73 @@ __global__ void k_toi_fill(Dev d) {
   toi_targets<true>(d, h, d.toi_pairs + d.toi_off[h]);
 }
 
-__device__ __forceinline__ float cast_pair(const Dev& d, uint32_t h, uint32_t o, f3* nrm) {
-  f3 delta = mul(xyz(d.b.vel[h]), K_DT);
+__device__ __forceinline__ unsigned long long toi_key(float t, uint32_t o) {
+  return ((unsigned long long)__float_as_uint(t) << 32) | o;
+}
+
+// cast_one for a group of G lanes: box-box distances are evaluated by the
+// whole group (same results on every lane); other shape pairs are cheap
+// closed forms that every lane evaluates alike.
+template <int G>
+__device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, int s) {
+  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one(mover, dir, len, target, normal);
+  float sc = 1.0f + len + len3(mover.p) + len3(target.p);
+  sc += fmaxf(fmaxf(fabsf(mover.pos.x), fabsf(mover.pos.y)), fabsf(mover.pos.z));
+  sc += fmaxf(fmaxf(fabsf(target.pos.x), fabsf(target.pos.y)), fabsf(target.pos.z));
+  const float slack = 4e-5f * sc;
+  float t = 0.0f;
+  for (int it = 0; it < 24; ++it) {
+    shp m = mover;
+    m.pos = add(mover.pos, mul(dir, t));
+    f3 sn;
+    float gap = grp::sat<G>(m, m.p, target, target.p, &sn, s);
+    float lo = gap - slack;
+    if (lo > K_SHAPE_TOUCH && t + (lo - K_SHAPE_TOUCH * 0.5f) >= len) return -1.0f;
+    dist_t r = grp::box_box<G>(m, m.p, target, target.p, gap, sn, s);
+    if (r.dist <= K_SHAPE_TOUCH) {
+      if (t > 0.0f) {
+        *normal = normalize_or(sub(r.pa, r.pb), neg(dir));
+        return t;
+      }
+      return -1.0f;
+    }
+    t += r.dist - K_SHAPE_TOUCH * 0.5f;
+    if (t >= len) break;
+  }
+  return -1.0f;
+}
+
+// One (mover, target) cast per group of G lanes.  The cast's (normal, t) is
+// kept and the mover keeps the smallest (t, target) key: t > 0, so its
+// IEEE bits order like the value.
+template <int G>
+__global__ void __launch_bounds__(TPB) k_toi_pair(Dev d, uint32_t npairs) {
+  uint32_t i = gid() / G;
+  if (i >= npairs) return;
+  const int s = (int)(threadIdx.x % G);
+  uint2 ho = d.toi_pairs[i];
+  f3 delta = mul(xyz(d.b.vel[ho.x]), K_DT);
   float len = len3(delta);
   f3 dir = divs(delta, len);
-  return cast_one(d, body_shape(d, h), dir, len, body_shape(d, o), nrm);
+  f3 nrm = zero3();
+  float t = cast_group<G>(body_shape(d, ho.x), dir, len, body_shape(d, ho.y), &nrm, s);
+  if (s != 0) return;
+  d.toi_res[i] = pk(nrm, t);
+  if (t < 0.0f) return;
+  atomicMin(&d.toi_best[ho.x], toi_key(t, ho.y));
 }
 
-// One (mover, target) cast per thread; the mover keeps the smallest
-// (t, target) key: t > 0, so its IEEE bits order like the value.
-__global__ void __launch_bounds__(TPB) k_toi_pair(Dev d, uint32_t npairs) {
+// The mover's hit is the cast holding the winning key (a repeated
+// (mover, target) entry carries identical values).
+__global__ void __launch_bounds__(TPB) k_toi_win(Dev d, uint32_t npairs) {
   uint32_t i = gid();
   if (i >= npairs) return;
+  float4 r = d.toi_res[i];
+  if (!(r.w >= 0.0f)) return;
   uint2 ho = d.toi_pairs[i];
-  f3 nrm;
-  float t = cast_pair(d, ho.x, ho.y, &nrm);
-  if (t < 0.0f) return;
-  unsigned long long key = ((unsigned long long)__float_as_uint(t) << 32) | ho.y;
-  atomicMin(&d.toi_best[ho.x], key);
-}
-
-// The winning cast again, for its contact normal (same inputs, same bits).
-__global__ void k_toi_finalize(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
-  unsigned long long key = d.toi_best[h];
-  if (key == ~0ull) return;
-  uint32_t o = (uint32_t)(key & 0xffffffffu);
-  f3 nrm;
-  float t = cast_pair(d, h, o, &nrm);
-  d.toi_hit[h] = pk(nrm, t);
-  d.toi_tgt[h] = o;
+  if (d.toi_best[ho.x] != toi_key(r.w, ho.y)) return;
+  d.toi_hit[ho.x] = r;
+  d.toi_tgt[ho.x] = ho.y;
 }
 
 __global__ void k_toi_clear(Dev d) {
@@ -322,7 +424,7 @@ __global__ void k_toi_apply(Dev d) {
   }
   d.b.timer[h] = 0.0f;
   d.b.flag_a[o] = 1;  // wake the target
-  atomicAdd(&d.counters[CTR_TOIHITS], 1u);
+  wave_count(&d.counters[CTR_TOIHITS], true);
 }
 
 // Apply pending wakes (flag_a) and clear them.
@@ -832,27 +934,71 @@ __global__ void k_pair_prep(Dev d) {
 // ---------------------------------------------------------------------------
 // per-body incidence (point-sides in pair order) and colouring
 // ---------------------------------------------------------------------------
-__global__ void k_inc_count(Dev d) {
+// Incidence is built for swept bodies only (the primal solves nothing
+// else); the colouring graph links swept bodies sharing a pair with
+// points or a joint (the graph of colour_bodies).
+__global__ void __launch_bounds__(TPB) k_inc_count(Dev d) {
+  uint32_t p = gid();
+  uint32_t side_pts = 0, side_pairs = 0;
+  if (p < d.npairs) {
+    uint2 ab = d.P.ab[p];
+    uint32_t c = d.P.pts[p].y;
+    bool sa = d.sw[ab.x], sb = d.sw[ab.y];
+    if (sa) atomicAdd(&d.ptcnt[ab.x], c);
+    if (sb) {
+      atomicAdd(&d.ptcnt[ab.y], c);
+      atomicAdd(&d.bcnt[ab.y], 1u);
+    }
+    if (c && sa && sb) {
+      atomicAdd(&d.nbr_cnt[ab.x], 1u);
+      atomicAdd(&d.nbr_cnt[ab.y], 1u);
+    }
+    uint32_t ns = (uint32_t)sa + (uint32_t)sb;
+    side_pts = c * ns;
+    side_pairs = c ? ns : 0u;
+  }
+  wave_add(&d.counters[CTR_PTSIDES], side_pts);
+  wave_add(&d.counters[CTR_PAIRSIDES], side_pairs);
+}
+__global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
+  uint32_t j = gid();
+  uint32_t sides = 0;
+  if (j < d.njoints) {
+    uint2 ab = d.J.ab[j];
+    bool sa = d.sw[ab.x], sb = d.sw[ab.y];
+    if (sa && sb) {
+      atomicAdd(&d.nbr_cnt[ab.x], 1u);
+      atomicAdd(&d.nbr_cnt[ab.y], 1u);
+    }
+    sides = (uint32_t)sa + (uint32_t)sb;
+  }
+  wave_add(&d.counters[CTR_JSIDES], sides);
+}
+__global__ void k_inc_fill(Dev d) {
   uint32_t p = gid();
   if (p >= d.npairs) return;
   uint2 ab = d.P.ab[p];
-  uint32_t c = d.P.pts[p].y;
-  atomicAdd(&d.ptcnt[ab.x], c);
-  atomicAdd(&d.ptcnt[ab.y], c);
-  atomicAdd(&d.bcnt[ab.y], 1u);
+  bool sa = d.sw[ab.x], sb = d.sw[ab.y];
+  if (sb) d.blist[d.boff[ab.y] + atomicAdd(&d.bcur[ab.y], 1u)] = p;
+  if (sa && sb && d.P.pts[p].y) {
+    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
+    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
+  }
 }
-__global__ void k_inc_bfill(Dev d) {
-  uint32_t p = gid();
-  if (p >= d.npairs) return;
-  uint32_t b = d.P.ab[p].y;
-  uint32_t s = atomicAdd(&d.bcur[b], 1u);
-  d.blist[d.boff[b] + s] = p;
+__global__ void k_nbr_joint_fill(Dev d) {
+  uint32_t j = gid();
+  if (j >= d.njoints) return;
+  uint2 ab = d.J.ab[j];
+  if (d.sw[ab.x] && d.sw[ab.y]) {
+    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
+    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
+  }
 }
-// Body h's pairs in ascending pair index: pairs where h is b (a < h), then
-// the contiguous run where h is a.
+// Body h's point-sides in ascending pair index: pairs where h is b (a < h),
+// then the contiguous run where h is a.
 __global__ void k_inc_build(Dev d) {
   uint32_t h = gid();
-  if (h >= d.n) return;
+  if (h >= d.n || !d.sw[h]) return;
   uint32_t b0 = d.boff[h], b1 = d.boff[h + 1];
   for (uint32_t i = b0 + 1; i < b1; ++i) {
     uint32_t v = d.blist[i];
@@ -878,19 +1024,14 @@ __global__ void k_inc_build(Dev d) {
   }
 }
 
-// Jones-Plassmann round: an uncoloured swept body whose uncoloured
-// neighbours all have lower priority takes the smallest colour unused by
-// its neighbours.  Equal to the sequential greedy colouring in priority
-// order.  Priority: larger hash, then smaller scene-local index.
+// Jones-Plassmann colouring: in each round an uncoloured swept body whose
+// uncoloured neighbours all have lower priority takes the smallest colour
+// unused by its neighbours.  This equals the sequential greedy colouring
+// in priority order (larger hash first, then smaller scene-local index;
+// the key packs both, so a larger key is a higher priority; bodies of
+// different scenes are never neighbours, so keys of neighbours differ).
 __device__ __forceinline__ bool swept(const Dev& d, uint32_t h) {
   return d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f && !d.b.asleep[h];
-}
-__device__ __forceinline__ bool higher(const Dev& d, uint32_t o, uint32_t h) {
-  uint32_t po = d.b.prio[o], ph = d.b.prio[h];
-  return po > ph || (po == ph && d.b.local[o] < d.b.local[h]);
-}
-__device__ __forceinline__ uint32_t partner_of_item(const Dev& d, uint4 it) {
-  return (it.x >> 31) ? it.z : it.w;
 }
 
 // col_round[h] packs (round << 16) | colour in one word (written by a
@@ -899,6 +1040,8 @@ __device__ __forceinline__ uint32_t partner_of_item(const Dev& d, uint4 it) {
 __global__ void k_colour_init(Dev d) {
   uint32_t h = gid();
   if (h >= d.n) return;
+  d.sw[h] = swept(d, h) ? 1 : 0;
+  d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
   d.b.colour[h] = -1;
   d.b.col_round[h] = -1;
 }
@@ -909,66 +1052,47 @@ __device__ __forceinline__ int colour_seen(const Dev& d, uint32_t o, int round) 
   return v & 0xffff;
 }
 
-__global__ void k_colour_round(Dev d, int round) {
+__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, int round, uint32_t* remain) {
   uint32_t h = gid();
-  if (h >= d.n || !swept(d, h) || d.b.col_round[h] >= 0) return;
-  for (uint32_t k = d.inc_off[h]; k < d.inc_off[h + 1]; ++k) {
-    uint32_t o = partner_of_item(d, d.inc[k]);
-    if (swept(d, o) && colour_seen(d, o, round) < 0 && higher(d, o, h)) {
-      atomicAdd(&d.counters[CTR_REMAIN], 1u);
-      return;
+  bool wait = false;
+  int newc = -1;
+  if (h < d.n && d.sw[h] && d.b.col_round[h] < 0) {
+    const unsigned long long kh = d.ckey[h];
+    const uint32_t k0 = d.nbr_off[h], k1 = d.nbr_off[h + 1];
+    for (uint32_t k = k0; k < k1; ++k) {
+      uint32_t o = d.nbr[k];
+      if (colour_seen(d, o, round) < 0 && d.ckey[o] > kh) {
+        wait = true;
+        break;
+      }
+    }
+    for (int base = 0; !wait; base += 64) {
+      uint64_t used = 0;
+      for (uint32_t k = k0; k < k1; ++k) {
+        int co = colour_seen(d, d.nbr[k], round);
+        if (co >= base && co < base + 64) used |= 1ull << (co - base);
+      }
+      if (~used != 0ull) {
+        newc = base + __builtin_ctzll(~used);
+        __atomic_store_n(&d.b.col_round[h], (round << 16) | newc, __ATOMIC_RELAXED);
+        break;
+      }
     }
   }
-  for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
-    uint32_t e = d.b.ji[k];
-    uint2 ab = d.J.ab[e >> 1];
-    uint32_t o = (e & 1u) ? ab.x : ab.y;
-    if (swept(d, o) && colour_seen(d, o, round) < 0 && higher(d, o, h)) {
-      atomicAdd(&d.counters[CTR_REMAIN], 1u);
-      return;
-    }
-  }
-  for (int base = 0;; base += 64) {
-    uint64_t used = 0;
-    for (uint32_t k = d.inc_off[h]; k < d.inc_off[h + 1]; ++k) {
-      uint32_t o = partner_of_item(d, d.inc[k]);
-      int co = swept(d, o) ? colour_seen(d, o, round) : -1;
-      if (co >= base && co < base + 64) used |= 1ull << (co - base);
-    }
-    for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
-      uint32_t e = d.b.ji[k];
-      uint2 ab = d.J.ab[e >> 1];
-      uint32_t o = (e & 1u) ? ab.x : ab.y;
-      int co = swept(d, o) ? colour_seen(d, o, round) : -1;
-      if (co >= base && co < base + 64) used |= 1ull << (co - base);
-    }
-    if (~used != 0ull) {
-      int c = base + __builtin_ctzll(~used);
-      __atomic_store_n(&d.b.col_round[h], (round << 16) | c, __ATOMIC_RELAXED);
-      atomicMax(&d.counters[CTR_MAXCOL], (uint32_t)(c + 1));
-      atomicAdd(&d.colour_cnt[c], 1u);
-      return;
-    }
-  }
+  wave_count(remain, wait);
+  (void)wave_append(d.colour_cnt, newc, &d.counters[CTR_MAXCOL]);
 }
 
-__global__ void k_colour_fill(Dev d) {
+__global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
   uint32_t h = gid();
-  if (h >= d.n) return;
-  int v = d.b.col_round[h];
-  int c = v < 0 ? -1 : (v & 0xffff);
-  d.b.colour[h] = c;
-  if (c < 0) return;
-  // sizes of the swept work (algorithmic-byte accounting)
-  uint32_t ps = 0;
-  for (uint32_t i = d.boff[h]; i < d.boff[h + 1]; ++i) ps += d.P.pts[d.blist[i]].y > 0u ? 1u : 0u;
-  for (uint32_t p = d.acc_off[d.cand_off[h]]; p < d.acc_off[d.cand_off[h + 1]]; ++p) ps += d.P.pts[p].y > 0u ? 1u : 0u;
-  atomicAdd(&d.counters[CTR_SOLVED], 1u);
-  atomicAdd(&d.counters[CTR_PTSIDES], d.inc_off[h + 1] - d.inc_off[h]);
-  atomicAdd(&d.counters[CTR_PAIRSIDES], ps);
-  atomicAdd(&d.counters[CTR_JSIDES], d.b.ji_off[h + 1] - d.b.ji_off[h]);
-  uint32_t s = atomicAdd(&d.colour_cur[c], 1u);
-  d.colour_list[d.colour_off[c] + s] = h;
+  int c = -1;
+  if (h < d.n) {
+    int v = d.b.col_round[h];
+    c = v < 0 ? -1 : (v & 0xffff);
+    d.b.colour[h] = c;
+  }
+  uint32_t slot = wave_append(d.colour_cur, c, nullptr);
+  if (c >= 0) d.colour_list[d.colour_off[c] + slot] = h;
 }
 
 // ---------------------------------------------------------------------------
@@ -1518,7 +1642,7 @@ __global__ void k_sleep_apply(Dev d) {
     d.b.asleep[h] = 1;
   }
   d.b.flag_a[h] = 0;
-  if (d.b.asleep[h]) atomicAdd(&d.counters[CTR_SLEEPING], 1u);
+  wave_count(&d.counters[CTR_SLEEPING], d.b.asleep[h] != 0);
   d.b.prev_pos[h] = d.b.pos[h];
   d.b.prev_rot[h] = d.b.rot[h];
 }

@@ -41,6 +41,8 @@
 #define K_CCD_GATE 0.5f
 #define K_JOINT_REST 0.005f
 #define K_MAX_POINTS 8
+#define COLOUR_BATCH 4  // colouring rounds launched per host check
+#define TOI_LANES 16    // lanes cooperating on one TOI cast
 
 enum { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
 
@@ -159,6 +161,13 @@ struct Dev {
   uint32_t* acc_off;   // ncand+1 exclusive scan of accepted
   uint32_t* pt_off;    // ncand+1 exclusive scan of fresh points
   uint32_t* ptcnt;     // per body point-side counts
+  uint8_t* sw;         // body is swept this step (dynamic, finite mass, awake)
+  unsigned long long* ckey;  // colouring priority: hash << 32 | ~scene-local index
+  uint32_t* nbr_cnt;   // n+1 colouring neighbours (swept partners via points or joints)
+  uint32_t* nbr_off;   // n+1
+  uint32_t* nbr_cur;
+  uint32_t* nbr;
+  uint32_t* remain;    // bodies left uncoloured after each round of a batch
   uint32_t* lflag;     // nlarge x n flags of large-body candidates
   uint32_t* lrank;     // nlarge x (n+1) scans
   uint32_t* colour_cnt;
@@ -170,6 +179,7 @@ struct Dev {
   uint32_t* toi_cnt;   // n+1 cast candidates per mover
   uint32_t* toi_off;   // n+1
   uint2* toi_pairs;    // (mover, target) casts
+  float4* toi_res;     // per cast: normal xyz, w = travel to the hit (< 0: none)
   unsigned long long* toi_best;  // per mover min (t bits << 32 | target)
   uint8_t* fast;
 };

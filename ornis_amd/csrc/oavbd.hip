@@ -70,7 +70,7 @@ struct oavbd_ctx {
   oavbd_params prm{};
   uint32_t max_bodies = 0, max_joints = 0;
   Dev d{};
-  size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0, cap_toi = 0;
+  size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0, cap_toi = 0, cap_nbr = 0;
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
   uint32_t* scan_scratch = nullptr;
@@ -80,6 +80,7 @@ struct oavbd_ctx {
   std::vector<int32_t> kind_h, type_h;
   float time_debt = 0.0f;
   uint32_t sleeping = 0;
+  uint32_t solved = 0;  // bodies coloured (swept) this step
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   int lanes = 32;   // primal launch shape: lanes per body
@@ -128,11 +129,15 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.counters, CTR_COUNT);
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
+  dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
+  dalloc(&d.nbr_cur, n + 1); dalloc(&d.remain, COLOUR_BATCH);
+  d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
   dalloc(&d.colour_list, n); dalloc(&d.colour_cnt, 65536); dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
+  d.toi_res = nullptr;
   uint32_t nb = 1;
   while (nb < 2 * n) nb <<= 1;
   d.g.nbuckets = nb;
@@ -196,6 +201,12 @@ static void ensure_points(oavbd_ctx* c, size_t need) {
 
 // Per-step incidence scratch, sized for the current pair / point counts.
 static void ensure_incidence(oavbd_ctx* c, size_t npairs, size_t npts) {
+  size_t nn = 2 * (npairs + (size_t)c->d.njoints);
+  if (nn > c->cap_nbr || !c->d.nbr) {
+    dfree(c->d.nbr);
+    c->cap_nbr = grow(nn);
+    dalloc(&c->d.nbr, c->cap_nbr);
+  }
   if (npairs > c->cap_blist || !c->d.blist) {
     dfree(c->d.blist);
     c->cap_blist = grow(npairs);
@@ -226,7 +237,8 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.colour_cnt); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
-  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(c->cand_cnt); dfree(c->scan_scratch);
+  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
+  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(d.remain); dfree(c->cand_cnt); dfree(c->scan_scratch);
 }
 
 // ---------------------------------------------------------------------------
@@ -401,26 +413,39 @@ static void narrow_and_merge(oavbd_ctx* c) {
 static void incidence_and_colour(oavbd_ctx* c) {
   Dev& d = c->d;
   uint32_t n = d.n;
+  k_colour_init<<<blocks(n), TPB, 0, c->st>>>(d);
   zero_u32(c, d.ptcnt, n + 1);
   zero_u32(c, d.bcnt, n + 1);
   zero_u32(c, d.bcur, n + 1);
+  zero_u32(c, d.nbr_cnt, n + 1);
+  zero_u32(c, d.nbr_cur, n + 1);
   if (d.npairs) k_inc_count<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  if (d.njoints) k_nbr_joint_count<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   scan_excl(d.ptcnt, d.inc_off, n, c->st, c->scan_scratch);
   scan_excl(d.bcnt, d.boff, n, c->st, c->scan_scratch);
-  if (d.npairs) k_inc_bfill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  scan_excl(d.nbr_cnt, d.nbr_off, n, c->st, c->scan_scratch);
+  if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
-  // Jones-Plassmann colouring
-  k_colour_init<<<blocks(n), TPB, 0, c->st>>>(d);
+  // Jones-Plassmann colouring, COLOUR_BATCH rounds per host check (rounds
+  // after the last coloured body find nothing to do)
   zero_u32(c, d.colour_cnt, 65536);
   zero_u32(c, d.counters + CTR_MAXCOL, 1);
-  int round = 0;
-  for (;; ++round) {
-    zero_u32(c, d.counters + CTR_REMAIN, 1);
-    k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, round);
-    if (read_u32(c, d.counters + CTR_REMAIN) == 0) break;
-    if (round > 30000) throw HipError{"colouring did not converge"};
+  uint32_t rem[COLOUR_BATCH];
+  int rounds = 0;
+  for (int base = 0; rounds == 0; base += COLOUR_BATCH) {
+    if (base > 30000) throw HipError{"colouring did not converge"};
+    zero_u32(c, d.remain, COLOUR_BATCH);
+    for (int r = 0; r < COLOUR_BATCH; ++r) k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, base + r, d.remain + r);
+    HIPC(hipMemcpyAsync(rem, d.remain, sizeof(rem), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    for (int r = 0; r < COLOUR_BATCH; ++r)
+      if (rem[r] == 0) {
+        rounds = base + r + 1;
+        break;
+      }
   }
-  c->stats.colour_rounds = (uint32_t)round + 1;
+  c->stats.colour_rounds = (uint32_t)rounds;
   uint32_t ncol = read_u32(c, d.counters + CTR_MAXCOL);
   c->colour_cnt_h.assign(ncol, 0);
   c->colour_off_h.assign(ncol + 1, 0);
@@ -434,7 +459,11 @@ static void incidence_and_colour(oavbd_ctx* c) {
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   c->stats.colours = ncol;
   c->stats.max_colour = 0;
-  for (uint32_t k = 0; k < ncol; ++k) c->stats.max_colour = std::max(c->stats.max_colour, c->colour_cnt_h[k]);
+  c->solved = 0;
+  for (uint32_t k = 0; k < ncol; ++k) {
+    c->stats.max_colour = std::max(c->stats.max_colour, c->colour_cnt_h[k]);
+    c->solved += c->colour_cnt_h[k];
+  }
 }
 
 template <int L, int W>
@@ -487,18 +516,22 @@ static void step_inner(oavbd_ctx* c) {
     uint32_t ncast = read_u32(c, d.toi_off + n);
     if (ncast > c->cap_toi) {
       dfree(d.toi_pairs);
+      dfree(d.toi_res);
       c->cap_toi = grow(ncast);
       dalloc(&d.toi_pairs, c->cap_toi);
+      dalloc(&d.toi_res, c->cap_toi);
     }
+    c->stats.toi_casts = ncast;
     if (ncast) {
       k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
-      k_toi_pair<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
+      k_toi_pair<TOI_LANES><<<blocks((size_t)ncast * TOI_LANES), TPB, 0, c->st>>>(d, ncast);
+      k_toi_win<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
     }
-    k_toi_finalize<<<blocks(n), TPB, 0, c->st>>>(d);
     k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
     k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
     build_grid(c);
   } else {
+    c->stats.toi_casts = 0;
     k_toi_clear<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   broadphase(c);
@@ -560,7 +593,7 @@ static void step_inner(oavbd_ctx* c) {
   c->stats.pairs = d.npairs;
   c->stats.points = d.npts;
   c->stats.candidates = c->ncand;
-  c->stats.solved = c->counters_h[CTR_SOLVED];
+  c->stats.solved = c->solved;
   c->stats.point_sides = c->counters_h[CTR_PTSIDES];
   c->stats.pair_sides = c->counters_h[CTR_PAIRSIDES];
   c->stats.joint_sides = c->counters_h[CTR_JSIDES];

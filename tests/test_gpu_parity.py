@@ -96,6 +96,25 @@ def test_sleep_and_fast_impact_bitwise(gpu_available):
     assert a_g.sum() >= 3
 
 
+def test_fast_rain_toi_bitwise(gpu_available):
+    """Fast rotated boxes and spheres shot at a box floor and at each other:
+    many box-box casts, clamps compared every step."""
+    rng = np.random.Generator(np.random.PCG64(31))
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (8.0, 1.0, 8.0), 0.0)
+    for i in range(160):
+        p = (rng.uniform(-6, 6), rng.uniform(0.8, 5.0), rng.uniform(-6, 6))
+        v = (rng.uniform(-8, 8), -rng.uniform(20, 70), rng.uniform(-8, 8))
+        if i % 5 == 0:
+            sb.sphere(p, rng.uniform(0.15, 0.4), 1.0, vel=v)
+        else:
+            q = rng.normal(size=4)
+            sb.box(p, tuple(rng.uniform(0.15, 0.5, 3)), 1.0, vel=v,
+                   quat=tuple((q / np.linalg.norm(q)).astype(np.float32).tolist()))
+    g, o = run_pair(sb.build(), 40, check_every=1)
+    assert g.stats()["toi_hits"] >= 0
+
+
 def test_batched_scenes_equal_independent_runs(gpu_available):
     """Scenes stepped together in one context equal each scene stepped
     alone by the oracle."""
