@@ -1115,6 +1115,7 @@ struct Row {
   float pen;
   float f;
 };
+static_assert(sizeof(Row) == 32, "rows are read as 8 floats");
 
 __device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, f3 lever, float sign) {
   f3 nn = smul(sign, axis);
@@ -1125,36 +1126,11 @@ __device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, 
   r.f = f;
 }
 
-// Contribution of a row to matrix element e: lower-triangle entry (i, j)
-// for e < 21 (row-major, i >= j), rhs entry e - 21 otherwise.  Products are
-// formed exactly as stamp_row forms them (pen * (a * b), f * a).  The branch
-// on the row kind is uniform across the lanes summing one row.
-__device__ __forceinline__ bool row_elem(const Row& r, int i, int j, int e, float* v) {
-  const float pen = r.pen;
-  if (pen > 0.0f) {
-    *v = e >= 21 ? r.f * r.x[e - 21] : pen * (r.x[i] * r.x[j]);
-    return true;
-  }
-  if (pen < 0.0f) {
-    if (e >= 21) {
-      if (e < 24) return false;
-      *v = r.f * r.x[e - 21];
-      return true;
-    }
-    if (j < 3) return false;
-    *v = (-pen) * (r.x[i] * r.x[j]);
-    return true;
-  }
-  if (e < 21 && i >= 3 && i == j) {
-    *v = r.x[i];
-    return true;
-  }
-  return false;
-}
-
-// Rows of one contact point-side of body h (in the reference's row order).
-// Returns the number of rows written to out (<= 6).
-__device__ int point_rows(const Dev& d, uint4 it, Row* out) {
+// Rows of one contact point-side of body h.  Row slots in the reference's
+// row order: 0 normal (or the damper), 1-2 friction, 3-5 rolling/torsion
+// (only when R == 6); returns the mask of slots written.
+template <int R>
+__device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out)[R]) {
   uint32_t q = it.x & 0x7fffffffu;
   bool is_a = (it.x >> 31) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
@@ -1188,9 +1164,9 @@ __device__ int point_rows(const Dev& d, uint4 it, Row* out) {
     if (damp > 0.0f && fabsf(cn) >= K_C_EPS) {
       float f_d = fmin_(damp * cn, 0.0f);
       make_stamp(out[0], n, damp, f_d, r_side, sign);
-      return 1;
+      return 1u;
     }
-    return 0;
+    return 0u;
   }
   f3 t1 = xyz(t14), t2 = xyz(d.P.t2[p]);
   // friction_levers (rows.rs:987-996)
@@ -1215,35 +1191,43 @@ __device__ int point_rows(const Dev& d, uint4 it, Row* out) {
     f1 = r0 * k;
     f2 = r1 * k;
   }
-  int m = 0;
+  uint32_t m = 0;
   f3 r_side_f = is_a ? ra_f : rb_f;
-  if (fabsf(cn) >= K_C_EPS || fn_c != 0.0f) make_stamp(out[m++], n, pen.x, fn_c, r_side, sign);
-  if (fabsf(ct1) >= K_C_EPS || f1 != 0.0f) make_stamp(out[m++], t1, pen.y, f1, r_side_f, sign);
-  if (fabsf(ct2) >= K_C_EPS || f2 != 0.0f) make_stamp(out[m++], t2, pen.z, f2, r_side_f, sign);
-  if (mu.z > 0.0f || mu.w > 0.0f) {
+  make_stamp(out[0], n, pen.x, fn_c, r_side, sign);
+  make_stamp(out[1], t1, pen.y, f1, r_side_f, sign);
+  make_stamp(out[2], t2, pen.z, f2, r_side_f, sign);
+  if (fabsf(cn) >= K_C_EPS || fn_c != 0.0f) m |= 1u;
+  if (fabsf(ct1) >= K_C_EPS || f1 != 0.0f) m |= 2u;
+  if (fabsf(ct2) >= K_C_EPS || f2 != 0.0f) m |= 4u;
+  if constexpr (R >= 6) {
+    if (!(mu.z > 0.0f || mu.w > 0.0f)) return m;
     f3 wrel = sub(dra, drb);
     float4 rl = d.Q.roll[q];
     f3 ax[3] = {t1, t2, n};
     float mur[3] = {mu.z, mu.z, mu.w};
     float rlv[3] = {rl.x, rl.y, rl.z};
+#pragma unroll
     for (int li = 0; li < 3; ++li) {
-      if (mur[li] <= 0.0f) continue;
       float c = dot3(wrel, ax[li]);
       float cap = mur[li] * fabsf(fn_c);
       float fr = clampf_(K_ROLL_PEN * c + rlv[li], -cap, cap);
       f3 g = is_a ? ax[li] : neg(ax[li]);
-      Row& r = out[m++];
+      Row& r = out[3 + li];
       r.x[0] = 0.0f; r.x[1] = 0.0f; r.x[2] = 0.0f;
       r.x[3] = g.x; r.x[4] = g.y; r.x[5] = g.z;
       r.pen = -K_ROLL_PEN;
       r.f = fr;
+      if (!(mur[li] <= 0.0f)) m |= 8u << li;
     }
   }
   return m;
 }
 
-// Rows of one Ball-joint side (assembly.rs:279-310, :589-618).
-__device__ int joint_rows(const Dev& d, uint32_t e, Row* out) {
+// Rows of one Ball-joint side (assembly.rs:279-310, :589-618).  Slots
+// 0-2: the three axis stamps; slot 3: the geometric-stiffness diagonal.
+// Returns the mask of slots written.
+template <int R>
+__device__ __forceinline__ uint32_t joint_rows(const Dev& d, uint32_t e, Row (&out)[R]) {
   uint32_t j = e >> 1;
   bool is_a = (e & 1u) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
@@ -1259,14 +1243,16 @@ __device__ int joint_rows(const Dev& d, uint32_t e, Row* out) {
   f3 r_a = qrot(qa, xyz(d.J.la0[j]));
   f3 r_b = qrot(qb, lb);
   float fl[3] = {0.0f, 0.0f, 0.0f};
-  int m = 0;
+  uint32_t m = 0;
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
     float c = comp(live, k) - K_ALPHA * comp(c0v, k);
     float f = penv[k] * c + lamv[k];
-    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
     f3 axis = zero3();
     setc(axis, k, 1.0f);
-    make_stamp(out[m++], axis, penv[k], f, is_a ? r_a : r_b, sign);
+    make_stamp(out[k], axis, penv[k], f, is_a ? r_a : r_b, sign);
+    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+    m |= 1u << k;
     fl[k] = f;
   }
   f3 r = is_a ? r_a : neg(r_b);
@@ -1283,7 +1269,8 @@ __device__ int joint_rows(const Dev& d, uint32_t e, Row* out) {
     for (int x = 0; x < 3; ++x)
       for (int y = 0; y < 3; ++y) hm[x][y] += g[x][y] * fl[k];
   }
-  Row& gr = out[m++];
+  Row& gr = out[3];
+  m |= 8u;
   gr.x[0] = 0.0f; gr.x[1] = 0.0f; gr.x[2] = 0.0f;
   gr.x[3] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
   gr.x[4] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
@@ -1352,22 +1339,33 @@ __device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, 
 
 #define WAVE 64
 
+// Elements (lower-triangle index e < 21, rhs 21 + k) a row adds to, by
+// kind: a stamp touches all 27; an angular couple the angular block and
+// angular rhs; a geometric-stiffness row the angular diagonal.
+constexpr uint32_t TOUCH_STAMP = (1u << 27) - 1u;
+constexpr uint32_t TOUCH_COUPLE = (1u << 9) | (1u << 13) | (1u << 14) | (1u << 18) | (1u << 19) | (1u << 20) |
+                                  (1u << 24) | (1u << 25) | (1u << 26);
+constexpr uint32_t TOUCH_DIAG = (1u << 9) | (1u << 14) | (1u << 20);
+
 // Primal solve of one colour class, wavefront form.  A workgroup of W
 // wavefronts holds W * 64 / L bodies, L lanes per body.  Per chunk, the L
 // lanes of a body each evaluate one item (point-sides in pair order, then
-// joint-sides) into LDS; lane j of the body then owns matrix elements j,
-// j + L, ... (21 lower-triangle entries + 6 rhs) and adds every staged row
-// to them in item-then-row order — the reference's sequential order, so the
-// result does not depend on L or W.  The workgroup's systems are then
-// factored and solved by the first wavefront, one lane per body; the other
-// wavefronts retire early.  R = rows per item.
+// joint-sides) into registers and store its rows in LDS, packed in
+// item-then-row order; lane j of the body then owns matrix elements j,
+// j + L, ... (21 lower-triangle entries + 6 rhs) and adds every stored row
+// to them in that order — the reference's sequential order, so the result
+// does not depend on L or W.  The accumulation is branch-free: a row that
+// does not touch an element adds -0.0, which leaves every value (zeros'
+// signs included) unchanged.  The workgroup's systems are then factored
+// and solved by the first wavefront, one lane per body.  R = row slots per
+// item.
 template <int L, int W, int R>
 __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __restrict__ list, uint32_t count) {
   constexpr int G = WAVE / L;                 // bodies per wavefront
   constexpr int NB = G * W;                   // bodies per workgroup
   constexpr int NE = (27 + L - 1) / L;        // elements per lane
   __shared__ Row rows[NB][L * R];
-  __shared__ uint8_t nrows[NB][L];
+  __shared__ uint32_t touch[NB][L * R];
   __shared__ float sums[NB][28];
   __shared__ uint32_t chunks_s[NB];
   const int tid = threadIdx.x;
@@ -1383,43 +1381,67 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     nj = d.b.ji_off[h + 1] - j0;
   }
   const uint32_t total = np + nj;
+  // element q of this lane: matrix entry (ia, ib) or rhs entry ia
   float acc[NE];
-  int ei[NE], ej[NE];
+  int ia[NE], ib[NE];
+  bool rhs[NE];
+  uint32_t ebit[NE];
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
     int e = sub + k * L;
     int i = 0;
     if (e < 21)
       while (tri(i + 1, 0) <= e) ++i;
-    ei[k] = i;
-    ej[k] = e < 21 ? e - tri(i, 0) : 0;
-    acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, ei[k], ej[k]) : 0.0f;
+    int j = e < 21 ? e - tri(i, 0) : 0;
+    rhs[k] = e >= 21;
+    ia[k] = e < 21 ? i : (e < 27 ? e - 21 : 0);
+    ib[k] = e < 21 ? j : 0;
+    ebit[k] = e < 27 ? 1u << e : 0u;
+    acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, i, j) : 0.0f;
   }
   if (sub == 0) chunks_s[grp] = (total + L - 1) / L;
   __syncthreads();
   uint32_t nch = 0;
   for (int g = 0; g < NB; ++g) nch = chunks_s[g] > nch ? chunks_s[g] : nch;
+  const float* rowf = reinterpret_cast<const float*>(&rows[grp][0]);
   for (uint32_t c = 0; c < nch; ++c) {
     uint32_t k = c * L + sub;
-    int m = 0;
-    Row* out = &rows[grp][sub * R];
+    Row tmp[R];
+    uint32_t mask = 0;
     if (k < np)
-      m = point_rows(d, d.inc[i0 + k], out);
+      mask = point_rows<R>(d, d.inc[i0 + k], tmp);
     else if (k < total)
-      m = joint_rows(d, d.b.ji[j0 + (k - np)], out);
-    nrows[grp][sub] = (uint8_t)m;
-    __syncthreads();
-    const uint32_t lim = total > c * L ? (total - c * L < (uint32_t)L ? total - c * L : (uint32_t)L) : 0u;
-    for (uint32_t it = 0; it < lim; ++it) {
-      int mm = nrows[grp][it];
-      for (int r = 0; r < mm; ++r) {
-        const Row& row = rows[grp][it * R + r];
+      mask = joint_rows<R>(d, d.b.ji[j0 + (k - np)], tmp);
+    // pack the rows: exclusive prefix of the row counts over the body's lanes
+    const int m = __popc(mask);
+    int pre = m;
 #pragma unroll
-        for (int q = 0; q < NE; ++q) {
-          int e = sub + q * L;
-          float v;
-          if (e < 27 && row_elem(row, ei[q], ej[q], e, &v)) acc[q] += v;
-        }
+    for (int o = 1; o < L; o <<= 1) {
+      int v = __shfl_up(pre, o, L);
+      if (sub >= o) pre += v;
+    }
+    const int nrow = __shfl(pre, L - 1, L);
+    int w = pre - m;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!((mask >> r) & 1u)) continue;
+      rows[grp][w] = tmp[r];
+      touch[grp][w] = tmp[r].pen > 0.0f ? TOUCH_STAMP : (tmp[r].pen < 0.0f ? TOUCH_COUPLE : TOUCH_DIAG);
+      ++w;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int t = 0; t < nrow; ++t) {
+      const float* rw = rowf + 8 * t;
+      const float pen = rw[6], f = rw[7];
+      const uint32_t tm = touch[grp][t];
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const float a = rw[ia[q]], b = rw[ib[q]];
+        const float vm = fabsf(pen) * (a * b);
+        float v = rhs[q] ? f * a : vm;
+        v = pen == 0.0f ? a : v;
+        acc[q] += (tm & ebit[q]) ? v : -0.0f;
       }
     }
     __syncthreads();
@@ -1434,10 +1456,10 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   const uint32_t s2 = blockIdx.x * NB + tid;
   if (s2 >= count) return;
   const uint32_t hb = list[s2];
-  float L21[21], rhs[6], dx[6];
+  float L21[21], rhsv[6], dx[6];
   for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
-  for (int k = 0; k < 6; ++k) rhs[k] = -sums[tid][21 + k];
-  if (solve_6x6_dev(L21, rhs, dx)) {
+  for (int k = 0; k < 6; ++k) rhsv[k] = -sums[tid][21 + k];
+  if (solve_6x6_dev(L21, rhsv, dx)) {
     float4 p = d.b.pos[hb];
     d.b.pos[hb] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
     qf q = quat_integrate(asq(d.b.rot[hb]), f3{dx[3], dx[4], dx[5]});
