@@ -186,8 +186,9 @@ __global__ void k_grid_fill(Dev d) {
   if (h >= d.n) return;
   uint32_t bk = d.g.bucket[h];
   if (bk == 0xffffffffu) return;
-  uint32_t slot = atomicAdd(&d.g.cursor[bk], 1u);
-  d.g.items[d.g.start[bk] + slot] = h;
+  uint32_t at = d.g.start[bk] + atomicAdd(&d.g.cursor[bk], 1u);
+  d.g.items[at] = h;
+  d.g.ipos[at] = pk(xyz(d.b.pos[h]), d.b.shape[h].w);
 }
 
 // Distinct buckets of the cells in [lo, hi] (at most cap); returns count.
@@ -273,12 +274,13 @@ __device__ __forceinline__ bool sweep_may_touch(f3 p0, f3 dir, float len, float 
 }
 
 // Candidate targets of fast mover h: grid members near its swept bounding
-// sphere, then the large bodies, after the target filters and a bounding-
-// sphere sweep test (a target outside it cannot come within the touch
-// distance, so it cannot produce a hit).  Order is irrelevant: the hit is
-// the minimum of (t, target index) over all targets.
-template <bool FILL>
-__device__ uint32_t toi_targets(const Dev& d, uint32_t h, uint2* out) {
+// sphere, then the large bodies, after a bounding-sphere sweep test (a
+// target outside it cannot come within the touch distance, so it cannot
+// produce a hit) and the target filters.  Order is irrelevant: the hit is
+// the minimum of (t, target index) over all targets.  emit(o) receives
+// every target; returns the count.
+template <typename Emit>
+__device__ uint32_t toi_targets(const Dev& d, uint32_t h, Emit emit) {
   f3 delta = mul(xyz(d.b.vel[h]), K_DT);
   float len = len3(delta);
   if (len < K_NEAR_ZERO) return 0;
@@ -286,11 +288,10 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, uint2* out) {
   f3 p0 = xyz(d.b.pos[h]);
   float rm = d.b.shape[h].w;
   uint32_t m = 0;
-  auto consider = [&](uint32_t o) {
+  auto consider = [&](uint32_t o, float4 po) {
+    if (isfinite(po.w) && !sweep_may_touch(p0, dir, len, rm, xyz(po), po.w)) return;
     if (!toi_target_ok(d, h, o)) return;
-    float ro = d.b.shape[o].w;
-    if (isfinite(ro) && !sweep_may_touch(p0, dir, len, rm, xyz(d.b.pos[o]), ro)) return;
-    if (FILL) out[m] = make_uint2(h, o);
+    emit(m, o);
     ++m;
   };
   float r = rm + d.g.cell * 0.5f + 0.01f;
@@ -304,28 +305,41 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, uint2* out) {
     int nbk = gather_buckets(d, lx, ly, lz, hx, hy, hz, bks, 64);
     for (int i = 0; i < nbk; ++i) {
       uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
-      for (uint32_t k = 0; k < c0; ++k) consider(d.g.items[s0 + k]);
+      for (uint32_t k = 0; k < c0; ++k) consider(d.g.items[s0 + k], d.g.ipos[s0 + k]);
     }
   } else {
-    for (uint32_t o = 0; o < d.n; ++o)
-      if (!d.g.is_large[o]) consider(o);
+    for (uint32_t k = 0; k < d.g.start[d.g.nbuckets]; ++k) consider(d.g.items[k], d.g.ipos[k]);
   }
-  for (uint32_t k = 0; k < d.g.nlarge; ++k) consider(d.g.large[k]);
+  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
+    uint32_t o = d.g.large[k];
+    consider(o, pk(xyz(d.b.pos[o]), d.b.shape[o].w));
+  }
   return m;
 }
 
+// Counting pass; the first TOI_CACHE targets are kept for the fill pass.
 __global__ void k_toi_count(Dev d) {
   uint32_t h = gid();
   if (h >= d.n) return;
   d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
   d.toi_best[h] = ~0ull;
-  d.toi_cnt[h] = d.fast[h] ? toi_targets<false>(d, h, nullptr) : 0u;
+  uint32_t* cache = d.toi_cache + (size_t)h * TOI_CACHE;
+  d.toi_cnt[h] = d.fast[h] ? toi_targets(d, h, [&](uint32_t i, uint32_t o) {
+    if (i < TOI_CACHE) cache[i] = o;
+  }) : 0u;
 }
 
 __global__ void k_toi_fill(Dev d) {
   uint32_t h = gid();
   if (h >= d.n || !d.fast[h]) return;
-  toi_targets<true>(d, h, d.toi_pairs + d.toi_off[h]);
+  uint2* out = d.toi_pairs + d.toi_off[h];
+  uint32_t m = d.toi_cnt[h];
+  if (m <= TOI_CACHE) {
+    const uint32_t* cache = d.toi_cache + (size_t)h * TOI_CACHE;
+    for (uint32_t i = 0; i < m; ++i) out[i] = make_uint2(h, cache[i]);
+  } else {
+    toi_targets(d, h, [&](uint32_t i, uint32_t o) { out[i] = make_uint2(h, o); });
+  }
 }
 
 __device__ __forceinline__ unsigned long long toi_key(float t, uint32_t o) {
@@ -442,17 +456,28 @@ __global__ void k_apply_wake(Dev d) {
 // (rows.rs:432-479): scene, layer/mask, pin-joint no-collide, bounding
 // spheres within GEN_MARGIN.  Output is grouped by a and sorted by b.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool prefilter(const Dev& d, uint32_t a, uint32_t b) {
+// The non-geometric prefilters (scene, layer/mask, pin-joint no-collide).
+__device__ __forceinline__ bool prefilter_ids(const Dev& d, uint32_t a, uint32_t b) {
   if (d.b.scene[a] != d.b.scene[b]) return false;
   if (!pair_allowed_dev(d, a, b)) return false;
-  if (!d.b.trigger[a] && !d.b.trigger[b] && d.b.jp_off[a] != d.b.jp_off[a + 1] && jointed_dev(d, a, b)) return false;
-  float ra = d.b.shape[a].w, rb = d.b.shape[b].w;
-  return !(len3(sub(xyz(d.b.pos[a]), xyz(d.b.pos[b]))) > ra + rb + K_GEN_MARGIN);
+  return !(!d.b.trigger[a] && !d.b.trigger[b] && d.b.jp_off[a] != d.b.jp_off[a + 1] && jointed_dev(d, a, b));
+}
+// Bounding spheres within GEN_MARGIN.
+__device__ __forceinline__ bool spheres_near(f3 pa, float ra, f3 pb, float rb) {
+  return !(len3(sub(pa, pb)) > ra + rb + K_GEN_MARGIN);
+}
+__device__ __forceinline__ bool prefilter(const Dev& d, uint32_t a, uint32_t b) {
+  return spheres_near(xyz(d.b.pos[a]), d.b.shape[a].w, xyz(d.b.pos[b]), d.b.shape[b].w) && prefilter_ids(d, a, b);
 }
 
-template <bool FILL>
-__device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, uint2* out) {
+// Candidates (a, o > a) of small body a: grid members of the 27 cells
+// around it (sphere test on the packed item data first), then the large
+// bodies.  emit(i, o) receives the i-th candidate; returns the count.
+template <typename Emit>
+__device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, Emit emit) {
   float4 p = d.b.pos[a];
+  const f3 pa = xyz(p);
+  const float ra = d.b.shape[a].w;
   int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
   uint32_t bks[27];
   int nbk = gather_buckets(d, cx - 1, cy - 1, cz - 1, cx + 1, cy + 1, cz + 1, bks, 27);
@@ -460,27 +485,27 @@ __device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, uint2* ou
   for (int i = 0; i < nbk; ++i) {
     uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
     for (uint32_t k = 0; k < c0; ++k) {
+      float4 q = d.g.ipos[s0 + k];
+      if (!spheres_near(pa, ra, xyz(q), q.w)) continue;
       uint32_t o = d.g.items[s0 + k];
-      if (o > a && prefilter(d, a, o)) {
-        if (FILL) out[m] = make_uint2(a, o);
-        ++m;
-      }
+      if (o > a && prefilter_ids(d, a, o)) emit(m++, o);
     }
   }
   for (uint32_t k = 0; k < d.g.nlarge; ++k) {
     uint32_t o = d.g.large[k];
-    if (o > a && prefilter(d, a, o)) {
-      if (FILL) out[m] = make_uint2(a, o);
-      ++m;
-    }
+    if (o > a && prefilter(d, a, o)) emit(m++, o);
   }
   return m;
 }
 
+// Counting pass; the first BP_CACHE candidates are kept for the fill pass.
 __global__ void k_bp_count(Dev d, uint32_t* cnt) {
   uint32_t a = gid();
   if (a >= d.n) return;
-  cnt[a] = d.g.is_large[a] ? 0u : bp_small<false>(d, a, nullptr);
+  uint32_t* cache = d.bp_cache + (size_t)a * BP_CACHE;
+  cnt[a] = d.g.is_large[a] ? 0u : bp_small(d, a, [&](uint32_t i, uint32_t o) {
+    if (i < BP_CACHE) cache[i] = o;
+  });
 }
 
 // Large bodies pair with every later body passing the prefilter.
@@ -503,7 +528,13 @@ __global__ void k_bp_fill(Dev d) {
   uint32_t a = gid();
   if (a >= d.n || d.g.is_large[a]) return;
   uint2* out = d.C.ab + d.cand_off[a];
-  uint32_t m = bp_small<true>(d, a, out);
+  const uint32_t m = d.cand_off[a + 1] - d.cand_off[a];
+  if (m <= BP_CACHE) {
+    const uint32_t* cache = d.bp_cache + (size_t)a * BP_CACHE;
+    for (uint32_t i = 0; i < m; ++i) out[i] = make_uint2(a, cache[i]);
+  } else {
+    bp_small(d, a, [&](uint32_t i, uint32_t o) { out[i] = make_uint2(a, o); });
+  }
   for (uint32_t i = 1; i < m; ++i) {  // insertion sort by b
     uint2 v = out[i];
     uint32_t j = i;

@@ -43,6 +43,8 @@
 #define K_MAX_POINTS 8
 #define COLOUR_BATCH 4  // colouring rounds launched per host check
 #define TOI_LANES 16    // lanes cooperating on one TOI cast
+#define BP_CACHE 16     // broadphase candidates kept per body between count and fill
+#define TOI_CACHE 32    // TOI targets kept per mover between count and fill
 
 enum { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
 
@@ -119,6 +121,7 @@ struct Grid {
   uint32_t* start;   // per bucket (exclusive scan)
   uint32_t* cursor;  // per bucket fill cursor
   uint32_t* items;   // body ids bucketed
+  float4* ipos;      // per item: position xyz, bound radius (same order as items)
   uint32_t* bucket;  // per body bucket id (0xffffffff = large / not gridded)
   uint32_t nbuckets; // power of two
   float cell;
@@ -180,6 +183,8 @@ struct Dev {
   uint32_t* toi_off;   // n+1
   uint2* toi_pairs;    // (mover, target) casts
   float4* toi_res;     // per cast: normal xyz, w = travel to the hit (< 0: none)
+  uint32_t* bp_cache;  // n x BP_CACHE candidates found by the counting pass
+  uint32_t* toi_cache; // n x TOI_CACHE targets found by the counting pass
   unsigned long long* toi_best;  // per mover min (t bits << 32 | target)
   uint8_t* fast;
 };

@@ -142,7 +142,8 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   while (nb < 2 * n) nb <<= 1;
   d.g.nbuckets = nb;
   dalloc(&d.g.cnt, nb); dalloc(&d.g.start, nb + 1); dalloc(&d.g.cursor, nb);
-  dalloc(&d.g.items, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
+  dalloc(&d.g.items, n); dalloc(&d.g.ipos, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
+  dalloc(&d.bp_cache, (size_t)n * BP_CACHE); dalloc(&d.toi_cache, (size_t)n * TOI_CACHE);
   d.g.large = nullptr;
   d.lflag = nullptr;
   d.lrank = nullptr;
@@ -230,7 +231,7 @@ static void free_all(oavbd_ctx* c) {
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
-  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.bucket); dfree(d.g.large);
+  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large);
   dfree(d.g.is_large);
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
