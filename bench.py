@@ -47,6 +47,17 @@ def cpu_baseline(scene, seconds, max_steps):
                       f"steps 1-{done} of the same 100k-box workload from its initial state, {dt:.1f} s"}
 
 
+def measured_traffic():
+    """HBM bytes per k_primal launch on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a plain copy of
+    known size); None if that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_primal_traffic_100k.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +71,9 @@ def main():
     ap.add_argument("--waves", type=int, default=1, help="primal wavefronts per workgroup")
     ap.add_argument("--sweep-shapes", type=int, default=0,
                     help="after warmup, time this many steps per primal launch shape and exit")
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
+                         "for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the 100k-box scene fits one MI355X and multi-GPU "
@@ -68,6 +82,15 @@ def main():
 
     from ornis_amd import scenes
     from ornis_amd.avbd import AvbdEngine
+
+    if args.copy_check:
+        import torch
+        x = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
 
     scene = scenes.box_drop_lattice()
     n_dyn = int((scene["body_type"] == 1).sum())
@@ -109,6 +132,7 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = n_dyn * args.steps / wall
     achieved = kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9 if kt["primal_ms"] > 0 else 0.0
+    traffic = measured_traffic()
     dual_gbs = kt["dual_bytes"] / (kt["dual_ms"] * 1e-3) / 1e9 if kt["dual_ms"] > 0 else 0.0
     cpu = None
     if not args.no_cpu:
@@ -130,7 +154,7 @@ def main():
                                "dt 1/60, 10 iterations, FP32", "bodies": n_dyn, "iters": 10,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_primal",
                      "bytes_per_launch": kt["primal_bytes"] / max(kt["primal_launches"], 1),
                      "avg_launch_us": 1e3 * kt["primal_ms"] / max(kt["primal_launches"], 1)},

@@ -60,14 +60,20 @@ __device__ __forceinline__ void edge(int i, int* c0, int* c1) {
   }
 }
 
-// separating_axis over the group.
+// separating_axis over the group.  With sweep != nullptr it also tells
+// whether box a, translated along `dir` by up to `rem`, stays farther than
+// `reach` from b along one of the axes: along a fixed axis the projected
+// separation is linear in the translation, so its minimum over the sweep is
+// at an end or zero where it changes sign; `slack` bounds the rounding.
 template <int G>
-__device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, int s) {
+__device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, int s, bool* sweep = nullptr,
+                     f3 dir = f3{0.0f, 0.0f, 0.0f}, float rem = 0.0f, float reach = 0.0f, float slack = 0.0f) {
   m3c ra = mat3_from_quat(a.rot), rb = mat3_from_quat(b.rot);
   f3 delta = sub(b.pos, a.pos);
   float best = -__builtin_inff();
   int bk = 99;
   f3 bn{0.0f, 1.0f, 0.0f};
+  bool clear = false;
   for (int k = s; k < 15; k += G) {
     f3 n;
     if (k < 3) {
@@ -90,6 +96,17 @@ __device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, i
       bk = k;
       bn = proj >= 0.0f ? n : neg(n);
     }
+    if (sweep) {
+      float q2 = proj - rem * dot3(dir, n);
+      bool same = (proj > 0.0f && q2 > 0.0f) || (proj < 0.0f && q2 < 0.0f);
+      float lo = same ? fminf(fabsf(proj), fabsf(q2)) : 0.0f;
+      if (lo - ea - eb - slack > reach) clear = true;
+    }
+  }
+  if (sweep) {
+    int c = clear ? 1 : 0;
+    for (int m = G / 2; m > 0; m >>= 1) c |= xori<G>(c, m);
+    *sweep = c != 0;
   }
   // larger gap wins; equal gaps go to the earlier axis
   float g = best;

@@ -361,7 +361,9 @@ __device__ float cast_group(const shp& mover, f3 dir, float len, const shp& targ
     shp m = mover;
     m.pos = add(mover.pos, mul(dir, t));
     f3 sn;
-    float gap = grp::sat<G>(m, m.p, target, target.p, &sn, s);
+    bool clear;
+    float gap = grp::sat<G>(m, m.p, target, target.p, &sn, s, &clear, dir, len - t, K_SHAPE_TOUCH, slack);
+    if (clear) return -1.0f;  // no iterate can come within the touch distance
     float lo = gap - slack;
     if (lo > K_SHAPE_TOUCH && t + (lo - K_SHAPE_TOUCH * 0.5f) >= len) return -1.0f;
     dist_t r = grp::box_box<G>(m, m.p, target, target.p, gap, sn, s);
@@ -1415,7 +1417,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   // element q of this lane: matrix entry (ia, ib) or rhs entry ia
   float acc[NE];
   int ia[NE], ib[NE];
-  bool rhs[NE];
+  bool rhs[NE], skip_couple[NE];
   uint32_t ebit[NE];
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
@@ -1428,6 +1430,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     ia[k] = e < 21 ? i : (e < 27 ? e - 21 : 0);
     ib[k] = e < 21 ? j : 0;
     ebit[k] = e < 27 ? 1u << e : 0u;
+    skip_couple[k] = e < 21 ? j < 3 : e - 21 < 3;
     acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, i, j) : 0.0f;
   }
   if (sub == 0) chunks_s[grp] = (total + L - 1) / L;
@@ -1452,6 +1455,11 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
       if (sub >= o) pre += v;
     }
     const int nrow = __shfl(pre, L - 1, L);
+    // rows of point items come first: the lanes below np - c * L hold points
+    const int64_t pl64 = (int64_t)np - (int64_t)c * L;
+    const int pl = pl64 <= 0 ? 0 : (pl64 >= L ? L : (int)pl64);
+    const int npr = __shfl(pre, pl > 0 ? pl - 1 : 0, L);
+    const int nprow = pl > 0 ? npr : 0;
     int w = pre - m;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -1461,8 +1469,23 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
       ++w;
     }
     __syncthreads();
+    // contact rows: stamps (and angular couples when R == 6)
 #pragma unroll 2
-    for (int t = 0; t < nrow; ++t) {
+    for (int t = 0; t < nprow; ++t) {
+      const float* rw = rowf + 8 * t;
+      const float pen = rw[6], f = rw[7];
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        const float a = rw[ia[q]], b = rw[ib[q]];
+        const float sc = rhs[q] ? f : fabsf(pen);
+        const float bb = rhs[q] ? 1.0f : b;
+        float v = sc * (a * bb);
+        if constexpr (R >= 6) v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
+        acc[q] += v;
+      }
+    }
+    // joint rows: stamps and geometric-stiffness diagonals
+    for (int t = nprow; t < nrow; ++t) {
       const float* rw = rowf + 8 * t;
       const float pen = rw[6], f = rw[7];
       const uint32_t tm = touch[grp][t];
