@@ -5,7 +5,7 @@
 // so that witnesses and gaps are bit-identical:
 //   seg_seg_closest      crates/physics/src/collision/distance.rs:75-113
 //   point_obb_closest    distance.rs:117-138
-//   obb_corners          distance.rs:142-156
+//   box_corner           obb_corners, distance.rs:142-156
 //   sphere_sphere        distance.rs:180-189
 //   sphere_obb           distance.rs:192-216
 //   halfspace_distance   distance.rs:607-641 (+ gjk::support, gjk.rs:68-79)
@@ -95,21 +95,6 @@ HD f3 point_obb_closest(f3 p, f3 pos, f3 half, qf rot) {
     q = cl;
   }
   return add(pos, qrot(rot, q));
-}
-
-HD void obb_corners(f3 pos, f3 half, qf rot, f3 out[8]) {
-  f3 x = qrot(rot, f3{1.0f * half.x, 0.0f * half.x, 0.0f * half.x});
-  f3 y = qrot(rot, f3{0.0f * half.y, 1.0f * half.y, 0.0f * half.y});
-  f3 z = qrot(rot, f3{0.0f * half.z, 0.0f * half.z, 1.0f * half.z});
-  f3 px = add(pos, x), mx = sub(pos, x);
-  out[0] = add(add(px, y), z);
-  out[1] = sub(add(px, y), z);
-  out[2] = add(sub(px, y), z);
-  out[3] = sub(sub(px, y), z);
-  out[4] = add(add(mx, y), z);
-  out[5] = sub(add(mx, y), z);
-  out[6] = add(sub(mx, y), z);
-  out[7] = sub(sub(mx, y), z);
 }
 
 HD dist_t sphere_sphere(const shp& a, float ra, const shp& b, float rb) {
@@ -214,40 +199,74 @@ HD void support_face(const shp& body, f3 half, f3 dir, shp* out, f3* ext) {
   *ext = e;
 }
 
+// Box axes scaled by the half extents, as obb_corners (distance.rs:142-156)
+// forms them.
+struct box_axes_t {
+  f3 x, y, z;
+};
+HD box_axes_t box_axes(f3 half, qf rot) {
+  return box_axes_t{qrot(rot, f3{1.0f * half.x, 0.0f * half.x, 0.0f * half.x}),
+                    qrot(rot, f3{0.0f * half.y, 1.0f * half.y, 0.0f * half.y}),
+                    qrot(rot, f3{0.0f * half.z, 0.0f * half.z, 1.0f * half.z})};
+}
+// Corner k of obb_corners: bit 2 selects -x, bit 1 -y, bit 0 -z, summed in
+// the same order, so every corner is bit-identical to the reference's.
+HD f3 box_corner(f3 pos, const box_axes_t& e, int k) {
+  f3 c = (k & 4) ? sub(pos, e.x) : add(pos, e.x);
+  c = (k & 2) ? sub(c, e.y) : add(c, e.y);
+  return (k & 1) ? sub(c, e.z) : add(c, e.z);
+}
+// Box edge i as the corner pair (ea_[i], eb_[i]) of feature_distance:
+// ea_ = {0,1,2,3, 0,1,4,5, 0,2,4,6}, eb_ = {4,5,6,7, 2,3,6,7, 1,3,5,7}.
+HD void box_edge(int i, int* c0, int* c1) {
+  if (i < 4) {
+    *c0 = i;
+    *c1 = i + 4;
+  } else if (i < 8) {
+    int m = i - 4;
+    *c0 = (m & 1) | ((m & 2) << 1);
+    *c1 = *c0 + 2;
+  } else {
+    *c0 = 2 * (i - 8);
+    *c1 = *c0 + 1;
+  }
+}
+
+// Candidate c of feature_distance's scan (0-7 corners of a against b,
+// 8-15 corners of b against a, 16-159 edge pairs, a's edge major): the
+// closest points x (on a) and y (on b) and |y - x|^2.
+HD float feature_candidate(const shp& a, f3 ha, const box_axes_t& ea, const shp& b, f3 hb, const box_axes_t& eb,
+                           int c, f3* x, f3* y) {
+  if (c < 8) {
+    *x = box_corner(a.pos, ea, c);
+    *y = point_obb_closest(*x, b.pos, hb, b.rot);
+  } else if (c < 16) {
+    *y = box_corner(b.pos, eb, c - 8);
+    *x = point_obb_closest(*y, a.pos, ha, a.rot);
+  } else {
+    int i = (c - 16) / 12, j = (c - 16) % 12;
+    int a0, a1, b0, b1;
+    box_edge(i, &a0, &a1);
+    box_edge(j, &b0, &b1);
+    seg_seg_closest(box_corner(a.pos, ea, a0), box_corner(a.pos, ea, a1), box_corner(b.pos, eb, b0),
+                    box_corner(b.pos, eb, b1), x, y);
+  }
+  return len2(sub(*y, *x));
+}
+
+// Closest features of two boxes: the first minimum over the fixed scan.
 HD dist_t feature_distance(const shp& a, f3 ha, const shp& b, f3 hb) {
-  f3 ca[8], cb[8];
-  obb_corners(a.pos, ha, a.rot, ca);
-  obb_corners(b.pos, hb, b.rot, cb);
+  const box_axes_t ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
   dist_t res{__builtin_inff(), a.pos, b.pos};
   float best = __builtin_inff();
-  for (int i = 0; i < 8; ++i) {
-    f3 y = point_obb_closest(ca[i], b.pos, hb, b.rot);
-    float sq = len2(sub(y, ca[i]));
+  for (int c = 0; c < 160; ++c) {
+    f3 x, y;
+    float sq = feature_candidate(a, ha, ea, b, hb, eb, c, &x, &y);
     if (sq < best) {
       best = sq;
-      res = dist_t{sqrtf(sq), ca[i], y};
+      res = dist_t{sqrtf(sq), x, y};
     }
   }
-  for (int i = 0; i < 8; ++i) {
-    f3 x = point_obb_closest(cb[i], a.pos, ha, a.rot);
-    float sq = len2(sub(cb[i], x));
-    if (sq < best) {
-      best = sq;
-      res = dist_t{sqrtf(sq), x, cb[i]};
-    }
-  }
-  const unsigned char ea_[12] = {0, 1, 2, 3, 0, 1, 4, 5, 0, 2, 4, 6};
-  const unsigned char eb_[12] = {4, 5, 6, 7, 2, 3, 6, 7, 1, 3, 5, 7};
-  for (int i = 0; i < 12; ++i)
-    for (int j = 0; j < 12; ++j) {
-      f3 x, y;
-      seg_seg_closest(ca[ea_[i]], ca[eb_[i]], cb[ea_[j]], cb[eb_[j]], &x, &y);
-      float sq = len2(sub(y, x));
-      if (sq < best) {
-        best = sq;
-        res = dist_t{sqrtf(sq), x, y};
-      }
-    }
   return res;
 }
 

@@ -29,37 +29,6 @@ __device__ __forceinline__ f3 from_lane(f3 v, int src) {
   return f3{__shfl(v.x, src, G), __shfl(v.y, src, G), __shfl(v.z, src, G)};
 }
 
-// Box axes scaled by the half extents, as obb_corners forms them.
-struct axes {
-  f3 x, y, z;
-};
-__device__ __forceinline__ axes box_axes(f3 half, qf rot) {
-  return axes{qrot(rot, f3{1.0f * half.x, 0.0f * half.x, 0.0f * half.x}),
-              qrot(rot, f3{0.0f * half.y, 1.0f * half.y, 0.0f * half.y}),
-              qrot(rot, f3{0.0f * half.z, 0.0f * half.z, 1.0f * half.z})};
-}
-// Corner k of obb_corners: bit 2 selects -x, bit 1 -y, bit 0 -z, summed in
-// the same order.
-__device__ __forceinline__ f3 corner(f3 pos, const axes& e, int k) {
-  f3 c = (k & 4) ? sub(pos, e.x) : add(pos, e.x);
-  c = (k & 2) ? sub(c, e.y) : add(c, e.y);
-  return (k & 1) ? sub(c, e.z) : add(c, e.z);
-}
-// Box edge i as the corner pair (ea_[i], eb_[i]) of feature_distance.
-__device__ __forceinline__ void edge(int i, int* c0, int* c1) {
-  if (i < 4) {
-    *c0 = i;
-    *c1 = i + 4;
-  } else if (i < 8) {
-    int m = i - 4;
-    *c0 = (m & 1) | ((m & 2) << 1);
-    *c1 = *c0 + 2;
-  } else {
-    *c0 = 2 * (i - 8);
-    *c1 = *c0 + 1;
-  }
-}
-
 // separating_axis over the group.  With sweep != nullptr it also tells
 // whether box a, translated along `dir` by up to `rem`, stays farther than
 // `reach` from b along one of the axes: along a fixed axis the projected
@@ -131,30 +100,13 @@ __device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, i
 // feature_distance over the group.
 template <int G>
 __device__ dist_t feature(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
-  const axes ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
+  const box_axes_t ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
   float best = __builtin_inff();
   int bc = 0x7fffffff;
   f3 bx = a.pos, by = b.pos;
   for (int c = s; c < 160; c += G) {
     f3 x, y;
-    float sq;
-    if (c < 8) {
-      x = corner(a.pos, ea, c);
-      y = point_obb_closest(x, b.pos, hb, b.rot);
-      sq = len2(sub(y, x));
-    } else if (c < 16) {
-      y = corner(b.pos, eb, c - 8);
-      x = point_obb_closest(y, a.pos, ha, a.rot);
-      sq = len2(sub(y, x));
-    } else {
-      int i = (c - 16) / 12, j = (c - 16) % 12;
-      int a0, a1, b0, b1;
-      edge(i, &a0, &a1);
-      edge(j, &b0, &b1);
-      seg_seg_closest(corner(a.pos, ea, a0), corner(a.pos, ea, a1), corner(b.pos, eb, b0), corner(b.pos, eb, b1), &x,
-                      &y);
-      sq = len2(sub(y, x));
-    }
+    float sq = feature_candidate(a, ha, ea, b, hb, eb, c, &x, &y);
     if (sq < best) {
       best = sq;
       bc = c;

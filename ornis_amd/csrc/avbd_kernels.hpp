@@ -153,6 +153,12 @@ __global__ void k_begin(Dev d) {
 // every candidate it yields is re-tested exactly)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int cell_of(float x, float cell) { return (int)floorf(x / cell); }
+// Cell coordinates packed 21 bits each (cells 2^21 apart share a key; a
+// search never spans that far, and every candidate is re-tested exactly).
+__device__ __forceinline__ unsigned long long cell_key(int x, int y, int z) {
+  return ((unsigned long long)((uint32_t)x & 0x1fffffu) << 42) | ((unsigned long long)((uint32_t)y & 0x1fffffu) << 21) |
+         (unsigned long long)((uint32_t)z & 0x1fffffu);
+}
 __device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t nb) {
   uint32_t h = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u;
   h ^= h >> 13;
@@ -176,7 +182,9 @@ __global__ void k_grid_count(Dev d) {
     return;
   }
   float4 p = d.b.pos[h];
-  uint32_t bk = bucket_of(cell_of(p.x, d.g.cell), cell_of(p.y, d.g.cell), cell_of(p.z, d.g.cell), d.g.nbuckets);
+  int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
+  uint32_t bk = bucket_of(cx, cy, cz, d.g.nbuckets);
+  d.g.hkey[h] = cell_key(cx, cy, cz);
   d.g.bucket[h] = bk;
   atomicAdd(&d.g.cnt[bk], 1u);
 }
@@ -189,6 +197,7 @@ __global__ void k_grid_fill(Dev d) {
   uint32_t at = d.g.start[bk] + atomicAdd(&d.g.cursor[bk], 1u);
   d.g.items[at] = h;
   d.g.ipos[at] = pk(xyz(d.b.pos[h]), d.b.shape[h].w);
+  d.g.ikey[at] = d.g.hkey[h];
 }
 
 // Distinct buckets of the cells in [lo, hi] (at most cap); returns count.
@@ -301,12 +310,17 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, Emit emit) {
   int lz = cell_of(fminf(p0.z, p1.z) - r, d.g.cell), hz = cell_of(fmaxf(p0.z, p1.z) + r, d.g.cell);
   long span = (long)(hx - lx + 1) * (long)(hy - ly + 1) * (long)(hz - lz + 1);
   if (span > 0 && span <= 64) {
-    uint32_t bks[64];
-    int nbk = gather_buckets(d, lx, ly, lz, hx, hy, hz, bks, 64);
-    for (int i = 0; i < nbk; ++i) {
-      uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
-      for (uint32_t k = 0; k < c0; ++k) consider(d.g.items[s0 + k], d.g.ipos[s0 + k]);
-    }
+    // each cell's members, found in its bucket by their cell key (buckets
+    // shared by several visited cells are then scanned once per cell)
+    for (int x = lx; x <= hx; ++x)
+      for (int y = ly; y <= hy; ++y)
+        for (int z = lz; z <= hz; ++z) {
+          uint32_t bk = bucket_of(x, y, z, d.g.nbuckets);
+          unsigned long long key = cell_key(x, y, z);
+          uint32_t s0 = d.g.start[bk], c0 = d.g.cnt[bk];
+          for (uint32_t k = 0; k < c0; ++k)
+            if (d.g.ikey[s0 + k] == key) consider(d.g.items[s0 + k], d.g.ipos[s0 + k]);
+        }
   } else {
     for (uint32_t k = 0; k < d.g.start[d.g.nbuckets]; ++k) consider(d.g.items[k], d.g.ipos[k]);
   }
@@ -561,24 +575,50 @@ __global__ void k_bp_large_fill(Dev d) {
 // ---------------------------------------------------------------------------
 // narrowphase: discover_pair (rows.rs:432-756) for every candidate
 // ---------------------------------------------------------------------------
+// Index of pair (a, b) in the previous step's table (-1 if absent): a
+// binary search for b in a's run of that table.
 __device__ __forceinline__ int find_old_pair(const Dev& d, uint32_t a, uint32_t b) {
-  uint64_t key = ((uint64_t)a << 32) | b;
-  int lo = 0, hi = (int)d.npairs_old;
+  uint32_t lo = d.poff_old[a], hi = d.poff_old[a + 1];
+  const uint32_t end = hi;
   while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    uint2 ab = d.Pold.ab[mid];
-    uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
-    if (k < key)
+    uint32_t mid = (lo + hi) >> 1;
+    if (d.Pold.ab[mid].y < b)
       lo = mid + 1;
     else
       hi = mid;
   }
-  if (lo < (int)d.npairs_old) {
-    uint2 ab = d.Pold.ab[lo];
-    if (ab.x == a && ab.y == b) return lo;
-  }
-  return -1;
+  return (lo < end && d.Pold.ab[lo].y == b) ? (int)lo : -1;
 }
+
+// First pair of every body's a-run: pairs are grouped by their first body
+// in body order, so the run of h starts where h's accepted candidates do.
+__global__ void k_pair_offsets(Dev d) {
+  uint32_t h = gid();
+  if (h <= d.n) d.poff[h] = d.acc_off[d.cand_off[h]];
+}
+
+// Fresh contact points of one candidate, kept in registers (fixed slots,
+// static indices).
+struct FreshPts {
+  f3 a[K_MAX_POINTS], b[K_MAX_POINTS];
+  int n;
+  __device__ __forceinline__ bool dup(f3 x, f3 y) const {
+    bool r = false;
+#pragma unroll
+    for (int i = 0; i < K_MAX_POINTS; ++i)
+      if (i < n && len3(sub(a[i], x)) < K_POINT_MATCH && len3(sub(b[i], y)) < K_POINT_MATCH) r = true;
+    return r;
+  }
+  __device__ __forceinline__ void push(f3 x, f3 y) {
+#pragma unroll
+    for (int i = 0; i < K_MAX_POINTS; ++i)
+      if (i == n) {
+        a[i] = x;
+        b[i] = y;
+      }
+    ++n;
+  }
+};
 
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   uint32_t c = gid();
@@ -642,13 +682,8 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   float plane_b = B.kind == SH_BOX ? dot3(normal, B.pos) - (-1.0f) * dot3(abs3(qrot(qconj(B.rot), normal)), B.p)
                                    : dot3(normal, dd.pb);
   qf inv_a = qconj(A.rot), inv_b = qconj(B.rot);
-  f3 fra[K_MAX_POINTS], frb[K_MAX_POINTS];
-  int nf = 0;
-  auto dup = [&](f3 x, f3 y) {
-    for (int i = 0; i < nf; ++i)
-      if (len3(sub(fra[i], x)) < K_POINT_MATCH && len3(sub(frb[i], y)) < K_POINT_MATCH) return true;
-    return false;
-  };
+  FreshPts fp;
+  fp.n = 0;
   if (!separated) {
     for (int side = 0; side < 2; ++side) {
       const shp& body = side == 0 ? A : B;
@@ -658,7 +693,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
       f3 pc = side == 0 ? pa_c : pb_c;
       f3 sn = smul(sign, normal);
       float slop = K_EXPAND_SLOP + fmax_(-sgn, 0.0f);
-      for (int ci = 0; ci < 8 && nf < K_MAX_POINTS; ++ci) {
+      for (int ci = 0; ci < 8 && fp.n < K_MAX_POINTS; ++ci) {
         float sx = (ci & 4) ? 1.0f : -1.0f, sy = (ci & 2) ? 1.0f : -1.0f, sz = (ci & 1) ? 1.0f : -1.0f;
         f3 corner{sx * body.p.x, sy * body.p.y, sz * body.p.z};
         f3 world = add(body.pos, qrot(body.rot, corner));
@@ -677,32 +712,22 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
         }
         f3 ral = qrot(inv_a, sub(pa, A.pos));
         f3 rbl = qrot(inv_b, sub(pb, B.pos));
-        if (dup(ral, rbl)) continue;
-        fra[nf] = ral;
-        frb[nf] = rbl;
-        ++nf;
+        if (fp.dup(ral, rbl)) continue;
+        fp.push(ral, rbl);
       }
     }
-    if (nf >= 2 && nf < K_MAX_POINTS) {
+    if (fp.n >= 2 && fp.n < K_MAX_POINTS) {
       f3 mid = ra < rb ? pa_c : (rb < ra ? pb_c : mul(add(pa_c, pb_c), 0.5f));
       f3 pa = add(mid, mul(normal, plane_a - dot3(normal, mid)));
       f3 pb = add(mid, mul(normal, plane_b - dot3(normal, mid)));
       f3 ral = qrot(inv_a, sub(pa, A.pos));
       f3 rbl = qrot(inv_b, sub(pb, B.pos));
-      if (!dup(ral, rbl)) {
-        fra[nf] = ral;
-        frb[nf] = rbl;
-        ++nf;
-      }
+      if (!fp.dup(ral, rbl)) fp.push(ral, rbl);
     }
-    if (nf < 3) {
+    if (fp.n < 3) {
       f3 ral = qrot(inv_a, sub(dd.pa, A.pos));
       f3 rbl = qrot(inv_b, sub(dd.pb, B.pos));
-      if (!dup(ral, rbl)) {
-        fra[nf] = ral;
-        frb[nf] = rbl;
-        ++nf;
-      }
+      if (!fp.dup(ral, rbl)) fp.push(ral, rbl);
     }
   }
   bool wake;
@@ -719,12 +744,14 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
     }
   }
   d.C.acc[c] = 1u;
-  d.C.npts[c] = (uint32_t)nf;
+  d.C.npts[c] = (uint32_t)fp.n;
   d.C.disc0[c] = pk(normal, sgn);
   d.C.disc1[c] = make_float4(mu1, mu1, __int_as_float(old), wake ? 1.0f : 0.0f);
-  for (int k = 0; k < nf; ++k) {
-    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fra[k].x, fra[k].y, fra[k].z, frb[k].x);
-    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(frb[k].y, frb[k].z, 0.0f, 0.0f);
+#pragma unroll
+  for (int k = 0; k < K_MAX_POINTS; ++k) {
+    if (k >= fp.n) break;
+    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fp.a[k].x, fp.a[k].y, fp.a[k].z, fp.b[k].x);
+    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(fp.b[k].y, fp.b[k].z, 0.0f, 0.0f);
   }
 }
 
@@ -1358,7 +1385,11 @@ __device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, 
   if (e < 21) {
     if (i < 3) return i == j ? m_dt2 : 0.0f;
     if (j < 3) return 0.0f;
-    return iw[i - 3][j - 3] / (K_DT * K_DT);
+    // row i - 3, column j - 3 of iw, selected without indexing a private
+    // array by a runtime value (which would go through scratch memory)
+    f3 r = i == 3 ? f3{iw[0][0], iw[0][1], iw[0][2]} : (i == 4 ? f3{iw[1][0], iw[1][1], iw[1][2]}
+                                                              : f3{iw[2][0], iw[2][1], iw[2][2]});
+    return comp(r, j - 3) / (K_DT * K_DT);
   }
   int k = e - 21;
   if (k < 3) {

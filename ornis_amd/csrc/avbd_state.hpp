@@ -122,6 +122,8 @@ struct Grid {
   uint32_t* cursor;  // per bucket fill cursor
   uint32_t* items;   // body ids bucketed
   float4* ipos;      // per item: position xyz, bound radius (same order as items)
+  unsigned long long* ikey;  // per item: its cell, packed (cell_key)
+  unsigned long long* hkey;  // per body: its cell, packed
   uint32_t* bucket;  // per body bucket id (0xffffffff = large / not gridded)
   uint32_t nbuckets; // power of two
   float cell;
@@ -162,6 +164,8 @@ struct Dev {
   uint32_t* blist;
   uint32_t* cand_off;  // n+1 candidate offsets per first body
   uint32_t* acc_off;   // ncand+1 exclusive scan of accepted
+  uint32_t* poff;      // n+1 first pair of each body's a-run in the current table
+  uint32_t* poff_old;  // n+1 the same for the previous step's table
   uint32_t* pt_off;    // ncand+1 exclusive scan of fresh points
   uint32_t* ptcnt;     // per body point-side counts
   uint8_t* sw;         // body is swept this step (dynamic, finite mass, awake)

@@ -129,6 +129,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.counters, CTR_COUNT);
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
+  dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1);
   dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
   dalloc(&d.nbr_cur, n + 1); dalloc(&d.remain, COLOUR_BATCH);
   d.nbr = nullptr;
@@ -142,7 +143,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   while (nb < 2 * n) nb <<= 1;
   d.g.nbuckets = nb;
   dalloc(&d.g.cnt, nb); dalloc(&d.g.start, nb + 1); dalloc(&d.g.cursor, nb);
-  dalloc(&d.g.items, n); dalloc(&d.g.ipos, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
+  dalloc(&d.g.items, n); dalloc(&d.g.ipos, n); dalloc(&d.g.ikey, n); dalloc(&d.g.hkey, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
   dalloc(&d.bp_cache, (size_t)n * BP_CACHE); dalloc(&d.toi_cache, (size_t)n * TOI_CACHE);
   d.g.large = nullptr;
   d.lflag = nullptr;
@@ -231,12 +232,12 @@ static void free_all(oavbd_ctx* c) {
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
-  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large);
+  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large);
   dfree(d.g.is_large);
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
-  dfree(d.cand_off); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
+  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.colour_cnt); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(d.remain); dfree(c->cand_cnt); dfree(c->scan_scratch);
@@ -390,6 +391,7 @@ static void broadphase(oavbd_ctx* c) {
 static void narrow_and_merge(oavbd_ctx* c) {
   Dev& d = c->d;
   std::swap(d.P, d.Pold);
+  std::swap(d.poff, d.poff_old);
   std::swap(d.Q, d.Qold);
   std::swap(c->cap_pairs, c->cap_pairs_old);
   std::swap(c->cap_pts, c->cap_pts_old);
@@ -398,6 +400,7 @@ static void narrow_and_merge(oavbd_ctx* c) {
   uint32_t nc = c->ncand;
   if (nc) k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
   scan_excl(d.C.acc, d.acc_off, nc, c->st, c->scan_scratch);
+  k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
   scan_excl(d.C.npts, d.pt_off, nc, c->st, c->scan_scratch);
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT + 1, d.pt_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
@@ -807,6 +810,8 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
     HIPC(hipMemset(d.b.flag_a, 0, n));
     HIPC(hipMemset(d.b.flag_b, 0, n));
     HIPC(hipMemset(d.b.colour, 0xff, n * sizeof(int32_t)));
+    HIPC(hipMemset(d.poff, 0, (n + 1) * sizeof(uint32_t)));
+    HIPC(hipMemset(d.poff_old, 0, (n + 1) * sizeof(uint32_t)));
     d.npairs = d.npts = d.npairs_old = d.npts_old = 0;
     c->joints.clear();
     upload_joints(c);
