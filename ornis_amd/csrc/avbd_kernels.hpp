@@ -597,29 +597,6 @@ __global__ void k_pair_offsets(Dev d) {
   if (h <= d.n) d.poff[h] = d.acc_off[d.cand_off[h]];
 }
 
-// Fresh contact points of one candidate, kept in registers (fixed slots,
-// static indices).
-struct FreshPts {
-  f3 a[K_MAX_POINTS], b[K_MAX_POINTS];
-  int n;
-  __device__ __forceinline__ bool dup(f3 x, f3 y) const {
-    bool r = false;
-#pragma unroll
-    for (int i = 0; i < K_MAX_POINTS; ++i)
-      if (i < n && len3(sub(a[i], x)) < K_POINT_MATCH && len3(sub(b[i], y)) < K_POINT_MATCH) r = true;
-    return r;
-  }
-  __device__ __forceinline__ void push(f3 x, f3 y) {
-#pragma unroll
-    for (int i = 0; i < K_MAX_POINTS; ++i)
-      if (i == n) {
-        a[i] = x;
-        b[i] = y;
-      }
-    ++n;
-  }
-};
-
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   uint32_t c = gid();
   if (c >= ncand) return;
@@ -682,8 +659,18 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   float plane_b = B.kind == SH_BOX ? dot3(normal, B.pos) - (-1.0f) * dot3(abs3(qrot(qconj(B.rot), normal)), B.p)
                                    : dot3(normal, dd.pb);
   qf inv_a = qconj(A.rot), inv_b = qconj(B.rot);
-  FreshPts fp;
-  fp.n = 0;
+  f3 fra[K_MAX_POINTS], frb[K_MAX_POINTS];
+  int nf = 0;
+  auto dup = [&](f3 x, f3 y) {
+    for (int i = 0; i < nf; ++i)
+      if (len3(sub(fra[i], x)) < K_POINT_MATCH && len3(sub(frb[i], y)) < K_POINT_MATCH) return true;
+    return false;
+  };
+  auto push = [&](f3 x, f3 y) {
+    fra[nf] = x;
+    frb[nf] = y;
+    ++nf;
+  };
   if (!separated) {
     for (int side = 0; side < 2; ++side) {
       const shp& body = side == 0 ? A : B;
@@ -693,7 +680,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
       f3 pc = side == 0 ? pa_c : pb_c;
       f3 sn = smul(sign, normal);
       float slop = K_EXPAND_SLOP + fmax_(-sgn, 0.0f);
-      for (int ci = 0; ci < 8 && fp.n < K_MAX_POINTS; ++ci) {
+      for (int ci = 0; ci < 8 && nf < K_MAX_POINTS; ++ci) {
         float sx = (ci & 4) ? 1.0f : -1.0f, sy = (ci & 2) ? 1.0f : -1.0f, sz = (ci & 1) ? 1.0f : -1.0f;
         f3 corner{sx * body.p.x, sy * body.p.y, sz * body.p.z};
         f3 world = add(body.pos, qrot(body.rot, corner));
@@ -712,22 +699,22 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
         }
         f3 ral = qrot(inv_a, sub(pa, A.pos));
         f3 rbl = qrot(inv_b, sub(pb, B.pos));
-        if (fp.dup(ral, rbl)) continue;
-        fp.push(ral, rbl);
+        if (dup(ral, rbl)) continue;
+        push(ral, rbl);
       }
     }
-    if (fp.n >= 2 && fp.n < K_MAX_POINTS) {
+    if (nf >= 2 && nf < K_MAX_POINTS) {
       f3 mid = ra < rb ? pa_c : (rb < ra ? pb_c : mul(add(pa_c, pb_c), 0.5f));
       f3 pa = add(mid, mul(normal, plane_a - dot3(normal, mid)));
       f3 pb = add(mid, mul(normal, plane_b - dot3(normal, mid)));
       f3 ral = qrot(inv_a, sub(pa, A.pos));
       f3 rbl = qrot(inv_b, sub(pb, B.pos));
-      if (!fp.dup(ral, rbl)) fp.push(ral, rbl);
+      if (!dup(ral, rbl)) push(ral, rbl);
     }
-    if (fp.n < 3) {
+    if (nf < 3) {
       f3 ral = qrot(inv_a, sub(dd.pa, A.pos));
       f3 rbl = qrot(inv_b, sub(dd.pb, B.pos));
-      if (!fp.dup(ral, rbl)) fp.push(ral, rbl);
+      if (!dup(ral, rbl)) push(ral, rbl);
     }
   }
   bool wake;
@@ -744,14 +731,12 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
     }
   }
   d.C.acc[c] = 1u;
-  d.C.npts[c] = (uint32_t)fp.n;
+  d.C.npts[c] = (uint32_t)nf;
   d.C.disc0[c] = pk(normal, sgn);
   d.C.disc1[c] = make_float4(mu1, mu1, __int_as_float(old), wake ? 1.0f : 0.0f);
-#pragma unroll
-  for (int k = 0; k < K_MAX_POINTS; ++k) {
-    if (k >= fp.n) break;
-    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fp.a[k].x, fp.a[k].y, fp.a[k].z, fp.b[k].x);
-    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(fp.b[k].y, fp.b[k].z, 0.0f, 0.0f);
+  for (int k = 0; k < nf; ++k) {
+    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fra[k].x, fra[k].y, fra[k].z, frb[k].x);
+    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(frb[k].y, frb[k].z, 0.0f, 0.0f);
   }
 }
 
