@@ -47,6 +47,53 @@ def cpu_baseline(scene, seconds, max_steps):
                       f"steps 1-{done} of the same 100k-box workload from its initial state, {dt:.1f} s"}
 
 
+def batched_leg(steps, loop_scenes, loop_steps):
+    """4,096 independent box-drop scenes (8-512 bodies each, log-uniform,
+    each with its own 10 x 10 m floor) stepped together in one context,
+    against the same scenes stepped one context per scene on the same GPU
+    (timed on a sample of scenes and steps, reported per scene-step)."""
+    from ornis_amd import scenes
+    from ornis_amd.avbd import AvbdEngine
+    parts, allsc = scenes.batched_drops()
+    n_scenes = len(parts)
+    eng = AvbdEngine(iters=10)
+    eng.load(allsc, scene_ids=allsc["scene_id"])
+    eng.step_n(5)
+    eng.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(50, steps - done)
+        eng.step_n(k)
+        done += k
+        eng.sync()
+        print(f"bench.py: batched {done}/{steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    eng.close()
+    batched = n_scenes * steps / wall
+    # one context per scene, the loop a caller without batching would run
+    engines = [AvbdEngine(scene=parts[i], iters=10) for i in range(loop_scenes)]
+    for e in engines:
+        e.step_n(5)
+        e.sync()
+    t0 = time.perf_counter()
+    for _ in range(loop_steps):
+        for e in engines:
+            e.step_n(1)
+    for e in engines:
+        e.sync()
+    lwall = time.perf_counter() - t0
+    for e in engines:
+        e.close()
+    loop = loop_scenes * loop_steps / lwall
+    return {"metric": "batched scenes/s", "value": batched, "unit": "scene-steps/s", "scenes": n_scenes,
+            "bodies": int(len(allsc["shape"])), "steps": steps, "ms_per_step": 1e3 * wall / steps,
+            "per_scene_loop": {"value": loop, "unit": "scene-steps/s",
+                               "sample": f"scenes 0-{loop_scenes - 1}, {loop_steps} steps each after 5 warmup, "
+                                         f"one context per scene, {lwall:.2f} s"},
+            "speedup_vs_loop": batched / loop}
+
+
 def measured_traffic():
     """HBM bytes per k_primal launch on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a plain copy of
@@ -71,6 +118,10 @@ def main():
     ap.add_argument("--waves", type=int, default=1, help="primal wavefronts per workgroup")
     ap.add_argument("--sweep-shapes", type=int, default=0,
                     help="after warmup, time this many steps per primal launch shape and exit")
+    ap.add_argument("--batched-steps", type=int, default=300,
+                    help="timed steps of the 4,096-scene batched workload (0: skip it)")
+    ap.add_argument("--loop-scenes", type=int, default=32)
+    ap.add_argument("--loop-steps", type=int, default=30)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
                          "for HBM byte counters)")
@@ -134,6 +185,10 @@ def main():
     achieved = kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9 if kt["primal_ms"] > 0 else 0.0
     traffic = measured_traffic()
     dual_gbs = kt["dual_bytes"] / (kt["dual_ms"] * 1e-3) / 1e9 if kt["dual_ms"] > 0 else 0.0
+    eng.close()
+    batched = None
+    if args.batched_steps:
+        batched = batched_leg(args.batched_steps, args.loop_scenes, args.loop_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_max_steps)
@@ -163,6 +218,7 @@ def main():
                                "dual": kt["dual_ms"] / max(kt["steps"], 1)},
         "dual_roofline": {"achieved": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS},
         "final_stats": st,
+        "batched": batched,
     }
     print(json.dumps(line))
 

@@ -36,6 +36,20 @@
 // count; the exact reference prefilters run on every candidate, so the
 // discovered set (and everything downstream) is identical to the O(n^2)
 // scan, which stays selectable for tests.
+//
+// What pins this restatement (tests/test_oracle.py):
+//   - the reference's golden file crates/physics/tests/data/avbd_snapshot.hex
+//     (9 bodies x 120 steps, ORDER_REFERENCE, bit-exact with libm and with
+//     the portable trig): discovery, damper, TOI, sleep, dual, Ball joint;
+//   - closed forms: solve_6x6 vs a dense solve and its breakdown cases,
+//     stamp_row vs the analytic J J^T, contact_force audit answers (A3, A6,
+//     push-only), tangent basis, quat_integrate / quat_diff_vec inverse;
+//   - invariants: stack stands, damper momentum, determinism, grid = brute
+//     discovery, proper greedy colouring, FP64 tracks FP32.
+// Parity unpinned: glam Quat op order under non-identity rotations (the
+// golden file only holds identity rotations), the reference's libm
+// sin_cos / asin bits (MATH_PORTABLE is pinned to libm within 2 ulp, not to
+// the reference), kinematic bodies and rolling / torsion rows.
 
 #include <algorithm>
 #include <cmath>

@@ -254,19 +254,58 @@ HD float feature_candidate(const shp& a, f3 ha, const box_axes_t& ea, const shp&
   return len2(sub(*y, *x));
 }
 
-// Closest features of two boxes: the first minimum over the fixed scan.
+// Closest features of two boxes: the first minimum over the fixed scan
+// (corners precomputed once; with the constant trip counts the compiler
+// keeps them in registers).
+HD void obb_corners(f3 pos, f3 half, qf rot, f3 out[8]) {
+  f3 x = qrot(rot, f3{1.0f * half.x, 0.0f * half.x, 0.0f * half.x});
+  f3 y = qrot(rot, f3{0.0f * half.y, 1.0f * half.y, 0.0f * half.y});
+  f3 z = qrot(rot, f3{0.0f * half.z, 0.0f * half.z, 1.0f * half.z});
+  f3 px = add(pos, x), mx = sub(pos, x);
+  out[0] = add(add(px, y), z);
+  out[1] = sub(add(px, y), z);
+  out[2] = add(sub(px, y), z);
+  out[3] = sub(sub(px, y), z);
+  out[4] = add(add(mx, y), z);
+  out[5] = sub(add(mx, y), z);
+  out[6] = add(sub(mx, y), z);
+  out[7] = sub(sub(mx, y), z);
+}
+
 HD dist_t feature_distance(const shp& a, f3 ha, const shp& b, f3 hb) {
-  const box_axes_t ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
+  f3 ca[8], cb[8];
+  obb_corners(a.pos, ha, a.rot, ca);
+  obb_corners(b.pos, hb, b.rot, cb);
   dist_t res{__builtin_inff(), a.pos, b.pos};
   float best = __builtin_inff();
-  for (int c = 0; c < 160; ++c) {
-    f3 x, y;
-    float sq = feature_candidate(a, ha, ea, b, hb, eb, c, &x, &y);
+  for (int i = 0; i < 8; ++i) {
+    f3 y = point_obb_closest(ca[i], b.pos, hb, b.rot);
+    float sq = len2(sub(y, ca[i]));
     if (sq < best) {
       best = sq;
-      res = dist_t{sqrtf(sq), x, y};
+      res = dist_t{sqrtf(sq), ca[i], y};
     }
   }
+  for (int i = 0; i < 8; ++i) {
+    f3 x = point_obb_closest(cb[i], a.pos, ha, a.rot);
+    float sq = len2(sub(cb[i], x));
+    if (sq < best) {
+      best = sq;
+      res = dist_t{sqrtf(sq), x, cb[i]};
+    }
+  }
+  const unsigned char ea_[12] = {0, 1, 2, 3, 0, 1, 4, 5, 0, 2, 4, 6};
+  const unsigned char eb_[12] = {4, 5, 6, 7, 2, 3, 6, 7, 1, 3, 5, 7};
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < 12; ++j) {
+      f3 x, y;
+      seg_seg_closest(ca[ea_[i]], ca[eb_[i]], cb[ea_[j]], cb[eb_[j]], &x, &y);
+      float sq = len2(sub(y, x));
+      if (sq < best) {
+        best = sq;
+        res = dist_t{sqrtf(sq), x, y};
+      }
+    }
   return res;
 }
 
@@ -283,7 +322,9 @@ HD dist_t box_box_from_sat(const shp& a, f3 ha, const shp& b, f3 hb, float gap, 
 }
 
 // shape_distance; for box-box pairs *sat receives the SAT gap
-// (box_box_signed_gap) the discovery reads as the signed gap.
+// (box_box_signed_gap) the discovery reads as the signed gap.  BOXES =
+// false drops the box-box arm for callers that never pass two boxes.
+template <bool BOXES = true>
 HD dist_t shape_distance(const shp& a, const shp& b, float* sat) {
   *sat = 0.0f;
   if (a.kind == SH_HALFSPACE) return halfspace_distance(a, b, true);
@@ -294,8 +335,12 @@ HD dist_t shape_distance(const shp& a, const shp& b, float* sat) {
     dist_t d = sphere_obb(b, b.p.x, a, a.p);
     return dist_t{d.dist, d.pb, d.pa};
   }
-  f3 normal;
-  float gap = separating_axis(a, a.p, b, b.p, &normal);
-  *sat = gap;
-  return box_box_from_sat(a, a.p, b, b.p, gap, normal);
+  if constexpr (!BOXES) {
+    return dist_t{__builtin_inff(), a.pos, b.pos};
+  } else {
+    f3 normal;
+    float gap = separating_axis(a, a.p, b, b.p, &normal);
+    *sat = gap;
+    return box_box_from_sat(a, a.p, b, b.p, gap, normal);
+  }
 }

@@ -153,14 +153,18 @@ __global__ void k_begin(Dev d) {
 // every candidate it yields is re-tested exactly)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int cell_of(float x, float cell) { return (int)floorf(x / cell); }
-// Cell coordinates packed 21 bits each (cells 2^21 apart share a key; a
-// search never spans that far, and every candidate is re-tested exactly).
-__device__ __forceinline__ unsigned long long cell_key(int x, int y, int z) {
-  return ((unsigned long long)((uint32_t)x & 0x1fffffu) << 42) | ((unsigned long long)((uint32_t)y & 0x1fffffu) << 21) |
-         (unsigned long long)((uint32_t)z & 0x1fffffu);
+// Cells are per scene: bodies of different scenes never pair, so a scene's
+// cells only hold (up to hash collisions) its own bodies, however many
+// scenes share the same space.  The key packs cell coordinates and scene,
+// 16 bits each (cells 2^16 apart, or scenes 2^16 apart, share a key; a
+// search never spans that far, and every candidate is re-tested exactly,
+// scene included).
+__device__ __forceinline__ unsigned long long cell_key(int x, int y, int z, uint32_t sc) {
+  return ((unsigned long long)((uint32_t)x & 0xffffu)) | ((unsigned long long)((uint32_t)y & 0xffffu) << 16) |
+         ((unsigned long long)((uint32_t)z & 0xffffu) << 32) | ((unsigned long long)(sc & 0xffffu) << 48);
 }
-__device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t nb) {
-  uint32_t h = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u;
+__device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t sc, uint32_t nb) {
+  uint32_t h = (uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u ^ sc * 2654435761u;
   h ^= h >> 13;
   h *= 0x5bd1e995u;
   h ^= h >> 15;
@@ -183,8 +187,9 @@ __global__ void k_grid_count(Dev d) {
   }
   float4 p = d.b.pos[h];
   int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
-  uint32_t bk = bucket_of(cx, cy, cz, d.g.nbuckets);
-  d.g.hkey[h] = cell_key(cx, cy, cz);
+  const uint32_t sc = d.b.scene[h];
+  uint32_t bk = bucket_of(cx, cy, cz, sc, d.g.nbuckets);
+  d.g.hkey[h] = cell_key(cx, cy, cz, sc);
   d.g.bucket[h] = bk;
   atomicAdd(&d.g.cnt[bk], 1u);
 }
@@ -201,13 +206,13 @@ __global__ void k_grid_fill(Dev d) {
 }
 
 // Distinct buckets of the cells in [lo, hi] (at most cap); returns count.
-__device__ __forceinline__ int gather_buckets(const Dev& d, int lx, int ly, int lz, int hx, int hy, int hz,
-                                              uint32_t* out, int cap) {
+__device__ __forceinline__ int gather_buckets(const Dev& d, uint32_t sc, int lx, int ly, int lz, int hx, int hy,
+                                              int hz, uint32_t* out, int cap) {
   int m = 0;
   for (int x = lx; x <= hx; ++x)
     for (int y = ly; y <= hy; ++y)
       for (int z = lz; z <= hz; ++z) {
-        uint32_t bk = bucket_of(x, y, z, d.g.nbuckets);
+        uint32_t bk = bucket_of(x, y, z, sc, d.g.nbuckets);
         bool dup = false;
         for (int i = 0; i < m; ++i)
           if (out[i] == bk) dup = true;
@@ -235,8 +240,9 @@ __device__ __forceinline__ bool toi_target_ok(const Dev& d, uint32_t h, uint32_t
 // remaining sweep, the loop would end without a hit whatever the exact
 // distance: the expensive feature-distance pass is skipped and the outcome
 // is unchanged.
+template <bool BOXES = true>
 HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* normal) {
-  const bool bb = mover.kind == SH_BOX && target.kind == SH_BOX;
+  const bool bb = BOXES && mover.kind == SH_BOX && target.kind == SH_BOX;
   float slack = 0.0f;
   if (bb) {
     float s = 1.0f + len + len3(mover.p) + len3(target.p);
@@ -257,7 +263,7 @@ HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* no
       r = box_box_from_sat(m, m.p, target, target.p, gap, sn);
     } else {
       float sat;
-      r = shape_distance(m, target, &sat);
+      r = shape_distance<BOXES>(m, target, &sat);
     }
     if (r.dist <= K_SHAPE_TOUCH) {
       if (t > 0.0f) {
@@ -303,6 +309,7 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, Emit emit) {
     emit(m, o);
     ++m;
   };
+  const uint32_t sch = d.b.scene[h];
   float r = rm + d.g.cell * 0.5f + 0.01f;
   f3 p1 = add(p0, delta);
   int lx = cell_of(fminf(p0.x, p1.x) - r, d.g.cell), hx = cell_of(fmaxf(p0.x, p1.x) + r, d.g.cell);
@@ -315,17 +322,20 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, Emit emit) {
     for (int x = lx; x <= hx; ++x)
       for (int y = ly; y <= hy; ++y)
         for (int z = lz; z <= hz; ++z) {
-          uint32_t bk = bucket_of(x, y, z, d.g.nbuckets);
-          unsigned long long key = cell_key(x, y, z);
+          uint32_t bk = bucket_of(x, y, z, sch, d.g.nbuckets);
+          unsigned long long key = cell_key(x, y, z, sch);
           uint32_t s0 = d.g.start[bk], c0 = d.g.cnt[bk];
           for (uint32_t k = 0; k < c0; ++k)
             if (d.g.ikey[s0 + k] == key) consider(d.g.items[s0 + k], d.g.ipos[s0 + k]);
         }
   } else {
-    for (uint32_t k = 0; k < d.g.start[d.g.nbuckets]; ++k) consider(d.g.items[k], d.g.ipos[k]);
+    for (uint32_t k = d.g.soff[sch]; k < d.g.soff[sch + 1]; ++k) {  // every small body of the scene
+      uint32_t o = d.g.sitems[k];
+      if (!d.g.is_large[o]) consider(o, pk(xyz(d.b.pos[o]), d.b.shape[o].w));
+    }
   }
-  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
-    uint32_t o = d.g.large[k];
+  for (uint32_t k = d.g.lsc_off[sch]; k < d.g.lsc_off[sch + 1]; ++k) {
+    uint32_t o = d.g.lsc[k];
     consider(o, pk(xyz(d.b.pos[o]), d.b.shape[o].w));
   }
   return m;
@@ -365,7 +375,7 @@ __device__ __forceinline__ unsigned long long toi_key(float t, uint32_t o) {
 // closed forms that every lane evaluates alike.
 template <int G>
 __device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, int s) {
-  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one(mover, dir, len, target, normal);
+  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one<false>(mover, dir, len, target, normal);
   float sc = 1.0f + len + len3(mover.p) + len3(target.p);
   sc += fmaxf(fmaxf(fabsf(mover.pos.x), fabsf(mover.pos.y)), fabsf(mover.pos.z));
   sc += fmaxf(fmaxf(fabsf(target.pos.x), fabsf(target.pos.y)), fabsf(target.pos.z));
@@ -425,11 +435,6 @@ __global__ void __launch_bounds__(TPB) k_toi_win(Dev d, uint32_t npairs) {
   if (d.toi_best[ho.x] != toi_key(r.w, ho.y)) return;
   d.toi_hit[ho.x] = r;
   d.toi_tgt[ho.x] = ho.y;
-}
-
-__global__ void k_toi_clear(Dev d) {
-  uint32_t h = gid();
-  if (h < d.n) d.toi_hit[h] = make_float4(-1.0f, 0.0f, 0.0f, -1.0f);
 }
 
 __global__ void k_toi_apply(Dev d) {
@@ -496,7 +501,7 @@ __device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, Emit emit
   const float ra = d.b.shape[a].w;
   int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
   uint32_t bks[27];
-  int nbk = gather_buckets(d, cx - 1, cy - 1, cz - 1, cx + 1, cy + 1, cz + 1, bks, 27);
+  int nbk = gather_buckets(d, d.b.scene[a], cx - 1, cy - 1, cz - 1, cx + 1, cy + 1, cz + 1, bks, 27);
   uint32_t m = 0;
   for (int i = 0; i < nbk; ++i) {
     uint32_t s0 = d.g.start[bks[i]], c0 = d.g.cnt[bks[i]];
@@ -507,8 +512,9 @@ __device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, Emit emit
       if (o > a && prefilter_ids(d, a, o)) emit(m++, o);
     }
   }
-  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
-    uint32_t o = d.g.large[k];
+  const uint32_t sc = d.b.scene[a];
+  for (uint32_t k = d.g.lsc_off[sc]; k < d.g.lsc_off[sc + 1]; ++k) {
+    uint32_t o = d.g.lsc[k];
     if (o > a && prefilter(d, a, o)) emit(m++, o);
   }
   return m;
@@ -524,20 +530,33 @@ __global__ void k_bp_count(Dev d, uint32_t* cnt) {
   });
 }
 
-// Large bodies pair with every later body passing the prefilter.
-__global__ void k_bp_large_flag(Dev d) {
-  uint32_t b = gid();
-  if (b >= d.n) return;
-  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
-    uint32_t L = d.g.large[k];
-    d.lflag[(size_t)k * d.n + b] = (L < b && prefilter(d, L, b)) ? 1u : 0u;
+// Large bodies pair with every later body of their scene passing the
+// prefilter.  Flag row k covers large body k's scene (bodies in index
+// order); t indexes all rows back to back.
+__device__ __forceinline__ uint32_t large_row(const Dev& d, uint32_t t) {
+  uint32_t lo = 0, hi = d.g.nlarge;  // last row whose base is <= t
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (d.g.lrange[mid].z <= t)
+      lo = mid;
+    else
+      hi = mid;
   }
+  return lo;
+}
+__global__ void k_bp_large_flag(Dev d) {
+  uint32_t t = gid();
+  if (t >= d.g.lflat) return;
+  uint4 r = d.g.lrange[large_row(d, t)];
+  uint32_t b = d.g.sitems[r.x + (t - r.z)];
+  d.lflag[t] = (r.w < b && prefilter(d, r.w, b)) ? 1u : 0u;
 }
 
 __global__ void k_bp_large_count(Dev d, uint32_t* cnt) {
   uint32_t k = gid();
   if (k >= d.g.nlarge) return;
-  cnt[d.g.large[k]] = d.lrank[(size_t)k * (d.n + 1) + d.n];
+  uint4 r = d.g.lrange[k];
+  cnt[r.w] = d.lrank[r.z + (r.y - r.x)] - d.lrank[r.z];
 }
 
 __global__ void k_bp_fill(Dev d) {
@@ -563,13 +582,11 @@ __global__ void k_bp_fill(Dev d) {
 }
 
 __global__ void k_bp_large_fill(Dev d) {
-  uint32_t b = gid();
-  if (b >= d.n) return;
-  for (uint32_t k = 0; k < d.g.nlarge; ++k) {
-    if (!d.lflag[(size_t)k * d.n + b]) continue;
-    uint32_t L = d.g.large[k];
-    d.C.ab[d.cand_off[L] + d.lrank[(size_t)k * (d.n + 1) + b]] = make_uint2(L, b);
-  }
+  uint32_t t = gid();
+  if (t >= d.g.lflat || !d.lflag[t]) return;
+  uint4 r = d.g.lrange[large_row(d, t)];
+  uint32_t b = d.g.sitems[r.x + (t - r.z)];
+  d.C.ab[d.cand_off[r.w] + d.lrank[t] - d.lrank[r.z]] = make_uint2(r.w, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -637,7 +654,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
     if (len2(normal) < K_AXIS_REST_LEN2) normal = sub(A.pos, B.pos);
     normal = normalize_or(normal, f3{0.0f, 1.0f, 0.0f});
     if (old >= 0) {
-      if (dot3(normal, xyz(d.Pold.n[old])) < 0.0f) normal = neg(normal);
+      if (dot3(normal, xyz(d.Pold.n(old))) < 0.0f) normal = neg(normal);
     } else if (dot3(normal, sub(A.pos, B.pos)) < 0.0f) {
       normal = neg(normal);
     }
@@ -752,8 +769,8 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
   int old = __float_as_int(d1.z);
   float4 ma = d.b.mat[ab.x], mb = d.b.mat[ab.y];
   d.P.ab[p] = ab;
-  d.P.n[p] = d0;
-  d.P.mu[p] = make_float4(d1.x, d1.y, fmax_(ma.z, mb.z), fmax_(ma.w, mb.w));
+  d.P.n(p) = d0;
+  d.P.mu(p) = make_float4(d1.x, d1.y, fmax_(ma.z, mb.z), fmax_(ma.w, mb.w));
   d.P.pts[p] = make_uint2(start, cnt);
   if (d1.w != 0.0f) {
     d.b.flag_a[ab.x] = 1;
@@ -769,31 +786,31 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
     f3 ral{f0.x, f0.y, f0.z}, rbl{f0.w, f1.x, f1.y};
     int m = -1;
     for (uint32_t j = 0; j < op.y; ++j)
-      if (len3(sub(xyz(d.Qold.ra[op.x + j]), ral)) < K_LAMBDA_MATCH) {
+      if (len3(sub(xyz(d.Qold.ra(op.x + j)), ral)) < K_LAMBDA_MATCH) {
         m = (int)(op.x + j);
         break;
       }
     uint32_t q = start + k;
     if (m >= 0) {
-      float4 ora = d.Qold.ra[m], orb = d.Qold.rb[m];
+      float4 ora = d.Qold.ra(m), orb = d.Qold.rb(m);
       bool stuck = orb.w != 0.0f;
       bool keep = stuck && touching && !smooth;
       f3 ra = keep ? xyz(ora) : ral;
       f3 rb = keep ? xyz(orb) : rbl;
-      float4 l = d.Qold.lam[m], pe = d.Qold.pen[m], ro = d.Qold.roll[m];
-      d.Q.ra[q] = pk(ra, 0.0f);
-      d.Q.rb[q] = pk(rb, stuck ? 1.0f : 0.0f);
-      d.Q.lam[q] = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
-      d.Q.pen[q] = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
+      float4 l = d.Qold.lam(m), pe = d.Qold.pen(m), ro = d.Qold.roll[m];
+      d.Q.ra(q) = pk(ra, 0.0f);
+      d.Q.rb(q) = pk(rb, stuck ? 1.0f : 0.0f);
+      d.Q.lam(q) = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+      d.Q.pen(q) = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
                                clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
       d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
     } else {
       float z = 0.0f * AG;
       float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
-      d.Q.ra[q] = pk(ral, 0.0f);
-      d.Q.rb[q] = pk(rbl, 1.0f);
-      d.Q.lam[q] = make_float4(z, z, z, 0.0f);
-      d.Q.pen[q] = make_float4(p1, p1, p1, 0.0f);
+      d.Q.ra(q) = pk(ral, 0.0f);
+      d.Q.rb(q) = pk(rbl, 1.0f);
+      d.Q.lam(q) = make_float4(z, z, z, 0.0f);
+      d.Q.pen(q) = make_float4(p1, p1, p1, 0.0f);
       d.Q.roll[q] = make_float4(z, z, z, 0.0f);
     }
     d.Q.pair[q] = p;
@@ -935,7 +952,7 @@ __global__ void k_pair_prep(Dev d) {
   uint32_t p = gid();
   if (p >= d.npairs) return;
   uint2 ab = d.P.ab[p];
-  float4 n4 = d.P.n[p];
+  float4 n4 = d.P.n(p);
   f3 n = xyz(n4);
   float gap = n4.w;
   f3 t1, t2u;
@@ -953,26 +970,26 @@ __global__ void k_pair_prep(Dev d) {
       if (m_red > 0.0f) damp = fmin_(2.0f * m_red / (K_DT * K_DT) / npts, K_PEN_MAX);
     }
   }
-  d.P.t1[p] = pk(t1, damp);
-  d.P.t2[p] = pk(t2, 0.0f);
+  d.P.t1(p) = pk(t1, damp);
+  d.P.t2(p) = pk(t2, 0.0f);
   bool sphere = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
   float4 pa0 = d.b.pos0[ab.x], pb0 = d.b.pos0[ab.y];
   qf qa0 = asq(d.b.rot0[ab.x]), qb0 = asq(d.b.rot0[ab.y]);
   for (uint32_t k = 0; k < pts.y; ++k) {
     uint32_t q = pts.x + k;
-    float4 ra = d.Q.ra[q];
+    float4 ra = d.Q.ra(q);
     float c0 = 0.0f;
     if (gap <= 0.0f) {
       if (sphere) {
         c0 = gap + K_MARGIN;
       } else {
         f3 pa = add(xyz(pa0), qrot(qa0, xyz(ra)));
-        f3 pb = add(xyz(pb0), qrot(qb0, xyz(d.Q.rb[q])));
+        f3 pb = add(xyz(pb0), qrot(qb0, xyz(d.Q.rb(q))));
         c0 = dot3(n, sub(pa, pb)) + K_MARGIN;
       }
     }
     ra.w = c0;
-    d.Q.ra[q] = ra;
+    d.Q.ra(q) = ra;
   }
 }
 
@@ -1082,9 +1099,18 @@ __device__ __forceinline__ bool swept(const Dev& d, uint32_t h) {
 // col_round[h] packs (round << 16) | colour in one word (written by a
 // single store), -1 while uncoloured; a neighbour coloured in the current
 // round still counts as uncoloured.
+// Also clears the per-step incidence and colouring counters.
 __global__ void k_colour_init(Dev d) {
   uint32_t h = gid();
-  if (h >= d.n) return;
+  if (h < CSTAT_HEAD + 65536) d.cstat[h] = 0u;
+  if (h < 65536) d.colour_cur[h] = 0u;
+  if (h > d.n) return;
+  d.ptcnt[h] = 0u;
+  d.bcnt[h] = 0u;
+  d.bcur[h] = 0u;
+  d.nbr_cnt[h] = 0u;
+  d.nbr_cur[h] = 0u;
+  if (h == d.n) return;
   d.sw[h] = swept(d, h) ? 1 : 0;
   d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
   d.b.colour[h] = -1;
@@ -1097,8 +1123,9 @@ __device__ __forceinline__ int colour_seen(const Dev& d, uint32_t o, int round) 
   return v & 0xffff;
 }
 
-__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, int round, uint32_t* remain) {
+__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, int round, uint32_t* remain, uint32_t* clear_next) {
   uint32_t h = gid();
+  if (h == 0) *clear_next = 0u;
   bool wait = false;
   int newc = -1;
   if (h < d.n && d.sw[h] && d.b.col_round[h] < 0) {
@@ -1125,7 +1152,7 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, int round, uint32_t
     }
   }
   wave_count(remain, wait);
-  (void)wave_append(d.colour_cnt, newc, &d.counters[CTR_MAXCOL]);
+  (void)wave_append(d.colour_cnt, newc, d.cmax);
 }
 
 __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
@@ -1138,6 +1165,26 @@ __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
   }
   uint32_t slot = wave_append(d.colour_cur, c, nullptr);
   if (c >= 0) d.colour_list[d.colour_off[c] + slot] = h;
+}
+
+// Solver records for the iteration loop (pos, rot, pos0, drot per body);
+// the loop reads and writes only these, the arrays are refreshed after it.
+__global__ void k_sb_pack(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  float4* rec = d.sb + 4 * (size_t)h;
+  rec[0] = d.b.pos[h];
+  rec[1] = d.b.rot[h];
+  rec[2] = d.b.pos0[h];
+  rec[3] = d.b.drot[h];
+}
+__global__ void k_sb_unpack(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  const float4* rec = d.sb + 4 * (size_t)h;
+  d.b.pos[h] = rec[0];
+  d.b.rot[h] = rec[1];
+  d.b.drot[h] = rec[3];
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,20 +1228,22 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
   float sign = is_a ? 1.0f : -1.0f;
   uint32_t p = it.y;
   uint2 ab = make_uint2(it.z, it.w);
-  float4 n4 = d.P.n[p];
+  float4 n4 = d.P.n(p);
   f3 n = xyz(n4);
   float gap = n4.w;
   bool touching = gap <= 0.0f;
-  float4 ra4 = d.Q.ra[q];
+  float4 ra4 = d.Q.ra(q);
   float c0 = touching ? ra4.w : 0.0f;
-  float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
-  float4 a0 = d.b.pos0[ab.x], b0 = d.b.pos0[ab.y];
-  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
-  f3 dra = xyz(d.b.drot[ab.x]), drb = xyz(d.b.drot[ab.y]);
+  const float4* sa = d.sb + 4 * (size_t)ab.x;
+  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  float4 pa4 = sa[0], pb4 = sbb[0];
+  float4 a0 = sa[2], b0 = sbb[2];
+  qf qa = asq(sa[1]), qb = asq(sbb[1]);
+  f3 dra = xyz(sa[3]), drb = xyz(sbb[3]);
   f3 dpa = sub(xyz(pa4), xyz(a0)), dpb = sub(xyz(pb4), xyz(b0));
   // contact_levers (rows.rs:963-973)
   f3 ra_w = a0.w >= 0.0f ? mul(neg(n), a0.w) : qrot(qa, xyz(ra4));
-  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(d.Q.rb[q]));
+  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(d.Q.rb(q)));
   // row_c_levers (rows.rs:1006-1025)
   auto row_c = [&](f3 u, float cc, f3 la, f3 lb) {
     f3 d_a = add(dpa, mul(u, dot3(cross3(la, u), dra)));
@@ -1203,7 +1252,7 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
   };
   float cn = row_c(n, c0, ra_w, rb_w);
   f3 r_side = is_a ? ra_w : rb_w;
-  float4 t14 = d.P.t1[p];
+  float4 t14 = d.P.t1(p);
   if (!touching) {
     float damp = t14.w;
     if (damp > 0.0f && fabsf(cn) >= K_C_EPS) {
@@ -1213,15 +1262,15 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
     }
     return 0u;
   }
-  f3 t1 = xyz(t14), t2 = xyz(d.P.t2[p]);
+  f3 t1 = xyz(t14), t2 = xyz(d.P.t2(p));
   // friction_levers (rows.rs:987-996)
   float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
   f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
   f3 rb_f = add(rb_w, mul(n, sep * 0.5f));
   float ct1 = row_c(t1, 0.0f, ra_f, rb_f);
   float ct2 = row_c(t2, 0.0f, ra_f, rb_f);
-  float4 pen = d.Q.pen[q], lam = d.Q.lam[q];
-  float4 mu = d.P.mu[p];
+  float4 pen = d.Q.pen(q), lam = d.Q.lam(q);
+  float4 mu = d.P.mu(p);
   // contact_force (rows.rs:1032-1060)
   float fn_c = fmin_(pen.x * cn + lam.x, 0.0f);
   float r0 = pen.y * ct1 + lam.y, r1 = pen.z * ct2 + lam.z;
@@ -1277,10 +1326,12 @@ __device__ __forceinline__ uint32_t joint_rows(const Dev& d, uint32_t e, Row (&o
   bool is_a = (e & 1u) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
   uint2 ab = d.J.ab[j];
-  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  const float4* sa = d.sb + 4 * (size_t)ab.x;
+  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  qf qa = asq(sa[1]), qb = asq(sbb[1]);
   f3 la = xyz(d.J.la[j]), lb = xyz(d.J.lb[j]);
-  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(qa, la));
-  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, lb));
+  f3 pa = add(xyz(sa[0]), qrot(qa, la));
+  f3 pb = add(xyz(sbb[0]), qrot(qb, lb));
   f3 live = sub(pa, pb);
   f3 c0v = xyz(d.J.c0v[j]);
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
@@ -1362,9 +1413,9 @@ __device__ bool solve_6x6_dev(const float* L21, const float* rhs, float* x) {
 // Inertial block and rhs of body h (assembly.rs:74-96): acc[0..20] lower
 // triangle, acc[21..26] rhs.
 __device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, int i, int j) {
-  float4 p = d.b.pos[h];
+  float4 p = d.sb[4 * (size_t)h];
   float m_dt2 = 1.0f / p.w / (K_DT * K_DT);
-  qf q = asq(d.b.rot[h]);
+  qf q = asq(d.sb[4 * (size_t)h + 1]);
   float iw[3][3];
   world_inertia(xyz(d.b.inertia[h]), q, iw);
   if (e < 21) {
@@ -1530,11 +1581,12 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
   for (int k = 0; k < 6; ++k) rhsv[k] = -sums[tid][21 + k];
   if (solve_6x6_dev(L21, rhsv, dx)) {
-    float4 p = d.b.pos[hb];
-    d.b.pos[hb] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
-    qf q = quat_integrate(asq(d.b.rot[hb]), f3{dx[3], dx[4], dx[5]});
-    d.b.rot[hb] = pkq(q);
-    d.b.drot[hb] = pk(quat_diff_vec(q, asq(d.b.rot0[hb])), 0.0f);
+    float4* rec = d.sb + 4 * (size_t)hb;
+    float4 p = rec[0];
+    rec[0] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
+    qf q = quat_integrate(asq(rec[1]), f3{dx[3], dx[4], dx[5]});
+    rec[1] = pkq(q);
+    rec[3] = pk(quat_diff_vec(q, asq(d.b.rot0[hb])), 0.0f);
   } else {
     atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
   }
@@ -1548,15 +1600,17 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
   if (q >= d.npts) return;
   uint32_t p = d.Q.pair[q];
   uint2 ab = d.P.ab[p];
-  float4 n4 = d.P.n[p];
+  float4 n4 = d.P.n(p);
   f3 n = xyz(n4);
   bool touching = n4.w <= 0.0f;
-  float4 ra4 = d.Q.ra[q], rb4 = d.Q.rb[q];
+  float4 ra4 = d.Q.ra(q), rb4 = d.Q.rb(q);
   float c0 = touching ? ra4.w : 0.0f;
-  float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
-  float4 a0 = d.b.pos0[ab.x], b0 = d.b.pos0[ab.y];
-  qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
-  f3 dra = xyz(d.b.drot[ab.x]), drb = xyz(d.b.drot[ab.y]);
+  const float4* sa = d.sb + 4 * (size_t)ab.x;
+  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  float4 pa4 = sa[0], pb4 = sbb[0];
+  float4 a0 = sa[2], b0 = sbb[2];
+  qf qa = asq(sa[1]), qb = asq(sbb[1]);
+  f3 dra = xyz(sa[3]), drb = xyz(sbb[3]);
   f3 dpa = sub(xyz(pa4), xyz(a0)), dpb = sub(xyz(pb4), xyz(b0));
   f3 ra_w = a0.w >= 0.0f ? mul(neg(n), a0.w) : qrot(qa, xyz(ra4));
   f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(rb4));
@@ -1566,14 +1620,14 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
     return cc * (1.0f - K_ALPHA) + dot3(u, sub(d_a, d_b));
   };
   float cn = row_c(n, c0, ra_w, rb_w);
-  f3 t1 = xyz(d.P.t1[p]), t2 = xyz(d.P.t2[p]);
+  f3 t1 = xyz(d.P.t1(p)), t2 = xyz(d.P.t2(p));
   float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
   f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
   f3 rb_f = add(rb_w, mul(n, sep * 0.5f));
   float ct1 = row_c(t1, 0.0f, ra_f, rb_f);
   float ct2 = row_c(t2, 0.0f, ra_f, rb_f);
-  float4 pen = d.Q.pen[q], lam = d.Q.lam[q];
-  float4 mu = d.P.mu[p];
+  float4 pen = d.Q.pen(q), lam = d.Q.lam(q);
+  float4 mu = d.P.mu(p);
   float fn_c = fmin_(pen.x * cn + lam.x, 0.0f);
   float r0 = pen.y * ct1 + lam.y, r1 = pen.z * ct2 + lam.z;
   float b1 = fabsf(fn_c) * mu.x, b2 = fabsf(fn_c) * mu.y;
@@ -1592,7 +1646,7 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
     float fs = sqrtf(r0 * r0 + r1 * r1);
     bool stuck = fs <= bnd && sqrtf(ct1 * ct1 + ct2 * ct2) < K_STICK;
     rb4.w = stuck ? 1.0f : 0.0f;
-    d.Q.rb[q] = rb4;
+    d.Q.rb(q) = rb4;
   }
   float4 nl = lam, npn = pen;
   if (touching && fabsf(cn) >= K_C_EPS) {
@@ -1609,8 +1663,8 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
       if (fabsf(ct2) >= K_C_EPS) npn.z = fmin_(pen.z + K_BETA * fabsf(ct2), K_PEN_MAX);
     }
   }
-  d.Q.lam[q] = nl;
-  d.Q.pen[q] = npn;
+  d.Q.lam(q) = nl;
+  d.Q.pen(q) = npn;
   if (touching && (mu.z > 0.0f || mu.w > 0.0f)) {
     f3 wrel = sub(dra, drb);
     float lam_n = fabsf(nl.x);
@@ -1632,8 +1686,10 @@ __global__ void k_dual_joint(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
   uint2 ab = d.J.ab[j];
-  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(asq(d.b.rot[ab.x]), xyz(d.J.la[j])));
-  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(asq(d.b.rot[ab.y]), xyz(d.J.lb[j])));
+  const float4* sa = d.sb + 4 * (size_t)ab.x;
+  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  f3 pa = add(xyz(sa[0]), qrot(asq(sa[1]), xyz(d.J.la[j])));
+  f3 pb = add(xyz(sbb[0]), qrot(asq(sbb[1]), xyz(d.J.lb[j])));
   f3 live = sub(pa, pb);
   f3 c0v = xyz(d.J.c0v[j]);
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
@@ -1671,7 +1727,7 @@ __global__ void k_bdf1(Dev d) {
 __global__ void k_supported(Dev d) {
   uint32_t p = gid();
   if (p < d.npairs) {
-    if (d.P.n[p].w <= 0.0f && d.P.pts[p].y > 0u) {
+    if (d.P.n(p).w <= 0.0f && d.P.pts[p].y > 0u) {
       uint2 ab = d.P.ab[p];
       d.b.supported[ab.x] = 1;
       d.b.supported[ab.y] = 1;

@@ -41,7 +41,12 @@
 #define K_CCD_GATE 0.5f
 #define K_JOINT_REST 0.005f
 #define K_MAX_POINTS 8
-#define COLOUR_BATCH 4  // colouring rounds launched per host check
+#define COLOUR_BATCH 8  // colouring rounds launched per host check
+#define CSTAT_HEAD 64    // words before the colour sizes in cstat: the round-count ring, the colour
+                         // count and the sizes sit on separate 128-byte lines (their atomics do not contend)
+#define CSTAT_MAX 32     // word of the colour count
+#define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
+static_assert(CSTAT_MAX >= 2 * COLOUR_BATCH && CSTAT_HEAD >= CSTAT_MAX + 32, "cstat layout");
 #define TOI_LANES 16    // lanes cooperating on one TOI cast
 #define BP_CACHE 16     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
@@ -87,22 +92,31 @@ struct Bodies {
   uint32_t* ji;
 };
 
+// Pairs and points keep the fields one solve reads together in one
+// 64-byte record (4 x float4), so a gather touches one cache line.
 struct Pairs {
   uint2* ab;         // body indices, a < b
-  float4* n;         // normal xyz (B -> A), w = signed gap
-  float4* t1;        // friction tangent t1, w = damper stiffness (separated pairs)
-  float4* t2;        // t2 = t1 x n
-  float4* mu;        // mu1, mu2, rolling, torsion coefficients
+  float4* rec;       // 4 per pair: see n / t1 / t2 / mu
   uint2* pts;        // first point, point count
+  // normal xyz (B -> A), w = signed gap
+  __host__ __device__ float4& n(size_t p) const { return rec[4 * p]; }
+  // friction tangent t1, w = damper stiffness (separated pairs)
+  __host__ __device__ float4& t1(size_t p) const { return rec[4 * p + 1]; }
+  // t2 = t1 x n
+  __host__ __device__ float4& t2(size_t p) const { return rec[4 * p + 2]; }
+  // mu1, mu2, rolling, torsion coefficients
+  __host__ __device__ float4& mu(size_t p) const { return rec[4 * p + 3]; }
 };
-
 struct Points {
-  float4* ra;        // anchor on A (body frame), w = step-start violation c0
-  float4* rb;        // anchor on B (body frame), w = stuck (0/1)
-  float4* lam;       // duals xyz
-  float4* pen;       // penalties xyz
+  float4* rec;       // 4 per point: see ra / rb / lam / pen
   float4* roll;      // rolling/torsion duals xyz
   uint32_t* pair;    // owning pair
+  // anchor on A (body frame), w = step-start violation c0
+  __host__ __device__ float4& ra(size_t q) const { return rec[4 * q]; }
+  // anchor on B (body frame), w = stuck (0/1)
+  __host__ __device__ float4& rb(size_t q) const { return rec[4 * q + 1]; }
+  __host__ __device__ float4& lam(size_t q) const { return rec[4 * q + 2]; }  // duals xyz
+  __host__ __device__ float4& pen(size_t q) const { return rec[4 * q + 3]; }  // penalties xyz
 };
 
 struct Joints {
@@ -127,8 +141,14 @@ struct Grid {
   uint32_t* bucket;  // per body bucket id (0xffffffff = large / not gridded)
   uint32_t nbuckets; // power of two
   float cell;
-  uint32_t* large;   // large body ids
+  uint32_t* large;   // large body ids, ascending
   uint32_t nlarge;
+  uint32_t* lsc_off; // per dense scene: range of its large bodies in lsc
+  uint32_t* lsc;     // large bodies grouped by scene (ascending within a scene)
+  uint32_t* sitems;  // all bodies sorted by (scene, index)
+  uint32_t* soff;    // per dense scene: its range in sitems
+  uint4* lrange;     // per large body k: scene range in sitems (begin, end), flag base, body id
+  uint32_t lflat;    // sum over large bodies of their scene's size (flag rows)
   uint8_t* is_large;
 };
 
@@ -165,6 +185,8 @@ struct Dev {
   uint32_t* cand_off;  // n+1 candidate offsets per first body
   uint32_t* acc_off;   // ncand+1 exclusive scan of accepted
   uint32_t* poff;      // n+1 first pair of each body's a-run in the current table
+  float4* sb;          // 4 per body during the primal/dual loop: pos, rot, pos0, drot
+                       // (one 64-byte record, so a partner's state is one cache line)
   uint32_t* poff_old;  // n+1 the same for the previous step's table
   uint32_t* pt_off;    // ncand+1 exclusive scan of fresh points
   uint32_t* ptcnt;     // per body point-side counts
@@ -174,10 +196,12 @@ struct Dev {
   uint32_t* nbr_off;   // n+1
   uint32_t* nbr_cur;
   uint32_t* nbr;
-  uint32_t* remain;    // bodies left uncoloured after each round of a batch
-  uint32_t* lflag;     // nlarge x n flags of large-body candidates
-  uint32_t* lrank;     // nlarge x (n+1) scans
-  uint32_t* colour_cnt;
+  uint32_t* remain;    // cstat: bodies left uncoloured after each round (ring of two batches)
+  uint32_t* lflag;     // per large body, one flag per body of its scene (lflat in all)
+  uint32_t* lrank;     // exclusive scan of lflag (lflat + 1)
+  uint32_t* cstat;     // colouring status: remain ring 2 x COLOUR_BATCH, colour count, sizes
+  uint32_t* cmax;      // cstat + CSTAT_MAX: number of colours
+  uint32_t* colour_cnt;  // cstat + CSTAT_HEAD: bodies per colour
   uint32_t* colour_off;
   uint32_t* colour_cur;
   uint32_t* colour_list;
@@ -204,5 +228,6 @@ enum {
   CTR_PTSIDES = 7,
   CTR_PAIRSIDES = 8,
   CTR_JSIDES = 9,
+  CTR_SCANERR = 10,
   CTR_COUNT = 12
 };

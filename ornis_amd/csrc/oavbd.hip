@@ -74,6 +74,7 @@ struct oavbd_ctx {
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
   uint32_t* scan_scratch = nullptr;
+  LookbackScan ls;
   uint32_t* counters_h = nullptr;  // pinned
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
@@ -129,12 +130,12 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.counters, CTR_COUNT);
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
-  dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1);
+  dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4);
   dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
-  dalloc(&d.nbr_cur, n + 1); dalloc(&d.remain, COLOUR_BATCH);
+  dalloc(&d.nbr_cur, n + 1);
   d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
-  dalloc(&d.colour_list, n); dalloc(&d.colour_cnt, 65536); dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
+  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
@@ -146,9 +147,21 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.g.items, n); dalloc(&d.g.ipos, n); dalloc(&d.g.ikey, n); dalloc(&d.g.hkey, n); dalloc(&d.g.bucket, n); dalloc(&d.g.is_large, n);
   dalloc(&d.bp_cache, (size_t)n * BP_CACHE); dalloc(&d.toi_cache, (size_t)n * TOI_CACHE);
   d.g.large = nullptr;
+  d.g.lsc_off = nullptr;
+  d.g.lsc = nullptr;
+  d.g.sitems = nullptr;
+  d.g.soff = nullptr;
+  d.g.lrange = nullptr;
+  d.g.lflat = 0;
   d.lflag = nullptr;
   d.lrank = nullptr;
   dalloc(&c->scan_scratch, ScanScratch::need((size_t)1 << 28));
+  c->ls.max_tiles = (uint32_t)((((size_t)1 << 28) + SCAN_TILE - 1) / SCAN_TILE) + 1;
+  dalloc(&c->ls.state, c->ls.max_tiles);
+  dalloc(&c->ls.ticket, 1);
+  HIPC(hipMemset(c->ls.state, 0, (size_t)c->ls.max_tiles * sizeof(unsigned long long)));
+  HIPC(hipMemset(c->ls.ticket, 0, sizeof(uint32_t)));
+  c->ls.epoch = 0;
   d.blist = nullptr;
   d.inc = nullptr;
   d.acc_off = nullptr;
@@ -156,17 +169,16 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
 }
 
 static void free_pairs(Pairs& P) {
-  dfree(P.ab); dfree(P.n); dfree(P.t1); dfree(P.t2); dfree(P.mu); dfree(P.pts);
+  dfree(P.ab); dfree(P.rec); dfree(P.pts);
 }
 static void alloc_pairs(Pairs& P, size_t cap) {
-  dalloc(&P.ab, cap); dalloc(&P.n, cap); dalloc(&P.t1, cap); dalloc(&P.t2, cap); dalloc(&P.mu, cap); dalloc(&P.pts, cap);
+  dalloc(&P.ab, cap); dalloc(&P.rec, 4 * cap); dalloc(&P.pts, cap);
 }
 static void free_points(Points& Q) {
-  dfree(Q.ra); dfree(Q.rb); dfree(Q.lam); dfree(Q.pen); dfree(Q.roll); dfree(Q.pair);
+  dfree(Q.rec); dfree(Q.roll); dfree(Q.pair);
 }
 static void alloc_points(Points& Q, size_t cap) {
-  dalloc(&Q.ra, cap); dalloc(&Q.rb, cap); dalloc(&Q.lam, cap); dalloc(&Q.pen, cap); dalloc(&Q.roll, cap);
-  dalloc(&Q.pair, cap);
+  dalloc(&Q.rec, 4 * cap); dalloc(&Q.roll, cap); dalloc(&Q.pair, cap);
 }
 
 static size_t grow(size_t need) {
@@ -195,7 +207,7 @@ static void ensure_pairs(oavbd_ctx* c, size_t need) {
 }
 
 static void ensure_points(oavbd_ctx* c, size_t need) {
-  if (need <= c->cap_pts && c->d.Q.ra) return;
+  if (need <= c->cap_pts && c->d.Q.rec) return;
   free_points(c->d.Q);
   c->cap_pts = grow(need);
   alloc_points(c->d.Q, c->cap_pts);
@@ -232,15 +244,15 @@ static void free_all(oavbd_ctx* c) {
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
-  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large);
+  dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
   dfree(d.g.is_large);
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
-  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
-  dfree(d.colour_cnt); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
+  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
+  dfree(d.cstat); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
-  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(d.remain); dfree(c->cand_cnt); dfree(c->scan_scratch);
+  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->scan_scratch); dfree(c->ls.state); dfree(c->ls.ticket);
 }
 
 // ---------------------------------------------------------------------------
@@ -257,6 +269,11 @@ __global__ void k_zero_u8(uint8_t* p, uint32_t n) {
 
 static void zero_u32(oavbd_ctx* c, uint32_t* p, uint32_t n) {
   if (n) k_zero_u32<<<blocks(n), TPB, 0, c->st>>>(p, n);
+}
+
+// Exclusive scan on the context stream (single launch, see avbd_scan.hpp).
+static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
+  scan_lookback(c->ls, in, out, n, c->st, c->d.counters + CTR_SCANERR);
 }
 
 static void read_counters(oavbd_ctx* c) {
@@ -367,7 +384,7 @@ static void build_grid(oavbd_ctx* c) {
   Dev& d = c->d;
   k_grid_clear<<<blocks(d.g.nbuckets), TPB, 0, c->st>>>(d);
   k_grid_count<<<blocks(d.n), TPB, 0, c->st>>>(d);
-  scan_excl(d.g.cnt, d.g.start, d.g.nbuckets, c->st, c->scan_scratch);
+  scan(c, d.g.cnt, d.g.start, d.g.nbuckets);
   k_grid_fill<<<blocks(d.n), TPB, 0, c->st>>>(d);
 }
 
@@ -376,16 +393,15 @@ static void broadphase(oavbd_ctx* c) {
   uint32_t n = d.n;
   k_bp_count<<<blocks(n), TPB, 0, c->st>>>(d, c->cand_cnt);
   if (d.g.nlarge) {
-    k_bp_large_flag<<<blocks(n), TPB, 0, c->st>>>(d);
-    for (uint32_t k = 0; k < d.g.nlarge; ++k)
-      scan_excl(d.lflag + (size_t)k * n, d.lrank + (size_t)k * (n + 1), n, c->st, c->scan_scratch);
+    k_bp_large_flag<<<blocks(d.g.lflat), TPB, 0, c->st>>>(d);
+    scan(c, d.lflag, d.lrank, d.g.lflat);  // one scan over all large bodies' flag rows
     k_bp_large_count<<<blocks(d.g.nlarge), TPB, 0, c->st>>>(d, c->cand_cnt);
   }
-  scan_excl(c->cand_cnt, d.cand_off, n, c->st, c->scan_scratch);
+  scan(c, c->cand_cnt, d.cand_off, n);
   c->ncand = read_u32(c, d.cand_off + n);
   ensure_cand(c, c->ncand);
   k_bp_fill<<<blocks(n), TPB, 0, c->st>>>(d);
-  if (d.g.nlarge) k_bp_large_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.g.nlarge) k_bp_large_fill<<<blocks(d.g.lflat), TPB, 0, c->st>>>(d);
 }
 
 static void narrow_and_merge(oavbd_ctx* c) {
@@ -399,9 +415,9 @@ static void narrow_and_merge(oavbd_ctx* c) {
   d.npts_old = d.npts;
   uint32_t nc = c->ncand;
   if (nc) k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
-  scan_excl(d.C.acc, d.acc_off, nc, c->st, c->scan_scratch);
+  scan(c, d.C.acc, d.acc_off, nc);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
-  scan_excl(d.C.npts, d.pt_off, nc, c->st, c->scan_scratch);
+  scan(c, d.C.npts, d.pt_off, nc);
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT + 1, d.pt_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipStreamSynchronize(c->st));
@@ -417,49 +433,49 @@ static void narrow_and_merge(oavbd_ctx* c) {
 static void incidence_and_colour(oavbd_ctx* c) {
   Dev& d = c->d;
   uint32_t n = d.n;
-  k_colour_init<<<blocks(n), TPB, 0, c->st>>>(d);
-  zero_u32(c, d.ptcnt, n + 1);
-  zero_u32(c, d.bcnt, n + 1);
-  zero_u32(c, d.bcur, n + 1);
-  zero_u32(c, d.nbr_cnt, n + 1);
-  zero_u32(c, d.nbr_cur, n + 1);
+  k_colour_init<<<blocks(std::max<uint32_t>(n + 1, CSTAT_HEAD + 65536)), TPB, 0, c->st>>>(d);
   if (d.npairs) k_inc_count<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_count<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-  scan_excl(d.ptcnt, d.inc_off, n, c->st, c->scan_scratch);
-  scan_excl(d.bcnt, d.boff, n, c->st, c->scan_scratch);
-  scan_excl(d.nbr_cnt, d.nbr_off, n, c->st, c->scan_scratch);
+  scan(c, d.ptcnt, d.inc_off, n);
+  scan(c, d.bcnt, d.boff, n);
+  scan(c, d.nbr_cnt, d.nbr_off, n);
   if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
   // Jones-Plassmann colouring, COLOUR_BATCH rounds per host check (rounds
-  // after the last coloured body find nothing to do)
-  zero_u32(c, d.colour_cnt, 65536);
-  zero_u32(c, d.counters + CTR_MAXCOL, 1);
-  uint32_t rem[COLOUR_BATCH];
+  // after the last coloured body find nothing to do).  Each check copies the
+  // round counts, the colour count and the first colour sizes in one go.
+  // The round counts use two halves of a ring: batch b counts into half
+  // b % 2 and clears half (b + 1) % 2 for the next batch.
+  uint32_t head[CSTAT_HEAD + CSTAT_EAGER];
   int rounds = 0;
-  for (int base = 0; rounds == 0; base += COLOUR_BATCH) {
+  for (int base = 0, b = 0; rounds == 0; base += COLOUR_BATCH, ++b) {
     if (base > 30000) throw HipError{"colouring did not converge"};
-    zero_u32(c, d.remain, COLOUR_BATCH);
-    for (int r = 0; r < COLOUR_BATCH; ++r) k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, base + r, d.remain + r);
-    HIPC(hipMemcpyAsync(rem, d.remain, sizeof(rem), hipMemcpyDeviceToHost, c->st));
+    uint32_t* cur = d.remain + (b & 1) * COLOUR_BATCH;
+    uint32_t* nxt = d.remain + ((b + 1) & 1) * COLOUR_BATCH;
+    for (int r = 0; r < COLOUR_BATCH; ++r)
+      k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, base + r, cur + r, nxt + r);
+    HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
     HIPC(hipStreamSynchronize(c->st));
     for (int r = 0; r < COLOUR_BATCH; ++r)
-      if (rem[r] == 0) {
+      if (head[(b & 1) * COLOUR_BATCH + r] == 0) {
         rounds = base + r + 1;
         break;
       }
   }
   c->stats.colour_rounds = (uint32_t)rounds;
-  uint32_t ncol = read_u32(c, d.counters + CTR_MAXCOL);
+  uint32_t ncol = head[CSTAT_MAX];
   c->colour_cnt_h.assign(ncol, 0);
   c->colour_off_h.assign(ncol + 1, 0);
   if (ncol) {
-    HIPC(hipMemcpy(c->colour_cnt_h.data(), d.colour_cnt, ncol * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (ncol <= CSTAT_EAGER)
+      std::copy(head + CSTAT_HEAD, head + CSTAT_HEAD + ncol, c->colour_cnt_h.begin());
+    else
+      HIPC(hipMemcpy(c->colour_cnt_h.data(), d.colour_cnt, ncol * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t k = 0; k < ncol; ++k) c->colour_off_h[k + 1] = c->colour_off_h[k] + c->colour_cnt_h[k];
     HIPC(hipMemcpyAsync(d.colour_off, c->colour_off_h.data(), (ncol + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                         c->st));
   }
-  zero_u32(c, d.colour_cur, std::max<uint32_t>(ncol, 1));
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   c->stats.colours = ncol;
   c->stats.max_colour = 0;
@@ -512,12 +528,13 @@ static void step_inner(oavbd_ctx* c) {
   zero_u32(c, d.counters, CTR_COUNT);
   if (c->sleeping) wake_joint_motion(c);
   k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
-  uint32_t nfast = read_u32(c, d.counters + CTR_FAST);
   build_grid(c);
-  if (nfast) {
-    k_toi_count<<<blocks(n), TPB, 0, c->st>>>(d);
-    scan_excl(d.toi_cnt, d.toi_off, n, c->st, c->scan_scratch);
-    uint32_t ncast = read_u32(c, d.toi_off + n);
+  // TOI: every body resets its clamp; fast movers count their targets
+  k_toi_count<<<blocks(n), TPB, 0, c->st>>>(d);
+  scan(c, d.toi_cnt, d.toi_off, n);
+  uint32_t ncast = read_u32(c, d.toi_off + n);
+  c->stats.toi_casts = ncast;
+  if (ncast) {
     if (ncast > c->cap_toi) {
       dfree(d.toi_pairs);
       dfree(d.toi_res);
@@ -525,18 +542,12 @@ static void step_inner(oavbd_ctx* c) {
       dalloc(&d.toi_pairs, c->cap_toi);
       dalloc(&d.toi_res, c->cap_toi);
     }
-    c->stats.toi_casts = ncast;
-    if (ncast) {
-      k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
-      k_toi_pair<TOI_LANES><<<blocks((size_t)ncast * TOI_LANES), TPB, 0, c->st>>>(d, ncast);
-      k_toi_win<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
-    }
+    k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_toi_pair<TOI_LANES><<<blocks((size_t)ncast * TOI_LANES), TPB, 0, c->st>>>(d, ncast);
+    k_toi_win<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
     k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
     k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
     build_grid(c);
-  } else {
-    c->stats.toi_casts = 0;
-    k_toi_clear<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   broadphase(c);
   narrow_and_merge(c);
@@ -548,6 +559,7 @@ static void step_inner(oavbd_ctx* c) {
   if (d.njoints) k_joint_prep<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   if (d.npairs) k_pair_prep<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   incidence_and_colour(c);
+  k_sb_pack<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 1);
   const uint32_t ncol = c->stats.colours;
   // In timing mode every primal / dual launch is bracketed by pool events;
@@ -577,6 +589,7 @@ static void step_inner(oavbd_ctx* c) {
     nd_ev += 2;
   }
   record(c, 3);
+  k_sb_unpack<<<blocks(n), TPB, 0, c->st>>>(d);
   k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
   uint32_t ns = std::max(d.npairs, d.njoints);
   if (ns) k_supported<<<blocks(ns), TPB, 0, c->st>>>(d);
@@ -590,6 +603,7 @@ static void step_inner(oavbd_ctx* c) {
   k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 4);
   read_counters(c);
+  if (c->counters_h[CTR_SCANERR]) throw HipError{"prefix scan look-back did not complete"};
   c->sleeping = c->counters_h[CTR_SLEEPING];
   c->stats.sleeping = c->sleeping;
   c->stats.breakdowns = c->counters_h[CTR_BREAKDOWN];
@@ -754,7 +768,16 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
       msk[i] = mask ? mask[i] : 0xffffffffu;
       scn[i] = scene ? scene[i] : 0u;
       trig[i] = trigger ? trigger[i] : 0;
-      if (scn[i] >= per_scene_count.size()) per_scene_count.resize(scn[i] + 1, 0);
+    }
+    // dense scene ranks (the device compares ranks only) and scene-local
+    // indices (the colouring priority)
+    std::vector<uint32_t> ids(scn.begin(), scn.end());
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    const uint32_t nsc = (uint32_t)ids.size();
+    per_scene_count.assign(nsc, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+      scn[i] = (uint32_t)(std::lower_bound(ids.begin(), ids.end(), scn[i]) - ids.begin());
       loc[i] = per_scene_count[scn[i]]++;
       prio[i] = prio_hash(loc[i]);
     }
@@ -783,13 +806,49 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
     for (uint32_t i = 0; i < n; ++i)
       if (mat[i].z > 0.0f || mat[i].w > 0.0f) c->rows6 = true;
     d.g.nlarge = (uint32_t)large.size();
-    dfree(d.g.large);
+    // bodies by scene, large bodies by scene, and each large body's flag row
+    // over the bodies of its scene
+    std::vector<uint32_t> soff(nsc + 1, 0), sitems(n), lsc_off(nsc + 1, 0), lsc(large.size());
+    for (uint32_t i = 0; i < n; ++i) soff[scn[i] + 1]++;
+    for (uint32_t k = 0; k < nsc; ++k) soff[k + 1] += soff[k];
+    {
+      std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
+      for (uint32_t i = 0; i < n; ++i) sitems[cur[scn[i]]++] = i;
+    }
+    for (uint32_t L : large) lsc_off[scn[L] + 1]++;
+    for (uint32_t k = 0; k < nsc; ++k) lsc_off[k + 1] += lsc_off[k];
+    {
+      std::vector<uint32_t> cur(lsc_off.begin(), lsc_off.end() - 1);
+      for (uint32_t L : large) lsc[cur[scn[L]]++] = L;
+    }
+    std::vector<uint4> lrange(large.size());
+    size_t lflat = 0;
+    for (size_t k = 0; k < large.size(); ++k) {
+      uint32_t sc = scn[large[k]];
+      lrange[k] = make_uint4(soff[sc], soff[sc + 1], (uint32_t)lflat, large[k]);
+      lflat += soff[sc + 1] - soff[sc];
+    }
+    if (lflat >= ((size_t)1 << 31)) throw HipError{"too many large-body candidates"};
+    d.g.lflat = (uint32_t)lflat;
+    dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
     dalloc(&d.g.large, large.size());
-    if (!large.empty()) HIPC(hipMemcpy(d.g.large, large.data(), large.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    dalloc(&d.g.soff, nsc + 1);
+    dalloc(&d.g.lsc_off, nsc + 1);
+    dalloc(&d.g.lsc, large.size());
+    dalloc(&d.g.sitems, n);
+    dalloc(&d.g.lrange, large.size());
+    if (!large.empty()) {
+      HIPC(hipMemcpy(d.g.large, large.data(), large.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      HIPC(hipMemcpy(d.g.lsc, lsc.data(), large.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+      HIPC(hipMemcpy(d.g.lrange, lrange.data(), large.size() * sizeof(uint4), hipMemcpyHostToDevice));
+    }
+    HIPC(hipMemcpy(d.g.lsc_off, lsc_off.data(), (nsc + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.g.sitems, sitems.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d.g.soff, soff.data(), (nsc + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
     dfree(d.lflag);
     dfree(d.lrank);
-    dalloc(&d.lflag, (size_t)d.g.nlarge * n);
-    dalloc(&d.lrank, (size_t)d.g.nlarge * (n + 1));
+    dalloc(&d.lflag, lflat);
+    dalloc(&d.lrank, lflat + 1);
     auto up4 = [&](float4* dst, const std::vector<float4>& v) {
       HIPC(hipMemcpy(dst, v.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     };
@@ -935,22 +994,23 @@ int oavbd_get_points(oavbd_ctx* c, float* out16, uint32_t cap, uint32_t* n) {
   uint32_t m = std::min(cap, c->d.npts);
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
-    std::vector<float4> ra(m), rb(m), lam(m), pen(m), roll(m);
+    std::vector<float4> rec(4 * (size_t)m), roll(m);
     if (m) {
-      HIPC(hipMemcpy(ra.data(), c->d.Q.ra, m * sizeof(float4), hipMemcpyDeviceToHost));
-      HIPC(hipMemcpy(rb.data(), c->d.Q.rb, m * sizeof(float4), hipMemcpyDeviceToHost));
-      HIPC(hipMemcpy(lam.data(), c->d.Q.lam, m * sizeof(float4), hipMemcpyDeviceToHost));
-      HIPC(hipMemcpy(pen.data(), c->d.Q.pen, m * sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(rec.data(), c->d.Q.rec, 4 * (size_t)m * sizeof(float4), hipMemcpyDeviceToHost));
       HIPC(hipMemcpy(roll.data(), c->d.Q.roll, m * sizeof(float4), hipMemcpyDeviceToHost));
     }
     for (uint32_t i = 0; i < m; ++i) {
+      const float4& ra = rec[4 * (size_t)i];
+      const float4& rb = rec[4 * (size_t)i + 1];
+      const float4& lam = rec[4 * (size_t)i + 2];
+      const float4& pen = rec[4 * (size_t)i + 3];
       float* o = out16 + 16 * (size_t)i;
-      o[0] = ra[i].x; o[1] = ra[i].y; o[2] = ra[i].z;
-      o[3] = rb[i].x; o[4] = rb[i].y; o[5] = rb[i].z;
-      o[6] = lam[i].x; o[7] = lam[i].y; o[8] = lam[i].z;
-      o[9] = pen[i].x; o[10] = pen[i].y; o[11] = pen[i].z;
+      o[0] = ra.x; o[1] = ra.y; o[2] = ra.z;
+      o[3] = rb.x; o[4] = rb.y; o[5] = rb.z;
+      o[6] = lam.x; o[7] = lam.y; o[8] = lam.z;
+      o[9] = pen.x; o[10] = pen.y; o[11] = pen.z;
       o[12] = roll[i].x; o[13] = roll[i].y; o[14] = roll[i].z;
-      o[15] = rb[i].w;
+      o[15] = rb.w;
     }
   });
   return OAVBD_OK;
