@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -28,23 +30,37 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "body-steps/s @100k boxes (10 AVBD iters) + % HBM roofline; batched scenes/s"
 
 
-def cpu_baseline(scene, seconds, max_steps):
+def cpu_baseline(scene, seconds, max_steps, start_step):
     """The CPU oracle in the reference's own order (reverse-handle sweep,
-    history-ordered pairs, libm), one thread, on the first steps of the
-    same workload, until `seconds` of CPU time or `max_steps` steps."""
+    history-ordered pairs, libm), one thread, on a bounded sample of the
+    same workload: the steps that follow step `start_step` of the GPU
+    trajectory (the GPU state there becomes the oracle's initial state;
+    contact history is rebuilt), until `seconds` of CPU time or
+    `max_steps` steps."""
     from oracle import pyoracle as po
+    from ornis_amd.avbd import AvbdEngine
+    s = dict(scene)
+    if start_step:
+        g = AvbdEngine(scene=scene, iters=10)
+        g.step_n(start_step)
+        st = g.state()
+        g.close()
+        s["pos"], s["quat"], s["vel"], s["angvel"] = (np.ascontiguousarray(st[:, 0:3]), np.ascontiguousarray(st[:, 3:7]),
+                                                      np.ascontiguousarray(st[:, 7:10]), np.ascontiguousarray(st[:, 10:13]))
     n_dyn = int((scene["body_type"] == 1).sum())
-    e = po.OracleEngine(scene, order=po.ORDER_REFERENCE, math=po.MATH_LIBM)
+    e = po.OracleEngine(s, order=po.ORDER_REFERENCE, math=po.MATH_LIBM)
     done, t0 = 0, time.perf_counter()
     while done < max_steps:
         e.step_inner(1)
         done += 1
+        print(f"bench.py: cpu baseline step {done}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
     return {"value": n_dyn * done / dt, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (reference sweep order, grid-accelerated discovery, FP32, 1 thread), "
-                      f"steps 1-{done} of the same 100k-box workload from its initial state, {dt:.1f} s"}
+            "sample": f"CPU oracle (reference sweep order, grid-accelerated discovery, FP32, 1 thread): "
+                      f"steps {start_step + 1}-{start_step + done} of the same 100k-box workload, starting from "
+                      f"the GPU trajectory's state at step {start_step} (contact history rebuilt), {dt:.1f} s"}
 
 
 def batched_leg(steps, loop_scenes, loop_steps):
@@ -111,7 +127,9 @@ def main():
     ap.add_argument("--steps", type=int, default=290)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-max-steps", type=int, default=4)
+    ap.add_argument("--cpu-max-steps", type=int, default=40)
+    ap.add_argument("--cpu-start-step", type=int, default=150,
+                    help="the CPU sample continues the trajectory from this step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="primal lanes per body (0: library default)")
@@ -191,7 +209,7 @@ def main():
         batched = batched_leg(args.batched_steps, args.loop_scenes, args.loop_steps)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_max_steps)
+        cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_max_steps, args.cpu_start_step)
     line = {
         "metric": METRIC,
         "value": value,
