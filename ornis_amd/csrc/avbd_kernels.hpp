@@ -1567,20 +1567,63 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     }
     __syncthreads();
   }
+  bool solved;
+  float dx[6];
+  if constexpr (NE == 1) {
+    // LDL^T factorization and forward substitution over the body's lanes,
+    // right-looking: step k finalises column k (l_ik = a_ik / d_k) and
+    // subtracts (l_ik * d_k) * l_jk from every later entry, and y_k l_rk
+    // from every later rhs entry.  Every entry sees the same operations in
+    // the same order as in solve_6x6's row-by-row form, so the bits agree.
+    const int e = sub;
+    const bool is_mat = e < 21, is_rhs = e >= 21 && e < 27;
+    const int i = ia[0], j = ib[0];  // rhs lanes: i = rhs row
+    float val = is_rhs ? -acc[0] : acc[0];
+    bool ok = true;
 #pragma unroll
-  for (int q = 0; q < NE; ++q) {
-    int e = sub + q * L;
-    if (e < 27) sums[grp][e] = acc[q];
+    for (int k = 0; k < 6; ++k) {
+      const float dgk = __shfl(val, tri(k, k), L);
+      ok = ok && !(dgk <= K_DEGENERATE);
+      if (is_mat && j == k && i > k) val = val / dgk;
+      const float li = __shfl(val, tri(i > k ? i : k, k), L);
+      const float lj = __shfl(val, tri(j > k ? j : k, k), L);
+      if (is_mat && j > k) val -= (li * dgk) * lj;
+      const float yk = __shfl(val, 21 + k, L);
+      if (is_rhs && i > k) val -= li * yk;
+    }
+    const float dgi = __shfl(val, tri(i < 6 ? i : 5, i < 6 ? i : 5), L);
+    if (is_rhs) val = val / dgi;  // z
+    if (e < 27) sums[grp][e] = val;
+    if (sub == 0) sums[grp][27] = ok ? 1.0f : 0.0f;
+    __syncthreads();
+    if (tid >= NB) return;
+    // back substitution, one lane per body
+    solved = sums[tid][27] != 0.0f;
+    const float* f = sums[tid];
+#pragma unroll
+    for (int r = 5; r >= 0; --r) {
+      float sacc = f[21 + r];
+#pragma unroll
+      for (int k = r + 1; k < 6; ++k) sacc -= f[tri(k, r)] * dx[k];
+      dx[r] = sacc;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      int e = sub + q * L;
+      if (e < 27) sums[grp][e] = acc[q];
+    }
+    __syncthreads();
+    if (tid >= NB) return;
+    float L21[21], rhsv[6];
+    for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
+    for (int k = 0; k < 6; ++k) rhsv[k] = -sums[tid][21 + k];
+    solved = solve_6x6_dev(L21, rhsv, dx);
   }
-  __syncthreads();
-  if (tid >= NB) return;
   const uint32_t s2 = blockIdx.x * NB + tid;
   if (s2 >= count) return;
   const uint32_t hb = list[s2];
-  float L21[21], rhsv[6], dx[6];
-  for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
-  for (int k = 0; k < 6; ++k) rhsv[k] = -sums[tid][21 + k];
-  if (solve_6x6_dev(L21, rhsv, dx)) {
+  if (solved) {
     float4* rec = d.sb + 4 * (size_t)hb;
     float4 p = rec[0];
     rec[0] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
