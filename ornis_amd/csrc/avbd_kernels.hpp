@@ -284,8 +284,8 @@ __device__ __forceinline__ bool sweep_may_touch(f3 p0, f3 dir, float len, float 
   float s = dot3(w, dir);
   s = s < 0.0f ? 0.0f : (s > len ? len : s);
   f3 c = sub(w, mul(dir, s));
-  float reach = rm + ro + K_SHAPE_TOUCH;
-  return len3(c) <= reach * 1.001f + 1e-4f;
+  float lim = (rm + ro + K_SHAPE_TOUCH) * 1.001f + 1e-4f;
+  return len2(c) <= lim * lim * 1.0001f;
 }
 
 // Candidate targets of fast mover h: grid members near its swept bounding
@@ -483,9 +483,17 @@ __device__ __forceinline__ bool prefilter_ids(const Dev& d, uint32_t a, uint32_t
   if (!pair_allowed_dev(d, a, b)) return false;
   return !(!d.b.trigger[a] && !d.b.trigger[b] && d.b.jp_off[a] != d.b.jp_off[a + 1] && jointed_dev(d, a, b));
 }
-// Bounding spheres within GEN_MARGIN.
+// Bounding spheres within GEN_MARGIN: |pa - pb| <= ra + rb + GEN_MARGIN as
+// the reference evaluates it.  Squared distances clearly inside or outside
+// decide without the square root; the narrow band around the limit takes
+// the exact form.
 __device__ __forceinline__ bool spheres_near(f3 pa, float ra, f3 pb, float rb) {
-  return !(len3(sub(pa, pb)) > ra + rb + K_GEN_MARGIN);
+  const f3 dv = sub(pa, pb);
+  const float r = ra + rb + K_GEN_MARGIN;
+  const float d2 = dot3(dv, dv), r2 = r * r;
+  if (d2 > r2 * 1.0001f) return false;
+  if (d2 < r2 * 0.9999f) return true;
+  return !(sqrtf(d2) > r);
 }
 __device__ __forceinline__ bool prefilter(const Dev& d, uint32_t a, uint32_t b) {
   return spheres_near(xyz(d.b.pos[a]), d.b.shape[a].w, xyz(d.b.pos[b]), d.b.shape[b].w) && prefilter_ids(d, a, b);
@@ -559,17 +567,30 @@ __global__ void k_bp_large_count(Dev d, uint32_t* cnt) {
   cnt[r.w] = d.lrank[r.z + (r.y - r.x)] - d.lrank[r.z];
 }
 
-__global__ void k_bp_fill(Dev d) {
+// Candidates of each small body, sorted by b: lists of up to BP_CACHE are
+// sorted in a per-thread LDS slice, longer ones in place.
+__global__ void __launch_bounds__(TPB) k_bp_fill(Dev d) {
+  __shared__ uint32_t srt[BP_CACHE][TPB];  // column per thread: no bank conflicts between lanes
   uint32_t a = gid();
   if (a >= d.n || d.g.is_large[a]) return;
   uint2* out = d.C.ab + d.cand_off[a];
   const uint32_t m = d.cand_off[a + 1] - d.cand_off[a];
+  const uint32_t t = threadIdx.x;
   if (m <= BP_CACHE) {
     const uint32_t* cache = d.bp_cache + (size_t)a * BP_CACHE;
-    for (uint32_t i = 0; i < m; ++i) out[i] = make_uint2(a, cache[i]);
-  } else {
-    bp_small(d, a, [&](uint32_t i, uint32_t o) { out[i] = make_uint2(a, o); });
+    for (uint32_t i = 0; i < m; ++i) {
+      uint32_t v = cache[i];
+      uint32_t j = i;
+      while (j > 0 && srt[j - 1][t] > v) {
+        srt[j][t] = srt[j - 1][t];
+        --j;
+      }
+      srt[j][t] = v;
+    }
+    for (uint32_t i = 0; i < m; ++i) out[i] = make_uint2(a, srt[i][t]);
+    return;
   }
+  bp_small(d, a, [&](uint32_t i, uint32_t o) { out[i] = make_uint2(a, o); });
   for (uint32_t i = 1; i < m; ++i) {  // insertion sort by b
     uint2 v = out[i];
     uint32_t j = i;
@@ -1086,19 +1107,20 @@ __global__ void k_inc_build(Dev d) {
   }
 }
 
-// Jones-Plassmann colouring: in each round an uncoloured swept body whose
-// uncoloured neighbours all have lower priority takes the smallest colour
-// unused by its neighbours.  This equals the sequential greedy colouring
-// in priority order (larger hash first, then smaller scene-local index;
-// the key packs both, so a larger key is a higher priority; bodies of
-// different scenes are never neighbours, so keys of neighbours differ).
+// Jones-Plassmann colouring, frontier form.  A swept body is coloured in the
+// round after all its higher-priority neighbours are, with the smallest
+// colour none of them holds -- exactly the sequential greedy colouring in
+// priority order (larger hash first, then smaller scene-local index; the
+// key packs both, so a larger key is a higher priority; bodies of different
+// scenes are never neighbours, so keys of neighbours differ).  pend[h]
+// counts h's uncoloured higher-priority neighbours; a body whose count
+// reaches zero joins the next round's frontier.  Neighbours coloured in the
+// same round are never adjacent, so reads of neighbour colours see only
+// earlier rounds.
 __device__ __forceinline__ bool swept(const Dev& d, uint32_t h) {
   return d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f && !d.b.asleep[h];
 }
 
-// col_round[h] packs (round << 16) | colour in one word (written by a
-// single store), -1 while uncoloured; a neighbour coloured in the current
-// round still counts as uncoloured.
 // Also clears the per-step incidence and colouring counters.
 __global__ void k_colour_init(Dev d) {
   uint32_t h = gid();
@@ -1114,55 +1136,75 @@ __global__ void k_colour_init(Dev d) {
   d.sw[h] = swept(d, h) ? 1 : 0;
   d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
   d.b.colour[h] = -1;
-  d.b.col_round[h] = -1;
 }
 
-__device__ __forceinline__ int colour_seen(const Dev& d, uint32_t o, int round) {
-  int v = __atomic_load_n(&d.b.col_round[o], __ATOMIC_RELAXED);
-  if (v < 0 || (v >> 16) == round) return -1;
-  return v & 0xffff;
+// Appends v to list for every active lane whose pred holds (one atomic per
+// wavefront).
+__device__ __forceinline__ void wave_push(uint32_t* list, uint32_t* n, bool pred, uint32_t v) {
+  unsigned long long m = __ballot(pred);
+  if (!m) return;
+  const uint32_t lane = __lane_id();
+  const int leader = __ffsll(m) - 1;
+  uint32_t base = 0;
+  if (lane == (uint32_t)leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
 }
 
-__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, int round, uint32_t* remain, uint32_t* clear_next) {
+// Pending counts; bodies with no higher-priority neighbour form round 0.
+__global__ void __launch_bounds__(TPB) k_colour_count(Dev d, uint32_t* out_list, uint32_t* out_n) {
   uint32_t h = gid();
-  if (h == 0) *clear_next = 0u;
-  bool wait = false;
-  int newc = -1;
-  if (h < d.n && d.sw[h] && d.b.col_round[h] < 0) {
+  bool ready = false;
+  if (h < d.n && d.sw[h]) {
     const unsigned long long kh = d.ckey[h];
-    const uint32_t k0 = d.nbr_off[h], k1 = d.nbr_off[h + 1];
-    for (uint32_t k = k0; k < k1; ++k) {
-      uint32_t o = d.nbr[k];
-      if (colour_seen(d, o, round) < 0 && d.ckey[o] > kh) {
-        wait = true;
-        break;
-      }
-    }
-    for (int base = 0; !wait; base += 64) {
-      uint64_t used = 0;
-      for (uint32_t k = k0; k < k1; ++k) {
-        int co = colour_seen(d, d.nbr[k], round);
-        if (co >= base && co < base + 64) used |= 1ull << (co - base);
-      }
-      if (~used != 0ull) {
-        newc = base + __builtin_ctzll(~used);
-        __atomic_store_n(&d.b.col_round[h], (round << 16) | newc, __ATOMIC_RELAXED);
-        break;
-      }
-    }
+    uint32_t cnt = 0;
+    for (uint32_t k = d.nbr_off[h]; k < d.nbr_off[h + 1]; ++k) cnt += d.ckey[d.nbr[k]] > kh ? 1u : 0u;
+    d.pend[h] = cnt;
+    ready = cnt == 0u;
   }
-  wave_count(remain, wait);
-  (void)wave_append(d.colour_cnt, newc, d.cmax);
+  wave_push(out_list, out_n, ready, h);
+}
+
+// One round: colour the frontier, release lower-priority neighbours into
+// the next frontier.  *size_out receives the frontier size (0: done);
+// *clear is zeroed for the round after next.
+__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_list, const uint32_t* in_n,
+                                                      uint32_t* out_list, uint32_t* out_n, uint32_t* clear,
+                                                      uint32_t* size_out) {
+  const uint32_t nin = *in_n;
+  if (gid() == 0) {
+    *clear = 0u;
+    *size_out = nin;
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nin; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    int newc = -1;
+    if (i < nin) {
+      const uint32_t h = in_list[i];
+      const unsigned long long kh = d.ckey[h];
+      const uint32_t k0 = d.nbr_off[h], k1 = d.nbr_off[h + 1];
+      for (int cb = 0; newc < 0; cb += 64) {
+        uint64_t used = 0;
+        for (uint32_t k = k0; k < k1; ++k) {
+          int co = d.b.colour[d.nbr[k]];
+          if (co >= cb && co < cb + 64) used |= 1ull << (co - cb);
+        }
+        if (~used != 0ull) newc = cb + __builtin_ctzll(~used);
+      }
+      d.b.colour[h] = newc;
+      for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t o = d.nbr[k];
+        if (d.ckey[o] < kh && atomicSub(&d.pend[o], 1u) == 1u) out_list[atomicAdd(out_n, 1u)] = o;
+      }
+    }
+    (void)wave_append(d.colour_cnt, newc, d.cmax);
+  }
 }
 
 __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
   uint32_t h = gid();
-  int c = -1;
-  if (h < d.n) {
-    int v = d.b.col_round[h];
-    c = v < 0 ? -1 : (v & 0xffff);
-    d.b.colour[h] = c;
-  }
+  const int c = h < d.n ? d.b.colour[h] : -1;
   uint32_t slot = wave_append(d.colour_cur, c, nullptr);
   if (c >= 0) d.colour_list[d.colour_off[c] + slot] = h;
 }
@@ -1484,6 +1526,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   // element q of this lane: matrix entry (ia, ib) or rhs entry ia
   float acc[NE];
   int ia[NE], ib[NE];
+  int ibf[NE];  // second factor of a contact row: x_j, or f (slot 7) for rhs lanes
   bool rhs[NE], skip_couple[NE];
   uint32_t ebit[NE];
 #pragma unroll
@@ -1496,6 +1539,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     rhs[k] = e >= 21;
     ia[k] = e < 21 ? i : (e < 27 ? e - 21 : 0);
     ib[k] = e < 21 ? j : 0;
+    ibf[k] = e < 21 ? j : 7;
     ebit[k] = e < 27 ? 1u << e : 0u;
     skip_couple[k] = e < 21 ? j < 3 : e - 21 < 3;
     acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, i, j) : 0.0f;
@@ -1536,17 +1580,18 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
       ++w;
     }
     __syncthreads();
-    // contact rows: stamps (and angular couples when R == 6)
-#pragma unroll 2
+    // contact rows: stamps (and angular couples when R == 6).  A matrix
+    // lane adds |pen| * (x_i * x_j); an rhs lane adds 1 * (x_k * f), which
+    // is f * x_k exactly (products commute, 1 * y = y).
+#pragma unroll 4
     for (int t = 0; t < nprow; ++t) {
       const float* rw = rowf + 8 * t;
-      const float pen = rw[6], f = rw[7];
+      const float pen = rw[6];
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
-        const float a = rw[ia[q]], b = rw[ib[q]];
-        const float sc = rhs[q] ? f : fabsf(pen);
-        const float bb = rhs[q] ? 1.0f : b;
-        float v = sc * (a * bb);
+        const float a = rw[ia[q]], b = rw[ibf[q]];
+        const float sc = rhs[q] ? 1.0f : fabsf(pen);
+        float v = sc * (a * b);
         if constexpr (R >= 6) v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
         acc[q] += v;
       }

@@ -45,8 +45,10 @@
 #define CSTAT_HEAD 64    // words before the colour sizes in cstat: the round-count ring, the colour
                          // count and the sizes sit on separate 128-byte lines (their atomics do not contend)
 #define CSTAT_MAX 32     // word of the colour count
+#define CSTAT_FRONT 24   // words of the three frontier counters
 #define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
-static_assert(CSTAT_MAX >= 2 * COLOUR_BATCH && CSTAT_HEAD >= CSTAT_MAX + 32, "cstat layout");
+static_assert(CSTAT_FRONT >= 2 * COLOUR_BATCH && CSTAT_MAX >= CSTAT_FRONT + 3 && CSTAT_HEAD >= CSTAT_MAX + 32,
+              "cstat layout");
 #define TOI_LANES 16    // lanes cooperating on one TOI cast
 #define BP_CACHE 16     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
@@ -80,7 +82,6 @@ struct Bodies {
   uint8_t* asleep;
   float* timer;
   int* colour;
-  int* col_round;
   uint8_t* flag_a;   // scratch flags (wake / awake / ready / freeze)
   uint8_t* flag_b;
   uint8_t* supported;
@@ -196,7 +197,10 @@ struct Dev {
   uint32_t* nbr_off;   // n+1
   uint32_t* nbr_cur;
   uint32_t* nbr;
-  uint32_t* remain;    // cstat: bodies left uncoloured after each round (ring of two batches)
+  uint32_t* remain;    // cstat: frontier size of each round (ring of two batches)
+  uint32_t* fcnt;      // cstat + CSTAT_FRONT: frontier counters (ring of three)
+  uint32_t* front;     // 2 x n frontier lists (ring of two)
+  uint32_t* pend;      // per body: uncoloured higher-priority neighbours
   uint32_t* lflag;     // per large body, one flag per body of its scene (lflat in all)
   uint32_t* lrank;     // exclusive scan of lflag (lflat + 1)
   uint32_t* cstat;     // colouring status: remain ring 2 x COLOUR_BATCH, colour count, sizes

@@ -121,7 +121,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&b.layer, n); dalloc(&b.mask, n); dalloc(&b.scene, n); dalloc(&b.prio, n); dalloc(&b.local, n);
   dalloc(&b.trigger, n); dalloc(&b.pos0, n); dalloc(&b.rot0, n); dalloc(&b.inertial, n);
   dalloc(&b.inertial_rot, n); dalloc(&b.drot, n); dalloc(&b.prev_pos, n); dalloc(&b.prev_rot, n);
-  dalloc(&b.asleep, n); dalloc(&b.timer, n); dalloc(&b.colour, n); dalloc(&b.col_round, n);
+  dalloc(&b.asleep, n); dalloc(&b.timer, n); dalloc(&b.colour, n);
   dalloc(&b.flag_a, n); dalloc(&b.flag_b, n); dalloc(&b.supported, n); dalloc(&b.comp, n);
   dalloc(&b.comp_any, n); dalloc(&b.jp_off, n + 1); dalloc(&b.ji_off, n + 1);
   b.jp = nullptr;
@@ -135,7 +135,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.nbr_cur, n + 1);
   d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
-  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
+  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); dalloc(&d.pend, n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
@@ -239,7 +239,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(b.mat); dfree(b.kind); dfree(b.type); dfree(b.layer); dfree(b.mask); dfree(b.scene); dfree(b.prio);
   dfree(b.local); dfree(b.trigger); dfree(b.pos0); dfree(b.rot0); dfree(b.inertial); dfree(b.inertial_rot);
   dfree(b.drot); dfree(b.prev_pos); dfree(b.prev_rot); dfree(b.asleep); dfree(b.timer); dfree(b.colour);
-  dfree(b.col_round); dfree(b.flag_a); dfree(b.flag_b); dfree(b.supported); dfree(b.comp); dfree(b.comp_any);
+  dfree(b.flag_a); dfree(b.flag_b); dfree(b.supported); dfree(b.comp); dfree(b.comp_any);
   dfree(b.jp_off); dfree(b.jp); dfree(b.ji_off); dfree(b.ji);
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
@@ -250,7 +250,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
-  dfree(d.cstat); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
+  dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->scan_scratch); dfree(c->ls.state); dfree(c->ls.ticket);
 }
@@ -443,25 +443,29 @@ static void incidence_and_colour(oavbd_ctx* c) {
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
   // Jones-Plassmann colouring, COLOUR_BATCH rounds per host check (rounds
-  // after the last coloured body find nothing to do).  Each check copies the
-  // round counts, the colour count and the first colour sizes in one go.
-  // The round counts use two halves of a ring: batch b counts into half
-  // b % 2 and clears half (b + 1) % 2 for the next batch.
+  // after the last frontier find nothing to do).  Each check copies the
+  // frontier sizes, the colour count and the first colour sizes in one go.
+  // Frontier lists alternate between two buffers, their counters rotate over
+  // three (round r reads counter r % 3, fills (r + 1) % 3, clears (r + 2) % 3).
+  k_colour_count<<<blocks(n), TPB, 0, c->st>>>(d, d.front, d.fcnt);
   uint32_t head[CSTAT_HEAD + CSTAT_EAGER];
   int rounds = 0;
-  for (int base = 0, b = 0; rounds == 0; base += COLOUR_BATCH, ++b) {
-    if (base > 30000) throw HipError{"colouring did not converge"};
-    uint32_t* cur = d.remain + (b & 1) * COLOUR_BATCH;
-    uint32_t* nxt = d.remain + ((b + 1) & 1) * COLOUR_BATCH;
-    for (int r = 0; r < COLOUR_BATCH; ++r)
-      k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, base + r, cur + r, nxt + r);
+  for (int b = 0;; ++b) {
+    if (b * COLOUR_BATCH > 30000) throw HipError{"colouring did not converge"};
+    for (int r = 0; r < COLOUR_BATCH; ++r) {
+      const int round = b * COLOUR_BATCH + r;
+      k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(round & 1) * n, d.fcnt + round % 3,
+                                                   d.front + (size_t)((round + 1) & 1) * n, d.fcnt + (round + 1) % 3,
+                                                   d.fcnt + (round + 2) % 3, d.remain + (b & 1) * COLOUR_BATCH + r);
+    }
     HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
     HIPC(hipStreamSynchronize(c->st));
-    for (int r = 0; r < COLOUR_BATCH; ++r)
-      if (head[(b & 1) * COLOUR_BATCH + r] == 0) {
-        rounds = base + r + 1;
-        break;
-      }
+    int r = 0;
+    while (r < COLOUR_BATCH && head[(b & 1) * COLOUR_BATCH + r] != 0) ++r;
+    if (r < COLOUR_BATCH) {
+      rounds = b * COLOUR_BATCH + r;
+      break;
+    }
   }
   c->stats.colour_rounds = (uint32_t)rounds;
   uint32_t ncol = head[CSTAT_MAX];
