@@ -1020,28 +1020,47 @@ __global__ void k_pair_prep(Dev d) {
 // Incidence is built for swept bodies only (the primal solves nothing
 // else); the colouring graph links swept bodies sharing a pair with
 // points or a joint (the graph of colour_bodies).
+// Adds the block's sum of v to *ctr with one atomic (every thread of the
+// TPB-thread block must call it).
+__device__ __forceinline__ void block_add(uint32_t* ctr, uint32_t v) {
+  __shared__ uint32_t part[TPB / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < TPB / 64; ++w) t += part[w];
+    if (t) atomicAdd(ctr, t);
+  }
+}
+
+// Per pair: the b side's counts (scattered bodies, few atomics each); the
+// a side of a whole a-run is summed by the run's first pair.
 __global__ void __launch_bounds__(TPB) k_inc_count(Dev d) {
   uint32_t p = gid();
-  uint32_t side_pts = 0, side_pairs = 0;
+  uint32_t side_pairs = 0;
   if (p < d.npairs) {
     uint2 ab = d.P.ab[p];
     uint32_t c = d.P.pts[p].y;
     bool sa = d.sw[ab.x], sb = d.sw[ab.y];
-    if (sa) atomicAdd(&d.ptcnt[ab.x], c);
     if (sb) {
       atomicAdd(&d.ptcnt[ab.y], c);
       atomicAdd(&d.bcnt[ab.y], 1u);
+      if (c && sa) atomicAdd(&d.nbr_cnt[ab.y], 1u);
     }
-    if (c && sa && sb) {
-      atomicAdd(&d.nbr_cnt[ab.x], 1u);
-      atomicAdd(&d.nbr_cnt[ab.y], 1u);
+    if (sa && p == d.poff[ab.x]) {
+      uint32_t pts = 0, nb = 0;
+      for (uint32_t q = p; q < d.poff[ab.x + 1]; ++q) {
+        uint32_t cq = d.P.pts[q].y;
+        pts += cq;
+        nb += (cq && d.sw[d.P.ab[q].y]) ? 1u : 0u;
+      }
+      atomicAdd(&d.ptcnt[ab.x], pts);
+      if (nb) atomicAdd(&d.nbr_cnt[ab.x], nb);
     }
-    uint32_t ns = (uint32_t)sa + (uint32_t)sb;
-    side_pts = c * ns;
-    side_pairs = c ? ns : 0u;
+    side_pairs = c ? (uint32_t)sa + (uint32_t)sb : 0u;
   }
-  wave_add(&d.counters[CTR_PTSIDES], side_pts);
-  wave_add(&d.counters[CTR_PAIRSIDES], side_pairs);
+  block_add(&d.counters[CTR_PAIRSIDES], side_pairs);
 }
 __global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
   uint32_t j = gid();
@@ -1055,7 +1074,7 @@ __global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
     }
     sides = (uint32_t)sa + (uint32_t)sb;
   }
-  wave_add(&d.counters[CTR_JSIDES], sides);
+  block_add(&d.counters[CTR_JSIDES], sides);
 }
 __global__ void k_inc_fill(Dev d) {
   uint32_t p = gid();
@@ -1154,6 +1173,7 @@ __device__ __forceinline__ void wave_push(uint32_t* list, uint32_t* n, bool pred
 // Pending counts; bodies with no higher-priority neighbour form round 0.
 __global__ void __launch_bounds__(TPB) k_colour_count(Dev d, uint32_t* out_list, uint32_t* out_n) {
   uint32_t h = gid();
+  if (h == 0) d.counters[CTR_PTSIDES] = d.inc_off[d.n];  // point-sides of swept bodies
   bool ready = false;
   if (h < d.n && d.sw[h]) {
     const unsigned long long kh = d.ckey[h];
@@ -1166,8 +1186,10 @@ __global__ void __launch_bounds__(TPB) k_colour_count(Dev d, uint32_t* out_list,
 }
 
 // One round: colour the frontier, release lower-priority neighbours into
-// the next frontier.  *size_out receives the frontier size (0: done);
-// *clear is zeroed for the round after next.
+// the next frontier.  CL lanes share one frontier body, each taking every
+// CL-th neighbour.  *size_out receives the frontier size (0: done); *clear
+// is zeroed for the round after next.
+#define CL 8
 __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_list, const uint32_t* in_n,
                                                       uint32_t* out_list, uint32_t* out_n, uint32_t* clear,
                                                       uint32_t* size_out) {
@@ -1176,37 +1198,62 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_
     *clear = 0u;
     *size_out = nin;
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < nin; base += stride) {
-    const uint32_t i = base + threadIdx.x;
+  constexpr uint32_t PER_BLOCK = TPB / CL;
+  const uint32_t s = threadIdx.x % CL;
+  for (uint32_t base = blockIdx.x * PER_BLOCK; base < nin; base += gridDim.x * PER_BLOCK) {
+    const uint32_t i = base + threadIdx.x / CL;
+    const bool act = i < nin;
+    const uint32_t h = act ? in_list[i] : 0u;
+    const unsigned long long kh = act ? d.ckey[h] : 0ull;
+    const uint32_t k0 = act ? d.nbr_off[h] : 0u, k1 = act ? d.nbr_off[h + 1] : 0u;
     int newc = -1;
-    if (i < nin) {
-      const uint32_t h = in_list[i];
-      const unsigned long long kh = d.ckey[h];
-      const uint32_t k0 = d.nbr_off[h], k1 = d.nbr_off[h + 1];
-      for (int cb = 0; newc < 0; cb += 64) {
-        uint64_t used = 0;
-        for (uint32_t k = k0; k < k1; ++k) {
-          int co = d.b.colour[d.nbr[k]];
-          if (co >= cb && co < cb + 64) used |= 1ull << (co - cb);
-        }
-        if (~used != 0ull) newc = cb + __builtin_ctzll(~used);
+    for (int cb = 0; act && newc < 0; cb += 64) {
+      unsigned long long used = 0;
+      for (uint32_t k = k0 + s; k < k1; k += CL) {
+        int co = d.b.colour[d.nbr[k]];
+        if (co >= cb && co < cb + 64) used |= 1ull << (co - cb);
       }
-      d.b.colour[h] = newc;
-      for (uint32_t k = k0; k < k1; ++k) {
-        const uint32_t o = d.nbr[k];
-        if (d.ckey[o] < kh && atomicSub(&d.pend[o], 1u) == 1u) out_list[atomicAdd(out_n, 1u)] = o;
-      }
+      for (int m = CL / 2; m > 0; m >>= 1) used |= __shfl_xor(used, m, CL);
+      if (~used != 0ull) newc = cb + __builtin_ctzll(~used);
     }
-    (void)wave_append(d.colour_cnt, newc, d.cmax);
+    if (act && s == 0) d.b.colour[h] = newc;
+    // release: wavefront-uniform loop over the largest neighbour count
+    uint32_t deg = k1 - k0, maxdeg = deg;
+    for (int m = 32; m > 0; m >>= 1) {
+      uint32_t o2 = __shfl_xor(maxdeg, m);
+      maxdeg = o2 > maxdeg ? o2 : maxdeg;
+    }
+    for (uint32_t kb = 0; kb < maxdeg; kb += CL) {
+      const uint32_t kk = kb + s;
+      bool rel = false;
+      uint32_t o = 0;
+      if (kk < deg) {
+        o = d.nbr[k0 + kk];
+        rel = d.ckey[o] < kh && atomicSub(&d.pend[o], 1u) == 1u;
+      }
+      wave_push(out_list, out_n, rel, o);
+    }
+    (void)wave_append(d.colour_cnt, s == 0 ? newc : -1, d.cmax);
   }
 }
 
+// Colour lists: slots within a block from LDS counters, one global atomic
+// per colour per block (colours >= 64, rare, take one atomic per body).
 __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
+  __shared__ uint32_t cnt[64], base[64];
   uint32_t h = gid();
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
+  __syncthreads();
   const int c = h < d.n ? d.b.colour[h] : -1;
-  uint32_t slot = wave_append(d.colour_cur, c, nullptr);
-  if (c >= 0) d.colour_list[d.colour_off[c] + slot] = h;
+  uint32_t local = 0;
+  if (c >= 0 && c < 64) local = atomicAdd(&cnt[c], 1u);
+  __syncthreads();
+  if (threadIdx.x < 64 && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&d.colour_cur[threadIdx.x], cnt[threadIdx.x]);
+  __syncthreads();
+  if (c >= 0 && c < 64)
+    d.colour_list[d.colour_off[c] + base[c] + local] = h;
+  else if (c >= 64)
+    d.colour_list[d.colour_off[c] + atomicAdd(&d.colour_cur[c], 1u)] = h;
 }
 
 // Solver records for the iteration loop (pos, rot, pos0, drot per body);

@@ -50,7 +50,7 @@
 static_assert(CSTAT_FRONT >= 2 * COLOUR_BATCH && CSTAT_MAX >= CSTAT_FRONT + 3 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
 #define TOI_LANES 16    // lanes cooperating on one TOI cast
-#define BP_CACHE 16     // broadphase candidates kept per body between count and fill
+#define BP_CACHE 32     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
 
 enum { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
@@ -232,6 +232,5 @@ enum {
   CTR_PTSIDES = 7,
   CTR_PAIRSIDES = 8,
   CTR_JSIDES = 9,
-  CTR_SCANERR = 10,
   CTR_COUNT = 12
 };

@@ -73,8 +73,7 @@ struct oavbd_ctx {
   size_t cap_cand = 0, cap_pairs = 0, cap_pts = 0, cap_pairs_old = 0, cap_pts_old = 0, cap_blist = 0, cap_inc = 0, cap_toi = 0, cap_nbr = 0;
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
-  uint32_t* scan_scratch = nullptr;
-  LookbackScan ls;
+  TileScan ts;
   uint32_t* counters_h = nullptr;  // pinned
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
@@ -155,13 +154,9 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   d.g.lflat = 0;
   d.lflag = nullptr;
   d.lrank = nullptr;
-  dalloc(&c->scan_scratch, ScanScratch::need((size_t)1 << 28));
-  c->ls.max_tiles = (uint32_t)((((size_t)1 << 28) + SCAN_TILE - 1) / SCAN_TILE) + 1;
-  dalloc(&c->ls.state, c->ls.max_tiles);
-  dalloc(&c->ls.ticket, 1);
-  HIPC(hipMemset(c->ls.state, 0, (size_t)c->ls.max_tiles * sizeof(unsigned long long)));
-  HIPC(hipMemset(c->ls.ticket, 0, sizeof(uint32_t)));
-  c->ls.epoch = 0;
+  dalloc(&c->ts.tile, SCAN_MAX_TILES + 1);
+  dalloc(&c->ts.done, 1);
+  HIPC(hipMemset(c->ts.done, 0, sizeof(uint32_t)));
   d.blist = nullptr;
   d.inc = nullptr;
   d.acc_off = nullptr;
@@ -195,7 +190,7 @@ static void ensure_cand(oavbd_ctx* c, size_t need) {
   dfree(c->d.acc_off); dfree(c->d.pt_off);
   dalloc(&C.ab, cap); dalloc(&C.acc, cap); dalloc(&C.npts, cap); dalloc(&C.disc0, cap); dalloc(&C.disc1, cap);
   dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
-  if (cap + 1 > ((size_t)1 << 28)) throw HipError{"candidate count exceeds the scan capacity"};
+  if (cap + 1 > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"candidate count exceeds the scan capacity"};
   c->cap_cand = cap;
 }
 
@@ -252,7 +247,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
-  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->scan_scratch); dfree(c->ls.state); dfree(c->ls.ticket);
+  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,9 +266,10 @@ static void zero_u32(oavbd_ctx* c, uint32_t* p, uint32_t n) {
   if (n) k_zero_u32<<<blocks(n), TPB, 0, c->st>>>(p, n);
 }
 
-// Exclusive scan on the context stream (single launch, see avbd_scan.hpp).
+// Exclusive scan on the context stream (two launches, see avbd_scan.hpp).
 static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
-  scan_lookback(c->ls, in, out, n, c->st, c->d.counters + CTR_SCANERR);
+  if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
+  scan_tiles(c->ts, in, out, n, c->st);
 }
 
 static void read_counters(oavbd_ctx* c) {
@@ -607,7 +603,6 @@ static void step_inner(oavbd_ctx* c) {
   k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 4);
   read_counters(c);
-  if (c->counters_h[CTR_SCANERR]) throw HipError{"prefix scan look-back did not complete"};
   c->sleeping = c->counters_h[CTR_SLEEPING];
   c->stats.sleeping = c->sleeping;
   c->stats.breakdowns = c->counters_h[CTR_BREAKDOWN];
