@@ -1781,7 +1781,6 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
     float fs = sqrtf(r0 * r0 + r1 * r1);
     bool stuck = fs <= bnd && sqrtf(ct1 * ct1 + ct2 * ct2) < K_STICK;
     rb4.w = stuck ? 1.0f : 0.0f;
-    d.Q.rb(q) = rb4;
   }
   float4 nl = lam, npn = pen;
   if (touching && fabsf(cn) >= K_C_EPS) {
@@ -1798,6 +1797,10 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
       if (fabsf(ct2) >= K_C_EPS) npn.z = fmin_(pen.z + K_BETA * fabsf(ct2), K_PEN_MAX);
     }
   }
+  // the whole 64-byte record is rewritten (ra unchanged), so no partial
+  // line is left for a read-modify-write
+  d.Q.ra(q) = ra4;
+  d.Q.rb(q) = rb4;
   d.Q.lam(q) = nl;
   d.Q.pen(q) = npn;
   if (touching && (mu.z > 0.0f || mu.w > 0.0f)) {

@@ -49,7 +49,7 @@
 #define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
 static_assert(CSTAT_FRONT >= 2 * COLOUR_BATCH && CSTAT_MAX >= CSTAT_FRONT + 3 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
-#define TOI_LANES 16    // lanes cooperating on one TOI cast
+#define TOI_LANES 32    // lanes cooperating on one TOI cast
 #define BP_CACHE 32     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
 
