@@ -136,6 +136,8 @@ def main():
     ap.add_argument("--waves", type=int, default=1, help="primal wavefronts per workgroup")
     ap.add_argument("--sweep-shapes", type=int, default=0,
                     help="after warmup, time this many steps per primal launch shape and exit")
+    ap.add_argument("--timing-every", type=int, default=10,
+                    help="record per-launch kernel events on one timed step in this many")
     ap.add_argument("--batched-steps", type=int, default=300,
                     help="timed steps of the 4,096-scene batched workload (0: skip it)")
     ap.add_argument("--loop-scenes", type=int, default=32)
@@ -184,13 +186,17 @@ def main():
                               "GBps": kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9}), flush=True)
         return
     eng.kernel_times(reset=True)
+    # Kernel timing: HIP events bracket every primal and dual launch of one
+    # step in every `timing_every` (the events themselves add launch gaps,
+    # so they are kept off the other steps).
+    every = max(args.timing_every, 1)
+    eng.set_timing(False)
     t0 = time.perf_counter()
-    chunk = 10
     done = 0
     while done < args.steps:
-        k = min(chunk, args.steps - done)
-        eng.step_n(k)
-        done += k
+        eng.set_timing(done % every == 0)
+        eng.step_n(1)
+        done += 1
         if args.progress:
             print(f"step {args.warmup + done} {eng.stats()} {eng.timing()}", file=sys.stderr, flush=True)
     eng.sync()
