@@ -194,9 +194,15 @@ def main():
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
-        eng.set_timing(done % every == 0)
-        eng.step_n(1)
-        done += 1
+        if done % every == 0:
+            eng.set_timing(True)
+            eng.step_n(1)
+            eng.set_timing(False)
+            done += 1
+        else:
+            k = min(every - done % every, args.steps - done)
+            eng.step_n(k)
+            done += k
         if args.progress:
             print(f"step {args.warmup + done} {eng.stats()} {eng.timing()}", file=sys.stderr, flush=True)
     eng.sync()

@@ -41,13 +41,13 @@
 #define K_CCD_GATE 0.5f
 #define K_JOINT_REST 0.005f
 #define K_MAX_POINTS 8
-#define COLOUR_BATCH 8  // colouring rounds launched per host check
-#define CSTAT_HEAD 64    // words before the colour sizes in cstat: the round-count ring, the colour
-                         // count and the sizes sit on separate 128-byte lines (their atomics do not contend)
-#define CSTAT_MAX 32     // word of the colour count
-#define CSTAT_FRONT 24   // words of the three frontier counters
+#define COLOUR_BATCH 8  // colouring rounds launched per host check after the first
+#define COLOUR_RING 64   // frontier sizes kept per round number modulo this (a batch is shorter)
+#define CSTAT_FRONT 96   // words of the three frontier counters (own cache line)
+#define CSTAT_MAX 128    // word of the colour count (own cache line)
+#define CSTAT_HEAD 160   // words before the colour sizes in cstat
 #define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
-static_assert(CSTAT_FRONT >= 2 * COLOUR_BATCH && CSTAT_MAX >= CSTAT_FRONT + 3 && CSTAT_HEAD >= CSTAT_MAX + 32,
+static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
 #define TOI_LANES 32    // lanes cooperating on one TOI cast
 #define BP_CACHE 32     // broadphase candidates kept per body between count and fill
@@ -197,13 +197,13 @@ struct Dev {
   uint32_t* nbr_off;   // n+1
   uint32_t* nbr_cur;
   uint32_t* nbr;
-  uint32_t* remain;    // cstat: frontier size of each round (ring of two batches)
+  uint32_t* remain;    // cstat: frontier size of round r at r % COLOUR_RING
   uint32_t* fcnt;      // cstat + CSTAT_FRONT: frontier counters (ring of three)
   uint32_t* front;     // 2 x n frontier lists (ring of two)
   uint32_t* pend;      // per body: uncoloured higher-priority neighbours
   uint32_t* lflag;     // per large body, one flag per body of its scene (lflat in all)
   uint32_t* lrank;     // exclusive scan of lflag (lflat + 1)
-  uint32_t* cstat;     // colouring status: remain ring 2 x COLOUR_BATCH, colour count, sizes
+  uint32_t* cstat;     // colouring status: frontier sizes by round, frontier counters, colour count, sizes
   uint32_t* cmax;      // cstat + CSTAT_MAX: number of colours
   uint32_t* colour_cnt;  // cstat + CSTAT_HEAD: bodies per colour
   uint32_t* colour_off;

@@ -80,7 +80,9 @@ struct oavbd_ctx {
   std::vector<int32_t> kind_h, type_h;
   float time_debt = 0.0f;
   uint32_t sleeping = 0;
+  bool may_sleep = false;  // bodies may be asleep (wake passes needed)
   uint32_t solved = 0;  // bodies coloured (swept) this step
+  int last_rounds = 0;  // colouring rounds of the previous step
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   int lanes = 32;   // primal launch shape: lanes per body
@@ -445,24 +447,27 @@ static void incidence_and_colour(oavbd_ctx* c) {
   // three (round r reads counter r % 3, fills (r + 1) % 3, clears (r + 2) % 3).
   k_colour_count<<<blocks(n), TPB, 0, c->st>>>(d, d.front, d.fcnt);
   uint32_t head[CSTAT_HEAD + CSTAT_EAGER];
-  int rounds = 0;
-  for (int b = 0;; ++b) {
-    if (b * COLOUR_BATCH > 30000) throw HipError{"colouring did not converge"};
-    for (int r = 0; r < COLOUR_BATCH; ++r) {
-      const int round = b * COLOUR_BATCH + r;
+  int rounds = -1;
+  // the first batch covers the previous step's round count (it changes
+  // slowly), so one check usually suffices
+  int batch = std::min(std::max(COLOUR_BATCH, c->last_rounds + 2), COLOUR_RING - 16);
+  for (int round0 = 0; rounds < 0; round0 += batch, batch = COLOUR_BATCH) {
+    if (round0 > 30000) throw HipError{"colouring did not converge"};
+    for (int r = 0; r < batch; ++r) {
+      const int round = round0 + r;
       k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(round & 1) * n, d.fcnt + round % 3,
                                                    d.front + (size_t)((round + 1) & 1) * n, d.fcnt + (round + 1) % 3,
-                                                   d.fcnt + (round + 2) % 3, d.remain + (b & 1) * COLOUR_BATCH + r);
+                                                   d.fcnt + (round + 2) % 3, d.remain + round % COLOUR_RING);
     }
     HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
     HIPC(hipStreamSynchronize(c->st));
-    int r = 0;
-    while (r < COLOUR_BATCH && head[(b & 1) * COLOUR_BATCH + r] != 0) ++r;
-    if (r < COLOUR_BATCH) {
-      rounds = b * COLOUR_BATCH + r;
-      break;
-    }
+    for (int r = 0; r < batch; ++r)
+      if (head[(round0 + r) % COLOUR_RING] == 0) {
+        rounds = round0 + r;
+        break;
+      }
   }
+  c->last_rounds = rounds;
   c->stats.colour_rounds = (uint32_t)rounds;
   uint32_t ncol = head[CSTAT_MAX];
   c->colour_cnt_h.assign(ncol, 0);
@@ -520,13 +525,15 @@ static void record(oavbd_ctx* c, int i) {
   if (c->timing) HIPC(hipEventRecord(c->ev[i], c->st));
 }
 
-static void step_inner(oavbd_ctx* c) {
+// One fixed substep.  `finish` reads the step's counters back (stats,
+// sleep count); other steps leave the stream running into the next one.
+static void step_inner(oavbd_ctx* c, bool finish) {
   Dev& d = c->d;
   uint32_t n = d.n;
   if (c->joints_dirty) upload_joints(c);
   record(c, 0);
   zero_u32(c, d.counters, CTR_COUNT);
-  if (c->sleeping) wake_joint_motion(c);
+  if (c->may_sleep) wake_joint_motion(c);
   k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
@@ -551,7 +558,7 @@ static void step_inner(oavbd_ctx* c) {
   }
   broadphase(c);
   narrow_and_merge(c);
-  if (c->sleeping) {
+  if (c->may_sleep) {
     wake_joint_motion(c);
     k_external_wake<<<blocks(n), TPB, 0, c->st>>>(d);
   }
@@ -602,15 +609,20 @@ static void step_inner(oavbd_ctx* c) {
   if (d.njoints) propagate(c, d.b.flag_a, 0);
   k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 4);
-  read_counters(c);
-  c->sleeping = c->counters_h[CTR_SLEEPING];
-  c->stats.sleeping = c->sleeping;
-  c->stats.breakdowns = c->counters_h[CTR_BREAKDOWN];
-  c->stats.toi_hits = c->counters_h[CTR_TOIHITS];
   c->stats.pairs = d.npairs;
   c->stats.points = d.npts;
   c->stats.candidates = c->ncand;
   c->stats.solved = c->solved;
+  if (!finish && !c->timing) {
+    c->may_sleep = true;  // the sleep count is not read back: wake passes run next step
+    return;
+  }
+  read_counters(c);
+  c->sleeping = c->counters_h[CTR_SLEEPING];
+  c->may_sleep = c->sleeping != 0;
+  c->stats.sleeping = c->sleeping;
+  c->stats.breakdowns = c->counters_h[CTR_BREAKDOWN];
+  c->stats.toi_hits = c->counters_h[CTR_TOIHITS];
   c->stats.point_sides = c->counters_h[CTR_PTSIDES];
   c->stats.pair_sides = c->counters_h[CTR_PAIRSIDES];
   c->stats.joint_sides = c->counters_h[CTR_JSIDES];
@@ -874,6 +886,7 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
     c->joints.clear();
     upload_joints(c);
     c->sleeping = 0;
+    c->may_sleep = false;
     c->time_debt = 0.0f;
     c->stats = oavbd_stats{};
     ensure_pairs(c, 1024);
@@ -908,7 +921,7 @@ int oavbd_step_n(oavbd_ctx* c, uint32_t nsteps) {
   if (!c) return OAVBD_EINVAL;
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
-    for (uint32_t s = 0; s < nsteps; ++s) step_inner(c);
+    for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps);
   });
   return OAVBD_OK;
 }
