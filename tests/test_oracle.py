@@ -262,3 +262,14 @@ def test_portable_sincos_asin_accuracy():
         a = L.orc_fmath_asin(F(x))
         ea.append(_ulp_err(np.array([a], F), np.arcsin(np.float64(x)))[0])
     assert max(ea) <= 2.0
+
+
+def test_fp32_tracks_fp64_on_ordered_scenes():
+    """FP32-vs-FP64 drift of the step (the GPU step equals the FP32 oracle
+    bit for bit, so this is the GPU path's drift too): within 1e-3 m on the
+    reference snapshot scene after 120 steps and 1e-4 m on a resting stack
+    after 300.  Chaotic piles diverge by design; tests/fp64_drift.py reports
+    them."""
+    from fp64_drift import drift
+    assert drift(snapshot_scene(), 120)["pos_max"] < 1e-3
+    assert drift(stack_scene(4), 300)["pos_max"] < 1e-4
