@@ -176,6 +176,29 @@ int oavbd_get_timing(oavbd_ctx* ctx, float* ms5);
  * Default 32 lanes, 1 wavefront. */
 int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
 
+/* Rung-1 staged bodies: n GpuAvbdMass records (16 bytes: inertia[3],
+ * inv_mass) and n GpuBodyState residuals (32 bytes: lin[3], pad, ang[3],
+ * pad) are solved on the device -- lumped inertial Hessian diagonal and its
+ * diagonal solve, 0 on axes at or below the degenerate threshold -- into n
+ * GpuBodyState deltas.  Replaces: the rung-1 kernel cs_main
+ * (crates/gpu-avbd/src/lib.rs:113-148) with the semantics of
+ * GpuAvbdStub::stage_diag_solve (crates/physics/src/gpu/avbd.rs:207-226),
+ * avbd_inertial_hessian_diag (:88-107) and avbd_diag_solve_cpu (:117-128).
+ * Independent of the context's bodies. */
+int oavbd_solve_diag(oavbd_ctx* ctx, const void* masses16, const void* rhs32, uint32_t n, float dt, void* delta32);
+
+/* Rung-2 staged systems: n packed GpuAvbdSystem records (96 bytes each:
+ * inertia[3], inv_mass | residual lin[3], pad, ang[3], pad | row axis[3],
+ * pen, lever[3], force, sign, pad[3]) are solved on the device -- lumped
+ * inertial Hessian diagonal, one stamped row, dense 6x6 LDL -- into n
+ * GpuBodyState deltas (32 bytes: lin[3], 0, ang[3], 0) and ok flags (1.0
+ * solved, 0.0 breakdown with a zero delta).  Replaces: WgpuAvbdSolver::upload
+ * / solve / try_download (crates/physics/src/gpu/avbd.rs:645-760) with the
+ * semantics of avbd_stage_contact_solve (:415-449); the record layouts are
+ * the reference's (:55-60, :264-281, :320-327; crates/physics/src/gpu/
+ * mod.rs:200-207).  Independent of the context's bodies. */
+int oavbd_solve_systems(oavbd_ctx* ctx, const void* systems96, uint32_t n, float dt, void* delta32, float* ok);
+
 /* Block until all work of the context has finished. */
 int oavbd_sync(oavbd_ctx* ctx);
 

@@ -19,6 +19,7 @@
 //   sleep / joint wake      crates/physics/src/avbd/sleep.rs:1-175
 //   shape_distance et al.   crates/physics/src/collision/distance.rs:63-891
 //   box SAT / features      crates/physics/src/collision/distance_box.rs:21-147
+//   staged GPU rungs 1, 2   crates/physics/src/gpu/avbd.rs:88-128, :207-226, :415-449
 //
 // Two orders:
 //   ORDER_REFERENCE  the reference exactly: reverse-handle Gauss-Seidel sweep,
@@ -45,7 +46,11 @@
 //     stamp_row vs the analytic J J^T, contact_force audit answers (A3, A6,
 //     push-only), tangent basis, quat_integrate / quat_diff_vec inverse;
 //   - invariants: stack stands, damper momentum, determinism, grid = brute
-//     discovery, proper greedy colouring, FP64 tracks FP32.
+//     discovery, proper greedy colouring, FP64 tracks FP32;
+//   - staged rungs (tests/test_stage_solve.py): the reference's known
+//     answers for the lumped Hessian diagonal and the staged contact cases
+//     (crates/physics/tests/gpu_batches.rs:537-572, :904-926, :968-1010),
+//     and a float64 dense solve of the stamped system on random systems.
 // Parity unpinned: glam Quat op order under non-identity rotations (the
 // golden file only holds identity rotations), the reference's libm
 // sin_cos / asin bits (MATH_PORTABLE is pinned to libm within 2 ulp, not to
@@ -2018,6 +2023,80 @@ void orc_get_toi(void* h, float* t, int32_t* target) {
 }
 
 // Unit hooks for analytic pins.
+// avbd_inertial_hessian_diag (crates/physics/src/gpu/avbd.rs:88-107): the
+// lumped diagonal [m/dt^2 x 3, I/dt^2 x 3] of a GpuAvbdMass {inertia[3],
+// inv_mass} (:55-60); non-positive or non-finite inv_mass or dt, and a
+// degenerate dt^2, give zeros.
+static void stage_hessian_diag(const float* mass4, float dt, float diag[6]) {
+  for (int k = 0; k < 6; ++k) diag[k] = 0.0f;
+  const float inv_mass = mass4[3];
+  if (!std::isfinite(inv_mass) || inv_mass <= 0.0f || !std::isfinite(dt) || dt <= 0.0f) return;
+  const float dt2 = dt * dt;
+  if (!std::isfinite(dt2) || dt2 <= K<float>::DEGENERATE()) return;
+  const float lin = 1.0f / (inv_mass * dt2);
+  diag[0] = lin;
+  diag[1] = lin;
+  diag[2] = lin;
+  diag[3] = mass4[0] / dt2;
+  diag[4] = mass4[1] / dt2;
+  diag[5] = mass4[2] / dt2;
+}
+
+// Rung-1 staged solve, GpuAvbdStub::stage_diag_solve (crates/physics/src/
+// gpu/avbd.rs:207-226): per body the lumped diagonal and avbd_diag_solve_cpu
+// (:117-128), rhs / diag on axes above DEGENERATE_EPS and 0 elsewhere.  mass4
+// holds GpuAvbdMass records, rhs8 and delta8 GpuBodyState records {lin[3],
+// pad, ang[3], pad} (gpu/mod.rs:200-207).
+void orc_stage_diag_solve_f32(const float* mass4, const float* rhs8, uint32_t n, float dt, float* delta8) {
+  for (uint32_t i = 0; i < n; ++i) {
+    float diag[6];
+    stage_hessian_diag(mass4 + 4 * (size_t)i, dt, diag);
+    const float* r = rhs8 + 8 * (size_t)i;
+    const float rhs[6] = {r[0], r[1], r[2], r[4], r[5], r[6]};
+    float x[6];
+    for (int k = 0; k < 6; ++k) x[k] = diag[k] > K<float>::DEGENERATE() ? rhs[k] / diag[k] : 0.0f;
+    float* d = delta8 + 8 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      d[k] = x[k];
+      d[4 + k] = x[3 + k];
+    }
+    d[3] = 0.0f;
+    d[7] = 0.0f;
+  }
+}
+
+// Rung-2 staged solve, avbd_stage_contact_solve (crates/physics/src/gpu/
+// avbd.rs:415-449) over packed GpuAvbdSystem records (:320-327; 24 floats:
+// GpuAvbdMass inertia[3], inv_mass (:55-60); GpuBodyState residual
+// {lin[3], pad, ang[3], pad} (gpu/mod.rs:200-207); GpuAvbdRow axis[3], pen,
+// lever[3], force, sign, pad[3] (:264-281)).  Per system: the lumped
+// inertial Hessian diagonal, one stamped row (avbd_stamp_row_cpu, :358-390,
+// the engine's stamp_row) and solve_6x6 (avbd_ldl_6x6_cpu, :398-403: a
+// breakdown gives a zero delta).  delta8 receives GpuBodyState records, ok
+// 1.0 / 0.0.
+void orc_stage_contact_solve_f32(const float* sys24, uint32_t n, float dt, float* delta8, float* ok) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* s = sys24 + 24 * (size_t)i;
+    float diag[6];
+    stage_hessian_diag(s, dt, diag);
+    float lhs[6][6], rhs[6] = {s[4], s[5], s[6], s[8], s[9], s[10]};
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) lhs[r][c] = r == c ? diag[r] : 0.0f;
+    Engine<float>::stamp_row(lhs, rhs, V3<float>{s[12], s[13], s[14]}, s[15], s[19], V3<float>{s[16], s[17], s[18]},
+                             s[20]);
+    float x[6];
+    const bool solved = Engine<float>::solve_6x6(lhs, rhs, x);
+    float* d = delta8 + 8 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      d[k] = solved ? x[k] : 0.0f;
+      d[4 + k] = solved ? x[3 + k] : 0.0f;
+    }
+    d[3] = 0.0f;
+    d[7] = 0.0f;
+    ok[i] = solved ? 1.0f : 0.0f;
+  }
+}
+
 int orc_solve_6x6_f32(const float* lhs36, const float* rhs6, float* x6) {
   float l[6][6], r[6];
   std::memcpy(l, lhs36, sizeof(l));

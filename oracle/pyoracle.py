@@ -64,6 +64,8 @@ def lib():
         L.orc_get_joints.argtypes = [ctypes.c_void_p, _f32p]
         L.orc_get_stats.argtypes = [ctypes.c_void_p, _i32p]
         L.orc_get_toi.argtypes = [ctypes.c_void_p, _f32p, _i32p]
+        L.orc_stage_diag_solve_f32.argtypes = [_f32p, _f32p, ctypes.c_uint32, ctypes.c_float, _f32p]
+        L.orc_stage_contact_solve_f32.argtypes = [_f32p, ctypes.c_uint32, ctypes.c_float, _f32p, _f32p]
         L.orc_solve_6x6_f32.restype = ctypes.c_int
         L.orc_solve_6x6_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_stamp_row_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p, ctypes.c_float]
@@ -85,6 +87,28 @@ def _p(a, t):
 
 def f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def stage_diag_solve(masses, rhs, dt=1.0 / 60.0):
+    """Rung-1 staged solve over (n, 4) mass records and (n, 8) residual
+    records; returns the (n, 8) deltas."""
+    m = f32(masses).reshape(-1, 4)
+    r = f32(rhs).reshape(-1, 8)
+    assert m.shape[0] == r.shape[0]
+    delta = np.zeros((m.shape[0], 8), np.float32)
+    lib().orc_stage_diag_solve_f32(_p(m, _f32p), _p(r, _f32p), m.shape[0], ctypes.c_float(dt), _p(delta, _f32p))
+    return delta
+
+
+def stage_contact_solve(systems, dt=1.0 / 60.0):
+    """Rung-2 staged solve over (n, 24) packed system records; returns
+    ((n, 8) deltas as body-state records, (n,) ok flags)."""
+    sysa = f32(systems).reshape(-1, 24)
+    n = sysa.shape[0]
+    delta = np.zeros((n, 8), np.float32)
+    ok = np.zeros(n, np.float32)
+    lib().orc_stage_contact_solve_f32(_p(sysa, _f32p), n, ctypes.c_float(dt), _p(delta, _f32p), _p(ok, _f32p))
+    return delta, ok
 
 
 class OracleEngine:

@@ -70,6 +70,10 @@ SIGNATURES = {
     "oavbd_get_timing": (ctypes.c_int, [ctypes.c_void_p, _f32p]),
     "oavbd_set_primal_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "oavbd_solve_diag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                        ctypes.c_float, ctypes.c_void_p]),
+    "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
+                                           ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
@@ -94,6 +98,40 @@ def load_library(path: str = LIB_PATH):
 
 def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+def pack_states(rhs):
+    """(n, 6) residuals as GpuBodyState records {lin[3], 0, ang[3], 0}
+    (crates/physics/src/gpu/mod.rs:200-207, from_residual :220-228)."""
+    r = np.atleast_2d(np.asarray(rhs, np.float32))
+    out = np.zeros((r.shape[0], 8), np.float32)
+    out[:, 0:3] = r[:, 0:3]
+    out[:, 4:7] = r[:, 3:6]
+    return out
+
+
+def pack_systems(inv_mass, inertia, rhs, axis, pen, lever, force, sign):
+    """Packed 96-byte staged systems, GpuAvbdSystem::new
+    (crates/physics/src/gpu/avbd.rs:329-337): GpuAvbdMass {inertia[3],
+    inv_mass} (:55-60), the base residual as a GpuBodyState {lin[3], 0,
+    ang[3], 0} (gpu/mod.rs:200-207) and one GpuAvbdRow {axis[3], pen,
+    lever[3], force, sign, 0, 0, 0} (:264-281).  Per-system arguments are
+    broadcast over the leading dimension; statics carry inv_mass 0 and a
+    zero inertia, as GpuAvbdMass::from_body (:62-75) writes them."""
+    inv_mass = np.atleast_1d(np.asarray(inv_mass, np.float32))
+    n = max(inv_mass.shape[0], np.atleast_2d(np.asarray(rhs)).shape[0])
+    out = np.zeros((n, 24), np.float32)
+    out[:, 0:3] = np.broadcast_to(np.asarray(inertia, np.float32), (n, 3))
+    out[:, 3] = np.broadcast_to(inv_mass, (n,))
+    r = np.broadcast_to(np.asarray(rhs, np.float32), (n, 6))
+    out[:, 4:7] = r[:, 0:3]
+    out[:, 8:11] = r[:, 3:6]
+    out[:, 12:15] = np.broadcast_to(np.asarray(axis, np.float32), (n, 3))
+    out[:, 15] = np.broadcast_to(np.asarray(pen, np.float32), (n,))
+    out[:, 16:19] = np.broadcast_to(np.asarray(lever, np.float32), (n, 3))
+    out[:, 19] = np.broadcast_to(np.asarray(force, np.float32), (n,))
+    out[:, 20] = np.broadcast_to(np.asarray(sign, np.float32), (n,))
+    return out
 
 
 class AvbdError(RuntimeError):
@@ -233,6 +271,36 @@ class AvbdEngine:
         s = OavbdStats()
         self._check(self._L.oavbd_get_stats(self._ctx, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in OavbdStats._fields_}
+
+    def solve_diag(self, masses, rhs, dt=1.0 / 60.0):
+        """Rung-1 staged bodies on the device: `masses` is (n, 4) float32
+        GpuAvbdMass records, `rhs` (n, 8) GpuBodyState residuals; returns the
+        (n, 8) GpuBodyState deltas."""
+        m = np.ascontiguousarray(masses, dtype=np.float32).reshape(-1, 4)
+        r = np.ascontiguousarray(rhs, dtype=np.float32).reshape(-1, 8)
+        if m.shape[0] != r.shape[0]:
+            raise ValueError("masses and rhs differ in length")
+        n = m.shape[0]
+        self._ensure_ctx(max(self.n, 1), self.njoints)
+        delta = np.zeros((n, 8), np.float32)
+        self._check(self._L.oavbd_solve_diag(self._ctx, m.ctypes.data_as(ctypes.c_void_p),
+                                             r.ctypes.data_as(ctypes.c_void_p), n, ctypes.c_float(dt),
+                                             delta.ctypes.data_as(ctypes.c_void_p)))
+        return delta
+
+    def solve_systems(self, systems, dt=1.0 / 60.0):
+        """Rung-2 staged systems on the device: `systems` is (n, 24) float32
+        (packed 96-byte GpuAvbdSystem records); returns ((n, 8) deltas as
+        GpuBodyState records, (n,) ok flags)."""
+        sysa = np.ascontiguousarray(systems, dtype=np.float32).reshape(-1, 24)
+        n = sysa.shape[0]
+        self._ensure_ctx(max(self.n, 1), self.njoints)
+        delta = np.zeros((n, 8), np.float32)
+        ok = np.zeros(n, np.float32)
+        self._check(self._L.oavbd_solve_systems(self._ctx, sysa.ctypes.data_as(ctypes.c_void_p), n,
+                                                ctypes.c_float(dt), delta.ctypes.data_as(ctypes.c_void_p),
+                                                _p(ok, _f32p)))
+        return delta, ok
 
     def set_primal_shape(self, lanes=32, waves=4):
         """Primal launch shape: lanes per body, wavefronts per workgroup.

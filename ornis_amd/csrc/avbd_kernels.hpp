@@ -1499,6 +1499,83 @@ __device__ bool solve_6x6_dev(const float* L21, const float* rhs, float* x) {
   return true;
 }
 
+// Lumped inertial Hessian diagonal [m/dt^2 x 3, I/dt^2 x 3] of a GpuAvbdMass
+// {inertia[3], inv_mass} (avbd_inertial_hessian_diag, crates/physics/src/
+// gpu/avbd.rs:88-107); statics, non-finite inputs and degenerate dt give zeros.
+__device__ __forceinline__ void lumped_diag(float4 m, float dt, float dg[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dg[k] = 0.0f;
+  if (isfinite(m.w) && m.w > 0.0f && isfinite(dt) && dt > 0.0f) {
+    const float dt2 = dt * dt;
+    if (isfinite(dt2) && !(dt2 <= K_DEGENERATE)) {
+      const float l = 1.0f / (m.w * dt2);
+      dg[0] = l;
+      dg[1] = l;
+      dg[2] = l;
+      dg[3] = m.x / dt2;
+      dg[4] = m.y / dt2;
+      dg[5] = m.z / dt2;
+    }
+  }
+}
+
+// Rung-1 staged solve (GpuAvbdStub::stage_diag_solve, crates/physics/src/gpu/
+// avbd.rs:207-226), one thread per body: the lumped diagonal of its
+// GpuAvbdMass record and the diagonal solve (avbd_diag_solve_cpu, :117-128):
+// x = rhs / diag on axes above the degenerate threshold, 0 elsewhere.  rhs and
+// delta are GpuBodyState records {lin[3], 0, ang[3], 0}.
+__global__ void k_solve_diag(const float4* __restrict__ mass, const float4* __restrict__ rhs, uint32_t n, float dt,
+                             float4* __restrict__ delta) {
+  uint32_t i = gid();
+  if (i >= n) return;
+  float dg[6];
+  lumped_diag(mass[i], dt, dg);
+  const float4 lin = rhs[2 * (size_t)i], ang = rhs[2 * (size_t)i + 1];
+  const float r[6] = {lin.x, lin.y, lin.z, ang.x, ang.y, ang.z};
+  float x[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) x[k] = dg[k] > K_DEGENERATE ? r[k] / dg[k] : 0.0f;
+  delta[2 * (size_t)i] = make_float4(x[0], x[1], x[2], 0.0f);
+  delta[2 * (size_t)i + 1] = make_float4(x[3], x[4], x[5], 0.0f);
+}
+
+// Rung-2 staged systems (avbd_stage_contact_solve, crates/physics/src/gpu/
+// avbd.rs:415-449), one thread per 96-byte GpuAvbdSystem record: the lumped
+// inertial Hessian diagonal, one stamped row (stamp_row's order: every entry
+// receives one product on top of a zero or the diagonal) and solve_6x6; a
+// breakdown gives a zero delta and ok = 0.  delta holds GpuBodyState records
+// {lin[3], 0, ang[3], 0}.
+__global__ void k_solve_systems(const float4* __restrict__ sys, uint32_t n, float dt, float4* __restrict__ delta,
+                                float* __restrict__ ok) {
+  uint32_t i = gid();
+  if (i >= n) return;
+  const float4* s = sys + 6 * (size_t)i;
+  const float4 m = s[0], lin = s[1], ang = s[2], ax = s[3], lv = s[4], sg = s[5];
+  float dg[6];
+  lumped_diag(m, dt, dg);
+  const f3 nn = smul(sg.x, xyz(ax));
+  const f3 t = cross3(xyz(lv), nn);
+  const float pen = ax.w, f = lv.w;
+  const float v[6] = {nn.x, nn.y, nn.z, t.x, t.y, t.z};
+  float L21[21], rhs[6] = {lin.x, lin.y, lin.z, ang.x, ang.y, ang.z};
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      // lhs[r][c] += pen * (a * b) with the factor order stamp_row uses for
+      // each block: nn nn, t t, and for the lower-left block nn[c] t[r - 3]
+      const float prod = (r >= 3 && c < 3) ? pen * (v[c] * v[r]) : pen * (v[r] * v[c]);
+      L21[tri(r, c)] = (r == c ? dg[r] : 0.0f) + prod;
+    }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) rhs[k] += f * v[k];
+  float x[6];
+  const bool solved = solve_6x6_dev(L21, rhs, x);
+  delta[2 * (size_t)i] = solved ? make_float4(x[0], x[1], x[2], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  delta[2 * (size_t)i + 1] = solved ? make_float4(x[3], x[4], x[5], 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ok[i] = solved ? 1.0f : 0.0f;
+}
+
 // Inertial block and rhs of body h (assembly.rs:74-96): acc[0..20] lower
 // triangle, acc[21..26] rhs.
 __device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, int i, int j) {

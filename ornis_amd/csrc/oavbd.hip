@@ -1089,6 +1089,53 @@ int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   return OAVBD_OK;
 }
 
+int oavbd_solve_diag(oavbd_ctx* c, const void* masses16, const void* rhs32, uint32_t n, float dt, void* delta32) {
+  if (!c || (n && (!masses16 || !rhs32 || !delta32))) return OAVBD_EINVAL;
+  if (!n) return OAVBD_OK;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    // Buffers live for this call only and are released on every exit path.
+    struct Staging {
+      float4 *mass = nullptr, *rhs = nullptr, *dl = nullptr;
+      ~Staging() { dfree(mass); dfree(rhs); dfree(dl); }
+    } b;
+    dalloc(&b.mass, n);
+    dalloc(&b.rhs, 2 * (size_t)n);
+    dalloc(&b.dl, 2 * (size_t)n);
+    HIPC(hipMemcpyAsync(b.mass, masses16, 16 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    HIPC(hipMemcpyAsync(b.rhs, rhs32, 32 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    k_solve_diag<<<blocks(n), TPB, 0, c->st>>>(b.mass, b.rhs, n, dt, b.dl);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(delta32, b.dl, 32 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_solve_systems(oavbd_ctx* c, const void* systems96, uint32_t n, float dt, void* delta32, float* ok) {
+  if (!c || (n && (!systems96 || !delta32 || !ok))) return OAVBD_EINVAL;
+  if (!n) return OAVBD_OK;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    // Buffers live for this call only and are released on every exit path.
+    struct Staging {
+      float4 *sys = nullptr, *dl = nullptr;
+      float* ok = nullptr;
+      ~Staging() { dfree(sys); dfree(dl); dfree(ok); }
+    } b;
+    dalloc(&b.sys, 6 * (size_t)n);
+    dalloc(&b.dl, 2 * (size_t)n);
+    dalloc(&b.ok, n);
+    HIPC(hipMemcpyAsync(b.sys, systems96, 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    k_solve_systems<<<blocks(n), TPB, 0, c->st>>>(b.sys, n, dt, b.dl, b.ok);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(delta32, b.dl, 32 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(ok, b.ok, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
 int oavbd_get_kernel_times(oavbd_ctx* c, oavbd_kernel_times* out, int reset) {
   if (!c || !out) return OAVBD_EINVAL;
   *out = c->kt;
