@@ -19,6 +19,8 @@
 //   k_bdf1 / sleep     velocities, renormalisation, sleep bookkeeping
 #pragma once
 
+#include <type_traits>
+
 #include "avbd_geom.hpp"
 #include "avbd_group.hpp"
 #include "avbd_state.hpp"
@@ -1250,10 +1252,16 @@ __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
   __syncthreads();
   if (threadIdx.x < 64 && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&d.colour_cur[threadIdx.x], cnt[threadIdx.x]);
   __syncthreads();
-  if (c >= 0 && c < 64)
-    d.colour_list[d.colour_off[c] + base[c] + local] = h;
-  else if (c >= 64)
-    d.colour_list[d.colour_off[c] + atomicAdd(&d.colour_cur[c], 1u)] = h;
+  if (c < 0) return;
+  // the primal's per-body record: body (bit 31: has joint sides), first
+  // incidence item, item count, first joint side
+  const uint32_t i0 = d.inc_off[h], j0 = d.b.ji_off[h];
+  const uint32_t nj = d.b.ji_off[h + 1] - j0;
+  const uint4 rec = make_uint4(h | (nj ? 0x80000000u : 0u), i0, d.inc_off[h + 1] - i0, j0);
+  if (c < 64)
+    d.colour_list[d.colour_off[c] + base[c] + local] = rec;
+  else
+    d.colour_list[d.colour_off[c] + atomicAdd(&d.colour_cur[c], 1u)] = rec;
 }
 
 // Solver records for the iteration loop (pos, rot, pos0, drot per body);
@@ -1279,6 +1287,25 @@ __global__ void k_sb_unpack(Dev d) {
 // ---------------------------------------------------------------------------
 // primal: solve_body (assembly.rs:73-903) for the bodies of one colour
 // ---------------------------------------------------------------------------
+
+// Value of lane `src` of this lane's L-lane group (src a compile-time
+// constant below L): one v_readlane per group of the wavefront instead of a
+// permute through LDS.
+template <int L>
+__device__ __forceinline__ float bcast_lane(float v, int src) {
+  const int lane = (int)__lane_id();
+  float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+#pragma unroll
+  for (int g = 1; g < 64 / L; ++g) {
+    const float o = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), g * L + src));
+    r = lane / L == g ? o : r;
+  }
+  return r;
+}
+
+// Keeps a loaded value in registers at this point: the compiler may neither
+// sink its load past the point nor re-load it later.
+__device__ __forceinline__ void pin4(float4& v) { asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
 
 // A constraint row as the accumulation sees it: a full stamp (stamp_row:
 // axis nn, angular lever t, penalty, force), an angular couple
@@ -1317,22 +1344,31 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
   float sign = is_a ? 1.0f : -1.0f;
   uint32_t p = it.y;
   uint2 ab = make_uint2(it.z, it.w);
-  float4 n4 = d.P.n(p);
+  // The four 64-byte records this item reads (pair, point, both bodies) are
+  // loaded together and pinned before any branch, so the item costs one
+  // memory round trip instead of one per branch that first uses a field.
+  const float4* prec = d.P.rec + 4 * (size_t)p;
+  const float4* qrec = d.Q.rec + 4 * (size_t)q;
+  const float4* sa = d.sb + 4 * (size_t)ab.x;
+  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  float4 n4 = prec[0], t14 = prec[1], t24 = prec[2], mu = prec[3];
+  float4 ra4 = qrec[0], rb4 = qrec[1], lam = qrec[2], pen = qrec[3];
+  float4 pa4 = sa[0], sa1 = sa[1], a0 = sa[2], sa3 = sa[3];
+  float4 pb4 = sbb[0], sb1 = sbb[1], b0 = sbb[2], sb3 = sbb[3];
+  pin4(n4); pin4(t14); pin4(t24); pin4(mu);
+  pin4(ra4); pin4(rb4); pin4(lam); pin4(pen);
+  pin4(pa4); pin4(sa1); pin4(a0); pin4(sa3);
+  pin4(pb4); pin4(sb1); pin4(b0); pin4(sb3);
   f3 n = xyz(n4);
   float gap = n4.w;
   bool touching = gap <= 0.0f;
-  float4 ra4 = d.Q.ra(q);
   float c0 = touching ? ra4.w : 0.0f;
-  const float4* sa = d.sb + 4 * (size_t)ab.x;
-  const float4* sbb = d.sb + 4 * (size_t)ab.y;
-  float4 pa4 = sa[0], pb4 = sbb[0];
-  float4 a0 = sa[2], b0 = sbb[2];
-  qf qa = asq(sa[1]), qb = asq(sbb[1]);
-  f3 dra = xyz(sa[3]), drb = xyz(sbb[3]);
+  qf qa = asq(sa1), qb = asq(sb1);
+  f3 dra = xyz(sa3), drb = xyz(sb3);
   f3 dpa = sub(xyz(pa4), xyz(a0)), dpb = sub(xyz(pb4), xyz(b0));
   // contact_levers (rows.rs:963-973)
   f3 ra_w = a0.w >= 0.0f ? mul(neg(n), a0.w) : qrot(qa, xyz(ra4));
-  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(d.Q.rb(q)));
+  f3 rb_w = b0.w >= 0.0f ? mul(n, b0.w) : qrot(qb, xyz(rb4));
   // row_c_levers (rows.rs:1006-1025)
   auto row_c = [&](f3 u, float cc, f3 la, f3 lb) {
     f3 d_a = add(dpa, mul(u, dot3(cross3(la, u), dra)));
@@ -1341,7 +1377,6 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
   };
   float cn = row_c(n, c0, ra_w, rb_w);
   f3 r_side = is_a ? ra_w : rb_w;
-  float4 t14 = d.P.t1(p);
   if (!touching) {
     float damp = t14.w;
     if (damp > 0.0f && fabsf(cn) >= K_C_EPS) {
@@ -1351,15 +1386,13 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
     }
     return 0u;
   }
-  f3 t1 = xyz(t14), t2 = xyz(d.P.t2(p));
+  f3 t1 = xyz(t14), t2 = xyz(t24);
   // friction_levers (rows.rs:987-996)
   float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
   f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
   f3 rb_f = add(rb_w, mul(n, sep * 0.5f));
   float ct1 = row_c(t1, 0.0f, ra_f, rb_f);
   float ct2 = row_c(t2, 0.0f, ra_f, rb_f);
-  float4 pen = d.Q.pen(q), lam = d.Q.lam(q);
-  float4 mu = d.P.mu(p);
   // contact_force (rows.rs:1032-1060)
   float fn_c = fmin_(pen.x * cn + lam.x, 0.0f);
   float r0 = pen.y * ct1 + lam.y, r1 = pen.z * ct2 + lam.z;
@@ -1576,31 +1609,44 @@ __global__ void k_solve_systems(const float4* __restrict__ sys, uint32_t n, floa
   ok[i] = solved ? 1.0f : 0.0f;
 }
 
-// Inertial block and rhs of body h (assembly.rs:74-96): acc[0..20] lower
-// triangle, acc[21..26] rhs.
-__device__ __forceinline__ float inertial_elem(const Dev& d, uint32_t h, int e, int i, int j) {
-  float4 p = d.sb[4 * (size_t)h];
-  float m_dt2 = 1.0f / p.w / (K_DT * K_DT);
-  qf q = asq(d.sb[4 * (size_t)h + 1]);
+// Inputs of the inertial block of body h: its solver record (pos, rot), its
+// body-frame inertia and its inertial targets.
+struct InertialIn {
+  float4 p, q, inertia, target, target_rot;
+};
+__device__ __forceinline__ InertialIn load_inertial(const Dev& d, uint32_t h) {
+  InertialIn in;
+  in.p = d.sb[4 * (size_t)h];
+  in.q = d.sb[4 * (size_t)h + 1];
+  in.inertia = d.b.inertia[h];
+  in.target = d.b.inertial[h];
+  in.target_rot = d.b.inertial_rot[h];
+  return in;
+}
+
+// Inertial block and rhs of a body (assembly.rs:74-96): out[0..20] the
+// lower triangle (m/dt^2 on the linear diagonal, I_world/dt^2 in the angular
+// block), out[21..26] the rhs (m/dt^2 (x - x~), I_world dq / dt^2), out[27] 0.
+__device__ __forceinline__ void inertial_block(const InertialIn& in, float out[28]) {
+  const float m_dt2 = 1.0f / in.p.w / (K_DT * K_DT);
+  const qf q = asq(in.q);
   float iw[3][3];
-  world_inertia(xyz(d.b.inertia[h]), q, iw);
-  if (e < 21) {
-    if (i < 3) return i == j ? m_dt2 : 0.0f;
-    if (j < 3) return 0.0f;
-    // row i - 3, column j - 3 of iw, selected without indexing a private
-    // array by a runtime value (which would go through scratch memory)
-    f3 r = i == 3 ? f3{iw[0][0], iw[0][1], iw[0][2]} : (i == 4 ? f3{iw[1][0], iw[1][1], iw[1][2]}
-                                                              : f3{iw[2][0], iw[2][1], iw[2][2]});
-    return comp(r, j - 3) / (K_DT * K_DT);
-  }
-  int k = e - 21;
-  if (k < 3) {
-    f3 rl = smul(m_dt2, sub(xyz(p), xyz(d.b.inertial[h])));
-    return comp(rl, k);
-  }
-  f3 dv = quat_diff_vec(q, asq(d.b.inertial_rot[h]));
-  f3 ra = divs(mat3_vec(iw, dv), K_DT * K_DT);
-  return comp(ra, k - 3);
+  world_inertia(xyz(in.inertia), q, iw);
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j)
+      out[tri(i, j)] = i < 3 ? (i == j ? m_dt2 : 0.0f) : (j < 3 ? 0.0f : iw[i - 3][j - 3] / (K_DT * K_DT));
+  const f3 rl = smul(m_dt2, sub(xyz(in.p), xyz(in.target)));
+  const f3 dv = quat_diff_vec(q, asq(in.target_rot));
+  const f3 ra = divs(mat3_vec(iw, dv), K_DT * K_DT);
+  out[21] = rl.x;
+  out[22] = rl.y;
+  out[23] = rl.z;
+  out[24] = ra.x;
+  out[25] = ra.y;
+  out[26] = ra.z;
+  out[27] = 0.0f;
 }
 
 #define WAVE 64
@@ -1626,10 +1672,12 @@ constexpr uint32_t TOUCH_DIAG = (1u << 9) | (1u << 14) | (1u << 20);
 // and solved by the first wavefront, one lane per body.  R = row slots per
 // item.
 template <int L, int W, int R>
-__global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __restrict__ list, uint32_t count) {
+__global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __restrict__ list, uint32_t count) {
   constexpr int G = WAVE / L;                 // bodies per wavefront
   constexpr int NB = G * W;                   // bodies per workgroup
   constexpr int NE = (27 + L - 1) / L;        // elements per lane
+  constexpr int U = 8;                        // contact rows per LDS round trip
+  static_assert((L * R) % U == 0, "the row buffer holds whole groups");
   __shared__ Row rows[NB][L * R];
   __shared__ uint32_t touch[NB][L * R];
   __shared__ float sums[NB][28];
@@ -1638,13 +1686,16 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   const int grp = tid / L, sub = tid % L;     // grp: body of this lane in the workgroup
   const uint32_t slot = blockIdx.x * NB + grp;
   const bool valid = slot < count;
-  const uint32_t h = valid ? list[slot] : 0u;
-  uint32_t i0 = 0, np = 0, j0 = 0, nj = 0;
+  // colour-list record: body (bit 31: has joint sides), first incidence
+  // item, item count, first joint side
+  uint32_t h = 0, i0 = 0, np = 0, j0 = 0, nj = 0;
   if (valid) {
-    i0 = d.inc_off[h];
-    np = d.inc_off[h + 1] - i0;
-    j0 = d.b.ji_off[h];
-    nj = d.b.ji_off[h + 1] - j0;
+    const uint4 cr = list[slot];
+    h = cr.x & 0x7fffffffu;
+    i0 = cr.y;
+    np = cr.z;
+    j0 = cr.w;
+    if (cr.x >> 31) nj = d.b.ji_off[h + 1] - j0;
   }
   const uint32_t total = np + nj;
   // element q of this lane: matrix entry (ia, ib) or rhs entry ia
@@ -1666,12 +1717,19 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     ibf[k] = e < 21 ? j : 7;
     ebit[k] = e < 27 ? 1u << e : 0u;
     skip_couple[k] = e < 21 ? j < 3 : e - 21 < 3;
-    acc[k] = (valid && e < 27) ? inertial_elem(d, h, e, i, j) : 0.0f;
+    acc[k] = 0.0f;
   }
   if (sub == 0) chunks_s[grp] = (total + L - 1) / L;
   __syncthreads();
   uint32_t nch = 0;
   for (int g = 0; g < NB; ++g) nch = chunks_s[g] > nch ? chunks_s[g] : nch;
+  // The body's inertial block (evaluated before the sweep) starts every
+  // element's sum; the rows are added to it in order.
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = sub + k * L;
+    acc[k] = (valid && e < 27) ? d.inb[28 * (size_t)h + e] : 0.0f;
+  }
   const float* rowf = reinterpret_cast<const float*>(&rows[grp][0]);
   for (uint32_t c = 0; c < nch; ++c) {
     uint32_t k = c * L + sub;
@@ -1681,21 +1739,25 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
       mask = point_rows<R>(d, d.inc[i0 + k], tmp);
     else if (k < total)
       mask = joint_rows<R>(d, d.b.ji[j0 + (k - np)], tmp);
-    // pack the rows: exclusive prefix of the row counts over the body's lanes
+    // pack the rows: exclusive prefix of the row counts (below 8) over the
+    // body's lanes, from one ballot per bit of the count; rows of point items
+    // come first (the lanes below np - c * L hold points)
     const int m = __popc(mask);
-    int pre = m;
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) {
-      int v = __shfl_up(pre, o, L);
-      if (sub >= o) pre += v;
-    }
-    const int nrow = __shfl(pre, L - 1, L);
-    // rows of point items come first: the lanes below np - c * L hold points
     const int64_t pl64 = (int64_t)np - (int64_t)c * L;
     const int pl = pl64 <= 0 ? 0 : (pl64 >= L ? L : (int)pl64);
-    const int npr = __shfl(pre, pl > 0 ? pl - 1 : 0, L);
-    const int nprow = pl > 0 ? npr : 0;
-    int w = pre - m;
+    const int lane = (int)__lane_id();
+    const int gbase = lane / L * L;
+    const uint64_t below = lane == 0 ? 0ull : ~0ull >> (64 - lane);
+    const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << (L & 63)) - 1ull)) << gbase;
+    const uint64_t pmask = (pl == 0 ? 0ull : ~0ull >> (64 - pl)) << gbase;
+    int w = 0, nrow = 0, nprow = 0;
+#pragma unroll
+    for (int bit = 0; bit < 3; ++bit) {
+      const uint64_t bm = __ballot((m >> bit) & 1);
+      w += __popcll(bm & gmask & below) << bit;
+      nrow += __popcll(bm & gmask) << bit;
+      nprow += __popcll(bm & pmask) << bit;
+    }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (!((mask >> r) & 1u)) continue;
@@ -1706,20 +1768,28 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     __syncthreads();
     // contact rows: stamps (and angular couples when R == 6).  A matrix
     // lane adds |pen| * (x_i * x_j); an rhs lane adds 1 * (x_k * f), which
-    // is f * x_k exactly (products commute, 1 * y = y).
-#pragma unroll 4
-    for (int t = 0; t < nprow; ++t) {
-      const float* rw = rowf + 8 * t;
-      const float pen = rw[6];
+    // is f * x_k exactly (products commute, 1 * y = y).  Rows are read U at
+    // a time, so one LDS round trip serves U rows; slots of the last group
+    // past the body's contact rows (the buffer holds whole groups) add -0.0.
+    auto add_group = [&](int t0, auto partial) {
 #pragma unroll
-      for (int q = 0; q < NE; ++q) {
-        const float a = rw[ia[q]], b = rw[ibf[q]];
-        const float sc = rhs[q] ? 1.0f : fabsf(pen);
-        float v = sc * (a * b);
-        if constexpr (R >= 6) v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
-        acc[q] += v;
+      for (int u = 0; u < U; ++u) {
+        const float* rw = rowf + 8 * (t0 + u);
+        const float pen = rw[6];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+          const float a = rw[ia[q]], b = rw[ibf[q]];
+          const float sc = rhs[q] ? 1.0f : fabsf(pen);
+          float v = sc * (a * b);
+          if constexpr (R >= 6) v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
+          if constexpr (decltype(partial)::value) v = t0 + u < nprow ? v : -0.0f;
+          acc[q] += v;
+        }
       }
-    }
+    };
+    const int nfull = nprow / U * U;
+    for (int t0 = 0; t0 < nfull; t0 += U) add_group(t0, std::false_type{});
+    if (nfull < nprow) add_group(nfull, std::true_type{});
     // joint rows: stamps and geometric-stiffness diagonals
     for (int t = nprow; t < nrow; ++t) {
       const float* rw = rowf + 8 * t;
@@ -1738,7 +1808,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   }
   bool solved;
   float dx[6];
-  if constexpr (NE == 1) {
+  if constexpr (NE == 1 && L < 64) {
     // LDL^T factorization and forward substitution over the body's lanes,
     // right-looking: step k finalises column k (l_ik = a_ik / d_k) and
     // subtracts (l_ik * d_k) * l_jk from every later entry, and y_k l_rk
@@ -1751,13 +1821,13 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
     bool ok = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      const float dgk = __shfl(val, tri(k, k), L);
+      const float dgk = bcast_lane<L>(val, tri(k, k));
       ok = ok && !(dgk <= K_DEGENERATE);
       if (is_mat && j == k && i > k) val = val / dgk;
       const float li = __shfl(val, tri(i > k ? i : k, k), L);
       const float lj = __shfl(val, tri(j > k ? j : k, k), L);
       if (is_mat && j > k) val -= (li * dgk) * lj;
-      const float yk = __shfl(val, 21 + k, L);
+      const float yk = bcast_lane<L>(val, 21 + k);
       if (is_rhs && i > k) val -= li * yk;
     }
     const float dgi = __shfl(val, tri(i < 6 ? i : 5, i < 6 ? i : 5), L);
@@ -1791,7 +1861,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
   }
   const uint32_t s2 = blockIdx.x * NB + tid;
   if (s2 >= count) return;
-  const uint32_t hb = list[s2];
+  const uint32_t hb = list[s2].x & 0x7fffffffu;
   if (solved) {
     float4* rec = d.sb + 4 * (size_t)hb;
     float4 p = rec[0];
@@ -1807,9 +1877,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal(Dev d, const uint32_t* __res
 // ---------------------------------------------------------------------------
 // dual update (joints_ext.rs:464-598), one thread per point / joint
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(TPB) k_dual(Dev d) {
-  uint32_t q = gid();
-  if (q >= d.npts) return;
+__device__ __forceinline__ void dual_point(const Dev& d, uint32_t q) {
   uint32_t p = d.Q.pair[q];
   uint2 ab = d.P.ab[p];
   float4 n4 = d.P.n(p);
@@ -1895,6 +1963,33 @@ __global__ void __launch_bounds__(TPB) k_dual(Dev d) {
     }
     d.Q.roll[q] = make_float4(rlv[0], rlv[1], rlv[2], 0.0f);
   }
+}
+
+// Inertial block (27 values) of every coloured body for the next primal
+// sweep.  It depends only on the body's own pose, which changes only when
+// the body itself is solved, so it is evaluated once per iteration off the
+// sweep's critical path.
+__device__ __forceinline__ void store_inertial_block(const Dev& d, uint32_t h) {
+  if (d.b.colour[h] < 0) return;
+  float v[28];
+  inertial_block(load_inertial(d, h), v);
+  float4* out = reinterpret_cast<float4*>(d.inb + 28 * (size_t)h);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) out[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+}
+
+// Dual update of every point (threads below npts) and, when `next` is set,
+// the inertial blocks of the next iteration's sweep (threads above).
+__global__ void __launch_bounds__(TPB) k_dual(Dev d, uint32_t next) {
+  uint32_t g = gid();
+  if (g < d.npts) return dual_point(d, g);
+  g -= d.npts;
+  if (next && g < d.n) store_inertial_block(d, g);
+}
+
+__global__ void k_inertial_blocks(Dev d) {
+  uint32_t h = gid();
+  if (h < d.n) store_inertial_block(d, h);
 }
 
 __global__ void k_dual_joint(Dev d) {

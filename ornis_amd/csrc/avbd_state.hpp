@@ -187,6 +187,7 @@ struct Dev {
   uint32_t* acc_off;   // ncand+1 exclusive scan of accepted
   uint32_t* poff;      // n+1 first pair of each body's a-run in the current table
   float4* sb;          // 4 per body during the primal/dual loop: pos, rot, pos0, drot
+  float* inb;           // 28 per body: inertial block (lower triangle, rhs) of the next sweep
                        // (one 64-byte record, so a partner's state is one cache line)
   uint32_t* poff_old;  // n+1 the same for the previous step's table
   uint32_t* pt_off;    // ncand+1 exclusive scan of fresh points
@@ -208,7 +209,7 @@ struct Dev {
   uint32_t* colour_cnt;  // cstat + CSTAT_HEAD: bodies per colour
   uint32_t* colour_off;
   uint32_t* colour_cur;
-  uint32_t* colour_list;
+  uint4* colour_list;       // per coloured body: h | joints << 31, first item, items, first joint side
   float4* toi_hit;     // normal xyz, w = travel to the hit (< 0: none)
   uint32_t* toi_tgt;
   uint32_t* toi_cnt;   // n+1 cast candidates per mover

@@ -131,7 +131,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.counters, CTR_COUNT);
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
-  dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4);
+  dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4); dalloc(&d.inb, (size_t)n * 28);
   dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
   dalloc(&d.nbr_cur, n + 1);
   d.nbr = nullptr;
@@ -246,7 +246,7 @@ static void free_all(oavbd_ctx* c) {
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
-  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
+  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
@@ -492,7 +492,7 @@ static void incidence_and_colour(oavbd_ctx* c) {
 }
 
 template <int L, int W>
-static void launch_lw(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+static void launch_lw(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
   constexpr int NB = (WAVE / L) * W;
   if (c->rows6)
     k_primal<L, W, 6><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
@@ -501,7 +501,7 @@ static void launch_lw(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
 }
 
 template <int L>
-static void launch_l(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+static void launch_l(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
   switch (c->waves) {
     case 1: return launch_lw<L, 1>(c, list, cnt);
     case 2: return launch_lw<L, 2>(c, list, cnt);
@@ -511,8 +511,12 @@ static void launch_l(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
 }
 
 // Primal launch of one colour class: `lanes` lanes per body, `waves`
-// wavefronts per workgroup (the configured launch shape).
-static void launch_primal(oavbd_ctx* c, const uint32_t* list, uint32_t cnt) {
+// wavefronts per workgroup (the configured launch shape).  A colour of fewer
+// than PRIMAL_TAIL bodies does not fill the chip; its time is the latency of
+// its slowest body, which 64 lanes per body shorten (fewer chunks of items).
+constexpr uint32_t PRIMAL_TAIL = 2048;
+static void launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+  if (cnt < PRIMAL_TAIL) return launch_l<64>(c, list, cnt);
   switch (c->lanes) {
     case 8: return launch_l<8>(c, list, cnt);
     case 16: return launch_l<16>(c, list, cnt);
@@ -567,6 +571,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   if (d.npairs) k_pair_prep<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   incidence_and_colour(c);
   k_sb_pack<<<blocks(n), TPB, 0, c->st>>>(d);
+  k_inertial_blocks<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 1);
   const uint32_t ncol = c->stats.colours;
   // In timing mode every primal / dual launch is bracketed by pool events;
@@ -590,7 +595,9 @@ static void step_inner(oavbd_ctx* c, bool finish) {
       np_ev += 2;
     }
     if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev), c->st));
-    if (d.npts) k_dual<<<blocks(d.npts), TPB, 0, c->st>>>(d);
+    // the dual update, and the next sweep's inertial blocks
+    const uint32_t next = it + 1 < c->prm.iters ? 1u : 0u;
+    if (d.npts || next) k_dual<<<blocks((size_t)d.npts + (next ? n : 0u)), TPB, 0, c->st>>>(d, next);
     if (d.njoints) k_dual_joint<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
     if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev + 1), c->st));
     nd_ev += 2;
