@@ -1147,6 +1147,7 @@ __global__ void k_colour_init(Dev d) {
   uint32_t h = gid();
   if (h < CSTAT_HEAD + 65536) d.cstat[h] = 0u;
   if (h < 65536) d.colour_cur[h] = 0u;
+  if (h < 2 * CCLS) d.ccls[h] = 0u;
   if (h > d.n) return;
   d.ptcnt[h] = 0u;
   d.bcnt[h] = 0u;
@@ -1157,6 +1158,19 @@ __global__ void k_colour_init(Dev d) {
   d.sw[h] = swept(d, h) ? 1 : 0;
   d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
   d.b.colour[h] = -1;
+}
+
+// Size class of a body for the primal's colour lists: class 0 holds the
+// bodies with the most items (point-sides and joint-sides), class 7 the
+// fewest.  Within a colour the list runs from class 0 to class 7, so the
+// bodies sharing a wavefront have similar work and the longest solves start
+// first.
+__device__ __forceinline__ uint32_t item_class(uint32_t items) {
+  return items > 96u ? 0u : items > 64u ? 1u : items > 48u ? 2u : items > 32u ? 3u
+       : items > 24u ? 4u : items > 16u ? 5u : items > 8u ? 6u : 7u;
+}
+__device__ __forceinline__ uint32_t body_items(const Dev& d, uint32_t h) {
+  return d.inc_off[h + 1] - d.inc_off[h] + d.b.ji_off[h + 1] - d.b.ji_off[h];
 }
 
 // Appends v to list for every active lane whose pred holds (one atomic per
@@ -1239,18 +1253,38 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_
   }
 }
 
-// Colour lists: slots within a block from LDS counters, one global atomic
-// per colour per block (colours >= 64, rare, take one atomic per body).
-__global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
-  __shared__ uint32_t cnt[64], base[64];
+// Bodies per (colour, size class), from per-block LDS histograms.
+__global__ void __launch_bounds__(TPB) k_colour_classes(Dev d) {
+  __shared__ uint32_t cnt[CCLS];
   uint32_t h = gid();
-  if (threadIdx.x < 64) cnt[threadIdx.x] = 0u;
+  for (uint32_t i = threadIdx.x; i < CCLS; i += TPB) cnt[i] = 0u;
   __syncthreads();
   const int c = h < d.n ? d.b.colour[h] : -1;
-  uint32_t local = 0;
-  if (c >= 0 && c < 64) local = atomicAdd(&cnt[c], 1u);
+  if (c >= 0 && c < CCLS_COLOURS) atomicAdd(&cnt[(uint32_t)c * 8u + item_class(body_items(d, h))], 1u);
   __syncthreads();
-  if (threadIdx.x < 64 && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&d.colour_cur[threadIdx.x], cnt[threadIdx.x]);
+  for (uint32_t i = threadIdx.x; i < CCLS; i += TPB)
+    if (cnt[i]) atomicAdd(&d.ccls[i], cnt[i]);
+}
+
+// Colour lists: each colour below CCLS_COLOURS is laid out by size class
+// (class 0 first); slots within a block come from LDS counters, with one
+// global atomic per (colour, class) per block.  Colours beyond (rare) take
+// one atomic per body and are not ordered.
+__global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
+  __shared__ uint32_t cnt[CCLS], base[CCLS];
+  uint32_t h = gid();
+  for (uint32_t i = threadIdx.x; i < CCLS; i += TPB) cnt[i] = 0u;
+  __syncthreads();
+  const int c = h < d.n ? d.b.colour[h] : -1;
+  uint32_t items = 0, slot = 0, local = 0;
+  if (c >= 0) items = body_items(d, h);
+  if (c >= 0 && c < CCLS_COLOURS) {
+    slot = (uint32_t)c * 8u + item_class(items);
+    local = atomicAdd(&cnt[slot], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < CCLS; i += TPB)
+    if (cnt[i]) base[i] = atomicAdd(&d.ccls[CCLS + i], cnt[i]);
   __syncthreads();
   if (c < 0) return;
   // the primal's per-body record: body (bit 31: has joint sides), first
@@ -1258,10 +1292,13 @@ __global__ void __launch_bounds__(TPB) k_colour_fill(Dev d) {
   const uint32_t i0 = d.inc_off[h], j0 = d.b.ji_off[h];
   const uint32_t nj = d.b.ji_off[h + 1] - j0;
   const uint4 rec = make_uint4(h | (nj ? 0x80000000u : 0u), i0, d.inc_off[h + 1] - i0, j0);
-  if (c < 64)
-    d.colour_list[d.colour_off[c] + base[c] + local] = rec;
-  else
+  if (c < CCLS_COLOURS) {
+    uint32_t off = d.colour_off[c];
+    for (uint32_t k = (uint32_t)c * 8u; k < slot; ++k) off += d.ccls[k];
+    d.colour_list[off + base[slot] + local] = rec;
+  } else {
     d.colour_list[d.colour_off[c] + atomicAdd(&d.colour_cur[c], 1u)] = rec;
+  }
 }
 
 // Solver records for the iteration loop (pos, rot, pos0, drot per body);

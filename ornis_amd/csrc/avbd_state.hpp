@@ -47,6 +47,8 @@
 #define CSTAT_MAX 128    // word of the colour count (own cache line)
 #define CSTAT_HEAD 160   // words before the colour sizes in cstat
 #define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
+#define CCLS_COLOURS 64  // colours whose lists are ordered by size class
+#define CCLS (CCLS_COLOURS * 8)  // (colour, size class) counters
 static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
 #define TOI_LANES 32    // lanes cooperating on one TOI cast
@@ -209,6 +211,7 @@ struct Dev {
   uint32_t* colour_cnt;  // cstat + CSTAT_HEAD: bodies per colour
   uint32_t* colour_off;
   uint32_t* colour_cur;
+  uint32_t* ccls;        // 2 * CCLS: bodies per (colour, size class), then fill cursors
   uint4* colour_list;       // per coloured body: h | joints << 31, first item, items, first joint side
   float4* toi_hit;     // normal xyz, w = travel to the hit (< 0: none)
   uint32_t* toi_tgt;

@@ -136,7 +136,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.nbr_cur, n + 1);
   d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
-  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); dalloc(&d.pend, n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536);
+  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); dalloc(&d.pend, n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536); dalloc(&d.ccls, 2 * CCLS);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
@@ -247,7 +247,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
-  dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.colour_list); dfree(d.toi_hit);
+  dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
 }
@@ -481,6 +481,7 @@ static void incidence_and_colour(oavbd_ctx* c) {
     HIPC(hipMemcpyAsync(d.colour_off, c->colour_off_h.data(), (ncol + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                         c->st));
   }
+  k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   c->stats.colours = ncol;
   c->stats.max_colour = 0;
