@@ -662,7 +662,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
     sgn = sat;
   } else {
     float sat;
-    dd = shape_distance(A, B, &sat);
+    dd = shape_distance<false>(A, B, &sat);  // never box-box here
     sgn = dd.dist;
   }
   // orient_contact_normal (rows.rs:303-333)

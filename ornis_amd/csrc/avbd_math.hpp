@@ -128,7 +128,17 @@ HD m3c mat3_from_quat(qf q) {
   m.c2 = f3{xz + wy, yz - wx, 1.0f - (xx + yy)};
   return m;
 }
-HD f3 mcol(const m3c& m, int i) { return i == 0 ? m.c0 : (i == 1 ? m.c1 : m.c2); }
+// Column i (0, 1 or 2), selected with bit masks: with a lane-dependent i a
+// select between the columns is otherwise lowered to a table in private
+// (scratch) memory.
+HD float bitsel3(float a, float b, float c, int i) {
+  const int m0 = -(int)(i == 0), m1 = -(int)(i == 1), m2 = -(int)(i == 2);
+  return __int_as_float((__float_as_int(a) & m0) | (__float_as_int(b) & m1) | (__float_as_int(c) & m2));
+}
+HD f3 mcol(const m3c& m, int i) {
+  return f3{bitsel3(m.c0.x, m.c1.x, m.c2.x, i), bitsel3(m.c0.y, m.c1.y, m.c2.y, i),
+            bitsel3(m.c0.z, m.c1.z, m.c2.z, i)};
+}
 
 // tangent_basis (crates/physics/src/math.rs:152-160).
 HD void tangent_basis(f3 n, f3* t1, f3* t2) {
