@@ -1078,25 +1078,32 @@ __global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
   }
   block_add(&d.counters[CTR_JSIDES], sides);
 }
+// Colouring neighbours of a swept pair or joint (a, b): each body lists its
+// higher-priority neighbours from the front of its range (their number ends
+// in nbr_cur) and its lower-priority ones from the back (nbr_dn).
+__device__ __forceinline__ void nbr_link(const Dev& d, uint2 ab) {
+  const bool b_up = d.ckey[ab.y] > d.ckey[ab.x];
+  if (b_up) {
+    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
+    d.nbr[d.nbr_off[ab.y + 1] - 1u - atomicAdd(&d.nbr_dn[ab.y], 1u)] = ab.x;
+  } else {
+    d.nbr[d.nbr_off[ab.x + 1] - 1u - atomicAdd(&d.nbr_dn[ab.x], 1u)] = ab.y;
+    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
+  }
+}
 __global__ void k_inc_fill(Dev d) {
   uint32_t p = gid();
   if (p >= d.npairs) return;
   uint2 ab = d.P.ab[p];
   bool sa = d.sw[ab.x], sb = d.sw[ab.y];
   if (sb) d.blist[d.boff[ab.y] + atomicAdd(&d.bcur[ab.y], 1u)] = p;
-  if (sa && sb && d.P.pts[p].y) {
-    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
-    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
-  }
+  if (sa && sb && d.P.pts[p].y) nbr_link(d, ab);
 }
 __global__ void k_nbr_joint_fill(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
   uint2 ab = d.J.ab[j];
-  if (d.sw[ab.x] && d.sw[ab.y]) {
-    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
-    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
-  }
+  if (d.sw[ab.x] && d.sw[ab.y]) nbr_link(d, ab);
 }
 // Body h's point-sides in ascending pair index: pairs where h is b (a < h),
 // then the contiguous run where h is a.
@@ -1154,6 +1161,7 @@ __global__ void k_colour_init(Dev d) {
   d.bcur[h] = 0u;
   d.nbr_cnt[h] = 0u;
   d.nbr_cur[h] = 0u;
+  d.nbr_dn[h] = 0u;
   if (h == d.n) return;
   d.sw[h] = swept(d, h) ? 1 : 0;
   d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
@@ -1192,9 +1200,7 @@ __global__ void __launch_bounds__(TPB) k_colour_count(Dev d, uint32_t* out_list,
   if (h == 0) d.counters[CTR_PTSIDES] = d.inc_off[d.n];  // point-sides of swept bodies
   bool ready = false;
   if (h < d.n && d.sw[h]) {
-    const unsigned long long kh = d.ckey[h];
-    uint32_t cnt = 0;
-    for (uint32_t k = d.nbr_off[h]; k < d.nbr_off[h + 1]; ++k) cnt += d.ckey[d.nbr[k]] > kh ? 1u : 0u;
+    const uint32_t cnt = d.nbr_cur[h];  // higher-priority neighbours
     d.pend[h] = cnt;
     ready = cnt == 0u;
   }
@@ -1220,12 +1226,14 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_
     const uint32_t i = base + threadIdx.x / CL;
     const bool act = i < nin;
     const uint32_t h = act ? in_list[i] : 0u;
-    const unsigned long long kh = act ? d.ckey[h] : 0ull;
+    // neighbours [k0, ku) have higher priority (all coloured by now),
+    // [ku, k1) lower
     const uint32_t k0 = act ? d.nbr_off[h] : 0u, k1 = act ? d.nbr_off[h + 1] : 0u;
+    const uint32_t ku = k0 + (act ? d.nbr_cur[h] : 0u);
     int newc = -1;
     for (int cb = 0; act && newc < 0; cb += 64) {
       unsigned long long used = 0;
-      for (uint32_t k = k0 + s; k < k1; k += CL) {
+      for (uint32_t k = k0 + s; k < ku; k += CL) {
         int co = d.b.colour[d.nbr[k]];
         if (co >= cb && co < cb + 64) used |= 1ull << (co - cb);
       }
@@ -1233,8 +1241,9 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_
       if (~used != 0ull) newc = cb + __builtin_ctzll(~used);
     }
     if (act && s == 0) d.b.colour[h] = newc;
-    // release: wavefront-uniform loop over the largest neighbour count
-    uint32_t deg = k1 - k0, maxdeg = deg;
+    // release the lower-priority neighbours: wavefront-uniform loop over the
+    // largest count
+    uint32_t deg = k1 - ku, maxdeg = deg;
     for (int m = 32; m > 0; m >>= 1) {
       uint32_t o2 = __shfl_xor(maxdeg, m);
       maxdeg = o2 > maxdeg ? o2 : maxdeg;
@@ -1244,8 +1253,8 @@ __global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_
       bool rel = false;
       uint32_t o = 0;
       if (kk < deg) {
-        o = d.nbr[k0 + kk];
-        rel = d.ckey[o] < kh && atomicSub(&d.pend[o], 1u) == 1u;
+        o = d.nbr[ku + kk];
+        rel = atomicSub(&d.pend[o], 1u) == 1u;
       }
       wave_push(out_list, out_n, rel, o);
     }

@@ -36,9 +36,20 @@ __device__ __forceinline__ uint32_t block_scan_excl(uint32_t t, uint32_t* s, uin
   return r;
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n, uint32_t ntiles,
-                                                            uint32_t* tile, uint32_t* done) {
+// Up to SCAN_JOBS arrays of the same length scanned by one launch pair
+// (blockIdx.y selects the array); each has its own tile offsets and ticket.
+#define SCAN_JOBS 3
+struct ScanJobs {
+  const uint32_t* in[SCAN_JOBS];
+  uint32_t* out[SCAN_JOBS];
+};
+
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint32_t n, uint32_t ntiles,
+                                                            uint32_t* tiles, uint32_t* dones) {
   __shared__ uint32_t s[SCAN_BLOCK];
+  const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
+  uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
+  uint32_t* done = dones + blockIdx.y;
   __shared__ bool last;
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_TILE + tid * SCAN_PER_THREAD;
@@ -79,9 +90,12 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(const uint32_t* __re
   }
 }
 
-__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                           uint32_t n, uint32_t ntiles, const uint32_t* tile) {
+__global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32_t n, uint32_t ntiles,
+                                                           const uint32_t* tiles) {
   __shared__ uint32_t s[SCAN_BLOCK];
+  const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
+  uint32_t* __restrict__ out = jobs.out[blockIdx.y];
+  const uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_TILE + tid * SCAN_PER_THREAD;
   uint32_t v[SCAN_PER_THREAD];
@@ -103,17 +117,18 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(const uint32_t* __res
 
 // Device state of the two-launch scan (one per stream).
 struct TileScan {
-  uint32_t* tile = nullptr;  // SCAN_MAX_TILES + 1 tile sums / offsets
-  uint32_t* done = nullptr;  // zero-initialised ticket
+  uint32_t* tile = nullptr;  // SCAN_JOBS x (SCAN_MAX_TILES + 1) tile sums / offsets
+  uint32_t* done = nullptr;  // SCAN_JOBS zero-initialised tickets
 };
 
-// out must hold n + 1 entries; n <= SCAN_MAX_TILES * SCAN_TILE.
-static inline void scan_tiles(TileScan& ts, const uint32_t* in, uint32_t* out, uint32_t n, hipStream_t st) {
+// Scans the first `jobs` arrays of js, n entries each (outputs hold n + 1);
+// n <= SCAN_MAX_TILES * SCAN_TILE.
+static inline void scan_tiles(TileScan& ts, const ScanJobs& js, int jobs, uint32_t n, hipStream_t st) {
   if (n == 0) {
-    k_scan_zero<<<1, 1, 0, st>>>(out);
+    for (int j = 0; j < jobs; ++j) k_scan_zero<<<1, 1, 0, st>>>(js.out[j]);
     return;
   }
   uint32_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-  k_scan_reduce<<<nt, SCAN_BLOCK, 0, st>>>(in, n, nt, ts.tile, ts.done);
-  k_scan_apply<<<nt, SCAN_BLOCK, 0, st>>>(in, out, n, nt, ts.tile);
+  k_scan_reduce<<<dim3(nt, jobs), SCAN_BLOCK, 0, st>>>(js, n, nt, ts.tile, ts.done);
+  k_scan_apply<<<dim3(nt, jobs), SCAN_BLOCK, 0, st>>>(js, n, nt, ts.tile);
 }

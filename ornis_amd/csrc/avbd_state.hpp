@@ -198,7 +198,8 @@ struct Dev {
   unsigned long long* ckey;  // colouring priority: hash << 32 | ~scene-local index
   uint32_t* nbr_cnt;   // n+1 colouring neighbours (swept partners via points or joints)
   uint32_t* nbr_off;   // n+1
-  uint32_t* nbr_cur;
+  uint32_t* nbr_cur;   // fill cursor of the higher-priority neighbours (their count after the fill)
+  uint32_t* nbr_dn;    // fill cursor of the lower-priority neighbours (from the back)
   uint32_t* nbr;
   uint32_t* remain;    // cstat: frontier size of round r at r % COLOUR_RING
   uint32_t* fcnt;      // cstat + CSTAT_FRONT: frontier counters (ring of three)

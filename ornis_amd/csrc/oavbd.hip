@@ -133,7 +133,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
   dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4); dalloc(&d.inb, (size_t)n * 28);
   dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
-  dalloc(&d.nbr_cur, n + 1);
+  dalloc(&d.nbr_cur, n + 1); dalloc(&d.nbr_dn, n + 1);
   d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
   dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); dalloc(&d.pend, n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536); dalloc(&d.ccls, 2 * CCLS);
@@ -156,9 +156,9 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   d.g.lflat = 0;
   d.lflag = nullptr;
   d.lrank = nullptr;
-  dalloc(&c->ts.tile, SCAN_MAX_TILES + 1);
-  dalloc(&c->ts.done, 1);
-  HIPC(hipMemset(c->ts.done, 0, sizeof(uint32_t)));
+  dalloc(&c->ts.tile, (size_t)SCAN_JOBS * (SCAN_MAX_TILES + 1));
+  dalloc(&c->ts.done, SCAN_JOBS);
+  HIPC(hipMemset(c->ts.done, 0, SCAN_JOBS * sizeof(uint32_t)));
   d.blist = nullptr;
   d.inc = nullptr;
   d.acc_off = nullptr;
@@ -249,7 +249,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
-  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
+  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr_dn); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
 }
 
 // ---------------------------------------------------------------------------
@@ -271,7 +271,17 @@ static void zero_u32(oavbd_ctx* c, uint32_t* p, uint32_t n) {
 // Exclusive scan on the context stream (two launches, see avbd_scan.hpp).
 static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
   if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
-  scan_tiles(c->ts, in, out, n, c->st);
+  ScanJobs js{};
+  js.in[0] = in;
+  js.out[0] = out;
+  scan_tiles(c->ts, js, 1, n, c->st);
+}
+// Three arrays of the same length in one launch pair.
+static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
+                  const uint32_t* in2, uint32_t* out2, uint32_t n) {
+  if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
+  ScanJobs js{{in0, in1, in2}, {out0, out1, out2}};
+  scan_tiles(c->ts, js, 3, n, c->st);
 }
 
 static void read_counters(oavbd_ctx* c) {
@@ -434,9 +444,7 @@ static void incidence_and_colour(oavbd_ctx* c) {
   k_colour_init<<<blocks(std::max<uint32_t>(n + 1, CSTAT_HEAD + 65536)), TPB, 0, c->st>>>(d);
   if (d.npairs) k_inc_count<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_count<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-  scan(c, d.ptcnt, d.inc_off, n);
-  scan(c, d.bcnt, d.boff, n);
-  scan(c, d.nbr_cnt, d.nbr_off, n);
+  scan3(c, d.ptcnt, d.inc_off, d.bcnt, d.boff, d.nbr_cnt, d.nbr_off, n);
   if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
