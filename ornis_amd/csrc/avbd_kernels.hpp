@@ -45,29 +45,6 @@ __device__ __forceinline__ void wave_add(uint32_t* ctr, uint32_t v) {
   if (__lane_id() == 0 && v) atomicAdd(ctr, v);
 }
 
-// Active lanes holding the same key k >= 0 reserve consecutive slots of
-// cnt[k] with one atomic per distinct key; returns the lane's slot.  If mx
-// is set it receives max(k + 1).
-__device__ __forceinline__ uint32_t wave_append(uint32_t* cnt, int k, uint32_t* mx) {
-  uint32_t slot = 0;
-  const uint32_t lane = __lane_id();
-  unsigned long long pending = __ballot(k >= 0);
-  while (pending) {
-    int leader = __ffsll(pending) - 1;
-    int kl = __shfl(k, leader);
-    unsigned long long grp = __ballot(k == kl);
-    uint32_t base = 0;
-    if (lane == (uint32_t)leader) {
-      base = atomicAdd(&cnt[kl], (uint32_t)__popcll(grp));
-      if (mx) atomicMax(mx, (uint32_t)(kl + 1));
-    }
-    base = __shfl(base, leader);
-    if (k == kl) slot = base + (uint32_t)__popcll(grp & ((1ull << lane) - 1ull));
-    pending &= ~grp;
-  }
-  return slot;
-}
-
 // ---------------------------------------------------------------------------
 // body helpers
 // ---------------------------------------------------------------------------
@@ -1048,14 +1025,16 @@ __global__ void __launch_bounds__(TPB) k_inc_count(Dev d) {
     if (sb) {
       atomicAdd(&d.ptcnt[ab.y], c);
       atomicAdd(&d.bcnt[ab.y], 1u);
-      if (c && sa) atomicAdd(&d.nbr_cnt[ab.y], 1u);
+      if (c && sa && d.ckey[ab.x] > d.ckey[ab.y]) atomicAdd(&d.nbr_cnt[ab.y], 1u);  // a is above b
     }
     if (sa && p == d.poff[ab.x]) {
       uint32_t pts = 0, nb = 0;
+      const unsigned long long ka = d.ckey[ab.x];
       for (uint32_t q = p; q < d.poff[ab.x + 1]; ++q) {
         uint32_t cq = d.P.pts[q].y;
+        const uint32_t bq = d.P.ab[q].y;
         pts += cq;
-        nb += (cq && d.sw[d.P.ab[q].y]) ? 1u : 0u;
+        nb += (cq && d.sw[bq] && d.ckey[bq] > ka) ? 1u : 0u;  // b_q above a
       }
       atomicAdd(&d.ptcnt[ab.x], pts);
       if (nb) atomicAdd(&d.nbr_cnt[ab.x], nb);
@@ -1070,26 +1049,17 @@ __global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
   if (j < d.njoints) {
     uint2 ab = d.J.ab[j];
     bool sa = d.sw[ab.x], sb = d.sw[ab.y];
-    if (sa && sb) {
-      atomicAdd(&d.nbr_cnt[ab.x], 1u);
-      atomicAdd(&d.nbr_cnt[ab.y], 1u);
-    }
+    if (sa && sb) atomicAdd(&d.nbr_cnt[d.ckey[ab.y] > d.ckey[ab.x] ? ab.x : ab.y], 1u);
     sides = (uint32_t)sa + (uint32_t)sb;
   }
   block_add(&d.counters[CTR_JSIDES], sides);
 }
-// Colouring neighbours of a swept pair or joint (a, b): each body lists its
-// higher-priority neighbours from the front of its range (their number ends
-// in nbr_cur) and its lower-priority ones from the back (nbr_dn).
+// Colouring neighbours of a swept pair or joint (a, b): the body of lower
+// priority lists the other (each body lists its higher-priority neighbours).
 __device__ __forceinline__ void nbr_link(const Dev& d, uint2 ab) {
   const bool b_up = d.ckey[ab.y] > d.ckey[ab.x];
-  if (b_up) {
-    d.nbr[d.nbr_off[ab.x] + atomicAdd(&d.nbr_cur[ab.x], 1u)] = ab.y;
-    d.nbr[d.nbr_off[ab.y + 1] - 1u - atomicAdd(&d.nbr_dn[ab.y], 1u)] = ab.x;
-  } else {
-    d.nbr[d.nbr_off[ab.x + 1] - 1u - atomicAdd(&d.nbr_dn[ab.x], 1u)] = ab.y;
-    d.nbr[d.nbr_off[ab.y] + atomicAdd(&d.nbr_cur[ab.y], 1u)] = ab.x;
-  }
+  const uint32_t lo = b_up ? ab.x : ab.y, hi = b_up ? ab.y : ab.x;
+  d.nbr[d.nbr_off[lo] + atomicAdd(&d.nbr_cur[lo], 1u)] = hi;
 }
 __global__ void k_inc_fill(Dev d) {
   uint32_t p = gid();
@@ -1135,16 +1105,15 @@ __global__ void k_inc_build(Dev d) {
   }
 }
 
-// Jones-Plassmann colouring, frontier form.  A swept body is coloured in the
-// round after all its higher-priority neighbours are, with the smallest
-// colour none of them holds -- exactly the sequential greedy colouring in
-// priority order (larger hash first, then smaller scene-local index; the
-// key packs both, so a larger key is a higher priority; bodies of different
-// scenes are never neighbours, so keys of neighbours differ).  pend[h]
-// counts h's uncoloured higher-priority neighbours; a body whose count
-// reaches zero joins the next round's frontier.  Neighbours coloured in the
-// same round are never adjacent, so reads of neighbour colours see only
-// earlier rounds.
+// Jones-Plassmann colouring, pull form.  A swept body is coloured once all
+// its higher-priority neighbours are, with the smallest colour none of them
+// holds -- exactly the sequential greedy colouring in priority order (larger
+// hash first, then smaller scene-local index; the key packs both, so a
+// larger key is a higher priority; bodies of different scenes are never
+// neighbours, so keys of neighbours differ).  Each body lists only its
+// higher-priority neighbours; passes over the pending bodies repeat until
+// none is left (no counters are decremented, so a pass costs no atomics per
+// neighbour).
 __device__ __forceinline__ bool swept(const Dev& d, uint32_t h) {
   return d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f && !d.b.asleep[h];
 }
@@ -1161,7 +1130,6 @@ __global__ void k_colour_init(Dev d) {
   d.bcur[h] = 0u;
   d.nbr_cnt[h] = 0u;
   d.nbr_cur[h] = 0u;
-  d.nbr_dn[h] = 0u;
   if (h == d.n) return;
   d.sw[h] = swept(d, h) ? 1 : 0;
   d.ckey[h] = ((unsigned long long)d.b.prio[h] << 32) | (0xffffffffu - d.b.local[h]);
@@ -1194,85 +1162,133 @@ __device__ __forceinline__ void wave_push(uint32_t* list, uint32_t* n, bool pred
   if (pred) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
 }
 
-// Pending counts; bodies with no higher-priority neighbour form round 0.
-__global__ void __launch_bounds__(TPB) k_colour_count(Dev d, uint32_t* out_list, uint32_t* out_n) {
-  uint32_t h = gid();
-  if (h == 0) d.counters[CTR_PTSIDES] = d.inc_off[d.n];  // point-sides of swept bodies
-  bool ready = false;
-  if (h < d.n && d.sw[h]) {
-    const uint32_t cnt = d.nbr_cur[h];  // higher-priority neighbours
-    d.pend[h] = cnt;
-    ready = cnt == 0u;
-  }
-  wave_push(out_list, out_n, ready, h);
-}
-
-// One round: colour the frontier, release lower-priority neighbours into
-// the next frontier.  CL lanes share one frontier body, each taking every
-// CL-th neighbour.  *size_out receives the frontier size (0: done); *clear
-// is zeroed for the round after next.
+// Colour of body h, by the CL lanes of its group: if all of h's
+// higher-priority neighbours have colours, h takes the smallest colour none
+// of them uses and this returns false; otherwise h stays uncoloured and
+// this returns true.  A neighbour coloured concurrently may be seen or not;
+// either way a body is coloured only from the final colours of all its
+// higher-priority neighbours, so the colouring is the sequential greedy one
+// in priority order.  All lanes of the group call this together.
 #define CL 8
-__global__ void __launch_bounds__(TPB) k_colour_round(Dev d, const uint32_t* in_list, const uint32_t* in_n,
-                                                      uint32_t* out_list, uint32_t* out_n, uint32_t* clear,
-                                                      uint32_t* size_out) {
-  const uint32_t nin = *in_n;
-  if (gid() == 0) {
-    *clear = 0u;
-    *size_out = nin;
-  }
-  constexpr uint32_t PER_BLOCK = TPB / CL;
+__device__ __forceinline__ bool colour_body(const Dev& d, uint32_t h, bool act) {
+  constexpr int CU = 4;  // colour reads in flight per lane
   const uint32_t s = threadIdx.x % CL;
-  for (uint32_t base = blockIdx.x * PER_BLOCK; base < nin; base += gridDim.x * PER_BLOCK) {
-    const uint32_t i = base + threadIdx.x / CL;
-    const bool act = i < nin;
-    const uint32_t h = act ? in_list[i] : 0u;
-    // neighbours [k0, ku) have higher priority (all coloured by now),
-    // [ku, k1) lower
-    const uint32_t k0 = act ? d.nbr_off[h] : 0u, k1 = act ? d.nbr_off[h + 1] : 0u;
-    const uint32_t ku = k0 + (act ? d.nbr_cur[h] : 0u);
-    int newc = -1;
-    for (int cb = 0; act && newc < 0; cb += 64) {
-      unsigned long long used = 0;
-      for (uint32_t k = k0 + s; k < ku; k += CL) {
-        int co = d.b.colour[d.nbr[k]];
-        if (co >= cb && co < cb + 64) used |= 1ull << (co - cb);
+  const uint32_t k0 = act ? d.nbr_off[h] : 0u, k1 = k0 + (act ? d.nbr_cur[h] : 0u);
+  int newc = -1;
+  bool waiting = false;
+  for (int cb = 0; act && newc < 0 && !waiting; cb += 64) {
+    unsigned long long used = 0;
+    int wait = 0;
+    for (uint32_t k = k0 + s; k < k1; k += CU * CL) {
+      int co[CU];
+#pragma unroll
+      for (int u = 0; u < CU; ++u) co[u] = k + u * CL < k1 ? d.b.colour[d.nbr[k + u * CL]] : 0x7fffffff;
+#pragma unroll
+      for (int u = 0; u < CU; ++u) {
+        wait |= co[u] < 0 ? 1 : 0;
+        if (co[u] >= cb && co[u] < cb + 64) used |= 1ull << (co[u] - cb);
       }
-      for (int m = CL / 2; m > 0; m >>= 1) used |= __shfl_xor(used, m, CL);
-      if (~used != 0ull) newc = cb + __builtin_ctzll(~used);
     }
-    if (act && s == 0) d.b.colour[h] = newc;
-    // release the lower-priority neighbours: wavefront-uniform loop over the
-    // largest count
-    uint32_t deg = k1 - ku, maxdeg = deg;
-    for (int m = 32; m > 0; m >>= 1) {
-      uint32_t o2 = __shfl_xor(maxdeg, m);
-      maxdeg = o2 > maxdeg ? o2 : maxdeg;
+    for (int m = CL / 2; m > 0; m >>= 1) {
+      used |= __shfl_xor(used, m, CL);
+      wait |= __shfl_xor(wait, m, CL);
     }
-    for (uint32_t kb = 0; kb < maxdeg; kb += CL) {
-      const uint32_t kk = kb + s;
-      bool rel = false;
-      uint32_t o = 0;
-      if (kk < deg) {
-        o = d.nbr[ku + kk];
-        rel = atomicSub(&d.pend[o], 1u) == 1u;
-      }
-      wave_push(out_list, out_n, rel, o);
+    waiting = wait != 0;
+    if (!waiting && ~used != 0ull) newc = cb + __builtin_ctzll(~used);
+  }
+  if (act && s == 0 && !waiting) d.b.colour[h] = newc;
+  return act && waiting;
+}
+
+// One pass over all bodies, CL lanes each: swept bodies still uncoloured
+// try to take their colour.  *deferred receives the number left
+// uncoloured (one atomic per workgroup).
+__global__ void __launch_bounds__(COLOUR_PASS_TPB) k_colour_pass(Dev d, uint32_t* deferred) {
+  __shared__ uint32_t s_def;
+  if (threadIdx.x == 0) s_def = 0u;
+  __syncthreads();
+  const uint32_t h = gid() / CL;
+  const bool act = h < d.n && d.sw[h] && d.b.colour[h] < 0;
+  const bool waiting = colour_body(d, h, act);
+  if (waiting && threadIdx.x % CL == 0) atomicAdd(&s_def, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_def) atomicAdd(deferred, s_def);
+}
+
+// The swept bodies still uncoloured, listed into `list` / *list_n (one
+// atomic per workgroup).
+__global__ void __launch_bounds__(TPB) k_colour_collect(Dev d, uint32_t* list, uint32_t* list_n) {
+  __shared__ uint32_t s_n, s_base;
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  const uint32_t h = gid();
+  const bool pend = h < d.n && d.sw[h] && d.b.colour[h] < 0;
+  const uint32_t slot = pend ? atomicAdd(&s_n, 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) s_base = atomicAdd(list_n, s_n);
+  __syncthreads();
+  if (pend) list[s_base + slot] = h;
+}
+
+// The remaining passes, from round0 on, in one workgroup over the listed
+// bodies: passes are separated by workgroup barriers instead of launches
+// (few bodies are still uncoloured by then).  Writes the number of passes to
+// cstat[CSTAT_ROUNDS].  The loop ends when no body is left; every pass
+// colours at least the listed body of highest priority, so n + 1 passes
+// always suffice.  Lists alternate between two buffers, their counters
+// rotate over three (pass r reads r % 3, fills (r + 1) % 3, clears
+// (r + 2) % 3).
+__global__ void __launch_bounds__(COLOUR_TAIL_TPB) k_colour_tail(Dev d, uint32_t round0) {
+  constexpr uint32_t PER = COLOUR_TAIL_TPB / CL;
+  const size_t n = d.n;
+  for (uint32_t round = round0; round <= round0 + (uint32_t)n; ++round) {
+    const uint32_t* in_list = d.front + (size_t)(round & 1u) * n;
+    uint32_t* out_list = d.front + (size_t)((round + 1u) & 1u) * n;
+    uint32_t* out_n = d.fcnt + (round + 1u) % 3u;
+    // counters are updated by device atomics: read them past this CU's cache
+    const uint32_t nin = __hip_atomic_load(d.fcnt + round % 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(d.fcnt + (round + 2u) % 3u, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (nin == 0u) d.cstat[CSTAT_ROUNDS] = round;
     }
-    (void)wave_append(d.colour_cnt, s == 0 ? newc : -1, d.cmax);
+    if (nin == 0u) break;
+    for (uint32_t base = 0; base < nin; base += PER) {
+      const uint32_t i = base + threadIdx.x / CL;
+      const bool act = i < nin;
+      const uint32_t h = act ? in_list[i] : 0u;
+      const bool waiting = colour_body(d, h, act);
+      wave_push(out_list, out_n, waiting && threadIdx.x % CL == 0, h);
+    }
+    __threadfence();
+    __syncthreads();
   }
 }
 
-// Bodies per (colour, size class), from per-block LDS histograms.
+// Bodies per colour and per (colour, size class), and the number of colours,
+// from per-block LDS histograms (colours from CCLS_COLOURS on, rare, take
+// one atomic per body).
 __global__ void __launch_bounds__(TPB) k_colour_classes(Dev d) {
   __shared__ uint32_t cnt[CCLS];
+  __shared__ uint32_t s_max;
   uint32_t h = gid();
+  if (h == 0) d.counters[CTR_PTSIDES] = d.inc_off[d.n];  // point-sides of swept bodies
   for (uint32_t i = threadIdx.x; i < CCLS; i += TPB) cnt[i] = 0u;
+  if (threadIdx.x == 0) s_max = 0u;
   __syncthreads();
   const int c = h < d.n ? d.b.colour[h] : -1;
   if (c >= 0 && c < CCLS_COLOURS) atomicAdd(&cnt[(uint32_t)c * 8u + item_class(body_items(d, h))], 1u);
+  if (c >= CCLS_COLOURS) atomicAdd(&d.colour_cnt[c], 1u);
+  if (c >= 0) atomicMax(&s_max, (uint32_t)c + 1u);
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < CCLS; i += TPB)
     if (cnt[i]) atomicAdd(&d.ccls[i], cnt[i]);
+  if (threadIdx.x < CCLS_COLOURS) {
+    uint32_t t = 0;
+    for (int k = 0; k < 8; ++k) t += cnt[threadIdx.x * 8 + k];
+    if (t) atomicAdd(&d.colour_cnt[threadIdx.x], t);
+  }
+  if (threadIdx.x == 0 && s_max) atomicMax(d.cmax, s_max);
 }
 
 // Colour lists: each colour below CCLS_COLOURS is laid out by size class

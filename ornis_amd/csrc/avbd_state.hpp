@@ -41,10 +41,13 @@
 #define K_CCD_GATE 0.5f
 #define K_JOINT_REST 0.005f
 #define K_MAX_POINTS 8
-#define COLOUR_BATCH 8  // colouring rounds launched per host check after the first
 #define COLOUR_RING 64   // frontier sizes kept per round number modulo this (a batch is shorter)
 #define CSTAT_FRONT 96   // words of the three frontier counters (own cache line)
 #define CSTAT_MAX 128    // word of the colour count (own cache line)
+#define CSTAT_ROUNDS 129 // word of the number of colouring rounds
+#define COLOUR_TAIL_TPB 1024  // threads of the one-workgroup colouring tail
+#define COLOUR_PASS_TPB 1024  // threads per workgroup of a colouring pass (one count atomic each)
+#define COLOUR_TAIL_FRONTIER 512  // frontier size from which the tail takes over
 #define CSTAT_HEAD 160   // words before the colour sizes in cstat
 #define CSTAT_EAGER 64   // colour sizes fetched with every colouring check
 #define CCLS_COLOURS 64  // colours whose lists are ordered by size class
@@ -196,15 +199,13 @@ struct Dev {
   uint32_t* ptcnt;     // per body point-side counts
   uint8_t* sw;         // body is swept this step (dynamic, finite mass, awake)
   unsigned long long* ckey;  // colouring priority: hash << 32 | ~scene-local index
-  uint32_t* nbr_cnt;   // n+1 colouring neighbours (swept partners via points or joints)
+  uint32_t* nbr_cnt;   // n+1 higher-priority colouring neighbours (swept partners via points or joints)
   uint32_t* nbr_off;   // n+1
-  uint32_t* nbr_cur;   // fill cursor of the higher-priority neighbours (their count after the fill)
-  uint32_t* nbr_dn;    // fill cursor of the lower-priority neighbours (from the back)
+  uint32_t* nbr_cur;   // fill cursor (the number of higher-priority neighbours after the fill)
   uint32_t* nbr;
   uint32_t* remain;    // cstat: frontier size of round r at r % COLOUR_RING
   uint32_t* fcnt;      // cstat + CSTAT_FRONT: frontier counters (ring of three)
   uint32_t* front;     // 2 x n frontier lists (ring of two)
-  uint32_t* pend;      // per body: uncoloured higher-priority neighbours
   uint32_t* lflag;     // per large body, one flag per body of its scene (lflat in all)
   uint32_t* lrank;     // exclusive scan of lflag (lflat + 1)
   uint32_t* cstat;     // colouring status: frontier sizes by round, frontier counters, colour count, sizes

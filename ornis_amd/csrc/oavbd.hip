@@ -82,7 +82,7 @@ struct oavbd_ctx {
   uint32_t sleeping = 0;
   bool may_sleep = false;  // bodies may be asleep (wake passes needed)
   uint32_t solved = 0;  // bodies coloured (swept) this step
-  int last_rounds = 0;  // colouring rounds of the previous step
+  int grid_rounds = 8;  // colouring rounds launched over the grid before the one-workgroup tail
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   int lanes = 32;   // primal launch shape: lanes per body
@@ -133,10 +133,10 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
   dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4); dalloc(&d.inb, (size_t)n * 28);
   dalloc(&d.sw, n); dalloc(&d.ckey, n); dalloc(&d.nbr_cnt, n + 1); dalloc(&d.nbr_off, n + 1);
-  dalloc(&d.nbr_cur, n + 1); dalloc(&d.nbr_dn, n + 1);
+  dalloc(&d.nbr_cur, n + 1);
   d.nbr = nullptr;
   dalloc(&c->cand_cnt, n + 1);
-  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); dalloc(&d.pend, n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536); dalloc(&d.ccls, 2 * CCLS);
+  dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536); dalloc(&d.ccls, 2 * CCLS);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
@@ -247,9 +247,9 @@ static void free_all(oavbd_ctx* c) {
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
-  dfree(d.cstat); dfree(d.front); dfree(d.pend); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
+  dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
-  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr_dn); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
+  dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,34 +448,29 @@ static void incidence_and_colour(oavbd_ctx* c) {
   if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
-  // Jones-Plassmann colouring, COLOUR_BATCH rounds per host check (rounds
-  // after the last frontier find nothing to do).  Each check copies the
-  // frontier sizes, the colour count and the first colour sizes in one go.
-  // Frontier lists alternate between two buffers, their counters rotate over
-  // three (round r reads counter r % 3, fills (r + 1) % 3, clears (r + 2) % 3).
-  k_colour_count<<<blocks(n), TPB, 0, c->st>>>(d, d.front, d.fcnt);
+  // Jones-Plassmann colouring in passes.  The first `grid` passes run over
+  // all bodies, one launch each; the bodies still uncoloured after them are
+  // listed and finished by one single-workgroup launch.  `grid` is the pass
+  // after which at most COLOUR_TAIL_FRONTIER bodies were left in the previous
+  // step (the counts change slowly).  One check then copies the per-pass
+  // counts, the pass and colour counts and the first colour sizes.
   uint32_t head[CSTAT_HEAD + CSTAT_EAGER];
-  int rounds = -1;
-  // the first batch covers the previous step's round count (it changes
-  // slowly), so one check usually suffices
-  int batch = std::min(std::max(COLOUR_BATCH, c->last_rounds + 2), COLOUR_RING - 16);
-  for (int round0 = 0; rounds < 0; round0 += batch, batch = COLOUR_BATCH) {
-    if (round0 > 30000) throw HipError{"colouring did not converge"};
-    for (int r = 0; r < batch; ++r) {
-      const int round = round0 + r;
-      k_colour_round<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(round & 1) * n, d.fcnt + round % 3,
-                                                   d.front + (size_t)((round + 1) & 1) * n, d.fcnt + (round + 1) % 3,
-                                                   d.fcnt + (round + 2) % 3, d.remain + round % COLOUR_RING);
-    }
-    HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
-    HIPC(hipStreamSynchronize(c->st));
-    for (int r = 0; r < batch; ++r)
-      if (head[(round0 + r) % COLOUR_RING] == 0) {
-        rounds = round0 + r;
-        break;
-      }
+  const int grid = c->grid_rounds;
+  for (int round = 0; round < grid; ++round)
+    k_colour_pass<<<blocks((size_t)n * CL, COLOUR_PASS_TPB), COLOUR_PASS_TPB, 0, c->st>>>(d, d.remain + round);
+  k_colour_collect<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(grid & 1) * n, d.fcnt + grid % 3);
+  k_colour_tail<<<1, COLOUR_TAIL_TPB, 0, c->st>>>(d, (uint32_t)grid);
+  k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
+  HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+  int rounds = -1, next_grid = -1;
+  for (int r = 0; r < grid; ++r) {
+    if (rounds < 0 && head[r] == 0) rounds = r + 1;
+    if (next_grid < 0 && head[r] <= COLOUR_TAIL_FRONTIER) next_grid = r + 1;
   }
-  c->last_rounds = rounds;
+  if (rounds < 0) rounds = (int)head[CSTAT_ROUNDS];
+  // no grid pass got down to few uncoloured bodies: start the tail later
+  c->grid_rounds = std::max(1, std::min(next_grid >= 0 ? next_grid : grid + 2, COLOUR_RING - 16));
   c->stats.colour_rounds = (uint32_t)rounds;
   uint32_t ncol = head[CSTAT_MAX];
   c->colour_cnt_h.assign(ncol, 0);
@@ -489,7 +484,6 @@ static void incidence_and_colour(oavbd_ctx* c) {
     HIPC(hipMemcpyAsync(d.colour_off, c->colour_off_h.data(), (ncol + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                         c->st));
   }
-  k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   c->stats.colours = ncol;
   c->stats.max_colour = 0;
