@@ -29,20 +29,19 @@
 
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
-// Wavefront-aggregated atomics: a counter hit by every thread of a large
-// launch is updated once per wavefront instead of once per thread.
-
-// Adds the number of active lanes whose `pred` holds.
-__device__ __forceinline__ void wave_count(uint32_t* ctr, bool pred) {
-  unsigned long long m = __ballot(pred);
-  if (pred && __lane_id() == (unsigned)(__ffsll((unsigned long long)m) - 1)) atomicAdd(ctr, (uint32_t)__popcll(m));
-}
-
-// Adds the wavefront's sum of v.  Every lane of the wavefront must be
-// active here (the calling kernel has no early return before it).
-__device__ __forceinline__ void wave_add(uint32_t* ctr, uint32_t v) {
+// Adds the block's sum of v to *ctr with one atomic (every thread of the
+// TPB-thread block must call it).  Counters hit by a whole launch use this:
+// atomics on one address from every wavefront serialise.
+__device__ __forceinline__ void block_add(uint32_t* ctr, uint32_t v) {
+  __shared__ uint32_t part[TPB / 64];
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if (__lane_id() == 0 && v) atomicAdd(ctr, v);
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < TPB / 64; ++w) t += part[w];
+    if (t) atomicAdd(ctr, t);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -98,9 +97,7 @@ __device__ __forceinline__ bool jointed_dev(const Dev& d, uint32_t a, uint32_t b
 // ---------------------------------------------------------------------------
 // phase 1: torque and TOI mover flags (mod.rs:855-871, :882-895)
 // ---------------------------------------------------------------------------
-__global__ void k_begin(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ uint8_t begin_body(const Dev& d, uint32_t h) {
   float4 p = d.b.pos[h];
   bool solvable = d.b.type[h] == BT_DYNAMIC && p.w > 0.0f;
   uint8_t fast = 0;
@@ -124,7 +121,13 @@ __global__ void k_begin(Dev d) {
     }
   }
   d.fast[h] = fast;
-  wave_count(&d.counters[CTR_FAST], fast != 0);
+  return fast;
+}
+__global__ void __launch_bounds__(TPB) k_begin(Dev d) {
+  uint32_t h = gid();
+  uint8_t fast = 0;
+  if (h < d.n) fast = begin_body(d, h);
+  block_add(&d.counters[CTR_FAST], fast);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,11 +419,9 @@ __global__ void __launch_bounds__(TPB) k_toi_win(Dev d, uint32_t npairs) {
   d.toi_tgt[ho.x] = ho.y;
 }
 
-__global__ void k_toi_apply(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ bool toi_apply_body(const Dev& d, uint32_t h) {
   float4 hit = d.toi_hit[h];
-  if (!(hit.w >= 0.0f) || !d.fast[h]) return;
+  if (!(hit.w >= 0.0f) || !d.fast[h]) return false;
   uint32_t o = d.toi_tgt[h];
   f3 nrm = xyz(hit);
   float4 v4 = d.b.vel[h];
@@ -438,7 +439,13 @@ __global__ void k_toi_apply(Dev d) {
   }
   d.b.timer[h] = 0.0f;
   d.b.flag_a[o] = 1;  // wake the target
-  wave_count(&d.counters[CTR_TOIHITS], true);
+  return true;
+}
+__global__ void __launch_bounds__(TPB) k_toi_apply(Dev d) {
+  uint32_t h = gid();
+  bool hit = false;
+  if (h < d.n) hit = toi_apply_body(d, h);
+  block_add(&d.counters[CTR_TOIHITS], hit ? 1u : 0u);
 }
 
 // Apply pending wakes (flag_a) and clear them.
@@ -999,19 +1006,6 @@ __global__ void k_pair_prep(Dev d) {
 // Incidence is built for swept bodies only (the primal solves nothing
 // else); the colouring graph links swept bodies sharing a pair with
 // points or a joint (the graph of colour_bodies).
-// Adds the block's sum of v to *ctr with one atomic (every thread of the
-// TPB-thread block must call it).
-__device__ __forceinline__ void block_add(uint32_t* ctr, uint32_t v) {
-  __shared__ uint32_t part[TPB / 64];
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < TPB / 64; ++w) t += part[w];
-    if (t) atomicAdd(ctr, t);
-  }
-}
 
 // Per pair: the b side's counts (scattered bodies, few atomics each); the
 // a side of a whole a-run is summed by the run's first pair.
@@ -2150,9 +2144,7 @@ __global__ void k_sleep_timer(Dev d) {
 
 // sleep_body (mod.rs:822-828) where frozen; refresh the driver baseline
 // (mod.rs:1139-1142); count sleepers.
-__global__ void k_sleep_apply(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ bool sleep_apply_body(const Dev& d, uint32_t h) {
   if (d.b.flag_a[h] && !d.b.asleep[h] && d.b.type[h] == BT_DYNAMIC) {
     float4 v = d.b.vel[h], w = d.b.angvel[h], in = d.b.inertia[h], p = d.b.pos[h];
     d.b.vel[h] = make_float4(0.0f, 0.0f, 0.0f, v.w);
@@ -2162,7 +2154,13 @@ __global__ void k_sleep_apply(Dev d) {
     d.b.asleep[h] = 1;
   }
   d.b.flag_a[h] = 0;
-  wave_count(&d.counters[CTR_SLEEPING], d.b.asleep[h] != 0);
   d.b.prev_pos[h] = d.b.pos[h];
   d.b.prev_rot[h] = d.b.rot[h];
+  return d.b.asleep[h] != 0;
+}
+__global__ void __launch_bounds__(TPB) k_sleep_apply(Dev d) {
+  uint32_t h = gid();
+  bool asleep = false;
+  if (h < d.n) asleep = sleep_apply_body(d, h);
+  block_add(&d.counters[CTR_SLEEPING], asleep ? 1u : 0u);
 }
