@@ -1734,8 +1734,8 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
   constexpr int NE = (27 + L - 1) / L;        // elements per lane
   constexpr int U = 8;                        // contact rows per LDS round trip
   static_assert((L * R) % U == 0, "the row buffer holds whole groups");
-  __shared__ Row rows[NB][L * R];
-  __shared__ uint32_t touch[NB][L * R];
+  constexpr int RS = 9;                       // LDS words per row: x[6], pen, f, 1.0
+  __shared__ float rows[NB][L * R * RS];
   __shared__ float sums[NB][28];
   __shared__ uint32_t chunks_s[NB];
   const int tid = threadIdx.x;
@@ -1754,10 +1754,12 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
     if (cr.x >> 31) nj = d.b.ji_off[h + 1] - j0;
   }
   const uint32_t total = np + nj;
-  // element q of this lane: matrix entry (ia, ib) or rhs entry ia
+  // element q of this lane: matrix entry (ia, ib) or rhs entry ia.  A
+  // contact row adds rw[isc] * (rw[ia] * rw[ibf]) to it: pen * (x_i * x_j)
+  // on matrix lanes, f * (x_k * 1) on rhs lanes (word 8 of a row is 1.0).
   float acc[NE];
   int ia[NE], ib[NE];
-  int ibf[NE];  // second factor of a contact row: x_j, or f (slot 7) for rhs lanes
+  int ibf[NE], ibo[NE], isc[NE];
   bool rhs[NE], skip_couple[NE];
   uint32_t ebit[NE];
 #pragma unroll
@@ -1771,6 +1773,8 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
     ia[k] = e < 21 ? i : (e < 27 ? e - 21 : 0);
     ib[k] = e < 21 ? j : 0;
     ibf[k] = e < 21 ? j : 7;
+    ibo[k] = e < 21 ? j : 8;
+    isc[k] = e < 21 ? 6 : 7;
     ebit[k] = e < 27 ? 1u << e : 0u;
     skip_couple[k] = e < 21 ? j < 3 : e - 21 < 3;
     acc[k] = 0.0f;
@@ -1786,7 +1790,7 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
     const int e = sub + k * L;
     acc[k] = (valid && e < 27) ? d.inb[28 * (size_t)h + e] : 0.0f;
   }
-  const float* rowf = reinterpret_cast<const float*>(&rows[grp][0]);
+  const float* rowf = &rows[grp][0];
   for (uint32_t c = 0; c < nch; ++c) {
     uint32_t k = c * L + sub;
     Row tmp[R];
@@ -1817,27 +1821,36 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (!((mask >> r) & 1u)) continue;
-      rows[grp][w] = tmp[r];
-      touch[grp][w] = tmp[r].pen > 0.0f ? TOUCH_STAMP : (tmp[r].pen < 0.0f ? TOUCH_COUPLE : TOUCH_DIAG);
+      float* o = &rows[grp][RS * w];
+#pragma unroll
+      for (int x = 0; x < 6; ++x) o[x] = tmp[r].x[x];
+      o[6] = tmp[r].pen;
+      o[7] = tmp[r].f;
+      o[8] = 1.0f;
       ++w;
     }
     __syncthreads();
     // contact rows: stamps (and angular couples when R == 6).  A matrix
-    // lane adds |pen| * (x_i * x_j); an rhs lane adds 1 * (x_k * f), which
-    // is f * x_k exactly (products commute, 1 * y = y).  Rows are read U at
-    // a time, so one LDS round trip serves U rows; slots of the last group
-    // past the body's contact rows (the buffer holds whole groups) add -0.0.
+    // lane adds pen * (x_i * x_j) (|pen| with couples, whose pen is
+    // negative); an rhs lane adds f * (x_k * 1), which is f * x_k exactly.
+    // Rows are read U at a time, so one LDS round trip serves U rows; slots
+    // of the last group past the body's contact rows (the buffer holds whole
+    // groups) add -0.0.
     auto add_group = [&](int t0, auto partial) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const float* rw = rowf + 8 * (t0 + u);
-        const float pen = rw[6];
+        const float* rw = rowf + RS * (t0 + u);
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
-          const float a = rw[ia[q]], b = rw[ibf[q]];
-          const float sc = rhs[q] ? 1.0f : fabsf(pen);
-          float v = sc * (a * b);
-          if constexpr (R >= 6) v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
+          float v;
+          if constexpr (R >= 6) {
+            const float pen = rw[6];
+            const float sc = rhs[q] ? 1.0f : fabsf(pen);
+            v = sc * (rw[ia[q]] * rw[ibf[q]]);
+            v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
+          } else {
+            v = rw[isc[q]] * (rw[ia[q]] * rw[ibo[q]]);
+          }
           if constexpr (decltype(partial)::value) v = t0 + u < nprow ? v : -0.0f;
           acc[q] += v;
         }
@@ -1848,9 +1861,9 @@ __global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __res
     if (nfull < nprow) add_group(nfull, std::true_type{});
     // joint rows: stamps and geometric-stiffness diagonals
     for (int t = nprow; t < nrow; ++t) {
-      const float* rw = rowf + 8 * t;
+      const float* rw = rowf + RS * t;
       const float pen = rw[6], f = rw[7];
-      const uint32_t tm = touch[grp][t];
+      const uint32_t tm = pen > 0.0f ? TOUCH_STAMP : (pen < 0.0f ? TOUCH_COUPLE : TOUCH_DIAG);
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
         const float a = rw[ia[q]], b = rw[ib[q]];

@@ -9,7 +9,7 @@
 #include <stdint.h>
 
 #define SCAN_BLOCK 256
-#define SCAN_PER_THREAD 4
+#define SCAN_PER_THREAD 16
 #define SCAN_TILE (SCAN_BLOCK * SCAN_PER_THREAD)
 
 __global__ void k_scan_zero(uint32_t* __restrict__ out) { out[0] = 0u; }
