@@ -133,7 +133,8 @@ HD m3c mat3_from_quat(qf q) {
 // (scratch) memory.
 HD float bitsel3(float a, float b, float c, int i) {
   const int m0 = -(int)(i == 0), m1 = -(int)(i == 1), m2 = -(int)(i == 2);
-  return __int_as_float((__float_as_int(a) & m0) | (__float_as_int(b) & m1) | (__float_as_int(c) & m2));
+  return __builtin_bit_cast(float, (__builtin_bit_cast(int, a) & m0) | (__builtin_bit_cast(int, b) & m1) |
+                                       (__builtin_bit_cast(int, c) & m2));
 }
 HD f3 mcol(const m3c& m, int i) {
   return f3{bitsel3(m.c0.x, m.c1.x, m.c2.x, i), bitsel3(m.c0.y, m.c1.y, m.c2.y, i),
