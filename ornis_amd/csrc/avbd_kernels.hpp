@@ -324,15 +324,69 @@ __device__ uint32_t toi_targets(const Dev& d, uint32_t h, Emit emit) {
 }
 
 // Counting pass; the first TOI_CACHE targets are kept for the fill pass.
-__global__ void k_toi_count(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
-  d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
-  d.toi_best[h] = ~0ull;
-  uint32_t* cache = d.toi_cache + (size_t)h * TOI_CACHE;
-  d.toi_cnt[h] = d.fast[h] ? toi_targets(d, h, [&](uint32_t i, uint32_t o) {
-    if (i < TOI_CACHE) cache[i] = o;
-  }) : 0u;
+// Resets every body's clamp and counts the targets of fast bodies,
+// BP_LANES lanes per body: lane s takes the swept cells s, s + BP_LANES, ...
+// (or every BP_LANES-th small body of the scene when the sweep spans more
+// than 64 cells) and every BP_LANES-th large body, the same targets as
+// toi_targets.  The first TOI_CACHE targets found, in any order (every cast
+// is independent), are kept for the fill pass.
+__global__ void __launch_bounds__(TPB) k_toi_count(Dev d) {
+  __shared__ uint32_t s_cnt[TPB / BP_LANES];
+  const uint32_t h = gid() / BP_LANES;
+  const uint32_t s = threadIdx.x % BP_LANES, lb = threadIdx.x / BP_LANES;
+  if (s == 0) s_cnt[lb] = 0u;
+  __syncthreads();
+  if (h < d.n && s == 0) {
+    d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
+    d.toi_best[h] = ~0ull;
+  }
+  bool moving = false;
+  if (h < d.n && d.fast[h]) {
+    const f3 delta = mul(xyz(d.b.vel[h]), K_DT);
+    const float len = len3(delta);
+    moving = !(len < K_NEAR_ZERO);
+    if (moving) {
+      uint32_t* cache = d.toi_cache + (size_t)h * TOI_CACHE;
+      const f3 dir = divs(delta, len);
+      const f3 p0 = xyz(d.b.pos[h]);
+      const float rm = d.b.shape[h].w;
+      auto consider = [&](uint32_t o, float4 po) {
+        if (isfinite(po.w) && !sweep_may_touch(p0, dir, len, rm, xyz(po), po.w)) return;
+        if (!toi_target_ok(d, h, o)) return;
+        const uint32_t slot = atomicAdd(&s_cnt[lb], 1u);
+        if (slot < TOI_CACHE) cache[slot] = o;
+      };
+      const uint32_t sch = d.b.scene[h];
+      const float r = rm + d.g.cell * 0.5f + 0.01f;
+      const f3 p1 = add(p0, delta);
+      const int lx = cell_of(fminf(p0.x, p1.x) - r, d.g.cell), hx = cell_of(fmaxf(p0.x, p1.x) + r, d.g.cell);
+      const int ly = cell_of(fminf(p0.y, p1.y) - r, d.g.cell), hy = cell_of(fmaxf(p0.y, p1.y) + r, d.g.cell);
+      const int lz = cell_of(fminf(p0.z, p1.z) - r, d.g.cell), hz = cell_of(fmaxf(p0.z, p1.z) + r, d.g.cell);
+      const long nx = hx - lx + 1, ny = hy - ly + 1, nz = hz - lz + 1;
+      const long span = nx * ny * nz;
+      if (span > 0 && span <= 64) {
+        for (long ci = s; ci < span; ci += BP_LANES) {
+          const int x = lx + (int)(ci / (ny * nz)), y = ly + (int)(ci / nz % ny), z = lz + (int)(ci % nz);
+          const uint32_t bk = bucket_of(x, y, z, sch, d.g.nbuckets);
+          const unsigned long long key = cell_key(x, y, z, sch);
+          const uint32_t s0 = d.g.start[bk], c0 = d.g.cnt[bk];
+          for (uint32_t k = 0; k < c0; ++k)
+            if (d.g.ikey[s0 + k] == key) consider(d.g.items[s0 + k], d.g.ipos[s0 + k]);
+        }
+      } else {
+        for (uint32_t k = d.g.soff[sch] + s; k < d.g.soff[sch + 1]; k += BP_LANES) {
+          const uint32_t o = d.g.sitems[k];
+          if (!d.g.is_large[o]) consider(o, pk(xyz(d.b.pos[o]), d.b.shape[o].w));
+        }
+      }
+      for (uint32_t k = d.g.lsc_off[sch] + s; k < d.g.lsc_off[sch + 1]; k += BP_LANES) {
+        const uint32_t o = d.g.lsc[k];
+        consider(o, pk(xyz(d.b.pos[o]), d.b.shape[o].w));
+      }
+    }
+  }
+  __syncthreads();
+  if (h < d.n && s == 0) d.toi_cnt[h] = moving ? s_cnt[lb] : 0u;
 }
 
 __global__ void k_toi_fill(Dev d) {
@@ -514,14 +568,51 @@ __device__ __forceinline__ uint32_t bp_small(const Dev& d, uint32_t a, Emit emit
   return m;
 }
 
-// Counting pass; the first BP_CACHE candidates are kept for the fill pass.
-__global__ void k_bp_count(Dev d, uint32_t* cnt) {
-  uint32_t a = gid();
-  if (a >= d.n) return;
-  uint32_t* cache = d.bp_cache + (size_t)a * BP_CACHE;
-  cnt[a] = d.g.is_large[a] ? 0u : bp_small(d, a, [&](uint32_t i, uint32_t o) {
-    if (i < BP_CACHE) cache[i] = o;
-  });
+// Counting pass, BP_LANES lanes per body: lane s scans the body's cells s,
+// s + BP_LANES, ... (a bucket shared with an earlier cell of the 27 is
+// skipped, as bp_small skips it) and every BP_LANES-th large body.  The first
+// BP_CACHE candidates found, in any order (the fill pass sorts them), are
+// kept for the fill pass; slots come from a per-body LDS counter.
+__global__ void __launch_bounds__(TPB) k_bp_count(Dev d, uint32_t* cnt) {
+  __shared__ uint32_t s_cnt[TPB / BP_LANES];
+  const uint32_t a = gid() / BP_LANES;
+  const uint32_t s = threadIdx.x % BP_LANES, lb = threadIdx.x / BP_LANES;
+  if (s == 0) s_cnt[lb] = 0u;
+  __syncthreads();
+  if (a < d.n && !d.g.is_large[a]) {
+    uint32_t* cache = d.bp_cache + (size_t)a * BP_CACHE;
+    auto emit = [&](uint32_t o) {
+      const uint32_t slot = atomicAdd(&s_cnt[lb], 1u);
+      if (slot < BP_CACHE) cache[slot] = o;
+    };
+    const float4 p = d.b.pos[a];
+    const f3 pa = xyz(p);
+    const float ra = d.b.shape[a].w;
+    const uint32_t sc = d.b.scene[a];
+    const int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
+    for (uint32_t ci = s; ci < 27; ci += BP_LANES) {
+      const uint32_t bk = bucket_of(cx - 1 + (int)(ci / 9), cy - 1 + (int)(ci / 3 % 3), cz - 1 + (int)(ci % 3), sc,
+                                    d.g.nbuckets);
+      bool dup = false;
+      for (uint32_t cj = 0; cj < ci; ++cj)
+        dup = dup || bucket_of(cx - 1 + (int)(cj / 9), cy - 1 + (int)(cj / 3 % 3), cz - 1 + (int)(cj % 3), sc,
+                               d.g.nbuckets) == bk;
+      if (dup) continue;
+      const uint32_t s0 = d.g.start[bk], c0 = d.g.cnt[bk];
+      for (uint32_t k = 0; k < c0; ++k) {
+        const float4 q = d.g.ipos[s0 + k];
+        if (!spheres_near(pa, ra, xyz(q), q.w)) continue;
+        const uint32_t o = d.g.items[s0 + k];
+        if (o > a && prefilter_ids(d, a, o)) emit(o);
+      }
+    }
+    for (uint32_t k = d.g.lsc_off[sc] + s; k < d.g.lsc_off[sc + 1]; k += BP_LANES) {
+      const uint32_t o = d.g.lsc[k];
+      if (o > a && prefilter(d, a, o)) emit(o);
+    }
+  }
+  __syncthreads();
+  if (a < d.n && s == 0) cnt[a] = d.g.is_large[a] ? 0u : s_cnt[lb];
 }
 
 // Large bodies pair with every later body of their scene passing the

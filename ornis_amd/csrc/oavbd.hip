@@ -399,7 +399,7 @@ static void build_grid(oavbd_ctx* c) {
 static void broadphase(oavbd_ctx* c) {
   Dev& d = c->d;
   uint32_t n = d.n;
-  k_bp_count<<<blocks(n), TPB, 0, c->st>>>(d, c->cand_cnt);
+  k_bp_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d, c->cand_cnt);
   if (d.g.nlarge) {
     k_bp_large_flag<<<blocks(d.g.lflat), TPB, 0, c->st>>>(d);
     scan(c, d.lflag, d.lrank, d.g.lflat);  // one scan over all large bodies' flag rows
@@ -544,7 +544,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
-  k_toi_count<<<blocks(n), TPB, 0, c->st>>>(d);
+  k_toi_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d);
   scan(c, d.toi_cnt, d.toi_off, n);
   uint32_t ncast = read_u32(c, d.toi_off + n);
   c->stats.toi_casts = ncast;
