@@ -55,7 +55,9 @@
 static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
 #define TOI_LANES 32
-#define BP_LANES 8        // lanes per body of the broadphase count    // lanes cooperating on one TOI cast
+#ifndef BP_LANES
+#define BP_LANES 8        // lanes per body of the broadphase and TOI target searches
+#endif    // lanes cooperating on one TOI cast
 #define BP_CACHE 32     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
 
