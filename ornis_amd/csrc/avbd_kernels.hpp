@@ -856,9 +856,14 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
 }
 
 // Merge (rows.rs:795-931) fused with the warm-start decay (mod.rs:1019-1040).
+// K_MAX_POINTS lanes per candidate: lane k merges fresh point k; lane j
+// also holds old point j, whose anchor every lane tests (in order, so the
+// first match wins as in the serial scan); lane 0 writes the pair.
 __global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
-  uint32_t c = gid();
-  if (c >= ncand || !d.C.acc[c]) return;
+  constexpr int MG = K_MAX_POINTS;
+  const uint32_t c = gid() / MG;
+  const uint32_t k = gid() % MG;
+  if (c >= ncand || !d.C.acc[c]) return;  // whole groups
   uint32_t p = d.acc_off[c];
   uint32_t start = d.pt_off[c];
   uint32_t cnt = d.C.npts[c];
@@ -866,53 +871,58 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
   float4 d0 = d.C.disc0[c], d1 = d.C.disc1[c];
   int old = __float_as_int(d1.z);
   float4 ma = d.b.mat[ab.x], mb = d.b.mat[ab.y];
-  d.P.ab[p] = ab;
-  d.P.n(p) = d0;
-  d.P.mu(p) = make_float4(d1.x, d1.y, fmax_(ma.z, mb.z), fmax_(ma.w, mb.w));
-  d.P.pts[p] = make_uint2(start, cnt);
-  if (d1.w != 0.0f) {
-    d.b.flag_a[ab.x] = 1;
-    d.b.flag_a[ab.y] = 1;
+  if (k == 0) {
+    d.P.ab[p] = ab;
+    d.P.n(p) = d0;
+    d.P.mu(p) = make_float4(d1.x, d1.y, fmax_(ma.z, mb.z), fmax_(ma.w, mb.w));
+    d.P.pts[p] = make_uint2(start, cnt);
+    if (d1.w != 0.0f) {
+      d.b.flag_a[ab.x] = 1;
+      d.b.flag_a[ab.y] = 1;
+    }
   }
   bool touching = d0.w <= 0.0f;
   bool smooth = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
   const float AG = K_ALPHA * K_GAMMA;
   uint2 op = old >= 0 ? d.Pold.pts[old] : make_uint2(0u, 0u);
-  for (uint32_t k = 0; k < cnt; ++k) {
+  const f3 oldra = k < op.y ? xyz(d.Qold.ra(op.x + k)) : f3{0.0f, 0.0f, 0.0f};
+  f3 ral{0.0f, 0.0f, 0.0f}, rbl{0.0f, 0.0f, 0.0f};
+  if (k < cnt) {
     float4 f0 = d.C.fresh[(size_t)c * 16 + 2 * k];
     float4 f1 = d.C.fresh[(size_t)c * 16 + 2 * k + 1];
-    f3 ral{f0.x, f0.y, f0.z}, rbl{f0.w, f1.x, f1.y};
-    int m = -1;
-    for (uint32_t j = 0; j < op.y; ++j)
-      if (len3(sub(xyz(d.Qold.ra(op.x + j)), ral)) < K_LAMBDA_MATCH) {
-        m = (int)(op.x + j);
-        break;
-      }
-    uint32_t q = start + k;
-    if (m >= 0) {
-      float4 ora = d.Qold.ra(m), orb = d.Qold.rb(m);
-      bool stuck = orb.w != 0.0f;
-      bool keep = stuck && touching && !smooth;
-      f3 ra = keep ? xyz(ora) : ral;
-      f3 rb = keep ? xyz(orb) : rbl;
-      float4 l = d.Qold.lam(m), pe = d.Qold.pen(m), ro = d.Qold.roll[m];
-      d.Q.ra(q) = pk(ra, 0.0f);
-      d.Q.rb(q) = pk(rb, stuck ? 1.0f : 0.0f);
-      d.Q.lam(q) = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
-      d.Q.pen(q) = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
-                               clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
-      d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
-    } else {
-      float z = 0.0f * AG;
-      float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
-      d.Q.ra(q) = pk(ral, 0.0f);
-      d.Q.rb(q) = pk(rbl, 1.0f);
-      d.Q.lam(q) = make_float4(z, z, z, 0.0f);
-      d.Q.pen(q) = make_float4(p1, p1, p1, 0.0f);
-      d.Q.roll[q] = make_float4(z, z, z, 0.0f);
-    }
-    d.Q.pair[q] = p;
+    ral = f3{f0.x, f0.y, f0.z};
+    rbl = f3{f0.w, f1.x, f1.y};
   }
+  int m = -1;
+  for (uint32_t j = 0; j < op.y; ++j) {
+    const f3 o = grp::from_lane<MG>(oldra, (int)j);
+    if (m < 0 && len3(sub(o, ral)) < K_LAMBDA_MATCH) m = (int)(op.x + j);
+  }
+  if (k >= cnt) return;
+  uint32_t q = start + k;
+  if (m >= 0) {
+    float4 ora = d.Qold.ra(m), orb = d.Qold.rb(m);
+    bool stuck = orb.w != 0.0f;
+    bool keep = stuck && touching && !smooth;
+    f3 ra = keep ? xyz(ora) : ral;
+    f3 rb = keep ? xyz(orb) : rbl;
+    float4 l = d.Qold.lam(m), pe = d.Qold.pen(m), ro = d.Qold.roll[m];
+    d.Q.ra(q) = pk(ra, 0.0f);
+    d.Q.rb(q) = pk(rb, stuck ? 1.0f : 0.0f);
+    d.Q.lam(q) = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+    d.Q.pen(q) = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
+                             clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+    d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
+  } else {
+    float z = 0.0f * AG;
+    float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
+    d.Q.ra(q) = pk(ral, 0.0f);
+    d.Q.rb(q) = pk(rbl, 1.0f);
+    d.Q.lam(q) = make_float4(z, z, z, 0.0f);
+    d.Q.pen(q) = make_float4(p1, p1, p1, 0.0f);
+    d.Q.roll[q] = make_float4(z, z, z, 0.0f);
+  }
+  d.Q.pair[q] = p;
 }
 
 // ---------------------------------------------------------------------------

@@ -276,7 +276,13 @@ static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
   js.out[0] = out;
   scan_tiles(c->ts, js, 1, n, c->st);
 }
-// Three arrays of the same length in one launch pair.
+// Two or three arrays of the same length in one launch pair.
+static void scan2(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
+                  uint32_t n) {
+  if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
+  ScanJobs js{{in0, in1, nullptr}, {out0, out1, nullptr}};
+  scan_tiles(c->ts, js, 2, n, c->st);
+}
 static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
                   const uint32_t* in2, uint32_t* out2, uint32_t n) {
   if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
@@ -423,9 +429,8 @@ static void narrow_and_merge(oavbd_ctx* c) {
   d.npts_old = d.npts;
   uint32_t nc = c->ncand;
   if (nc) k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
-  scan(c, d.C.acc, d.acc_off, nc);
+  scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
-  scan(c, d.C.npts, d.pt_off, nc);
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT + 1, d.pt_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipStreamSynchronize(c->st));
@@ -434,7 +439,7 @@ static void narrow_and_merge(oavbd_ctx* c) {
   ensure_pairs(c, d.npairs);
   ensure_points(c, d.npts);
   ensure_incidence(c, d.npairs, d.npts);
-  if (nc) k_merge<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+  if (nc) k_merge<<<blocks((size_t)nc * K_MAX_POINTS), TPB, 0, c->st>>>(d, nc);
   k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d);
 }
 
