@@ -712,28 +712,63 @@ __global__ void k_pair_offsets(Dev d) {
   if (h <= d.n) d.poff[h] = d.acc_off[d.cand_off[h]];
 }
 
+// Narrowphase gate: filters, the old pair, and for two boxes the SAT gap
+// with the creation gate.  Survivors are listed (one atomic per workgroup)
+// for k_narrow with their SAT result stashed in disc0 (normal, gap) and the
+// old pair in disc1.z, so the expensive part runs on full wavefronts.
+__global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
+  __shared__ uint32_t s_n, s_base;
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  const uint32_t c = gid();
+  bool live = false;
+  if (c < ncand) {
+    d.C.acc[c] = 0u;
+    d.C.npts[c] = 0u;
+    const uint2 ab = d.C.ab[c];
+    const uint32_t ia = ab.x, ib = ab.y;
+    if (!d.b.trigger[ia] && !d.b.trigger[ib]) {  // trigger overlaps form no pair
+      const int old = find_old_pair(d, ia, ib);
+      live = true;
+      if (d.b.kind[ia] == SH_BOX && d.b.kind[ib] == SH_BOX) {
+        const shp A = body_shape(d, ia), B = body_shape(d, ib);
+        f3 sn;
+        const float sat = separating_axis(A, A.p, B, B.p, &sn);
+        if (old < 0) {
+          // Creation gate bound: reach <= GEN_MARGIN + |vb - va| dt (|normal| = 1).
+          const f3 va = xyz(d.b.vel[ia]), vb = xyz(d.b.vel[ib]);
+          const float reach_hi = K_GEN_MARGIN + len3(sub(vb, va)) * K_DT * 1.0001f + 1e-6f;
+          live = !(sat > reach_hi);
+        }
+        d.C.disc0[c] = pk(sn, sat);
+      }
+      d.C.disc1[c] = make_float4(0.0f, 0.0f, __int_as_float(old), 0.0f);
+    }
+  }
+  const uint32_t slot = live ? atomicAdd(&s_n, 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) s_base = atomicAdd(&d.counters[CTR_NARROW], s_n);
+  __syncthreads();
+  if (live) d.nlist[s_base + slot] = c;
+}
+
+// Narrowphase of the gate's survivors (thread t takes the t-th listed
+// candidate; threads past the list return at once).
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
-  uint32_t c = gid();
-  if (c >= ncand) return;
-  d.C.acc[c] = 0u;
-  d.C.npts[c] = 0u;
+  const uint32_t t = gid();
+  if (t >= ncand || t >= d.counters[CTR_NARROW]) return;
+  const uint32_t c = d.nlist[t];
   uint2 ab = d.C.ab[c];
   uint32_t ia = ab.x, ib = ab.y;
-  if (d.b.trigger[ia] || d.b.trigger[ib]) return;  // trigger overlaps form no pair
   shp A = body_shape(d, ia), B = body_shape(d, ib);
-  int old = find_old_pair(d, ia, ib);
+  const int old = __float_as_int(d.C.disc1[c].z);
   f3 va = xyz(d.b.vel[ia]), vb = xyz(d.b.vel[ib]);
   dist_t dd;
   float sgn;
   if (A.kind == SH_BOX && B.kind == SH_BOX) {
-    f3 sn;
-    float sat = separating_axis(A, A.p, B, B.p, &sn);
-    if (old < 0) {
-      // Creation gate bound: reach <= GEN_MARGIN + |vb - va| dt (|normal| = 1).
-      float reach_hi = K_GEN_MARGIN + len3(sub(vb, va)) * K_DT * 1.0001f + 1e-6f;
-      if (sat > reach_hi) return;
-    }
-    dd = box_box_from_sat(A, A.p, B, B.p, sat, sn);
+    const float4 g = d.C.disc0[c];
+    const float sat = g.w;
+    dd = box_box_from_sat(A, A.p, B, B.p, sat, xyz(g));
     sgn = sat;
   } else {
     float sat;

@@ -171,6 +171,7 @@ struct Cands {
 };
 
 struct Dev {
+  uint32_t* nlist;     // per candidate capacity: the narrowphase gate's survivors
   Bodies b;
   uint32_t n;
   Pairs P;           // current pair table
@@ -241,5 +242,6 @@ enum {
   CTR_PTSIDES = 7,
   CTR_PAIRSIDES = 8,
   CTR_JSIDES = 9,
+  CTR_NARROW = 10,  // candidates past the narrowphase gate
   CTR_COUNT = 12
 };

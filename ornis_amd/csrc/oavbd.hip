@@ -189,8 +189,9 @@ static void ensure_cand(oavbd_ctx* c, size_t need) {
   size_t cap = grow(need);
   Cands& C = c->d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
-  dfree(c->d.acc_off); dfree(c->d.pt_off);
+  dfree(c->d.acc_off); dfree(c->d.pt_off); dfree(c->d.nlist);
   dalloc(&C.ab, cap); dalloc(&C.acc, cap); dalloc(&C.npts, cap); dalloc(&C.disc0, cap); dalloc(&C.disc1, cap);
+  dalloc(&c->d.nlist, cap);
   dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
   if (cap + 1 > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"candidate count exceeds the scan capacity"};
   c->cap_cand = cap;
@@ -246,7 +247,7 @@ static void free_all(oavbd_ctx* c) {
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
-  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
+  dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.nlist); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
@@ -428,7 +429,10 @@ static void narrow_and_merge(oavbd_ctx* c) {
   d.npairs_old = d.npairs;
   d.npts_old = d.npts;
   uint32_t nc = c->ncand;
-  if (nc) k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+  if (nc) {
+    k_narrow_gate<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+    k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+  }
   scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
   HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
