@@ -232,6 +232,18 @@ HD void box_edge(int i, int* c0, int* c1) {
   }
 }
 
+// Lower bound on the distance from edge i of a box (centre pos, scaled axes
+// e) to the box o with half extents ho: the distance from the edge's
+// midpoint to that box less the edge's half length.
+HD float edge_box_gap(f3 pos, const box_axes_t& e, int i, const shp& o, f3 ho) {
+  int c0, c1;
+  box_edge(i, &c0, &c1);
+  const f3 p0 = box_corner(pos, e, c0), p1 = box_corner(pos, e, c1);
+  const f3 l = abs3(qrot(qconj(o.rot), sub(mul(add(p0, p1), 0.5f), o.pos)));
+  const f3 out{fmaxf(l.x - ho.x, 0.0f), fmaxf(l.y - ho.y, 0.0f), fmaxf(l.z - ho.z, 0.0f)};
+  return len3(out) - 0.5f * len3(sub(p1, p0));
+}
+
 // Candidate c of feature_distance's scan (0-7 corners of a against b,
 // 8-15 corners of b against a, 16-159 edge pairs, a's edge major): the
 // closest points x (on a) and y (on b) and |y - x|^2.

@@ -97,6 +97,105 @@ __device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, i
   return g;
 }
 
+// The calling lane's group's bits of a wave-wide ballot (bit k: lane k of
+// the group).
+template <int G>
+__device__ __forceinline__ uint64_t ballot(bool p) {
+  const uint64_t b = __ballot(p);
+  if constexpr (G == 64) return b;
+  return (b >> (__lane_id() & ~(unsigned)(G - 1))) & ((1ull << G) - 1ull);
+}
+
+// Index of the n-th set bit (n < popcount(m)).
+__device__ __forceinline__ int nth_bit(uint32_t m, uint32_t n) {
+  for (uint32_t q = 0; q < n; ++q) m &= m - 1u;
+  return __builtin_ctz(m);
+}
+
+// feature_distance over the group with the edge pairs pruned (G >= 16).
+// Lane s < 16 evaluates corner candidate s; their minimum U bounds the
+// result from above.  An edge whose lower-bound distance to the other box
+// (edge_box_gap) exceeds sqrt(U) by more than `eps` cannot hold the minimum,
+// nor can any edge pair it belongs to: eps covers the rounding of the
+// bound, of the box corners and of a candidate's closest points (a few ulps
+// of the coordinate scale, against eps = 2e-5 of it), so a pruned pair's
+// computed |y - x|^2 is strictly above U.  The bound takes both rotations
+// as rigid, so nothing is pruned unless both quaternions are unit to 1e-6
+// (a quaternion off unit length by d distorts the corners by up to 2d of
+// the extents).  The surviving pairs are dealt out to the lanes in index
+// order and the (value, index) reduction picks the candidate — and the
+// bits — of the full scan.
+template <int G>
+__device__ dist_t feature_pruned(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
+  static_assert(G >= 16 && G <= 64, "one corner candidate per lane");
+  const box_axes_t ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
+  float best = __builtin_inff();
+  int bc = 0x7fffffff;
+  f3 bx = a.pos, by = b.pos;
+  if (s < 16) {
+    f3 x, y;
+    const float sq = feature_candidate(a, ha, ea, b, hb, eb, s, &x, &y);
+    if (sq < best) {
+      best = sq;
+      bc = s;
+      bx = x;
+      by = y;
+    }
+  }
+  float U = best;
+  for (int m = G / 2; m > 0; m >>= 1) U = fminf(U, xorf<G>(U, m));
+  const float pa = fmaxf(fmaxf(fabsf(a.pos.x), fabsf(a.pos.y)), fabsf(a.pos.z));
+  const float pb = fmaxf(fmaxf(fabsf(b.pos.x), fabsf(b.pos.y)), fabsf(b.pos.z));
+  const float eps = 2e-5f * (1.0f + fmaxf(pa, pb) + 2.0f * (ha.x + ha.y + ha.z + hb.x + hb.y + hb.z));
+  const float na = (a.rot.x * a.rot.x + a.rot.y * a.rot.y) + (a.rot.z * a.rot.z + a.rot.w * a.rot.w);
+  const float nq = (b.rot.x * b.rot.x + b.rot.y * b.rot.y) + (b.rot.z * b.rot.z + b.rot.w * b.rot.w);
+  const bool rigid = fabsf(na - 1.0f) < 1e-6f && fabsf(nq - 1.0f) < 1e-6f;
+  auto keep = [&](float gap) {
+    const float r = gap - eps;
+    return !(rigid && r > 0.0f && r * r * 0.99999f > U);
+  };
+  bool ka = false, kb = false;
+  if (s < 12) {
+    ka = keep(edge_box_gap(a.pos, ea, s, b, hb));
+    kb = keep(edge_box_gap(b.pos, eb, s, a, ha));
+  }
+  const uint32_t ma = (uint32_t)ballot<G>(ka), mb = (uint32_t)ballot<G>(kb);
+  const uint32_t nb = __popc(mb), total = __popc(ma) * nb;
+  // r / nb for r < 144, nb <= 12: (r * ceil(2^16 / nb)) >> 16 is exact
+  const uint32_t inv = nb ? (65536u + nb - 1u) / nb : 0u;
+  for (uint32_t r = (uint32_t)s; r < total; r += G) {
+    const uint32_t qa = (r * inv) >> 16;
+    const int i = nth_bit(ma, qa), j = nth_bit(mb, r - qa * nb);
+    int a0, a1, b0, b1;
+    box_edge(i, &a0, &a1);
+    box_edge(j, &b0, &b1);
+    f3 x, y;
+    seg_seg_closest(box_corner(a.pos, ea, a0), box_corner(a.pos, ea, a1), box_corner(b.pos, eb, b0),
+                    box_corner(b.pos, eb, b1), &x, &y);
+    const float sq = len2(sub(y, x));
+    if (sq < best) {  // candidates rise along the lane's sequence
+      best = sq;
+      bc = 16 + i * 12 + j;
+      bx = x;
+      by = y;
+    }
+  }
+  // smaller squared distance wins; ties go to the earlier candidate
+  float q = best;
+  int c = bc, src = s;
+  for (int m = G / 2; m > 0; m >>= 1) {
+    const float oq = xorf<G>(q, m);
+    const int oc = xori<G>(c, m), os = xori<G>(src, m);
+    if (oq < q || (oq == q && oc < c)) {
+      q = oq;
+      c = oc;
+      src = os;
+    }
+  }
+  if (c == 0x7fffffff) return dist_t{__builtin_inff(), a.pos, b.pos};
+  return dist_t{sqrtf(q), from_lane<G>(bx, src), from_lane<G>(by, src)};
+}
+
 // feature_distance over the group.
 template <int G>
 __device__ dist_t feature(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
@@ -129,15 +228,15 @@ __device__ dist_t feature(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
   return dist_t{sqrtf(q), from_lane<G>(bx, c % G), from_lane<G>(by, c % G)};
 }
 
-// box_box_from_sat over the group.
+// box_box_from_sat over the group (G >= 16: pruned feature scan).
 template <int G>
 __device__ dist_t box_box(const shp& a, f3 ha, const shp& b, f3 hb, float gap, f3 normal, int s) {
-  if (gap > 0.0f) return feature<G>(a, ha, b, hb, s);
+  if (gap > 0.0f) return feature_pruned<G>(a, ha, b, hb, s);
   shp fa, fb;
   f3 ea, eb;
   support_face(a, ha, normal, &fa, &ea);
   support_face(b, hb, neg(normal), &fb, &eb);
-  dist_t r = feature<G>(fa, ea, fb, eb, s);
+  dist_t r = feature_pruned<G>(fa, ea, fb, eb, s);
   r.dist = gap;
   return r;
 }

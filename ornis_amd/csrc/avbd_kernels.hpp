@@ -409,23 +409,40 @@ __device__ __forceinline__ unsigned long long toi_key(float t, uint32_t o) {
 // cast_one for a group of G lanes: box-box distances are evaluated by the
 // whole group (same results on every lane); other shape pairs are cheap
 // closed forms that every lane evaluates alike.
-template <int G>
-__device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, int s) {
-  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one<false>(mover, dir, len, target, normal);
+// Rounding allowance of the swept separating-axis bound of a box-box cast.
+__device__ __forceinline__ float cast_slack(const shp& mover, float len, const shp& target) {
   float sc = 1.0f + len + len3(mover.p) + len3(target.p);
   sc += fmaxf(fmaxf(fabsf(mover.pos.x), fabsf(mover.pos.y)), fabsf(mover.pos.z));
   sc += fmaxf(fmaxf(fabsf(target.pos.x), fabsf(target.pos.y)), fabsf(target.pos.z));
-  const float slack = 4e-5f * sc;
+  return 4e-5f * sc;
+}
+
+// The box-box cast's iterate at travel t: the mover translated to t, its
+// separating-axis gap and normal.  Returns true when the cast misses from
+// here: an axis keeps the rest of the sweep beyond the touch distance, or
+// the gap cannot be closed within the travel left.
+template <int G>
+__device__ __forceinline__ bool cast_bound(const shp& mover, f3 dir, float len, const shp& target, float t,
+                                           float slack, int s, shp* m, f3* sn, float* gap) {
+  *m = mover;
+  m->pos = add(mover.pos, mul(dir, t));
+  bool clear;
+  *gap = grp::sat<G>(*m, m->p, target, target.p, sn, s, &clear, dir, len - t, K_SHAPE_TOUCH, slack);
+  if (clear) return true;  // no iterate can come within the touch distance
+  float lo = *gap - slack;
+  return lo > K_SHAPE_TOUCH && t + (lo - K_SHAPE_TOUCH * 0.5f) >= len;
+}
+
+template <int G>
+__device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, int s) {
+  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one<false>(mover, dir, len, target, normal);
+  const float slack = cast_slack(mover, len, target);
   float t = 0.0f;
   for (int it = 0; it < 24; ++it) {
-    shp m = mover;
-    m.pos = add(mover.pos, mul(dir, t));
+    shp m;
     f3 sn;
-    bool clear;
-    float gap = grp::sat<G>(m, m.p, target, target.p, &sn, s, &clear, dir, len - t, K_SHAPE_TOUCH, slack);
-    if (clear) return -1.0f;  // no iterate can come within the touch distance
-    float lo = gap - slack;
-    if (lo > K_SHAPE_TOUCH && t + (lo - K_SHAPE_TOUCH * 0.5f) >= len) return -1.0f;
+    float gap;
+    if (cast_bound<G>(mover, dir, len, target, t, slack, s, &m, &sn, &gap)) return -1.0f;
     dist_t r = grp::box_box<G>(m, m.p, target, target.p, gap, sn, s);
     if (r.dist <= K_SHAPE_TOUCH) {
       if (t > 0.0f) {
@@ -440,24 +457,69 @@ __device__ float cast_group(const shp& mover, f3 dir, float len, const shp& targ
   return -1.0f;
 }
 
-// One (mover, target) cast per group of G lanes.  The cast's (normal, t) is
-// kept and the mover keeps the smallest (t, target) key: t > 0, so its
-// IEEE bits order like the value.
-template <int G>
-__global__ void __launch_bounds__(TPB) k_toi_pair(Dev d, uint32_t npairs) {
-  uint32_t i = gid() / G;
-  if (i >= npairs) return;
-  const int s = (int)(threadIdx.x % G);
-  uint2 ho = d.toi_pairs[i];
-  f3 delta = mul(xyz(d.b.vel[ho.x]), K_DT);
-  float len = len3(delta);
-  f3 dir = divs(delta, len);
-  f3 nrm = zero3();
-  float t = cast_group<G>(body_shape(d, ho.x), dir, len, body_shape(d, ho.y), &nrm, s);
-  if (s != 0) return;
+// A cast's outcome: its (normal, t) is kept and the mover keeps the
+// smallest (t, target) key: t > 0, so its IEEE bits order like the value.
+__device__ __forceinline__ void toi_result(const Dev& d, uint32_t i, uint2 ho, f3 nrm, float t) {
   d.toi_res[i] = pk(nrm, t);
   if (t < 0.0f) return;
   atomicMin(&d.toi_best[ho.x], toi_key(t, ho.y));
+}
+
+// First look at every cast, G lanes per cast.  Casts that are not box-box
+// are cheap closed forms and finish here; a box-box cast finishes here when
+// cast_group's first iterate (travel 0, the same arithmetic) already misses
+// — most casts, since movers mostly pass their neighbours.  The rest are
+// listed for k_toi_pair.
+template <int G>
+__global__ void __launch_bounds__(TPB) k_toi_gate(Dev d, uint32_t npairs) {
+  __shared__ uint32_t s_n, s_base;
+  if (threadIdx.x == 0) s_n = 0u;
+  __syncthreads();
+  const uint32_t i = gid() / G;
+  const int s = (int)(threadIdx.x % G);
+  bool live = false;
+  if (i < npairs) {
+    const uint2 ho = d.toi_pairs[i];
+    const f3 delta = mul(xyz(d.b.vel[ho.x]), K_DT);
+    const float len = len3(delta);
+    const f3 dir = divs(delta, len);
+    const shp mover = body_shape(d, ho.x), target = body_shape(d, ho.y);
+    f3 nrm = zero3();
+    float t = -1.0f;
+    if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) {
+      t = cast_one<false>(mover, dir, len, target, &nrm);
+    } else {
+      shp m;
+      f3 sn;
+      float gap;
+      live = !cast_bound<G>(mover, dir, len, target, 0.0f, cast_slack(mover, len, target), s, &m, &sn, &gap);
+    }
+    if (s == 0 && !live) toi_result(d, i, ho, nrm, t);
+  }
+  const uint32_t slot = (live && s == 0) ? atomicAdd(&s_n, 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0 && s_n) s_base = atomicAdd(&d.counters[CTR_TOI_LIVE], s_n);
+  __syncthreads();
+  if (live && s == 0) d.toi_live[s_base + slot] = i;
+}
+
+// The listed box-box casts in full, one per group of G lanes, over a
+// grid-stride loop (the list length is only known on the device).
+template <int G>
+__global__ void __launch_bounds__(TPB) k_toi_pair(Dev d) {
+  const uint32_t live = d.counters[CTR_TOI_LIVE];
+  const uint32_t groups = gridDim.x * (TPB / G);
+  const int s = (int)(threadIdx.x % G);
+  for (uint32_t q = gid() / G; q < live; q += groups) {
+    const uint32_t i = d.toi_live[q];
+    const uint2 ho = d.toi_pairs[i];
+    const f3 delta = mul(xyz(d.b.vel[ho.x]), K_DT);
+    const float len = len3(delta);
+    const f3 dir = divs(delta, len);
+    f3 nrm = zero3();
+    const float t = cast_group<G>(body_shape(d, ho.x), dir, len, body_shape(d, ho.y), &nrm, s);
+    if (s == 0) toi_result(d, i, ho, nrm, t);
+  }
 }
 
 // The mover's hit is the cast holding the winning key (a repeated
@@ -752,6 +814,26 @@ __global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
   if (live) d.nlist[s_base + slot] = c;
 }
 
+// Closest features of the listed box-box candidates, G lanes per candidate
+// (box_box_from_sat with the gate's SAT result and the pruned group scan),
+// stashed in the candidate's first two point slots for k_narrow, which
+// reads them before it writes its points.
+template <int G>
+__global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
+  const uint32_t t = gid() / G;
+  const int s = (int)(threadIdx.x % G);
+  if (t >= ncand || t >= d.counters[CTR_NARROW]) return;  // whole groups
+  const uint32_t c = d.nlist[t];
+  const uint2 ab = d.C.ab[c];
+  if (!(d.b.kind[ab.x] == SH_BOX && d.b.kind[ab.y] == SH_BOX)) return;
+  const shp A = body_shape(d, ab.x), B = body_shape(d, ab.y);
+  const float4 g = d.C.disc0[c];
+  const dist_t r = grp::box_box<G>(A, A.p, B, B.p, g.w, xyz(g), s);
+  if (s != 0) return;
+  d.C.fresh[(size_t)c * 16] = make_float4(r.dist, r.pa.x, r.pa.y, r.pa.z);
+  d.C.fresh[(size_t)c * 16 + 1] = make_float4(r.pb.x, r.pb.y, r.pb.z, 0.0f);
+}
+
 // Narrowphase of the gate's survivors (thread t takes the t-th listed
 // candidate; threads past the list return at once).
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
@@ -766,10 +848,9 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   dist_t dd;
   float sgn;
   if (A.kind == SH_BOX && B.kind == SH_BOX) {
-    const float4 g = d.C.disc0[c];
-    const float sat = g.w;
-    dd = box_box_from_sat(A, A.p, B, B.p, sat, xyz(g));
-    sgn = sat;
+    const float4 r0 = d.C.fresh[(size_t)c * 16], r1 = d.C.fresh[(size_t)c * 16 + 1];
+    dd = dist_t{r0.x, f3{r0.y, r0.z, r0.w}, f3{r1.x, r1.y, r1.z}};
+    sgn = d.C.disc0[c].w;
   } else {
     float sat;
     dd = shape_distance<false>(A, B, &sat);  // never box-box here

@@ -54,7 +54,9 @@
 #define CCLS (CCLS_COLOURS * 8)  // (colour, size class) counters
 static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
-#define TOI_LANES 32
+#define TOI_LANES 64
+#define TOI_GATE_LANES 16  // lanes per cast of the TOI gate (15 separating axes)
+#define NARROW_LANES 16    // lanes per box-box candidate of the narrowphase distance
 #ifndef BP_LANES
 #define BP_LANES 8        // lanes per body of the broadphase and TOI target searches
 #endif    // lanes cooperating on one TOI cast
@@ -225,6 +227,7 @@ struct Dev {
   uint32_t* toi_off;   // n+1
   uint2* toi_pairs;    // (mover, target) casts
   float4* toi_res;     // per cast: normal xyz, w = travel to the hit (< 0: none)
+  uint32_t* toi_live;  // box-box casts left by the first swept bound (CTR_TOI_LIVE of them)
   uint32_t* bp_cache;  // n x BP_CACHE candidates found by the counting pass
   uint32_t* toi_cache; // n x TOI_CACHE targets found by the counting pass
   unsigned long long* toi_best;  // per mover min (t bits << 32 | target)
@@ -243,5 +246,6 @@ enum {
   CTR_PAIRSIDES = 8,
   CTR_JSIDES = 9,
   CTR_NARROW = 10,  // candidates past the narrowphase gate
+  CTR_TOI_LIVE = 11,  // casts past the TOI gate
   CTR_COUNT = 12
 };

@@ -141,6 +141,7 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
   d.toi_res = nullptr;
+  d.toi_live = nullptr;
   uint32_t nb = 1;
   while (nb < 2 * n) nb <<= 1;
   d.g.nbuckets = nb;
@@ -249,7 +250,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.nlist); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
-  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res);
+  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
 }
 
@@ -431,6 +432,7 @@ static void narrow_and_merge(oavbd_ctx* c) {
   uint32_t nc = c->ncand;
   if (nc) {
     k_narrow_gate<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+    k_narrow_dist<NARROW_LANES><<<blocks((size_t)nc * NARROW_LANES), TPB, 0, c->st>>>(d, nc);
     k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
   }
   scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc);
@@ -561,12 +563,17 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     if (ncast > c->cap_toi) {
       dfree(d.toi_pairs);
       dfree(d.toi_res);
+      dfree(d.toi_live);
       c->cap_toi = grow(ncast);
       dalloc(&d.toi_pairs, c->cap_toi);
       dalloc(&d.toi_res, c->cap_toi);
+      dalloc(&d.toi_live, c->cap_toi);
     }
     k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
-    k_toi_pair<TOI_LANES><<<blocks((size_t)ncast * TOI_LANES), TPB, 0, c->st>>>(d, ncast);
+    // the gate settles most casts; the survivors' count stays on the device,
+    // so the full casts run a grid-stride loop over a quarter-size grid
+    k_toi_gate<TOI_GATE_LANES><<<blocks((size_t)ncast * TOI_GATE_LANES), TPB, 0, c->st>>>(d, ncast);
+    k_toi_pair<TOI_LANES><<<blocks((size_t)(ncast + 3) / 4 * TOI_LANES), TPB, 0, c->st>>>(d);
     k_toi_win<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
     k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
     k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
