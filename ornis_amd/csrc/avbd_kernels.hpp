@@ -1479,6 +1479,27 @@ __global__ void __launch_bounds__(COLOUR_TAIL_TPB) k_colour_tail(Dev d, uint32_t
 // Bodies per colour and per (colour, size class), and the number of colours,
 // from per-block LDS histograms (colours from CCLS_COLOURS on, rare, take
 // one atomic per body).
+// Exclusive offsets of the colour classes (colour_off[0..ncol], ncol the
+// colour count in cstat) from their sizes: one wavefront, 64 colours a round.
+__global__ void __launch_bounds__(64) k_colour_offsets(Dev d) {
+  constexpr int WV = 64;
+  const uint32_t ncol = d.cstat[CSTAT_MAX];
+  const int lane = (int)threadIdx.x;
+  uint32_t base = 0;
+  for (uint32_t c0 = 0; c0 < ncol; c0 += WV) {
+    const uint32_t k = c0 + (uint32_t)lane;
+    const uint32_t v = k < ncol ? d.colour_cnt[k] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < WV; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, WV);
+      if (lane >= o) x += y;
+    }
+    if (k < ncol) d.colour_off[k] = base + x - v;
+    base += __shfl(x, WV - 1, WV);
+  }
+  if (lane == 0) d.colour_off[ncol] = base;
+}
+
 __global__ void __launch_bounds__(TPB) k_colour_classes(Dev d) {
   __shared__ uint32_t cnt[CCLS];
   __shared__ uint32_t s_max;

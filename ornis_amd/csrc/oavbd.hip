@@ -64,6 +64,10 @@ struct HostJoint {
 
 }  // namespace
 
+// Word offset of the colouring check in the pinned readback buffer (the
+// words below it hold the counters and single-word reads).
+constexpr uint32_t HEAD_H = 64;
+
 struct oavbd_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -74,7 +78,7 @@ struct oavbd_ctx {
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
   TileScan ts;
-  uint32_t* counters_h = nullptr;  // pinned
+  uint32_t* counters_h = nullptr;  // pinned: counters, read_u32 words, colouring check from HEAD_H
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
   std::vector<int32_t> kind_h, type_h;
@@ -92,6 +96,7 @@ struct oavbd_ctx {
   std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
   oavbd_kernel_times kt{};
   hipEvent_t ev[5] = {};
+  hipEvent_t ev_rb = nullptr;  // after the colouring check's copy (host waits on it, the stream runs on)
   float ms[5] = {};
   std::string err;
 };
@@ -464,16 +469,30 @@ static void incidence_and_colour(oavbd_ctx* c) {
   // listed and finished by one single-workgroup launch.  `grid` is the pass
   // after which at most COLOUR_TAIL_FRONTIER bodies were left in the previous
   // step (the counts change slowly).  One check then copies the per-pass
-  // counts, the pass and colour counts and the first colour sizes.
-  uint32_t head[CSTAT_HEAD + CSTAT_EAGER];
+  // counts, the pass and colour counts and the first colour sizes; the
+  // colour offsets and lists are made on the device meanwhile, and the host
+  // waits for the check (colour_wait) only once the rest of the set-up is
+  // queued behind it.
+  uint32_t* head = c->counters_h + HEAD_H;
   const int grid = c->grid_rounds;
   for (int round = 0; round < grid; ++round)
     k_colour_pass<<<blocks((size_t)n * CL, COLOUR_PASS_TPB), COLOUR_PASS_TPB, 0, c->st>>>(d, d.remain + round);
   k_colour_collect<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(grid & 1) * n, d.fcnt + grid % 3);
   k_colour_tail<<<1, COLOUR_TAIL_TPB, 0, c->st>>>(d, (uint32_t)grid);
   k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
-  HIPC(hipMemcpyAsync(head, d.cstat, sizeof(head), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipStreamSynchronize(c->st));
+  HIPC(hipMemcpyAsync(head, d.cstat, (CSTAT_HEAD + CSTAT_EAGER) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipEventRecord(c->ev_rb, c->st));
+  k_colour_offsets<<<1, 64, 0, c->st>>>(d);
+  k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+}
+
+// The colouring check: pass and colour counts, colour sizes and offsets on
+// the host (the device made the same offsets).
+static void colour_wait(oavbd_ctx* c) {
+  Dev& d = c->d;
+  const uint32_t* head = c->counters_h + HEAD_H;
+  const int grid = c->grid_rounds;
+  HIPC(hipEventSynchronize(c->ev_rb));
   int rounds = -1, next_grid = -1;
   for (int r = 0; r < grid; ++r) {
     if (rounds < 0 && head[r] == 0) rounds = r + 1;
@@ -492,10 +511,7 @@ static void incidence_and_colour(oavbd_ctx* c) {
     else
       HIPC(hipMemcpy(c->colour_cnt_h.data(), d.colour_cnt, ncol * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t k = 0; k < ncol; ++k) c->colour_off_h[k + 1] = c->colour_off_h[k] + c->colour_cnt_h[k];
-    HIPC(hipMemcpyAsync(d.colour_off, c->colour_off_h.data(), (ncol + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                        c->st));
   }
-  k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   c->stats.colours = ncol;
   c->stats.max_colour = 0;
   c->solved = 0;
@@ -592,6 +608,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   k_sb_pack<<<blocks(n), TPB, 0, c->st>>>(d);
   k_inertial_blocks<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 1);
+  colour_wait(c);
   const uint32_t ncol = c->stats.colours;
   // In timing mode every primal / dual launch is bracketed by pool events;
   // they are resolved after the step's final synchronisation.
@@ -716,8 +733,10 @@ int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_j
   try {
     HIPC(hipSetDevice(device));
     HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-    HIPC(hipHostMalloc((void**)&c->counters_h, 64 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPC(hipHostMalloc((void**)&c->counters_h, (HEAD_H + CSTAT_HEAD + CSTAT_EAGER) * sizeof(uint32_t),
+                       hipHostMallocDefault));
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+    HIPC(hipEventCreateWithFlags(&c->ev_rb, hipEventDisableTiming));
     alloc_bodies(c, max_bodies);
   } catch (const HipError& e) {
     std::string m = e.msg;
@@ -737,6 +756,7 @@ void oavbd_destroy(oavbd_ctx* c) {
   free_all(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_rb) (void)hipEventDestroy(c->ev_rb);
   for (auto& e : c->pev) (void)hipEventDestroy(e);
   for (auto& e : c->dev_ev) (void)hipEventDestroy(e);
   if (c->counters_h) (void)hipHostFree(c->counters_h);
