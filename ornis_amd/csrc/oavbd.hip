@@ -67,6 +67,7 @@ struct HostJoint {
 // Word offset of the colouring check in the pinned readback buffer (the
 // words below it hold the counters and single-word reads).
 constexpr uint32_t HEAD_H = 64;
+constexpr uint32_t POST_FLAG = 63;  // word of the readback sequence flag
 
 struct oavbd_ctx {
   int device = 0;
@@ -78,7 +79,9 @@ struct oavbd_ctx {
   uint32_t ncand = 0;
   uint32_t* cand_cnt = nullptr;  // n + 1
   TileScan ts;
-  uint32_t* counters_h = nullptr;  // pinned: counters, read_u32 words, colouring check from HEAD_H
+  uint32_t* counters_h = nullptr;  // pinned, coherent: counters, read_u32 words, colouring check from HEAD_H
+  uint32_t* counters_hd = nullptr;  // the same memory as the device addresses it
+  uint32_t post_seq = 0, colour_seq = 0;
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
   std::vector<int32_t> kind_h, type_h;
@@ -96,7 +99,6 @@ struct oavbd_ctx {
   std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
   oavbd_kernel_times kt{};
   hipEvent_t ev[5] = {};
-  hipEvent_t ev_rb = nullptr;  // after the colouring check's copy (host waits on it, the stream runs on)
   float ms[5] = {};
   std::string err;
 };
@@ -302,9 +304,51 @@ static void read_counters(oavbd_ctx* c) {
   HIPC(hipStreamSynchronize(c->st));
 }
 
+// Readbacks the step waits for mid-stream: a one-workgroup kernel writes the
+// words straight into the pinned buffer and then raises a sequence flag
+// there, which the host polls (a stream synchronisation would add the copy
+// launch and the runtime's wake-up to every wait).
+struct PostJob {
+  const uint32_t* src[2] = {nullptr, nullptr};  // single words ...
+  uint32_t dst[2] = {0u, 0u};                   // ... to these buffer words
+  const uint32_t* run = nullptr;                // and a run of words
+  uint32_t drun = 0u, nrun = 0u;
+};
+__global__ void __launch_bounds__(256) k_post(PostJob j, uint32_t* buf, uint32_t seq) {
+  const uint32_t t = threadIdx.x;
+  if (t == 0 && j.src[0]) buf[j.dst[0]] = *j.src[0];
+  if (t == 1 && j.src[1]) buf[j.dst[1]] = *j.src[1];
+  for (uint32_t i = t; i < j.nrun; i += blockDim.x) buf[j.drun + i] = j.run[i];
+  __threadfence_system();
+  __syncthreads();
+  if (t == 0) __hip_atomic_store(buf + POST_FLAG, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+static uint32_t post(oavbd_ctx* c, const PostJob& j) {
+  const uint32_t seq = ++c->post_seq;
+  k_post<<<1, 256, 0, c->st>>>(j, c->counters_hd, seq);
+  return seq;
+}
+
+// Waits for post `seq`; an idle stream without the flag, or a failed one,
+// is an error rather than a hang.
+static void post_wait(oavbd_ctx* c, uint32_t seq) {
+  uint32_t spins = 0;
+  while (__atomic_load_n(c->counters_h + POST_FLAG, __ATOMIC_ACQUIRE) != seq) {
+    if ((++spins & 1023u) != 0u) continue;
+    const hipError_t e = hipStreamQuery(c->st);
+    if (e == hipErrorNotReady) continue;
+    HIPC(e);
+    if (__atomic_load_n(c->counters_h + POST_FLAG, __ATOMIC_ACQUIRE) != seq)
+      throw HipError{"readback flag not raised by an idle stream"};
+  }
+}
+
 static uint32_t read_u32(oavbd_ctx* c, const uint32_t* dptr) {
-  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipStreamSynchronize(c->st));
+  PostJob j;
+  j.src[0] = dptr;
+  j.dst[0] = CTR_COUNT;
+  post_wait(c, post(c, j));
   return c->counters_h[CTR_COUNT];
 }
 
@@ -442,9 +486,14 @@ static void narrow_and_merge(oavbd_ctx* c) {
   }
   scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
-  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT, d.acc_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipMemcpyAsync(c->counters_h + CTR_COUNT + 1, d.pt_off + nc, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipStreamSynchronize(c->st));
+  {
+    PostJob j;
+    j.src[0] = d.acc_off + nc;
+    j.src[1] = d.pt_off + nc;
+    j.dst[0] = CTR_COUNT;
+    j.dst[1] = CTR_COUNT + 1;
+    post_wait(c, post(c, j));
+  }
   d.npairs = c->counters_h[CTR_COUNT];
   d.npts = c->counters_h[CTR_COUNT + 1];
   ensure_pairs(c, d.npairs);
@@ -473,15 +522,19 @@ static void incidence_and_colour(oavbd_ctx* c) {
   // colour offsets and lists are made on the device meanwhile, and the host
   // waits for the check (colour_wait) only once the rest of the set-up is
   // queued behind it.
-  uint32_t* head = c->counters_h + HEAD_H;
   const int grid = c->grid_rounds;
   for (int round = 0; round < grid; ++round)
     k_colour_pass<<<blocks((size_t)n * CL, COLOUR_PASS_TPB), COLOUR_PASS_TPB, 0, c->st>>>(d, d.remain + round);
   k_colour_collect<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(grid & 1) * n, d.fcnt + grid % 3);
   k_colour_tail<<<1, COLOUR_TAIL_TPB, 0, c->st>>>(d, (uint32_t)grid);
   k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
-  HIPC(hipMemcpyAsync(head, d.cstat, (CSTAT_HEAD + CSTAT_EAGER) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipEventRecord(c->ev_rb, c->st));
+  {
+    PostJob j;
+    j.run = d.cstat;
+    j.drun = HEAD_H;
+    j.nrun = CSTAT_HEAD + CSTAT_EAGER;
+    c->colour_seq = post(c, j);
+  }
   k_colour_offsets<<<1, 64, 0, c->st>>>(d);
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
 }
@@ -492,7 +545,7 @@ static void colour_wait(oavbd_ctx* c) {
   Dev& d = c->d;
   const uint32_t* head = c->counters_h + HEAD_H;
   const int grid = c->grid_rounds;
-  HIPC(hipEventSynchronize(c->ev_rb));
+  post_wait(c, c->colour_seq);
   int rounds = -1, next_grid = -1;
   for (int r = 0; r < grid; ++r) {
     if (rounds < 0 && head[r] == 0) rounds = r + 1;
@@ -734,9 +787,10 @@ int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_j
     HIPC(hipSetDevice(device));
     HIPC(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
     HIPC(hipHostMalloc((void**)&c->counters_h, (HEAD_H + CSTAT_HEAD + CSTAT_EAGER) * sizeof(uint32_t),
-                       hipHostMallocDefault));
+                       hipHostMallocCoherent));
+    HIPC(hipHostGetDevicePointer((void**)&c->counters_hd, c->counters_h, 0));
+    c->counters_h[POST_FLAG] = 0u;
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
-    HIPC(hipEventCreateWithFlags(&c->ev_rb, hipEventDisableTiming));
     alloc_bodies(c, max_bodies);
   } catch (const HipError& e) {
     std::string m = e.msg;
@@ -756,7 +810,6 @@ void oavbd_destroy(oavbd_ctx* c) {
   free_all(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  if (c->ev_rb) (void)hipEventDestroy(c->ev_rb);
   for (auto& e : c->pev) (void)hipEventDestroy(e);
   for (auto& e : c->dev_ev) (void)hipEventDestroy(e);
   if (c->counters_h) (void)hipHostFree(c->counters_h);
