@@ -837,6 +837,9 @@ __global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
 // Narrowphase of the gate's survivors (thread t takes the t-th listed
 // candidate; threads past the list return at once).
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
+  // the manifold points (local offsets on a and on b) of each thread, one
+  // LDS column per thread: the point loops index them at run time
+  __shared__ float s_pts[K_MAX_POINTS * 6][TPB];
   const uint32_t t = gid();
   if (t >= ncand || t >= d.counters[CTR_NARROW]) return;
   const uint32_t c = d.nlist[t];
@@ -890,16 +893,22 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   float plane_b = B.kind == SH_BOX ? dot3(normal, B.pos) - (-1.0f) * dot3(abs3(qrot(qconj(B.rot), normal)), B.p)
                                    : dot3(normal, dd.pb);
   qf inv_a = qconj(A.rot), inv_b = qconj(B.rot);
-  f3 fra[K_MAX_POINTS], frb[K_MAX_POINTS];
+  const int tx = (int)threadIdx.x;
+  auto fra = [&](int i) { return f3{s_pts[6 * i][tx], s_pts[6 * i + 1][tx], s_pts[6 * i + 2][tx]}; };
+  auto frb = [&](int i) { return f3{s_pts[6 * i + 3][tx], s_pts[6 * i + 4][tx], s_pts[6 * i + 5][tx]}; };
   int nf = 0;
   auto dup = [&](f3 x, f3 y) {
     for (int i = 0; i < nf; ++i)
-      if (len3(sub(fra[i], x)) < K_POINT_MATCH && len3(sub(frb[i], y)) < K_POINT_MATCH) return true;
+      if (len3(sub(fra(i), x)) < K_POINT_MATCH && len3(sub(frb(i), y)) < K_POINT_MATCH) return true;
     return false;
   };
   auto push = [&](f3 x, f3 y) {
-    fra[nf] = x;
-    frb[nf] = y;
+    s_pts[6 * nf][tx] = x.x;
+    s_pts[6 * nf + 1][tx] = x.y;
+    s_pts[6 * nf + 2][tx] = x.z;
+    s_pts[6 * nf + 3][tx] = y.x;
+    s_pts[6 * nf + 4][tx] = y.y;
+    s_pts[6 * nf + 5][tx] = y.z;
     ++nf;
   };
   if (!separated) {
@@ -966,8 +975,9 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   d.C.disc0[c] = pk(normal, sgn);
   d.C.disc1[c] = make_float4(mu1, mu1, __int_as_float(old), wake ? 1.0f : 0.0f);
   for (int k = 0; k < nf; ++k) {
-    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(fra[k].x, fra[k].y, fra[k].z, frb[k].x);
-    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(frb[k].y, frb[k].z, 0.0f, 0.0f);
+    const f3 x = fra(k), y = frb(k);
+    d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(x.x, x.y, x.z, y.x);
+    d.C.fresh[(size_t)c * 16 + 2 * k + 1] = make_float4(y.y, y.z, 0.0f, 0.0f);
   }
 }
 

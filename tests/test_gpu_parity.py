@@ -115,6 +115,27 @@ def test_fast_rain_toi_bitwise(gpu_available):
     assert g.stats()["toi_hits"] >= 0
 
 
+def test_unnormalised_rotations_bitwise(gpu_available):
+    """Rotations given off unit length (static boxes keep them; dynamic ones
+    are renormalised after their first step): the box-box feature scans take
+    no shortcut for them, and the results stay those of the full scan."""
+    rng = np.random.Generator(np.random.PCG64(45))
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (8.0, 1.0, 8.0), 0.0)
+    for i in range(6):
+        q = rng.normal(size=4)
+        q = q / np.linalg.norm(q) * (0.95 if i % 2 else 1.05)
+        sb.box((-4.0 + 1.6 * i, 0.6, 0.0), (0.6, 0.4, 0.5), 0.0, quat=tuple(q.astype(np.float32).tolist()))
+    for i in range(40):
+        q = rng.normal(size=4)
+        q = q / np.linalg.norm(q) * rng.uniform(0.97, 1.03)
+        p = (rng.uniform(-4.5, 4.5), rng.uniform(1.5, 6.0), rng.uniform(-1.0, 1.0))
+        sb.box(p, tuple(rng.uniform(0.2, 0.5, 3)), 1.0, quat=tuple(q.astype(np.float32).tolist()),
+               vel=(0.0, -rng.uniform(0.0, 30.0), 0.0))
+    g, _ = run_pair(sb.build(), 80, check_every=5)
+    assert g.stats()["pairs"] >= 20
+
+
 def test_batched_scenes_equal_independent_runs(gpu_available):
     """Scenes stepped together in one context equal each scene stepped
     alone by the oracle."""
