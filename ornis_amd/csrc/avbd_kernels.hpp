@@ -775,15 +775,21 @@ __global__ void k_pair_offsets(Dev d) {
 }
 
 // Narrowphase gate: filters, the old pair, and for two boxes the SAT gap
-// with the creation gate.  Survivors are listed (one atomic per workgroup)
-// for k_narrow with their SAT result stashed in disc0 (normal, gap) and the
-// old pair in disc1.z, so the expensive part runs on full wavefronts.
+// with the creation gate.  Survivors are listed (one atomic per workgroup
+// and list) for k_narrow with their SAT result stashed in disc0 (normal,
+// gap) and the old pair in disc1.z, so the expensive part runs on full
+// wavefronts: overlapping box pairs (whose feature scans are the longest)
+// from the back of the candidate range, the rest from the front, so that
+// wavefronts hold work of one kind.
 __global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
-  __shared__ uint32_t s_n, s_base;
-  if (threadIdx.x == 0) s_n = 0u;
+  __shared__ uint32_t s_n, s_base, s_p, s_pbase;  // front and back lists
+  if (threadIdx.x == 0) {
+    s_n = 0u;
+    s_p = 0u;
+  }
   __syncthreads();
   const uint32_t c = gid();
-  bool live = false;
+  bool live = false, overlap = false;
   if (c < ncand) {
     d.C.acc[c] = 0u;
     d.C.npts[c] = 0u;
@@ -802,16 +808,33 @@ __global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
           const float reach_hi = K_GEN_MARGIN + len3(sub(vb, va)) * K_DT * 1.0001f + 1e-6f;
           live = !(sat > reach_hi);
         }
+        overlap = !(sat > 0.0f);
         d.C.disc0[c] = pk(sn, sat);
       }
       d.C.disc1[c] = make_float4(0.0f, 0.0f, __int_as_float(old), 0.0f);
     }
   }
-  const uint32_t slot = live ? atomicAdd(&s_n, 1u) : 0u;
+  const uint32_t slot = live ? atomicAdd(overlap ? &s_p : &s_n, 1u) : 0u;
   __syncthreads();
-  if (threadIdx.x == 0 && s_n) s_base = atomicAdd(&d.counters[CTR_NARROW], s_n);
+  if (threadIdx.x == 0 && (s_n || s_p)) {
+    // both lists' bases from one atomic on the shared word (no carry: each
+    // count stays below the candidate count)
+    const unsigned long long b = atomicAdd(reinterpret_cast<unsigned long long*>(d.counters + CTR_NARROW),
+                                           ((unsigned long long)s_p << 32) | s_n);
+    s_base = (uint32_t)b;
+    s_pbase = (uint32_t)(b >> 32);
+  }
   __syncthreads();
-  if (live) d.nlist[s_base + slot] = c;
+  if (live) d.nlist[overlap ? ncand - 1u - (s_pbase + slot) : s_base + slot] = c;
+}
+
+// Entry t of the gate's list: the first counters[CTR_NARROW] entries of
+// nlist and the last counters[CTR_NARROW_PEN]; false between them.
+__device__ __forceinline__ bool narrow_entry(const Dev& d, uint32_t t, uint32_t ncand, uint32_t* c) {
+  if (t >= ncand) return false;
+  if (t >= d.counters[CTR_NARROW] && t < ncand - d.counters[CTR_NARROW_PEN]) return false;
+  *c = d.nlist[t];
+  return true;
 }
 
 // Closest features of the listed box-box candidates, G lanes per candidate
@@ -822,8 +845,8 @@ template <int G>
 __global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
   const uint32_t t = gid() / G;
   const int s = (int)(threadIdx.x % G);
-  if (t >= ncand || t >= d.counters[CTR_NARROW]) return;  // whole groups
-  const uint32_t c = d.nlist[t];
+  uint32_t c;
+  if (!narrow_entry(d, t, ncand, &c)) return;  // whole groups
   const uint2 ab = d.C.ab[c];
   if (!(d.b.kind[ab.x] == SH_BOX && d.b.kind[ab.y] == SH_BOX)) return;
   const shp A = body_shape(d, ab.x), B = body_shape(d, ab.y);
@@ -834,15 +857,15 @@ __global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
   d.C.fresh[(size_t)c * 16 + 1] = make_float4(r.pb.x, r.pb.y, r.pb.z, 0.0f);
 }
 
-// Narrowphase of the gate's survivors (thread t takes the t-th listed
-// candidate; threads past the list return at once).
+// Narrowphase of the gate's survivors (thread t takes list entry t; threads
+// off the list return at once).
 __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
   // the manifold points (local offsets on a and on b) of each thread, one
   // LDS column per thread: the point loops index them at run time
   __shared__ float s_pts[K_MAX_POINTS * 6][TPB];
   const uint32_t t = gid();
-  if (t >= ncand || t >= d.counters[CTR_NARROW]) return;
-  const uint32_t c = d.nlist[t];
+  uint32_t c;
+  if (!narrow_entry(d, t, ncand, &c)) return;
   uint2 ab = d.C.ab[c];
   uint32_t ia = ab.x, ib = ab.y;
   shp A = body_shape(d, ia), B = body_shape(d, ib);

@@ -234,6 +234,8 @@ struct Dev {
   uint8_t* fast;
 };
 
+// CTR_NARROW and CTR_NARROW_PEN share one 64-bit word (even index) that the
+// narrowphase gate bumps with one atomic.
 enum {
   CTR_FAST = 0,
   CTR_REMAIN = 1,
@@ -245,7 +247,9 @@ enum {
   CTR_PTSIDES = 7,
   CTR_PAIRSIDES = 8,
   CTR_JSIDES = 9,
-  CTR_NARROW = 10,  // candidates past the narrowphase gate
-  CTR_TOI_LIVE = 11,  // casts past the TOI gate
-  CTR_COUNT = 12
+  CTR_NARROW = 10,  // candidates past the narrowphase gate, listed from the front
+  CTR_NARROW_PEN = 11,  // overlapping box pairs past the gate, listed from the back
+  CTR_TOI_LIVE = 12,  // casts past the TOI gate
+  CTR_COUNT = 13
 };
+static_assert(CTR_NARROW % 2 == 0 && CTR_NARROW_PEN == CTR_NARROW + 1, "narrowphase list counters share a word");
