@@ -1,8 +1,9 @@
 // avbd_scan.hpp — exclusive prefix sum of uint32 arrays on the device.
 //
 // out[0..n] receives the exclusive scan of in[0..n) with out[n] = total.
-// Blocks of 256 threads take tiles of 1024 elements.  Integer addition is
-// associative, so the result does not depend on the blocking.
+// Blocks of SCAN_BLOCK threads take tiles of SCAN_TILE elements, each thread
+// a run of SCAN_PER_THREAD.  Integer addition is associative, so the result
+// does not depend on the blocking.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,20 +21,49 @@ __global__ void k_scan_zero(uint32_t* __restrict__ out) { out[0] = 0u; }
 // scans itself in LDS and adds its offset.  No workgroup waits on another.
 #define SCAN_MAX_TILES (SCAN_BLOCK * 64)  // tile offsets scanned by one workgroup: n <= 2^24
 
+// Exclusive scan over the workgroup: wavefront scans by shuffles, then the
+// wavefront totals through LDS (s holds SCAN_BLOCK / 64 words).  Integer
+// sums, so the grouping does not matter.
 __device__ __forceinline__ uint32_t block_scan_excl(uint32_t t, uint32_t* s, uint32_t* total) {
-  const uint32_t tid = threadIdx.x;
-  s[tid] = t;
-  __syncthreads();
-  for (uint32_t off = 1; off < SCAN_BLOCK; off <<= 1) {
-    uint32_t add = tid >= off ? s[tid - off] : 0u;
-    __syncthreads();
-    s[tid] += add;
-    __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint32_t x = t;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
   }
-  *total = s[SCAN_BLOCK - 1];
-  uint32_t r = s[tid] - t;
+  if (lane == 63u) s[w] = x;
   __syncthreads();
-  return r;
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < SCAN_BLOCK / 64; ++k) {
+    const uint32_t v = s[k];
+    if (k < w) pre += v;
+    tot += v;
+  }
+  *total = tot;
+  __syncthreads();  // s may be reused at once
+  return pre + x - t;
+}
+
+// The SCAN_PER_THREAD consecutive entries of a thread (zero past n); whole
+// 16-byte vectors when the run is in range and aligned.
+__device__ __forceinline__ void load_run(const uint32_t* __restrict__ in, uint32_t base, uint32_t n,
+                                         uint32_t (&v)[SCAN_PER_THREAD]) {
+  if (base + SCAN_PER_THREAD <= n && ((reinterpret_cast<uintptr_t>(in + base) & 15u) == 0u)) {
+    const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD / 4; ++k) {
+      const uint4 q = p[k];
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) v[k] = (base + k < n) ? in[base + k] : 0u;
+  }
 }
 
 // Up to SCAN_JOBS arrays of the same length scanned by one launch pair
@@ -46,16 +76,18 @@ struct ScanJobs {
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint32_t n, uint32_t ntiles,
                                                             uint32_t* tiles, uint32_t* dones) {
-  __shared__ uint32_t s[SCAN_BLOCK];
+  __shared__ uint32_t s[SCAN_BLOCK / 64];
   const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
   uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
   uint32_t* done = dones + blockIdx.y;
   __shared__ bool last;
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_TILE + tid * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  load_run(in, base, n, v);
   uint32_t t = 0;
 #pragma unroll
-  for (int k = 0; k < SCAN_PER_THREAD; ++k) t += (base + k < n) ? in[base + k] : 0u;
+  for (int k = 0; k < SCAN_PER_THREAD; ++k) t += v[k];
   uint32_t total;
   (void)block_scan_excl(t, s, &total);
   if (tid == 0) {
@@ -65,52 +97,60 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint3
   }
   __syncthreads();
   if (!last) return;
-  // the last tile: exclusive offsets of all tiles, SCAN_MAX_TILES at most
+  // the last tile: exclusive offsets of all tiles (SCAN_MAX_TILES at most),
+  // SCAN_BLOCK at a time
   __threadfence();
-  constexpr int PER = SCAN_MAX_TILES / SCAN_BLOCK;
-  uint32_t v[PER];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    uint32_t q = tid * PER + k;
-    v[k] = q < ntiles ? __hip_atomic_load(&tile[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-    sum += v[k];
-  }
-  uint32_t all;
-  uint32_t run = block_scan_excl(sum, s, &all);
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    uint32_t q = tid * PER + k;
-    if (q < ntiles) tile[q] = run;
-    run += v[k];
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < ntiles; c0 += SCAN_BLOCK) {
+    const uint32_t q = c0 + tid;
+    const uint32_t x = q < ntiles ? __hip_atomic_load(&tile[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    uint32_t sum;
+    const uint32_t ex = block_scan_excl(x, s, &sum);
+    if (q < ntiles) tile[q] = carry + ex;
+    carry += sum;
   }
   if (tid == 0) {
-    tile[ntiles] = all;
+    tile[ntiles] = carry;
     *done = 0u;  // ready for the next scan on this stream
   }
 }
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32_t n, uint32_t ntiles,
                                                            const uint32_t* tiles) {
-  __shared__ uint32_t s[SCAN_BLOCK];
+  __shared__ uint32_t s[SCAN_BLOCK / 64];
   const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
   uint32_t* __restrict__ out = jobs.out[blockIdx.y];
   const uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_TILE + tid * SCAN_PER_THREAD;
   uint32_t v[SCAN_PER_THREAD];
+  load_run(in, base, n, v);
   uint32_t t = 0;
 #pragma unroll
-  for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0u;
-    t += v[k];
-  }
+  for (int k = 0; k < SCAN_PER_THREAD; ++k) t += v[k];
   uint32_t total;
   uint32_t run = tile[blockIdx.x] + block_scan_excl(t, s, &total);
+  if (base + SCAN_PER_THREAD <= n && ((reinterpret_cast<uintptr_t>(out + base) & 15u) == 0u)) {
+    uint4* p = reinterpret_cast<uint4*>(out + base);
 #pragma unroll
-  for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
+    for (int k = 0; k < SCAN_PER_THREAD / 4; ++k) {
+      uint4 q;
+      q.x = run;
+      run += v[4 * k];
+      q.y = run;
+      run += v[4 * k + 1];
+      q.z = run;
+      run += v[4 * k + 2];
+      q.w = run;
+      run += v[4 * k + 3];
+      p[k] = q;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+      if (base + k < n) out[base + k] = run;
+      run += v[k];
+    }
   }
   if (blockIdx.x == ntiles - 1 && tid == 0) out[n] = tile[ntiles];
 }

@@ -155,6 +155,29 @@ def test_batched_scenes_equal_independent_runs(gpu_available):
         off += len(so)
 
 
+def test_large_context_equals_small_bitwise(gpu_available):
+    """Over 2^19 bodies in one context, so the hash grid's bucket scan spans
+    more than one round of tile offsets: the first scenes step exactly as in
+    a context of their own."""
+    parts, allsc = scenes.batched_drops(n_scenes=4700, nmin=8, nmax=512, seed=5)
+    assert len(allsc["shape"]) > 2 ** 19
+    few = scenes.concat(parts[:4])
+    few_ids = np.concatenate([np.full(len(p["shape"]), i, np.uint32) for i, p in enumerate(parts[:4])])
+    big, small = AvbdEngine(), AvbdEngine()
+    try:
+        big.load(allsc, scene_ids=allsc["scene_id"])
+        small.load(few, scene_ids=few_ids)
+        for _ in range(30):
+            big.step_n(1)
+            small.step_n(1)
+        k = len(few["shape"])
+        assert big.stats()["pairs"] > 0
+        assert np.array_equal(big.state()[:k].view(np.uint32), small.state().view(np.uint32))
+    finally:
+        big.close()
+        small.close()
+
+
 def test_gpu_run_to_run_bitwise(gpu_available):
     s = scenes.box_drop_scene(300, np.random.Generator(np.random.PCG64(13)))
     outs = []
