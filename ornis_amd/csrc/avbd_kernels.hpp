@@ -565,13 +565,16 @@ __global__ void __launch_bounds__(TPB) k_toi_apply(Dev d) {
 }
 
 // Apply pending wakes (flag_a) and clear them.
-__global__ void k_apply_wake(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void apply_wake_body(const Dev& d, uint32_t h) {
   if (d.b.flag_a[h]) {
     d.b.flag_a[h] = 0;
     wake_body_dev(d, h);
   }
+}
+__global__ void k_apply_wake(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  apply_wake_body(d, h);
 }
 
 // ---------------------------------------------------------------------------
@@ -1112,9 +1115,7 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
 }
 
 // wake_joint_motion (sleep.rs:107-137): per-body part.
-__global__ void k_awake_init(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void awake_init_body(const Dev& d, uint32_t h) {
   uint8_t aw = 0;
   if (d.b.type[h] == BT_DYNAMIC) {
     float4 v = d.b.vel[h], w = d.b.angvel[h], t = d.b.torque[h];
@@ -1127,6 +1128,11 @@ __global__ void k_awake_init(Dev d) {
   }
   d.b.flag_b[h] = aw;
 }
+__global__ void k_awake_init(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  awake_init_body(d, h);
+}
 __global__ void k_awake_joints(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
@@ -1136,25 +1142,30 @@ __global__ void k_awake_joints(Dev d) {
     if (d.b.type[ab.y] == BT_DYNAMIC) d.b.flag_b[ab.y] = 1;
   }
 }
+__device__ __forceinline__ void awake_apply_body(const Dev& d, uint32_t h) {
+  if (d.b.flag_b[h] && d.b.asleep[h]) wake_body_dev(d, h);
+}
 __global__ void k_awake_apply(Dev d) {
   uint32_t h = gid();
   if (h >= d.n) return;
-  if (d.b.flag_b[h] && d.b.asleep[h]) wake_body_dev(d, h);
+  awake_apply_body(d, h);
 }
 
 // External wake (mod.rs:984-992).
+__device__ __forceinline__ void external_wake_body(const Dev& d, uint32_t h) {
+  if (!d.b.asleep[h]) return;
+  if (len3(xyz(d.b.vel[h])) > K_SLEEP_LIN || len3(xyz(d.b.angvel[h])) > K_SLEEP_ANG) wake_body_dev(d, h);
+}
 __global__ void k_external_wake(Dev d) {
   uint32_t h = gid();
-  if (h >= d.n || !d.b.asleep[h]) return;
-  if (len3(xyz(d.b.vel[h])) > K_SLEEP_LIN || len3(xyz(d.b.angvel[h])) > K_SLEEP_ANG) wake_body_dev(d, h);
+  if (h >= d.n) return;
+  external_wake_body(d, h);
 }
 
 // ---------------------------------------------------------------------------
 // inertial targets + warm-started positions (mod.rs:993-1049)
 // ---------------------------------------------------------------------------
-__global__ void k_inertial(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void inertial_body(const Dev& d, uint32_t h) {
   float4 p = d.b.pos[h];
   float4 q = d.b.rot[h];
   float4 p0, q0;
@@ -1187,6 +1198,41 @@ __global__ void k_inertial(Dev d) {
   }
   f3 dr = quat_diff_vec(asq(q), asq(q0));
   d.b.drot[h] = pk(dr, 0.0f);
+}
+__global__ void k_inertial(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  inertial_body(d, h);
+}
+
+// Scenes without joints: the wake passes after the narrowphase (votes of
+// new contacts, then with sleepers about, joint-free motion and external
+// velocities) and the inertial targets, all per body, in one launch.
+__global__ void k_wake_inertial(Dev d, uint32_t may_sleep) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  apply_wake_body(d, h);
+  if (may_sleep) {
+    awake_init_body(d, h);
+    awake_apply_body(d, h);
+    external_wake_body(d, h);
+  }
+  inertial_body(d, h);
+}
+
+// Scenes without joints: the step's opening wake pass (with sleepers about)
+// and k_begin's per-body work in one launch.
+__global__ void __launch_bounds__(TPB) k_wake_begin(Dev d, uint32_t may_sleep) {
+  uint32_t h = gid();
+  uint8_t fast = 0;
+  if (h < d.n) {
+    if (may_sleep) {
+      awake_init_body(d, h);
+      awake_apply_body(d, h);
+    }
+    fast = begin_body(d, h);
+  }
+  block_add(&d.counters[CTR_FAST], fast);
 }
 
 __global__ void k_joint_prep(Dev d) {
@@ -1602,13 +1648,16 @@ __global__ void k_sb_pack(Dev d) {
   rec[2] = d.b.pos0[h];
   rec[3] = d.b.drot[h];
 }
-__global__ void k_sb_unpack(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void sb_unpack_body(const Dev& d, uint32_t h) {
   const float4* rec = d.sb + 4 * (size_t)h;
   d.b.pos[h] = rec[0];
   d.b.rot[h] = rec[1];
   d.b.drot[h] = rec[3];
+}
+__global__ void k_sb_unpack(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  sb_unpack_body(d, h);
 }
 
 // ---------------------------------------------------------------------------
@@ -2358,9 +2407,7 @@ __global__ void k_dual_joint(Dev d) {
 // ---------------------------------------------------------------------------
 // finish: BDF1 (mod.rs:1062-1078) and sleep (mod.rs:1088-1122)
 // ---------------------------------------------------------------------------
-__global__ void k_bdf1(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void bdf1_body(const Dev& d, uint32_t h) {
   float4 p = d.b.pos[h];
   d.b.supported[h] = 0;
   if (!(d.b.type[h] == BT_DYNAMIC && p.w > 0.0f) || d.b.asleep[h]) return;
@@ -2372,6 +2419,11 @@ __global__ void k_bdf1(Dev d) {
   d.b.vel[h] = pk(v, v4.w);
   d.b.angvel[h] = pk(w, w4.w);
   d.b.rot[h] = pkq(qnormalize(q));
+}
+__global__ void k_bdf1(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  bdf1_body(d, h);
 }
 
 __global__ void k_supported(Dev d) {
@@ -2396,6 +2448,16 @@ __global__ void k_ready_init(Dev d) {
   if (h >= d.n) return;
   d.b.flag_b[h] = 1;
 }
+
+// Scenes without joints: the solver records back to the bodies, BDF1 and
+// the sleep readiness flag, per body in one launch.
+__global__ void k_finish_bodies(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  sb_unpack_body(d, h);
+  bdf1_body(d, h);
+  d.b.flag_b[h] = 1;
+}
 __global__ void k_ready_joints(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
@@ -2407,9 +2469,7 @@ __global__ void k_ready_joints(Dev d) {
 }
 
 // Sleep timers; flag_a = freeze.
-__global__ void k_sleep_timer(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n) return;
+__device__ __forceinline__ void sleep_timer_body(const Dev& d, uint32_t h) {
   uint8_t asl = d.b.asleep[h];
   d.b.flag_a[h] = asl;
   if (!(d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f) || asl) return;
@@ -2424,6 +2484,11 @@ __global__ void k_sleep_timer(Dev d) {
   } else {
     d.b.timer[h] = 0.0f;
   }
+}
+__global__ void k_sleep_timer(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  sleep_timer_body(d, h);
 }
 
 // sleep_body (mod.rs:822-828) where frozen; refresh the driver baseline
@@ -2446,5 +2511,16 @@ __global__ void __launch_bounds__(TPB) k_sleep_apply(Dev d) {
   uint32_t h = gid();
   bool asleep = false;
   if (h < d.n) asleep = sleep_apply_body(d, h);
+  block_add(&d.counters[CTR_SLEEPING], asleep ? 1u : 0u);
+}
+// Scenes without joints (no readiness to propagate): timer and sleep in
+// one launch.
+__global__ void __launch_bounds__(TPB) k_sleep_bodies(Dev d) {
+  uint32_t h = gid();
+  bool asleep = false;
+  if (h < d.n) {
+    sleep_timer_body(d, h);
+    asleep = sleep_apply_body(d, h);
+  }
   block_add(&d.counters[CTR_SLEEPING], asleep ? 1u : 0u);
 }

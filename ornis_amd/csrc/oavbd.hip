@@ -500,7 +500,6 @@ static void narrow_and_merge(oavbd_ctx* c) {
   ensure_points(c, d.npts);
   ensure_incidence(c, d.npairs, d.npts);
   if (nc) k_merge<<<blocks((size_t)nc * K_MAX_POINTS), TPB, 0, c->st>>>(d, nc);
-  k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d);
 }
 
 static void incidence_and_colour(oavbd_ctx* c) {
@@ -620,8 +619,14 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   if (c->joints_dirty) upload_joints(c);
   record(c, 0);
   zero_u32(c, d.counters, CTR_COUNT);
-  if (c->may_sleep) wake_joint_motion(c);
-  k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
+  // per-body passes are fused where no joint propagation sits between them
+  const uint32_t may_sleep = c->may_sleep ? 1u : 0u;
+  if (d.njoints == 0) {
+    k_wake_begin<<<blocks(n), TPB, 0, c->st>>>(d, may_sleep);
+  } else {
+    if (may_sleep) wake_joint_motion(c);
+    k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
+  }
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
   k_toi_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d);
@@ -650,11 +655,16 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   }
   broadphase(c);
   narrow_and_merge(c);
-  if (c->may_sleep) {
-    wake_joint_motion(c);
-    k_external_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+  if (d.njoints == 0) {
+    k_wake_inertial<<<blocks(n), TPB, 0, c->st>>>(d, may_sleep);
+  } else {
+    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+    if (may_sleep) {
+      wake_joint_motion(c);
+      k_external_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+    }
+    k_inertial<<<blocks(n), TPB, 0, c->st>>>(d);
   }
-  k_inertial<<<blocks(n), TPB, 0, c->st>>>(d);
   if (d.njoints) k_joint_prep<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   if (d.npairs) k_pair_prep<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   incidence_and_colour(c);
@@ -692,18 +702,22 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     nd_ev += 2;
   }
   record(c, 3);
-  k_sb_unpack<<<blocks(n), TPB, 0, c->st>>>(d);
-  k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
-  uint32_t ns = std::max(d.npairs, d.njoints);
-  if (ns) k_supported<<<blocks(ns), TPB, 0, c->st>>>(d);
-  k_ready_init<<<blocks(n), TPB, 0, c->st>>>(d);
-  if (d.njoints) {
+  if (d.njoints == 0) {
+    k_finish_bodies<<<blocks(n), TPB, 0, c->st>>>(d);
+    if (d.npairs) k_supported<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+    k_sleep_bodies<<<blocks(n), TPB, 0, c->st>>>(d);
+  } else {
+    k_sb_unpack<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
+    uint32_t ns = std::max(d.npairs, d.njoints);
+    if (ns) k_supported<<<blocks(ns), TPB, 0, c->st>>>(d);
+    k_ready_init<<<blocks(n), TPB, 0, c->st>>>(d);
     k_ready_joints<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
     propagate(c, d.b.flag_b, 0);
+    k_sleep_timer<<<blocks(n), TPB, 0, c->st>>>(d);
+    propagate(c, d.b.flag_a, 0);
+    k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   }
-  k_sleep_timer<<<blocks(n), TPB, 0, c->st>>>(d);
-  if (d.njoints) propagate(c, d.b.flag_a, 0);
-  k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 4);
   c->stats.pairs = d.npairs;
   c->stats.points = d.npts;
