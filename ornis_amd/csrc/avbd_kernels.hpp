@@ -655,14 +655,35 @@ __global__ void __launch_bounds__(TPB) k_bp_count(Dev d, uint32_t* cnt) {
     const float ra = d.b.shape[a].w;
     const uint32_t sc = d.b.scene[a];
     const int cx = cell_of(p.x, d.g.cell), cy = cell_of(p.y, d.g.cell), cz = cell_of(p.z, d.g.cell);
-    for (uint32_t ci = s; ci < 27; ci += BP_LANES) {
-      const uint32_t bk = bucket_of(cx - 1 + (int)(ci / 9), cy - 1 + (int)(ci / 3 % 3), cz - 1 + (int)(ci % 3), sc,
-                                    d.g.nbuckets);
-      bool dup = false;
-      for (uint32_t cj = 0; cj < ci; ++cj)
-        dup = dup || bucket_of(cx - 1 + (int)(cj / 9), cy - 1 + (int)(cj / 3 % 3), cz - 1 + (int)(cj % 3), sc,
-                               d.g.nbuckets) == bk;
-      if (dup) continue;
+    // lane s holds the buckets of cells s + BP_LANES m; the group's lanes
+    // exchange them once to find the cells whose bucket an earlier cell of
+    // the 27 already scans
+    constexpr int NS = (27 + BP_LANES - 1) / BP_LANES;
+    uint32_t bkm[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const uint32_t ci = s + BP_LANES * m;
+      bkm[m] = ci < 27u ? bucket_of(cx - 1 + (int)(ci / 9), cy - 1 + (int)(ci / 3 % 3), cz - 1 + (int)(ci % 3), sc,
+                                    d.g.nbuckets)
+                        : 0xffffffffu;
+    }
+    bool dupm[NS];
+#pragma unroll
+    for (int m = 0; m < NS; ++m) dupm[m] = false;
+#pragma unroll
+    for (int t = 0; t < BP_LANES; ++t)
+#pragma unroll
+      for (int m2 = 0; m2 < NS; ++m2) {
+        const uint32_t o = __shfl(bkm[m2], t, BP_LANES);  // bucket of cell t + BP_LANES m2
+        const uint32_t cj = (uint32_t)t + BP_LANES * m2;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) dupm[m] = dupm[m] || (cj < s + BP_LANES * m && o == bkm[m]);
+      }
+#pragma unroll
+    for (int m = 0; m < NS; ++m) {
+      const uint32_t ci = s + BP_LANES * m;
+      if (ci >= 27u || dupm[m]) continue;
+      const uint32_t bk = bkm[m];
       const uint32_t s0 = d.g.start[bk], c0 = d.g.cnt[bk];
       for (uint32_t k = 0; k < c0; ++k) {
         const float4 q = d.g.ipos[s0 + k];
