@@ -1,32 +1,101 @@
-// avbd_group.hpp — box-box distance evaluated by a group of G lanes of one
-// wavefront (G a power of two, 2..64).
+// avbd_group.hpp — box-box separating axes and closest features evaluated
+// by a group of G lanes of one wavefront (G = 16, 32 or 64).
 //
 // The serial separating_axis / feature_distance (avbd_geom.hpp) scan a
 // fixed list of candidates (15 axes; 8 + 8 corner-box and 144 edge-edge
-// tests) and keep the first best one.  Here lane s of the group evaluates
-// candidates s, s + G, s + 2G, ... with the same arithmetic, keeps its own
-// first best, and the group reduces (value, candidate index)
-// lexicographically, so the winner — and every bit of the result — is the
-// serial one.  Every lane of a group ends with the same result; all lanes
-// of a group must execute these functions together.
+// tests) and keep the first best one.  Here each lane evaluates some of the
+// candidates with the same arithmetic and keeps its own first best, and the
+// group picks (value, candidate index) lexicographically, so the winner —
+// and every bit of the result — is the serial one.  Every lane of a group
+// ends with the same result; all lanes of a group must execute these
+// functions together.
 #pragma once
 
 #include "avbd_geom.hpp"
 
 namespace grp {
 
-template <int G>
-__device__ __forceinline__ float xorf(float v, int m) {
-  return __shfl_xor(v, m, G);
-}
-template <int G>
-__device__ __forceinline__ int xori(int v, int m) {
-  return __shfl_xor(v, m, G);
-}
-// v of lane `src` (0..G-1) of the calling lane's group.
+// v of lane `src` (0..G-1) of the calling lane's group.  For a whole
+// wavefront src is uniform (every caller takes it from a group pick), so
+// a lane read replaces the permute.
 template <int G>
 __device__ __forceinline__ f3 from_lane(f3 v, int src) {
-  return f3{__shfl(v.x, src, G), __shfl(v.y, src, G), __shfl(v.z, src, G)};
+  if constexpr (G == 64) {
+    return f3{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), src)),
+              __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src)),
+              __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), src))};
+  } else {
+    return f3{__shfl(v.x, src, G), __shfl(v.y, src, G), __shfl(v.z, src, G)};
+  }
+}
+
+// Group reductions on the DPP paths (no LDS round trip), for groups of 16,
+// 32 or 64 lanes aligned in the wavefront.  Within each row of 16 lanes,
+// shifts by 1, 2, 4 and 8 leave the row's result in its lane 15 (a lane
+// whose source lies outside the row keeps its own value); the row
+// broadcasts then carry it to lane 31 / 63 for groups of 32 / 64, and every
+// lane reads its group's result from that lane.  Only selections (min, max,
+// or, lexicographic picks) are reduced, so the grouping cannot change a bit
+// of the result.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xf, false);
+}
+// Every lane's copy of its group's result lane (15, 31 or 63 of the group).
+template <int G>
+__device__ __forceinline__ int group_last(int v) {
+  if constexpr (G == 64) {
+    return __builtin_amdgcn_readlane(v, 63);
+  } else if constexpr (G == 32) {
+    const int a = __builtin_amdgcn_readlane(v, 31), b = __builtin_amdgcn_readlane(v, 63);
+    return (__lane_id() & 32u) ? b : a;
+  } else {
+    static_assert(G == 16, "groups of 16, 32 or 64 lanes");
+    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
+    const int r2 = __builtin_amdgcn_readlane(v, 47), r3 = __builtin_amdgcn_readlane(v, 63);
+    const uint32_t row = __lane_id() >> 4;
+    return row == 0u ? r0 : row == 1u ? r1 : row == 2u ? r2 : r3;
+  }
+}
+// Pick over the group of (v, i, l) triples: better(ov, oi, v, i) tells
+// whether the other triple wins.  Every lane ends with the group's pick.
+template <int G, class Better>
+__device__ __forceinline__ void group_pick(int& v, int& i, int& l, Better better) {
+#define OAVBD_PICK(CTRL, ROWS)                                                                    \
+  {                                                                                              \
+    const int ov = dpp_mov<CTRL, ROWS>(v), oi = dpp_mov<CTRL, ROWS>(i), ol = dpp_mov<CTRL, ROWS>(l); \
+    if (better(ov, oi, v, i)) {                                                                  \
+      v = ov;                                                                                    \
+      i = oi;                                                                                    \
+      l = ol;                                                                                    \
+    }                                                                                            \
+  }
+  OAVBD_PICK(0x111, 0xf)
+  OAVBD_PICK(0x112, 0xf)
+  OAVBD_PICK(0x114, 0xf)
+  OAVBD_PICK(0x118, 0xf)
+  if constexpr (G >= 32) OAVBD_PICK(0x142, 0xa)  // row_bcast:15 into rows 1 and 3
+  if constexpr (G == 64) OAVBD_PICK(0x143, 0xc)  // row_bcast:31 into rows 2 and 3
+#undef OAVBD_PICK
+  v = group_last<G>(v);
+  i = group_last<G>(i);
+  l = group_last<G>(l);
+}
+// Minimum (a NaN loses to any number) and "any" over the group.
+template <int G>
+__device__ __forceinline__ float group_fmin(float x) {
+  int v = __float_as_int(x), i = 0, l = 0;
+  group_pick<G>(v, i, l, [](int ov, int, int cv, int) {
+    const float o = __int_as_float(ov), c = __int_as_float(cv);
+    return o < c || (c != c && o == o);
+  });
+  return __int_as_float(v);
+}
+template <int G>
+__device__ __forceinline__ bool group_any(bool x) {
+  int v = x ? 1 : 0, i = 0, l = 0;
+  group_pick<G>(v, i, l, [](int ov, int, int cv, int) { return ov > cv; });
+  return v != 0;
 }
 
 // separating_axis over the group.  With sweep != nullptr it also tells
@@ -72,29 +141,20 @@ __device__ float sat(const shp& a, f3 ha, const shp& b, f3 hb, f3* normal_out, i
       if (lo - ea - eb - slack > reach) clear = true;
     }
   }
-  if (sweep) {
-    int c = clear ? 1 : 0;
-    for (int m = G / 2; m > 0; m >>= 1) c |= xori<G>(c, m);
-    *sweep = c != 0;
-  }
+  if (sweep) *sweep = group_any<G>(clear);
   // larger gap wins; equal gaps go to the earlier axis
-  float g = best;
-  int k = bk;
-  for (int m = G / 2; m > 0; m >>= 1) {
-    float og = xorf<G>(g, m);
-    int ok = xori<G>(k, m);
-    if (og > g || (og == g && ok < k)) {
-      g = og;
-      k = ok;
-    }
-  }
+  int gv = __float_as_int(best), k = bk, src = s;
+  group_pick<G>(gv, k, src, [](int og, int ok, int cg, int ck) {
+    const float o = __int_as_float(og), c = __int_as_float(cg);
+    return o > c || (o == c && ok < ck);
+  });
   if (k != 99) {
     bn = from_lane<G>(bn, k % G);
   } else {
     bn = f3{0.0f, 1.0f, 0.0f};
   }
   *normal_out = bn;
-  return g;
+  return __int_as_float(gv);
 }
 
 // The calling lane's group's bits of a wave-wide ballot (bit k: lane k of
@@ -142,8 +202,7 @@ __device__ dist_t feature_pruned(const shp& a, f3 ha, const shp& b, f3 hb, int s
       by = y;
     }
   }
-  float U = best;
-  for (int m = G / 2; m > 0; m >>= 1) U = fminf(U, xorf<G>(U, m));
+  const float U = group_fmin<G>(best);
   const float pa = fmaxf(fmaxf(fabsf(a.pos.x), fabsf(a.pos.y)), fabsf(a.pos.z));
   const float pb = fmaxf(fmaxf(fabsf(b.pos.x), fabsf(b.pos.y)), fabsf(b.pos.z));
   const float eps = 2e-5f * (1.0f + fmaxf(pa, pb) + 2.0f * (ha.x + ha.y + ha.z + hb.x + hb.y + hb.z));
@@ -181,51 +240,13 @@ __device__ dist_t feature_pruned(const shp& a, f3 ha, const shp& b, f3 hb, int s
     }
   }
   // smaller squared distance wins; ties go to the earlier candidate
-  float q = best;
-  int c = bc, src = s;
-  for (int m = G / 2; m > 0; m >>= 1) {
-    const float oq = xorf<G>(q, m);
-    const int oc = xori<G>(c, m), os = xori<G>(src, m);
-    if (oq < q || (oq == q && oc < c)) {
-      q = oq;
-      c = oc;
-      src = os;
-    }
-  }
+  int qv = __float_as_int(best), c = bc, src = s;
+  group_pick<G>(qv, c, src, [](int oq, int oc, int cq, int cc) {
+    const float o = __int_as_float(oq), q = __int_as_float(cq);
+    return o < q || (o == q && oc < cc);
+  });
   if (c == 0x7fffffff) return dist_t{__builtin_inff(), a.pos, b.pos};
-  return dist_t{sqrtf(q), from_lane<G>(bx, src), from_lane<G>(by, src)};
-}
-
-// feature_distance over the group.
-template <int G>
-__device__ dist_t feature(const shp& a, f3 ha, const shp& b, f3 hb, int s) {
-  const box_axes_t ea = box_axes(ha, a.rot), eb = box_axes(hb, b.rot);
-  float best = __builtin_inff();
-  int bc = 0x7fffffff;
-  f3 bx = a.pos, by = b.pos;
-  for (int c = s; c < 160; c += G) {
-    f3 x, y;
-    float sq = feature_candidate(a, ha, ea, b, hb, eb, c, &x, &y);
-    if (sq < best) {
-      best = sq;
-      bc = c;
-      bx = x;
-      by = y;
-    }
-  }
-  // smaller squared distance wins; ties go to the earlier candidate
-  float q = best;
-  int c = bc;
-  for (int m = G / 2; m > 0; m >>= 1) {
-    float oq = xorf<G>(q, m);
-    int oc = xori<G>(c, m);
-    if (oq < q || (oq == q && oc < c)) {
-      q = oq;
-      c = oc;
-    }
-  }
-  if (c == 0x7fffffff) return dist_t{__builtin_inff(), a.pos, b.pos};
-  return dist_t{sqrtf(q), from_lane<G>(bx, c % G), from_lane<G>(by, c % G)};
+  return dist_t{sqrtf(__int_as_float(qv)), from_lane<G>(bx, src), from_lane<G>(by, src)};
 }
 
 // box_box_from_sat over the group (G >= 16: pruned feature scan).
