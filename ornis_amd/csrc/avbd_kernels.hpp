@@ -1526,6 +1526,43 @@ __global__ void __launch_bounds__(COLOUR_PASS_TPB) k_colour_pass(Dev d, uint32_t
   if (threadIdx.x == 0 && s_def) atomicAdd(deferred, s_def);
 }
 
+// One pass over a list of uncoloured bodies (round r of the ring: reads
+// front list r & 1 with counter r % 3, lists the bodies still waiting into
+// list (r + 1) & 1 / counter (r + 1) % 3, and clears counter (r + 2) % 3 for
+// the next round), CL lanes per body over a grid-stride loop: the list's
+// length is only known on the device.  *deferred receives the number left.
+__global__ void __launch_bounds__(COLOUR_PASS_TPB) k_colour_pass_list(Dev d, uint32_t round, uint32_t* deferred) {
+  constexpr uint32_t PER = COLOUR_PASS_TPB / CL;  // bodies per workgroup and sweep of the loop
+  __shared__ uint32_t s_n, s_base, s_def;
+  const size_t n = d.n;
+  const uint32_t* in_list = d.front + (size_t)(round & 1u) * n;
+  uint32_t* out_list = d.front + (size_t)((round + 1u) & 1u) * n;
+  uint32_t* out_n = d.fcnt + (round + 1u) % 3u;
+  const uint32_t nin = d.fcnt[round % 3u];
+  if (gid() == 0) d.fcnt[(round + 2u) % 3u] = 0u;
+  if (threadIdx.x == 0) s_def = 0u;
+  const bool lead = threadIdx.x % CL == 0;
+  // the loop bound is uniform over the workgroup, so its barriers are too;
+  // the bodies left are listed with one atomic per workgroup and sweep
+  for (uint32_t b0 = blockIdx.x * PER; b0 < nin; b0 += gridDim.x * PER) {
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    const uint32_t t = b0 + threadIdx.x / CL;
+    const bool act = t < nin;
+    const uint32_t h = act ? in_list[t] : 0u;
+    const bool waiting = colour_body(d, h, act);
+    const uint32_t slot = waiting && lead ? atomicAdd(&s_n, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0 && s_n) {
+      s_base = atomicAdd(out_n, s_n);
+      s_def += s_n;
+    }
+    __syncthreads();
+    if (waiting && lead) out_list[s_base + slot] = h;
+  }
+  if (threadIdx.x == 0 && s_def) atomicAdd(deferred, s_def);
+}
+
 // The swept bodies still uncoloured, listed into `list` / *list_n (one
 // atomic per workgroup).
 __global__ void __launch_bounds__(TPB) k_colour_collect(Dev d, uint32_t* list, uint32_t* list_n) {

@@ -90,6 +90,7 @@ struct oavbd_ctx {
   bool may_sleep = false;  // bodies may be asleep (wake passes needed)
   uint32_t solved = 0;  // bodies coloured (swept) this step
   int grid_rounds = 8;  // colouring rounds launched over the grid before the one-workgroup tail
+  uint32_t colour_left1 = 0xffffffffu;  // bodies left after the first colouring pass, previous step
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   int lanes = 32;   // primal launch shape: lanes per body
@@ -521,10 +522,16 @@ static void incidence_and_colour(oavbd_ctx* c) {
   // colour offsets and lists are made on the device meanwhile, and the host
   // waits for the check (colour_wait) only once the rest of the set-up is
   // queued behind it.
+  // The first pass runs over all bodies; the bodies it leaves are listed,
+  // and the next passes run over the list of the previous pass's leftovers
+  // (grid sized from the previous step's leftovers after the first pass;
+  // the passes loop over whatever the list holds).
   const int grid = c->grid_rounds;
-  for (int round = 0; round < grid; ++round)
-    k_colour_pass<<<blocks((size_t)n * CL, COLOUR_PASS_TPB), COLOUR_PASS_TPB, 0, c->st>>>(d, d.remain + round);
-  k_colour_collect<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)(grid & 1) * n, d.fcnt + grid % 3);
+  k_colour_pass<<<blocks((size_t)n * CL, COLOUR_PASS_TPB), COLOUR_PASS_TPB, 0, c->st>>>(d, d.remain);
+  k_colour_collect<<<blocks(n), TPB, 0, c->st>>>(d, d.front + (size_t)1 * n, d.fcnt + 1);
+  const size_t lgrid = std::max<size_t>(blocks((size_t)std::min(c->colour_left1, n) * CL, COLOUR_PASS_TPB), 1);
+  for (int round = 1; round < grid; ++round)
+    k_colour_pass_list<<<lgrid, COLOUR_PASS_TPB, 0, c->st>>>(d, (uint32_t)round, d.remain + round);
   k_colour_tail<<<1, COLOUR_TAIL_TPB, 0, c->st>>>(d, (uint32_t)grid);
   k_colour_classes<<<blocks(n), TPB, 0, c->st>>>(d);
   {
@@ -554,6 +561,7 @@ static void colour_wait(oavbd_ctx* c) {
   // no grid pass got down to few uncoloured bodies: start the tail later
   c->grid_rounds = std::max(1, std::min(next_grid >= 0 ? next_grid : grid + 2, COLOUR_RING - 16));
   c->stats.colour_rounds = (uint32_t)rounds;
+  c->colour_left1 = std::max<uint32_t>(head[0], 1024u);
   uint32_t ncol = head[CSTAT_MAX];
   c->colour_cnt_h.assign(ncol, 0);
   c->colour_off_h.assign(ncol + 1, 0);
