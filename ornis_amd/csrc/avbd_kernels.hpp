@@ -1272,8 +1272,10 @@ __global__ void k_joint_prep(Dev d) {
 
 // Per-pair step constants: friction frame, separated-damper stiffness
 // (assembly.rs:129-174), step-start violation c0 per point (joints_ext.rs:21-32).
-__global__ void k_pair_prep(Dev d) {
-  uint32_t p = gid();
+// PREP_LANES lanes per pair, lane s taking points s, s + PREP_LANES, ...
+__global__ void __launch_bounds__(TPB) k_pair_prep(Dev d) {
+  const uint32_t p = gid() / PREP_LANES;
+  const uint32_t s = threadIdx.x % PREP_LANES;
   if (p >= d.npairs) return;
   uint2 ab = d.P.ab[p];
   float4 n4 = d.P.n(p);
@@ -1294,12 +1296,15 @@ __global__ void k_pair_prep(Dev d) {
       if (m_red > 0.0f) damp = fmin_(2.0f * m_red / (K_DT * K_DT) / npts, K_PEN_MAX);
     }
   }
-  d.P.t1(p) = pk(t1, damp);
-  d.P.t2(p) = pk(t2, 0.0f);
+  if (s == 0) {
+    d.P.t1(p) = pk(t1, damp);
+    d.P.t2(p) = pk(t2, 0.0f);
+  }
+  if (s >= pts.y) return;
   bool sphere = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
   float4 pa0 = d.b.pos0[ab.x], pb0 = d.b.pos0[ab.y];
   qf qa0 = asq(d.b.rot0[ab.x]), qb0 = asq(d.b.rot0[ab.y]);
-  for (uint32_t k = 0; k < pts.y; ++k) {
+  for (uint32_t k = s; k < pts.y; k += PREP_LANES) {
     uint32_t q = pts.x + k;
     float4 ra = d.Q.ra(q);
     float c0 = 0.0f;
@@ -1387,32 +1392,39 @@ __global__ void k_nbr_joint_fill(Dev d) {
   if (d.sw[ab.x] && d.sw[ab.y]) nbr_link(d, ab);
 }
 // Body h's point-sides in ascending pair index: pairs where h is b (a < h),
-// then the contiguous run where h is a.
-__global__ void k_inc_build(Dev d) {
-  uint32_t h = gid();
-  if (h >= d.n || !d.sw[h]) return;
-  uint32_t b0 = d.boff[h], b1 = d.boff[h + 1];
-  for (uint32_t i = b0 + 1; i < b1; ++i) {
-    uint32_t v = d.blist[i];
-    uint32_t j = i;
-    while (j > b0 && d.blist[j - 1] > v) {
-      d.blist[j] = d.blist[j - 1];
-      --j;
+// then the contiguous run where h is a.  INC_LANES lanes per body: lane s
+// places the b-side pairs s, s + INC_LANES, ... of h's (unordered) b-list
+// after the points of the b-side pairs of smaller index, and the a-run
+// pairs s, s + INC_LANES, ..., whose points follow each other in the point
+// array (they are numbered in pair order).
+__global__ void __launch_bounds__(TPB) k_inc_build(Dev d) {
+  const uint32_t h = gid() / INC_LANES;
+  const uint32_t s = threadIdx.x % INC_LANES;
+  if (h >= d.n || !d.sw[h]) return;  // whole groups
+  const uint32_t b0 = d.boff[h], nb = d.boff[h + 1] - b0;
+  const uint32_t w0 = d.inc_off[h];
+  uint32_t bpts = 0;
+  for (uint32_t i = s; i < nb; i += INC_LANES) {
+    const uint32_t p = d.blist[b0 + i];
+    uint32_t pre = 0;
+    for (uint32_t j = 0; j < nb; ++j) {
+      const uint32_t q = d.blist[b0 + j];
+      if (q < p) pre += d.P.pts[q].y;
     }
-    d.blist[j] = v;
+    const uint2 pts = d.P.pts[p];
+    const uint2 ab = d.P.ab[p];
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w0 + pre + k] = make_uint4((pts.x + k) | 0x80000000u, p, ab.x, ab.y);
+    bpts += pts.y;
   }
-  uint32_t w = d.inc_off[h];
-  for (uint32_t i = b0; i < b1; ++i) {
-    uint32_t p = d.blist[i];
-    uint2 pts = d.P.pts[p];
-    uint2 ab = d.P.ab[p];
-    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = make_uint4((pts.x + k) | 0x80000000u, p, ab.x, ab.y);
-  }
-  uint32_t a0 = d.acc_off[d.cand_off[h]], a1 = d.acc_off[d.cand_off[h + 1]];
-  for (uint32_t p = a0; p < a1; ++p) {
-    uint2 pts = d.P.pts[p];
-    uint2 ab = d.P.ab[p];
-    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w++] = make_uint4(pts.x + k, p, ab.x, ab.y);
+  for (int m = INC_LANES / 2; m > 0; m >>= 1) bpts += __shfl_xor(bpts, m, INC_LANES);
+  const uint32_t a0 = d.acc_off[d.cand_off[h]], a1 = d.acc_off[d.cand_off[h + 1]];
+  if (a0 >= a1) return;
+  const uint32_t x0 = d.P.pts[a0].x;
+  for (uint32_t p = a0 + s; p < a1; p += INC_LANES) {
+    const uint2 pts = d.P.pts[p];
+    const uint2 ab = d.P.ab[p];
+    const uint32_t w = w0 + bpts + (pts.x - x0);
+    for (uint32_t k = 0; k < pts.y; ++k) d.inc[w + k] = make_uint4(pts.x + k, p, ab.x, ab.y);
   }
 }
 

@@ -55,6 +55,8 @@
 static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
 #define TOI_LANES 64
+#define INC_LANES 8    // lanes per body building its incidence list
+#define PREP_LANES 4   // lanes per pair of the per-pair step constants
 #define TOI_GATE_LANES 16  // lanes per cast of the TOI gate (15 separating axes)
 #define NARROW_LANES 16    // lanes per box-box candidate of the narrowphase distance
 #ifndef BP_LANES

@@ -512,7 +512,7 @@ static void incidence_and_colour(oavbd_ctx* c) {
   scan3(c, d.ptcnt, d.inc_off, d.bcnt, d.boff, d.nbr_cnt, d.nbr_off, n);
   if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-  k_inc_build<<<blocks(n), TPB, 0, c->st>>>(d);
+  k_inc_build<<<blocks((size_t)n * INC_LANES), TPB, 0, c->st>>>(d);
   // Jones-Plassmann colouring in passes.  The first `grid` passes run over
   // all bodies, one launch each; the bodies still uncoloured after them are
   // listed and finished by one single-workgroup launch.  `grid` is the pass
@@ -674,7 +674,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     k_inertial<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   if (d.njoints) k_joint_prep<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-  if (d.npairs) k_pair_prep<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  if (d.npairs) k_pair_prep<<<blocks((size_t)d.npairs * PREP_LANES), TPB, 0, c->st>>>(d);
   incidence_and_colour(c);
   k_sb_pack<<<blocks(n), TPB, 0, c->st>>>(d);
   k_inertial_blocks<<<blocks(n), TPB, 0, c->st>>>(d);
