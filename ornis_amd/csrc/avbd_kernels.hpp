@@ -1029,15 +1029,15 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
 }
 
 // Merge (rows.rs:795-931) fused with the warm-start decay (mod.rs:1019-1040).
-// K_MAX_POINTS lanes per candidate: lane k merges fresh point k; lane j
+// K_MAX_POINTS lanes per new pair: lane k merges fresh point k; lane j
 // also holds old point j, whose anchor every lane tests (in order, so the
 // first match wins as in the serial scan); lane 0 writes the pair.
-__global__ void __launch_bounds__(TPB) k_merge(Dev d, uint32_t ncand) {
+__global__ void __launch_bounds__(TPB) k_merge(Dev d) {
   constexpr int MG = K_MAX_POINTS;
-  const uint32_t c = gid() / MG;
+  const uint32_t p = gid() / MG;
   const uint32_t k = gid() % MG;
-  if (c >= ncand || !d.C.acc[c]) return;  // whole groups
-  uint32_t p = d.acc_off[c];
+  if (p >= d.npairs) return;  // whole groups
+  const uint32_t c = d.pair_cand[p];
   uint32_t start = d.pt_off[c];
   uint32_t cnt = d.C.npts[c];
   uint2 ab = d.C.ab[c];

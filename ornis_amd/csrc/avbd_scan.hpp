@@ -72,6 +72,7 @@ __device__ __forceinline__ void load_run(const uint32_t* __restrict__ in, uint32
 struct ScanJobs {
   const uint32_t* in[SCAN_JOBS];
   uint32_t* out[SCAN_JOBS];
+  uint32_t* pick = nullptr;  // optional, job 0 of 0/1 flags: pick[out[i]] = i for every set flag
 };
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint32_t n, uint32_t ntiles,
@@ -130,6 +131,14 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32
   for (int k = 0; k < SCAN_PER_THREAD; ++k) t += v[k];
   uint32_t total;
   uint32_t run = tile[blockIdx.x] + block_scan_excl(t, s, &total);
+  if (jobs.pick && blockIdx.y == 0) {
+    uint32_t r = run;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+      if (v[k]) jobs.pick[r] = base + k;
+      r += v[k];
+    }
+  }
   if (base + SCAN_PER_THREAD <= n && ((reinterpret_cast<uintptr_t>(out + base) & 15u) == 0u)) {
     uint4* p = reinterpret_cast<uint4*>(out + base);
 #pragma unroll

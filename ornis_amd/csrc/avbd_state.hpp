@@ -229,6 +229,7 @@ struct Dev {
   uint32_t* toi_off;   // n+1
   uint2* toi_pairs;    // (mover, target) casts
   float4* toi_res;     // per cast: normal xyz, w = travel to the hit (< 0: none)
+  uint32_t* pair_cand;  // candidate of each new pair (from the acceptance scan)
   uint32_t* toi_live;  // box-box casts left by the first swept bound (CTR_TOI_LIVE of them)
   uint32_t* bp_cache;  // n x BP_CACHE candidates found by the counting pass
   uint32_t* toi_cache; // n x TOI_CACHE targets found by the counting pass

@@ -198,9 +198,10 @@ static void ensure_cand(oavbd_ctx* c, size_t need) {
   size_t cap = grow(need);
   Cands& C = c->d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
-  dfree(c->d.acc_off); dfree(c->d.pt_off); dfree(c->d.nlist);
+  dfree(c->d.acc_off); dfree(c->d.pt_off); dfree(c->d.nlist); dfree(c->d.pair_cand);
   dalloc(&C.ab, cap); dalloc(&C.acc, cap); dalloc(&C.npts, cap); dalloc(&C.disc0, cap); dalloc(&C.disc1, cap);
   dalloc(&c->d.nlist, cap);
+  dalloc(&c->d.pair_cand, cap);
   dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
   if (cap + 1 > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"candidate count exceeds the scan capacity"};
   c->cap_cand = cap;
@@ -288,9 +289,9 @@ static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
 }
 // Two or three arrays of the same length in one launch pair.
 static void scan2(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
-                  uint32_t n) {
+                  uint32_t n, uint32_t* pick0 = nullptr) {
   if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
-  ScanJobs js{{in0, in1, nullptr}, {out0, out1, nullptr}};
+  ScanJobs js{{in0, in1, nullptr}, {out0, out1, nullptr}, pick0};
   scan_tiles(c->ts, js, 2, n, c->st);
 }
 static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
@@ -485,7 +486,9 @@ static void narrow_and_merge(oavbd_ctx* c) {
     k_narrow_dist<NARROW_LANES><<<blocks((size_t)nc * NARROW_LANES), TPB, 0, c->st>>>(d, nc);
     k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
   }
-  scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc);
+  // the accepted candidates' flags scan to pair indices; the pair -> candidate
+  // map they also give lets the merge run over pairs only
+  scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc, d.pair_cand);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
   {
     PostJob j;
@@ -500,7 +503,7 @@ static void narrow_and_merge(oavbd_ctx* c) {
   ensure_pairs(c, d.npairs);
   ensure_points(c, d.npts);
   ensure_incidence(c, d.npairs, d.npts);
-  if (nc) k_merge<<<blocks((size_t)nc * K_MAX_POINTS), TPB, 0, c->st>>>(d, nc);
+  if (d.npairs) k_merge<<<blocks((size_t)d.npairs * K_MAX_POINTS), TPB, 0, c->st>>>(d);
 }
 
 static void incidence_and_colour(oavbd_ctx* c) {
