@@ -73,6 +73,10 @@ struct ScanJobs {
   const uint32_t* in[SCAN_JOBS];
   uint32_t* out[SCAN_JOBS];
   uint32_t* pick = nullptr;  // optional, job 0 of 0/1 flags: pick[out[i]] = i for every set flag
+  // optional, job 0: the total is also written to post[post_word] (pinned
+  // host memory) and then post[post_flag] = post_seq, for a host that polls
+  uint32_t* post = nullptr;
+  uint32_t post_word = 0, post_flag = 0, post_seq = 0;
 };
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint32_t n, uint32_t ntiles,
@@ -161,7 +165,14 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32
       run += v[k];
     }
   }
-  if (blockIdx.x == ntiles - 1 && tid == 0) out[n] = tile[ntiles];
+  if (blockIdx.x == ntiles - 1 && tid == 0) {
+    out[n] = tile[ntiles];
+    if (jobs.post && blockIdx.y == 0) {
+      jobs.post[jobs.post_word] = tile[ntiles];
+      __threadfence_system();
+      __hip_atomic_store(jobs.post + jobs.post_flag, jobs.post_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
 // Device state of the two-launch scan (one per stream).

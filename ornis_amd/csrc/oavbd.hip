@@ -354,6 +354,25 @@ static uint32_t read_u32(oavbd_ctx* c, const uint32_t* dptr) {
   return c->counters_h[CTR_COUNT];
 }
 
+// Scan whose total the host waits for: the scan's last workgroup posts it.
+static uint32_t scan_total(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
+  if (n == 0) {
+    scan(c, in, out, n);
+    return read_u32(c, out);
+  }
+  if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
+  ScanJobs js{};
+  js.in[0] = in;
+  js.out[0] = out;
+  js.post = c->counters_hd;
+  js.post_word = CTR_COUNT;
+  js.post_flag = POST_FLAG;
+  js.post_seq = ++c->post_seq;
+  scan_tiles(c->ts, js, 1, n, c->st);
+  post_wait(c, js.post_seq);
+  return c->counters_h[CTR_COUNT];
+}
+
 // ---------------------------------------------------------------------------
 // static joint structures: partner CSR (no-collide), incidence CSR (rows,
 // colouring), dynamic-joint components (sleep/wake propagation)
@@ -464,8 +483,7 @@ static void broadphase(oavbd_ctx* c) {
     scan(c, d.lflag, d.lrank, d.g.lflat);  // one scan over all large bodies' flag rows
     k_bp_large_count<<<blocks(d.g.nlarge), TPB, 0, c->st>>>(d, c->cand_cnt);
   }
-  scan(c, c->cand_cnt, d.cand_off, n);
-  c->ncand = read_u32(c, d.cand_off + n);
+  c->ncand = scan_total(c, c->cand_cnt, d.cand_off, n);
   ensure_cand(c, c->ncand);
   k_bp_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   if (d.g.nlarge) k_bp_large_fill<<<blocks(d.g.lflat), TPB, 0, c->st>>>(d);
@@ -641,8 +659,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
   k_toi_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d);
-  scan(c, d.toi_cnt, d.toi_off, n);
-  uint32_t ncast = read_u32(c, d.toi_off + n);
+  uint32_t ncast = scan_total(c, d.toi_cnt, d.toi_off, n);
   c->stats.toi_casts = ncast;
   if (ncast) {
     if (ncast > c->cap_toi) {
