@@ -54,14 +54,14 @@
 #define CCLS (CCLS_COLOURS * 8)  // (colour, size class) counters
 static_assert(CSTAT_FRONT >= COLOUR_RING && CSTAT_MAX >= CSTAT_FRONT + 32 && CSTAT_HEAD >= CSTAT_MAX + 32,
               "cstat layout");
-#define TOI_LANES 64
+#define TOI_LANES 64    // lanes cooperating on one TOI cast
 #define INC_LANES 8    // lanes per body building its incidence list
 #define PREP_LANES 4   // lanes per pair of the per-pair step constants
 #define TOI_GATE_LANES 16  // lanes per cast of the TOI gate (15 separating axes)
 #define NARROW_LANES 16    // lanes per box-box candidate of the narrowphase distance
 #ifndef BP_LANES
-#define BP_LANES 8        // lanes per body of the broadphase and TOI target searches
-#endif    // lanes cooperating on one TOI cast
+#define BP_LANES 16       // lanes per body of the broadphase and TOI target searches
+#endif
 #define BP_CACHE 32     // broadphase candidates kept per body between count and fill
 #define TOI_CACHE 32    // TOI targets kept per mover between count and fill
 
