@@ -73,9 +73,11 @@ struct ScanJobs {
   const uint32_t* in[SCAN_JOBS];
   uint32_t* out[SCAN_JOBS];
   uint32_t* pick = nullptr;  // optional, job 0 of 0/1 flags: pick[out[i]] = i for every set flag
-  // optional, job 0: the total is also written to post[post_word] (pinned
-  // host memory) and then post[post_flag] = post_seq, for a host that polls
+  // optional: job j's total is also written to post[post_word + j] (pinned
+  // host memory); the last job to get there sets post[post_flag] = post_seq,
+  // for a host that polls (post_ticket: a zeroed device word)
   uint32_t* post = nullptr;
+  uint32_t* post_ticket = nullptr;
   uint32_t post_word = 0, post_flag = 0, post_seq = 0;
 };
 
@@ -167,10 +169,13 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32
   }
   if (blockIdx.x == ntiles - 1 && tid == 0) {
     out[n] = tile[ntiles];
-    if (jobs.post && blockIdx.y == 0) {
-      jobs.post[jobs.post_word] = tile[ntiles];
+    if (jobs.post) {
+      jobs.post[jobs.post_word + blockIdx.y] = tile[ntiles];
       __threadfence_system();
-      __hip_atomic_store(jobs.post + jobs.post_flag, jobs.post_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (atomicAdd(jobs.post_ticket, 1u) == gridDim.y - 1u) {
+        *jobs.post_ticket = 0u;  // ready for the next post
+        __hip_atomic_store(jobs.post + jobs.post_flag, jobs.post_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
 }

@@ -166,8 +166,8 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   d.lflag = nullptr;
   d.lrank = nullptr;
   dalloc(&c->ts.tile, (size_t)SCAN_JOBS * (SCAN_MAX_TILES + 1));
-  dalloc(&c->ts.done, SCAN_JOBS);
-  HIPC(hipMemset(c->ts.done, 0, SCAN_JOBS * sizeof(uint32_t)));
+  dalloc(&c->ts.done, SCAN_JOBS + 1);  // tickets of the jobs, then the post ticket
+  HIPC(hipMemset(c->ts.done, 0, (SCAN_JOBS + 1) * sizeof(uint32_t)));
   d.blist = nullptr;
   d.inc = nullptr;
   d.acc_off = nullptr;
@@ -289,9 +289,16 @@ static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
 }
 // Two or three arrays of the same length in one launch pair.
 static void scan2(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
-                  uint32_t n, uint32_t* pick0 = nullptr) {
+                  uint32_t n, uint32_t* pick0 = nullptr, uint32_t post_seq = 0) {
   if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
   ScanJobs js{{in0, in1, nullptr}, {out0, out1, nullptr}, pick0};
+  if (post_seq && n) {  // both totals to the pinned buffer from CTR_COUNT, then the flag
+    js.post = c->counters_hd;
+    js.post_ticket = c->ts.done + SCAN_JOBS;
+    js.post_word = CTR_COUNT;
+    js.post_flag = POST_FLAG;
+    js.post_seq = post_seq;
+  }
   scan_tiles(c->ts, js, 2, n, c->st);
 }
 static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
@@ -365,6 +372,7 @@ static uint32_t scan_total(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint
   js.in[0] = in;
   js.out[0] = out;
   js.post = c->counters_hd;
+  js.post_ticket = c->ts.done + SCAN_JOBS;
   js.post_word = CTR_COUNT;
   js.post_flag = POST_FLAG;
   js.post_seq = ++c->post_seq;
@@ -506,9 +514,12 @@ static void narrow_and_merge(oavbd_ctx* c) {
   }
   // the accepted candidates' flags scan to pair indices; the pair -> candidate
   // map they also give lets the merge run over pairs only
-  scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc, d.pair_cand);
+  const uint32_t seq = nc ? ++c->post_seq : 0u;
+  scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc, d.pair_cand, seq);
   k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
-  {
+  if (nc) {
+    post_wait(c, seq);  // the scan posted both totals
+  } else {
     PostJob j;
     j.src[0] = d.acc_off + nc;
     j.src[1] = d.pt_off + nc;
