@@ -30,25 +30,33 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "body-steps/s @100k boxes (10 AVBD iters) + % HBM roofline; batched scenes/s"
 
 
-def cpu_baseline(scene, seconds, max_steps, start_step):
+def host_cores():
+    """Host threads this process may use (the affinity mask; at most 16,
+    the CPU share of one GPU on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(scene, state, start_step, seconds, max_steps, threads):
     """The CPU oracle in the reference's own order (reverse-handle sweep,
-    history-ordered pairs, libm), one thread, on a bounded sample of the
+    history-ordered pairs, libm) with pair discovery on `threads` threads
+    and a single-threaded sweep and dual update, as the reference runs them
+    (rows.rs:427, :769-782; mod.rs:1052-1059), on a bounded sample of the
     same workload: the steps that follow step `start_step` of the GPU
-    trajectory (the GPU state there becomes the oracle's initial state;
-    contact history is rebuilt), until `seconds` of CPU time or
-    `max_steps` steps."""
+    trajectory (its state there becomes the oracle's initial state; contact
+    history is rebuilt), until `seconds` of CPU time or `max_steps` steps."""
     from oracle import pyoracle as po
-    from ornis_amd.avbd import AvbdEngine
     s = dict(scene)
-    if start_step:
-        g = AvbdEngine(scene=scene, iters=10)
-        g.step_n(start_step)
-        st = g.state()
-        g.close()
-        s["pos"], s["quat"], s["vel"], s["angvel"] = (np.ascontiguousarray(st[:, 0:3]), np.ascontiguousarray(st[:, 3:7]),
-                                                      np.ascontiguousarray(st[:, 7:10]), np.ascontiguousarray(st[:, 10:13]))
+    s["pos"], s["quat"], s["vel"], s["angvel"] = (np.ascontiguousarray(state[:, 0:3]),
+                                                  np.ascontiguousarray(state[:, 3:7]),
+                                                  np.ascontiguousarray(state[:, 7:10]),
+                                                  np.ascontiguousarray(state[:, 10:13]))
     n_dyn = int((scene["body_type"] == 1).sum())
     e = po.OracleEngine(s, order=po.ORDER_REFERENCE, math=po.MATH_LIBM)
+    e.set_threads(threads)
     done, t0 = 0, time.perf_counter()
     while done < max_steps:
         e.step_inner(1)
@@ -57,10 +65,28 @@ def cpu_baseline(scene, seconds, max_steps, start_step):
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": n_dyn * done / dt, "unit": "body-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU oracle (reference sweep order, grid-accelerated discovery, FP32, 1 thread): "
-                      f"steps {start_step + 1}-{start_step + done} of the same 100k-box workload, starting from "
-                      f"the GPU trajectory's state at step {start_step} (contact history rebuilt), {dt:.1f} s"}
+    return {"value": n_dyn * done / dt, "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (reference sweep order, grid-accelerated discovery on {threads} threads, "
+                      f"single-threaded sweep and dual, FP32): steps {start_step + 1}-{start_step + done} of the "
+                      f"same 100k-box workload, starting from the GPU trajectory's state at step {start_step} "
+                      f"(contact history rebuilt), {dt:.1f} s"}
+
+
+def cpu_batched_sample(parts, n_scenes, steps, threads):
+    """The C4 scenes on the host: independent scenes, one CPU oracle each, run
+    `threads` at a time (ctypes releases the interpreter lock during a step),
+    on a sample of n_scenes scenes x steps steps; scene-steps/s."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import pyoracle as po
+    engines = [po.OracleEngine(parts[i], order=po.ORDER_REFERENCE, math=po.MATH_LIBM) for i in range(n_scenes)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda e: e.step_inner(steps), engines))
+    dt = time.perf_counter() - t0
+    bodies = sum(int((parts[i]["body_type"] == 1).sum()) for i in range(n_scenes))
+    return {"value": n_scenes * steps / dt, "unit": "scene-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle, scenes 0-{n_scenes - 1} ({bodies} dynamic bodies) x {steps} steps from rest, one "
+                      f"oracle per scene on {threads} threads, {dt:.1f} s"}
 
 
 def batched_leg(steps, loop_scenes, loop_steps):
@@ -110,15 +136,111 @@ def batched_leg(steps, loop_scenes, loop_steps):
             "speedup_vs_loop": batched / loop}
 
 
-def measured_traffic():
-    """HBM bytes per k_primal launch on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a plain copy of
-    known size); None if that summary is absent."""
+def measured_traffic(window):
+    """HBM bytes per k_primal launch counted by rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the counter's reading for a plain copy of
+    known size) over the same steps of this workload, from the committed
+    summary profiles/k_primal_traffic_100k.json; None when no summary
+    covers `window` (first-last step)."""
     path = os.path.join(ROOT, "profiles", "k_primal_traffic_100k.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        j = json.load(f)
+    w = j.get("windows", {}).get(window)
+    return None if w is None else w.get("hbm_bytes_per_launch")
+
+
+def trajectory_leg(scene, n_dyn, steps=300, windows=100, late=50, sample_every=5, capture_step=280):
+    """The workload's whole trajectory in a fresh context, end to end:
+    ms/step per window of `windows` steps, and over the last `late` steps
+    the per-phase times and the primal / dual launch rates (HIP events on
+    one step in `sample_every`).  The state after `capture_step` is
+    returned for the CPU baseline (read while the clock is stopped)."""
+    from ornis_amd.avbd import AvbdEngine
+    eng = AvbdEngine(scene=scene, iters=10)
+    eng.sync()
+    first_late = steps - late + 1
+    sampled = {i for i in range(first_late, steps + 1) if (i - first_late) % sample_every == 0}
+    phases = {"setup": 0.0, "primal": 0.0, "dual": 0.0, "finish": 0.0, "step": 0.0}
+    per_window, wall, captured = [], 0.0, None
+    eng.kernel_times(reset=True)
+    i = 1  # next step to run (1-based)
+    for w0 in range(1, steps + 1, windows):
+        w1 = min(w0 + windows - 1, steps)
+        t0 = time.perf_counter()
+        while i <= w1:
+            if i in sampled:
+                eng.set_timing(True)
+                eng.step_n(1)
+                eng.set_timing(False)
+                for k, v in eng.timing().items():
+                    phases[k] += v
+                i += 1
+                continue
+            j = i  # run up to the next sampled or captured step, inside the window
+            while j + 1 <= w1 and (j + 1) not in sampled and j != capture_step:
+                j += 1
+            eng.step_n(j - i + 1)
+            i = j + 1
+            if j == capture_step:
+                eng.sync()
+                t_pause = time.perf_counter()
+                captured = eng.state()
+                t0 += time.perf_counter() - t_pause
+        eng.sync()
+        dt = time.perf_counter() - t0
+        wall += dt
+        per_window.append({"steps": f"{w0}-{w1}", "ms_per_step": 1e3 * dt / (w1 - w0 + 1)})
+        print(f"bench.py: trajectory {w1}/{steps} steps, {wall:.1f} s", file=sys.stderr, flush=True)
+    kt = eng.kernel_times()
+    stats_end = eng.stats()
+    eng.close()
+    nph = max(len(sampled), 1)
+    primal_gbs = kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9 if kt["primal_ms"] > 0 else 0.0
+    dual_gbs = kt["dual_bytes"] / (kt["dual_ms"] * 1e-3) / 1e9 if kt["dual_ms"] > 0 else 0.0
+    late_window = f"{first_late}-{steps}"
+    return {
+        "steps": steps, "ms_per_step": 1e3 * wall / steps, "body_steps_per_s": n_dyn * steps / wall,
+        "windows": per_window,
+        "late_window": f"steps {late_window} (HIP events on every {sample_every}th step)",
+        "late_phases_ms_per_step": {k: v / nph for k, v in phases.items()},
+        "late_primal": {"achieved_GBs": primal_gbs, "frac": primal_gbs / HBM_PEAK_GBS,
+                        "bytes_per_launch": kt["primal_bytes"] / max(kt["primal_launches"], 1),
+                        "avg_launch_us": 1e3 * kt["primal_ms"] / max(kt["primal_launches"], 1),
+                        "launches_per_step": kt["primal_launches"] / max(kt["steps"], 1),
+                        "traffic": measured_traffic(late_window), "traffic_window": late_window},
+        "late_dual": {"achieved_GBs": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS,
+                      "bytes_per_launch": kt["dual_bytes"] / max(kt["dual_launches"], 1),
+                      "avg_launch_us": 1e3 * kt["dual_ms"] / max(kt["dual_launches"], 1)},
+        "final_stats": stats_end,
+    }, captured
+
+
+def chains_leg(steps):
+    """Contacts + joints: 2,000 chains of 32 boxes (0.1 x 0.1 x 0.4 m) linked
+    by ball joints, roots on a 45 x 45 grid at 20 m, falling onto a plane
+    scattered with 5,000 static spheres; 12 iterations, `steps` steps."""
+    from ornis_amd import scenes
+    from ornis_amd.avbd import AvbdEngine
+    s = scenes.chains()
+    n_dyn = int((s["body_type"] == 1).sum())
+    eng = AvbdEngine(scene=s, iters=12)
+    eng.sync()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(60, steps - done)
+        eng.step_n(k)
+        done += k
+        eng.sync()
+        print(f"bench.py: chains {done}/{steps} steps, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    return {"metric": "body-steps/s", "value": n_dyn * steps / wall, "unit": "body-steps/s", "bodies": n_dyn,
+            "joints": len(s["joints"]), "steps": steps, "ms_per_step": 1e3 * wall / steps, "iters": 12,
+            "final_stats": st}
 
 
 def main():
@@ -128,8 +250,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-max-steps", type=int, default=40)
-    ap.add_argument("--cpu-start-step", type=int, default=150,
-                    help="the CPU sample continues the trajectory from this step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="primal lanes per body (0: library default)")
@@ -138,13 +258,19 @@ def main():
                     help="after warmup, time this many steps per primal launch shape and exit")
     ap.add_argument("--timing-every", type=int, default=10,
                     help="record per-launch kernel events on one timed step in this many")
+    ap.add_argument("--trajectory-steps", type=int, default=300,
+                    help="steps of the end-to-end trajectory leg (0: skip it)")
+    ap.add_argument("--chains-steps", type=int, default=240, help="steps of the chains workload (0: skip it)")
     ap.add_argument("--batched-steps", type=int, default=300,
                     help="timed steps of the 4,096-scene batched workload (0: skip it)")
     ap.add_argument("--loop-scenes", type=int, default=32)
     ap.add_argument("--loop-steps", type=int, default=30)
+    ap.add_argument("--cpu-batched-scenes", type=int, default=64)
+    ap.add_argument("--cpu-batched-steps", type=int, default=30)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
                          "for HBM byte counters)")
+    ap.add_argument("--only-main", action="store_true", help="run the timed leg only (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the 100k-box scene fits one MI355X and multi-GPU "
@@ -213,15 +339,10 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = n_dyn * args.steps / wall
     achieved = kt["primal_bytes"] / (kt["primal_ms"] * 1e-3) / 1e9 if kt["primal_ms"] > 0 else 0.0
-    traffic = measured_traffic()
     dual_gbs = kt["dual_bytes"] / (kt["dual_ms"] * 1e-3) / 1e9 if kt["dual_ms"] > 0 else 0.0
     eng.close()
-    batched = None
-    if args.batched_steps:
-        batched = batched_leg(args.batched_steps, args.loop_scenes, args.loop_steps)
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(scene, args.cpu_seconds, args.cpu_max_steps, args.cpu_start_step)
+    window = f"{args.warmup + 1}-{args.warmup + args.steps}"
+    traffic = measured_traffic(window)
     line = {
         "metric": METRIC,
         "value": value,
@@ -237,19 +358,36 @@ def main():
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": "100k random boxes, 50x50x40 lattice drop into a 110 m container, "
                                "dt 1/60, 10 iterations, FP32", "bodies": n_dyn, "iters": 10,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "timed_steps": window},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_primal",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_primal",
+                     "window": f"steps {window} (HIP events on every {every}th step)",
                      "bytes_per_launch": kt["primal_bytes"] / max(kt["primal_launches"], 1),
                      "avg_launch_us": 1e3 * kt["primal_ms"] / max(kt["primal_launches"], 1)},
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
         "phases_ms_per_step": {"primal": kt["primal_ms"] / max(kt["steps"], 1),
                                "dual": kt["dual_ms"] / max(kt["steps"], 1)},
         "dual_roofline": {"achieved": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS},
         "final_stats": st,
-        "batched": batched,
     }
+    if not args.only_main:
+        captured = None
+        if args.trajectory_steps:
+            traj, captured = trajectory_leg(scene, n_dyn, steps=args.trajectory_steps)
+            line["trajectory"] = traj
+        if args.chains_steps:
+            line["chains"] = chains_leg(args.chains_steps)
+        parts = None
+        if args.batched_steps:
+            line["batched"] = batched_leg(args.batched_steps, args.loop_scenes, args.loop_steps)
+        if not args.no_cpu:
+            cores = host_cores()
+            if captured is not None:
+                line["cpu_baseline"] = cpu_baseline(scene, captured, 280, args.cpu_seconds, args.cpu_max_steps, cores)
+            if args.batched_steps and args.cpu_batched_scenes:
+                parts, _ = scenes.batched_drops()
+                line["batched"]["cpu_baseline"] = cpu_batched_sample(parts, args.cpu_batched_scenes,
+                                                                     args.cpu_batched_steps, cores)
     print(json.dumps(line))
 
 

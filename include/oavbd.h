@@ -74,10 +74,14 @@ typedef struct oavbd_stats {
  * while timing is enabled (oavbd_set_timing).  Every primal-solve and
  * dual-update launch is bracketed by HIP events on the context stream.
  * Algorithmic bytes count each datum a launch must read or write once, at
- * its stored size:
- *   primal, per iteration: 168 B per swept body + 72 B per point-side +
- *     144 B per pair-side + 168 B per joint-side;
- *   dual, per iteration: 116 B per point + 208 B per pair + 264 B per joint. */
+ * its packed size (3 floats per vector, 4 per quaternion):
+ *   primal, per iteration: 128 B per swept body (position, rotation, their
+ *     step-start values, inertial targets, inverse mass, inertia: 100 B
+ *     read; pose: 28 B written) + 96 B per pair-side (pair header 24 B,
+ *     partner pose and start pose 56 B, shape 16 B) + 48 B per point-side
+ *     (anchors, duals, penalties) + 168 B per joint-side;
+ *   dual, per iteration: 136 B per pair (24 B header + four 28 B poses) +
+ *     76 B per point (48 B read, 28 B written) + 264 B per joint. */
 typedef struct oavbd_kernel_times {
   double primal_ms;      /* summed primal launch durations                 */
   double dual_ms;        /* summed dual launch durations                   */

@@ -62,6 +62,8 @@
 #include <cstring>
 #include <limits>
 #include <unordered_map>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "oracle_math.hpp"
@@ -338,9 +340,16 @@ void seg_seg_closest(V3<T> a0, V3<T> a1, V3<T> b0, V3<T> b1, V3<T>* x, V3<T>* y)
 }
 
 // point_obb_closest (distance.rs:117-138).
+// ORC_PLANT_CONJ builds a deliberately wrong checker (the box's local frame
+// taken with the rotation instead of its conjugate): the float64 geometry
+// pins must reject that build (tests/test_oracle_rotation.py).
 template <class T>
 V3<T> point_obb_closest(V3<T> p, V3<T> pos, V3<T> half, Q<T> rot) {
+#ifdef ORC_PLANT_CONJ
+  V3<T> l = rotate(rot, p - pos);
+#else
   V3<T> l = rotate(conj(rot), p - pos);
+#endif
   V3<T> cl = vclamp(l, -half, half);
   V3<T> q;
   if (cl == l) {
@@ -583,6 +592,7 @@ struct Engine {
   int order = ORDER_REFERENCE;
   int math = MATH_LIBM;
   bool brute = false;
+  int threads = 1;  // discovery threads (the reference parallelises discovery)
   std::vector<int> colour;  // per body, -1 = not swept
   int ncolours = 0;
   Stats stats{};
@@ -733,7 +743,7 @@ struct Engine {
     int handle;
     bool valid;
   };
-  Hit cast_one(const SRef<T>& mover, V3<T> dir, T len, int handle, const SRef<T>& target) const {
+  static Hit cast_one(const SRef<T>& mover, V3<T> dir, T len, int handle, const SRef<T>& target) {
     Hit hit{T(0), {}, {}, handle, false};
     T t = T(0);
     for (int it = 0; it < 24; ++it) {
@@ -1137,9 +1147,29 @@ struct Engine {
     std::vector<std::pair<int, int>> cand;
     candidate_pairs(cand);
     std::vector<Disc> disc;
-    Disc d;
-    for (auto& c : cand)
-      if (discover_pair(c.first, c.second, use_index ? &index : nullptr, d)) disc.push_back(d);
+    // Discovery runs on `threads` threads above PAR_DISCOVERY_BODIES bodies
+    // (rows.rs:427, :769-782); chunks are concatenated in candidate order, so
+    // the result does not depend on the thread count.
+    const int nthr = (threads > 1 && bodies.size() > 256) ? threads : 1;
+    if (nthr == 1) {
+      Disc d;
+      for (auto& c : cand)
+        if (discover_pair(c.first, c.second, use_index ? &index : nullptr, d)) disc.push_back(d);
+    } else {
+      std::vector<std::vector<Disc>> part(nthr);
+      std::vector<std::thread> pool;
+      const size_t chunk = (cand.size() + nthr - 1) / nthr;
+      for (int t = 0; t < nthr; ++t)
+        pool.emplace_back([&, t]() {
+          Disc d;
+          const size_t lo = std::min(cand.size(), t * chunk), hi = std::min(cand.size(), lo + chunk);
+          for (size_t i = lo; i < hi; ++i)
+            if (discover_pair(cand[i].first, cand[i].second, use_index ? &index : nullptr, d)) part[t].push_back(d);
+        });
+      for (auto& th : pool) th.join();
+      for (auto& p : part)
+        for (auto& x : p) disc.push_back(std::move(x));
+    }
     // candidates are generated in (ia, ib) order already; sort defensively.
     std::stable_sort(disc.begin(), disc.end(),
                      [](const Disc& x, const Disc& y) { return x.ia != y.ia ? x.ia < y.ia : x.ib < y.ib; });
@@ -1803,14 +1833,14 @@ static Engine<T>* mk(const float* g, int order, int math, int iters, int brute) 
   return e;
 }
 
-#define DISPATCH(ctx, expr)     \
+#define DISPATCH(ctx, ...)      \
   do {                          \
     if ((ctx)->prec == 64) {    \
       auto* E = (ctx)->d;       \
-      expr;                     \
+      __VA_ARGS__;              \
     } else {                    \
       auto* E = (ctx)->f;       \
-      expr;                     \
+      __VA_ARGS__;              \
     }                           \
   } while (0)
 
@@ -2146,6 +2176,144 @@ void orc_shape_distance_f32(int sa, const float* pa_sp, const float* pa_pos, con
   out[1] = d.pa.x; out[2] = d.pa.y; out[3] = d.pa.z;
   out[4] = d.pb.x; out[5] = d.pb.y; out[6] = d.pb.z;
   out[7] = sat;
+}
+
+// ---- internal queries the reference's own unit tests make -----------------
+// (crates/physics/src/avbd/contact_tests.rs: pos0/rot0 + generate_pairs,
+// gap_c0, row_c on hand-built pairs), float32 engines only.
+
+// Step-start poses := current poses, then generate_pairs (no step).
+void orc_prepare_pairs(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      E->pos0[i] = E->bodies[i].pos;
+      E->rot0[i] = E->bodies[i].rot;
+    }
+    E->generate_pairs();
+  });
+}
+
+// Overwrite body i's pose (a driver teleport: get_body_mut).
+void orc_set_pose(void* h, int i, const float* pos, const float* quat) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->bodies[i].pos = {TT(pos[0]), TT(pos[1]), TT(pos[2])};
+    E->bodies[i].rot = {TT(quat[0]), TT(quat[1]), TT(quat[2]), TT(quat[3])};
+  });
+}
+
+// Overwrite body i's velocities.
+void orc_set_vel(void* h, int i, const float* vel, const float* angvel) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->bodies[i].vel = {TT(vel[0]), TT(vel[1]), TT(vel[2])};
+    E->bodies[i].angvel = {TT(angvel[0]), TT(angvel[1]), TT(angvel[2])};
+  });
+}
+
+// Step-start pose of body i.
+void orc_set_step_start(void* h, int i, const float* pos, const float* quat) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->pos0[i] = {TT(pos[0]), TT(pos[1]), TT(pos[2])};
+    E->rot0[i] = {TT(quat[0]), TT(quat[1]), TT(quat[2]), TT(quat[3])};
+  });
+}
+
+// Replace the pair table by one pair (a, b, normal n, friction mu, gap) with
+// one point (anchors ra, rb; unit penalties, zero duals).
+void orc_set_single_pair(void* h, int a, int b, const float* n, const float* mu, float gap, const float* ra,
+                         const float* rb) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    Pair<TT> p;
+    p.a = a;
+    p.b = b;
+    p.n = {TT(n[0]), TT(n[1]), TT(n[2])};
+    p.mu[0] = TT(mu[0]);
+    p.mu[1] = TT(mu[1]);
+    p.gap = TT(gap);
+    Point<TT> q;
+    q.ra = {TT(ra[0]), TT(ra[1]), TT(ra[2])};
+    q.rb = {TT(rb[0]), TT(rb[1]), TT(rb[2])};
+    for (int k = 0; k < 3; ++k) {
+      q.lam[k] = TT(0);
+      q.pen[k] = TT(1);
+      q.roll[k] = TT(0);
+    }
+    q.stuck = true;
+    p.pts.push_back(q);
+    E->pairs.clear();
+    E->pairs.push_back(p);
+  });
+}
+
+// Overwrite the anchors of point q of pair p.
+void orc_set_point(void* h, int p, int q, const float* ra, const float* rb) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->pairs[p].pts[q].ra = {TT(ra[0]), TT(ra[1]), TT(ra[2])};
+    E->pairs[p].pts[q].rb = {TT(rb[0]), TT(rb[1]), TT(rb[2])};
+  });
+}
+
+// gap_c0 of point q of pair p (joints_ext.rs:21-32).
+double orc_gap_c0(void* h, int p, int q) {
+  OracleCtx* c = (OracleCtx*)h;
+  double r = 0.0;
+  DISPATCH(c, r = (double)E->gap_c0(E->pairs[p], E->pairs[p].pts[q]));
+  return r;
+}
+
+// row_c (rows.rs:947-958): out = C, ra_w (3), rb_w (3).
+void orc_row_c(void* h, int p, int q, const float* axis, float c0, double* out7) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    const auto& pr = E->pairs[p];
+    const auto& pt = pr.pts[q];
+    V3<TT> ra_w, rb_w;
+    E->contact_levers(pr, pt, &ra_w, &rb_w);
+    TT cv = E->row_c_levers(pr, V3<TT>{TT(axis[0]), TT(axis[1]), TT(axis[2])}, TT(c0), ra_w, rb_w);
+    out7[0] = (double)cv;
+    out7[1] = (double)ra_w.x; out7[2] = (double)ra_w.y; out7[3] = (double)ra_w.z;
+    out7[4] = (double)rb_w.x; out7[5] = (double)rb_w.y; out7[6] = (double)rb_w.z;
+  });
+}
+
+// cast_shape against one target (distance.rs:841-891): out = t, normal3,
+// valid (1/0).
+void orc_cast_f32(int sa, const float* a_sp, const float* a_pos, const float* a_q, const float* delta, int sb,
+                  const float* b_sp, const float* b_pos, const float* b_q, float* out5) {
+  SRef<float> a{sa, {a_sp[0], a_sp[1], a_sp[2]}, {a_pos[0], a_pos[1], a_pos[2]}, {a_q[0], a_q[1], a_q[2], a_q[3]}};
+  SRef<float> b{sb, {b_sp[0], b_sp[1], b_sp[2]}, {b_pos[0], b_pos[1], b_pos[2]}, {b_q[0], b_q[1], b_q[2], b_q[3]}};
+  V3<float> d{delta[0], delta[1], delta[2]};
+  float len = length(d);
+  out5[0] = 0.0f; out5[1] = 0.0f; out5[2] = 0.0f; out5[3] = 0.0f; out5[4] = 0.0f;
+  if (len < K<float>::NEAR_ZERO()) return;
+  auto hit = Engine<float>::cast_one(a, d / len, len, 1, b);
+  out5[0] = hit.t;
+  out5[1] = hit.normal.x; out5[2] = hit.normal.y; out5[3] = hit.normal.z;
+  out5[4] = hit.valid ? 1.0f : 0.0f;
+}
+
+// world_inertia (rows.rs:194-207): out9 row-major.
+void orc_world_inertia_f32(const float* inertia, const float* q, float* out9) {
+  float m[3][3];
+  world_inertia(V3<float>{inertia[0], inertia[1], inertia[2]}, Q<float>{q[0], q[1], q[2], q[3]}, m);
+  std::memcpy(out9, m, sizeof(m));
+}
+
+// Discovery threads (results are identical for every count).
+void orc_set_threads(void* h, int threads) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, E->threads = threads < 1 ? 1 : threads);
 }
 
 void orc_fmath_sincos(float x, float* s, float* c) { oavbd_sincosf(x, s, c); }

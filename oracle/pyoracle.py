@@ -74,6 +74,7 @@ def lib():
         L.orc_quat_diff_vec_f32.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p]
         L.orc_tangent_basis_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_shape_distance_f32.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
+        L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_fmath_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
         L.orc_fmath_asin.restype = ctypes.c_float
         L.orc_fmath_asin.argtypes = [ctypes.c_float]
@@ -147,6 +148,10 @@ class OracleEngine:
                 self.h = None
         except Exception:
             pass
+
+    def set_threads(self, threads):
+        """Pair discovery on `threads` threads (bit-identical results)."""
+        self._L.orc_set_threads(self.h, int(threads))
 
     def step(self, dt=1.0 / 60.0, nsteps=1):
         self._L.orc_step(self.h, ctypes.c_float(dt), int(nsteps))

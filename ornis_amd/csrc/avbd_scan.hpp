@@ -172,7 +172,11 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32
     if (jobs.post) {
       jobs.post[jobs.post_word + blockIdx.y] = tile[ntiles];
       __threadfence_system();
-      if (atomicAdd(jobs.post_ticket, 1u) == gridDim.y - 1u) {
+      // The ticket is taken with acquire-release at system scope: the job
+      // that takes the last ticket sees the other jobs' post words (their
+      // release) before its own release store raises the flag.
+      if (__hip_atomic_fetch_add(jobs.post_ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM) ==
+          gridDim.y - 1u) {
         *jobs.post_ticket = 0u;  // ready for the next post
         __hip_atomic_store(jobs.post + jobs.post_flag, jobs.post_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
       }

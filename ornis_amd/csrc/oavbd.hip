@@ -798,9 +798,9 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     c->kt.dual_ms += dm;
     c->kt.primal_launches += np_ev / 2;
     c->kt.dual_launches += nd_ev / 2;
-    c->kt.primal_bytes += iters * (168.0 * s.solved + 72.0 * s.point_sides + 144.0 * s.pair_sides +
+    c->kt.primal_bytes += iters * (128.0 * s.solved + 96.0 * s.pair_sides + 48.0 * s.point_sides +
                                    168.0 * s.joint_sides);
-    c->kt.dual_bytes += iters * (116.0 * s.points + 208.0 * s.pairs + 264.0 * d.njoints);
+    c->kt.dual_bytes += iters * (136.0 * s.pairs + 76.0 * s.points + 264.0 * d.njoints);
     c->kt.steps += 1;
     c->ms[0] = setup;
     c->ms[1] = (float)pm;

@@ -1,6 +1,7 @@
 """GPU parity: the HIP step (through the C ABI) against the CPU oracle run
 in the same colour order on the same inputs.  Bodies, pair lists and
 colourings are compared bit for bit."""
+import ctypes
 import os
 
 import numpy as np
@@ -51,7 +52,8 @@ def test_snapshot_scene_bitwise(gpu_available):
 
 
 def test_pyramid_bitwise(gpu_available):
-    run_pair(scenes.box_pyramid(), 120, check_every=20)
+    """The 210-cube pyramid over its whole 300-step trajectory."""
+    run_pair(scenes.box_pyramid(), 300, check_every=20)
 
 
 def test_random_box_drop_bitwise(gpu_available):
@@ -199,3 +201,134 @@ def test_launch_shape_does_not_change_bits(gpu_available, shape):
     g.set_primal_shape(*shape)
     g.step_n(40)
     assert np.array_equal(ref.state().view(np.uint32), g.state().view(np.uint32))
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def test_accumulator_host_dt_bitwise(gpu_available):
+    """oavbd_step(dt) is PhysicsEngine::step (mod.rs:1212-1235): 120 Hz host
+    frames (a substep every second call), 1/30 s hitches (two substeps), a
+    1 s stall (capped at four), ignored non-positive and non-finite dts, and
+    odd fractions; bodies, pairs and TOI clamps equal the oracle's."""
+    s = scenes.box_drop_scene(60, _rng(41))
+    g = AvbdEngine(scene=s)
+    o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+    dts = [1.0 / 120.0] * 40 + [1.0 / 30.0] * 10 + [1.0, 0.0, -1.0, float("nan"), float("inf")] + \
+        [1.0 / 60.0] * 5 + [0.3 / 60.0] * 25 + [2.5 / 60.0] * 6
+    for i, dt in enumerate(dts):
+        g.step(dt)
+        o.step(dt, 1)
+        if i % 10 == 0 or i == len(dts) - 1:
+            assert_same(g, o, i)
+
+
+def test_rolling_torsion_friction_bitwise(gpu_available):
+    """Rolling and torsion friction rows (assembly.rs:209-250) and their
+    duals (joints_ext.rs:549-571): spheres rolling and boxes spinning on a
+    floor, with per-body rolling / torsion coefficients."""
+    rng = _rng(52)
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (12.0, 1.0, 12.0), 0.0)
+    for i in range(24):
+        p = (rng.uniform(-8, 8), rng.uniform(0.4, 2.0), rng.uniform(-8, 8))
+        if i % 2 == 0:
+            sb.sphere(p, rng.uniform(0.2, 0.5), 1.0 + i % 3, vel=(rng.uniform(-3, 3), 0.0, rng.uniform(-3, 3)),
+                      angvel=tuple(rng.uniform(-6, 6, 3)))
+        else:
+            q = rng.normal(size=4)
+            sb.box(p, tuple(rng.uniform(0.2, 0.5, 3)), 2.0, quat=tuple((q / np.linalg.norm(q)).astype(np.float32)),
+                   angvel=(0.0, rng.uniform(-8, 8), 0.0))
+    s = sb.build()
+    n = len(s["shape"])
+    s["rolling"] = rng.uniform(0.0, 0.1, n).astype(np.float32)
+    s["torsion"] = rng.uniform(0.0, 0.05, n).astype(np.float32)
+    s["rolling"][::5] = 0.0
+    g, o = run_pair(s, 150, check_every=10)
+    pg = g.points()
+    po_ = o.points()
+    assert np.array_equal(pg[:, 12:15].view(np.uint32), po_[:, 12:15].view(np.uint32))
+    assert np.abs(pg[:, 12:15]).max() > 0.0
+
+
+def test_kinematic_pusher_bitwise(gpu_available):
+    """A kinematic box teleported by its driver every step at 2 m/s pushes a
+    row of boxes (the driver rewind, mod.rs:993-1005: its step-start pose is
+    its previous pose, so contact rows see the motion within the step)."""
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (12.0, 1.0, 4.0), 0.0)
+    for i in range(5):
+        sb.box((-1.0 + 1.05 * i, 0.5, 0.0), (0.5, 0.5, 0.5), 1.0)
+    k = sb.box((-3.0, 0.6, 0.0), (0.5, 0.6, 1.0), 1.0, body_type=2)
+    s = sb.build()
+    g = AvbdEngine(scene=s)
+    o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+    L = o._L
+    L.orc_set_pose.argtypes = [ctypes.c_void_p, ctypes.c_int, po._f32p, po._f32p]
+    x = np.float32(-3.0)
+    for step in range(150):
+        x = np.float32(x + np.float32(2.0 / 60.0))
+        st = g.state()
+        st[k, 0] = x
+        g.set_state(st)
+        pos = np.ascontiguousarray(st[k, 0:3])
+        quat = np.ascontiguousarray(st[k, 3:7])
+        L.orc_set_pose(o.h, k, pos.ctypes.data_as(po._f32p), quat.ctypes.data_as(po._f32p))
+        g.step_n(1)
+        o.step_inner(1)
+        if step % 10 == 0 or step == 149:
+            assert_same(g, o, step)
+    assert g.state()[5, 0] > 4.5  # the row was pushed
+
+
+def test_layer_mask_trigger_filters_bitwise(gpu_available):
+    """pair_allowed layer/mask filtering and trigger bodies (rows.rs:366-368,
+    :511-523): boxes on three layers with masks that exclude one another,
+    and a trigger sphere that forms no contact."""
+    rng = _rng(63)
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (8.0, 1.0, 8.0), 0.0)
+    for i in range(60):
+        p = (rng.uniform(-3, 3), 0.6 + 0.5 * i / 6.0, rng.uniform(-3, 3))
+        sb.box(p, tuple(rng.uniform(0.2, 0.45, 3)), 1.0)
+    trig = sb.sphere((0.0, 1.0, 0.0), 1.5, 0.0)
+    s = sb.build()
+    n = len(s["shape"])
+    s["layer"] = np.array([1] + [1 << (i % 3) for i in range(60)] + [1], np.uint32)
+    s["mask"] = np.array([0xFFFFFFFF] + [0xFFFFFFFF if i % 3 == 0 else (1 | (1 << (i % 3))) for i in range(60)] +
+                         [0xFFFFFFFF], np.uint32)
+    s["trigger"] = np.zeros(n, np.uint8)
+    s["trigger"][trig] = 1
+    g, o = run_pair(s, 150, check_every=10)
+    ab = g.pairs()
+    assert not ((ab == trig).any(axis=1)).any()
+    lay, msk = s["layer"], s["mask"]
+    assert ((lay[ab[:, 0]] & msk[ab[:, 1]]) != 0).all() and ((lay[ab[:, 1]] & msk[ab[:, 0]]) != 0).all()
+
+
+def test_thin_wall_and_bounced_retouch_bitwise(gpu_available):
+    """avbd_fast_box_stops_at_thin_wall (avbd_engine.rs:1290-1313) and
+    avbd_bounced_retouch_catches_at_offset (:1770-1808) on the GPU, bit for
+    bit against the oracle, with the reference's behavioural bounds."""
+    sb = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+    sb.box((5.0, 0.0, 0.0), (0.02, 2.0, 2.0), 0.0)
+    sb.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 1.0, vel=(10.0, 0.0, 0.0), restitution=0.0)
+    g, _ = run_pair(sb.build(), 120, check_every=10)
+    assert 4.0 < g.state()[1, 0] < 5.0
+    for x0 in (-3.0, 0.0):
+        sb = scenes.SceneBuilder()
+        sb.box((0.0, -1.0, 0.0), (10.0, 1.0, 10.0), 0.0)
+        sb.box((x0, 8.0, 0.0), (0.4, 0.4, 0.4), 1.0, vel=(0.0, -30.0, 0.0))
+        g = AvbdEngine(scene=sb.build())
+        o = po.OracleEngine(sb.build(), order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+        min_y = np.inf
+        for step in range(300):
+            g.step_n(1)
+            o.step_inner(1)
+            min_y = min(min_y, g.state()[1, 1])
+            if step % 20 == 0:
+                assert_same(g, o, step)
+        assert_same(g, o, 299)
+        b = g.state()[1]
+        assert min_y > 0.0 and abs(b[1] - 0.4) < 0.05 and abs(b[0] - x0) < 0.15 and np.linalg.norm(b[7:10]) < 0.5
