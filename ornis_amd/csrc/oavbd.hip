@@ -1032,6 +1032,11 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
     HIPC(hipMemset(d.b.flag_a, 0, n));
     HIPC(hipMemset(d.b.flag_b, 0, n));
     HIPC(hipMemset(d.b.colour, 0xff, n * sizeof(int32_t)));
+    {  // no TOI clamp before the first step (travel < 0)
+      std::vector<float4> nohit(n, make_float4(0.0f, 0.0f, 0.0f, -1.0f));
+      HIPC(hipMemcpy(d.toi_hit, nohit.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+      HIPC(hipMemset(d.toi_tgt, 0, n * sizeof(uint32_t)));
+    }
     HIPC(hipMemset(d.poff, 0, (n + 1) * sizeof(uint32_t)));
     HIPC(hipMemset(d.poff_old, 0, (n + 1) * sizeof(uint32_t)));
     d.npairs = d.npts = d.npairs_old = d.npts_old = 0;
