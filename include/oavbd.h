@@ -203,6 +203,18 @@ int oavbd_solve_diag(oavbd_ctx* ctx, const void* masses16, const void* rhs32, ui
  * mod.rs:200-207).  Independent of the context's bodies. */
 int oavbd_solve_systems(oavbd_ctx* ctx, const void* systems96, uint32_t n, float dt, void* delta32, float* ok);
 
+/* Rung-2 staged systems exactly as oavbd_solve_systems, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The call allocates nothing: the records are staged through a fixed device
+ * array of the library, one per GPU (resolved when a context is created on
+ * it), holding at most 65536 systems per call (8.65 MB, held until the
+ * process ends); larger n returns OAVBD_ECAPACITY.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from several threads
+ * or contexts) race and must be serialised by the caller. */
+int oavbd_solve_systems_noalloc(oavbd_ctx* ctx, const void* systems96, uint32_t n, float dt, void* delta32,
+                                float* ok);
+
 /* Block until all work of the context has finished. */
 int oavbd_sync(oavbd_ctx* ctx);
 

@@ -74,6 +74,8 @@ SIGNATURES = {
                                         ctypes.c_float, ctypes.c_void_p]),
     "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "oavbd_solve_systems_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                   ctypes.c_float, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
@@ -300,6 +302,19 @@ class AvbdEngine:
         self._check(self._L.oavbd_solve_systems(self._ctx, sysa.ctypes.data_as(ctypes.c_void_p), n,
                                                 ctypes.c_float(dt), delta.ctypes.data_as(ctypes.c_void_p),
                                                 _p(ok, _f32p)))
+        return delta, ok
+
+    def solve_systems_noalloc(self, systems, dt=1.0 / 60.0):
+        """solve_systems through the library's fixed staging array (no device
+        allocation; at most 65536 systems per call)."""
+        sysa = np.ascontiguousarray(systems, dtype=np.float32).reshape(-1, 24)
+        n = sysa.shape[0]
+        self._ensure_ctx(max(self.n, 1), self.njoints)
+        delta = np.zeros((n, 8), np.float32)
+        ok = np.zeros(n, np.float32)
+        self._check(self._L.oavbd_solve_systems_noalloc(self._ctx, sysa.ctypes.data_as(ctypes.c_void_p), n,
+                                                        ctypes.c_float(dt), delta.ctypes.data_as(ctypes.c_void_p),
+                                                        _p(ok, _f32p)))
         return delta, ok
 
     def set_primal_shape(self, lanes=32, waves=4):

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -64,6 +65,33 @@ struct HostJoint {
 
 }  // namespace
 
+// Staging memory of oavbd_solve_systems_noalloc: a file-scope device array
+// (one copy per GPU, held until the process ends) so that entry point never
+// allocates.  It holds up to NOALLOC_MAX_SYSTEMS systems: the 96-byte records,
+// then the 32-byte deltas, then the ok flags.  Host code never passes
+// `scratch` itself to a kernel: it uses the device address resolved per GPU
+// by noalloc_resolve (at context creation, with that GPU current).
+constexpr uint32_t NOALLOC_MAX_SYSTEMS = 65536;
+constexpr size_t NOALLOC_BYTES = (size_t)NOALLOC_MAX_SYSTEMS * (96 + 32 + 4);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current GPU (`device`), resolved once.
+static void* noalloc_resolve(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) throw HipError{"device index out of range for the staging table"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) throw HipError{std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
 // Word offset of the colouring check in the pinned readback buffer (the
 // words below it hold the counters and single-word reads).
 constexpr uint32_t HEAD_H = 64;
@@ -101,6 +129,7 @@ struct oavbd_ctx {
   oavbd_kernel_times kt{};
   hipEvent_t ev[5] = {};
   float ms[5] = {};
+  void* noalloc = nullptr;  // this GPU's address of `scratch`
   std::string err;
 };
 
@@ -844,6 +873,7 @@ int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_j
     HIPC(hipHostGetDevicePointer((void**)&c->counters_hd, c->counters_h, 0));
     c->counters_h[POST_FLAG] = 0u;
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
+    c->noalloc = noalloc_resolve(device);
     alloc_bodies(c, max_bodies);
   } catch (const HipError& e) {
     std::string m = e.msg;
@@ -1288,6 +1318,31 @@ int oavbd_solve_systems(oavbd_ctx* c, const void* systems96, uint32_t n, float d
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(delta32, b.dl, 32 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPC(hipMemcpyAsync(ok, b.ok, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_solve_systems_noalloc(oavbd_ctx* c, const void* systems96, uint32_t n, float dt, void* delta32,
+                                float* ok) {
+  if (!c || (n && (!systems96 || !delta32 || !ok))) return OAVBD_EINVAL;
+  if (n > NOALLOC_MAX_SYSTEMS)
+    return fail(c, OAVBD_ECAPACITY, "oavbd_solve_systems_noalloc: more than 65536 systems per call");
+  if (!n) return OAVBD_OK;
+  if (!c->noalloc) return fail(c, OAVBD_EDEVICE, "staging memory not resolved for this device");
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    // the three regions of this GPU's staging array; every word a launch
+    // reads is written by this call first, so nothing needs clearing
+    unsigned char* base = static_cast<unsigned char*>(c->noalloc);
+    float4* sys = reinterpret_cast<float4*>(base);
+    float4* dl = reinterpret_cast<float4*>(base + (size_t)NOALLOC_MAX_SYSTEMS * 96);
+    float* okd = reinterpret_cast<float*>(base + (size_t)NOALLOC_MAX_SYSTEMS * (96 + 32));
+    HIPC(hipMemcpyAsync(sys, systems96, 96 * (size_t)n, hipMemcpyHostToDevice, c->st));
+    k_solve_systems<<<blocks(n), TPB, 0, c->st>>>(sys, n, dt, dl, okd);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(delta32, dl, 32 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(ok, okd, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPC(hipStreamSynchronize(c->st));
   });
   return OAVBD_OK;
