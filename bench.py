@@ -253,7 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--progress", action="store_true")
     ap.add_argument("--lanes", type=int, default=0, help="primal lanes per body (0: library default)")
-    ap.add_argument("--waves", type=int, default=1, help="primal wavefronts per workgroup")
+    ap.add_argument("--waves", type=int, default=2, help="primal wavefronts per workgroup")
     ap.add_argument("--sweep-shapes", type=int, default=0,
                     help="after warmup, time this many steps per primal launch shape and exit")
     ap.add_argument("--timing-every", type=int, default=10,
@@ -301,7 +301,7 @@ def main():
     if args.lanes:
         eng.set_primal_shape(args.lanes, args.waves)
     if args.sweep_shapes:
-        for shape in ((16, 1), (16, 4), (32, 1), (32, 2), (32, 4), (32, 8), (64, 1), (64, 4)):
+        for shape in ((16, 1), (16, 2), (16, 4), (16, 8)):
             eng.set_primal_shape(*shape)
             eng.kernel_times(reset=True)
             eng.step_n(args.sweep_shapes)
@@ -360,7 +360,7 @@ def main():
                                "dt 1/60, 10 iterations, FP32", "bodies": n_dyn, "iters": 10,
                    "parallelism": "single GPU", "timed_steps": window},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_primal",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_primal8",
                      "window": f"steps {window} (HIP events on every {every}th step)",
                      "bytes_per_launch": kt["primal_bytes"] / max(kt["primal_launches"], 1),
                      "avg_launch_us": 1e3 * kt["primal_ms"] / max(kt["primal_launches"], 1)},

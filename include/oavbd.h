@@ -175,9 +175,10 @@ int oavbd_set_timing(oavbd_ctx* ctx, int enable);
 int oavbd_get_timing(oavbd_ctx* ctx, float* ms5);
 
 /* Launch shape of the primal solve: `lanes` lanes cooperate on one body
- * (8, 16, 32 or 64) and a workgroup holds `waves` wavefronts (1, 2, 4 or
- * 8).  Results are bitwise independent of the shape; only speed changes.
- * Default 32 lanes, 1 wavefront. */
+ * (16; the lanes' row sums are combined by a fixed tree, so the lane
+ * count is part of the arithmetic) and a workgroup holds `waves` wavefronts
+ * (1, 2, 4 or 8).  Results are bitwise independent of `waves`; only speed
+ * changes.  Default 16 lanes, 2 wavefronts. */
 int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
 
 /* Rung-1 staged bodies: n GpuAvbdMass records (16 bytes: inertia[3],

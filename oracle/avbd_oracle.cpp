@@ -1372,6 +1372,93 @@ struct Engine {
     return true;
   }
 
+  // Row accumulation of one body solve.  ORDER_REFERENCE stamps every row
+  // straight into the 6x6 system in item order (stamp_row; the reference).
+  // ORDER_COLOUR is the GPU's 16-lane body solve: the rows of item k (the
+  // body's k-th point-side in pair order, then its joint sides) are summed
+  // into slot k mod 16, in item order within the slot, each slot starting at
+  // +0; the slots are added by a pairwise tree (((s0 + s1) + (s2 + s3)) +
+  // ((s4 + s5) + (s6 + s7))) + (the same over s8-s15), and that total is
+  // added to the inertial system.  A stamp adds pen * (x_i * x_j) to
+  // lower-triangle entry (i, j) and f * x_k to rhs k, with x = (sign * axis,
+  // r x (sign * axis)), as stamp_row does per entry.
+  static const int SLOTS = 16;
+  struct RowSum {
+    bool sliced = false;
+    int item = 0;      // index of the current item in the body's list
+    T part[SLOTS][27];  // per slot: lower triangle (tri(i, j)), then rhs
+    T (*lhs)[6] = nullptr;
+    T* rhs = nullptr;
+    static int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+    void init(bool s, T (*l)[6], T* r) {
+      sliced = s;
+      lhs = l;
+      rhs = r;
+      item = 0;
+      for (auto& p : part)
+        for (auto& v : p) v = T(0);
+    }
+    T* slot() { return part[item % SLOTS]; }
+    void stamp(V3<T> axis, T pen, T f, V3<T> r, T sign) {
+      if (!sliced) return stamp_row(lhs, rhs, axis, pen, f, r, sign);
+      V3<T> nn = sign * axis;
+      V3<T> t = cross(r, nn);
+      const T x[6] = {nn.x, nn.y, nn.z, t.x, t.y, t.z};
+      T* p = slot();
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j <= i; ++j) p[tri(i, j)] += pen * (x[i] * x[j]);
+      for (int k = 0; k < 6; ++k) p[21 + k] += f * x[k];
+    }
+    // angular couple (rolling / torsion row): axis g, stiffness pen, force f
+    void couple(V3<T> g, T pen, T f) {
+      if (!sliced) {
+        for (int x = 0; x < 3; ++x)
+          for (int y = 0; y < 3; ++y) lhs[3 + x][3 + y] += pen * (g[x] * g[y]);
+        rhs[3] += f * g.x;
+        rhs[4] += f * g.y;
+        rhs[5] += f * g.z;
+        return;
+      }
+      T* p = slot();
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j <= i; ++j) p[tri(3 + i, 3 + j)] += pen * (g[i] * g[j]);
+      for (int k = 0; k < 3; ++k) p[24 + k] += f * g[k];
+    }
+    // diagonal add to the angular block (geometric stiffness)
+    void diag(V3<T> hd) {
+      if (!sliced) {
+        lhs[3][3] += hd.x;
+        lhs[4][4] += hd.y;
+        lhs[5][5] += hd.z;
+        return;
+      }
+      T* p = slot();
+      p[tri(3, 3)] += hd.x;
+      p[tri(4, 4)] += hd.y;
+      p[tri(5, 5)] += hd.z;
+    }
+    // ORDER_COLOUR: the slot tree added to the inertial system (lower
+    // triangle and rhs; the upper triangle is not read by solve_6x6).
+    void finish() {
+      if (!sliced) return;
+      for (int e = 0; e < 27; ++e) {
+        T v[SLOTS];
+        for (int k = 0; k < SLOTS; ++k) v[k] = part[k][e];
+        for (int w = SLOTS / 2; w >= 1; w /= 2)
+          for (int k = 0; k < w; ++k) v[k] = v[2 * k] + v[2 * k + 1];
+        T tot = v[0];
+        if (e < 21) {
+          int i = 0;
+          while (tri(i + 1, 0) <= e) ++i;
+          int j = e - tri(i, 0);
+          lhs[i][j] = lhs[i][j] + tot;
+        } else {
+          rhs[e - 21] = rhs[e - 21] + tot;
+        }
+      }
+    }
+  };
+
   // solve_body (assembly.rs:73-903), contact + Ball arms.
   void solve_body(int h) {
     const T DT = K<T>::DT();
@@ -1393,6 +1480,8 @@ struct Engine {
     rhs[3] = ra.x;
     rhs[4] = ra.y;
     rhs[5] = ra.z;
+    RowSum acc;
+    acc.init(order == ORDER_COLOUR, lhs, rhs);
 
     for (int pi : inc_pairs[h]) {
       const Pair<T>& p = pairs[pi];
@@ -1400,6 +1489,10 @@ struct Engine {
       T sign = is_a ? T(1) : T(-1);
       bool touching = p.gap <= T(0);
       for (const Point<T>& pt : p.pts) {
+        struct NextItem {
+          RowSum& a;
+          ~NextItem() { ++a.item; }
+        } next_item{acc};
         T c0 = touching ? gap_c0(p, pt) : T(0);
         V3<T> r_up, r_lo;
         contact_levers(p, pt, &r_up, &r_lo);
@@ -1416,7 +1509,7 @@ struct Engine {
             if (m_red > T(0)) {
               T pen_d = fmin_(T(2) * m_red / DT2 / npts, K<T>::PEN_MAX());
               T f_d = fmin_(pen_d * cn, T(0));
-              stamp_row(lhs, rhs, p.n, pen_d, f_d, r_side, sign);
+              acc.stamp(p.n, pen_d, f_d, r_side, sign);
             }
           }
           continue;
@@ -1436,7 +1529,7 @@ struct Engine {
         V3<T> rs[3] = {r_side, r_side_f, r_side_f};
         for (int row = 0; row < 3; ++row) {
           if (!row_live(cvs[row], f[row])) continue;
-          stamp_row(lhs, rhs, axes[row], pt.pen[row], f[row], rs[row], sign);
+          acc.stamp(axes[row], pt.pen[row], f[row], rs[row], sign);
         }
         T mu_roll = fmax_(bodies[p.a].rolling, bodies[p.b].rolling);
         T mu_spin = fmax_(bodies[p.a].torsion, bodies[p.b].torsion);
@@ -1452,17 +1545,17 @@ struct Engine {
             T cap = mur[li] * std::fabs(f[0]);
             T fr = clampf(K<T>::ROLL_PEN() * c + pt.roll[li], -cap, cap);
             V3<T> g = is_a ? rax[li] : -rax[li];
-            for (int x = 0; x < 3; ++x)
-              for (int y = 0; y < 3; ++y) lhs[3 + x][3 + y] += K<T>::ROLL_PEN() * (g[x] * g[y]);
-            rhs[3] += fr * g.x;
-            rhs[4] += fr * g.y;
-            rhs[5] += fr * g.z;
+            acc.couple(g, K<T>::ROLL_PEN(), fr);
           }
         }
       }
     }
 
     for (int ji : inc_joints[h]) {
+      struct NextItem {
+        RowSum& a;
+        ~NextItem() { ++a.item; }
+      } next_item{acc};
       const Joint<T>& j = joints[ji];
       bool is_a = j.a == h;
       T sign = is_a ? T(1) : T(-1);
@@ -1483,7 +1576,7 @@ struct Engine {
         T f = j.pen_l[k] * c + j.lam_l[k];
         if (!row_live(c, f)) continue;
         V3<T> r_side = is_a ? rotate(a.rot, j.la + Z) : rotate(b.rot, j.lb);
-        stamp_row(lhs, rhs, axis, j.pen_l[k], f, r_side, sign);
+        acc.stamp(axis, j.pen_l[k], f, r_side, sign);
         fl[k] = f;
       }
       // geometric stiffness (assembly.rs:589-618; rows.rs:69-88)
@@ -1502,10 +1595,9 @@ struct Engine {
       }
       V3<T> hd{length(V3<T>{hm[0][0], hm[1][0], hm[2][0]}), length(V3<T>{hm[0][1], hm[1][1], hm[2][1]}),
                length(V3<T>{hm[0][2], hm[1][2], hm[2][2]})};
-      lhs[3][3] += hd.x;
-      lhs[4][4] += hd.y;
-      lhs[5][5] += hd.z;
+      acc.diag(hd);
     }
+    acc.finish();
 
     T neg[6] = {-rhs[0], -rhs[1], -rhs[2], -rhs[3], -rhs[4], -rhs[5]};
     T dx[6];

@@ -317,9 +317,9 @@ class AvbdEngine:
                                                         _p(ok, _f32p)))
         return delta, ok
 
-    def set_primal_shape(self, lanes=32, waves=4):
-        """Primal launch shape: lanes per body, wavefronts per workgroup.
-        Bitwise neutral."""
+    def set_primal_shape(self, lanes=16, waves=2):
+        """Primal launch shape: lanes per body (16) and wavefronts per
+        workgroup (bitwise neutral)."""
         self._check(self._L.oavbd_set_primal_shape(self._ctx, int(lanes), int(waves)))
 
     def set_timing(self, on=True):

@@ -1734,21 +1734,6 @@ __global__ void k_sb_unpack(Dev d) {
 // primal: solve_body (assembly.rs:73-903) for the bodies of one colour
 // ---------------------------------------------------------------------------
 
-// Value of lane `src` of this lane's L-lane group (src a compile-time
-// constant below L): one v_readlane per group of the wavefront instead of a
-// permute through LDS.
-template <int L>
-__device__ __forceinline__ float bcast_lane(float v, int src) {
-  const int lane = (int)__lane_id();
-  float r = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-#pragma unroll
-  for (int g = 1; g < 64 / L; ++g) {
-    const float o = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), g * L + src));
-    r = lane / L == g ? o : r;
-  }
-  return r;
-}
-
 // Keeps a loaded value in registers at this point: the compiler may neither
 // sink its load past the point nor re-load it later.
 __device__ __forceinline__ void pin4(float4& v) { asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w)); }
@@ -1783,28 +1768,33 @@ __device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, 
 // Rows of one contact point-side of body h.  Row slots in the reference's
 // row order: 0 normal (or the damper), 1-2 friction, 3-5 rolling/torsion
 // (only when R == 6); returns the mask of slots written.
-template <int R>
-__device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out)[R]) {
+// `own` is the solver record (pos, rot, pos0, drot) of the body being
+// solved, loaded once per body.  Each live row is handed to `emit(kind,
+// row)` as soon as it is formed, in row order (kind 0: stamp, 1: angular
+// couple), so at most one row is held in registers.
+template <int R, class Emit>
+__device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 (&own)[4], Emit&& emit) {
   uint32_t q = it.x & 0x7fffffffu;
   bool is_a = (it.x >> 31) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
   uint32_t p = it.y;
   uint2 ab = make_uint2(it.z, it.w);
-  // The four 64-byte records this item reads (pair, point, both bodies) are
-  // loaded together and pinned before any branch, so the item costs one
+  // The three 64-byte records this item reads (pair, point, partner body)
+  // are loaded together and pinned before any branch, so the item costs one
   // memory round trip instead of one per branch that first uses a field.
   const float4* prec = d.P.rec + 4 * (size_t)p;
   const float4* qrec = d.Q.rec + 4 * (size_t)q;
-  const float4* sa = d.sb + 4 * (size_t)ab.x;
-  const float4* sbb = d.sb + 4 * (size_t)ab.y;
+  const float4* so = d.sb + 4 * (size_t)(is_a ? ab.y : ab.x);
   float4 n4 = prec[0], t14 = prec[1], t24 = prec[2], mu = prec[3];
   float4 ra4 = qrec[0], rb4 = qrec[1], lam = qrec[2], pen = qrec[3];
-  float4 pa4 = sa[0], sa1 = sa[1], a0 = sa[2], sa3 = sa[3];
-  float4 pb4 = sbb[0], sb1 = sbb[1], b0 = sbb[2], sb3 = sbb[3];
+  float4 o0 = so[0], o1 = so[1], o2 = so[2], o3 = so[3];
   pin4(n4); pin4(t14); pin4(t24); pin4(mu);
   pin4(ra4); pin4(rb4); pin4(lam); pin4(pen);
-  pin4(pa4); pin4(sa1); pin4(a0); pin4(sa3);
-  pin4(pb4); pin4(sb1); pin4(b0); pin4(sb3);
+  pin4(o0); pin4(o1); pin4(o2); pin4(o3);
+  const float4 pa4 = is_a ? own[0] : o0, sa1 = is_a ? own[1] : o1, a0 = is_a ? own[2] : o2,
+               sa3 = is_a ? own[3] : o3;
+  const float4 pb4 = is_a ? o0 : own[0], sb1 = is_a ? o1 : own[1], b0 = is_a ? o2 : own[2],
+               sb3 = is_a ? o3 : own[3];
   f3 n = xyz(n4);
   float gap = n4.w;
   bool touching = gap <= 0.0f;
@@ -1823,14 +1813,15 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
   };
   float cn = row_c(n, c0, ra_w, rb_w);
   f3 r_side = is_a ? ra_w : rb_w;
+  Row row;
   if (!touching) {
     float damp = t14.w;
     if (damp > 0.0f && fabsf(cn) >= K_C_EPS) {
       float f_d = fmin_(damp * cn, 0.0f);
-      make_stamp(out[0], n, damp, f_d, r_side, sign);
-      return 1u;
+      make_stamp(row, n, damp, f_d, r_side, sign);
+      emit(0, row);
     }
-    return 0u;
+    return;
   }
   f3 t1 = xyz(t14), t2 = xyz(t24);
   // friction_levers (rows.rs:987-996)
@@ -1853,16 +1844,21 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
     f1 = r0 * k;
     f2 = r1 * k;
   }
-  uint32_t m = 0;
   f3 r_side_f = is_a ? ra_f : rb_f;
-  make_stamp(out[0], n, pen.x, fn_c, r_side, sign);
-  make_stamp(out[1], t1, pen.y, f1, r_side_f, sign);
-  make_stamp(out[2], t2, pen.z, f2, r_side_f, sign);
-  if (fabsf(cn) >= K_C_EPS || fn_c != 0.0f) m |= 1u;
-  if (fabsf(ct1) >= K_C_EPS || f1 != 0.0f) m |= 2u;
-  if (fabsf(ct2) >= K_C_EPS || f2 != 0.0f) m |= 4u;
+  if (fabsf(cn) >= K_C_EPS || fn_c != 0.0f) {
+    make_stamp(row, n, pen.x, fn_c, r_side, sign);
+    emit(0, row);
+  }
+  if (fabsf(ct1) >= K_C_EPS || f1 != 0.0f) {
+    make_stamp(row, t1, pen.y, f1, r_side_f, sign);
+    emit(0, row);
+  }
+  if (fabsf(ct2) >= K_C_EPS || f2 != 0.0f) {
+    make_stamp(row, t2, pen.z, f2, r_side_f, sign);
+    emit(0, row);
+  }
   if constexpr (R >= 6) {
-    if (!(mu.z > 0.0f || mu.w > 0.0f)) return m;
+    if (!(mu.z > 0.0f || mu.w > 0.0f)) return;
     f3 wrel = sub(dra, drb);
     float4 rl = d.Q.roll[q];
     f3 ax[3] = {t1, t2, n};
@@ -1874,22 +1870,20 @@ __device__ __forceinline__ uint32_t point_rows(const Dev& d, uint4 it, Row (&out
       float cap = mur[li] * fabsf(fn_c);
       float fr = clampf_(K_ROLL_PEN * c + rlv[li], -cap, cap);
       f3 g = is_a ? ax[li] : neg(ax[li]);
-      Row& r = out[3 + li];
-      r.x[0] = 0.0f; r.x[1] = 0.0f; r.x[2] = 0.0f;
-      r.x[3] = g.x; r.x[4] = g.y; r.x[5] = g.z;
-      r.pen = -K_ROLL_PEN;
-      r.f = fr;
-      if (!(mur[li] <= 0.0f)) m |= 8u << li;
+      row.x[0] = 0.0f; row.x[1] = 0.0f; row.x[2] = 0.0f;
+      row.x[3] = g.x; row.x[4] = g.y; row.x[5] = g.z;
+      row.pen = -K_ROLL_PEN;
+      row.f = fr;
+      if (!(mur[li] <= 0.0f)) emit(1, row);
     }
   }
-  return m;
 }
 
 // Rows of one Ball-joint side (assembly.rs:279-310, :589-618).  Slots
 // 0-2: the three axis stamps; slot 3: the geometric-stiffness diagonal.
 // Returns the mask of slots written.
-template <int R>
-__device__ __forceinline__ uint32_t joint_rows(const Dev& d, uint32_t e, Row (&out)[R]) {
+template <class Emit>
+__device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit) {
   uint32_t j = e >> 1;
   bool is_a = (e & 1u) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
@@ -1907,16 +1901,16 @@ __device__ __forceinline__ uint32_t joint_rows(const Dev& d, uint32_t e, Row (&o
   f3 r_a = qrot(qa, xyz(d.J.la0[j]));
   f3 r_b = qrot(qb, lb);
   float fl[3] = {0.0f, 0.0f, 0.0f};
-  uint32_t m = 0;
+  Row row;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float c = comp(live, k) - K_ALPHA * comp(c0v, k);
     float f = penv[k] * c + lamv[k];
+    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
     f3 axis = zero3();
     setc(axis, k, 1.0f);
-    make_stamp(out[k], axis, penv[k], f, is_a ? r_a : r_b, sign);
-    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
-    m |= 1u << k;
+    make_stamp(row, axis, penv[k], f, is_a ? r_a : r_b, sign);
+    emit(0, row);
     fl[k] = f;
   }
   f3 r = is_a ? r_a : neg(r_b);
@@ -1933,15 +1927,13 @@ __device__ __forceinline__ uint32_t joint_rows(const Dev& d, uint32_t e, Row (&o
     for (int x = 0; x < 3; ++x)
       for (int y = 0; y < 3; ++y) hm[x][y] += g[x][y] * fl[k];
   }
-  Row& gr = out[3];
-  m |= 8u;
-  gr.x[0] = 0.0f; gr.x[1] = 0.0f; gr.x[2] = 0.0f;
-  gr.x[3] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
-  gr.x[4] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
-  gr.x[5] = len3(f3{hm[0][2], hm[1][2], hm[2][2]});
-  gr.pen = 0.0f;
-  gr.f = 0.0f;
-  return m;
+  row.x[0] = 0.0f; row.x[1] = 0.0f; row.x[2] = 0.0f;
+  row.x[3] = len3(f3{hm[0][0], hm[1][0], hm[2][0]});
+  row.x[4] = len3(f3{hm[0][1], hm[1][1], hm[2][1]});
+  row.x[5] = len3(f3{hm[0][2], hm[1][2], hm[2][2]});
+  row.pen = 0.0f;
+  row.f = 0.0f;
+  emit(2, row);
 }
 
 // solve_6x6 (assembly.rs:25-69) on the 21-entry lower triangle.
@@ -2097,237 +2089,140 @@ __device__ __forceinline__ void inertial_block(const InertialIn& in, float out[2
 
 #define WAVE 64
 
-// Elements (lower-triangle index e < 21, rhs 21 + k) a row adds to, by
-// kind: a stamp touches all 27; an angular couple the angular block and
-// angular rhs; a geometric-stiffness row the angular diagonal.
-constexpr uint32_t TOUCH_STAMP = (1u << 27) - 1u;
-constexpr uint32_t TOUCH_COUPLE = (1u << 9) | (1u << 13) | (1u << 14) | (1u << 18) | (1u << 19) | (1u << 20) |
-                                  (1u << 24) | (1u << 25) | (1u << 26);
-constexpr uint32_t TOUCH_DIAG = (1u << 9) | (1u << 14) | (1u << 20);
+// Primal solve of one colour class (solve_body, assembly.rs:73-903), sixteen
+// lanes per body: a wavefront holds four bodies, a workgroup W wavefronts.
+// Lane s of a body owns slot s: it evaluates the body's items s, s + 16,
+// s + 32, ... (point-sides in pair order, then joint sides; the three
+// 64-byte records an item reads are loaded together, one memory round trip
+// per item) and adds their rows, in item order, to 27 partial sums in
+// registers (the lower triangle of the 6x6 system, then the rhs), starting
+// at +0.  A row stamp adds pen * (x_i * x_j) to entry (i, j) and f * x_k to
+// rhs k (stamp_row's products); a rolling / torsion couple touches the
+// angular block and rhs only; a geometric-stiffness row adds to the angular
+// diagonal.  The sixteen slots are then added pairwise (tree16) with DPP
+// exchanges (every lane of the body ends with the same bits), the total is
+// added to the body's inertial block
+// (evaluated once per iteration beside the dual update), and the 6x6 LDL^T
+// solve runs on every lane of the body; slot 0 writes the pose.  The CPU
+// oracle's colour order performs exactly these operations.  R = row slots
+// per point item (4: contact rows; 6: with rolling / torsion).
+constexpr int BODY_LANES = 16;
 
-// Primal solve of one colour class, wavefront form.  A workgroup of W
-// wavefronts holds W * 64 / L bodies, L lanes per body.  Per chunk, the L
-// lanes of a body each evaluate one item (point-sides in pair order, then
-// joint-sides) into registers and store its rows in LDS, packed in
-// item-then-row order; lane j of the body then owns matrix elements j,
-// j + L, ... (21 lower-triangle entries + 6 rhs) and adds every stored row
-// to them in that order — the reference's sequential order, so the result
-// does not depend on L or W.  The accumulation is branch-free: a row that
-// does not touch an element adds -0.0, which leaves every value (zeros'
-// signs included) unchanged.  The workgroup's systems are then factored
-// and solved by the first wavefront, one lane per body.  R = row slots per
-// item.
-template <int L, int W, int R>
-__global__ void __launch_bounds__(WAVE* W, 4) k_primal(Dev d, const uint4* __restrict__ list, uint32_t count) {
-  constexpr int G = WAVE / L;                 // bodies per wavefront
-  constexpr int NB = G * W;                   // bodies per workgroup
-  constexpr int NE = (27 + L - 1) / L;        // elements per lane
-  constexpr int U = 8;                        // contact rows per LDS round trip
-  static_assert((L * R) % U == 0, "the row buffer holds whole groups");
-  constexpr int RS = 9;                       // LDS words per row: x[6], pen, f, 1.0
-  __shared__ float rows[NB][L * R * RS];
-  __shared__ float sums[NB][28];
-  __shared__ uint32_t chunks_s[NB];
+__device__ __forceinline__ void acc_stamp(float (&acc)[27], const Row& r) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) acc[tri(i, j)] += r.pen * (r.x[i] * r.x[j]);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) acc[21 + k] += r.f * r.x[k];
+}
+// Angular couple: x = (0, g), stiffness -pen.
+__device__ __forceinline__ void acc_couple(float (&acc)[27], const Row& r) {
+  const float pen = -r.pen;
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+#pragma unroll
+    for (int j = 3; j <= i; ++j) acc[tri(i, j)] += pen * (r.x[i] * r.x[j]);
+#pragma unroll
+  for (int k = 3; k < 6; ++k) acc[21 + k] += r.f * r.x[k];
+}
+// Geometric-stiffness diagonal x[3..5].
+__device__ __forceinline__ void acc_diag(float (&acc)[27], const Row& r) {
+  acc[tri(3, 3)] += r.x[3];
+  acc[tri(4, 4)] += r.x[4];
+  acc[tri(5, 5)] += r.x[5];
+}
+
+// Pairwise tree over the 16 lanes of a DPP row: v + the value of lane
+// (this ^ 1), then ^ 2 within each quad, then the mirrored lane of the
+// 8-lane half row, then the mirrored lane of the row.  After each step the
+// lanes of a group hold identical bits (a + b == b + a), so the mirrors pair
+// whole groups: the result is ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 +
+// s7)), and the same for s8-s15, added, in every lane.
+__device__ __forceinline__ float tree16(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));  // row_mirror
+  return v;
+}
+
+template <int W, int R>
+__global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restrict__ list, uint32_t count) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
   const int tid = threadIdx.x;
-  const int grp = tid / L, sub = tid % L;     // grp: body of this lane in the workgroup
-  const uint32_t slot = blockIdx.x * NB + grp;
-  const bool valid = slot < count;
+  const int slot = tid % BODY_LANES;
+  const uint32_t idx = blockIdx.x * NB + tid / BODY_LANES;
   // colour-list record: body (bit 31: has joint sides), first incidence
-  // item, item count, first joint side
+  // item, item count, first joint side.  Lanes past the list take an empty
+  // body and skip the stores.
+  const bool valid = idx < count;
   uint32_t h = 0, i0 = 0, np = 0, j0 = 0, nj = 0;
   if (valid) {
-    const uint4 cr = list[slot];
+    const uint4 cr = list[idx];
     h = cr.x & 0x7fffffffu;
     i0 = cr.y;
     np = cr.z;
     j0 = cr.w;
     if (cr.x >> 31) nj = d.b.ji_off[h + 1] - j0;
   }
+  float acc[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc[e] = 0.0f;
+  float4 own[4];
+  {
+    const float4* rec = d.sb + 4 * (size_t)h;
+    own[0] = rec[0];
+    own[1] = rec[1];
+    own[2] = rec[2];
+    own[3] = rec[3];
+  }
+  auto add_row = [&](int kind, const Row& row) {
+    if (kind == 0)
+      acc_stamp(acc, row);
+    else if (kind == 1)
+      acc_couple(acc, row);
+    else
+      acc_diag(acc, row);
+  };
+  // the next item's descriptor is fetched while the current item is
+  // evaluated, so an item waits for one memory round trip, not two
+  uint4 next = slot < np ? d.inc[i0 + slot] : make_uint4(0u, 0u, 0u, 0u);
+  for (uint32_t k = slot; k < np; k += BODY_LANES) {
+    const uint4 it = next;
+    if (k + BODY_LANES < np) next = d.inc[i0 + k + BODY_LANES];
+    point_rows<R>(d, it, own, add_row);
+  }
+  // joint sides continue the item numbering after the point-sides
   const uint32_t total = np + nj;
-  // element q of this lane: matrix entry (ia, ib) or rhs entry ia.  A
-  // contact row adds rw[isc] * (rw[ia] * rw[ibf]) to it: pen * (x_i * x_j)
-  // on matrix lanes, f * (x_k * 1) on rhs lanes (word 8 of a row is 1.0).
-  float acc[NE];
-  int ia[NE], ib[NE];
-  int ibf[NE], ibo[NE], isc[NE];
-  bool rhs[NE], skip_couple[NE];
-  uint32_t ebit[NE];
+  for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
+       k += BODY_LANES)
+    joint_rows(d, d.b.ji[j0 + (k - np)], add_row);
+  float L21[21], rhs[6];
+  const float4* inb = reinterpret_cast<const float4*>(d.inb + 28 * (size_t)h);
+  float blk[28];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    int e = sub + k * L;
-    int i = 0;
-    if (e < 21)
-      while (tri(i + 1, 0) <= e) ++i;
-    int j = e < 21 ? e - tri(i, 0) : 0;
-    rhs[k] = e >= 21;
-    ia[k] = e < 21 ? i : (e < 27 ? e - 21 : 0);
-    ib[k] = e < 21 ? j : 0;
-    ibf[k] = e < 21 ? j : 7;
-    ibo[k] = e < 21 ? j : 8;
-    isc[k] = e < 21 ? 6 : 7;
-    ebit[k] = e < 27 ? 1u << e : 0u;
-    skip_couple[k] = e < 21 ? j < 3 : e - 21 < 3;
-    acc[k] = 0.0f;
+  for (int q = 0; q < 7; ++q) {
+    const float4 v = valid ? inb[q] : make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+    blk[4 * q] = v.x;
+    blk[4 * q + 1] = v.y;
+    blk[4 * q + 2] = v.z;
+    blk[4 * q + 3] = v.w;
   }
-  if (sub == 0) chunks_s[grp] = (total + L - 1) / L;
-  __syncthreads();
-  uint32_t nch = 0;
-  for (int g = 0; g < NB; ++g) nch = chunks_s[g] > nch ? chunks_s[g] : nch;
-  // The body's inertial block (evaluated before the sweep) starts every
-  // element's sum; the rows are added to it in order.
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = sub + k * L;
-    acc[k] = (valid && e < 27) ? d.inb[28 * (size_t)h + e] : 0.0f;
-  }
-  const float* rowf = &rows[grp][0];
-  for (uint32_t c = 0; c < nch; ++c) {
-    uint32_t k = c * L + sub;
-    Row tmp[R];
-    uint32_t mask = 0;
-    if (k < np)
-      mask = point_rows<R>(d, d.inc[i0 + k], tmp);
-    else if (k < total)
-      mask = joint_rows<R>(d, d.b.ji[j0 + (k - np)], tmp);
-    // pack the rows: exclusive prefix of the row counts (below 8) over the
-    // body's lanes, from one ballot per bit of the count; rows of point items
-    // come first (the lanes below np - c * L hold points)
-    const int m = __popc(mask);
-    const int64_t pl64 = (int64_t)np - (int64_t)c * L;
-    const int pl = pl64 <= 0 ? 0 : (pl64 >= L ? L : (int)pl64);
-    const int lane = (int)__lane_id();
-    const int gbase = lane / L * L;
-    const uint64_t below = lane == 0 ? 0ull : ~0ull >> (64 - lane);
-    const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << (L & 63)) - 1ull)) << gbase;
-    const uint64_t pmask = (pl == 0 ? 0ull : ~0ull >> (64 - pl)) << gbase;
-    int w = 0, nrow = 0, nprow = 0;
+  for (int e = 0; e < 21; ++e) L21[e] = blk[e] + tree16(acc[e]);
 #pragma unroll
-    for (int bit = 0; bit < 3; ++bit) {
-      const uint64_t bm = __ballot((m >> bit) & 1);
-      w += __popcll(bm & gmask & below) << bit;
-      nrow += __popcll(bm & gmask) << bit;
-      nprow += __popcll(bm & pmask) << bit;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (!((mask >> r) & 1u)) continue;
-      float* o = &rows[grp][RS * w];
-#pragma unroll
-      for (int x = 0; x < 6; ++x) o[x] = tmp[r].x[x];
-      o[6] = tmp[r].pen;
-      o[7] = tmp[r].f;
-      o[8] = 1.0f;
-      ++w;
-    }
-    __syncthreads();
-    // contact rows: stamps (and angular couples when R == 6).  A matrix
-    // lane adds pen * (x_i * x_j) (|pen| with couples, whose pen is
-    // negative); an rhs lane adds f * (x_k * 1), which is f * x_k exactly.
-    // Rows are read U at a time, so one LDS round trip serves U rows; slots
-    // of the last group past the body's contact rows (the buffer holds whole
-    // groups) add -0.0.
-    auto add_group = [&](int t0, auto partial) {
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const float* rw = rowf + RS * (t0 + u);
-#pragma unroll
-        for (int q = 0; q < NE; ++q) {
-          float v;
-          if constexpr (R >= 6) {
-            const float pen = rw[6];
-            const float sc = rhs[q] ? 1.0f : fabsf(pen);
-            v = sc * (rw[ia[q]] * rw[ibf[q]]);
-            v = (pen < 0.0f && skip_couple[q]) ? -0.0f : v;
-          } else {
-            v = rw[isc[q]] * (rw[ia[q]] * rw[ibo[q]]);
-          }
-          if constexpr (decltype(partial)::value) v = t0 + u < nprow ? v : -0.0f;
-          acc[q] += v;
-        }
-      }
-    };
-    const int nfull = nprow / U * U;
-    for (int t0 = 0; t0 < nfull; t0 += U) add_group(t0, std::false_type{});
-    if (nfull < nprow) add_group(nfull, std::true_type{});
-    // joint rows: stamps and geometric-stiffness diagonals
-    for (int t = nprow; t < nrow; ++t) {
-      const float* rw = rowf + RS * t;
-      const float pen = rw[6], f = rw[7];
-      const uint32_t tm = pen > 0.0f ? TOUCH_STAMP : (pen < 0.0f ? TOUCH_COUPLE : TOUCH_DIAG);
-#pragma unroll
-      for (int q = 0; q < NE; ++q) {
-        const float a = rw[ia[q]], b = rw[ib[q]];
-        const float vm = fabsf(pen) * (a * b);
-        float v = rhs[q] ? f * a : vm;
-        v = pen == 0.0f ? a : v;
-        acc[q] += (tm & ebit[q]) ? v : -0.0f;
-      }
-    }
-    __syncthreads();
-  }
-  bool solved;
+  for (int k = 0; k < 6; ++k) rhs[k] = -(blk[21 + k] + tree16(acc[21 + k]));
   float dx[6];
-  if constexpr (NE == 1 && L < 64) {
-    // LDL^T factorization and forward substitution over the body's lanes,
-    // right-looking: step k finalises column k (l_ik = a_ik / d_k) and
-    // subtracts (l_ik * d_k) * l_jk from every later entry, and y_k l_rk
-    // from every later rhs entry.  Every entry sees the same operations in
-    // the same order as in solve_6x6's row-by-row form, so the bits agree.
-    const int e = sub;
-    const bool is_mat = e < 21, is_rhs = e >= 21 && e < 27;
-    const int i = ia[0], j = ib[0];  // rhs lanes: i = rhs row
-    float val = is_rhs ? -acc[0] : acc[0];
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      const float dgk = bcast_lane<L>(val, tri(k, k));
-      ok = ok && !(dgk <= K_DEGENERATE);
-      if (is_mat && j == k && i > k) val = val / dgk;
-      const float li = __shfl(val, tri(i > k ? i : k, k), L);
-      const float lj = __shfl(val, tri(j > k ? j : k, k), L);
-      if (is_mat && j > k) val -= (li * dgk) * lj;
-      const float yk = bcast_lane<L>(val, 21 + k);
-      if (is_rhs && i > k) val -= li * yk;
-    }
-    const float dgi = __shfl(val, tri(i < 6 ? i : 5, i < 6 ? i : 5), L);
-    if (is_rhs) val = val / dgi;  // z
-    if (e < 27) sums[grp][e] = val;
-    if (sub == 0) sums[grp][27] = ok ? 1.0f : 0.0f;
-    __syncthreads();
-    if (tid >= NB) return;
-    // back substitution, one lane per body
-    solved = sums[tid][27] != 0.0f;
-    const float* f = sums[tid];
-#pragma unroll
-    for (int r = 5; r >= 0; --r) {
-      float sacc = f[21 + r];
-#pragma unroll
-      for (int k = r + 1; k < 6; ++k) sacc -= f[tri(k, r)] * dx[k];
-      dx[r] = sacc;
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-      int e = sub + q * L;
-      if (e < 27) sums[grp][e] = acc[q];
-    }
-    __syncthreads();
-    if (tid >= NB) return;
-    float L21[21], rhsv[6];
-    for (int e = 0; e < 21; ++e) L21[e] = sums[tid][e];
-    for (int k = 0; k < 6; ++k) rhsv[k] = -sums[tid][21 + k];
-    solved = solve_6x6_dev(L21, rhsv, dx);
-  }
-  const uint32_t s2 = blockIdx.x * NB + tid;
-  if (s2 >= count) return;
-  const uint32_t hb = list[s2].x & 0x7fffffffu;
+  const bool solved = solve_6x6_dev(L21, rhs, dx);
+  if (!valid || slot != 0) return;
   if (solved) {
-    float4* rec = d.sb + 4 * (size_t)hb;
+    float4* rec = d.sb + 4 * (size_t)h;
     float4 p = rec[0];
     rec[0] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
     qf q = quat_integrate(asq(rec[1]), f3{dx[3], dx[4], dx[5]});
     rec[1] = pkq(q);
-    rec[3] = pk(quat_diff_vec(q, asq(d.b.rot0[hb])), 0.0f);
+    rec[3] = pk(quat_diff_vec(q, asq(d.b.rot0[h])), 0.0f);
   } else {
     atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
   }

@@ -121,8 +121,7 @@ struct oavbd_ctx {
   uint32_t colour_left1 = 0xffffffffu;  // bodies left after the first colouring pass, previous step
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
-  int lanes = 32;   // primal launch shape: lanes per body
-  int waves = 1;    // primal launch shape: wavefronts per workgroup
+  int waves = 2;    // primal launch shape: wavefronts per workgroup
   bool rows6 = false;  // rolling/torsion rows present: 6 rows per item
   bool timing = false;
   std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
@@ -642,37 +641,28 @@ static void colour_wait(oavbd_ctx* c) {
   }
 }
 
-template <int L, int W>
-static void launch_lw(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
-  constexpr int NB = (WAVE / L) * W;
+template <int W>
+static void launch_w(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;
   if (c->rows6)
-    k_primal<L, W, 6><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    k_primal8<W, 6><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
   else
-    k_primal<L, W, 4><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    k_primal8<W, 4><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
 }
 
-template <int L>
-static void launch_l(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
-  switch (c->waves) {
-    case 1: return launch_lw<L, 1>(c, list, cnt);
-    case 2: return launch_lw<L, 2>(c, list, cnt);
-    case 8: return launch_lw<L, 8>(c, list, cnt);
-    default: return launch_lw<L, 4>(c, list, cnt);
-  }
-}
-
-// Primal launch of one colour class: `lanes` lanes per body, `waves`
-// wavefronts per workgroup (the configured launch shape).  A colour of fewer
-// than PRIMAL_TAIL bodies does not fill the chip; its time is the latency of
-// its slowest body, which 64 lanes per body shorten (fewer chunks of items).
-constexpr uint32_t PRIMAL_TAIL = 2048;
+// Primal launch of one colour class: sixteen lanes per body, `waves`
+// wavefronts per workgroup (the configured launch shape; results do not
+// depend on it).  A colour of fewer than PRIMAL_TAIL bodies does not fill
+// the chip: it runs one wavefront per workgroup, spreading its bodies over
+// as many compute units as possible.
+constexpr uint32_t PRIMAL_TAIL = 8192;
 static void launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
-  if (cnt < PRIMAL_TAIL) return launch_l<64>(c, list, cnt);
-  switch (c->lanes) {
-    case 8: return launch_l<8>(c, list, cnt);
-    case 16: return launch_l<16>(c, list, cnt);
-    case 64: return launch_l<64>(c, list, cnt);
-    default: return launch_l<32>(c, list, cnt);
+  if (cnt < PRIMAL_TAIL) return launch_w<1>(c, list, cnt);
+  switch (c->waves) {
+    case 1: return launch_w<1>(c, list, cnt);
+    case 2: return launch_w<2>(c, list, cnt);
+    case 8: return launch_w<8>(c, list, cnt);
+    default: return launch_w<4>(c, list, cnt);
   }
 }
 
@@ -1357,9 +1347,8 @@ int oavbd_get_kernel_times(oavbd_ctx* c, oavbd_kernel_times* out, int reset) {
 
 int oavbd_set_primal_shape(oavbd_ctx* c, uint32_t lanes, uint32_t waves) {
   if (!c) return OAVBD_EINVAL;
-  if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64) return OAVBD_EINVAL;
+  if (lanes != BODY_LANES) return fail(c, OAVBD_EINVAL, "the primal solve runs 16 lanes per body");
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return OAVBD_EINVAL;
-  c->lanes = (int)lanes;
   c->waves = (int)waves;
   return OAVBD_OK;
 }

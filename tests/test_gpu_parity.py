@@ -190,10 +190,10 @@ def test_gpu_run_to_run_bitwise(gpu_available):
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
 
 
-@pytest.mark.parametrize("shape", [(8, 1), (16, 2), (32, 1), (32, 8), (64, 4)])
+@pytest.mark.parametrize("shape", [(16, 1), (16, 4), (16, 8)])
 def test_launch_shape_does_not_change_bits(gpu_available, shape):
-    """The primal launch shape (lanes per body, wavefronts per workgroup)
-    is a pure speed knob."""
+    """The primal launch shape (wavefronts per workgroup) is a pure speed
+    knob."""
     s = scenes.box_drop_scene(150, np.random.Generator(np.random.PCG64(17)))
     ref = AvbdEngine(scene=s)
     ref.step_n(40)
