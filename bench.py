@@ -136,6 +136,61 @@ def batched_leg(steps, loop_scenes, loop_steps):
             "speedup_vs_loop": batched / loop}
 
 
+def differentiable_leg(steps, reps, cpu_scenes, threads):
+    """1,024 throw scenes x 64 boxes (65,536 bodies): `steps` steps forward
+    with a checkpoint every 10 steps, then the adjoint back to the initial
+    velocities, masses and frictions (each checkpoint interval recomputed on
+    the device, then differentiated).  One untimed pass, then `reps` timed
+    passes; scene-steps/s of the forward alone and of forward + backward.
+    The CPU line is the oracle's forward only (the reference has no adjoint),
+    scenes across `threads` threads."""
+    import torch
+    from ornis_amd import diff, scenes
+    from ornis_amd.avbd import AvbdEngine
+    parts, allsc = scenes.diff_throws(n_scenes=1024)
+    n_scenes = len(parts)
+    targets = torch.tensor(np.stack([p["target"] for p in parts]), dtype=torch.float32)
+    dyn = allsc["body_type"] == 1
+    eng = AvbdEngine(iters=10)
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        v0 = torch.tensor(allsc["vel"], requires_grad=True)
+        mass = torch.tensor(allsc["mass"], requires_grad=True)
+        fric = torch.tensor(allsc["friction"], requires_grad=True)
+        t0 = time.perf_counter()
+        final = diff.rollout(eng, allsc, v0, mass, fric, steps, every=10)
+        t1 = time.perf_counter()
+        diff.centroid_loss(final, allsc["scene_id"], dyn, targets).backward()
+        t2 = time.perf_counter()
+        if r:
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+        print(f"bench.py: differentiable pass {r}/{reps}: forward {t1 - t0:.3f} s, backward {t2 - t1:.3f} s",
+              file=sys.stderr, flush=True)
+    eng.close()
+    f, b = float(np.mean(fwd)), float(np.mean(bwd))
+    out = {"metric": "differentiable scene-steps/s (forward + backward)", "value": n_scenes * steps / (f + b),
+           "unit": "scene-steps/s", "scenes": n_scenes, "bodies": int(len(allsc["shape"])), "steps": steps,
+           "checkpoint_every": 10, "forward_s": f, "backward_s": b,
+           "forward_only": {"value": n_scenes * steps / f, "unit": "scene-steps/s"},
+           "timing": f"host clock around each call (inputs cross to the device and the gradients back), mean of "
+                     f"{reps} passes after one untimed"}
+    if cpu_scenes:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import pyoracle as po
+        engines = [po.OracleEngine({k: v for k, v in parts[i].items() if k != "target"}, order=po.ORDER_REFERENCE,
+                                   math=po.MATH_LIBM) for i in range(cpu_scenes)]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda e: e.step_inner(steps), engines))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_scenes * steps / dt, "unit": "scene-steps/s (forward only)",
+                               "cores": threads, "kind": "port",
+                               "sample": f"CPU oracle forward, scenes 0-{cpu_scenes - 1} x {steps} steps, one oracle "
+                                         f"per scene on {threads} threads, {dt:.1f} s"}
+    return out
+
+
 def measured_traffic(window):
     """HBM bytes per k_primal launch counted by rocprofv3 (FETCH_SIZE +
     WRITE_SIZE, each scaled by the counter's reading for a plain copy of
@@ -267,6 +322,10 @@ def main():
     ap.add_argument("--loop-steps", type=int, default=30)
     ap.add_argument("--cpu-batched-scenes", type=int, default=64)
     ap.add_argument("--cpu-batched-steps", type=int, default=30)
+    ap.add_argument("--diff-steps", type=int, default=120,
+                    help="steps of the 1,024-scene differentiable workload (0: skip it)")
+    ap.add_argument("--diff-reps", type=int, default=3)
+    ap.add_argument("--cpu-diff-scenes", type=int, default=64)
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
                          "for HBM byte counters)")
@@ -380,6 +439,9 @@ def main():
         parts = None
         if args.batched_steps:
             line["batched"] = batched_leg(args.batched_steps, args.loop_scenes, args.loop_steps)
+        if args.diff_steps:
+            line["differentiable"] = differentiable_leg(args.diff_steps, args.diff_reps,
+                                                        0 if args.no_cpu else args.cpu_diff_scenes, host_cores())
         if not args.no_cpu:
             cores = host_cores()
             if captured is not None:

@@ -216,6 +216,39 @@ int oavbd_solve_systems(oavbd_ctx* ctx, const void* systems96, uint32_t n, float
 int oavbd_solve_systems_noalloc(oavbd_ctx* ctx, const void* systems96, uint32_t n, float dt, void* delta32,
                                 float* ok);
 
+/* Device checkpoints of the solver state: bodies (pose, velocities, sleep
+ * state), the contact pair table with its points' anchors, duals,
+ * penalties and stuck flags, and the joint duals -- everything a step reads
+ * from earlier steps, so stepping on from a loaded checkpoint repeats the
+ * steps taken after the save bit for bit.  Replaces: the warm-start state
+ * the reference keeps across steps in AvbdEngine's pairs / joints
+ * (crates/physics/src/avbd/mod.rs:424-464), which its snapshot helpers
+ * read (:469-636).  Slots are numbered from 0; a slot keeps its device
+ * memory until the context is destroyed. */
+int oavbd_checkpoint_save(oavbd_ctx* ctx, uint32_t slot);
+int oavbd_checkpoint_load(oavbd_ctx* ctx, uint32_t slot);
+
+/* Differentiable stepping (no counterpart in the reference).
+ * oavbd_diff_forward runs nsteps fixed steps like oavbd_step_n and keeps a
+ * checkpoint every `every` steps (they use checkpoint slots 0..).
+ * oavbd_diff_backward then returns the gradient of a loss L given dL/d(final
+ * state) (n x 13, the oavbd_get_state layout): dL/d(initial state) (n x 13;
+ * pose, velocity, angular velocity), dL/d(mass) and dL/d(friction
+ * coefficient) per body (any output may be NULL).  Each interval between
+ * checkpoints is re-run (bit for bit) while its steps are recorded, and a
+ * hand-written adjoint of the step runs backwards over them: the active set
+ * of every step -- its pairs, contact anchors, normals and friction frames,
+ * colouring, warm-start matches, sleep states and the branches of the
+ * force clamps and penalty updates -- is held fixed and everything
+ * continuous is differentiated.  A pair's friction is the larger of its
+ * bodies' coefficients (rows.rs:32-52); its adjoint goes to that body (the
+ * second on a tie).  Scenes with joints, rolling / torsion friction or
+ * kinematic bodies, and steps where a TOI clamp fires, are rejected
+ * (OAVBD_EINVAL / OAVBD_EDEVICE with a message).  Gradients are summed with
+ * float atomics, so their last bits may vary from run to run. */
+int oavbd_diff_forward(oavbd_ctx* ctx, uint32_t nsteps, uint32_t every);
+int oavbd_diff_backward(oavbd_ctx* ctx, const float* dfinal13, float* dinit13, float* dmass, float* dfriction);
+
 /* Block until all work of the context has finished. */
 int oavbd_sync(oavbd_ctx* ctx);
 

@@ -140,6 +140,7 @@ struct Point {
   T lam[3], pen[3];
   bool stuck;
   T roll[3];
+  int src_pair = -1, src_pt = -1;  // point of the previous step's pair vector inherited from
 };
 
 template <class T>
@@ -601,6 +602,73 @@ struct Engine {
   // last step's TOI clamps: distance travelled to the hit (-1: none), target
   std::vector<T> toi_t;
   std::vector<int> toi_tgt;
+
+  // Frozen active set (test support for the differentiable GPU step): a
+  // recording run keeps, per step, the pair vector (normals, gaps, anchors,
+  // stuck flags and warm-start sources of its points, the damper's
+  // imminence decision), the colouring and which bodies were awake and which
+  // fell asleep; a replay run steps through exactly that active set with
+  // its own (possibly perturbed) bodies, all continuous quantities live.
+  struct FrozenPoint {
+    V3<T> ra, rb;
+    bool stuck;
+    int src_pair, src_pt;
+  };
+  struct FrozenPair {
+    int a, b;
+    V3<T> n;
+    T gap;
+    bool damped;
+    std::vector<FrozenPoint> pts;
+  };
+  struct FrozenStep {
+    std::vector<FrozenPair> pairs;
+    std::vector<int> colour;
+    std::vector<uint8_t> awake, slept;
+    int toi_hits = 0;
+  };
+  std::vector<FrozenStep> frozen;
+  int freeze_mode = 0;  // 0 off, 1 record, 2 replay
+  size_t replay_at = 0;
+  std::vector<uint8_t> frozen_damped;  // per pair of the replayed step
+  // The continuous branches of a recorded run (force clamps, the friction
+  // cone, live rows, penalty updates and caps, the spin deadband), in
+  // evaluation order; a replay takes every such branch from here, so it
+  // evaluates the same piece of the step's piecewise-smooth map.
+  std::vector<uint8_t> btape;
+  size_t bpos = 0;
+  bool br(bool v) {
+    if (freeze_mode == 1)
+      btape.push_back(v ? 1 : 0);
+    else if (freeze_mode == 2)
+      v = btape[bpos++] != 0;
+    return v;
+  }
+  T clamp_b(T x, T lo, T hi) {
+    if (br(x < lo)) x = lo;
+    if (br(x > hi)) x = hi;
+    return x;
+  }
+  // contact_force with its branches through br (identical results when no
+  // active set is recorded or replayed).
+  void contact_force_b(T cn, const T pens[3], const T lam[3], T ct0, T ct1, const T mu[2], T f[3]) {
+    T a = pens[0] * cn + lam[0];
+    T fn_c = br(a < T(0)) ? a : T(0);
+    T r0 = pens[1] * ct0 + lam[1], r1 = pens[2] * ct1 + lam[2];
+    T b1 = std::fabs(fn_c) * mu[0];
+    T b2 = std::fabs(fn_c) * mu[1];
+    if (br(b1 <= T(0) || b2 <= T(0))) {
+      f[0] = fn_c;
+      f[1] = br(b1 <= T(0)) ? T(0) : clamp_b(r0, -b1, b1);
+      f[2] = br(b2 <= T(0)) ? T(0) : clamp_b(r1, -b2, b2);
+      return;
+    }
+    T s = (r0 / b1) * (r0 / b1) + (r1 / b2) * (r1 / b2);
+    T k = br(s > T(1)) ? T(1) / std::sqrt(s) : T(1);
+    f[0] = fn_c;
+    f[1] = r0 * k;
+    f[2] = r1 * k;
+  }
 
   bool solvable(int h) const { return bodies[h].type == BT_DYNAMIC && bodies[h].inv_mass > T(0); }
   T eff_inv_mass(int h) const { return bodies[h].type == BT_DYNAMIC ? bodies[h].inv_mass : T(0); }
@@ -1210,6 +1278,8 @@ struct Engine {
             }
           Point<T> np;
           if (old) {
+            np.src_pair = idx;
+            np.src_pt = (int)(old - pair.pts.data());
             bool keep = old->stuck && touching && !smooth;
             np.ra = keep ? old->ra : f.first;
             np.rb = keep ? old->rb : f.second;
@@ -1501,14 +1571,16 @@ struct Engine {
         if (!touching) {
           V3<T> a_vel = bodies[p.a].vel, b_vel = bodies[p.b].vel;
           T approach = fmax_(dot(b_vel - a_vel, p.n), T(0));
-          if (p.gap - approach * DT > T(0)) continue;
-          if (std::fabs(cn) >= K<T>::C_EPS()) {
+          const bool imminent = freeze_mode == 2 ? frozen_damped[pi] != 0 : !(p.gap - approach * DT > T(0));
+          if (!imminent) continue;
+          if (br(std::fabs(cn) >= K<T>::C_EPS())) {
             T npts = T(std::max<size_t>(p.pts.size(), 1));
             T ka = eff_inv_mass(p.a), kb = eff_inv_mass(p.b);
             T m_red = ka + kb > K<T>::NEAR_ZERO() ? T(1) / (ka + kb) : T(0);
             if (m_red > T(0)) {
               T pen_d = fmin_(T(2) * m_red / DT2 / npts, K<T>::PEN_MAX());
-              T f_d = fmin_(pen_d * cn, T(0));
+              T fd0 = pen_d * cn;
+              T f_d = br(fd0 < T(0)) ? fd0 : T(0);
               acc.stamp(p.n, pen_d, f_d, r_side, sign);
             }
           }
@@ -1523,12 +1595,12 @@ struct Engine {
         T ct1 = row_c_levers(p, t1, T(0), ra_f, rb_f);
         T ct2 = row_c_levers(p, t2, T(0), ra_f, rb_f);
         T f[3];
-        contact_force(cn, pt.pen, pt.lam, ct1, ct2, p.mu, f);
+        contact_force_b(cn, pt.pen, pt.lam, ct1, ct2, p.mu, f);
         V3<T> axes[3] = {p.n, t1, t2};
         T cvs[3] = {cn, ct1, ct2};
         V3<T> rs[3] = {r_side, r_side_f, r_side_f};
         for (int row = 0; row < 3; ++row) {
-          if (!row_live(cvs[row], f[row])) continue;
+          if (!br(row_live(cvs[row], f[row]))) continue;
           acc.stamp(axes[row], pt.pen[row], f[row], rs[row], sign);
         }
         T mu_roll = fmax_(bodies[p.a].rolling, bodies[p.b].rolling);
@@ -1628,16 +1700,19 @@ struct Engine {
         T pen[3] = {pt.pen[0], pt.pen[1], pt.pen[2]};
         T lam[3] = {pt.lam[0], pt.lam[1], pt.lam[2]};
         T f[3];
-        contact_force(cn, pen, lam, ct1, ct2, p.mu, f);
+        contact_force_b(cn, pen, lam, ct1, ct2, p.mu, f);
         {
           T bnd = std::fabs(f[0]) * fmax_(p.mu[0], p.mu[1]);
           T u0 = pen[1] * ct1 + lam[1], u1 = pen[2] * ct2 + lam[2];
           T fs = std::sqrt(u0 * u0 + u1 * u1);
           pt.stuck = fs <= bnd && std::sqrt(ct1 * ct1 + ct2 * ct2) < K<T>::STICK();
         }
-        if (touching && std::fabs(cn) >= K<T>::C_EPS()) {
+        if (touching && br(std::fabs(cn) >= K<T>::C_EPS())) {
           pt.lam[0] = f[0];
-          if (f[0] < T(0)) pt.pen[0] = fmin_(pt.pen[0] + K<T>::BETA() * std::fabs(cn), K<T>::PEN_MAX());
+          if (br(f[0] < T(0))) {
+            T pn = pt.pen[0] + K<T>::BETA() * std::fabs(cn);
+            pt.pen[0] = br(pn < K<T>::PEN_MAX()) ? pn : K<T>::PEN_MAX();
+          }
         }
         T b1 = std::fabs(f[0]) * p.mu[0];
         T b2 = std::fabs(f[0]) * p.mu[1];
@@ -1647,9 +1722,15 @@ struct Engine {
         if (touching) {
           pt.lam[1] = f[1];
           pt.lam[2] = f[2];
-          if (s_ell <= T(1)) {
-            if (std::fabs(ct1) >= K<T>::C_EPS()) pt.pen[1] = fmin_(pt.pen[1] + K<T>::BETA() * std::fabs(ct1), K<T>::PEN_MAX());
-            if (std::fabs(ct2) >= K<T>::C_EPS()) pt.pen[2] = fmin_(pt.pen[2] + K<T>::BETA() * std::fabs(ct2), K<T>::PEN_MAX());
+          if (br(s_ell <= T(1))) {
+            if (br(std::fabs(ct1) >= K<T>::C_EPS())) {
+              T pn = pt.pen[1] + K<T>::BETA() * std::fabs(ct1);
+              pt.pen[1] = br(pn < K<T>::PEN_MAX()) ? pn : K<T>::PEN_MAX();
+            }
+            if (br(std::fabs(ct2) >= K<T>::C_EPS())) {
+              T pn = pt.pen[2] + K<T>::BETA() * std::fabs(ct2);
+              pt.pen[2] = br(pn < K<T>::PEN_MAX()) ? pn : K<T>::PEN_MAX();
+            }
           }
         }
         T mu_roll = fmax_(bodies[p.a].rolling, bodies[p.b].rolling);
@@ -1758,6 +1839,127 @@ struct Engine {
   }
 
   // step_inner (mod.rs:846-1143), state-changing phases (events omitted).
+  void record_frozen_step() {
+    const int n = (int)bodies.size();
+    FrozenStep fs;
+    fs.toi_hits = stats.toi_hits;
+    fs.colour = colour;
+    fs.awake.assign(n, 0);
+    for (int h = 0; h < n; ++h) fs.awake[h] = solvable(h) && !asleep[h] ? 1 : 0;
+    for (const Pair<T>& p : pairs) {
+      FrozenPair fp;
+      fp.a = p.a;
+      fp.b = p.b;
+      fp.n = p.n;
+      fp.gap = p.gap;
+      T approach = fmax_(dot(bodies[p.b].vel - bodies[p.a].vel, p.n), T(0));
+      fp.damped = !(p.gap - approach * K<T>::DT() > T(0));
+      for (const Point<T>& q : p.pts) fp.pts.push_back(FrozenPoint{q.ra, q.rb, q.stuck, q.src_pair, q.src_pt});
+      fs.pairs.push_back(fp);
+    }
+    frozen.push_back(std::move(fs));
+  }
+
+  // One step through the recorded active set `replay_at` (colour order).
+  void step_frozen() {
+    const T DT = K<T>::DT();
+    const int n = (int)bodies.size();
+    const FrozenStep& fs = frozen[replay_at++];
+    stats = Stats{};
+    for (int h = 0; h < n; ++h)
+      if (fs.awake[h] && asleep[h]) wake_body(h);
+    std::vector<Pair<T>> next;
+    frozen_damped.assign(fs.pairs.size(), 0);
+    for (size_t i = 0; i < fs.pairs.size(); ++i) {
+      const FrozenPair& fp = fs.pairs[i];
+      Pair<T> p;
+      p.a = fp.a;
+      p.b = fp.b;
+      p.n = fp.n;
+      p.gap = fp.gap;
+      p.mu[0] = fmax_(bodies[fp.a].friction, bodies[fp.b].friction);
+      p.mu[1] = p.mu[0];
+      frozen_damped[i] = fp.damped ? 1 : 0;
+      for (const FrozenPoint& fq : fp.pts) {
+        Point<T> q;
+        q.ra = fq.ra;
+        q.rb = fq.rb;
+        q.stuck = fq.stuck;
+        q.src_pair = fq.src_pair;
+        q.src_pt = fq.src_pt;
+        if (fq.src_pair >= 0) {
+          const Point<T>& o = pairs[fq.src_pair].pts[fq.src_pt];
+          for (int k = 0; k < 3; ++k) {
+            q.lam[k] = o.lam[k];
+            q.pen[k] = o.pen[k];
+            q.roll[k] = o.roll[k];
+          }
+        } else {
+          for (int k = 0; k < 3; ++k) {
+            q.lam[k] = T(0);
+            q.pen[k] = K<T>::PEN_INIT();
+            q.roll[k] = T(0);
+          }
+        }
+        p.pts.push_back(q);
+      }
+      next.push_back(std::move(p));
+    }
+    pairs = std::move(next);
+    for (int h = 0; h < n; ++h) {
+      pos0[h] = bodies[h].pos;
+      rot0[h] = bodies[h].rot;
+      if (!solvable(h)) {
+        inertial[h] = bodies[h].pos;
+        inertial_rot[h] = bodies[h].rot;
+        continue;
+      }
+      const Body<T>& b = bodies[h];
+      inertial[h] = (b.pos + b.vel * DT) + gravity * (DT * DT);
+      inertial_rot[h] = quat_integrate(b.rot, b.angvel * DT, math);
+    }
+    const T AG = K<T>::ALPHA() * K<T>::GAMMA();
+    for (auto& p : pairs)
+      for (auto& pt : p.pts) {
+        for (int k = 0; k < 3; ++k) {
+          pt.lam[k] *= AG;
+          pt.roll[k] *= AG;
+        }
+        for (int k = 0; k < 3; ++k) pt.pen[k] = clamp_b(pt.pen[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
+      }
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h) || asleep[h]) continue;
+      bodies[h].pos = inertial[h];
+      bodies[h].rot = inertial_rot[h];
+    }
+    build_incidence();
+    colour = fs.colour;
+    ncolours = 0;
+    for (int c : colour) ncolours = std::max(ncolours, c + 1);
+    std::vector<std::vector<int>> cls(ncolours);
+    for (int h = 0; h < n; ++h)
+      if (colour[h] >= 0) cls[colour[h]].push_back(h);
+    for (int it = 0; it < iters; ++it) {
+      for (int c = 0; c < ncolours; ++c)
+        for (int h : cls[c]) solve_body(h);
+      dual_update();
+    }
+    for (int h = 0; h < n; ++h) {
+      if (!solvable(h) || asleep[h]) continue;
+      Body<T>& b = bodies[h];
+      b.vel = (b.pos - pos0[h]) / DT;
+      V3<T> spin = quat_diff_vec(b.rot, rot0[h], math);
+      b.angvel = br(length(spin) < K<T>::SPIN_REST()) ? V3<T>{T(0), T(0), T(0)} : spin / DT;
+      b.rot = qnormalize(b.rot);
+    }
+    for (int h = 0; h < n; ++h)
+      if (fs.slept[h]) sleep_body(h);
+    for (int h = 0; h < n; ++h) {
+      prev_pos[h] = bodies[h].pos;
+      prev_rot[h] = bodies[h].rot;
+    }
+  }
+
   void step_inner() {
     const T DT = K<T>::DT();
     int n = (int)bodies.size();
@@ -1807,7 +2009,7 @@ struct Engine {
           pt.lam[k] *= AG;
           pt.roll[k] *= AG;
         }
-        for (int k = 0; k < 3; ++k) pt.pen[k] = clampf(pt.pen[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
+        for (int k = 0; k < 3; ++k) pt.pen[k] = clamp_b(pt.pen[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
       }
     for (auto& j : joints)
       for (int k = 0; k < 3; ++k) {
@@ -1822,6 +2024,7 @@ struct Engine {
     build_incidence();
     if (order == ORDER_COLOUR) {
       colour_bodies();
+      if (freeze_mode == 1) record_frozen_step();
       std::vector<std::vector<int>> cls(ncolours);
       for (int h = 0; h < n; ++h)
         if (colour[h] >= 0) cls[colour[h]].push_back(h);
@@ -1846,7 +2049,7 @@ struct Engine {
       Body<T>& b = bodies[h];
       b.vel = (b.pos - pos0[h]) / DT;
       V3<T> spin = quat_diff_vec(b.rot, rot0[h], math);
-      b.angvel = length(spin) < K<T>::SPIN_REST() ? V3<T>{T(0), T(0), T(0)} : spin / DT;
+      b.angvel = br(length(spin) < K<T>::SPIN_REST()) ? V3<T>{T(0), T(0), T(0)} : spin / DT;
       b.rot = qnormalize(b.rot);
     }
     std::vector<uint8_t> supported(n, 0);
@@ -1876,6 +2079,10 @@ struct Engine {
       }
     }
     propagate_joint_flags(freeze, 0);
+    if (freeze_mode == 1) {
+      frozen.back().slept.assign(n, 0);
+      for (int h = 0; h < n; ++h) frozen.back().slept[h] = freeze[h] && !asleep[h] ? 1 : 0;
+    }
     for (int h = 0; h < n; ++h)
       if (freeze[h] && !asleep[h]) sleep_body(h);
     for (int h = 0; h < n; ++h) {
@@ -1973,6 +2180,33 @@ static void add_bodies(Engine<T>* E, int n, const int* shape, const float* sp, c
   }
 }
 
+template <class A, class B>
+static void convert_frozen(const Engine<A>& src, Engine<B>& dst) {
+  dst.frozen.clear();
+  for (const auto& fs : src.frozen) {
+    typename Engine<B>::FrozenStep o;
+    o.colour = fs.colour;
+    o.awake = fs.awake;
+    o.slept = fs.slept;
+    o.toi_hits = fs.toi_hits;
+    for (const auto& fp : fs.pairs) {
+      typename Engine<B>::FrozenPair q;
+      q.a = fp.a;
+      q.b = fp.b;
+      q.n = V3<B>{B(fp.n.x), B(fp.n.y), B(fp.n.z)};
+      q.gap = B(fp.gap);
+      q.damped = fp.damped;
+      for (const auto& pt : fp.pts)
+        q.pts.push_back(typename Engine<B>::FrozenPoint{V3<B>{B(pt.ra.x), B(pt.ra.y), B(pt.ra.z)},
+                                                        V3<B>{B(pt.rb.x), B(pt.rb.y), B(pt.rb.z)}, pt.stuck,
+                                                        pt.src_pair, pt.src_pt});
+      o.pairs.push_back(std::move(q));
+    }
+    dst.frozen.push_back(std::move(o));
+  }
+  dst.btape = src.btape;
+}
+
 extern "C" {
 
 void* orc_create(int prec, const float* gravity, int order, int math, int iters, int brute) {
@@ -2018,7 +2252,12 @@ void orc_step(void* h, float dt, int nsteps) {
 
 void orc_step_inner(void* h, int nsteps) {
   OracleCtx* c = (OracleCtx*)h;
-  for (int i = 0; i < nsteps; ++i) DISPATCH(c, E->step_inner());
+  for (int i = 0; i < nsteps; ++i) DISPATCH(c, {
+      if (E->freeze_mode == 2)
+        E->step_frozen();
+      else
+        E->step_inner();
+    });
 }
 
 int orc_body_count(void* h) {
@@ -2406,6 +2645,81 @@ void orc_world_inertia_f32(const float* inertia, const float* q, float* out9) {
 void orc_set_threads(void* h, int threads) {
   OracleCtx* c = (OracleCtx*)h;
   DISPATCH(c, E->threads = threads < 1 ? 1 : threads);
+}
+
+// Frozen active set: mode 1 records every step's active set, mode 2
+// replays the recorded steps from the first (orc_step_inner then steps
+// through them); orc_copy_frozen gives one engine another's recording.
+void orc_set_freeze(void* h, int mode) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    E->freeze_mode = mode;
+    E->replay_at = 0;
+    E->bpos = 0;
+    if (mode == 1) {
+      E->frozen.clear();
+      E->btape.clear();
+    }
+  });
+}
+int orc_frozen_steps(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = 0;
+  DISPATCH(c, r = (int)E->frozen.size());
+  return r;
+}
+// TOI clamps recorded in step k (a frozen step must have none).
+int orc_frozen_toi(void* h, int k) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = 0;
+  DISPATCH(c, r = E->frozen[k].toi_hits);
+  return r;
+}
+// Pair list of recorded step k: (a, b) per pair into ab (cap pairs), count returned.
+int orc_frozen_pairs(void* h, int k, uint32_t* ab, int cap) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = 0;
+  DISPATCH(c, {
+    const auto& ps = E->frozen[k].pairs;
+    r = (int)ps.size();
+    for (int i = 0; i < r && i < cap; ++i) {
+      ab[2 * i] = (uint32_t)ps[i].a;
+      ab[2 * i + 1] = (uint32_t)ps[i].b;
+    }
+  });
+  return r;
+}
+void orc_copy_frozen(void* dst, void* src) {
+  OracleCtx* d = (OracleCtx*)dst;
+  OracleCtx* s = (OracleCtx*)src;
+  if (d->prec == 64 && s->prec == 32) return convert_frozen(*s->f, *d->d);
+  if (d->prec == 64 && s->prec == 64) {
+    d->d->frozen = s->d->frozen;
+    d->d->btape = s->d->btape;
+  }
+  if (d->prec == 32 && s->prec == 32) {
+    d->f->frozen = s->f->frozen;
+    d->f->btape = s->f->btape;
+  }
+}
+
+// Exact parameters of body i in the engine's precision (finite-difference
+// checks perturb them by less than a float's resolution): initial velocity,
+// mass (inverse mass and inertia follow, as RigidBody::build sets them) and
+// friction.
+void orc_set_params_f64(void* h, int i, const double* vel3, double mass, double friction) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    auto& b = E->bodies[i];
+    if (vel3) b.vel = V3<TT>{TT(vel3[0]), TT(vel3[1]), TT(vel3[2])};
+    if (b.type == BT_DYNAMIC && mass > 0.0) {
+      b.mass = TT(mass);
+      b.inv_mass = TT(1) / b.mass;
+      b.inertia = shape_inertia(b.shape, b.sp, b.mass);
+    }
+    if (friction == friction) b.friction = TT(friction);
+  });
 }
 
 void orc_fmath_sincos(float x, float* s, float* c) { oavbd_sincosf(x, s, c); }

@@ -75,6 +75,15 @@ def lib():
         L.orc_tangent_basis_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_shape_distance_f32.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_freeze.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_frozen_steps.restype = ctypes.c_int
+        L.orc_frozen_steps.argtypes = [ctypes.c_void_p]
+        L.orc_frozen_toi.restype = ctypes.c_int
+        L.orc_frozen_toi.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_frozen_pairs.restype = ctypes.c_int
+        L.orc_frozen_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_int]
+        L.orc_copy_frozen.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p, ctypes.c_double, ctypes.c_double]
         L.orc_fmath_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
         L.orc_fmath_asin.restype = ctypes.c_float
         L.orc_fmath_asin.argtypes = [ctypes.c_float]
@@ -152,6 +161,37 @@ class OracleEngine:
     def set_threads(self, threads):
         """Pair discovery on `threads` threads (bit-identical results)."""
         self._L.orc_set_threads(self.h, int(threads))
+
+    def record_active_sets(self):
+        """Record every following step's active set (colour order only)."""
+        self._L.orc_set_freeze(self.h, 1)
+
+    def replay_active_sets(self, source=None):
+        """Step through a recorded active set (this engine's own, or
+        `source`'s) from its first step; step_inner then replays."""
+        if source is not None:
+            self._L.orc_copy_frozen(self.h, source.h)
+        self._L.orc_set_freeze(self.h, 2)
+
+    def set_params(self, vel=None, mass=None, friction=None):
+        """Exact (float64) initial velocities (n x 3), masses and friction
+        coefficients (n) of every body."""
+        for i in range(self.n):
+            v = None if vel is None else np.ascontiguousarray(vel[i], dtype=np.float64)
+            self._L.orc_set_params_f64(self.h, i, _p(v, _f64p), float(mass[i]) if mass is not None else -1.0,
+                                       float(friction[i]) if friction is not None else float("nan"))
+
+    def recorded_steps(self):
+        return self._L.orc_frozen_steps(self.h)
+
+    def recorded_toi_hits(self, k):
+        return self._L.orc_frozen_toi(self.h, int(k))
+
+    def recorded_pairs(self, k):
+        m = self._L.orc_frozen_pairs(self.h, int(k), None, 0)
+        ab = np.zeros((m, 2), np.uint32)
+        self._L.orc_frozen_pairs(self.h, int(k), _p(ab, _u32p), m)
+        return ab
 
     def step(self, dt=1.0 / 60.0, nsteps=1):
         self._L.orc_step(self.h, ctypes.c_float(dt), int(nsteps))

@@ -76,6 +76,10 @@ SIGNATURES = {
                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "oavbd_solve_systems_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_float, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "oavbd_checkpoint_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "oavbd_checkpoint_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "oavbd_diff_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "oavbd_diff_backward": (ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, _f32p, _f32p]),
     "oavbd_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
@@ -316,6 +320,30 @@ class AvbdEngine:
                                                         ctypes.c_float(dt), delta.ctypes.data_as(ctypes.c_void_p),
                                                         _p(ok, _f32p)))
         return delta, ok
+
+    def checkpoint_save(self, slot=0):
+        """Keep the whole solver state (bodies, contact pairs with their warm
+        starts, joint duals) in device checkpoint `slot`."""
+        self._check(self._L.oavbd_checkpoint_save(self._ctx, int(slot)))
+
+    def checkpoint_load(self, slot=0):
+        """Return to checkpoint `slot`; the steps that follow repeat bit for bit."""
+        self._check(self._L.oavbd_checkpoint_load(self._ctx, int(slot)))
+
+    def diff_forward(self, nsteps, every=10):
+        """nsteps differentiable steps with a checkpoint every `every` steps."""
+        self._check(self._L.oavbd_diff_forward(self._ctx, int(nsteps), int(every)))
+
+    def diff_backward(self, dfinal):
+        """Gradients through the last diff_forward: given dL/d(final state)
+        (n x 13), returns (dL/d(initial state) n x 13, dL/d(mass) n,
+        dL/d(friction) n)."""
+        g = np.ascontiguousarray(dfinal, dtype=np.float32).reshape(self.n, 13)
+        d0 = np.zeros((self.n, 13), np.float32)
+        dm = np.zeros(self.n, np.float32)
+        df = np.zeros(self.n, np.float32)
+        self._check(self._L.oavbd_diff_backward(self._ctx, _p(g, _f32p), _p(d0, _f32p), _p(dm, _f32p), _p(df, _f32p)))
+        return d0, dm, df
 
     def set_primal_shape(self, lanes=16, waves=2):
         """Primal launch shape: lanes per body (16) and wavefronts per

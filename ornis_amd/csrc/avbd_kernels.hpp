@@ -1086,6 +1086,7 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d) {
     d.Q.pen(q) = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
                              clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
     d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
+    d.Q.src[q] = m;
   } else {
     float z = 0.0f * AG;
     float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
@@ -1094,6 +1095,7 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d) {
     d.Q.lam(q) = make_float4(z, z, z, 0.0f);
     d.Q.pen(q) = make_float4(p1, p1, p1, 0.0f);
     d.Q.roll[q] = make_float4(z, z, z, 0.0f);
+    d.Q.src[q] = -1;
   }
   d.Q.pair[q] = p;
 }
@@ -1772,8 +1774,14 @@ __device__ __forceinline__ void make_stamp(Row& r, f3 axis, float pen, float f, 
 // solved, loaded once per body.  Each live row is handed to `emit(kind,
 // row)` as soon as it is formed, in row order (kind 0: stamp, 1: angular
 // couple), so at most one row is held in registers.
-template <int R, class Emit>
-__device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 (&own)[4], Emit&& emit) {
+// Partner record of a point item: the current solver record by default.
+struct SbPartner {
+  const float4* sb;
+  __device__ __forceinline__ const float4* operator()(uint32_t h) const { return sb + 4 * (size_t)h; }
+};
+template <int R, class Emit, class Partner>
+__device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 (&own)[4], Emit&& emit,
+                                           const Partner& partner) {
   uint32_t q = it.x & 0x7fffffffu;
   bool is_a = (it.x >> 31) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
@@ -1784,7 +1792,7 @@ __device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 
   // memory round trip instead of one per branch that first uses a field.
   const float4* prec = d.P.rec + 4 * (size_t)p;
   const float4* qrec = d.Q.rec + 4 * (size_t)q;
-  const float4* so = d.sb + 4 * (size_t)(is_a ? ab.y : ab.x);
+  const float4* so = partner(is_a ? ab.y : ab.x);
   float4 n4 = prec[0], t14 = prec[1], t24 = prec[2], mu = prec[3];
   float4 ra4 = qrec[0], rb4 = qrec[1], lam = qrec[2], pen = qrec[3];
   float4 o0 = so[0], o1 = so[1], o2 = so[2], o3 = so[3];
@@ -2191,7 +2199,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
   for (uint32_t k = slot; k < np; k += BODY_LANES) {
     const uint4 it = next;
     if (k + BODY_LANES < np) next = d.inc[i0 + k + BODY_LANES];
-    point_rows<R>(d, it, own, add_row);
+    point_rows<R>(d, it, own, add_row, SbPartner{d.sb});
   }
   // joint sides continue the item numbering after the point-sides
   const uint32_t total = np + nj;

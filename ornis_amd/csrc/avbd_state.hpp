@@ -124,6 +124,7 @@ struct Points {
   float4* rec;       // 4 per point: see ra / rb / lam / pen
   float4* roll;      // rolling/torsion duals xyz
   uint32_t* pair;    // owning pair
+  int* src;          // previous step's point this one inherits its duals from (-1: fresh)
   // anchor on A (body frame), w = step-start violation c0
   __host__ __device__ float4& ra(size_t q) const { return rec[4 * q]; }
   // anchor on B (body frame), w = stuck (0/1)

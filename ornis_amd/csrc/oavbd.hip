@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/oavbd.h"
+#include "avbd_diff.hpp"
 #include "avbd_kernels.hpp"
 #include "avbd_scan.hpp"
 
@@ -97,6 +98,37 @@ static void* noalloc_resolve(int device) {
 constexpr uint32_t HEAD_H = 64;
 constexpr uint32_t POST_FLAG = 63;  // word of the readback sequence flag
 
+// Device copy of everything a step reads from earlier steps (bodies, the
+// pair table with its points, joint duals): restoring one makes the next
+// steps repeat bit for bit.
+struct Snap {
+  bool valid = false;
+  size_t cap_pairs = 0, cap_pts = 0;
+  uint32_t npairs = 0, npts = 0, sleeping = 0;
+  float time_debt = 0.0f;
+  float4 *pos = nullptr, *rot = nullptr, *vel = nullptr, *angvel = nullptr, *torque = nullptr, *inertia = nullptr,
+         *prev_pos = nullptr, *prev_rot = nullptr;
+  uint8_t *asleep = nullptr, *flag_a = nullptr;
+  float* timer = nullptr;
+  uint2* ab = nullptr;
+  float4* prec = nullptr;
+  uint2* ppts = nullptr;
+  uint32_t* poff = nullptr;
+  float4 *qrec = nullptr, *qroll = nullptr;
+  uint32_t* qpair = nullptr;
+  int* qsrc = nullptr;
+  float4 *jlam = nullptr, *jpen = nullptr;
+};
+
+// One recorded step of the backward pass: device buffers (grown on demand)
+// and the host copies of its sizes and colour ranges.
+struct TapeHost {
+  Tape t{};
+  size_t cap_n = 0, cap_pairs = 0, cap_pts = 0;
+  uint32_t iters = 0;
+  std::vector<uint32_t> col_off, col_cnt;
+};
+
 struct oavbd_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -129,6 +161,13 @@ struct oavbd_ctx {
   hipEvent_t ev[5] = {};
   float ms[5] = {};
   void* noalloc = nullptr;  // this GPU's address of `scratch`
+  // differentiable stepping
+  std::vector<Snap> snaps;        // checkpoints of oavbd_diff_forward (last: the final state)
+  std::vector<TapeHost> tapes;    // recorded steps of one checkpoint interval
+  TapeHost* rec = nullptr;        // step being recorded
+  uint32_t diff_steps = 0, diff_every = 0, diff_max_pts = 0, diff_max_pairs = 0;
+  Adj adj{};
+  size_t adj_cap_n = 0, adj_cap_pts = 0, adj_cap_pairs = 0;
   std::string err;
 };
 
@@ -209,10 +248,10 @@ static void alloc_pairs(Pairs& P, size_t cap) {
   dalloc(&P.ab, cap); dalloc(&P.rec, 4 * cap); dalloc(&P.pts, cap);
 }
 static void free_points(Points& Q) {
-  dfree(Q.rec); dfree(Q.roll); dfree(Q.pair);
+  dfree(Q.rec); dfree(Q.roll); dfree(Q.pair); dfree(Q.src);
 }
 static void alloc_points(Points& Q, size_t cap) {
-  dalloc(&Q.rec, 4 * cap); dalloc(&Q.roll, cap); dalloc(&Q.pair, cap);
+  dalloc(&Q.rec, 4 * cap); dalloc(&Q.roll, cap); dalloc(&Q.pair, cap); dalloc(&Q.src, cap);
 }
 
 static size_t grow(size_t need) {
@@ -269,6 +308,19 @@ static void ensure_incidence(oavbd_ctx* c, size_t npairs, size_t npts) {
   }
 }
 
+static void snap_free(Snap& s);
+static void tape_free(TapeHost& h);
+static void adj_free(oavbd_ctx* c) {
+  for (auto& s : c->snaps) snap_free(s);
+  c->snaps.clear();
+  for (auto& t : c->tapes) tape_free(t);
+  c->tapes.clear();
+  Adj& a = c->adj;
+  dfree(a.xb); dfree(a.vb); dfree(a.x0b); dfree(a.yb); dfree(a.lb); dfree(a.lb_prev); dfree(a.pb); dfree(a.mass);
+  dfree(a.fric);
+  c->adj_cap_n = c->adj_cap_pts = c->adj_cap_pairs = 0;
+}
+
 static void free_all(oavbd_ctx* c) {
   Bodies& b = c->d.b;
   dfree(b.pos); dfree(b.rot); dfree(b.vel); dfree(b.angvel); dfree(b.torque); dfree(b.inertia); dfree(b.shape);
@@ -289,6 +341,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
+  adj_free(c);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,6 +536,153 @@ static void upload_joints(oavbd_ctx* c) {
 }
 
 // ---------------------------------------------------------------------------
+// checkpoints and recorded steps (differentiable stepping)
+// ---------------------------------------------------------------------------
+template <class T>
+static void dcopy(oavbd_ctx* c, T* dst, const T* src, size_t n) {
+  if (n) HIPC(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+}
+
+static void snap_free(Snap& s) {
+  dfree(s.pos); dfree(s.rot); dfree(s.vel); dfree(s.angvel); dfree(s.torque); dfree(s.inertia); dfree(s.prev_pos);
+  dfree(s.prev_rot); dfree(s.asleep); dfree(s.flag_a); dfree(s.timer); dfree(s.ab); dfree(s.prec); dfree(s.ppts);
+  dfree(s.poff); dfree(s.qrec); dfree(s.qroll); dfree(s.qpair); dfree(s.qsrc); dfree(s.jlam); dfree(s.jpen);
+  s.valid = false;
+  s.cap_pairs = s.cap_pts = 0;
+}
+
+static void snap_save(oavbd_ctx* c, Snap& s) {
+  Dev& d = c->d;
+  const uint32_t n = d.n, nj = d.njoints;
+  if (!s.pos) {
+    dalloc(&s.pos, n); dalloc(&s.rot, n); dalloc(&s.vel, n); dalloc(&s.angvel, n); dalloc(&s.torque, n);
+    dalloc(&s.inertia, n); dalloc(&s.prev_pos, n); dalloc(&s.prev_rot, n); dalloc(&s.asleep, n); dalloc(&s.flag_a, n);
+    dalloc(&s.timer, n); dalloc(&s.poff, n + 1); dalloc(&s.jlam, nj); dalloc(&s.jpen, nj);
+  }
+  if (d.npairs > s.cap_pairs) {
+    dfree(s.ab); dfree(s.prec); dfree(s.ppts);
+    s.cap_pairs = grow(d.npairs);
+    dalloc(&s.ab, s.cap_pairs); dalloc(&s.prec, 4 * s.cap_pairs); dalloc(&s.ppts, s.cap_pairs);
+  }
+  if (d.npts > s.cap_pts) {
+    dfree(s.qrec); dfree(s.qroll); dfree(s.qpair); dfree(s.qsrc);
+    s.cap_pts = grow(d.npts);
+    dalloc(&s.qrec, 4 * s.cap_pts); dalloc(&s.qroll, s.cap_pts); dalloc(&s.qpair, s.cap_pts); dalloc(&s.qsrc, s.cap_pts);
+  }
+  Bodies& b = d.b;
+  dcopy(c, s.pos, b.pos, n); dcopy(c, s.rot, b.rot, n); dcopy(c, s.vel, b.vel, n); dcopy(c, s.angvel, b.angvel, n);
+  dcopy(c, s.torque, b.torque, n); dcopy(c, s.inertia, b.inertia, n); dcopy(c, s.prev_pos, b.prev_pos, n);
+  dcopy(c, s.prev_rot, b.prev_rot, n); dcopy(c, s.asleep, b.asleep, n); dcopy(c, s.flag_a, b.flag_a, n);
+  dcopy(c, s.timer, b.timer, n); dcopy(c, s.poff, d.poff, n + 1);
+  dcopy(c, s.ab, d.P.ab, d.npairs); dcopy(c, s.prec, d.P.rec, 4 * (size_t)d.npairs); dcopy(c, s.ppts, d.P.pts, d.npairs);
+  dcopy(c, s.qrec, d.Q.rec, 4 * (size_t)d.npts); dcopy(c, s.qroll, d.Q.roll, d.npts); dcopy(c, s.qpair, d.Q.pair, d.npts);
+  dcopy(c, s.qsrc, d.Q.src, d.npts);
+  dcopy(c, s.jlam, d.J.lam, nj); dcopy(c, s.jpen, d.J.pen, nj);
+  s.npairs = d.npairs;
+  s.npts = d.npts;
+  s.time_debt = c->time_debt;
+  s.sleeping = c->sleeping;
+  s.valid = true;
+}
+
+static void snap_load(oavbd_ctx* c, const Snap& s) {
+  Dev& d = c->d;
+  const uint32_t n = d.n, nj = d.njoints;
+  ensure_pairs(c, s.npairs);
+  ensure_points(c, s.npts);
+  ensure_incidence(c, s.npairs, s.npts);
+  Bodies& b = d.b;
+  dcopy(c, b.pos, s.pos, n); dcopy(c, b.rot, s.rot, n); dcopy(c, b.vel, s.vel, n); dcopy(c, b.angvel, s.angvel, n);
+  dcopy(c, b.torque, s.torque, n); dcopy(c, b.inertia, s.inertia, n); dcopy(c, b.prev_pos, s.prev_pos, n);
+  dcopy(c, b.prev_rot, s.prev_rot, n); dcopy(c, b.asleep, s.asleep, n); dcopy(c, b.flag_a, s.flag_a, n);
+  dcopy(c, b.timer, s.timer, n); dcopy(c, d.poff, s.poff, n + 1);
+  dcopy(c, d.P.ab, s.ab, s.npairs); dcopy(c, d.P.rec, s.prec, 4 * (size_t)s.npairs); dcopy(c, d.P.pts, s.ppts, s.npairs);
+  dcopy(c, d.Q.rec, s.qrec, 4 * (size_t)s.npts); dcopy(c, d.Q.roll, s.qroll, s.npts); dcopy(c, d.Q.pair, s.qpair, s.npts);
+  dcopy(c, d.Q.src, s.qsrc, s.npts);
+  dcopy(c, d.J.lam, s.jlam, nj); dcopy(c, d.J.pen, s.jpen, nj);
+  d.npairs = s.npairs;
+  d.npts = s.npts;
+  c->time_debt = s.time_debt;
+  c->sleeping = s.sleeping;
+  c->may_sleep = true;  // conservative: the wake passes run
+}
+
+static void tape_free(TapeHost& h) {
+  Tape& t = h.t;
+  dfree(t.sb); dfree(t.qrec); dfree(t.ab); dfree(t.prec); dfree(t.ppts); dfree(t.qpair); dfree(t.qsrc); dfree(t.inc);
+  dfree(t.clist); dfree(t.colour); dfree(t.rot0); dfree(t.vel); dfree(t.angvel); dfree(t.inertial);
+  dfree(t.inertial_rot); dfree(t.inertia); dfree(t.slept);
+  h.cap_n = h.cap_pairs = h.cap_pts = 0;
+}
+
+// Start recording the current step (called once its set-up is queued and
+// the colour counts are on the host): the active set, the body data the
+// reverse pass reads and the solver records before the sweep.
+__global__ void k_tape_slept(Tape t, Dev d) {
+  const uint32_t h = gid();
+  if (h >= t.n) return;
+  t.slept[h] = (t.colour[h] >= 0 && d.b.asleep[h]) ? 1 : 0;
+}
+
+static void tape_begin(oavbd_ctx* c) {
+  TapeHost& h = *c->rec;
+  Dev& d = c->d;
+  Tape& t = h.t;
+  const uint32_t n = d.n, iters = c->prm.iters;
+  if (h.cap_n < n || h.iters != iters) {
+    dfree(t.sb); dfree(t.clist); dfree(t.colour); dfree(t.rot0); dfree(t.vel); dfree(t.angvel); dfree(t.inertial);
+    dfree(t.inertial_rot); dfree(t.inertia); dfree(t.slept);
+    dalloc(&t.sb, (size_t)(iters + 1) * n * 4); dalloc(&t.clist, n); dalloc(&t.colour, n); dalloc(&t.rot0, n);
+    dalloc(&t.vel, n); dalloc(&t.angvel, n); dalloc(&t.inertial, n); dalloc(&t.inertial_rot, n);
+    dalloc(&t.inertia, n); dalloc(&t.slept, n);
+    h.cap_n = n;
+    h.iters = iters;
+    dfree(t.qrec);
+    h.cap_pts = 0;
+  }
+  if (d.npairs > h.cap_pairs) {
+    dfree(t.ab); dfree(t.prec); dfree(t.ppts);
+    h.cap_pairs = grow(d.npairs);
+    dalloc(&t.ab, h.cap_pairs); dalloc(&t.prec, 4 * h.cap_pairs); dalloc(&t.ppts, h.cap_pairs);
+  }
+  if (d.npts > h.cap_pts) {
+    dfree(t.qrec); dfree(t.qpair); dfree(t.qsrc); dfree(t.inc);
+    h.cap_pts = grow(d.npts);
+    dalloc(&t.qrec, (size_t)iters * h.cap_pts * 4); dalloc(&t.qpair, h.cap_pts); dalloc(&t.qsrc, h.cap_pts);
+    dalloc(&t.inc, 2 * h.cap_pts);
+  }
+  t.qcap = h.cap_pts;
+  t.n = n;
+  t.npairs = d.npairs;
+  t.npts = d.npts;
+  t.iters = iters;
+  dcopy(c, t.ab, d.P.ab, d.npairs); dcopy(c, t.prec, d.P.rec, 4 * (size_t)d.npairs); dcopy(c, t.ppts, d.P.pts, d.npairs);
+  dcopy(c, t.qpair, d.Q.pair, d.npts); dcopy(c, t.qsrc, d.Q.src, d.npts);
+  dcopy(c, t.inc, d.inc, 2 * (size_t)d.npts);
+  dcopy(c, t.clist, d.colour_list, n); dcopy(c, t.colour, d.b.colour, n);
+  dcopy(c, t.rot0, d.b.rot0, n); dcopy(c, t.vel, d.b.vel, n); dcopy(c, t.angvel, d.b.angvel, n);
+  dcopy(c, t.inertial, d.b.inertial, n); dcopy(c, t.inertial_rot, d.b.inertial_rot, n);
+  dcopy(c, t.inertia, d.b.inertia, n);
+  dcopy(c, t.sb, d.sb, 4 * (size_t)n);
+  h.col_off = c->colour_off_h;
+  h.col_cnt = c->colour_cnt_h;
+}
+// At the start of iteration it: the point records (duals) it reads.
+static void tape_iter_start(oavbd_ctx* c, uint32_t it) {
+  Tape& t = c->rec->t;
+  dcopy(c, t.qrec + (size_t)it * t.qcap * 4, c->d.Q.rec, 4 * (size_t)c->d.npts);
+}
+// After iteration it's sweep: the solver records.
+static void tape_iter_end(oavbd_ctx* c, uint32_t it) {
+  Tape& t = c->rec->t;
+  dcopy(c, t.sb + (size_t)(it + 1) * t.n * 4, c->d.sb, 4 * (size_t)t.n);
+}
+static void tape_end(oavbd_ctx* c) {
+  Tape& t = c->rec->t;
+  k_tape_slept<<<blocks(t.n), TPB, 0, c->st>>>(t, c->d);
+}
+
+// ---------------------------------------------------------------------------
 // step phases
 // ---------------------------------------------------------------------------
 static void propagate(oavbd_ctx* c, uint8_t* flags, uint8_t value) {
@@ -557,6 +757,8 @@ static void narrow_and_merge(oavbd_ctx* c) {
   }
   d.npairs = c->counters_h[CTR_COUNT];
   d.npts = c->counters_h[CTR_COUNT + 1];
+  c->diff_max_pairs = std::max(c->diff_max_pairs, d.npairs);
+  c->diff_max_pts = std::max(c->diff_max_pts, d.npts);
   ensure_pairs(c, d.npairs);
   ensure_points(c, d.npts);
   ensure_incidence(c, d.npairs, d.npts);
@@ -742,7 +944,9 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     }
     return pool[i];
   };
+  if (c->rec) tape_begin(c);
   for (uint32_t it = 0; it < c->prm.iters; ++it) {
+    if (c->rec) tape_iter_start(c, it);
     for (uint32_t k = 0; k < ncol; ++k) {
       uint32_t cnt = c->colour_cnt_h[k];
       if (!cnt) continue;
@@ -751,6 +955,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
       if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
       np_ev += 2;
     }
+    if (c->rec) tape_iter_end(c, it);
     if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev), c->st));
     // the dual update, and the next sweep's inertial blocks
     const uint32_t next = it + 1 < c->prm.iters ? 1u : 0u;
@@ -777,6 +982,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     k_sleep_apply<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   record(c, 4);
+  if (c->rec) tape_end(c);
   c->stats.pairs = d.npairs;
   c->stats.points = d.npts;
   c->stats.candidates = c->ncand;
@@ -828,6 +1034,68 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     c->ms[4] = all;
     (void)iter;
   }
+}
+
+// ---------------------------------------------------------------------------
+// backward pass
+// ---------------------------------------------------------------------------
+__global__ void k_zero_f4(float4* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__global__ void k_zero_f(float* p, uint32_t n) {
+  const uint32_t i = gid();
+  if (i < n) p[i] = 0.0f;
+}
+static void zero_f4(oavbd_ctx* c, float4* p, size_t n) {
+  if (n) k_zero_f4<<<(uint32_t)((n + TPB - 1) / TPB), TPB, 0, c->st>>>(p, n);
+}
+
+static void adj_alloc(oavbd_ctx* c) {
+  Adj& a = c->adj;
+  const uint32_t n = c->d.n;
+  if (c->adj_cap_n < n) {
+    dfree(a.xb); dfree(a.vb); dfree(a.x0b); dfree(a.yb); dfree(a.mass); dfree(a.fric);
+    dalloc(&a.xb, 2 * (size_t)n); dalloc(&a.vb, 2 * (size_t)n); dalloc(&a.x0b, 2 * (size_t)n);
+    dalloc(&a.yb, 2 * (size_t)n); dalloc(&a.mass, n); dalloc(&a.fric, n);
+    c->adj_cap_n = n;
+  }
+  const size_t npts = std::max<size_t>(c->diff_max_pts, 1), npairs = std::max<size_t>(c->diff_max_pairs, 1);
+  if (c->adj_cap_pts < npts) {
+    dfree(a.lb); dfree(a.lb_prev);
+    dalloc(&a.lb, 2 * npts); dalloc(&a.lb_prev, 2 * npts);
+    c->adj_cap_pts = npts;
+  }
+  if (c->adj_cap_pairs < npairs) {
+    dfree(a.pb);
+    dalloc(&a.pb, npairs);
+    c->adj_cap_pairs = npairs;
+  }
+}
+
+// Reverse of one recorded step: the adjoints of its outputs (body state,
+// final duals) become adjoints of its inputs; parameter adjoints accumulate.
+static void adjoint_step(oavbd_ctx* c, TapeHost& h) {
+  Adj& a = c->adj;
+  const Tape& t = h.t;
+  // the duals' adjoints: this step's final duals <- what the next step's
+  // warm start sent back; the previous step's start from zero
+  std::swap(a.lb, a.lb_prev);
+  zero_f4(c, a.lb_prev, 2 * c->adj_cap_pts);
+  k_adj_finish<<<blocks(t.n), TPB, 0, c->st>>>(t, a);
+  for (int it = (int)t.iters - 1; it >= 0; --it) {
+    if (t.npts) k_adj_dual<<<blocks(t.npts), TPB, 0, c->st>>>(c->d, t, a, (uint32_t)it);
+    for (int k = (int)h.col_cnt.size() - 1; k >= 0; --k) {
+      const uint32_t cnt = h.col_cnt[k];
+      if (!cnt) continue;
+      constexpr int NB = (WAVE / BODY_LANES) * 2;
+      k_adj_colour<2><<<blocks(cnt, NB), WAVE * 2, 0, c->st>>>(c->d, t, a, (uint32_t)it, k, t.clist + h.col_off[k], cnt);
+    }
+  }
+  if (t.npairs) k_adj_pairs<<<blocks(t.npairs), TPB, 0, c->st>>>(c->d, t, a);
+  if (t.npts) k_adj_warm<<<blocks(t.npts), TPB, 0, c->st>>>(t, a);
+  k_adj_prelude<<<blocks(t.n), TPB, 0, c->st>>>(c->d, t, a);
+  HIPC(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1334,6 +1602,119 @@ int oavbd_solve_systems_noalloc(oavbd_ctx* c, const void* systems96, uint32_t n,
     HIPC(hipMemcpyAsync(delta32, dl, 32 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPC(hipMemcpyAsync(ok, okd, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
     HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_checkpoint_save(oavbd_ctx* c, uint32_t slot) {
+  if (!c || slot > 4096) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    if (c->snaps.size() <= slot) c->snaps.resize(slot + 1);
+    snap_save(c, c->snaps[slot]);
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_checkpoint_load(oavbd_ctx* c, uint32_t slot) {
+  if (!c) return OAVBD_EINVAL;
+  if (slot >= c->snaps.size() || !c->snaps[slot].valid) return fail(c, OAVBD_EINVAL, "no checkpoint in that slot");
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    snap_load(c, c->snaps[slot]);
+    HIPC(hipStreamSynchronize(c->st));
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_diff_forward(oavbd_ctx* c, uint32_t nsteps, uint32_t every) {
+  if (!c || every == 0) return OAVBD_EINVAL;
+  if (c->d.njoints) return fail(c, OAVBD_EINVAL, "differentiable stepping covers contact scenes (no joints)");
+  if (c->rows6) return fail(c, OAVBD_EINVAL, "differentiable stepping covers sliding friction (no rolling/torsion)");
+  for (int t : c->type_h)
+    if (t == BT_KINEMATIC) return fail(c, OAVBD_EINVAL, "differentiable stepping has no kinematic bodies");
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    const uint32_t nseg = (nsteps + every - 1) / every;
+    for (size_t k = nseg + 1; k < c->snaps.size(); ++k) snap_free(c->snaps[k]);
+    c->snaps.resize(nseg + 1);
+    c->diff_max_pairs = c->d.npairs;
+    c->diff_max_pts = c->d.npts;
+    for (uint32_t s = 0; s < nsteps; ++s) {
+      if (s % every == 0) snap_save(c, c->snaps[s / every]);
+      step_inner(c, s + 1 == nsteps);
+    }
+    snap_save(c, c->snaps[nseg]);
+    HIPC(hipStreamSynchronize(c->st));
+    c->diff_steps = nsteps;
+    c->diff_every = every;
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_diff_backward(oavbd_ctx* c, const float* dfinal13, float* dinit13, float* dmass, float* dfriction) {
+  if (!c || !dfinal13) return OAVBD_EINVAL;
+  if (!c->diff_steps) return fail(c, OAVBD_EINVAL, "oavbd_diff_forward has not run");
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    const uint32_t n = c->d.n, every = c->diff_every, nsteps = c->diff_steps;
+    const uint32_t nseg = (nsteps + every - 1) / every;
+    adj_alloc(c);
+    Adj& a = c->adj;
+    {
+      std::vector<float4> xb(2 * (size_t)n), vb(2 * (size_t)n);
+      for (uint32_t i = 0; i < n; ++i) {
+        const float* g = dfinal13 + 13 * (size_t)i;
+        xb[2 * i] = make_float4(g[0], g[1], g[2], 0.0f);
+        xb[2 * i + 1] = make_float4(g[3], g[4], g[5], g[6]);
+        vb[2 * i] = make_float4(g[7], g[8], g[9], 0.0f);
+        vb[2 * i + 1] = make_float4(g[10], g[11], g[12], 0.0f);
+      }
+      HIPC(hipMemcpyAsync(a.xb, xb.data(), xb.size() * sizeof(float4), hipMemcpyHostToDevice, c->st));
+      HIPC(hipMemcpyAsync(a.vb, vb.data(), vb.size() * sizeof(float4), hipMemcpyHostToDevice, c->st));
+    }
+    zero_f4(c, a.x0b, 2 * (size_t)n);
+    zero_f4(c, a.yb, 2 * (size_t)n);
+    zero_f4(c, a.lb, 2 * c->adj_cap_pts);
+    zero_f4(c, a.lb_prev, 2 * c->adj_cap_pts);
+    zero_f4(c, a.pb, c->adj_cap_pairs);
+    k_zero_f<<<blocks(n), TPB, 0, c->st>>>(a.mass, n);
+    k_zero_f<<<blocks(n), TPB, 0, c->st>>>(a.fric, n);
+    if (c->tapes.size() < every) c->tapes.resize(every);
+    for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+      const uint32_t len = std::min(every, nsteps - (uint32_t)seg * every);
+      snap_load(c, c->snaps[seg]);
+      for (uint32_t k = 0; k < len; ++k) {
+        c->rec = &c->tapes[k];
+        step_inner(c, true);
+        c->rec = nullptr;
+        if (c->stats.toi_hits) throw HipError{"a TOI clamp inside the differentiated steps (not differentiable)"};
+      }
+      // the recorded steps may have grown the pair / point maxima
+      if (c->diff_max_pts > c->adj_cap_pts || c->diff_max_pairs > c->adj_cap_pairs)
+        throw HipError{"recorded steps exceed the forward pass's pair / point counts"};
+      for (int k = (int)len - 1; k >= 0; --k) adjoint_step(c, c->tapes[k]);
+    }
+    snap_load(c, c->snaps[nseg]);
+    std::vector<float4> xb(2 * (size_t)n), vb(2 * (size_t)n);
+    std::vector<float> m(n), f(n);
+    HIPC(hipMemcpyAsync(xb.data(), a.xb, xb.size() * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(vb.data(), a.vb, vb.size() * sizeof(float4), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(m.data(), a.mass, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipMemcpyAsync(f.data(), a.fric, n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    HIPC(hipStreamSynchronize(c->st));
+    for (uint32_t i = 0; i < n; ++i) {
+      if (dinit13) {
+        float* o = dinit13 + 13 * (size_t)i;
+        o[0] = xb[2 * i].x; o[1] = xb[2 * i].y; o[2] = xb[2 * i].z;
+        o[3] = xb[2 * i + 1].x; o[4] = xb[2 * i + 1].y; o[5] = xb[2 * i + 1].z; o[6] = xb[2 * i + 1].w;
+        o[7] = vb[2 * i].x; o[8] = vb[2 * i].y; o[9] = vb[2 * i].z;
+        o[10] = vb[2 * i + 1].x; o[11] = vb[2 * i + 1].y; o[12] = vb[2 * i + 1].z;
+      }
+      if (dmass) dmass[i] = m[i];
+      if (dfriction) dfriction[i] = f[i];
+    }
   });
   return OAVBD_OK;
 }

@@ -249,3 +249,36 @@ def batched_drops(n_scenes=4096, nmin=8, nmax=512, seed=0xA5BD2026):
     allsc = concat(parts)
     allsc["scene_id"] = np.concatenate([np.full(len(p["shape"]), i, np.uint32) for i, p in enumerate(parts)])
     return parts, allsc
+
+
+def throw_scene(rng, n_boxes=64, half=0.25, spacing=1.6, mu=None):
+    """One differentiable-workload scene: n_boxes boxes (half-extents
+    `half`) on an 8-wide grid `spacing` apart, centres 0.3-0.6 m above a
+    ground plane at y = 0, thrown with initial velocity uniform [-3, 3] m/s
+    per axis; one friction coefficient (uniform [0.2, 0.9] unless given) for
+    every body of the scene, the plane included; masses lognormal(0, 0.3);
+    and a target point for the scene's final centroid.  Speeds stay below
+    the TOI clamp's reach (7.5 m/s for these boxes)."""
+    sb = SceneBuilder()
+    mu = float(rng.uniform(0.2, 0.9)) if mu is None else float(mu)
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=mu, restitution=0.0)
+    side = 8
+    for i in range(n_boxes):
+        gx, gz = i % side, i // side
+        pos = ((gx - (side - 1) / 2.0) * spacing, rng.uniform(0.3, 0.6), (gz - (side - 1) / 2.0) * spacing)
+        vel = tuple(rng.uniform(-3.0, 3.0, 3))
+        m = float(rng.lognormal(0.0, 0.3))
+        sb.box(pos, (half, half, half), m, vel=vel, friction=mu, restitution=0.0)
+    s = sb.build()
+    s["target"] = np.array([rng.uniform(-2.0, 2.0), 0.25, rng.uniform(-2.0, 2.0)], np.float32)
+    return s
+
+
+def diff_throws(n_scenes=1024, n_boxes=64, seed=0xA5BD2026):
+    """The differentiable workload: n_scenes independent throw scenes;
+    returns (list of scenes, concatenated scene with scene_id)."""
+    rng = np.random.Generator(np.random.PCG64(seed + 5))
+    parts = [throw_scene(rng, n_boxes) for _ in range(n_scenes)]
+    allsc = concat([{k: v for k, v in p.items() if k != "target"} for p in parts])
+    allsc["scene_id"] = np.concatenate([np.full(len(p["shape"]), i, np.uint32) for i, p in enumerate(parts)])
+    return parts, allsc
