@@ -68,6 +68,7 @@ typedef struct oavbd_stats {
   uint32_t pair_sides;  /* contact pair-sides (with points) of swept bodies  */
   uint32_t joint_sides; /* joint sides of swept bodies                       */
   uint32_t toi_casts;   /* (mover, target) casts tested by the TOI clamp     */
+  uint32_t joints;      /* Ball joints held by the context                   */
 } oavbd_stats;
 
 /* Kernel time and algorithmic bytes accumulated since the last reset
@@ -116,6 +117,73 @@ int oavbd_set_bodies(oavbd_ctx* ctx, uint32_t n, const int32_t* shape, const flo
                      const int32_t* body_type, const uint32_t* layer, const uint32_t* mask,
                      const uint8_t* trigger, const float* rolling, const float* torsion,
                      const uint32_t* scene);
+
+/* Append n bodies (same arrays and defaults as oavbd_set_bodies) after the
+ * existing ones, which keep their state, pairs, warm starts, joints and
+ * sleep state; *first_index (may be NULL) receives the first new index.
+ * Replaces: PhysicsEngine::add_body (mod.rs:1236-1239), whose handles are
+ * table indices, with the scratch of ensure_scratch (mod.rs:1183-1199):
+ * new bodies start awake with a zero sleep timer. */
+int oavbd_add_bodies(oavbd_ctx* ctx, uint32_t n, const int32_t* shape, const float* shape_param3,
+                     const float* pos3, const float* quat4, const float* vel3, const float* angvel3,
+                     const float* mass, const float* friction, const float* restitution,
+                     const int32_t* body_type, const uint32_t* layer, const uint32_t* mask,
+                     const uint8_t* trigger, const float* rolling, const float* torsion,
+                     const uint32_t* scene, uint32_t* first_index);
+
+/* Remove body h by swap-remove: the last body takes index h.  Pairs and
+ * joints on h are dropped; the others are renumbered in place (a pair keeps
+ * its (a, b) orientation, so one whose first body now has the larger index
+ * finds no match at the next step and restarts cold, as in the reference);
+ * warm starts of the rest are kept; every sleeping body wakes.  An index out
+ * of range is a no-op.  Replaces: PhysicsEngine::remove_body (engine.rs:
+ * 40-43; mod.rs:1241-1300; trigger and contact events are not produced). */
+int oavbd_remove_body(oavbd_ctx* ctx, uint32_t h);
+
+/* The contact state carried from one step to the next (the warm starts of
+ * the reference's AvbdPoint / AvbdPair, crates/physics/src/avbd/mod.rs:
+ * 253-281): each pair with its body indices, its range of points, the last
+ * step's normal and signed gap, friction tangents and coefficients; each
+ * point with its body-frame anchors, stuck flag, duals and penalties. */
+typedef struct oavbd_contact_pair {
+  uint32_t a, b;             /* bodies (a < b as discovered) */
+  uint32_t first_point;      /* index of the pair's first point */
+  uint32_t npoints;          /* 0..8 (a pair may hold no points) */
+  float normal[4];           /* xyz: unit normal from B to A; w: signed gap */
+  float tangent1[4];         /* xyz: friction tangent; w: damper stiffness */
+  float tangent2[4];         /* xyz: t1 x n */
+  float friction[4];         /* mu1, mu2, rolling, torsion */
+} oavbd_contact_pair;        /* 80 bytes */
+typedef struct oavbd_contact_point {
+  float ra[4];               /* xyz: anchor on A, body frame; w: step-start violation */
+  float rb[4];               /* xyz: anchor on B, body frame; w: stuck (0 or 1) */
+  float lam[4];              /* xyz: normal and two friction duals */
+  float pen[4];              /* xyz: their penalties */
+  float roll[4];             /* xyz: rolling and torsion duals */
+} oavbd_contact_point;       /* 80 bytes */
+
+/* Export the contact state: *npairs and *npoints receive the counts; with
+ * non-NULL arrays, cap_pairs / cap_points must hold them (else ECAPACITY).
+ * Pairs are in canonical (a, b) order with their points contiguous.
+ * Replaces: the pair vector a checkpoint or solver migration carries
+ * (mod.rs:469-636 hand over bodies, joints and event state; the warm starts
+ * are this build's addition, for bit-exact continuation). */
+int oavbd_get_contacts(oavbd_ctx* ctx, oavbd_contact_pair* pairs, uint32_t cap_pairs,
+                       oavbd_contact_point* points, uint32_t cap_points, uint32_t* npairs, uint32_t* npoints);
+
+/* Replace the contact state by an exported one (or one a host built):
+ * bodies in range and distinct, (a, b) unique, 0..8 points per pair inside
+ * the point array.  The pairs are stored sorted by (a, b). */
+int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t npairs,
+                       const oavbd_contact_point* points, uint32_t npoints);
+
+/* Set every body's sleep flag and timer (n = body count).  A dynamic body
+ * set asleep takes the state sleep_body leaves (zero velocities, inverse
+ * mass and inertia: mod.rs:822-832); one set awake from sleep gets its mass
+ * properties back (wake_body, mod.rs:833-842).  With oavbd_set_state and
+ * oavbd_set_contacts this restores a step boundary exported with
+ * oavbd_get_state, oavbd_get_sleep and oavbd_get_contacts. */
+int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
 /* Ball joint between bodies a and b with body-local anchors.  Replaces:
  * PhysicsEngine::add_joint with JointKind::Ball (mod.rs:1308-1510,

@@ -688,6 +688,56 @@ struct Engine {
     return (int)bodies.size() - 1;
   }
 
+  // remove_body (mod.rs:1241-1300): the last body moves into slot hi; pairs
+  // and joints on hi go, the rest are renumbered in place (a pair keeps its
+  // (a, b) orientation, so a renumbered pair whose first body now sorts
+  // after its second finds no match at the next merge and restarts cold);
+  // the per-body vectors swap-remove alongside, and every sleeping body
+  // wakes.  (Trigger and contact events are not modelled.)
+  void remove_body(int hi) {
+    const int n = (int)bodies.size();
+    if (hi < 0 || hi >= n) return;
+    const int last = n - 1;
+    auto swap_rm = [&](auto& v) {
+      if (hi < (int)v.size()) {
+        v[hi] = v.back();
+        v.pop_back();
+      }
+    };
+    swap_rm(bodies);
+    auto map = [&](int h) { return h == last ? hi : h; };
+    std::vector<Pair<T>> kept;
+    kept.reserve(pairs.size());
+    for (auto& p : pairs) {
+      if (p.a == hi || p.b == hi) continue;
+      p.a = map(p.a);
+      p.b = map(p.b);
+      kept.push_back(std::move(p));
+    }
+    pairs = std::move(kept);
+    std::vector<Joint<T>> kj;
+    for (auto& j : joints) {
+      if (j.a == hi || j.b == hi) continue;
+      j.a = map(j.a);
+      j.b = map(j.b);
+      kj.push_back(j);
+    }
+    joints = std::move(kj);
+    swap_rm(sleep_timer);
+    swap_rm(asleep);
+    swap_rm(prev_pos);
+    swap_rm(prev_rot);
+    // step scratch of this restatement, kept parallel to the bodies
+    swap_rm(pos0);
+    swap_rm(rot0);
+    swap_rm(inertial);
+    swap_rm(inertial_rot);
+    swap_rm(colour);
+    swap_rm(toi_t);
+    swap_rm(toi_tgt);
+    for (int h = 0; h < (int)bodies.size(); ++h) wake_body(h);
+  }
+
   // add_joint, Ball arm (mod.rs:1308-1510; joint.rs:1273-1279).
   int add_ball_joint(int a, int b, V3<T> la, V3<T> lb) {
     wake_body(a);
@@ -2347,6 +2397,62 @@ void orc_get_sleep(void* h, uint8_t* asleep, float* timer) {
     for (size_t i = 0; i < E->bodies.size(); ++i) {
       if (asleep) asleep[i] = E->asleep[i];
       if (timer) timer[i] = (float)E->sleep_timer[i];
+    }
+  });
+}
+
+void orc_remove_body(void* h, int hi) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, E->remove_body(hi));
+}
+
+// Replace the pair table (the warm-start state carried between steps):
+// pair i is (ab[2i], ab[2i+1]) with normal / gap n4[4i..], friction
+// mu2[2i..] and points pts2[2i] .. pts2[2i] + pts2[2i+1] - 1 of `point20`,
+// 20 floats each: ra3 _ rb3 stuck lam3 _ pen3 _ roll3 _.
+void orc_set_contacts(void* h, int npairs, const uint32_t* ab, const float* n4, const float* mu2,
+                      const uint32_t* pts2, const float* point20) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->pairs.clear();
+    for (int i = 0; i < npairs; ++i) {
+      Pair<TT> p;
+      p.a = (int)ab[2 * i];
+      p.b = (int)ab[2 * i + 1];
+      p.n = {TT(n4[4 * i]), TT(n4[4 * i + 1]), TT(n4[4 * i + 2])};
+      p.gap = TT(n4[4 * i + 3]);
+      p.mu[0] = TT(mu2[2 * i]);
+      p.mu[1] = TT(mu2[2 * i + 1]);
+      for (uint32_t k = 0; k < pts2[2 * i + 1]; ++k) {
+        const float* r = point20 + 20 * (size_t)(pts2[2 * i] + k);
+        Point<TT> q;
+        q.ra = {TT(r[0]), TT(r[1]), TT(r[2])};
+        q.rb = {TT(r[4]), TT(r[5]), TT(r[6])};
+        q.stuck = r[7] != 0.0f;
+        for (int j = 0; j < 3; ++j) {
+          q.lam[j] = TT(r[8 + j]);
+          q.pen[j] = TT(r[12 + j]);
+          q.roll[j] = TT(r[16 + j]);
+        }
+        p.pts.push_back(q);
+      }
+      E->pairs.push_back(std::move(p));
+    }
+  });
+}
+
+// Sleep flags and timers; a sleeping dynamic body also takes the state
+// sleep_body leaves (zero velocities, inverse mass and inertia).
+void orc_set_sleep(void* h, const uint8_t* asleep, const float* timer) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    for (size_t i = 0; i < E->bodies.size(); ++i) {
+      E->sleep_timer[i] = TT(timer[i]);
+      E->asleep[i] = 0;
+      if (asleep[i]) E->sleep_body((int)i);
+      E->asleep[i] = asleep[i] ? 1 : 0;
     }
   });
 }

@@ -83,6 +83,10 @@ def lib():
         L.orc_frozen_pairs.restype = ctypes.c_int
         L.orc_frozen_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_int]
         L.orc_copy_frozen.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_pose.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
+        L.orc_remove_body.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_contacts.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, _f32p, _f32p, _u32p, _f32p]
+        L.orc_set_sleep.argtypes = [ctypes.c_void_p, _u8p, _f32p]
         L.orc_set_params_f64.argtypes = [ctypes.c_void_p, ctypes.c_int, _f64p, ctypes.c_double, ctypes.c_double]
         L.orc_fmath_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
         L.orc_fmath_asin.restype = ctypes.c_float
@@ -149,6 +153,37 @@ class OracleEngine:
         self._L.orc_add_bodies(self.h, n, _p(shape, _i32p), *[_p(a, _f32p) for a in arrs], _p(layer, _u32p),
                                _p(mask, _u32p), _p(trig, _u8p), _p(btype, _i32p), _p(roll, _f32p), _p(tors, _f32p))
         self.n += n
+
+    def set_pose(self, i, pos, quat):
+        """Overwrite body i's pose (a driver teleport)."""
+        self._L.orc_set_pose(self.h, int(i), _p(f32(pos), _f32p), _p(f32(quat), _f32p))
+
+    def remove_body(self, h):
+        """Swap-remove body h (the reference's remove_body)."""
+        if 0 <= int(h) < self.n:
+            self._L.orc_remove_body(self.h, int(h))
+            self.n -= 1
+
+    def set_contacts(self, pairs, points):
+        """Replace the pair table by the library's export form
+        (oavbd_get_contacts): structured arrays with fields a, b,
+        first_point, npoints, normal (xyz + gap), friction, and ra, rb
+        (w: stuck), lam, pen, roll."""
+        ab = np.ascontiguousarray(np.stack([pairs["a"], pairs["b"]], axis=1), dtype=np.uint32)
+        n4 = np.ascontiguousarray(pairs["normal"], dtype=np.float32)
+        mu2 = np.ascontiguousarray(pairs["friction"][:, 0:2], dtype=np.float32)
+        pts = np.ascontiguousarray(np.stack([pairs["first_point"], pairs["npoints"]], axis=1), dtype=np.uint32)
+        pt = np.ascontiguousarray(np.concatenate([points[k] for k in ("ra", "rb", "lam", "pen", "roll")], axis=1),
+                                  dtype=np.float32)
+        self._L.orc_set_contacts(self.h, len(ab), _p(ab, _u32p), _p(n4, _f32p), _p(mu2, _f32p), _p(pts, _u32p),
+                                 _p(pt, _f32p))
+
+    def set_sleep(self, asleep, timer):
+        """Sleep flags and timers of every body (sleeping bodies take the
+        state sleep_body leaves)."""
+        a = np.ascontiguousarray(asleep, dtype=np.uint8)
+        t = f32(timer)
+        self._L.orc_set_sleep(self.h, _p(a, _u8p), _p(t, _f32p))
 
     def __del__(self):
         try:

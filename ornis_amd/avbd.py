@@ -35,7 +35,7 @@ class OavbdParams(ctypes.Structure):
 class OavbdStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in (
         "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
-        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts")]
+        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts", "joints")]
 
 
 class OavbdKernelTimes(ctypes.Structure):
@@ -43,6 +43,14 @@ class OavbdKernelTimes(ctypes.Structure):
                 ("dual_bytes", ctypes.c_double), ("primal_launches", ctypes.c_uint64),
                 ("dual_launches", ctypes.c_uint64), ("steps", ctypes.c_uint64)]
 
+
+# oavbd_contact_pair / oavbd_contact_point (include/oavbd.h), 80 bytes each.
+CONTACT_PAIR = np.dtype([("a", "<u4"), ("b", "<u4"), ("first_point", "<u4"), ("npoints", "<u4"),
+                         ("normal", "<f4", 4), ("tangent1", "<f4", 4), ("tangent2", "<f4", 4),
+                         ("friction", "<f4", 4)])
+CONTACT_POINT = np.dtype([("ra", "<f4", 4), ("rb", "<f4", 4), ("lam", "<f4", 4), ("pen", "<f4", 4),
+                          ("roll", "<f4", 4)])
+assert CONTACT_PAIR.itemsize == 80 and CONTACT_POINT.itemsize == 80
 
 # Every entry point of include/oavbd.h with its ctypes signature.
 SIGNATURES = {
@@ -52,6 +60,14 @@ SIGNATURES = {
     "oavbd_destroy": (None, [ctypes.c_void_p]),
     "oavbd_set_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                         _f32p, _f32p, _f32p, _i32p, _u32p, _u32p, _u8p, _f32p, _f32p, _u32p]),
+    "oavbd_add_bodies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                        _f32p, _f32p, _f32p, _i32p, _u32p, _u32p, _u8p, _f32p, _f32p, _u32p, _u32p]),
+    "oavbd_remove_body": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "oavbd_get_contacts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.c_uint32, _u32p, _u32p]),
+    "oavbd_set_contacts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                          ctypes.c_uint32]),
+    "oavbd_set_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
     "oavbd_add_ball_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                             _u32p]),
     "oavbd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
@@ -178,13 +194,11 @@ class AvbdEngine:
         mj = self._max_joints or max(nj, 64)
         self._check(self._L.oavbd_create(ctypes.byref(prm), mb, mj, self._device, ctypes.byref(self._ctx)))
 
-    def load(self, s, scene_ids=None):
-        """Replace all bodies and joints with a scene (see ornis_amd.scenes)."""
+    @staticmethod
+    def _body_arrays(s, scene_ids=None):
+        """The oavbd_set_bodies / oavbd_add_bodies arrays of a scene dict,
+        kept alive in the returned tuple, and the pointer arguments."""
         n = len(s["shape"])
-        joints = s.get("joints", [])
-        if "gravity" in s:
-            self._gravity = tuple(float(g) for g in s["gravity"])
-        self._ensure_ctx(n, len(joints))
         c = np.ascontiguousarray
         shape = c(s["shape"], dtype=np.int32)
         f = {k: c(s[k], dtype=np.float32) for k in ("shape_param", "pos", "quat", "vel", "angvel", "mass",
@@ -197,16 +211,66 @@ class AvbdEngine:
         tors = c(s.get("torsion", np.zeros(n)), dtype=np.float32)
         sid = s.get("scene_id") if scene_ids is None else scene_ids
         sid = None if sid is None else c(sid, dtype=np.uint32)
-        self._keep = (shape, f, btype, layer, mask, trig, roll, tors, sid)
-        self._check(self._L.oavbd_set_bodies(
-            self._ctx, n, _p(shape, _i32p), _p(f["shape_param"], _f32p), _p(f["pos"], _f32p), _p(f["quat"], _f32p),
-            _p(f["vel"], _f32p), _p(f["angvel"], _f32p), _p(f["mass"], _f32p), _p(f["friction"], _f32p),
-            _p(f["restitution"], _f32p), _p(btype, _i32p), _p(layer, _u32p), _p(mask, _u32p), _p(trig, _u8p),
-            _p(roll, _f32p), _p(tors, _f32p), _p(sid, _u32p)))
+        keep = (shape, f, btype, layer, mask, trig, roll, tors, sid)
+        args = (n, _p(shape, _i32p), _p(f["shape_param"], _f32p), _p(f["pos"], _f32p), _p(f["quat"], _f32p),
+                _p(f["vel"], _f32p), _p(f["angvel"], _f32p), _p(f["mass"], _f32p), _p(f["friction"], _f32p),
+                _p(f["restitution"], _f32p), _p(btype, _i32p), _p(layer, _u32p), _p(mask, _u32p), _p(trig, _u8p),
+                _p(roll, _f32p), _p(tors, _f32p), _p(sid, _u32p))
+        return keep, args
+
+    def load(self, s, scene_ids=None):
+        """Replace all bodies and joints with a scene (see ornis_amd.scenes)."""
+        n = len(s["shape"])
+        joints = s.get("joints", [])
+        if "gravity" in s:
+            self._gravity = tuple(float(g) for g in s["gravity"])
+        self._ensure_ctx(n, len(joints))
+        self._keep, args = self._body_arrays(s, scene_ids)
+        self._check(self._L.oavbd_set_bodies(self._ctx, *args))
         self.n = n
         self.njoints = 0
         for (a, b, la, lb) in joints:
             self.add_ball_joint(a, b, la, lb)
+
+    # PhysicsEngine::add_body / remove_body
+    def add_bodies(self, s):
+        """Append the bodies of scene dict `s` (no joints); returns the
+        first new index.  Existing bodies keep state and warm starts."""
+        keep, args = self._body_arrays(s)
+        first = ctypes.c_uint32()
+        self._check(self._L.oavbd_add_bodies(self._ctx, *args, ctypes.byref(first)))
+        self.n += args[0]
+        return first.value
+
+    def remove_body(self, h):
+        """Swap-remove body h (the last body takes index h)."""
+        if not 0 <= int(h) < self.n:
+            return
+        self._check(self._L.oavbd_remove_body(self._ctx, int(h)))
+        self.n -= 1
+        self.njoints = self.stats()["joints"]
+
+    def contacts(self):
+        """The contact state carried between steps: (pairs, points) as
+        structured arrays of CONTACT_PAIR / CONTACT_POINT."""
+        m, q = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._L.oavbd_get_contacts(self._ctx, None, 0, None, 0, ctypes.byref(m), ctypes.byref(q)))
+        pairs = np.zeros(m.value, CONTACT_PAIR)
+        points = np.zeros(q.value, CONTACT_POINT)
+        self._check(self._L.oavbd_get_contacts(self._ctx, pairs.ctypes.data, m.value, points.ctypes.data, q.value,
+                                               ctypes.byref(m), ctypes.byref(q)))
+        return pairs, points
+
+    def set_contacts(self, pairs, points):
+        pairs = np.ascontiguousarray(pairs, dtype=CONTACT_PAIR)
+        points = np.ascontiguousarray(points, dtype=CONTACT_POINT)
+        self._check(self._L.oavbd_set_contacts(self._ctx, pairs.ctypes.data, len(pairs), points.ctypes.data,
+                                               len(points)))
+
+    def set_sleep(self, asleep, timer):
+        a = np.ascontiguousarray(asleep, dtype=np.uint8)
+        t = np.ascontiguousarray(timer, dtype=np.float32)
+        self._check(self._L.oavbd_set_sleep(self._ctx, _p(a, _u8p), _p(t, _f32p), self.n))
 
     def add_ball_joint(self, a, b, la, lb):
         la = np.ascontiguousarray(la, dtype=np.float32)

@@ -27,6 +27,9 @@ namespace {
 struct HipError {
   std::string msg;
 };
+struct ArgError {  // an invalid argument found while working (EINVAL)
+  std::string msg;
+};
 
 #define HIPC(expr)                                                                                   \
   do {                                                                                               \
@@ -120,6 +123,16 @@ struct Snap {
   float4 *jlam = nullptr, *jpen = nullptr;
 };
 
+// The arrays oavbd_set_bodies was given, defaults filled in: bodies are
+// appended or swap-removed here and the device table rebuilt from them.
+struct BodyInputs {
+  uint32_t n = 0;
+  std::vector<int32_t> shape, type;
+  std::vector<float> sp3, pos3, quat4, vel3, ang3, mass, friction, restitution, rolling, torsion;
+  std::vector<uint32_t> layer, mask, scene;
+  std::vector<uint8_t> trigger;
+};
+
 // One recorded step of the backward pass: device buffers (grown on demand)
 // and the host copies of its sizes and colour ranges.
 struct TapeHost {
@@ -144,6 +157,7 @@ struct oavbd_ctx {
   uint32_t post_seq = 0, colour_seq = 0;
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
+  BodyInputs inputs;  // the body table as last uploaded (set_bodies / add / remove)
   std::vector<int32_t> kind_h, type_h;
   float time_debt = 0.0f;
   uint32_t sleeping = 0;
@@ -182,6 +196,8 @@ static int fail(oavbd_ctx* c, int code, const std::string& m) {
       __VA_ARGS__;                                          \
     } catch (const HipError& e_) {                          \
       return fail((c), OAVBD_EDEVICE, e_.msg);              \
+    } catch (const ArgError& e_) {                          \
+      return fail((c), OAVBD_EINVAL, e_.msg);               \
     } catch (const std::bad_alloc&) {                       \
       return fail((c), OAVBD_ENOMEM, "host allocation failed"); \
     }                                                       \
@@ -469,6 +485,15 @@ static uint32_t scan_total(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint
 static void upload_joints(oavbd_ctx* c) {
   Dev& d = c->d;
   uint32_t n = d.n, nj = (uint32_t)c->joints.size();
+  // joints already on the device keep their duals (a new joint does not
+  // reset the others' warm starts: add_joint only pushes, mod.rs:1308-1510)
+  const uint32_t kept = std::min(d.njoints, nj);
+  std::vector<float4> kept_lam(kept), kept_pen(kept);
+  if (kept) {
+    HIPC(hipStreamSynchronize(c->st));
+    HIPC(hipMemcpy(kept_lam.data(), d.J.lam, kept * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(kept_pen.data(), d.J.pen, kept * sizeof(float4), hipMemcpyDeviceToHost));
+  }
   dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
   dfree(d.b.jp); dfree(d.b.ji);
   dalloc(&d.J.ab, nj); dalloc(&d.J.la, nj); dalloc(&d.J.lb, nj); dalloc(&d.J.la0, nj); dalloc(&d.J.lam, nj);
@@ -483,8 +508,8 @@ static void upload_joints(oavbd_ctx* c) {
     lb[j] = make_float4(h.lb[0], h.lb[1], h.lb[2], 0.0f);
     // the reference stamps A's lever as la + Vec3::ZERO (assembly.rs:297-304)
     la0[j] = make_float4(h.la[0] + 0.0f, h.la[1] + 0.0f, h.la[2] + 0.0f, 0.0f);
-    lam[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    pen[j] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    lam[j] = j < kept ? kept_lam[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     jp_cnt[h.a]++;
     jp_cnt[h.b]++;
     ji_cnt[h.a]++;
@@ -531,6 +556,7 @@ static void upload_joints(oavbd_ctx* c) {
     HIPC(hipMemcpy(d.b.ji, ji.data(), ji.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
   HIPC(hipMemcpy(d.b.comp, comp.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   d.njoints = nj;
   c->joints_dirty = false;
 }
@@ -1133,6 +1159,7 @@ int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_j
     for (auto& e : c->ev) HIPC(hipEventCreate(&e));
     c->noalloc = noalloc_resolve(device);
     alloc_bodies(c, max_bodies);
+    HIPC(hipDeviceSynchronize());  // the null-stream clears land before the context stream runs
   } catch (const HipError& e) {
     std::string m = e.msg;
     oavbd_destroy(c);
@@ -1158,22 +1185,38 @@ void oavbd_destroy(oavbd_ctx* c) {
   delete c;
 }
 
-int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float* sp3, const float* pos3,
-                     const float* quat4, const float* vel3, const float* angvel3, const float* mass,
-                     const float* friction, const float* restitution, const int32_t* body_type,
-                     const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger, const float* rolling,
-                     const float* torsion, const uint32_t* scene) {
-  if (!c) return OAVBD_EINVAL;
-  if (n > c->max_bodies) return fail(c, OAVBD_ECAPACITY, "more bodies than max_bodies");
-  if (!shape || !sp3 || !pos3 || !quat4 || !vel3 || !angvel3 || !mass || !friction || !restitution || !body_type)
-    return fail(c, OAVBD_EINVAL, "null body array");
-  for (uint32_t i = 0; i < n; ++i) {
-    if (shape[i] < 0 || shape[i] > 2) return fail(c, OAVBD_EINVAL, "bad shape kind");
-    if (body_type[i] < 0 || body_type[i] > 2) return fail(c, OAVBD_EINVAL, "bad body type");
-    if (shape[i] == SH_HALFSPACE && body_type[i] == BT_DYNAMIC && mass[i] > 0.0f)
-      return fail(c, OAVBD_EINVAL, "dynamic half-space");
+// Awake mass properties (RigidBody::build / wake_restore, body.rs:346-376,
+// :412-418; shape.rs:751-767): inverse mass and body-frame inertia
+// diagonal of a dynamic body, zero otherwise.
+static void awake_mass(int shape, f3 p, int type, float m, float* inv, f3* I) {
+  *inv = 0.0f;
+  *I = zero3();
+  if (type != BT_DYNAMIC) return;
+  *inv = 1.0f / m;
+  if (shape == SH_SPHERE) {
+    float v = 0.4f * m * p.x * p.x;
+    *I = f3{v, v, v};
+  } else if (shape == SH_BOX) {
+    f3 s2 = mul(p, 2.0f);
+    float x = s2.x * s2.x, y = s2.y * s2.y, z = s2.z * s2.z;
+    float k = m / 12.0f;
+    *I = f3{k * (y + z), k * (z + x), k * (x + y)};
   }
-  GUARD(c, {
+}
+
+// Uploads `in` as the whole body table: derived per-body data, scene ranks,
+// the broadphase's large-body lists, and a cold start (no pairs, joints or
+// warm starts, everything awake).
+static void set_bodies_impl(oavbd_ctx* c, const BodyInputs& in) {
+  const uint32_t n = in.n;
+  const int32_t* shape = in.shape.data();
+  const float *sp3 = in.sp3.data(), *pos3 = in.pos3.data(), *quat4 = in.quat4.data(), *vel3 = in.vel3.data(),
+              *angvel3 = in.ang3.data(), *mass = in.mass.data(), *friction = in.friction.data(),
+              *restitution = in.restitution.data(), *rolling = in.rolling.data(), *torsion = in.torsion.data();
+  const int32_t* body_type = in.type.data();
+  const uint32_t *layer = in.layer.data(), *mask = in.mask.data(), *scene = in.scene.data();
+  const uint8_t* trigger = in.trigger.data();
+  {
     HIPC(hipSetDevice(c->device));
     Dev& d = c->d;
     d.n = n;
@@ -1190,21 +1233,9 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
       if (t == BT_DYNAMIC && !(m > 0.0f)) t = BT_STATIC;  // RigidBody::build
       c->type_h[i] = t;
       f3 p{sp3[3 * i], sp3[3 * i + 1], sp3[3 * i + 2]};
-      float inv = 0.0f, mm = 0.0f;
+      float inv = 0.0f, mm = t == BT_DYNAMIC ? m : 0.0f;
       f3 I = zero3();
-      if (t == BT_DYNAMIC) {
-        mm = m;
-        inv = 1.0f / m;
-        if (shape[i] == SH_SPHERE) {
-          float v = 0.4f * m * p.x * p.x;
-          I = f3{v, v, v};
-        } else if (shape[i] == SH_BOX) {
-          f3 s2 = mul(p, 2.0f);
-          float x = s2.x * s2.x, y = s2.y * s2.y, z = s2.z * s2.z;
-          float k = m / 12.0f;
-          I = f3{k * (y + z), k * (z + x), k * (x + y)};
-        }
-      }
+      awake_mass(shape[i], p, t, m, &inv, &I);
       float br = shape[i] == SH_SPHERE ? p.x : (shape[i] == SH_BOX ? sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) : INFINITY);
       brad[i] = br;
       pos[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], inv);
@@ -1337,6 +1368,416 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
     ensure_pairs(c, 1024);
     ensure_points(c, 4096);
     ensure_incidence(c, 1024, 4096);
+    // the uploads above ran on the null stream, which the context's
+    // non-blocking stream does not wait for
+    HIPC(hipDeviceSynchronize());
+  }
+}
+
+// Validates n bodies given as the oavbd_set_bodies arrays and appends them,
+// defaults filled in, to `in`.
+static int append_inputs(oavbd_ctx* c, BodyInputs& in, uint32_t n, const int32_t* shape, const float* sp3,
+                         const float* pos3, const float* quat4, const float* vel3, const float* angvel3,
+                         const float* mass, const float* friction, const float* restitution,
+                         const int32_t* body_type, const uint32_t* layer, const uint32_t* mask,
+                         const uint8_t* trigger, const float* rolling, const float* torsion, const uint32_t* scene) {
+  if ((size_t)in.n + n > c->max_bodies) return fail(c, OAVBD_ECAPACITY, "more bodies than max_bodies");
+  if (n && (!shape || !sp3 || !pos3 || !quat4 || !vel3 || !angvel3 || !mass || !friction || !restitution ||
+            !body_type))
+    return fail(c, OAVBD_EINVAL, "null body array");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (shape[i] < 0 || shape[i] > 2) return fail(c, OAVBD_EINVAL, "bad shape kind");
+    if (body_type[i] < 0 || body_type[i] > 2) return fail(c, OAVBD_EINVAL, "bad body type");
+    if (shape[i] == SH_HALFSPACE && body_type[i] == BT_DYNAMIC && mass[i] > 0.0f)
+      return fail(c, OAVBD_EINVAL, "dynamic half-space");
+  }
+  auto add = [n](auto& v, const auto* src, size_t per, auto dflt) {
+    for (uint32_t i = 0; i < n; ++i)
+      for (size_t k = 0; k < per; ++k) v.push_back(src ? src[per * i + k] : dflt);
+  };
+  add(in.shape, shape, 1, 0);
+  add(in.type, body_type, 1, 0);
+  add(in.sp3, sp3, 3, 0.0f);
+  add(in.pos3, pos3, 3, 0.0f);
+  add(in.quat4, quat4, 4, 0.0f);
+  add(in.vel3, vel3, 3, 0.0f);
+  add(in.ang3, angvel3, 3, 0.0f);
+  add(in.mass, mass, 1, 0.0f);
+  add(in.friction, friction, 1, 0.0f);
+  add(in.restitution, restitution, 1, 0.0f);
+  add(in.rolling, rolling, 1, 0.0f);
+  add(in.torsion, torsion, 1, 0.0f);
+  add(in.layer, layer, 1, 1u);
+  add(in.mask, mask, 1, 0xffffffffu);
+  add(in.scene, scene, 1, 0u);
+  add(in.trigger, trigger, 1, (uint8_t)0);
+  in.n += n;
+  return OAVBD_OK;
+}
+
+int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float* sp3, const float* pos3,
+                     const float* quat4, const float* vel3, const float* angvel3, const float* mass,
+                     const float* friction, const float* restitution, const int32_t* body_type,
+                     const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger, const float* rolling,
+                     const float* torsion, const uint32_t* scene) {
+  if (!c) return OAVBD_EINVAL;
+  BodyInputs in;
+  const int r = append_inputs(c, in, n, shape, sp3, pos3, quat4, vel3, angvel3, mass, friction, restitution,
+                              body_type, layer, mask, trigger, rolling, torsion, scene);
+  if (r != OAVBD_OK) return r;
+  GUARD(c, {
+    set_bodies_impl(c, in);
+    c->inputs = std::move(in);
+  });
+  return OAVBD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// persistent state on the host: contact export / import, sleep state, and
+// incremental body edits (append, swap-remove) through a rebuild
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+template <class T>
+static void d2h(std::vector<T>& v, const T* src, size_t n) {
+  v.resize(n);
+  if (n) HIPC(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+}
+template <class T>
+static void h2d(T* dst, const std::vector<T>& v, size_t n) {
+  if (n) HIPC(hipMemcpy(dst, v.data(), n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// The pair table with its points, as the device holds it between steps.
+struct HostContacts {
+  std::vector<uint2> ab, pts;  // per pair: bodies, (first point, count)
+  std::vector<float4> prec;    // 4 per pair: normal+gap, t1, t2, friction
+  std::vector<float4> qrec;    // 4 per point: ra, rb, lam, pen
+  std::vector<float4> qroll;   // per point
+};
+
+// Everything per body and per joint a step reads from earlier steps.
+struct HostBodies {
+  std::vector<float4> pos, rot, vel, ang, torque, inertia, prev_pos, prev_rot, toi_hit;
+  std::vector<uint8_t> asleep;
+  std::vector<float> timer;
+  std::vector<uint32_t> toi_tgt;
+  std::vector<int32_t> colour;
+  std::vector<float4> jlam, jpen;
+};
+
+static HostContacts download_contacts(oavbd_ctx* c) {
+  const Dev& d = c->d;
+  HostContacts h;
+  d2h(h.ab, d.P.ab, d.npairs);
+  d2h(h.pts, d.P.pts, d.npairs);
+  d2h(h.prec, d.P.rec, 4 * (size_t)d.npairs);
+  d2h(h.qrec, d.Q.rec, 4 * (size_t)d.npts);
+  d2h(h.qroll, d.Q.roll, d.npts);
+  return h;
+}
+
+static HostBodies download_bodies(oavbd_ctx* c) {
+  const Dev& d = c->d;
+  const uint32_t n = d.n;
+  HostBodies h;
+  d2h(h.pos, d.b.pos, n); d2h(h.rot, d.b.rot, n); d2h(h.vel, d.b.vel, n); d2h(h.ang, d.b.angvel, n);
+  d2h(h.torque, d.b.torque, n); d2h(h.inertia, d.b.inertia, n); d2h(h.prev_pos, d.b.prev_pos, n);
+  d2h(h.prev_rot, d.b.prev_rot, n); d2h(h.toi_hit, d.toi_hit, n); d2h(h.asleep, d.b.asleep, n);
+  d2h(h.timer, d.b.timer, n); d2h(h.toi_tgt, d.toi_tgt, n); d2h(h.colour, d.b.colour, n);
+  d2h(h.jlam, d.J.lam, d.njoints);
+  d2h(h.jpen, d.J.pen, d.njoints);
+  return h;
+}
+
+// Writes the first hb.pos.size() bodies and the joint duals back.
+static void upload_bodies(oavbd_ctx* c, const HostBodies& h) {
+  Dev& d = c->d;
+  const size_t m = h.pos.size();
+  h2d(d.b.pos, h.pos, m); h2d(d.b.rot, h.rot, m); h2d(d.b.vel, h.vel, m); h2d(d.b.angvel, h.ang, m);
+  h2d(d.b.torque, h.torque, m); h2d(d.b.inertia, h.inertia, m); h2d(d.b.prev_pos, h.prev_pos, m);
+  h2d(d.b.prev_rot, h.prev_rot, m); h2d(d.toi_hit, h.toi_hit, m); h2d(d.b.asleep, h.asleep, m);
+  h2d(d.b.timer, h.timer, m); h2d(d.toi_tgt, h.toi_tgt, m); h2d(d.b.colour, h.colour, m);
+  h2d(d.J.lam, h.jlam, std::min<size_t>(h.jlam.size(), d.njoints));
+  h2d(d.J.pen, h.jpen, std::min<size_t>(h.jpen.size(), d.njoints));
+  HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
+  uint32_t sleeping = 0;
+  for (uint8_t a : h.asleep) sleeping += a ? 1u : 0u;
+  c->sleeping = sleeping;
+  c->may_sleep = sleeping != 0;
+}
+
+// Installs a pair table as the current one: stored sorted by (a, b) with
+// each pair's points contiguous, plus the per-body offsets of the a-runs the
+// next merge searches.  Throws (EINVAL) on an invalid table.
+static void upload_contacts(oavbd_ctx* c, const HostContacts& h) {
+  Dev& d = c->d;
+  const uint32_t n = d.n, np = (uint32_t)h.ab.size();
+  const size_t nq = h.qroll.size();
+  for (uint32_t i = 0; i < np; ++i) {
+    const uint2 ab = h.ab[i], pt = h.pts[i];
+    if (ab.x >= n || ab.y >= n || ab.x == ab.y) throw ArgError{"contact pair with a bad body index"};
+    if (pt.y > K_MAX_POINTS || (size_t)pt.x + pt.y > nq)
+      throw ArgError{"contact pair with a bad point range"};
+  }
+  std::vector<uint32_t> order(np);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    return h.ab[x].x != h.ab[y].x ? h.ab[x].x < h.ab[y].x : h.ab[x].y < h.ab[y].y;
+  });
+  for (uint32_t i = 1; i < np; ++i)
+    if (h.ab[order[i]].x == h.ab[order[i - 1]].x && h.ab[order[i]].y == h.ab[order[i - 1]].y)
+      throw ArgError{"duplicate contact pair"};
+  std::vector<uint2> ab(np), pts(np);
+  std::vector<float4> prec(4 * (size_t)np);
+  std::vector<float4> qrec, qroll;
+  std::vector<uint32_t> qpair;
+  for (uint32_t i = 0; i < np; ++i) {
+    const uint32_t o = order[i];
+    ab[i] = h.ab[o];
+    pts[i] = make_uint2((uint32_t)qroll.size(), h.pts[o].y);
+    for (int k = 0; k < 4; ++k) prec[4 * (size_t)i + k] = h.prec[4 * (size_t)o + k];
+    for (uint32_t k = 0; k < h.pts[o].y; ++k) {
+      const size_t q = (size_t)h.pts[o].x + k;
+      for (int j = 0; j < 4; ++j) qrec.push_back(h.qrec[4 * q + j]);
+      qroll.push_back(h.qroll[q]);
+      qpair.push_back(i);
+    }
+  }
+  const uint32_t nqs = (uint32_t)qroll.size();
+  ensure_pairs(c, std::max<uint32_t>(np, 1));
+  ensure_points(c, std::max<uint32_t>(nqs, 1));
+  ensure_incidence(c, np, nqs);
+  h2d(d.P.ab, ab, np);
+  h2d(d.P.pts, pts, np);
+  h2d(d.P.rec, prec, 4 * (size_t)np);
+  h2d(d.Q.rec, qrec, 4 * (size_t)nqs);
+  h2d(d.Q.roll, qroll, nqs);
+  h2d(d.Q.pair, qpair, nqs);
+  std::vector<int> src(nqs, -1);
+  h2d(d.Q.src, src, nqs);
+  std::vector<uint32_t> poff(n + 1);
+  uint32_t i = 0;
+  for (uint32_t b = 0; b <= n; ++b) {
+    while (i < np && ab[i].x < b) ++i;
+    poff[b] = i;
+  }
+  h2d(d.poff, poff, n + 1);
+  HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
+  d.npairs = np;
+  d.npts = nqs;
+  c->stats.pairs = np;
+  c->stats.points = nqs;
+}
+
+// Rebuilds the device table from `in` (set_bodies_impl), then restores the
+// first bodies' state, the joints with their duals, and the pair table.
+static void rebuild(oavbd_ctx* c, BodyInputs in, const HostBodies& hb, std::vector<HostJoint> joints,
+                    const HostContacts& hc) {
+  const float debt = c->time_debt;
+  set_bodies_impl(c, in);
+  c->inputs = std::move(in);
+  c->time_debt = debt;
+  c->joints = std::move(joints);
+  upload_joints(c);
+  upload_bodies(c, hb);
+  upload_contacts(c, hc);
+}
+
+extern "C" {
+
+int oavbd_add_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float* sp3, const float* pos3,
+                     const float* quat4, const float* vel3, const float* angvel3, const float* mass,
+                     const float* friction, const float* restitution, const int32_t* body_type,
+                     const uint32_t* layer, const uint32_t* mask, const uint8_t* trigger, const float* rolling,
+                     const float* torsion, const uint32_t* scene, uint32_t* first_index) {
+  if (!c) return OAVBD_EINVAL;
+  BodyInputs in = c->inputs;
+  const uint32_t first = in.n;
+  const int r = append_inputs(c, in, n, shape, sp3, pos3, quat4, vel3, angvel3, mass, friction, restitution,
+                              body_type, layer, mask, trigger, rolling, torsion, scene);
+  if (r != OAVBD_OK) return r;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    if (c->joints_dirty) upload_joints(c);
+    HostBodies hb = download_bodies(c);
+    HostContacts hc = download_contacts(c);
+    rebuild(c, std::move(in), hb, c->joints, hc);
+  });
+  if (first_index) *first_index = first;
+  return OAVBD_OK;
+}
+
+int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
+  if (!c) return OAVBD_EINVAL;
+  if (h >= c->inputs.n) return OAVBD_OK;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    if (c->joints_dirty) upload_joints(c);
+    const uint32_t last = c->inputs.n - 1;
+    BodyInputs in = c->inputs;
+    auto rm = [h, last](auto& v, size_t per) {
+      for (size_t k = 0; k < per; ++k) v[per * h + k] = v[per * last + k];
+      v.resize(per * last);
+    };
+    rm(in.shape, 1); rm(in.type, 1); rm(in.sp3, 3); rm(in.pos3, 3); rm(in.quat4, 4); rm(in.vel3, 3);
+    rm(in.ang3, 3); rm(in.mass, 1); rm(in.friction, 1); rm(in.restitution, 1); rm(in.rolling, 1);
+    rm(in.torsion, 1); rm(in.layer, 1); rm(in.mask, 1); rm(in.scene, 1); rm(in.trigger, 1);
+    in.n = last;
+    HostBodies hb = download_bodies(c);
+    rm(hb.pos, 1); rm(hb.rot, 1); rm(hb.vel, 1); rm(hb.ang, 1); rm(hb.torque, 1); rm(hb.inertia, 1);
+    rm(hb.prev_pos, 1); rm(hb.prev_rot, 1); rm(hb.toi_hit, 1); rm(hb.asleep, 1); rm(hb.timer, 1);
+    rm(hb.toi_tgt, 1); rm(hb.colour, 1);
+    auto map = [h, last](uint32_t x) { return x == last ? h : x; };
+    // pairs and joints on h go; the rest keep their order and orientation
+    HostContacts old = download_contacts(c), hc;
+    for (size_t i = 0; i < old.ab.size(); ++i) {
+      const uint2 ab = old.ab[i];
+      if (ab.x == h || ab.y == h) continue;
+      hc.ab.push_back(make_uint2(map(ab.x), map(ab.y)));
+      hc.pts.push_back(make_uint2((uint32_t)hc.qroll.size(), old.pts[i].y));
+      for (int k = 0; k < 4; ++k) hc.prec.push_back(old.prec[4 * i + k]);
+      for (uint32_t k = 0; k < old.pts[i].y; ++k) {
+        const size_t q = (size_t)old.pts[i].x + k;
+        for (int j = 0; j < 4; ++j) hc.qrec.push_back(old.qrec[4 * q + j]);
+        hc.qroll.push_back(old.qroll[q]);
+      }
+    }
+    std::vector<HostJoint> joints;
+    std::vector<float4> jlam, jpen;
+    for (size_t j = 0; j < c->joints.size(); ++j) {
+      HostJoint J = c->joints[j];
+      if (J.a == h || J.b == h) continue;
+      J.a = map(J.a);
+      J.b = map(J.b);
+      joints.push_back(J);
+      jlam.push_back(hb.jlam[j]);
+      jpen.push_back(hb.jpen[j]);
+    }
+    hb.jlam = std::move(jlam);
+    hb.jpen = std::move(jpen);
+    rebuild(c, std::move(in), hb, std::move(joints), hc);
+    // every sleeping body wakes (wake_body through the device's own path)
+    Dev& d = c->d;
+    if (c->sleeping) {
+      std::vector<uint8_t> flags(hb.asleep);
+      h2d(d.b.flag_a, flags, flags.size());
+      HIPC(hipDeviceSynchronize());
+      k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d);
+      HIPC(hipStreamSynchronize(c->st));
+      c->sleeping = 0;
+    }
+    c->may_sleep = false;
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_get_contacts(oavbd_ctx* c, oavbd_contact_pair* pairs, uint32_t cap_pairs, oavbd_contact_point* points,
+                       uint32_t cap_points, uint32_t* npairs, uint32_t* npoints) {
+  if (!c || !npairs || !npoints) return OAVBD_EINVAL;
+  *npairs = c->d.npairs;
+  *npoints = c->d.npts;
+  if ((pairs && cap_pairs < c->d.npairs) || (points && cap_points < c->d.npts))
+    return fail(c, OAVBD_ECAPACITY, "contact export arrays too small");
+  if (!pairs && !points) return OAVBD_OK;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    const HostContacts h = download_contacts(c);
+    if (pairs)
+      for (size_t i = 0; i < h.ab.size(); ++i) {
+        oavbd_contact_pair& o = pairs[i];
+        o.a = h.ab[i].x;
+        o.b = h.ab[i].y;
+        o.first_point = h.pts[i].x;
+        o.npoints = h.pts[i].y;
+        std::memcpy(o.normal, &h.prec[4 * i], sizeof(float4));
+        std::memcpy(o.tangent1, &h.prec[4 * i + 1], sizeof(float4));
+        std::memcpy(o.tangent2, &h.prec[4 * i + 2], sizeof(float4));
+        std::memcpy(o.friction, &h.prec[4 * i + 3], sizeof(float4));
+      }
+    if (points)
+      for (size_t q = 0; q < h.qroll.size(); ++q) {
+        oavbd_contact_point& o = points[q];
+        std::memcpy(o.ra, &h.qrec[4 * q], sizeof(float4));
+        std::memcpy(o.rb, &h.qrec[4 * q + 1], sizeof(float4));
+        std::memcpy(o.lam, &h.qrec[4 * q + 2], sizeof(float4));
+        std::memcpy(o.pen, &h.qrec[4 * q + 3], sizeof(float4));
+        std::memcpy(o.roll, &h.qroll[q], sizeof(float4));
+      }
+  });
+  return OAVBD_OK;
+}
+
+int oavbd_set_contacts(oavbd_ctx* c, const oavbd_contact_pair* pairs, uint32_t npairs,
+                       const oavbd_contact_point* points, uint32_t npoints) {
+  if (!c || (npairs && !pairs) || (npoints && !points)) return OAVBD_EINVAL;
+  HostContacts h;
+  h.ab.resize(npairs);
+  h.pts.resize(npairs);
+  h.prec.resize(4 * (size_t)npairs);
+  h.qrec.resize(4 * (size_t)npoints);
+  h.qroll.resize(npoints);
+  for (uint32_t i = 0; i < npairs; ++i) {
+    const oavbd_contact_pair& p = pairs[i];
+    h.ab[i] = make_uint2(p.a, p.b);
+    h.pts[i] = make_uint2(p.first_point, p.npoints);
+    std::memcpy(&h.prec[4 * (size_t)i], p.normal, sizeof(float4));
+    std::memcpy(&h.prec[4 * (size_t)i + 1], p.tangent1, sizeof(float4));
+    std::memcpy(&h.prec[4 * (size_t)i + 2], p.tangent2, sizeof(float4));
+    std::memcpy(&h.prec[4 * (size_t)i + 3], p.friction, sizeof(float4));
+  }
+  for (uint32_t q = 0; q < npoints; ++q) {
+    const oavbd_contact_point& p = points[q];
+    std::memcpy(&h.qrec[4 * (size_t)q], p.ra, sizeof(float4));
+    std::memcpy(&h.qrec[4 * (size_t)q + 1], p.rb, sizeof(float4));
+    std::memcpy(&h.qrec[4 * (size_t)q + 2], p.lam, sizeof(float4));
+    std::memcpy(&h.qrec[4 * (size_t)q + 3], p.pen, sizeof(float4));
+    std::memcpy(&h.qroll[q], p.roll, sizeof(float4));
+  }
+  try {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    upload_contacts(c, h);
+  } catch (const ArgError& e) {
+    return fail(c, OAVBD_EINVAL, e.msg);
+  } catch (const HipError& e) {
+    return fail(c, OAVBD_EDEVICE, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(c, OAVBD_ENOMEM, "host allocation failed");
+  }
+  return OAVBD_OK;
+}
+
+int oavbd_set_sleep(oavbd_ctx* c, const uint8_t* asleep, const float* timer, uint32_t n) {
+  if (!c || !asleep || !timer || n != c->d.n) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    HostBodies hb = download_bodies(c);
+    const BodyInputs& in = c->inputs;
+    for (uint32_t i = 0; i < n; ++i) {
+      const bool dyn = c->type_h[i] == BT_DYNAMIC;
+      const bool to_sleep = asleep[i] != 0;
+      if (dyn && to_sleep) {  // sleep_body (mod.rs:822-832)
+        hb.vel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hb.ang[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hb.pos[i].w = 0.0f;
+        hb.inertia[i] = make_float4(0.0f, 0.0f, 0.0f, hb.inertia[i].w);
+      } else if (dyn && hb.asleep[i]) {  // wake_restore (body.rs:412-418)
+        float inv;
+        f3 I;
+        awake_mass(in.shape[i], f3{in.sp3[3 * i], in.sp3[3 * i + 1], in.sp3[3 * i + 2]}, BT_DYNAMIC, in.mass[i],
+                   &inv, &I);
+        hb.pos[i].w = inv;
+        hb.inertia[i] = make_float4(I.x, I.y, I.z, hb.inertia[i].w);
+      }
+      hb.asleep[i] = (dyn && to_sleep) ? 1 : 0;
+      hb.timer[i] = timer[i];
+    }
+    upload_bodies(c, hb);
   });
   return OAVBD_OK;
 }
@@ -1353,8 +1794,10 @@ int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3]
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
     uint8_t one = 1;
+    HIPC(hipStreamSynchronize(c->st));
     HIPC(hipMemcpy(c->d.b.flag_a + a, &one, 1, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d.b.flag_a + b, &one, 1, hipMemcpyHostToDevice));
+    HIPC(hipDeviceSynchronize());
     k_apply_wake<<<blocks(c->d.n), TPB, 0, c->st>>>(c->d);
     HIPC(hipStreamSynchronize(c->st));
   });
@@ -1415,6 +1858,7 @@ int oavbd_set_state(oavbd_ctx* c, const float* in13, uint32_t n) {
   if (!c || !in13 || n > c->d.n) return OAVBD_EINVAL;
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
     std::vector<float4> p(n), q(n), v(n), w(n);
     HIPC(hipMemcpy(p.data(), c->d.b.pos, n * sizeof(float4), hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; ++i) {
@@ -1428,6 +1872,7 @@ int oavbd_set_state(oavbd_ctx* c, const float* in13, uint32_t n) {
     HIPC(hipMemcpy(c->d.b.rot, q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d.b.vel, v.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     HIPC(hipMemcpy(c->d.b.angvel, w.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   });
   return OAVBD_OK;
 }
@@ -1531,6 +1976,7 @@ int oavbd_get_joints(oavbd_ctx* c, float* out6, uint32_t n) {
 int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   if (!c || !out) return OAVBD_EINVAL;
   *out = c->stats;
+  out->joints = (uint32_t)c->joints.size();
   return OAVBD_OK;
 }
 

@@ -99,3 +99,61 @@ def test_discovery_thread_count_does_not_change_bits():
         outs.append((e.state(), e.pairs()[0]))
     assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
     assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_removing_an_unrelated_body_keeps_the_moved_tails_joint():
+    """removing_an_unrelated_body_preserves_the_moved_tails_joint
+    (solver_lifecycle.rs:41-78), with a Ball joint in place of the Distance
+    joint: the tail moves into the removed spare's slot with its joint, and
+    the joint pulls it back from a 1 m teleport within 600 steps."""
+    for order in (po.ORDER_REFERENCE, po.ORDER_COLOUR):
+        b = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+        b.sphere((0.0, 0.0, 0.0), 0.2, 0.0)
+        b.sphere((20.0, 0.0, 0.0), 0.2, 1.0)
+        b.sphere((0.0, 2.0, 0.0), 0.2, 1.0)
+        b.ball_joint(0, 2, (0.0, 0.0, 0.0), (0.0, -2.0, 0.0))
+        s = b.build()
+        s["mask"][:] = 0
+        e = po.OracleEngine(s, order=order)
+        e.remove_body(1)
+        assert e.n == 2
+        st = e.state()
+        assert np.array_equal(st[1], po.OracleEngine(s, order=order).state()[2])
+        e.set_pose(1, (0.0, 3.0, 0.0), st[1, 3:7])
+        e.step(DT, 600)
+        assert abs(e.state()[1, 1] - 2.0) < 0.04
+        assert np.abs(e.joints(1)).max() > 0.0
+
+
+def test_remove_body_drops_renumbers_and_wakes():
+    """remove_body (mod.rs:1241-1300) on a settled pile: the removed body's
+    pairs go, the last body's pairs follow it to the freed index with their
+    warm starts (while the pair keeps a < b), the rest are untouched, and
+    every sleeping body wakes."""
+    s = scenes.box_drop_scene(40, np.random.Generator(np.random.PCG64(12)))
+    e = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+    e.step_inner(240)
+    assert e.stats()["sleeping"] > 0
+    n = e.n
+    last = n - 1
+    ab, npts, _, _ = e.pairs()
+    pts = e.points()
+    off = np.concatenate([[0], np.cumsum(npts)]).astype(np.int64)
+    before = {tuple(p): pts[off[i]:off[i + 1]] for i, p in enumerate(ab.tolist())}
+    touched = [p for p in before if last in p]
+    assert touched, "the last body should rest on something"
+    h = int(next(p for p in before if last not in p and p[0] != 0)[0])
+    st = e.state()
+    e.remove_body(h)
+    assert e.n == n - 1
+    assert np.array_equal(e.state()[h], st[last])
+    assert not e.sleep()[0].any()
+    ab2, npts2, _, _ = e.pairs()
+    pts2 = e.points()
+    off2 = np.concatenate([[0], np.cumsum(npts2)]).astype(np.int64)
+    after = {tuple(p): pts2[off2[i]:off2[i + 1]] for i, p in enumerate(ab2.tolist())}
+    remap = lambda x: h if x == last else x  # noqa: E731
+    want = {(remap(a), remap(b)): v for (a, b), v in before.items() if h not in (a, b)}
+    assert set(after) == set(want)
+    for k, v in want.items():
+        assert np.array_equal(after[k].view(np.uint32), v.view(np.uint32))
