@@ -326,6 +326,9 @@ def main():
                     help="steps of the 1,024-scene differentiable workload (0: skip it)")
     ap.add_argument("--diff-reps", type=int, default=3)
     ap.add_argument("--cpu-diff-scenes", type=int, default=64)
+    ap.add_argument("--autotune-steps", type=int, default=0,
+                    help="after the warmup, autotune the primal launch variants over this many steps "
+                         "(state restored; 0: library default table)")
     ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
                     help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
                          "for HBM byte counters)")
@@ -356,6 +359,11 @@ def main():
         if args.progress:
             print(f"warmup {s} {eng.stats()}", file=sys.stderr, flush=True)
     eng.sync()
+    table = None
+    eng_tune = None
+    if args.autotune_steps:
+        table = eng.autotune(args.autotune_steps)
+        eng_tune = eng.tune_us
     eng.set_timing(True)
     if args.lanes:
         eng.set_primal_shape(args.lanes, args.waves)
@@ -428,6 +436,9 @@ def main():
                                "dual": kt["dual_ms"] / max(kt["steps"], 1)},
         "dual_roofline": {"achieved": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS},
         "final_stats": st,
+        "primal_table": None if table is None else [int(x) for x in table],
+        "primal_tune_us": None if table is None else
+        {f"2^{b}": [round(float(x), 1) for x in eng_tune[:, b]] for b in range(24) if (eng_tune[:, b] > 0).any()},
     }
     if not args.only_main:
         captured = None

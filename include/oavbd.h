@@ -249,6 +249,30 @@ int oavbd_get_timing(oavbd_ctx* ctx, float* ms5);
  * changes.  Default 16 lanes, 2 wavefronts. */
 int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
 
+/* Primal launch variants, chosen per colour class by its size (bucket b
+ * holds classes of 2^b .. 2^(b+1) - 1 bodies, b < 24): 0-3 sixteen lanes
+ * per body with 1, 2, 4 or 8 wavefronts per workgroup; 4-6 one body per
+ * wavefront, its contact records staged in LDS first, 1, 2 or 4 wavefronts
+ * per workgroup; 7-9 sixteen lanes per body with the next item's records
+ * fetched while the current item is evaluated, 1, 2 or 4 wavefronts per
+ * workgroup.  Every variant gives the same bits.  -1 leaves a bucket to
+ * the default rule (variant 0 below 8,192 bodies, the oavbd_set_primal_shape
+ * wavefront count above).  No reference counterpart: the reference sweeps
+ * bodies serially (mod.rs:1052-1059).
+ *
+ * oavbd_autotune runs the next nsteps (1..64) steps once per variant from
+ * one saved state, times every primal launch with HIP events, and gives
+ * each bucket the variant with the least mean launch time (unmeasured
+ * buckets take the nearest measured bucket's choice).  The context's state,
+ * statistics and kernel times are restored afterwards; table_out (24
+ * entries, may be NULL) receives the table and mean_us_out (10 x 24, may be
+ * NULL) the mean launch time in microseconds of every variant in every
+ * bucket (-1: no launch).  The table can be carried to other contexts with
+ * oavbd_set_primal_table (n <= 24 entries). */
+int oavbd_autotune(oavbd_ctx* ctx, uint32_t nsteps, int32_t* table_out, float* mean_us_out);
+int oavbd_set_primal_table(oavbd_ctx* ctx, const int32_t* table, uint32_t n);
+int oavbd_get_primal_table(oavbd_ctx* ctx, int32_t* table, uint32_t n);
+
 /* Rung-1 staged bodies: n GpuAvbdMass records (16 bytes: inertia[3],
  * inv_mass) and n GpuBodyState residuals (32 bytes: lin[3], pad, ang[3],
  * pad) are solved on the device -- lumped inertial Hessian diagonal and its

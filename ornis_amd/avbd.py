@@ -44,6 +44,8 @@ class OavbdKernelTimes(ctypes.Structure):
                 ("dual_launches", ctypes.c_uint64), ("steps", ctypes.c_uint64)]
 
 
+PRIMAL_VARIANTS = 10  # oavbd_autotune's launch variants (include/oavbd.h)
+
 # oavbd_contact_pair / oavbd_contact_point (include/oavbd.h), 80 bytes each.
 CONTACT_PAIR = np.dtype([("a", "<u4"), ("b", "<u4"), ("first_point", "<u4"), ("npoints", "<u4"),
                          ("normal", "<f4", 4), ("tangent1", "<f4", 4), ("tangent2", "<f4", 4),
@@ -86,6 +88,9 @@ SIGNATURES = {
     "oavbd_get_timing": (ctypes.c_int, [ctypes.c_void_p, _f32p]),
     "oavbd_set_primal_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "oavbd_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _i32p, _f32p]),
+    "oavbd_set_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
+    "oavbd_get_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_solve_diag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_void_p]),
     "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
@@ -413,6 +418,24 @@ class AvbdEngine:
         """Primal launch shape: lanes per body (16) and wavefronts per
         workgroup (bitwise neutral)."""
         self._check(self._L.oavbd_set_primal_shape(self._ctx, int(lanes), int(waves)))
+
+    def autotune(self, nsteps=3):
+        """Time the next nsteps steps under every primal launch variant
+        (state restored afterwards) and keep the fastest per class size;
+        returns the 24-entry table."""
+        t = np.zeros(24, np.int32)
+        self.tune_us = np.zeros((PRIMAL_VARIANTS, 24), np.float32)
+        self._check(self._L.oavbd_autotune(self._ctx, int(nsteps), _p(t, _i32p), _p(self.tune_us, _f32p)))
+        return t
+
+    def set_primal_table(self, table):
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        self._check(self._L.oavbd_set_primal_table(self._ctx, _p(t, _i32p), len(t)))
+
+    def primal_table(self):
+        t = np.zeros(24, np.int32)
+        self._check(self._L.oavbd_get_primal_table(self._ctx, _p(t, _i32p), 24))
+        return t
 
     def set_timing(self, on=True):
         self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))

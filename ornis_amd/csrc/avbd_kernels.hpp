@@ -1779,26 +1779,28 @@ struct SbPartner {
   const float4* sb;
   __device__ __forceinline__ const float4* operator()(uint32_t h) const { return sb + 4 * (size_t)h; }
 };
-template <int R, class Emit, class Partner>
-__device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 (&own)[4], Emit&& emit,
-                                           const Partner& partner) {
+// The three 64-byte records a point item reads: its pair, its point and
+// the partner body's solver record.
+struct ItemRec {
+  float4 n4, t14, t24, mu;    // pair: normal + gap, t1 + damper, t2, friction
+  float4 ra4, rb4, lam, pen;  // point
+  float4 o0, o1, o2, o3;      // partner: pos, rot, pos0, drot
+};
+template <class Partner>
+__device__ __forceinline__ const float4* item_partner(const Partner& partner, uint4 it) {
+  return partner((it.x >> 31) == 0u ? it.w : it.z);
+}
+
+// The rows of a point item from its records (the arithmetic of point_rows).
+template <int R, class Emit>
+__device__ __forceinline__ void item_rows(const Dev& d, uint4 it, const ItemRec& r, const float4 (&own)[4],
+                                          Emit&& emit) {
   uint32_t q = it.x & 0x7fffffffu;
   bool is_a = (it.x >> 31) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
-  uint32_t p = it.y;
-  uint2 ab = make_uint2(it.z, it.w);
-  // The three 64-byte records this item reads (pair, point, partner body)
-  // are loaded together and pinned before any branch, so the item costs one
-  // memory round trip instead of one per branch that first uses a field.
-  const float4* prec = d.P.rec + 4 * (size_t)p;
-  const float4* qrec = d.Q.rec + 4 * (size_t)q;
-  const float4* so = partner(is_a ? ab.y : ab.x);
-  float4 n4 = prec[0], t14 = prec[1], t24 = prec[2], mu = prec[3];
-  float4 ra4 = qrec[0], rb4 = qrec[1], lam = qrec[2], pen = qrec[3];
-  float4 o0 = so[0], o1 = so[1], o2 = so[2], o3 = so[3];
-  pin4(n4); pin4(t14); pin4(t24); pin4(mu);
-  pin4(ra4); pin4(rb4); pin4(lam); pin4(pen);
-  pin4(o0); pin4(o1); pin4(o2); pin4(o3);
+  const float4 n4 = r.n4, t14 = r.t14, t24 = r.t24, mu = r.mu;
+  const float4 ra4 = r.ra4, rb4 = r.rb4, lam = r.lam, pen = r.pen;
+  const float4 o0 = r.o0, o1 = r.o1, o2 = r.o2, o3 = r.o3;
   const float4 pa4 = is_a ? own[0] : o0, sa1 = is_a ? own[1] : o1, a0 = is_a ? own[2] : o2,
                sa3 = is_a ? own[3] : o3;
   const float4 pb4 = is_a ? o0 : own[0], sb1 = is_a ? o1 : own[1], b0 = is_a ? o2 : own[2],
@@ -1885,6 +1887,25 @@ __device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 
       if (!(mur[li] <= 0.0f)) emit(1, row);
     }
   }
+}
+
+template <int R, class Emit, class Partner>
+__device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 (&own)[4], Emit&& emit,
+                                           const Partner& partner) {
+  // The three records are loaded together and pinned before any branch, so
+  // the item costs one memory round trip instead of one per branch that
+  // first uses a field.
+  const float4* prec = d.P.rec + 4 * (size_t)it.y;
+  const float4* qrec = d.Q.rec + 4 * (size_t)(it.x & 0x7fffffffu);
+  const float4* so = item_partner(partner, it);
+  ItemRec r;
+  r.n4 = prec[0]; r.t14 = prec[1]; r.t24 = prec[2]; r.mu = prec[3];
+  r.ra4 = qrec[0]; r.rb4 = qrec[1]; r.lam = qrec[2]; r.pen = qrec[3];
+  r.o0 = so[0]; r.o1 = so[1]; r.o2 = so[2]; r.o3 = so[3];
+  pin4(r.n4); pin4(r.t14); pin4(r.t24); pin4(r.mu);
+  pin4(r.ra4); pin4(r.rb4); pin4(r.lam); pin4(r.pen);
+  pin4(r.o0); pin4(r.o1); pin4(r.o2); pin4(r.o3);
+  item_rows<R>(d, it, r, own, emit);
 }
 
 // Rows of one Ball-joint side (assembly.rs:279-310, :589-618).  Slots
@@ -2155,7 +2176,22 @@ __device__ __forceinline__ float tree16(float v) {
   return v;
 }
 
-template <int W, int R>
+// The three records of a point item, loaded without waiting for them.
+__device__ __forceinline__ ItemRec load_item(const Dev& d, uint4 it) {
+  const float4* prec = d.P.rec + 4 * (size_t)it.y;
+  const float4* qrec = d.Q.rec + 4 * (size_t)(it.x & 0x7fffffffu);
+  const float4* so = item_partner(SbPartner{d.sb}, it);
+  ItemRec r;
+  r.n4 = prec[0]; r.t14 = prec[1]; r.t24 = prec[2]; r.mu = prec[3];
+  r.ra4 = qrec[0]; r.rb4 = qrec[1]; r.lam = qrec[2]; r.pen = qrec[3];
+  r.o0 = so[0]; r.o1 = so[1]; r.o2 = so[2]; r.o3 = so[3];
+  return r;
+}
+
+// PF = 1: the records of a slot's next item are requested before its
+// current item is evaluated (one item of records in flight behind the
+// arithmetic, 48 more registers); PF = 0: only the next descriptor.
+template <int W, int R, int PF = 0>
 __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restrict__ list, uint32_t count) {
   constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
   const int tid = threadIdx.x;
@@ -2193,13 +2229,33 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
     else
       acc_diag(acc, row);
   };
-  // the next item's descriptor is fetched while the current item is
-  // evaluated, so an item waits for one memory round trip, not two
-  uint4 next = slot < np ? d.inc[i0 + slot] : make_uint4(0u, 0u, 0u, 0u);
-  for (uint32_t k = slot; k < np; k += BODY_LANES) {
-    const uint4 it = next;
-    if (k + BODY_LANES < np) next = d.inc[i0 + k + BODY_LANES];
-    point_rows<R>(d, it, own, add_row, SbPartner{d.sb});
+  if constexpr (PF == 0) {
+    // the next item's descriptor is fetched while the current item is
+    // evaluated, so an item waits for one memory round trip, not two
+    uint4 next = slot < np ? d.inc[i0 + slot] : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t k = slot; k < np; k += BODY_LANES) {
+      const uint4 it = next;
+      if (k + BODY_LANES < np) next = d.inc[i0 + k + BODY_LANES];
+      point_rows<R>(d, it, own, add_row, SbPartner{d.sb});
+    }
+  } else {
+    uint4 cur = make_uint4(0u, 0u, 0u, 0u), nxt = cur;
+    ItemRec rc{};
+    if ((uint32_t)slot < np) {
+      cur = d.inc[i0 + slot];
+      rc = load_item(d, cur);
+    }
+    if ((uint32_t)slot + BODY_LANES < np) nxt = d.inc[i0 + slot + BODY_LANES];
+    for (uint32_t k = slot; k < np; k += BODY_LANES) {
+      ItemRec rn{};
+      uint4 nn = make_uint4(0u, 0u, 0u, 0u);
+      if (k + BODY_LANES < np) rn = load_item(d, nxt);
+      if (k + 2 * BODY_LANES < np) nn = d.inc[i0 + k + 2 * BODY_LANES];
+      item_rows<R>(d, cur, rc, own, add_row);
+      cur = nxt;
+      rc = rn;
+      nxt = nn;
+    }
   }
   // joint sides continue the item numbering after the point-sides
   const uint32_t total = np + nj;
@@ -2224,6 +2280,128 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
   float dx[6];
   const bool solved = solve_6x6_dev(L21, rhs, dx);
   if (!valid || slot != 0) return;
+  if (solved) {
+    float4* rec = d.sb + 4 * (size_t)h;
+    float4 p = rec[0];
+    rec[0] = make_float4(p.x + dx[0], p.y + dx[1], p.z + dx[2], p.w);
+    qf q = quat_integrate(asq(rec[1]), f3{dx[3], dx[4], dx[5]});
+    rec[1] = pkq(q);
+    rec[3] = pk(quat_diff_vec(q, asq(d.b.rot0[h])), 0.0f);
+  } else {
+    atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
+  }
+}
+
+// Primal solve of a colour class with one body per wavefront, for classes
+// too small to hide memory latency with many bodies in flight.  All 64
+// lanes first stage the body's item descriptors, then the three records of
+// every item (12 float4 each), in LDS, with every load of a phase in flight
+// at once: two memory round trips for the whole body instead of one per
+// item and slot.  Lanes 0-15 then evaluate the items from LDS exactly as
+// k_primal8 does (slot s: items s, s + 16, ..., the same item_rows
+// arithmetic, tree16 and solve), so the two kernels give identical bits.
+// Items past PRIMAL_LDS_ITEMS are read from memory as in k_primal8.
+constexpr int PRIMAL_LDS_ITEMS = 64;
+template <int W, int R>
+__global__ void __launch_bounds__(WAVE* W) k_primal_lds(Dev d, const uint4* __restrict__ list, uint32_t count) {
+  constexpr int NE = PRIMAL_LDS_ITEMS * 12;  // float4 records per body
+  __shared__ uint4 s_desc[W][PRIMAL_LDS_ITEMS];
+  __shared__ float4 s_rec[W][NE];
+  const int wv = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+  const uint32_t idx = blockIdx.x * W + wv;
+  const bool valid = idx < count;
+  uint32_t h = 0, i0 = 0, np = 0, j0 = 0, nj = 0;
+  if (valid) {
+    const uint4 cr = list[idx];
+    h = cr.x & 0x7fffffffu;
+    i0 = cr.y;
+    np = cr.z;
+    j0 = cr.w;
+    if (cr.x >> 31) nj = d.b.ji_off[h + 1] - j0;
+  }
+  const uint32_t ns = np < (uint32_t)PRIMAL_LDS_ITEMS ? np : (uint32_t)PRIMAL_LDS_ITEMS;
+  if ((uint32_t)lane < ns) s_desc[wv][lane] = d.inc[i0 + lane];
+  __syncthreads();
+  {
+    constexpr int U = NE / WAVE;
+    const uint32_t ne = ns * 12u;
+    float4 v[U];
+    // every lane loads (a lane past the body's records reads pair, point
+    // and body 0, which always exist), so all loads are issued together
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t e = (uint32_t)lane + (uint32_t)u * WAVE;
+      const bool live = e < ne;
+      const uint32_t part = e % 12u;
+      const uint4 it = live ? s_desc[wv][e / 12u] : make_uint4(0u, 0u, 0u, 0u);
+      const float4* src = part < 4u   ? d.P.rec + 4 * (size_t)it.y + part
+                          : part < 8u ? d.Q.rec + 4 * (size_t)(it.x & 0x7fffffffu) + (part - 4u)
+                                      : item_partner(SbPartner{d.sb}, it) + (part - 8u);
+      v[u] = *src;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t e = (uint32_t)lane + (uint32_t)u * WAVE;
+      if (e < ne) s_rec[wv][e] = v[u];
+    }
+  }
+  __syncthreads();
+  float acc[27];
+#pragma unroll
+  for (int e = 0; e < 27; ++e) acc[e] = 0.0f;
+  float4 own[4];
+  {
+    const float4* rec = d.sb + 4 * (size_t)h;
+    own[0] = rec[0];
+    own[1] = rec[1];
+    own[2] = rec[2];
+    own[3] = rec[3];
+  }
+  auto add_row = [&](int kind, const Row& row) {
+    if (kind == 0)
+      acc_stamp(acc, row);
+    else if (kind == 1)
+      acc_couple(acc, row);
+    else
+      acc_diag(acc, row);
+  };
+  const int slot = lane;  // lanes 16-63 hold no items: their sums stay +0
+  if (slot < BODY_LANES) {
+    for (uint32_t k = slot; k < np; k += BODY_LANES) {
+      if (k < ns) {
+        const float4* r4 = &s_rec[wv][12 * k];
+        ItemRec r;
+        r.n4 = r4[0]; r.t14 = r4[1]; r.t24 = r4[2]; r.mu = r4[3];
+        r.ra4 = r4[4]; r.rb4 = r4[5]; r.lam = r4[6]; r.pen = r4[7];
+        r.o0 = r4[8]; r.o1 = r4[9]; r.o2 = r4[10]; r.o3 = r4[11];
+        item_rows<R>(d, s_desc[wv][k], r, own, add_row);
+      } else {
+        point_rows<R>(d, d.inc[i0 + k], own, add_row, SbPartner{d.sb});
+      }
+    }
+    const uint32_t total = np + nj;
+    for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
+         k += BODY_LANES)
+      joint_rows(d, d.b.ji[j0 + (k - np)], add_row);
+  }
+  float L21[21], rhs[6];
+  const float4* inb = reinterpret_cast<const float4*>(d.inb + 28 * (size_t)h);
+  float blk[28];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const float4 v = valid ? inb[q] : make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+    blk[4 * q] = v.x;
+    blk[4 * q + 1] = v.y;
+    blk[4 * q + 2] = v.z;
+    blk[4 * q + 3] = v.w;
+  }
+#pragma unroll
+  for (int e = 0; e < 21; ++e) L21[e] = blk[e] + tree16(acc[e]);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) rhs[k] = -(blk[21 + k] + tree16(acc[21 + k]));
+  float dx[6];
+  const bool solved = solve_6x6_dev(L21, rhs, dx);
+  if (!valid || lane != 0) return;
   if (solved) {
     float4* rec = d.sb + 4 * (size_t)h;
     float4 p = rec[0];

@@ -142,6 +142,10 @@ struct TapeHost {
   std::vector<uint32_t> col_off, col_cnt;
 };
 
+// Primal launch variants (see launch_primal) and colour-size buckets.
+enum { PV_W1 = 0, PV_W2, PV_W4, PV_W8, PV_L1, PV_L2, PV_L4, PV_P1, PV_P2, PV_P4, PV_COUNT };
+constexpr int PRIMAL_BUCKETS = 24;
+
 struct oavbd_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -168,6 +172,12 @@ struct oavbd_ctx {
   oavbd_stats stats{};
   std::vector<uint32_t> colour_cnt_h, colour_off_h;
   int waves = 2;    // primal launch shape: wavefronts per workgroup
+  int8_t ptable[PRIMAL_BUCKETS];  // primal variant per colour-size bucket (-1: default rule)
+  int force_variant = -1;         // every primal launch takes this variant (autotuning)
+  std::vector<std::pair<uint32_t, int>> plog;  // (class size, variant) of each timed primal launch
+  bool tuning = false;
+  double tune_ms[PV_COUNT][PRIMAL_BUCKETS] = {};
+  uint32_t tune_n[PV_COUNT][PRIMAL_BUCKETS] = {};
   bool rows6 = false;  // rolling/torsion rows present: 6 rows per item
   bool timing = false;
   std::vector<hipEvent_t> pev, dev_ev;  // per-launch event pools
@@ -877,21 +887,56 @@ static void launch_w(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
   else
     k_primal8<W, 4><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
 }
+template <int W>
+static void launch_pf(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;
+  if (c->rows6)
+    k_primal8<W, 6, 1><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+  else
+    k_primal8<W, 4, 1><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+}
+template <int W>
+static void launch_lds(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+  if (c->rows6)
+    k_primal_lds<W, 6><<<blocks(cnt, W), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+  else
+    k_primal_lds<W, 4><<<blocks(cnt, W), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+}
 
-// Primal launch of one colour class: sixteen lanes per body, `waves`
-// wavefronts per workgroup (the configured launch shape; results do not
-// depend on it).  A colour of fewer than PRIMAL_TAIL bodies does not fill
-// the chip: it runs one wavefront per workgroup, spreading its bodies over
-// as many compute units as possible.
+// Primal launch of one colour class.  The launch shapes (variants): sixteen
+// lanes per body with 1, 2, 4 or 8 wavefronts per workgroup (k_primal8);
+// one body per wavefront with its records staged in LDS, 1, 2 or 4
+// wavefronts per workgroup (k_primal_lds); sixteen lanes per body with the
+// next item's records in flight behind the current one, 1, 2 or 4
+// wavefronts per workgroup (k_primal8 with PF = 1).  Every variant gives the same
+// bits.  The variant comes from the autotuned table for the class's size
+// (log2 bucket), else the default: sixteen lanes, one wavefront per
+// workgroup below PRIMAL_TAIL bodies (a small class spread over as many
+// compute units as possible), the configured count above.  Returns the
+// variant launched.
 constexpr uint32_t PRIMAL_TAIL = 8192;
-static void launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
-  if (cnt < PRIMAL_TAIL) return launch_w<1>(c, list, cnt);
-  switch (c->waves) {
-    case 1: return launch_w<1>(c, list, cnt);
-    case 2: return launch_w<2>(c, list, cnt);
-    case 8: return launch_w<8>(c, list, cnt);
-    default: return launch_w<4>(c, list, cnt);
+static int size_bucket(uint32_t cnt) {
+  int b = 0;
+  while ((cnt >> (b + 1)) != 0u && b + 1 < PRIMAL_BUCKETS) ++b;
+  return b;
+}
+static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+  int v = c->force_variant;
+  if (v < 0) v = c->ptable[size_bucket(cnt)];
+  if (v < 0) v = cnt < PRIMAL_TAIL ? PV_W1 : (c->waves == 1 ? PV_W1 : c->waves == 2 ? PV_W2 : c->waves == 8 ? PV_W8 : PV_W4);
+  switch (v) {
+    case PV_W1: launch_w<1>(c, list, cnt); break;
+    case PV_W2: launch_w<2>(c, list, cnt); break;
+    case PV_W4: launch_w<4>(c, list, cnt); break;
+    case PV_W8: launch_w<8>(c, list, cnt); break;
+    case PV_L1: launch_lds<1>(c, list, cnt); break;
+    case PV_L2: launch_lds<2>(c, list, cnt); break;
+    case PV_P1: launch_pf<1>(c, list, cnt); break;
+    case PV_P2: launch_pf<2>(c, list, cnt); break;
+    case PV_P4: launch_pf<4>(c, list, cnt); break;
+    default: launch_lds<4>(c, list, cnt); v = PV_L4; break;
   }
+  return v;
 }
 
 static void record(oavbd_ctx* c, int i) {
@@ -977,8 +1022,12 @@ static void step_inner(oavbd_ctx* c, bool finish) {
       uint32_t cnt = c->colour_cnt_h[k];
       if (!cnt) continue;
       if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev), c->st));
-      launch_primal(c, d.colour_list + c->colour_off_h[k], cnt);
-      if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
+      const int v = launch_primal(c, d.colour_list + c->colour_off_h[k], cnt);
+      if (c->timing) {
+        HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
+        if (c->plog.size() <= np_ev / 2) c->plog.resize(np_ev / 2 + 1);
+        c->plog[np_ev / 2] = {cnt, v};
+      }
       np_ev += 2;
     }
     if (c->rec) tape_iter_end(c, it);
@@ -1037,6 +1086,12 @@ static void step_inner(oavbd_ctx* c, bool finish) {
       float t = 0;
       HIPC(hipEventElapsedTime(&t, c->pev[i], c->pev[i + 1]));
       pm += t;
+      if (c->tuning) {
+        const auto& l = c->plog[i / 2];
+        const int b = size_bucket(l.first);
+        c->tune_ms[l.second][b] += t;
+        c->tune_n[l.second][b] += 1;
+      }
     }
     for (size_t i = 0; i < nd_ev; i += 2) {
       float t = 0;
@@ -1142,6 +1197,7 @@ int oavbd_create(const oavbd_params* params, uint32_t max_bodies, uint32_t max_j
   *out = nullptr;
   oavbd_ctx* c = new (std::nothrow) oavbd_ctx();
   if (!c) return OAVBD_ENOMEM;
+  std::fill(c->ptable, c->ptable + PRIMAL_BUCKETS, (int8_t)-1);
   if (params)
     c->prm = *params;
   else
@@ -2177,6 +2233,107 @@ int oavbd_set_primal_shape(oavbd_ctx* c, uint32_t lanes, uint32_t waves) {
   if (lanes != BODY_LANES) return fail(c, OAVBD_EINVAL, "the primal solve runs 16 lanes per body");
   if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return OAVBD_EINVAL;
   c->waves = (int)waves;
+  return OAVBD_OK;
+}
+
+int oavbd_set_primal_table(oavbd_ctx* c, const int32_t* table, uint32_t n) {
+  if (!c || (n && !table) || n > PRIMAL_BUCKETS) return OAVBD_EINVAL;
+  for (uint32_t b = 0; b < n; ++b)
+    if (table[b] < -1 || table[b] >= PV_COUNT) return fail(c, OAVBD_EINVAL, "bad primal variant");
+  for (uint32_t b = 0; b < PRIMAL_BUCKETS; ++b) c->ptable[b] = b < n ? (int8_t)table[b] : (int8_t)-1;
+  return OAVBD_OK;
+}
+
+int oavbd_get_primal_table(oavbd_ctx* c, int32_t* table, uint32_t n) {
+  if (!c || !table || n > PRIMAL_BUCKETS) return OAVBD_EINVAL;
+  for (uint32_t b = 0; b < n; ++b) table[b] = c->ptable[b];
+  return OAVBD_OK;
+}
+
+// Runs the next `nsteps` steps once per primal variant from the same saved
+// state, every primal launch timed with HIP events and credited to its
+// class-size bucket; each bucket then takes the variant with the least mean
+// launch time, and buckets no launch fell in take the nearest measured
+// bucket's choice.  The state (bodies, pairs, warm starts, sleep), the
+// statistics and the kernel-time accumulators are restored afterwards, so
+// the call changes nothing but the table (all variants give the same bits).
+int oavbd_autotune(oavbd_ctx* c, uint32_t nsteps, int32_t* table_out, float* mean_us_out) {
+  if (!c || nsteps == 0 || nsteps > 64) return OAVBD_EINVAL;
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    if (c->joints_dirty) upload_joints(c);
+    Snap snap;
+    snap_save(c, snap);
+    const oavbd_stats st0 = c->stats;
+    const oavbd_kernel_times kt0 = c->kt;
+    const bool timing0 = c->timing, may_sleep0 = c->may_sleep;
+    const uint32_t sleeping0 = c->sleeping;
+    float ms0[5];
+    std::copy(c->ms, c->ms + 5, ms0);
+    for (int v = 0; v < PV_COUNT; ++v)
+      for (int b = 0; b < PRIMAL_BUCKETS; ++b) {
+        c->tune_ms[v][b] = 0.0;
+        c->tune_n[v][b] = 0;
+      }
+    c->tuning = true;
+    c->timing = true;
+    try {
+      for (int v = 0; v < PV_COUNT; ++v) {
+        snap_load(c, snap);
+        c->may_sleep = may_sleep0;
+        c->sleeping = sleeping0;
+        c->force_variant = v;
+        for (uint32_t k = 0; k < nsteps; ++k) step_inner(c, true);
+      }
+    } catch (...) {
+      c->tuning = false;
+      c->force_variant = -1;
+      snap_free(snap);
+      throw;
+    }
+    c->tuning = false;
+    c->force_variant = -1;
+    snap_load(c, snap);
+    HIPC(hipStreamSynchronize(c->st));
+    snap_free(snap);
+    c->may_sleep = may_sleep0;
+    c->sleeping = sleeping0;
+    c->stats = st0;
+    c->kt = kt0;
+    c->timing = timing0;
+    std::copy(ms0, ms0 + 5, c->ms);
+    int8_t pick[PRIMAL_BUCKETS];
+    for (int b = 0; b < PRIMAL_BUCKETS; ++b) {
+      pick[b] = -1;
+      double best = 0.0;
+      for (int v = 0; v < PV_COUNT; ++v) {
+        if (!c->tune_n[v][b]) continue;
+        const double mean = c->tune_ms[v][b] / c->tune_n[v][b];
+        if (pick[b] < 0 || mean < best) {
+          best = mean;
+          pick[b] = (int8_t)v;
+        }
+      }
+    }
+    for (int b = 0; b < PRIMAL_BUCKETS; ++b) {
+      int8_t v = pick[b];
+      for (int r = 1; v < 0 && r < PRIMAL_BUCKETS; ++r) {
+        if (b - r >= 0 && pick[b - r] >= 0)
+          v = pick[b - r];
+        else if (b + r < PRIMAL_BUCKETS && pick[b + r] >= 0)
+          v = pick[b + r];
+      }
+      c->ptable[b] = v;
+    }
+    if (table_out)
+      for (int b = 0; b < PRIMAL_BUCKETS; ++b) table_out[b] = c->ptable[b];
+    if (mean_us_out)
+      for (int v = 0; v < PV_COUNT; ++v)
+        for (int b = 0; b < PRIMAL_BUCKETS; ++b)
+          mean_us_out[v * PRIMAL_BUCKETS + b] =
+              c->tune_n[v][b] ? (float)(1e3 * c->tune_ms[v][b] / c->tune_n[v][b]) : -1.0f;
+  });
   return OAVBD_OK;
 }
 

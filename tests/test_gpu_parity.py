@@ -203,6 +203,58 @@ def test_launch_shape_does_not_change_bits(gpu_available, shape):
     assert np.array_equal(ref.state().view(np.uint32), g.state().view(np.uint32))
 
 
+def _plate_scene():
+    """A heavy plate on the floor carrying a 6 x 6 grid of cubes: the plate
+    solves 185 contact point-sides (more than the LDS variant stages), every
+    body with rolling and torsion friction."""
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -0.5, 0.0), (10.0, 0.5, 10.0), 0.0, friction=0.6, restitution=0.0)
+    sb.box((0.0, 0.2, 0.0), (3.0, 0.2, 3.0), 50.0, friction=0.6, restitution=0.0)
+    for i in range(36):
+        sb.box((-2.5 + (i % 6), 0.7, -2.5 + (i // 6)), (0.3, 0.3, 0.3), 1.0, friction=0.5, restitution=0.0)
+    s = sb.build()
+    s["rolling"][:] = 0.02
+    s["torsion"][:] = 0.01
+    return s
+
+
+@pytest.mark.parametrize("variant", range(10))
+def test_primal_variant_does_not_change_bits(gpu_available, variant):
+    """Every primal launch variant (sixteen lanes per body at 1-8
+    wavefronts per workgroup, one body per wavefront with its records staged
+    in LDS, or sixteen lanes with the next item's records prefetched) gives
+    the oracle's bits: a random drop, and the plate
+    scene whose plate has more items than the LDS holds, with rolling and
+    torsion rows."""
+    for s in (scenes.box_drop_scene(300, np.random.Generator(np.random.PCG64(19))), _plate_scene()):
+        g = AvbdEngine(scene=s)
+        g.set_primal_table([variant] * 24)
+        o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+        g.step_n(60)
+        o.step_inner(60)
+        assert np.array_equal(g.state().view(np.uint32), o.state().view(np.uint32))
+        assert np.array_equal(g.points().view(np.uint32), o.points().view(np.uint32))
+
+
+def test_autotune_restores_state_and_keeps_bits(gpu_available):
+    """oavbd_autotune leaves the trajectory untouched: a tuned context steps
+    to the same bits as an untuned one, and its table names real variants."""
+    s = scenes.box_drop_scene(400, np.random.Generator(np.random.PCG64(23)))
+    ref = AvbdEngine(scene=s)
+    ref.step_n(50)
+    g = AvbdEngine(scene=s)
+    g.step_n(30)
+    before = (g.state(), g.points(), g.sleep()[1])
+    table = g.autotune(3)
+    after = (g.state(), g.points(), g.sleep()[1])
+    for x, y in zip(before, after):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    assert ((table >= 0) & (table < 10)).all()
+    assert np.array_equal(g.primal_table(), table)
+    g.step_n(20)
+    assert np.array_equal(ref.state().view(np.uint32), g.state().view(np.uint32))
+
+
 def _rng(seed):
     return np.random.Generator(np.random.PCG64(seed))
 
