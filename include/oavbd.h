@@ -68,7 +68,7 @@ typedef struct oavbd_stats {
   uint32_t pair_sides;  /* contact pair-sides (with points) of swept bodies  */
   uint32_t joint_sides; /* joint sides of swept bodies                       */
   uint32_t toi_casts;   /* (mover, target) casts tested by the TOI clamp     */
-  uint32_t joints;      /* Ball joints held by the context                   */
+  uint32_t joints;      /* joints held by the context                        */
 } oavbd_stats;
 
 /* Kernel time and algorithmic bytes accumulated since the last reset
@@ -185,10 +185,27 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
  * oavbd_get_state, oavbd_get_sleep and oavbd_get_contacts. */
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
-/* Ball joint between bodies a and b with body-local anchors.  Replaces:
- * PhysicsEngine::add_joint with JointKind::Ball (mod.rs:1308-1510,
- * crates/physics/src/joint.rs:1273-1279).  Returns the joint index through
- * *out_index (may be NULL).  EINVAL for a == b or an out-of-range body. */
+/* Joint kinds of oavbd_add_joint (the reference's JointKind arms without
+ * limit or motor; the other kinds are not built). */
+enum { OAVBD_JOINT_BALL = 0, OAVBD_JOINT_REVOLUTE = 1, OAVBD_JOINT_PRISMATIC = 2, OAVBD_JOINT_FIXED = 3 };
+
+/* Joint of `kind` between bodies a and b: body-local anchors la, lb and,
+ * for Revolute and Prismatic, body-local hinge / slide axes axis_a, axis_b
+ * (normalised here; NULL for the other kinds).  A Fixed joint captures the
+ * bodies' current relative rotation and anchor offset as its assembly pose.
+ * Both bodies wake first, as in the reference, even when the spec is then
+ * rejected: EINVAL for a == b, an out-of-range body, an unknown kind or a
+ * degenerate axis.  Jointed bodies do not collide, except through a Fixed
+ * joint.  Returns the joint index through *out_index (may be NULL).
+ * Replaces: PhysicsEngine::add_joint for JointKind::{Ball, Revolute,
+ * Prismatic, Fixed} with limit and motor None (mod.rs:1308-1510; frames of
+ * resolve_joint, crates/physics/src/joint.rs:1251-1323). */
+int oavbd_add_joint(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                    const float axis_a[3], const float axis_b[3], uint32_t* out_index);
+
+/* oavbd_add_joint with OAVBD_JOINT_BALL.  Replaces: PhysicsEngine::add_joint
+ * with JointKind::Ball (mod.rs:1308-1510, crates/physics/src/joint.rs:
+ * 1273-1279). */
 int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                          uint32_t* out_index);
 
@@ -229,8 +246,8 @@ int oavbd_get_points(oavbd_ctx* ctx, float* out16, uint32_t cap, uint32_t* n);
  * ContactEventKind::Hit, crates/physics/src/avbd/mod.rs:919-937). */
 int oavbd_get_toi(oavbd_ctx* ctx, float* t, int32_t* target, uint32_t n);
 
-/* Ball-joint duals: 6 floats each (lam3, pen3). */
-int oavbd_get_joints(oavbd_ctx* ctx, float* out6, uint32_t n);
+/* Joint duals, 12 floats each: linear lam3, pen3, angular lam3, pen3. */
+int oavbd_get_joints(oavbd_ctx* ctx, float* out12, uint32_t n);
 
 int oavbd_get_stats(oavbd_ctx* ctx, oavbd_stats* out);
 

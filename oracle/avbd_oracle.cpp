@@ -83,6 +83,7 @@ struct K {
   static T ALPHA() { return CF(0.99); }
   static T GAMMA() { return CF(0.999); }
   static T BETA() { return CF(10000.0); }
+  static T BETA_ANG() { return CF(100.0); }
   static T MARGIN() { return CF(0.01); }
   static T PEN_MIN() { return CF(1.0); }
   static T PEN_MAX() { return CF(1.0e10); }
@@ -152,11 +153,20 @@ struct Pair {
   std::vector<Point<T>> pts;
 };
 
+// Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300): the
+// remaining kinds are out of its scope.
+enum JointKindC : int { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3 };
+
 template <class T>
 struct Joint {
   int a, b;
+  int kind = JK_BALL;
   V3<T> la, lb;
+  V3<T> ax_a{T(1), T(0), T(0)}, ax_b{T(1), T(0), T(0)};  // unit hinge / slide axes (body frames)
+  V3<T> dref{T(0), T(0), T(0)};                          // Fixed: assembly anchor offset in A's frame
+  Q<T> q_ref{T(0), T(0), T(0), T(1)};                    // Fixed: assembly relative rotation conj(qa) qb
   T lam_l[3], pen_l[3];
+  T lam_a[3], pen_a[3];
 };
 
 template <class T>
@@ -738,22 +748,46 @@ struct Engine {
     for (int h = 0; h < (int)bodies.size(); ++h) wake_body(h);
   }
 
-  // add_joint, Ball arm (mod.rs:1308-1510; joint.rs:1273-1279).
-  int add_ball_joint(int a, int b, V3<T> la, V3<T> lb) {
+  // add_joint for Ball, Revolute, Prismatic and Fixed without limits or
+  // motors (mod.rs:1308-1510) with the frames of resolve_joint
+  // (joint.rs:1251-1323): hinge / slide axes normalised (a degenerate axis
+  // rejects), Fixed captures the assembly rotation and anchor offset.
+  int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb) {
     wake_body(a);
     wake_body(b);
     if (a == b || a < 0 || b < 0 || a >= (int)bodies.size() || b >= (int)bodies.size()) return -1;
+    if (kind < JK_BALL || kind > JK_FIXED) return -1;
     Joint<T> j;
     j.a = a;
     j.b = b;
+    j.kind = kind;
     j.la = la;
     j.lb = lb;
+    const V3<T> Z{T(0), T(0), T(1)};
+    if (kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
+      if (dot(axa, axa) < K<T>::DEGENERATE() || dot(axb, axb) < K<T>::DEGENERATE()) return -1;
+      j.ax_a = normalize_or(axa, Z);
+      j.ax_b = normalize_or(axb, Z);
+    }
+    if (kind == JK_FIXED) {
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      j.q_ref = quat_mul(conj(A.rot), B.rot);
+      V3<T> ra = rotate(A.rot, la), rb = rotate(B.rot, lb);
+      j.dref = rotate(conj(A.rot), (B.pos + rb) - (A.pos + ra));
+    }
     for (int k = 0; k < 3; ++k) {
       j.lam_l[k] = T(0);
       j.pen_l[k] = K<T>::JOINT_PEN_INIT();
+      j.lam_a[k] = T(0);
+      j.pen_a[k] = K<T>::JOINT_PEN_INIT();
     }
     joints.push_back(j);
     return (int)joints.size() - 1;
+  }
+  int add_ball_joint(int a, int b, V3<T> la, V3<T> lb) {
+    const V3<T> X{T(1), T(0), T(0)};
+    return add_joint(JK_BALL, a, b, la, lb, X, X);
   }
 
   // sleep_body / wake_body (mod.rs:822-842; body.rs:403-418).
@@ -781,8 +815,9 @@ struct Engine {
     return (a.layer & b.mask) != 0 && (b.layer & a.mask) != 0;
   }
   bool ball_jointed(int ia, int ib) const {
+    // every kind here but Fixed disables contact between its bodies
     for (const auto& j : joints)
-      if ((j.a == ia && j.b == ib) || (j.a == ib && j.b == ia)) return true;
+      if (j.kind != JK_FIXED && ((j.a == ia && j.b == ib) || (j.a == ib && j.b == ia))) return true;
     return false;
   }
 
@@ -791,7 +826,20 @@ struct Engine {
     const Body<T>& a = bodies[j.a];
     const Body<T>& b = bodies[j.b];
     V3<T> delta = (b.pos + rotate(b.rot, j.lb)) - (a.pos + rotate(a.rot, j.la));
-    return length(delta);
+    V3<T> axis = rotate(a.rot, j.ax_a);
+    switch (j.kind) {
+      case JK_REVOLUTE:
+        return fmax_(length(delta), length(axis - rotate(b.rot, j.ax_b)));
+      case JK_FIXED:
+        return fmax_(length(delta - rotate(a.rot, j.dref)),
+                     length(quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math)));
+      case JK_PRISMATIC: {
+        V3<T> perp = delta - axis * dot(delta, axis);
+        return fmax_(length(perp), length(axis - rotate(b.rot, j.ax_b)));
+      }
+      default:
+        return length(delta);
+    }
   }
   bool joint_driven(const Joint<T>& j) const {
     int m[2] = {j.a, j.b};
@@ -1691,18 +1739,41 @@ struct Engine {
       V3<T> c0v = pa0 - pb0;
       V3<T> fl{T(0), T(0), T(0)};
       const V3<T> Z{T(0), T(0), T(0)};
-      for (int k = 0; k < 3; ++k) {
-        V3<T> axis{T(0), T(0), T(0)};
-        axis[k] = T(1);
-        T c = live[k] - K<T>::ALPHA() * c0v[k];
-        T f = j.pen_l[k] * c + j.lam_l[k];
-        if (!row_live(c, f)) continue;
-        V3<T> r_side = is_a ? rotate(a.rot, j.la + Z) : rotate(b.rot, j.lb);
-        acc.stamp(axis, j.pen_l[k], f, r_side, sign);
-        fl[k] = f;
+      const bool fixed = j.kind == JK_FIXED;
+      // live slide / hinge axis, from A's side (assembly.rs:273-275)
+      const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
+      if (j.kind != JK_PRISMATIC) {
+        // Ball / Revolute / Fixed: three world-axis rows at the anchors
+        // (assembly.rs:279-310), Fixed offset by the assembly delta
+        const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;
+        const V3<T> in0 = fixed ? c0v + rotate(rot0[j.a], j.dref) : c0v;
+        for (int k = 0; k < 3; ++k) {
+          V3<T> axis{T(0), T(0), T(0)};
+          axis[k] = T(1);
+          T c = lv[k] - K<T>::ALPHA() * in0[k];
+          T f = j.pen_l[k] * c + j.lam_l[k];
+          if (!row_live(c, f)) continue;
+          V3<T> r_side = is_a ? rotate(a.rot, j.la + (fixed ? j.dref : Z)) : rotate(b.rot, j.lb);
+          acc.stamp(axis, j.pen_l[k], f, r_side, sign);
+          fl[k] = f;
+        }
+      } else {
+        // Prismatic: the two rows across the live slide axis (:311-327)
+        V3<T> u, v;
+        tangent_basis(wa, &u, &v);
+        const V3<T> axes[2] = {u, v};
+        for (int li = 0; li < 2; ++li) {
+          const V3<T> ax = axes[li];
+          T c = dot(live - c0v * K<T>::ALPHA(), ax);
+          T f = j.pen_l[li] * c + j.lam_l[li];
+          if (!row_live(c, f)) continue;
+          V3<T> r_side = is_a ? rotate(a.rot, j.la) : rotate(b.rot, j.lb);
+          acc.stamp(ax, j.pen_l[li], f, r_side, sign);
+          fl = fl + ax * f;
+        }
       }
       // geometric stiffness (assembly.rs:589-618; rows.rs:69-88)
-      V3<T> r = is_a ? rotate(a.rot, j.la + Z) : -rotate(b.rot, j.lb);
+      V3<T> r = is_a ? rotate(a.rot, j.la + (fixed ? j.dref : Z)) : -rotate(b.rot, j.lb);
       T hm[3][3] = {};
       for (int k = 0; k < 3; ++k) {
         T g[3][3] = {};
@@ -1718,6 +1789,34 @@ struct Engine {
       V3<T> hd{length(V3<T>{hm[0][0], hm[1][0], hm[2][0]}), length(V3<T>{hm[0][1], hm[1][1], hm[2][1]}),
                length(V3<T>{hm[0][2], hm[1][2], hm[2][2]})};
       acc.diag(hd);
+      // angular equality rows (assembly.rs:620-648, :787-808)
+      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC) {
+        const V3<T> axa = rotate(a.rot, j.ax_a), axb = rotate(b.rot, j.ax_b);
+        const V3<T> axa0 = rotate(rot0[j.a], j.ax_a), axb0 = rotate(rot0[j.b], j.ax_b);
+        V3<T> t1, t2;
+        tangent_basis(axa0, &t1, &t2);
+        const V3<T> ts[2] = {t1, t2};
+        for (int li = 0; li < 2; ++li) {
+          const V3<T> t = ts[li];
+          T c = dot(t, axa - axb) - K<T>::ALPHA() * dot(t, axa0 - axb0);
+          T f = j.pen_a[li] * c + j.lam_a[li];
+          if (!row_live(c, f)) continue;
+          V3<T> g = is_a ? cross(axa, t) : -cross(axb, t);
+          acc.couple(g, j.pen_a[li], f);
+        }
+      } else if (fixed) {
+        const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
+        const V3<T> diff0 = quat_diff_vec(quat_mul(conj(rot0[j.a]), rot0[j.b]), j.q_ref, math);
+        for (int k = 0; k < 3; ++k) {
+          T c = diff[k] - K<T>::ALPHA() * diff0[k];
+          T f = j.pen_a[k] * c + j.lam_a[k];
+          if (!row_live(c, f)) continue;
+          V3<T> e{T(0), T(0), T(0)};
+          e[k] = T(1);
+          const V3<T> dir = rotate(a.rot, e);
+          acc.couple(is_a ? -dir : dir, j.pen_a[k], f);
+        }
+      }
     }
     acc.finish();
 
@@ -1810,11 +1909,51 @@ struct Engine {
       V3<T> pa0 = pos0[j.a] + rotate(rot0[j.a], j.la);
       V3<T> pb0 = pos0[j.b] + rotate(rot0[j.b], j.lb);
       V3<T> live = pa - pb, c0v = pa0 - pb0;
-      for (int k = 0; k < 3; ++k) {
-        T c = live[k] - K<T>::ALPHA() * c0v[k];
-        if (std::fabs(c) >= K<T>::C_EPS()) {
-          j.lam_l[k] += j.pen_l[k] * c;
-          j.pen_l[k] = fmin_(j.pen_l[k] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+      const bool fixed = j.kind == JK_FIXED;
+      if (j.kind != JK_PRISMATIC) {
+        const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;
+        const V3<T> in0 = fixed ? c0v + rotate(rot0[j.a], j.dref) : c0v;
+        for (int k = 0; k < 3; ++k) {
+          T c = lv[k] - K<T>::ALPHA() * in0[k];
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            j.lam_l[k] += j.pen_l[k] * c;
+            j.pen_l[k] = fmin_(j.pen_l[k] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+          }
+        }
+      } else {
+        const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
+        V3<T> u, v;
+        tangent_basis(wa, &u, &v);
+        const V3<T> axes[2] = {u, v};
+        for (int li = 0; li < 2; ++li) {
+          T c = dot(live - c0v * K<T>::ALPHA(), axes[li]);
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            j.lam_l[li] += j.pen_l[li] * c;
+            j.pen_l[li] = fmin_(j.pen_l[li] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+          }
+        }
+      }
+      // angular equality rows (joints_ext.rs:767-822)
+      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC) {
+        const V3<T> axa = rotate(a.rot, j.ax_a), axb = rotate(b.rot, j.ax_b);
+        const V3<T> axa0 = rotate(rot0[j.a], j.ax_a), axb0 = rotate(rot0[j.b], j.ax_b);
+        V3<T> t1, t2;
+        tangent_basis(axa0, &t1, &t2);
+        const V3<T> ts[2] = {t1, t2};
+        for (int li = 0; li < 2; ++li) {
+          T c = dot(ts[li], axa - axb) - K<T>::ALPHA() * dot(ts[li], axa0 - axb0);
+          if (std::fabs(c) < K<T>::C_EPS()) continue;
+          j.lam_a[li] = j.pen_a[li] * c + j.lam_a[li];
+          j.pen_a[li] = fmin_(j.pen_a[li] + K<T>::BETA_ANG() * std::fabs(c), K<T>::PEN_MAX());
+        }
+      } else if (fixed) {
+        const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
+        const V3<T> diff0 = quat_diff_vec(quat_mul(conj(rot0[j.a]), rot0[j.b]), j.q_ref, math);
+        for (int k = 0; k < 3; ++k) {
+          T c = diff[k] - K<T>::ALPHA() * diff0[k];
+          if (std::fabs(c) < K<T>::C_EPS()) continue;
+          j.lam_a[k] += j.pen_a[k] * c;
+          j.pen_a[k] = fmin_(j.pen_a[k] + K<T>::BETA_ANG() * std::fabs(c), K<T>::PEN_MAX());
         }
       }
     }
@@ -2064,7 +2203,9 @@ struct Engine {
     for (auto& j : joints)
       for (int k = 0; k < 3; ++k) {
         j.lam_l[k] *= AG;
+        j.lam_a[k] *= AG;
         j.pen_l[k] = clampf(j.pen_l[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
+        j.pen_a[k] = clampf(j.pen_a[k] * K<T>::GAMMA(), K<T>::PEN_MIN(), K<T>::PEN_MAX());
       }
     for (int h = 0; h < n; ++h) {
       if (!solvable(h) || asleep[h]) continue;
@@ -2285,6 +2426,18 @@ void orc_add_bodies(void* h, int n, const int* shape, const float* sp, const flo
                          type, rolling, torsion));
 }
 
+int orc_add_joint(void* h, int kind, int a, int b, const float* la, const float* lb, const float* axa,
+                  const float* axb) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = -1;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    r = E->add_joint(kind, a, b, V3<TT>{TT(la[0]), TT(la[1]), TT(la[2])}, V3<TT>{TT(lb[0]), TT(lb[1]), TT(lb[2])},
+                     V3<TT>{TT(axa[0]), TT(axa[1]), TT(axa[2])}, V3<TT>{TT(axb[0]), TT(axb[1]), TT(axb[2])});
+  });
+  return r;
+}
+
 int orc_add_ball_joint(void* h, int a, int b, const float* la, const float* lb) {
   OracleCtx* c = (OracleCtx*)h;
   int r = -1;
@@ -2457,14 +2610,16 @@ void orc_set_sleep(void* h, const uint8_t* asleep, const float* timer) {
   });
 }
 
-// joints: 6 floats each: lam_l3 pen_l3.
+// joints: 12 floats each: lam_l3 pen_l3 lam_a3 pen_a3.
 void orc_get_joints(void* h, float* out) {
   OracleCtx* c = (OracleCtx*)h;
   DISPATCH(c, {
     for (size_t i = 0; i < E->joints.size(); ++i)
       for (int k = 0; k < 3; ++k) {
-        out[6 * i + k] = (float)E->joints[i].lam_l[k];
-        out[6 * i + 3 + k] = (float)E->joints[i].pen_l[k];
+        out[12 * i + k] = (float)E->joints[i].lam_l[k];
+        out[12 * i + 3 + k] = (float)E->joints[i].pen_l[k];
+        out[12 * i + 6 + k] = (float)E->joints[i].lam_a[k];
+        out[12 * i + 9 + k] = (float)E->joints[i].pen_a[k];
       }
   });
 }
@@ -2648,6 +2803,49 @@ void orc_set_vel(void* h, int i, const float* vel, const float* angvel) {
     using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
     E->bodies[i].vel = {TT(vel[0]), TT(vel[1]), TT(vel[2])};
     E->bodies[i].angvel = {TT(angvel[0]), TT(angvel[1]), TT(angvel[2])};
+  });
+}
+
+// One primal body solve / one dual update outside a step (unit pins of
+// the joint rows); the incidence lists are rebuilt first.
+void orc_solve_body(void* h, int body) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    E->build_incidence();
+    E->solve_body(body);
+  });
+}
+void orc_dual_update(void* h) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, E->dual_update());
+}
+// Joint j's duals: 12 floats lam_l3 pen_l3 lam_a3 pen_a3.
+void orc_set_joint_duals(void* h, int j, const float* d12) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    for (int k = 0; k < 3; ++k) {
+      E->joints[j].lam_l[k] = TT(d12[k]);
+      E->joints[j].pen_l[k] = TT(d12[3 + k]);
+      E->joints[j].lam_a[k] = TT(d12[6 + k]);
+      E->joints[j].pen_a[k] = TT(d12[9 + k]);
+    }
+  });
+}
+// Inertial target (position and rotation) of body i.
+void orc_set_inertial(void* h, int i, const float* pos, const float* quat) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->inertial[i] = {TT(pos[0]), TT(pos[1]), TT(pos[2])};
+    E->inertial_rot[i] = {TT(quat[0]), TT(quat[1]), TT(quat[2]), TT(quat[3])};
+  });
+}
+// Body i's inertia diagonal (body frame).
+void orc_get_inertia(void* h, int i, float* out3) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (int k = 0; k < 3; ++k) out3[k] = (float)E->bodies[i].inertia[k];
   });
 }
 

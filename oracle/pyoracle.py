@@ -47,6 +47,9 @@ def lib():
         L.orc_destroy.argtypes = [ctypes.c_void_p]
         L.orc_add_bodies.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                      _f32p, _f32p, _f32p, _u32p, _u32p, _u8p, _i32p, _f32p, _f32p]
+        L.orc_add_joint.restype = ctypes.c_int
+        L.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p,
+                                    _f32p]
         L.orc_add_ball_joint.restype = ctypes.c_int
         L.orc_add_ball_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]
@@ -84,6 +87,12 @@ def lib():
         L.orc_frozen_pairs.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_int]
         L.orc_copy_frozen.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.orc_set_pose.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
+        L.orc_set_step_start.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
+        L.orc_solve_body.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_dual_update.argtypes = [ctypes.c_void_p]
+        L.orc_set_joint_duals.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p]
+        L.orc_set_inertial.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
+        L.orc_get_inertia.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p]
         L.orc_remove_body.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_set_contacts.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, _f32p, _f32p, _u32p, _f32p]
         L.orc_set_sleep.argtypes = [ctypes.c_void_p, _u8p, _f32p]
@@ -101,6 +110,13 @@ def _p(a, t):
 
 def f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def quat_integrate(q, v, math=MATH_LIBM):
+    """quat_integrate (rows.rs:110-119) in float32."""
+    out = np.zeros(4, np.float32)
+    lib().orc_quat_integrate_f32(_p(f32(q), _f32p), _p(f32(v), _f32p), int(math), _p(out, _f32p))
+    return out
 
 
 def stage_diag_solve(masses, rhs, dt=1.0 / 60.0):
@@ -135,10 +151,8 @@ class OracleEngine:
         self.h = self._L.orc_create(int(precision), _p(g, _f32p), int(order), int(math), int(iters), int(brute))
         self.n = 0
         self.add_bodies(scene)
-        for (a, b, la, lb) in scene.get("joints", []):
-            r = self._L.orc_add_ball_joint(self.h, int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p))
-            if r < 0:
-                raise ValueError("bad joint")
+        for jt in scene.get("joints", []):
+            self.add_joint(*jt)
 
     def add_bodies(self, s):
         n = len(s["shape"])
@@ -157,6 +171,27 @@ class OracleEngine:
     def set_pose(self, i, pos, quat):
         """Overwrite body i's pose (a driver teleport)."""
         self._L.orc_set_pose(self.h, int(i), _p(f32(pos), _f32p), _p(f32(quat), _f32p))
+
+    # unit-pin hooks: single body solves and dual updates outside a step
+    def set_step_start(self, i, pos, quat):
+        self._L.orc_set_step_start(self.h, int(i), _p(f32(pos), _f32p), _p(f32(quat), _f32p))
+
+    def set_inertial(self, i, pos, quat):
+        self._L.orc_set_inertial(self.h, int(i), _p(f32(pos), _f32p), _p(f32(quat), _f32p))
+
+    def set_joint_duals(self, j, d12):
+        self._L.orc_set_joint_duals(self.h, int(j), _p(f32(d12), _f32p))
+
+    def solve_body(self, i):
+        self._L.orc_solve_body(self.h, int(i))
+
+    def dual_update(self):
+        self._L.orc_dual_update(self.h)
+
+    def inertia(self, i):
+        out = np.zeros(3, np.float32)
+        self._L.orc_get_inertia(self.h, int(i), _p(out, _f32p))
+        return out
 
     def remove_body(self, h):
         """Swap-remove body h (the reference's remove_body)."""
@@ -270,8 +305,18 @@ class OracleEngine:
         self._L.orc_get_sleep(self.h, _p(a, _u8p), _p(t, _f32p))
         return a, t
 
+    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
+        """add_joint: kind 0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed (local
+        anchors la, lb; local hinge / slide axes for kinds 1 and 2)."""
+        r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
+                                  _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))
+        if r < 0:
+            raise ValueError("bad joint")
+        return r
+
     def joints(self, nj):
-        out = np.zeros((nj, 6), np.float32)
+        """Joint duals, nj x 12: lam_l3 pen_l3 lam_a3 pen_a3."""
+        out = np.zeros((nj, 12), np.float32)
         self._L.orc_get_joints(self.h, _p(out, _f32p))
         return out
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "oavbd_set_contacts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
                                           ctypes.c_uint32]),
     "oavbd_set_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
+    "oavbd_add_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _f32p,
+                                       _f32p, _f32p, _f32p, _u32p]),
     "oavbd_add_ball_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                             _u32p]),
     "oavbd_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float]),
@@ -234,8 +236,8 @@ class AvbdEngine:
         self._check(self._L.oavbd_set_bodies(self._ctx, *args))
         self.n = n
         self.njoints = 0
-        for (a, b, la, lb) in joints:
-            self.add_ball_joint(a, b, la, lb)
+        for jt in joints:
+            self.add_joint(*jt)
 
     # PhysicsEngine::add_body / remove_body
     def add_bodies(self, s):
@@ -276,6 +278,18 @@ class AvbdEngine:
         a = np.ascontiguousarray(asleep, dtype=np.uint8)
         t = np.ascontiguousarray(timer, dtype=np.float32)
         self._check(self._L.oavbd_set_sleep(self._ctx, _p(a, _u8p), _p(t, _f32p), self.n))
+
+    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
+        """Joint of `kind` (0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed)."""
+        la = np.ascontiguousarray(la, dtype=np.float32)
+        lb = np.ascontiguousarray(lb, dtype=np.float32)
+        xa = np.ascontiguousarray(axis_a, dtype=np.float32)
+        xb = np.ascontiguousarray(axis_b, dtype=np.float32)
+        idx = ctypes.c_uint32()
+        self._check(self._L.oavbd_add_joint(self._ctx, int(kind), int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
+                                            _p(xa, _f32p), _p(xb, _f32p), ctypes.byref(idx)))
+        self.njoints += 1
+        return idx.value
 
     def add_ball_joint(self, a, b, la, lb):
         la = np.ascontiguousarray(la, dtype=np.float32)
@@ -338,7 +352,7 @@ class AvbdEngine:
         return t, tg
 
     def joints(self):
-        out = np.zeros((self.njoints, 6), np.float32)
+        out = np.zeros((self.njoints, 12), np.float32)
         self._check(self._L.oavbd_get_joints(self._ctx, _p(out, _f32p), self.njoints))
         return out
 

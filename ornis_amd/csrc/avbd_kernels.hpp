@@ -207,9 +207,20 @@ __device__ __forceinline__ int gather_buckets(const Dev& d, uint32_t sc, int lx,
 // TOI clamp (mod.rs:882-978; cast_shape, distance.rs:841-891).  Every cast
 // reads the step-top poses; clamps are applied afterwards.
 // ---------------------------------------------------------------------------
+// Bodies joined to h by a joint of any kind (the TOI step excludes every
+// joined partner, Fixed ones included: mod.rs:900-905), through h's
+// incidence list.
+__device__ __forceinline__ bool joined_dev(const Dev& d, uint32_t h, uint32_t o) {
+  for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
+    const uint32_t e = d.b.ji[k];
+    const uint2 ab = d.J.ab[e >> 1];
+    if ((e & 1u) == 0u ? ab.y == o : ab.x == o) return true;
+  }
+  return false;
+}
 __device__ __forceinline__ bool toi_target_ok(const Dev& d, uint32_t h, uint32_t o) {
   if (o == h || d.b.trigger[o] || d.b.scene[o] != d.b.scene[h] || !pair_allowed_dev(d, h, o)) return false;
-  return !jointed_dev(d, h, o);
+  return !joined_dev(d, h, o);
 }
 
 // Conservative advancement of mover against target (cast_shape,
@@ -1119,7 +1130,8 @@ __global__ void k_prop_clear(Dev d) {
   d.b.comp_any[h] = 0;
 }
 
-// Ball joint: driven (a non-dynamic member teleported) or off its rest pose.
+// A joint driven (a non-dynamic member teleported) or off its rest pose
+// (joint_rest_error, sleep.rs:30-66).
 __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   uint2 ab = d.J.ab[j];
   uint32_t m[2] = {ab.x, ab.y};
@@ -1132,9 +1144,23 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
         return true;
     }
   }
-  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(asq(d.b.rot[ab.x]), xyz(d.J.la[j])));
-  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(asq(d.b.rot[ab.y]), xyz(d.J.lb[j])));
-  return len3(sub(pb, pa)) > K_JOINT_REST;
+  const qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  f3 pa = add(xyz(d.b.pos[ab.x]), qrot(qa, xyz(d.J.la[j])));
+  f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, xyz(d.J.lb[j])));
+  const f3 delta = sub(pb, pa);
+  float err;
+  const int kind = d.J.kind[j];
+  if (kind == JK_BALL) {
+    err = len3(delta);
+  } else if (kind == JK_FIXED) {  // joint_rest_error (sleep.rs:30-66)
+    err = fmax_(len3(sub(delta, qrot(qa, xyz(d.J.dref[j])))), len3(quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]))));
+  } else {
+    const f3 axis = qrot(qa, xyz(d.J.ax[2 * j]));
+    const float e_axis = len3(sub(axis, qrot(qb, xyz(d.J.ax[2 * j + 1]))));
+    err = kind == JK_REVOLUTE ? fmax_(len3(delta), e_axis)
+                              : fmax_(len3(sub(delta, mul(axis, dot3(delta, axis)))), e_axis);
+  }
+  return err > K_JOINT_REST;
 }
 
 // wake_joint_motion (sleep.rs:107-137): per-body part.
@@ -1258,18 +1284,44 @@ __global__ void __launch_bounds__(TPB) k_wake_begin(Dev d, uint32_t may_sleep) {
   block_add(&d.counters[CTR_FAST], fast);
 }
 
+// Joint warm-start decay (mod.rs:1034-1041) and the step-start terms of
+// the joint rows: the anchor separation (Fixed: plus rot0_a dref,
+// assembly.rs:286-290), the hinge rows' tangent basis of the step-start
+// axis with t . (axa0 - axb0) (:620-634), the Fixed step-start rotation
+// error (:790).
+__device__ __forceinline__ float4 decay_lam(float4 l) {
+  const float AG = K_ALPHA * K_GAMMA;
+  return make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+}
+__device__ __forceinline__ float4 decay_pen(float4 pe) {
+  return make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
+                     clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+}
 __global__ void k_joint_prep(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
-  const float AG = K_ALPHA * K_GAMMA;
-  float4 l = d.J.lam[j], pe = d.J.pen[j];
-  d.J.lam[j] = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
-  d.J.pen[j] = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
-                           clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+  d.J.lam[j] = decay_lam(d.J.lam[j]);
+  d.J.pen[j] = decay_pen(d.J.pen[j]);
+  d.J.lama[j] = decay_lam(d.J.lama[j]);
+  d.J.pena[j] = decay_pen(d.J.pena[j]);
+  const int kind = d.J.kind[j];
   uint2 ab = d.J.ab[j];
-  f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(asq(d.b.rot0[ab.x]), xyz(d.J.la[j])));
-  f3 pb0 = add(xyz(d.b.pos0[ab.y]), qrot(asq(d.b.rot0[ab.y]), xyz(d.J.lb[j])));
-  d.J.c0v[j] = pk(sub(pa0, pb0), 0.0f);
+  const qf ra0 = asq(d.b.rot0[ab.x]), rb0 = asq(d.b.rot0[ab.y]);
+  f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(ra0, xyz(d.J.la[j])));
+  f3 pb0 = add(xyz(d.b.pos0[ab.y]), qrot(rb0, xyz(d.J.lb[j])));
+  f3 c0v = sub(pa0, pb0);
+  if (kind == JK_FIXED) c0v = add(c0v, qrot(ra0, xyz(d.J.dref[j])));
+  d.J.c0v[j] = pk(c0v, 0.0f);
+  if (kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
+    const f3 axa0 = qrot(ra0, xyz(d.J.ax[2 * j])), axb0 = qrot(rb0, xyz(d.J.ax[2 * j + 1]));
+    f3 t1, t2;
+    tangent_basis(axa0, &t1, &t2);
+    const f3 dax = sub(axa0, axb0);
+    d.J.a0[2 * j] = pk(t1, dot3(t1, dax));
+    d.J.a0[2 * j + 1] = pk(t2, dot3(t2, dax));
+  } else if (kind == JK_FIXED) {
+    d.J.a0[2 * j] = pk(quat_diff_vec(qmul(qconj(ra0), rb0), asq(d.J.qref[j])), 0.0f);
+  }
 }
 
 // Per-pair step constants: friction frame, separated-damper stiffness
@@ -1908,14 +1960,20 @@ __device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 
   item_rows<R>(d, it, r, own, emit);
 }
 
-// Rows of one Ball-joint side (assembly.rs:279-310, :589-618).  Slots
-// 0-2: the three axis stamps; slot 3: the geometric-stiffness diagonal.
-// Returns the mask of slots written.
+// Rows of one joint side, in the reference's order (assembly.rs:253-818):
+// the linear rows (Ball / Revolute / Fixed: three world-axis rows at the
+// anchors, Fixed offset by the assembly delta; Prismatic: two rows across
+// the live slide axis), the geometric-stiffness diagonal, then the angular
+// rows (Revolute / Prismatic: two rows aligning the axes; Fixed: three rows
+// on the relative-rotation error about A's axes).  Linear rows are stamps
+// (kind 0), the diagonal kind 2, angular rows couples (kind 1: x = (0, g),
+// its pen holding -K as the rolling rows do).
 template <class Emit>
 __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit) {
   uint32_t j = e >> 1;
   bool is_a = (e & 1u) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
+  const int kind = d.J.kind[j];
   uint2 ab = d.J.ab[j];
   const float4* sa = d.sb + 4 * (size_t)ab.x;
   const float4* sbb = d.sb + 4 * (size_t)ab.y;
@@ -1931,16 +1989,37 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   f3 r_b = qrot(qb, lb);
   float fl[3] = {0.0f, 0.0f, 0.0f};
   Row row;
+  if (kind != JK_PRISMATIC) {
+    if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float c = comp(live, k) - K_ALPHA * comp(c0v, k);
-    float f = penv[k] * c + lamv[k];
-    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
-    f3 axis = zero3();
-    setc(axis, k, 1.0f);
-    make_stamp(row, axis, penv[k], f, is_a ? r_a : r_b, sign);
-    emit(0, row);
-    fl[k] = f;
+    for (int k = 0; k < 3; ++k) {
+      float c = comp(live, k) - K_ALPHA * comp(c0v, k);
+      float f = penv[k] * c + lamv[k];
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      f3 axis = zero3();
+      setc(axis, k, 1.0f);
+      make_stamp(row, axis, penv[k], f, is_a ? r_a : r_b, sign);
+      emit(0, row);
+      fl[k] = f;
+    }
+  } else {
+    const f3 wa = normalize_or(qrot(qa, xyz(d.J.ax[2 * j])), f3{0.0f, 0.0f, 1.0f});
+    f3 u, v;
+    tangent_basis(wa, &u, &v);
+    const f3 dl = sub(live, mul(c0v, K_ALPHA));
+    const f3 r_side = is_a ? qrot(qa, la) : r_b;
+#pragma unroll
+    for (int li = 0; li < 2; ++li) {
+      const f3 ax = li == 0 ? u : v;
+      float c = dot3(dl, ax);
+      float f = penv[li] * c + lamv[li];
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      make_stamp(row, ax, penv[li], f, r_side, sign);
+      emit(0, row);
+      fl[0] = fl[0] + ax.x * f;
+      fl[1] = fl[1] + ax.y * f;
+      fl[2] = fl[2] + ax.z * f;
+    }
   }
   f3 r = is_a ? r_a : neg(r_b);
   float hm[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
@@ -1963,6 +2042,42 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   row.pen = 0.0f;
   row.f = 0.0f;
   emit(2, row);
+  if (kind == JK_BALL) return;
+  const float4 lama = d.J.lama[j], pena = d.J.pena[j];
+  const float lav[3] = {lama.x, lama.y, lama.z}, pav[3] = {pena.x, pena.y, pena.z};
+  auto angular = [&](f3 g, float pn, float f) {
+    row.x[0] = 0.0f; row.x[1] = 0.0f; row.x[2] = 0.0f;
+    row.x[3] = g.x; row.x[4] = g.y; row.x[5] = g.z;
+    row.pen = -pn;
+    row.f = f;
+    emit(1, row);
+  };
+  if (kind == JK_FIXED) {
+    const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]));
+    const f3 diff0 = xyz(d.J.a0[2 * j]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float c = comp(diff, k) - K_ALPHA * comp(diff0, k);
+      float f = pav[k] * c + lav[k];
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      f3 ek = zero3();
+      setc(ek, k, 1.0f);
+      const f3 dir = qrot(qa, ek);
+      angular(is_a ? neg(dir) : dir, pav[k], f);
+    }
+    return;
+  }
+  const f3 axa = qrot(qa, xyz(d.J.ax[2 * j])), axb = qrot(qb, xyz(d.J.ax[2 * j + 1]));
+  const f3 dax = sub(axa, axb);
+#pragma unroll
+  for (int li = 0; li < 2; ++li) {
+    const float4 t4 = d.J.a0[2 * j + li];
+    const f3 t = xyz(t4);
+    float c = dot3(t, dax) - K_ALPHA * t4.w;
+    float f = pav[li] * c + lav[li];
+    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+    angular(is_a ? cross3(axa, t) : neg(cross3(axb, t)), pav[li], f);
+  }
 }
 
 // solve_6x6 (assembly.rs:25-69) on the 21-entry lower triangle.
@@ -2532,27 +2647,72 @@ __global__ void k_inertial_blocks(Dev d) {
   if (h < d.n) store_inertial_block(d, h);
 }
 
+// Joint duals (joints_ext.rs:543-822): the linear rows accumulate
+// lam += K c and ramp K by BETA |c|; the angular rows the same with
+// BETA_ANG, the hinge rows committing lam = K c + lam.
+#define K_BETA_ANG 100.0f
 __global__ void k_dual_joint(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
+  const int kind = d.J.kind[j];
   uint2 ab = d.J.ab[j];
   const float4* sa = d.sb + 4 * (size_t)ab.x;
   const float4* sbb = d.sb + 4 * (size_t)ab.y;
-  f3 pa = add(xyz(sa[0]), qrot(asq(sa[1]), xyz(d.J.la[j])));
-  f3 pb = add(xyz(sbb[0]), qrot(asq(sbb[1]), xyz(d.J.lb[j])));
+  const qf qa = asq(sa[1]), qb = asq(sbb[1]);
+  f3 pa = add(xyz(sa[0]), qrot(qa, xyz(d.J.la[j])));
+  f3 pb = add(xyz(sbb[0]), qrot(qb, xyz(d.J.lb[j])));
   f3 live = sub(pa, pb);
   f3 c0v = xyz(d.J.c0v[j]);
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
   float lv[3] = {lam.x, lam.y, lam.z}, pv[3] = {pen.x, pen.y, pen.z};
-  for (int k = 0; k < 3; ++k) {
-    float c = comp(live, k) - K_ALPHA * comp(c0v, k);
-    if (fabsf(c) >= K_C_EPS) {
-      lv[k] += pv[k] * c;
-      pv[k] = fmin_(pv[k] + K_BETA * fabsf(c), K_PEN_MAX);
+  if (kind != JK_PRISMATIC) {
+    if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
+    for (int k = 0; k < 3; ++k) {
+      float c = comp(live, k) - K_ALPHA * comp(c0v, k);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[k] += pv[k] * c;
+        pv[k] = fmin_(pv[k] + K_BETA * fabsf(c), K_PEN_MAX);
+      }
+    }
+  } else {
+    const f3 wa = normalize_or(qrot(qa, xyz(d.J.ax[2 * j])), f3{0.0f, 0.0f, 1.0f});
+    f3 u, v;
+    tangent_basis(wa, &u, &v);
+    const f3 dl = sub(live, mul(c0v, K_ALPHA));
+    for (int li = 0; li < 2; ++li) {
+      float c = dot3(dl, li == 0 ? u : v);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[li] += pv[li] * c;
+        pv[li] = fmin_(pv[li] + K_BETA * fabsf(c), K_PEN_MAX);
+      }
     }
   }
   d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], 0.0f);
   d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+  if (kind == JK_BALL) return;
+  float4 la4 = d.J.lama[j], pa4 = d.J.pena[j];
+  float av[3] = {la4.x, la4.y, la4.z}, qv[3] = {pa4.x, pa4.y, pa4.z};
+  if (kind == JK_FIXED) {
+    const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]));
+    const f3 diff0 = xyz(d.J.a0[2 * j]);
+    for (int k = 0; k < 3; ++k) {
+      float c = comp(diff, k) - K_ALPHA * comp(diff0, k);
+      if (fabsf(c) < K_C_EPS) continue;
+      av[k] += qv[k] * c;
+      qv[k] = fmin_(qv[k] + K_BETA_ANG * fabsf(c), K_PEN_MAX);
+    }
+  } else {
+    const f3 dax = sub(qrot(qa, xyz(d.J.ax[2 * j])), qrot(qb, xyz(d.J.ax[2 * j + 1])));
+    for (int li = 0; li < 2; ++li) {
+      const float4 t4 = d.J.a0[2 * j + li];
+      float c = dot3(xyz(t4), dax) - K_ALPHA * t4.w;
+      if (fabsf(c) < K_C_EPS) continue;
+      av[li] = qv[li] * c + av[li];
+      qv[li] = fmin_(qv[li] + K_BETA_ANG * fabsf(c), K_PEN_MAX);
+    }
+  }
+  d.J.lama[j] = make_float4(av[0], av[1], av[2], 0.0f);
+  d.J.pena[j] = make_float4(qv[0], qv[1], qv[2], 0.0f);
 }
 
 // ---------------------------------------------------------------------------

@@ -133,14 +133,25 @@ struct Points {
   __host__ __device__ float4& pen(size_t q) const { return rec[4 * q + 3]; }  // penalties xyz
 };
 
+// Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
+enum { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3 };
+
 struct Joints {
   uint2* ab;
+  int* kind;         // JK_*
   float4* la;        // local anchor on A
   float4* lb;        // local anchor on B
-  float4* la0;       // la + 0 (the lever form the reference stamps with)
+  float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
+  float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames
+  float4* dref;      // Fixed: assembly anchor offset in A's frame
+  float4* qref;      // Fixed: assembly relative rotation conj(qa) qb
   float4* lam;       // lam_l xyz
   float4* pen;       // pen_l xyz
-  float4* c0v;       // step-start anchor separation
+  float4* lama;      // lam_a xyz
+  float4* pena;      // pen_a xyz
+  float4* c0v;       // per step: step-start anchor separation (Fixed: + rot0_a dref)
+  float4* a0;        // per step, 2 per joint: hinge rows t1 | t.(axa0 - axb0), t2 | ...
+                     // (tangent basis of the step-start axis); Fixed: step-start rotation error
 };
 
 // Broadphase grid + candidate scratch.

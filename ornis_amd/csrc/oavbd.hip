@@ -65,6 +65,11 @@ inline uint32_t prio_hash(uint32_t x) {
 struct HostJoint {
   uint32_t a, b;
   float la[3], lb[3];
+  int kind = 0;                      // JK_*
+  float axa[3] = {1.0f, 0.0f, 0.0f};  // unit hinge / slide axes (body frames)
+  float axb[3] = {1.0f, 0.0f, 0.0f};
+  float dref[3] = {0.0f, 0.0f, 0.0f};  // Fixed: assembly anchor offset in A's frame
+  float qref[4] = {0.0f, 0.0f, 0.0f, 1.0f};  // Fixed: assembly relative rotation
 };
 
 }  // namespace
@@ -120,7 +125,7 @@ struct Snap {
   float4 *qrec = nullptr, *qroll = nullptr;
   uint32_t* qpair = nullptr;
   int* qsrc = nullptr;
-  float4 *jlam = nullptr, *jpen = nullptr;
+  float4 *jlam = nullptr, *jpen = nullptr, *jlama = nullptr, *jpena = nullptr;
 };
 
 // The arrays oavbd_set_bodies was given, defaults filled in: bodies are
@@ -357,7 +362,8 @@ static void free_all(oavbd_ctx* c) {
   dfree(b.jp_off); dfree(b.jp); dfree(b.ji_off); dfree(b.ji);
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
-  dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
+  dfree(d.J.ab); dfree(d.J.kind); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.ax); dfree(d.J.dref);
+  dfree(d.J.qref); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.lama); dfree(d.J.pena); dfree(d.J.c0v); dfree(d.J.a0);
   dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
   dfree(d.g.is_large);
   Cands& C = d.C;
@@ -498,42 +504,68 @@ static void upload_joints(oavbd_ctx* c) {
   // joints already on the device keep their duals (a new joint does not
   // reset the others' warm starts: add_joint only pushes, mod.rs:1308-1510)
   const uint32_t kept = std::min(d.njoints, nj);
-  std::vector<float4> kept_lam(kept), kept_pen(kept);
+  std::vector<float4> kept_lam(kept), kept_pen(kept), kept_lama(kept), kept_pena(kept);
   if (kept) {
     HIPC(hipStreamSynchronize(c->st));
     HIPC(hipMemcpy(kept_lam.data(), d.J.lam, kept * sizeof(float4), hipMemcpyDeviceToHost));
     HIPC(hipMemcpy(kept_pen.data(), d.J.pen, kept * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(kept_lama.data(), d.J.lama, kept * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(kept_pena.data(), d.J.pena, kept * sizeof(float4), hipMemcpyDeviceToHost));
   }
-  dfree(d.J.ab); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.c0v);
+  Joints& J = d.J;
+  dfree(J.ab); dfree(J.kind); dfree(J.la); dfree(J.lb); dfree(J.la0); dfree(J.ax); dfree(J.dref); dfree(J.qref);
+  dfree(J.lam); dfree(J.pen); dfree(J.lama); dfree(J.pena); dfree(J.c0v); dfree(J.a0);
   dfree(d.b.jp); dfree(d.b.ji);
-  dalloc(&d.J.ab, nj); dalloc(&d.J.la, nj); dalloc(&d.J.lb, nj); dalloc(&d.J.la0, nj); dalloc(&d.J.lam, nj);
-  dalloc(&d.J.pen, nj); dalloc(&d.J.c0v, nj);
+  dalloc(&J.ab, nj); dalloc(&J.kind, nj); dalloc(&J.la, nj); dalloc(&J.lb, nj); dalloc(&J.la0, nj);
+  dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
+  dalloc(&J.lama, nj); dalloc(&J.pena, nj); dalloc(&J.c0v, nj); dalloc(&J.a0, 2 * (size_t)nj);
   std::vector<uint2> ab(nj);
-  std::vector<float4> la(nj), lb(nj), la0(nj), lam(nj), pen(nj);
+  std::vector<int> kind(nj);
+  std::vector<float4> la(nj), lb(nj), la0(nj), ax(2 * (size_t)nj), dref(nj), qref(nj), lam(nj), pen(nj), lama(nj),
+      pena(nj);
+  // no-collide partners: every kind but Fixed (rows.rs:437-450); the
+  // incidence lists (rows, colouring) hold every joint
   std::vector<uint32_t> jp_cnt(n + 1, 0), ji_cnt(n + 1, 0);
   for (uint32_t j = 0; j < nj; ++j) {
     const HostJoint& h = c->joints[j];
     ab[j] = make_uint2(h.a, h.b);
+    kind[j] = h.kind;
     la[j] = make_float4(h.la[0], h.la[1], h.la[2], 0.0f);
     lb[j] = make_float4(h.lb[0], h.lb[1], h.lb[2], 0.0f);
-    // the reference stamps A's lever as la + Vec3::ZERO (assembly.rs:297-304)
-    la0[j] = make_float4(h.la[0] + 0.0f, h.la[1] + 0.0f, h.la[2] + 0.0f, 0.0f);
+    // the reference stamps A's lever as la + Vec3::ZERO, or la + dref for a
+    // Fixed joint (assembly.rs:297-304)
+    const bool fixed = h.kind == JK_FIXED;
+    la0[j] = make_float4(h.la[0] + (fixed ? h.dref[0] : 0.0f), h.la[1] + (fixed ? h.dref[1] : 0.0f),
+                         h.la[2] + (fixed ? h.dref[2] : 0.0f), 0.0f);
+    ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], 0.0f);
+    ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], 0.0f);
+    dref[j] = make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    qref[j] = make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
     lam[j] = j < kept ? kept_lam[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    jp_cnt[h.a]++;
-    jp_cnt[h.b]++;
+    lama[j] = j < kept ? kept_lama[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    pena[j] = j < kept ? kept_pena[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    if (!fixed) {
+      jp_cnt[h.a]++;
+      jp_cnt[h.b]++;
+    }
     ji_cnt[h.a]++;
     ji_cnt[h.b]++;
   }
-  std::vector<uint32_t> jp_off(n + 1, 0);
-  for (uint32_t i = 0; i < n; ++i) jp_off[i + 1] = jp_off[i] + jp_cnt[i];
-  std::vector<uint32_t> jp(jp_off[n]), ji(jp_off[n]), cur(n, 0);
+  std::vector<uint32_t> jp_off(n + 1, 0), ji_off(n + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    jp_off[i + 1] = jp_off[i] + jp_cnt[i];
+    ji_off[i + 1] = ji_off[i] + ji_cnt[i];
+  }
+  std::vector<uint32_t> jp(jp_off[n]), ji(ji_off[n]), pcur(n, 0), icur(n, 0);
   for (uint32_t j = 0; j < nj; ++j) {  // joint order per body = ascending joint index
     const HostJoint& h = c->joints[j];
-    jp[jp_off[h.a] + cur[h.a]] = h.b;
-    ji[jp_off[h.a] + cur[h.a]++] = (j << 1) | 0u;
-    jp[jp_off[h.b] + cur[h.b]] = h.a;
-    ji[jp_off[h.b] + cur[h.b]++] = (j << 1) | 1u;
+    if (h.kind != JK_FIXED) {
+      jp[jp_off[h.a] + pcur[h.a]++] = h.b;
+      jp[jp_off[h.b] + pcur[h.b]++] = h.a;
+    }
+    ji[ji_off[h.a] + icur[h.a]++] = (j << 1) | 0u;
+    ji[ji_off[h.b] + icur[h.b]++] = (j << 1) | 1u;
   }
   // components of the graph of joints between two dynamic bodies
   std::vector<uint32_t> parent(n);
@@ -553,19 +585,12 @@ static void upload_joints(oavbd_ctx* c) {
   for (uint32_t i = 0; i < n; ++i) comp[i] = find(i);
   dalloc(&d.b.jp, jp.size());
   dalloc(&d.b.ji, ji.size());
-  HIPC(hipMemcpy(d.J.ab, ab.data(), nj * sizeof(uint2), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.J.la, la.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.J.lb, lb.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.J.la0, la0.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.J.lam, lam.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.J.pen, pen.data(), nj * sizeof(float4), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.b.jp_off, jp_off.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-  HIPC(hipMemcpy(d.b.ji_off, jp_off.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-  if (!jp.empty()) {
-    HIPC(hipMemcpy(d.b.jp, jp.data(), jp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(d.b.ji, ji.data(), ji.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  }
-  HIPC(hipMemcpy(d.b.comp, comp.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  auto up = [](auto* dst, const auto& v) {
+    if (!v.empty()) HIPC(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+  };
+  up(J.ab, ab); up(J.kind, kind); up(J.la, la); up(J.lb, lb); up(J.la0, la0); up(J.ax, ax); up(J.dref, dref);
+  up(J.qref, qref); up(J.lam, lam); up(J.pen, pen); up(J.lama, lama); up(J.pena, pena);
+  up(d.b.jp_off, jp_off); up(d.b.ji_off, ji_off); up(d.b.jp, jp); up(d.b.ji, ji); up(d.b.comp, comp);
   HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   d.njoints = nj;
   c->joints_dirty = false;
@@ -582,7 +607,7 @@ static void dcopy(oavbd_ctx* c, T* dst, const T* src, size_t n) {
 static void snap_free(Snap& s) {
   dfree(s.pos); dfree(s.rot); dfree(s.vel); dfree(s.angvel); dfree(s.torque); dfree(s.inertia); dfree(s.prev_pos);
   dfree(s.prev_rot); dfree(s.asleep); dfree(s.flag_a); dfree(s.timer); dfree(s.ab); dfree(s.prec); dfree(s.ppts);
-  dfree(s.poff); dfree(s.qrec); dfree(s.qroll); dfree(s.qpair); dfree(s.qsrc); dfree(s.jlam); dfree(s.jpen);
+  dfree(s.poff); dfree(s.qrec); dfree(s.qroll); dfree(s.qpair); dfree(s.qsrc); dfree(s.jlam); dfree(s.jpen); dfree(s.jlama); dfree(s.jpena);
   s.valid = false;
   s.cap_pairs = s.cap_pts = 0;
 }
@@ -594,6 +619,7 @@ static void snap_save(oavbd_ctx* c, Snap& s) {
     dalloc(&s.pos, n); dalloc(&s.rot, n); dalloc(&s.vel, n); dalloc(&s.angvel, n); dalloc(&s.torque, n);
     dalloc(&s.inertia, n); dalloc(&s.prev_pos, n); dalloc(&s.prev_rot, n); dalloc(&s.asleep, n); dalloc(&s.flag_a, n);
     dalloc(&s.timer, n); dalloc(&s.poff, n + 1); dalloc(&s.jlam, nj); dalloc(&s.jpen, nj);
+    dalloc(&s.jlama, nj); dalloc(&s.jpena, nj);
   }
   if (d.npairs > s.cap_pairs) {
     dfree(s.ab); dfree(s.prec); dfree(s.ppts);
@@ -614,6 +640,7 @@ static void snap_save(oavbd_ctx* c, Snap& s) {
   dcopy(c, s.qrec, d.Q.rec, 4 * (size_t)d.npts); dcopy(c, s.qroll, d.Q.roll, d.npts); dcopy(c, s.qpair, d.Q.pair, d.npts);
   dcopy(c, s.qsrc, d.Q.src, d.npts);
   dcopy(c, s.jlam, d.J.lam, nj); dcopy(c, s.jpen, d.J.pen, nj);
+  dcopy(c, s.jlama, d.J.lama, nj); dcopy(c, s.jpena, d.J.pena, nj);
   s.npairs = d.npairs;
   s.npts = d.npts;
   s.time_debt = c->time_debt;
@@ -636,6 +663,7 @@ static void snap_load(oavbd_ctx* c, const Snap& s) {
   dcopy(c, d.Q.rec, s.qrec, 4 * (size_t)s.npts); dcopy(c, d.Q.roll, s.qroll, s.npts); dcopy(c, d.Q.pair, s.qpair, s.npts);
   dcopy(c, d.Q.src, s.qsrc, s.npts);
   dcopy(c, d.J.lam, s.jlam, nj); dcopy(c, d.J.pen, s.jpen, nj);
+  dcopy(c, d.J.lama, s.jlama, nj); dcopy(c, d.J.pena, s.jpena, nj);
   d.npairs = s.npairs;
   d.npts = s.npts;
   c->time_debt = s.time_debt;
@@ -1519,7 +1547,7 @@ struct HostBodies {
   std::vector<float> timer;
   std::vector<uint32_t> toi_tgt;
   std::vector<int32_t> colour;
-  std::vector<float4> jlam, jpen;
+  std::vector<float4> jlam, jpen, jlama, jpena;
 };
 
 static HostContacts download_contacts(oavbd_ctx* c) {
@@ -1543,6 +1571,8 @@ static HostBodies download_bodies(oavbd_ctx* c) {
   d2h(h.timer, d.b.timer, n); d2h(h.toi_tgt, d.toi_tgt, n); d2h(h.colour, d.b.colour, n);
   d2h(h.jlam, d.J.lam, d.njoints);
   d2h(h.jpen, d.J.pen, d.njoints);
+  d2h(h.jlama, d.J.lama, d.njoints);
+  d2h(h.jpena, d.J.pena, d.njoints);
   return h;
 }
 
@@ -1556,6 +1586,8 @@ static void upload_bodies(oavbd_ctx* c, const HostBodies& h) {
   h2d(d.b.timer, h.timer, m); h2d(d.toi_tgt, h.toi_tgt, m); h2d(d.b.colour, h.colour, m);
   h2d(d.J.lam, h.jlam, std::min<size_t>(h.jlam.size(), d.njoints));
   h2d(d.J.pen, h.jpen, std::min<size_t>(h.jpen.size(), d.njoints));
+  h2d(d.J.lama, h.jlama, std::min<size_t>(h.jlama.size(), d.njoints));
+  h2d(d.J.pena, h.jpena, std::min<size_t>(h.jpena.size(), d.njoints));
   HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   uint32_t sleeping = 0;
   for (uint8_t a : h.asleep) sleeping += a ? 1u : 0u;
@@ -1702,7 +1734,7 @@ int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
       }
     }
     std::vector<HostJoint> joints;
-    std::vector<float4> jlam, jpen;
+    std::vector<float4> jlam, jpen, jlama, jpena;
     for (size_t j = 0; j < c->joints.size(); ++j) {
       HostJoint J = c->joints[j];
       if (J.a == h || J.b == h) continue;
@@ -1711,9 +1743,13 @@ int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
       joints.push_back(J);
       jlam.push_back(hb.jlam[j]);
       jpen.push_back(hb.jpen[j]);
+      jlama.push_back(hb.jlama[j]);
+      jpena.push_back(hb.jpena[j]);
     }
     hb.jlam = std::move(jlam);
     hb.jpen = std::move(jpen);
+    hb.jlama = std::move(jlama);
+    hb.jpena = std::move(jpena);
     rebuild(c, std::move(in), hb, std::move(joints), hc);
     // every sleeping body wakes (wake_body through the device's own path)
     Dev& d = c->d;
@@ -1838,27 +1874,63 @@ int oavbd_set_sleep(oavbd_ctx* c, const uint8_t* asleep, const float* timer, uin
   return OAVBD_OK;
 }
 
-int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
-                         uint32_t* out_index) {
+int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                    const float axis_a[3], const float axis_b[3], uint32_t* out_index) {
   if (!c || !la || !lb) return OAVBD_EINVAL;
-  if (a == b || a >= c->d.n || b >= c->d.n) return fail(c, OAVBD_EINVAL, "bad joint bodies");
-  if (c->joints.size() >= c->max_joints) return fail(c, OAVBD_ECAPACITY, "more joints than max_joints");
-  HostJoint j{a, b, {la[0], la[1], la[2]}, {lb[0], lb[1], lb[2]}};
-  c->joints.push_back(j);
-  c->joints_dirty = true;
-  // the reference wakes both members on add_joint (mod.rs:1317-1318)
+  const uint32_t n = c->d.n;
+  // the reference wakes both members before it validates the spec
+  // (mod.rs:1317-1318)
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
-    uint8_t one = 1;
     HIPC(hipStreamSynchronize(c->st));
-    HIPC(hipMemcpy(c->d.b.flag_a + a, &one, 1, hipMemcpyHostToDevice));
-    HIPC(hipMemcpy(c->d.b.flag_a + b, &one, 1, hipMemcpyHostToDevice));
+    uint8_t one = 1;
+    if (a < n) HIPC(hipMemcpy(c->d.b.flag_a + a, &one, 1, hipMemcpyHostToDevice));
+    if (b < n) HIPC(hipMemcpy(c->d.b.flag_a + b, &one, 1, hipMemcpyHostToDevice));
     HIPC(hipDeviceSynchronize());
-    k_apply_wake<<<blocks(c->d.n), TPB, 0, c->st>>>(c->d);
+    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(c->d);
     HIPC(hipStreamSynchronize(c->st));
   });
+  if (a == b || a >= n || b >= n) return fail(c, OAVBD_EINVAL, "bad joint bodies");
+  if (kind > JK_FIXED) return fail(c, OAVBD_EINVAL, "unsupported joint kind");
+  if (c->joints.size() >= c->max_joints) return fail(c, OAVBD_ECAPACITY, "more joints than max_joints");
+  HostJoint j{a, b, {la[0], la[1], la[2]}, {lb[0], lb[1], lb[2]}};
+  j.kind = (int)kind;
+  if (kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
+    // resolve_joint (joint.rs:1280-1312): a degenerate axis rejects
+    if (!axis_a || !axis_b) return OAVBD_EINVAL;
+    const f3 xa{axis_a[0], axis_a[1], axis_a[2]}, xb{axis_b[0], axis_b[1], axis_b[2]};
+    if (dot3(xa, xa) < K_DEGENERATE || dot3(xb, xb) < K_DEGENERATE)
+      return fail(c, OAVBD_EINVAL, "degenerate hinge / slide axis");
+    const f3 na = normalize_or(xa, f3{0.0f, 0.0f, 1.0f}), nb = normalize_or(xb, f3{0.0f, 0.0f, 1.0f});
+    j.axa[0] = na.x; j.axa[1] = na.y; j.axa[2] = na.z;
+    j.axb[0] = nb.x; j.axb[1] = nb.y; j.axb[2] = nb.z;
+  }
+  if (kind == JK_FIXED) {
+    // resolve_joint (joint.rs:1313-1323): the assembly rotation and anchor
+    // offset from the bodies' current poses
+    float4 p[2], q[2];
+    GUARD(c, {
+      HIPC(hipMemcpy(&p[0], c->d.b.pos + a, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&p[1], c->d.b.pos + b, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&q[0], c->d.b.rot + a, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&q[1], c->d.b.rot + b, sizeof(float4), hipMemcpyDeviceToHost));
+    });
+    const qf qa = asq(q[0]), qb = asq(q[1]);
+    const qf qr = qmul(qconj(qa), qb);
+    const f3 ra = qrot(qa, f3{la[0], la[1], la[2]}), rb = qrot(qb, f3{lb[0], lb[1], lb[2]});
+    const f3 dr = qrot(qconj(qa), sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
+    j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
+    j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
+  }
+  c->joints.push_back(j);
+  c->joints_dirty = true;
   if (out_index) *out_index = (uint32_t)c->joints.size() - 1;
   return OAVBD_OK;
+}
+
+int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                         uint32_t* out_index) {
+  return oavbd_add_joint(c, JK_BALL, a, b, la, lb, nullptr, nullptr, out_index);
 }
 
 int oavbd_step_n(oavbd_ctx* c, uint32_t nsteps) {
@@ -2012,19 +2084,23 @@ int oavbd_get_toi(oavbd_ctx* c, float* t, int32_t* target, uint32_t n) {
   return OAVBD_OK;
 }
 
-int oavbd_get_joints(oavbd_ctx* c, float* out6, uint32_t n) {
-  if (!c || !out6 || n > c->d.njoints) return OAVBD_EINVAL;
+int oavbd_get_joints(oavbd_ctx* c, float* out12, uint32_t n) {
+  if (!c || !out12 || n > c->d.njoints) return OAVBD_EINVAL;
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
-    std::vector<float4> lam(n), pen(n);
-    if (n) {
-      HIPC(hipMemcpy(lam.data(), c->d.J.lam, n * sizeof(float4), hipMemcpyDeviceToHost));
-      HIPC(hipMemcpy(pen.data(), c->d.J.pen, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPC(hipStreamSynchronize(c->st));
+    std::vector<float4> v[4];
+    const float4* src[4] = {c->d.J.lam, c->d.J.pen, c->d.J.lama, c->d.J.pena};
+    for (int k = 0; k < 4; ++k) {
+      v[k].resize(n);
+      if (n) HIPC(hipMemcpy(v[k].data(), src[k], n * sizeof(float4), hipMemcpyDeviceToHost));
     }
-    for (uint32_t i = 0; i < n; ++i) {
-      out6[6 * i] = lam[i].x; out6[6 * i + 1] = lam[i].y; out6[6 * i + 2] = lam[i].z;
-      out6[6 * i + 3] = pen[i].x; out6[6 * i + 4] = pen[i].y; out6[6 * i + 5] = pen[i].z;
-    }
+    for (uint32_t i = 0; i < n; ++i)
+      for (int k = 0; k < 4; ++k) {
+        out12[12 * i + 3 * k] = v[k][i].x;
+        out12[12 * i + 3 * k + 1] = v[k][i].y;
+        out12[12 * i + 3 * k + 2] = v[k][i].z;
+      }
   });
   return OAVBD_OK;
 }

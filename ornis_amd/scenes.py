@@ -25,6 +25,10 @@ BODY_KINEMATIC = 2
 _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 
 
+# joint kinds (include/oavbd.h)
+JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
+
+
 class SceneBuilder:
     """Accumulates bodies and ball joints; `build()` returns numpy arrays."""
 
@@ -60,6 +64,13 @@ class SceneBuilder:
 
     def ball_joint(self, a, b, la, lb):
         self.joints.append((int(a), int(b), tuple(la), tuple(lb)))
+        return len(self.joints) - 1
+
+    def joint(self, kind, a, b, la, lb, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
+        """A joint of kind JOINT_BALL / JOINT_REVOLUTE / JOINT_PRISMATIC /
+        JOINT_FIXED between bodies a and b: local anchors la, lb and (hinge /
+        slide kinds) local axes."""
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb), int(kind), tuple(axis_a), tuple(axis_b)))
         return len(self.joints) - 1
 
     def build(self):

@@ -1,0 +1,131 @@
+"""Revolute, Prismatic and Fixed joints on the GPU against the oracle
+(float32, colour order), bit for bit: bodies, pairs, colourings and the
+joints' linear and angular duals.  The scenes are the reference's own joint
+tests (crates/physics/tests/avbd_engine.rs: hinge :139-183, weld :891-935,
+slider :598-646 without its limit) and chains whose links cycle through
+the four kinds with tilted axes, falling into a field of spheres and onto
+each other, with a body removed mid-run."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from ornis_amd import scenes
+from ornis_amd.avbd import AvbdEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(g, o, nj, what):
+    sg, so = g.state(), o.state()
+    bad = np.where((sg.view(np.uint32) != so.view(np.uint32)).any(axis=1))[0]
+    assert bad.size == 0, f"{what}: {bad.size} bodies differ (first {bad[:5]}), max |d| {np.abs(sg - so).max():.3e}"
+    assert np.array_equal(g.pairs(), o.pairs()[0]), f"{what}: pair lists differ"
+    assert np.array_equal(g.colours(), o.colours()), f"{what}: colourings differ"
+    jg, jo = g.joints(), o.joints(nj)
+    assert np.array_equal(jg.view(np.uint32), jo.view(np.uint32)), f"{what}: joint duals differ"
+
+
+def _run(scene, steps, every=10):
+    g = AvbdEngine(scene=scene)
+    o = po.OracleEngine(scene, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+    nj = len(scene["joints"])
+    for s in range(steps):
+        g.step_n(1)
+        o.step_inner(1)
+        if s % every == every - 1:
+            _same(g, o, nj, f"step {s}")
+    return g, o
+
+
+def test_revolute_hinge_bitwise(gpu_available):
+    b = scenes.SceneBuilder()
+    a = b.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 0.0)
+    arm = b.box((0.3, -1.0, 0.0), (0.2, 1.0, 0.2), 1.0)
+    b.joint(scenes.JOINT_REVOLUTE, a, arm, (0, 0, 0), (-0.3, 1.0, 0.0), (0, 0, 1), (0, 0, 1))
+    g, _ = _run(b.build(), 300, every=20)
+    st = g.state()
+    assert np.abs(g.joints()[0, 6:9]).max() > 0 or np.abs(g.joints()[0, 0:3]).max() > 0
+
+
+def test_fixed_weld_drop_bitwise(gpu_available):
+    b = scenes.SceneBuilder()
+    b.box((0.0, -1.0, 0.0), (5.0, 1.0, 5.0), 0.0)
+    a = b.box((-0.5, 3.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    c = b.box((0.5, 3.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    b.joint(scenes.JOINT_FIXED, a, c, (0.5, 0, 0), (-0.5, 0, 0))
+    g, _ = _run(b.build(), 300, every=20)
+    st = g.state()
+    assert abs(np.linalg.norm(st[c, :3] - st[a, :3]) - 1.0) < 0.15 and st[a, 1] > 0.2
+
+
+def test_prismatic_slide_bitwise(gpu_available):
+    b = scenes.SceneBuilder()
+    anchor = b.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 0.0)
+    bob = b.sphere((0.0, -2.9, 0.0), 0.25, 1.0, vel=(0.3, 0.0, -0.2))
+    b.joint(scenes.JOINT_PRISMATIC, anchor, bob, (0, 0, 0), (0, 1, 0), (0, 1, 0), (0, 1, 0))
+    g, _ = _run(b.build(), 120, every=10)
+    p = g.state()[bob]
+    assert abs(p[0]) < 0.08 and abs(p[2]) < 0.08
+
+
+def mixed_chains(n_chains=8, links=8, seed=11):
+    """Chains of boxes whose consecutive links are joined by Ball,
+    Revolute, Prismatic and Fixed joints in turn (hinge / slide axes tilted
+    at random), the first link of every other chain pinned to a static
+    anchor, over a floor with static spheres."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = scenes.SceneBuilder()
+    b.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=0.5, restitution=0.0)
+    for i in range(12):
+        b.sphere((rng.uniform(-4, 4), 0.3, rng.uniform(-4, 4)), 0.3, 0.0, friction=0.5, restitution=0.0)
+    hy = 0.2
+    for c in range(n_chains):
+        x0, z0 = -3.5 + c, rng.uniform(-1.0, 1.0)
+        prev = None
+        if c % 2 == 0:
+            prev = b.box((x0, 6.0 + hy, z0), (0.1, 0.1, 0.1), 0.0)
+        for k in range(links):
+            q = rng.normal(size=4) * np.array([0.1, 0.1, 0.1, 1.0])
+            q = (q / np.linalg.norm(q)).astype(np.float32)
+            h = b.box((x0, 6.0 - 2 * hy * k, z0), (0.08, hy, 0.08), float(rng.uniform(0.5, 2.0)),
+                      quat=tuple(q.tolist()), friction=0.5, restitution=0.0)
+            if prev is not None:
+                kind = (c + k) % 4
+                axis = rng.normal(size=3)
+                axis = tuple((axis / np.linalg.norm(axis)).tolist())
+                b.joint(kind, prev, h, (0.0, -hy, 0.0), (0.0, hy, 0.0), axis, axis)
+            prev = h
+    return b.build()
+
+
+def test_mixed_joint_chains_bitwise(gpu_available):
+    s = mixed_chains()
+    kinds = [j[4] for j in s["joints"]]
+    assert set(kinds) == {0, 1, 2, 3}
+    g, o = _run(s, 150, every=10)
+    # a link in the middle of a chain goes: its joints go, the rest renumber
+    victim = int(s["joints"][len(s["joints"]) // 2][1])
+    g.remove_body(victim)
+    o.remove_body(victim)
+    for s_ in range(60):
+        g.step_n(1)
+        o.step_inner(1)
+    _same(g, o, g.njoints, "after a removal")
+    assert g.njoints < len(s["joints"])
+    assert np.isfinite(g.state()).all()
+
+
+def test_joint_spec_validation(gpu_available):
+    b = scenes.SceneBuilder()
+    b.box((0.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    b.box((1.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    g = AvbdEngine(scene=b.build())
+    from ornis_amd.avbd import AvbdError
+    for args in ((0, 0, (0, 0, 0), (0, 0, 0), 1, (0, 0, 1), (0, 0, 1)),       # self joint
+                 (0, 5, (0, 0, 0), (0, 0, 0), 0),                          # body out of range
+                 (0, 1, (0, 0, 0), (0, 0, 0), 7),                          # unknown kind
+                 (0, 1, (0, 0, 0), (0, 0, 0), 1, (0, 0, 0), (0, 0, 1)),    # degenerate hinge axis
+                 (0, 1, (0, 0, 0), (0, 0, 0), 2, (0, 0, 1), (1e-7, 0, 0))):  # degenerate slide axis
+        with pytest.raises(AvbdError):
+            g.add_joint(*args)
+    assert g.stats()["joints"] == 0
