@@ -191,19 +191,19 @@ def differentiable_leg(steps, reps, cpu_scenes, threads):
     return out
 
 
-def measured_traffic(window):
-    """HBM bytes per k_primal launch counted by rocprofv3 (FETCH_SIZE +
-    WRITE_SIZE, each scaled by the counter's reading for a plain copy of
-    known size) over the same steps of this workload, from the committed
-    summary profiles/k_primal_traffic_100k.json; None when no summary
-    covers `window` (first-last step)."""
+def measured_traffic(window, kernel="k_primal8"):
+    """HBM bytes per launch of `kernel` (k_primal8 or k_dual) counted by
+    rocprofv3 (FETCH_SIZE + WRITE_SIZE, each scaled by the counter's reading
+    for a plain copy of known size) over the same steps of this workload,
+    from the committed summary profiles/k_primal_traffic_100k.json; None
+    when no summary covers `window` (first-last step)."""
     path = os.path.join(ROOT, "profiles", "k_primal_traffic_100k.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         j = json.load(f)
     w = j.get("windows", {}).get(window)
-    return None if w is None else w.get("hbm_bytes_per_launch")
+    return None if w is None else w.get(kernel, {}).get("hbm_bytes_per_launch")
 
 
 def trajectory_leg(scene, n_dyn, steps=300, windows=100, late=50, sample_every=5, capture_step=280):
@@ -434,7 +434,10 @@ def main():
         "cpu_baseline": None,
         "phases_ms_per_step": {"primal": kt["primal_ms"] / max(kt["steps"], 1),
                                "dual": kt["dual_ms"] / max(kt["steps"], 1)},
-        "dual_roofline": {"achieved": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS},
+        "dual_roofline": {"achieved": dual_gbs, "frac": dual_gbs / HBM_PEAK_GBS, "kernel": "k_dual",
+                          "bytes_per_launch": kt["dual_bytes"] / max(kt["dual_launches"], 1),
+                          "avg_launch_us": 1e3 * kt["dual_ms"] / max(kt["dual_launches"], 1),
+                          "traffic": measured_traffic(window, "k_dual")},
         "final_stats": st,
         "primal_table": None if table is None else [int(x) for x in table],
         "primal_tune_us": None if table is None else

@@ -156,9 +156,9 @@ typedef struct oavbd_contact_pair {
 } oavbd_contact_pair;        /* 80 bytes */
 typedef struct oavbd_contact_point {
   float ra[4];               /* xyz: anchor on A, body frame; w: step-start violation */
-  float rb[4];               /* xyz: anchor on B, body frame; w: stuck (0 or 1) */
+  float rb[4];               /* xyz: anchor on B, body frame */
   float lam[4];              /* xyz: normal and two friction duals */
-  float pen[4];              /* xyz: their penalties */
+  float pen[4];              /* xyz: their penalties; w: stuck (0 or 1) */
   float roll[4];             /* xyz: rolling and torsion duals */
 } oavbd_contact_point;       /* 80 bytes */
 

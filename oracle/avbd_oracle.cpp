@@ -2562,7 +2562,7 @@ void orc_remove_body(void* h, int hi) {
 // Replace the pair table (the warm-start state carried between steps):
 // pair i is (ab[2i], ab[2i+1]) with normal / gap n4[4i..], friction
 // mu2[2i..] and points pts2[2i] .. pts2[2i] + pts2[2i+1] - 1 of `point20`,
-// 20 floats each: ra3 _ rb3 stuck lam3 _ pen3 _ roll3 _.
+// 20 floats each: ra3 _ rb3 _ lam3 _ pen3 stuck roll3 _.
 void orc_set_contacts(void* h, int npairs, const uint32_t* ab, const float* n4, const float* mu2,
                       const uint32_t* pts2, const float* point20) {
   OracleCtx* c = (OracleCtx*)h;
@@ -2582,7 +2582,7 @@ void orc_set_contacts(void* h, int npairs, const uint32_t* ab, const float* n4, 
         Point<TT> q;
         q.ra = {TT(r[0]), TT(r[1]), TT(r[2])};
         q.rb = {TT(r[4]), TT(r[5]), TT(r[6])};
-        q.stuck = r[7] != 0.0f;
+        q.stuck = r[15] != 0.0f;
         for (int j = 0; j < 3; ++j) {
           q.lam[j] = TT(r[8 + j]);
           q.pen[j] = TT(r[12 + j]);

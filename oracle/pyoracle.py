@@ -202,8 +202,8 @@ class OracleEngine:
     def set_contacts(self, pairs, points):
         """Replace the pair table by the library's export form
         (oavbd_get_contacts): structured arrays with fields a, b,
-        first_point, npoints, normal (xyz + gap), friction, and ra, rb
-        (w: stuck), lam, pen, roll."""
+        first_point, npoints, normal (xyz + gap), friction, and ra, rb, lam,
+        pen (w: stuck), roll."""
         ab = np.ascontiguousarray(np.stack([pairs["a"], pairs["b"]], axis=1), dtype=np.uint32)
         n4 = np.ascontiguousarray(pairs["normal"], dtype=np.float32)
         mu2 = np.ascontiguousarray(pairs["friction"][:, 0:2], dtype=np.float32)

@@ -1086,25 +1086,25 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d) {
   uint32_t q = start + k;
   if (m >= 0) {
     float4 ora = d.Qold.ra(m), orb = d.Qold.rb(m);
-    bool stuck = orb.w != 0.0f;
+    float4 l = d.Qold.lam(m), pe = d.Qold.pen(m), ro = d.Qold.roll[m];
+    bool stuck = pe.w != 0.0f;
     bool keep = stuck && touching && !smooth;
     f3 ra = keep ? xyz(ora) : ral;
     f3 rb = keep ? xyz(orb) : rbl;
-    float4 l = d.Qold.lam(m), pe = d.Qold.pen(m), ro = d.Qold.roll[m];
     d.Q.ra(q) = pk(ra, 0.0f);
-    d.Q.rb(q) = pk(rb, stuck ? 1.0f : 0.0f);
+    d.Q.rb(q) = pk(rb, 0.0f);
     d.Q.lam(q) = make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
     d.Q.pen(q) = make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
-                             clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), 0.0f);
+                             clampf_(pe.z * K_GAMMA, K_PEN_MIN, K_PEN_MAX), stuck ? 1.0f : 0.0f);
     d.Q.roll[q] = make_float4(ro.x * AG, ro.y * AG, ro.z * AG, 0.0f);
     d.Q.src[q] = m;
   } else {
     float z = 0.0f * AG;
     float p1 = clampf_(K_PEN_INIT * K_GAMMA, K_PEN_MIN, K_PEN_MAX);
     d.Q.ra(q) = pk(ral, 0.0f);
-    d.Q.rb(q) = pk(rbl, 1.0f);
+    d.Q.rb(q) = pk(rbl, 0.0f);
     d.Q.lam(q) = make_float4(z, z, z, 0.0f);
-    d.Q.pen(q) = make_float4(p1, p1, p1, 0.0f);
+    d.Q.pen(q) = make_float4(p1, p1, p1, 1.0f);
     d.Q.roll[q] = make_float4(z, z, z, 0.0f);
     d.Q.src[q] = -1;
   }
@@ -2576,11 +2576,12 @@ __device__ __forceinline__ void dual_point(const Dev& d, uint32_t q) {
     f1 = r0 * k;
     f2 = r1 * k;
   }
+  float npn_w = 0.0f;
   {
     float bnd = fabsf(fn_c) * fmax_(mu.x, mu.y);
     float fs = sqrtf(r0 * r0 + r1 * r1);
     bool stuck = fs <= bnd && sqrtf(ct1 * ct1 + ct2 * ct2) < K_STICK;
-    rb4.w = stuck ? 1.0f : 0.0f;
+    npn_w = stuck ? 1.0f : 0.0f;
   }
   float4 nl = lam, npn = pen;
   if (touching && fabsf(cn) >= K_C_EPS) {
@@ -2597,10 +2598,9 @@ __device__ __forceinline__ void dual_point(const Dev& d, uint32_t q) {
       if (fabsf(ct2) >= K_C_EPS) npn.z = fmin_(pen.z + K_BETA * fabsf(ct2), K_PEN_MAX);
     }
   }
-  // the whole 64-byte record is rewritten (ra unchanged), so no partial
-  // line is left for a read-modify-write
-  d.Q.ra(q) = ra4;
-  d.Q.rb(q) = rb4;
+  // the duals and penalties (with the stuck flag in pen.w) fill the
+  // record's second 32-byte half; the anchors are not rewritten
+  npn.w = npn_w;
   d.Q.lam(q) = nl;
   d.Q.pen(q) = npn;
   if (touching && (mu.z > 0.0f || mu.w > 0.0f)) {

@@ -127,10 +127,12 @@ struct Points {
   int* src;          // previous step's point this one inherits its duals from (-1: fresh)
   // anchor on A (body frame), w = step-start violation c0
   __host__ __device__ float4& ra(size_t q) const { return rec[4 * q]; }
-  // anchor on B (body frame), w = stuck (0/1)
+  // anchor on B (body frame)
   __host__ __device__ float4& rb(size_t q) const { return rec[4 * q + 1]; }
   __host__ __device__ float4& lam(size_t q) const { return rec[4 * q + 2]; }  // duals xyz
-  __host__ __device__ float4& pen(size_t q) const { return rec[4 * q + 3]; }  // penalties xyz
+  // penalties xyz, w = stuck (0/1): the dual writes lam and pen, one
+  // 32-byte half of the record
+  __host__ __device__ float4& pen(size_t q) const { return rec[4 * q + 3]; }
 };
 
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).

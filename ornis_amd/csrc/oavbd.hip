@@ -7,6 +7,7 @@
 // size the next launches (candidate pairs, pairs/points, colour classes).
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -907,28 +908,41 @@ static void colour_wait(oavbd_ctx* c) {
   }
 }
 
+// A kernel launch on the context stream; with events (timing), the start
+// and stop events ride on the dispatch itself (hipExtLaunchKernelGGL), so
+// they bracket the kernel's execution and not its dispatch.
+template <class K, class... A>
+static void launch_k(oavbd_ctx* c, K kernel, size_t grid, int threads, hipEvent_t e0, hipEvent_t e1, A... args) {
+  if (e0 || e1) {
+    hipExtLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(threads), 0, c->st, e0, e1, 0, args...);
+    HIPC(hipGetLastError());
+  } else {
+    kernel<<<grid, threads, 0, c->st>>>(args...);
+  }
+}
+
 template <int W>
-static void launch_w(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+static void launch_w(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
   constexpr int NB = (WAVE / BODY_LANES) * W;
   if (c->rows6)
-    k_primal8<W, 6><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal8<W, 6>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
   else
-    k_primal8<W, 4><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal8<W, 4>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
 }
 template <int W>
-static void launch_pf(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+static void launch_pf(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
   constexpr int NB = (WAVE / BODY_LANES) * W;
   if (c->rows6)
-    k_primal8<W, 6, 1><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal8<W, 6, 1>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
   else
-    k_primal8<W, 4, 1><<<blocks(cnt, NB), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal8<W, 4, 1>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
 }
 template <int W>
-static void launch_lds(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+static void launch_lds(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
   if (c->rows6)
-    k_primal_lds<W, 6><<<blocks(cnt, W), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal_lds<W, 6>, blocks(cnt, W), WAVE * W, e0, e1, c->d, list, cnt);
   else
-    k_primal_lds<W, 4><<<blocks(cnt, W), WAVE * W, 0, c->st>>>(c->d, list, cnt);
+    launch_k(c, k_primal_lds<W, 4>, blocks(cnt, W), WAVE * W, e0, e1, c->d, list, cnt);
 }
 
 // Primal launch of one colour class.  The launch shapes (variants): sixteen
@@ -948,21 +962,22 @@ static int size_bucket(uint32_t cnt) {
   while ((cnt >> (b + 1)) != 0u && b + 1 < PRIMAL_BUCKETS) ++b;
   return b;
 }
-static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt) {
+static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr) {
   int v = c->force_variant;
   if (v < 0) v = c->ptable[size_bucket(cnt)];
   if (v < 0) v = cnt < PRIMAL_TAIL ? PV_W1 : (c->waves == 1 ? PV_W1 : c->waves == 2 ? PV_W2 : c->waves == 8 ? PV_W8 : PV_W4);
   switch (v) {
-    case PV_W1: launch_w<1>(c, list, cnt); break;
-    case PV_W2: launch_w<2>(c, list, cnt); break;
-    case PV_W4: launch_w<4>(c, list, cnt); break;
-    case PV_W8: launch_w<8>(c, list, cnt); break;
-    case PV_L1: launch_lds<1>(c, list, cnt); break;
-    case PV_L2: launch_lds<2>(c, list, cnt); break;
-    case PV_P1: launch_pf<1>(c, list, cnt); break;
-    case PV_P2: launch_pf<2>(c, list, cnt); break;
-    case PV_P4: launch_pf<4>(c, list, cnt); break;
-    default: launch_lds<4>(c, list, cnt); v = PV_L4; break;
+    case PV_W1: launch_w<1>(c, list, cnt, e0, e1); break;
+    case PV_W2: launch_w<2>(c, list, cnt, e0, e1); break;
+    case PV_W4: launch_w<4>(c, list, cnt, e0, e1); break;
+    case PV_W8: launch_w<8>(c, list, cnt, e0, e1); break;
+    case PV_L1: launch_lds<1>(c, list, cnt, e0, e1); break;
+    case PV_L2: launch_lds<2>(c, list, cnt, e0, e1); break;
+    case PV_P1: launch_pf<1>(c, list, cnt, e0, e1); break;
+    case PV_P2: launch_pf<2>(c, list, cnt, e0, e1); break;
+    case PV_P4: launch_pf<4>(c, list, cnt, e0, e1); break;
+    default: launch_lds<4>(c, list, cnt, e0, e1); v = PV_L4; break;
   }
   return v;
 }
@@ -1049,22 +1064,29 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     for (uint32_t k = 0; k < ncol; ++k) {
       uint32_t cnt = c->colour_cnt_h[k];
       if (!cnt) continue;
-      if (c->timing) HIPC(hipEventRecord(pool_event(c->pev, np_ev), c->st));
-      const int v = launch_primal(c, d.colour_list + c->colour_off_h[k], cnt);
+      hipEvent_t e0 = c->timing ? pool_event(c->pev, np_ev) : nullptr;
+      hipEvent_t e1 = c->timing ? pool_event(c->pev, np_ev + 1) : nullptr;
+      const int v = launch_primal(c, d.colour_list + c->colour_off_h[k], cnt, e0, e1);
       if (c->timing) {
-        HIPC(hipEventRecord(pool_event(c->pev, np_ev + 1), c->st));
         if (c->plog.size() <= np_ev / 2) c->plog.resize(np_ev / 2 + 1);
         c->plog[np_ev / 2] = {cnt, v};
       }
       np_ev += 2;
     }
     if (c->rec) tape_iter_end(c, it);
-    if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev), c->st));
-    // the dual update, and the next sweep's inertial blocks
+    // the dual update, and the next sweep's inertial blocks (timed from the
+    // first dual kernel's start to the last one's end)
     const uint32_t next = it + 1 < c->prm.iters ? 1u : 0u;
-    if (d.npts || next) k_dual<<<blocks((size_t)d.npts + (next ? n : 0u)), TPB, 0, c->st>>>(d, next);
-    if (d.njoints) k_dual_joint<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-    if (c->timing) HIPC(hipEventRecord(pool_event(c->dev_ev, nd_ev + 1), c->st));
+    const bool dual_pts = d.npts || next, dual_j = d.njoints != 0;
+    hipEvent_t de0 = c->timing ? pool_event(c->dev_ev, nd_ev) : nullptr;
+    hipEvent_t de1 = c->timing ? pool_event(c->dev_ev, nd_ev + 1) : nullptr;
+    if (dual_pts)
+      launch_k(c, k_dual, blocks((size_t)d.npts + (next ? n : 0u)), TPB, de0, dual_j ? nullptr : de1, d, next);
+    if (dual_j) launch_k(c, k_dual_joint, blocks(d.njoints), TPB, dual_pts ? nullptr : de0, de1, d);
+    if (c->timing && !dual_pts && !dual_j) {
+      HIPC(hipEventRecord(de0, c->st));
+      HIPC(hipEventRecord(de1, c->st));
+    }
     nd_ev += 2;
   }
   record(c, 3);
@@ -2040,7 +2062,7 @@ int oavbd_get_points(oavbd_ctx* c, float* out16, uint32_t cap, uint32_t* n) {
       o[6] = lam.x; o[7] = lam.y; o[8] = lam.z;
       o[9] = pen.x; o[10] = pen.y; o[11] = pen.z;
       o[12] = roll[i].x; o[13] = roll[i].y; o[14] = roll[i].z;
-      o[15] = rb.w;
+      o[15] = pen.w;
     }
   });
   return OAVBD_OK;
