@@ -30,6 +30,17 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "body-steps/s @100k boxes (10 AVBD iters) + % HBM roofline; batched scenes/s"
 
 
+TAIL = (-1, 0)  # fused primal tail setting for every engine (-1: library default)
+
+
+def new_engine(**kw):
+    from ornis_amd.avbd import AvbdEngine
+    eng = AvbdEngine(**kw)
+    if TAIL[0] >= 0:
+        eng.set_primal_tail(*TAIL)
+    return eng
+
+
 def host_cores():
     """Host threads this process may use (the affinity mask; at most 16,
     the CPU share of one GPU on the box)."""
@@ -95,10 +106,9 @@ def batched_leg(steps, loop_scenes, loop_steps):
     against the same scenes stepped one context per scene on the same GPU
     (timed on a sample of scenes and steps, reported per scene-step)."""
     from ornis_amd import scenes
-    from ornis_amd.avbd import AvbdEngine
     parts, allsc = scenes.batched_drops()
     n_scenes = len(parts)
-    eng = AvbdEngine(iters=10)
+    eng = new_engine(iters=10)
     eng.load(allsc, scene_ids=allsc["scene_id"])
     eng.step_n(5)
     eng.sync()
@@ -114,7 +124,7 @@ def batched_leg(steps, loop_scenes, loop_steps):
     eng.close()
     batched = n_scenes * steps / wall
     # one context per scene, the loop a caller without batching would run
-    engines = [AvbdEngine(scene=parts[i], iters=10) for i in range(loop_scenes)]
+    engines = [new_engine(scene=parts[i], iters=10) for i in range(loop_scenes)]
     for e in engines:
         e.step_n(5)
         e.sync()
@@ -146,12 +156,11 @@ def differentiable_leg(steps, reps, cpu_scenes, threads):
     scenes across `threads` threads."""
     import torch
     from ornis_amd import diff, scenes
-    from ornis_amd.avbd import AvbdEngine
     parts, allsc = scenes.diff_throws(n_scenes=1024)
     n_scenes = len(parts)
     targets = torch.tensor(np.stack([p["target"] for p in parts]), dtype=torch.float32)
     dyn = allsc["body_type"] == 1
-    eng = AvbdEngine(iters=10)
+    eng = new_engine(iters=10)
     fwd, bwd = [], []
     for r in range(reps + 1):
         v0 = torch.tensor(allsc["vel"], requires_grad=True)
@@ -212,8 +221,7 @@ def trajectory_leg(scene, n_dyn, steps=300, windows=100, late=50, sample_every=5
     the per-phase times and the primal / dual launch rates (HIP events on
     one step in `sample_every`).  The state after `capture_step` is
     returned for the CPU baseline (read while the clock is stopped)."""
-    from ornis_amd.avbd import AvbdEngine
-    eng = AvbdEngine(scene=scene, iters=10)
+    eng = new_engine(scene=scene, iters=10)
     eng.sync()
     first_late = steps - late + 1
     sampled = {i for i in range(first_late, steps + 1) if (i - first_late) % sample_every == 0}
@@ -277,10 +285,9 @@ def chains_leg(steps):
     by ball joints, roots on a 45 x 45 grid at 20 m, falling onto a plane
     scattered with 5,000 static spheres; 12 iterations, `steps` steps."""
     from ornis_amd import scenes
-    from ornis_amd.avbd import AvbdEngine
     s = scenes.chains()
     n_dyn = int((s["body_type"] == 1).sum())
-    eng = AvbdEngine(scene=s, iters=12)
+    eng = new_engine(scene=s, iters=12)
     eng.sync()
     t0 = time.perf_counter()
     done = 0
@@ -333,6 +340,9 @@ def main():
                     help="before warmup, copy a MIB-sized device buffer 3 times (the plain-copy reference "
                          "for HBM byte counters)")
     ap.add_argument("--only-main", action="store_true", help="run the timed leg only (profiling runs)")
+    ap.add_argument("--tail", type=int, default=-1,
+                    help="bodies of the last colour classes fused into one primal launch (0: none; -1: default)")
+    ap.add_argument("--tail-waves", type=int, default=0, help="wavefront limit of the fused tail launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the 100k-box scene fits one MI355X and multi-GPU "
@@ -340,8 +350,9 @@ def main():
         sys.exit(2)
 
     from ornis_amd import scenes
-    from ornis_amd.avbd import AvbdEngine
 
+    global TAIL
+    TAIL = (args.tail, args.tail_waves)
     if args.copy_check:
         import torch
         x = torch.ones(args.copy_check * (1 << 18), dtype=torch.float32, device="cuda")
@@ -353,7 +364,7 @@ def main():
 
     scene = scenes.box_drop_lattice()
     n_dyn = int((scene["body_type"] == 1).sum())
-    eng = AvbdEngine(scene=scene, iters=10)
+    eng = new_engine(scene=scene, iters=10)
     for s in range(args.warmup):
         eng.step_n(1)
         if args.progress:

@@ -69,6 +69,7 @@ typedef struct oavbd_stats {
   uint32_t joint_sides; /* joint sides of swept bodies                       */
   uint32_t toi_casts;   /* (mover, target) casts tested by the TOI clamp     */
   uint32_t joints;      /* joints held by the context                        */
+  uint32_t tail_colours; /* last colour classes swept by one fused launch    */
 } oavbd_stats;
 
 /* Kernel time and algorithmic bytes accumulated since the last reset
@@ -273,7 +274,7 @@ int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
  * per workgroup; 7-9 sixteen lanes per body with the next item's records
  * fetched while the current item is evaluated, 1, 2 or 4 wavefronts per
  * workgroup.  Every variant gives the same bits.  -1 leaves a bucket to
- * the default rule (variant 0 below 8,192 bodies, the oavbd_set_primal_shape
+ * the default rule (variant 7 below 8,192 bodies, the oavbd_set_primal_shape
  * wavefront count above).  No reference counterpart: the reference sweeps
  * bodies serially (mod.rs:1052-1059).
  *
@@ -289,6 +290,20 @@ int oavbd_set_primal_shape(oavbd_ctx* ctx, uint32_t lanes, uint32_t waves);
 int oavbd_autotune(oavbd_ctx* ctx, uint32_t nsteps, int32_t* table_out, float* mean_us_out);
 int oavbd_set_primal_table(oavbd_ctx* ctx, const int32_t* table, uint32_t n);
 int oavbd_get_primal_table(oavbd_ctx* ctx, int32_t* table, uint32_t n);
+
+/* Fused tail: the last colour classes of a sweep, when at least two of them
+ * hold at most `max_bodies` bodies together, are solved by one launch in
+ * which a body waits only for its own partners of lower classes instead of
+ * for a whole class (the colour-by-colour order, the same bits).  0 turns
+ * it off (one launch per class), the default: on the workloads measured the
+ * fused launch is slower (its cross-workgroup waits pay for L2 write-backs
+ * and invalidations).  The launch holds one body
+ * per wavefront and at most `max_waves` wavefronts (0: as many as are
+ * resident on the device at once; fewer make each wavefront solve several
+ * bodies in list order).  The tail is also off while oavbd_autotune
+ * measures the per-class variants.  No reference counterpart (the
+ * reference sweeps bodies serially, mod.rs:1052-1059). */
+int oavbd_set_primal_tail(oavbd_ctx* ctx, uint32_t max_bodies, uint32_t max_waves);
 
 /* Rung-1 staged bodies: n GpuAvbdMass records (16 bytes: inertia[3],
  * inv_mass) and n GpuBodyState residuals (32 bytes: lin[3], pad, ang[3],

@@ -35,7 +35,8 @@ class OavbdParams(ctypes.Structure):
 class OavbdStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in (
         "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
-        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts", "joints")]
+        "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts", "joints",
+        "tail_colours")]
 
 
 class OavbdKernelTimes(ctypes.Structure):
@@ -93,6 +94,7 @@ SIGNATURES = {
     "oavbd_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, _i32p, _f32p]),
     "oavbd_set_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_get_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
+    "oavbd_set_primal_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_solve_diag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_void_p]),
     "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
@@ -181,6 +183,7 @@ class AvbdEngine:
         self._max_joints = max_joints
         self.n = 0
         self.njoints = 0
+        self._tail = None
         if scene is not None:
             self.load(scene)
 
@@ -200,6 +203,8 @@ class AvbdEngine:
         mb = self._max_bodies or max(n, 1)
         mj = self._max_joints or max(nj, 64)
         self._check(self._L.oavbd_create(ctypes.byref(prm), mb, mj, self._device, ctypes.byref(self._ctx)))
+        if self._tail is not None:
+            self._check(self._L.oavbd_set_primal_tail(self._ctx, *self._tail))
 
     @staticmethod
     def _body_arrays(s, scene_ids=None):
@@ -450,6 +455,14 @@ class AvbdEngine:
         t = np.zeros(24, np.int32)
         self._check(self._L.oavbd_get_primal_table(self._ctx, _p(t, _i32p), 24))
         return t
+
+    def set_primal_tail(self, max_bodies, max_waves=0):
+        """Bodies of the last colour classes swept by one fused launch (0: a
+        launch per class), and its wavefront limit (0: device capacity).  Kept
+        for the context a later load creates."""
+        self._tail = (int(max_bodies), int(max_waves))
+        if self._ctx:
+            self._check(self._L.oavbd_set_primal_tail(self._ctx, *self._tail))
 
     def set_timing(self, on=True):
         self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))

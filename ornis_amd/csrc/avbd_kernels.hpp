@@ -2303,22 +2303,26 @@ __device__ __forceinline__ ItemRec load_item(const Dev& d, uint4 it) {
   return r;
 }
 
+// Partner of a point item and of a joint side (the body whose solver
+// record the item reads besides the body's own).
+__device__ __forceinline__ uint32_t item_partner_id(uint4 it) { return (it.x >> 31) == 0u ? it.w : it.z; }
+__device__ __forceinline__ uint32_t joint_partner_id(const Dev& d, uint32_t e) {
+  const uint2 ab = d.J.ab[e >> 1];
+  return (e & 1u) == 0u ? ab.y : ab.x;
+}
+
+// One body's solve on sixteen lanes (slot = lane within the body's lane
+// group; lanes with valid == false hold no body and store nothing).  `cr` is
+// the body's colour-list record: body (bit 31: has joint sides), first
+// incidence item, item count, first joint side.  A lane calls wait(p) before
+// it reads the record of partner p.
 // PF = 1: the records of a slot's next item are requested before its
 // current item is evaluated (one item of records in flight behind the
 // arithmetic, 48 more registers); PF = 0: only the next descriptor.
-template <int W, int R, int PF = 0>
-__global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restrict__ list, uint32_t count) {
-  constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
-  const int tid = threadIdx.x;
-  const int slot = tid % BODY_LANES;
-  const uint32_t idx = blockIdx.x * NB + tid / BODY_LANES;
-  // colour-list record: body (bit 31: has joint sides), first incidence
-  // item, item count, first joint side.  Lanes past the list take an empty
-  // body and skip the stores.
-  const bool valid = idx < count;
+template <int R, int PF, class Wait>
+__device__ __forceinline__ void primal_body(const Dev& d, uint4 cr, bool valid, int slot, Wait&& wait) {
   uint32_t h = 0, i0 = 0, np = 0, j0 = 0, nj = 0;
   if (valid) {
-    const uint4 cr = list[idx];
     h = cr.x & 0x7fffffffu;
     i0 = cr.y;
     np = cr.z;
@@ -2336,6 +2340,14 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
     own[2] = rec[2];
     own[3] = rec[3];
   }
+  // the inertial block (evaluated beside the previous dual update); with
+  // PF = 1 it is requested before the items, else after them
+  const float4* inb = reinterpret_cast<const float4*>(d.inb + 28 * (size_t)h);
+  float4 ib[7];
+  if constexpr (PF == 1) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ib[q] = inb[q];
+  }
   auto add_row = [&](int kind, const Row& row) {
     if (kind == 0)
       acc_stamp(acc, row);
@@ -2351,21 +2363,23 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
     for (uint32_t k = slot; k < np; k += BODY_LANES) {
       const uint4 it = next;
       if (k + BODY_LANES < np) next = d.inc[i0 + k + BODY_LANES];
+      wait(item_partner_id(it));
       point_rows<R>(d, it, own, add_row, SbPartner{d.sb});
     }
-  } else {
-    uint4 cur = make_uint4(0u, 0u, 0u, 0u), nxt = cur;
-    ItemRec rc{};
-    if ((uint32_t)slot < np) {
-      cur = d.inc[i0 + slot];
-      rc = load_item(d, cur);
-    }
-    if ((uint32_t)slot + BODY_LANES < np) nxt = d.inc[i0 + slot + BODY_LANES];
+  } else if ((uint32_t)slot < np) {
+    // The loads run one item ahead and are unconditional (past the slot's
+    // last item they re-read the body's last item), so no branch joins on
+    // a load in flight and the next item's records arrive while the
+    // current item is evaluated.
+    const uint32_t last = i0 + np - 1u;
+    uint4 cur = d.inc[i0 + slot];
+    wait(item_partner_id(cur));
+    ItemRec rc = load_item(d, cur);
+    uint4 nxt = d.inc[min(i0 + slot + BODY_LANES, last)];
     for (uint32_t k = slot; k < np; k += BODY_LANES) {
-      ItemRec rn{};
-      uint4 nn = make_uint4(0u, 0u, 0u, 0u);
-      if (k + BODY_LANES < np) rn = load_item(d, nxt);
-      if (k + 2 * BODY_LANES < np) nn = d.inc[i0 + k + 2 * BODY_LANES];
+      wait(item_partner_id(nxt));
+      const ItemRec rn = load_item(d, nxt);
+      const uint4 nn = d.inc[min(i0 + k + 2 * BODY_LANES, last)];
       item_rows<R>(d, cur, rc, own, add_row);
       cur = nxt;
       rc = rn;
@@ -2375,14 +2389,20 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
   // joint sides continue the item numbering after the point-sides
   const uint32_t total = np + nj;
   for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
-       k += BODY_LANES)
-    joint_rows(d, d.b.ji[j0 + (k - np)], add_row);
+       k += BODY_LANES) {
+    const uint32_t e = d.b.ji[j0 + (k - np)];
+    wait(joint_partner_id(d, e));
+    joint_rows(d, e, add_row);
+  }
   float L21[21], rhs[6];
-  const float4* inb = reinterpret_cast<const float4*>(d.inb + 28 * (size_t)h);
+  if constexpr (PF == 0) {
+#pragma unroll
+    for (int q = 0; q < 7; ++q) ib[q] = inb[q];
+  }
   float blk[28];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
-    const float4 v = valid ? inb[q] : make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+    const float4 v = valid ? ib[q] : make_float4(1.0f, 0.0f, 1.0f, 0.0f);
     blk[4 * q] = v.x;
     blk[4 * q + 1] = v.y;
     blk[4 * q + 2] = v.z;
@@ -2404,6 +2424,82 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
     rec[3] = pk(quat_diff_vec(q, asq(d.b.rot0[h])), 0.0f);
   } else {
     atomicAdd(&d.counters[CTR_BREAKDOWN], 1u);
+  }
+}
+
+struct NoWait {
+  __device__ __forceinline__ void operator()(uint32_t) const {}
+};
+
+// Primal solve of one colour class: W wavefronts per workgroup, four bodies
+// per wavefront.  Lanes past the list take an empty body.
+template <int W, int R, int PF = 0>
+__global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restrict__ list, uint32_t count) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
+  const int tid = threadIdx.x;
+  const uint32_t idx = blockIdx.x * NB + tid / BODY_LANES;
+  const bool valid = idx < count;
+  const uint4 cr = valid ? list[idx] : make_uint4(0u, 0u, 0u, 0u);
+  primal_body<R, PF>(d, cr, valid, tid % BODY_LANES, NoWait{});
+}
+
+// The last colour classes of a sweep in one launch.  Each class's bodies
+// (its run of the colour list, from the device colour offsets) form
+// batches of four, a batch never spanning two classes; batches are numbered
+// class by class, and wavefront w of the `nwaves` launched solves batches
+// w, w + nwaves, ... in that order, four bodies at once on sixteen lanes
+// each, as k_primal8 does.  Before a lane reads the record of a partner
+// whose class lies in [kfirst, the body's class) it waits until the
+// partner's done word holds `stamp` (the partner was solved in this sweep);
+// after a body's solve its slot-0 lane stores `stamp` to the body's done
+// word (release; the waits acquire).  A body therefore reads exactly the
+// partner records the class-by-class launches give it: lower classes
+// already solved, higher ones not yet (they wait for it).  The bodies of a
+// batch share a class and so never wait for each other, and every wait is
+// for an earlier batch; with all nwaves wavefronts resident (the host sizes
+// the grid from the kernel's occupancy) the earliest unfinished batch can
+// always proceed.  A wait gives up after TAIL_SPIN_LIMIT polls and raises
+// the error word, so every wavefront ends.
+constexpr int TAIL_W = 4;  // wavefronts per workgroup
+constexpr uint32_t TAIL_SPIN_LIMIT = 1u << 22;
+__device__ __forceinline__ void tail_wait(const Dev& d, uint32_t p, int kfirst, int kown, uint32_t stamp) {
+  const int cp = d.b.colour[p];
+  if (cp < kfirst || cp >= kown) return;
+  uint32_t spins = 0;
+  while (__hip_atomic_load(d.b.done + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != stamp) {
+    if (++spins == TAIL_SPIN_LIMIT) {
+      atomicOr(d.tail_err, 1u);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+template <int R>
+__global__ void __launch_bounds__(WAVE* TAIL_W) k_primal_tail(Dev d, uint32_t kfirst, uint32_t ncol,
+                                                              uint32_t nbatch, uint32_t nwaves, uint32_t stamp) {
+  constexpr uint32_t BPW = WAVE / BODY_LANES;  // bodies per batch (per wavefront)
+  const int lane = threadIdx.x % WAVE;
+  const uint32_t w = blockIdx.x * TAIL_W + threadIdx.x / WAVE;
+  for (uint32_t b = w; b < nbatch; b += nwaves) {
+    // the batch's class k: the first whose batches reach past b
+    uint32_t k = kfirst, b0 = 0, c0 = 0, cn = 0;
+    for (;;) {
+      c0 = d.colour_off[k];
+      cn = d.colour_off[k + 1] - c0;
+      const uint32_t nb = (cn + BPW - 1) / BPW;
+      if (b < b0 + nb || k + 1 >= ncol) break;
+      b0 += nb;
+      ++k;
+    }
+    const uint32_t j = BPW * (b - b0) + (uint32_t)lane / BODY_LANES;
+    const bool valid = j < cn;
+    const uint4 cr = valid ? d.colour_list[c0 + j] : make_uint4(0u, 0u, 0u, 0u);
+    const int kown = (int)k;
+    primal_body<R, 1>(d, cr, valid, lane % BODY_LANES,
+                      [&](uint32_t p) { tail_wait(d, p, (int)kfirst, kown, stamp); });
+    if (valid && lane % BODY_LANES == 0)
+      __hip_atomic_store(d.b.done + (cr.x & 0x7fffffffu), stamp, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

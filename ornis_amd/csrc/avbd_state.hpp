@@ -94,6 +94,7 @@ struct Bodies {
   uint8_t* asleep;
   float* timer;
   int* colour;
+  uint32_t* done;    // sweep stamp of the body's last solve in a fused tail launch
   uint8_t* flag_a;   // scratch flags (wake / awake / ready / freeze)
   uint8_t* flag_b;
   uint8_t* supported;
@@ -203,6 +204,7 @@ struct Dev {
   Cands C;
   float gravity[3];
   uint32_t* counters;  // device counters (see CTR_*)
+  uint32_t* tail_err;  // raised by a fused tail launch whose wait gave up (never cleared)
   // per-step incidence
   uint32_t* inc_off;   // n+1
   uint4* inc;          // items per body in pair order: (point | side << 31, pair, a, b)

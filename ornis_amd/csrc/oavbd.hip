@@ -105,6 +105,13 @@ static void* noalloc_resolve(int device) {
 // Word offset of the colouring check in the pinned readback buffer (the
 // words below it hold the counters and single-word reads).
 constexpr uint32_t HEAD_H = 64;
+constexpr uint32_t TAILERR_H = CTR_COUNT + 2;  // copy of the fused tail's error word
+// Default bound on the bodies of the last colour classes that one fused
+// launch (k_primal_tail) sweeps: off.  On the 100k-box drop, the chains and
+// the batched scenes the fused launch is slower than a launch per class:
+// each of its waits and releases is an agent-scope acquire / release, which
+// on this GPU invalidates / writes back the XCD's L2.
+constexpr uint32_t TAIL_DEFAULT = 0;
 constexpr uint32_t POST_FLAG = 63;  // word of the readback sequence flag
 
 // Device copy of everything a step reads from earlier steps (bodies, the
@@ -180,6 +187,10 @@ struct oavbd_ctx {
   int waves = 2;    // primal launch shape: wavefronts per workgroup
   int8_t ptable[PRIMAL_BUCKETS];  // primal variant per colour-size bucket (-1: default rule)
   int force_variant = -1;         // every primal launch takes this variant (autotuning)
+  uint32_t tail_max = TAIL_DEFAULT;  // bodies of the last colour classes fused into one launch (0: none)
+  uint32_t tail_waves = 0;           // wavefront limit of the fused tail launch (0: resident capacity)
+  uint32_t tail_stamp = 0;           // stamp of the last fused tail launch
+  uint32_t tail_cap[2] = {0u, 0u};   // resident wavefronts of k_primal_tail<4> / <6>
   std::vector<std::pair<uint32_t, int>> plog;  // (class size, variant) of each timed primal launch
   bool tuning = false;
   double tune_ms[PV_COUNT][PRIMAL_BUCKETS] = {};
@@ -229,13 +240,16 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&b.layer, n); dalloc(&b.mask, n); dalloc(&b.scene, n); dalloc(&b.prio, n); dalloc(&b.local, n);
   dalloc(&b.trigger, n); dalloc(&b.pos0, n); dalloc(&b.rot0, n); dalloc(&b.inertial, n);
   dalloc(&b.inertial_rot, n); dalloc(&b.drot, n); dalloc(&b.prev_pos, n); dalloc(&b.prev_rot, n);
-  dalloc(&b.asleep, n); dalloc(&b.timer, n); dalloc(&b.colour, n);
+  dalloc(&b.asleep, n); dalloc(&b.timer, n); dalloc(&b.colour, n); dalloc(&b.done, n);
+  HIPC(hipMemset(b.done, 0, (size_t)n * sizeof(uint32_t)));
   dalloc(&b.flag_a, n); dalloc(&b.flag_b, n); dalloc(&b.supported, n); dalloc(&b.comp, n);
   dalloc(&b.comp_any, n); dalloc(&b.jp_off, n + 1); dalloc(&b.ji_off, n + 1);
   b.jp = nullptr;
   b.ji = nullptr;
   Dev& d = c->d;
   dalloc(&d.counters, CTR_COUNT);
+  dalloc(&d.tail_err, 1);
+  HIPC(hipMemset(d.tail_err, 0, sizeof(uint32_t)));
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
   dalloc(&d.cand_off, n + 1); dalloc(&d.ptcnt, n + 1);
   dalloc(&d.poff, n + 1); dalloc(&d.poff_old, n + 1); dalloc(&d.sb, (size_t)n * 4); dalloc(&d.inb, (size_t)n * 28);
@@ -358,7 +372,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(b.pos); dfree(b.rot); dfree(b.vel); dfree(b.angvel); dfree(b.torque); dfree(b.inertia); dfree(b.shape);
   dfree(b.mat); dfree(b.kind); dfree(b.type); dfree(b.layer); dfree(b.mask); dfree(b.scene); dfree(b.prio);
   dfree(b.local); dfree(b.trigger); dfree(b.pos0); dfree(b.rot0); dfree(b.inertial); dfree(b.inertial_rot);
-  dfree(b.drot); dfree(b.prev_pos); dfree(b.prev_rot); dfree(b.asleep); dfree(b.timer); dfree(b.colour);
+  dfree(b.drot); dfree(b.prev_pos); dfree(b.prev_rot); dfree(b.asleep); dfree(b.timer); dfree(b.colour); dfree(b.done);
   dfree(b.flag_a); dfree(b.flag_b); dfree(b.supported); dfree(b.comp); dfree(b.comp_any);
   dfree(b.jp_off); dfree(b.jp); dfree(b.ji_off); dfree(b.ji);
   Dev& d = c->d;
@@ -369,7 +383,7 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.g.is_large);
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
-  dfree(d.counters); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
+  dfree(d.counters); dfree(d.tail_err); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.nlist); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
@@ -424,7 +438,9 @@ static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint3
 
 static void read_counters(oavbd_ctx* c) {
   HIPC(hipMemcpyAsync(c->counters_h, c->d.counters, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipMemcpyAsync(c->counters_h + TAILERR_H, c->d.tail_err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
   HIPC(hipStreamSynchronize(c->st));
+  if (c->counters_h[TAILERR_H]) throw HipError{"a fused primal tail launch timed out waiting for a partner"};
 }
 
 // Readbacks the step waits for mid-stream: a one-workgroup kernel writes the
@@ -952,10 +968,12 @@ static void launch_lds(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t
 // next item's records in flight behind the current one, 1, 2 or 4
 // wavefronts per workgroup (k_primal8 with PF = 1).  Every variant gives the same
 // bits.  The variant comes from the autotuned table for the class's size
-// (log2 bucket), else the default: sixteen lanes, one wavefront per
-// workgroup below PRIMAL_TAIL bodies (a small class spread over as many
-// compute units as possible), the configured count above.  Returns the
-// variant launched.
+// (log2 bucket), else the default: below PRIMAL_TAIL bodies sixteen lanes
+// with the next item's records prefetched, one wavefront per workgroup (a
+// small class is bound by its slowest body's chain of item loads and
+// arithmetic, which the prefetch overlaps, and is spread over as many
+// compute units as possible); the configured wavefront count above.
+// Returns the variant launched.
 constexpr uint32_t PRIMAL_TAIL = 8192;
 static int size_bucket(uint32_t cnt) {
   int b = 0;
@@ -966,7 +984,7 @@ static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent
                          hipEvent_t e1 = nullptr) {
   int v = c->force_variant;
   if (v < 0) v = c->ptable[size_bucket(cnt)];
-  if (v < 0) v = cnt < PRIMAL_TAIL ? PV_W1 : (c->waves == 1 ? PV_W1 : c->waves == 2 ? PV_W2 : c->waves == 8 ? PV_W8 : PV_W4);
+  if (v < 0) v = cnt < PRIMAL_TAIL ? PV_P1 : (c->waves == 1 ? PV_W1 : c->waves == 2 ? PV_W2 : c->waves == 8 ? PV_W8 : PV_W4);
   switch (v) {
     case PV_W1: launch_w<1>(c, list, cnt, e0, e1); break;
     case PV_W2: launch_w<2>(c, list, cnt, e0, e1); break;
@@ -980,6 +998,54 @@ static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent
     default: launch_lds<4>(c, list, cnt, e0, e1); v = PV_L4; break;
   }
   return v;
+}
+
+// Wavefronts of k_primal_tail that fit on the device at once (the fused
+// tail launch is sized within this, so all its wavefronts are resident).
+static uint32_t tail_capacity(oavbd_ctx* c) {
+  uint32_t& cap = c->tail_cap[c->rows6 ? 1 : 0];
+  if (!cap) {
+    int nb = 0, ncu = 0;
+    if (c->rows6)
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_primal_tail<6>, WAVE * TAIL_W, 0));
+    else
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_primal_tail<4>, WAVE * TAIL_W, 0));
+    HIPC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    if (nb <= 0 || ncu <= 0) throw HipError{"k_primal_tail does not fit on the device"};
+    cap = (uint32_t)nb * (uint32_t)ncu * TAIL_W;
+  }
+  return cap;
+}
+
+// First colour class of the fused tail: the last classes holding at most
+// tail_max bodies together, when there are at least two of them; ncol
+// (no tail) otherwise, and while a variant is forced.
+static uint32_t tail_first(const oavbd_ctx* c, uint32_t ncol) {
+  if (!c->tail_max || c->force_variant >= 0 || ncol < 2) return ncol;
+  uint32_t k = ncol;
+  while (k > 0 && c->colour_off_h[ncol] - c->colour_off_h[k - 1] <= c->tail_max) --k;
+  return ncol - k >= 2 ? k : ncol;
+}
+
+// One fused launch of classes ktail..ncol-1 (k_primal_tail): batches of
+// four bodies of one class, one batch per wavefront at a time.
+static void launch_tail(oavbd_ctx* c, uint32_t ktail, uint32_t ncol, hipEvent_t e0, hipEvent_t e1) {
+  uint32_t nbatch = 0;
+  for (uint32_t k = ktail; k < ncol; ++k) nbatch += (c->colour_cnt_h[k] + 3u) / 4u;
+  if (!nbatch) return;
+  uint32_t cap = tail_capacity(c);
+  if (c->tail_waves) cap = std::min(cap, std::max<uint32_t>(c->tail_waves / TAIL_W, 1u) * TAIL_W);
+  const uint32_t nblk = std::min<uint32_t>((nbatch + TAIL_W - 1) / TAIL_W, cap / TAIL_W);
+  if (++c->tail_stamp == 0u) {  // the stamps wrapped: no done word may hold a live stamp
+    zero_u32(c, c->d.b.done, c->d.n);
+    c->tail_stamp = 1u;
+  }
+  if (c->rows6)
+    launch_k(c, k_primal_tail<6>, nblk, WAVE * TAIL_W, e0, e1, c->d, ktail, ncol, nbatch, nblk * TAIL_W,
+             c->tail_stamp);
+  else
+    launch_k(c, k_primal_tail<4>, nblk, WAVE * TAIL_W, e0, e1, c->d, ktail, ncol, nbatch, nblk * TAIL_W,
+             c->tail_stamp);
 }
 
 static void record(oavbd_ctx* c, int i) {
@@ -1059,14 +1125,20 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     return pool[i];
   };
   if (c->rec) tape_begin(c);
+  const uint32_t ktail = tail_first(c, ncol);
+  c->stats.tail_colours = ncol - ktail;
   for (uint32_t it = 0; it < c->prm.iters; ++it) {
     if (c->rec) tape_iter_start(c, it);
-    for (uint32_t k = 0; k < ncol; ++k) {
-      uint32_t cnt = c->colour_cnt_h[k];
+    for (uint32_t k = 0; k <= ktail && k < ncol; ++k) {
+      const uint32_t cnt = k < ktail ? c->colour_cnt_h[k] : c->colour_off_h[ncol] - c->colour_off_h[k];
       if (!cnt) continue;
       hipEvent_t e0 = c->timing ? pool_event(c->pev, np_ev) : nullptr;
       hipEvent_t e1 = c->timing ? pool_event(c->pev, np_ev + 1) : nullptr;
-      const int v = launch_primal(c, d.colour_list + c->colour_off_h[k], cnt, e0, e1);
+      int v = -1;  // the fused tail is no tuned variant
+      if (k < ktail)
+        v = launch_primal(c, d.colour_list + c->colour_off_h[k], cnt, e0, e1);
+      else
+        launch_tail(c, ktail, ncol, e0, e1);
       if (c->timing) {
         if (c->plog.size() <= np_ev / 2) c->plog.resize(np_ev / 2 + 1);
         c->plog[np_ev / 2] = {cnt, v};
@@ -1136,7 +1208,7 @@ static void step_inner(oavbd_ctx* c, bool finish) {
       float t = 0;
       HIPC(hipEventElapsedTime(&t, c->pev[i], c->pev[i + 1]));
       pm += t;
-      if (c->tuning) {
+      if (c->tuning && c->plog[i / 2].second >= 0) {
         const auto& l = c->plog[i / 2];
         const int b = size_bucket(l.first);
         c->tune_ms[l.second][b] += t;
@@ -2339,6 +2411,13 @@ int oavbd_set_primal_table(oavbd_ctx* c, const int32_t* table, uint32_t n) {
   for (uint32_t b = 0; b < n; ++b)
     if (table[b] < -1 || table[b] >= PV_COUNT) return fail(c, OAVBD_EINVAL, "bad primal variant");
   for (uint32_t b = 0; b < PRIMAL_BUCKETS; ++b) c->ptable[b] = b < n ? (int8_t)table[b] : (int8_t)-1;
+  return OAVBD_OK;
+}
+
+int oavbd_set_primal_tail(oavbd_ctx* c, uint32_t max_bodies, uint32_t max_waves) {
+  if (!c) return OAVBD_EINVAL;
+  c->tail_max = max_bodies;
+  c->tail_waves = max_waves;
   return OAVBD_OK;
 }
 

@@ -196,8 +196,10 @@ def test_launch_shape_does_not_change_bits(gpu_available, shape):
     knob."""
     s = scenes.box_drop_scene(150, np.random.Generator(np.random.PCG64(17)))
     ref = AvbdEngine(scene=s)
+    ref.set_primal_tail(0)
     ref.step_n(40)
     g = AvbdEngine(scene=s)
+    g.set_primal_tail(0)
     g.set_primal_shape(*shape)
     g.step_n(40)
     assert np.array_equal(ref.state().view(np.uint32), g.state().view(np.uint32))
@@ -228,12 +230,45 @@ def test_primal_variant_does_not_change_bits(gpu_available, variant):
     torsion rows."""
     for s in (scenes.box_drop_scene(300, np.random.Generator(np.random.PCG64(19))), _plate_scene()):
         g = AvbdEngine(scene=s)
+        g.set_primal_tail(0)  # every class its own launch, of the variant under test
         g.set_primal_table([variant] * 24)
         o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
         g.step_n(60)
         o.step_inner(60)
         assert np.array_equal(g.state().view(np.uint32), o.state().view(np.uint32))
         assert np.array_equal(g.points().view(np.uint32), o.points().view(np.uint32))
+
+
+@pytest.mark.parametrize("tail", [(0, 0), (40, 0), (4096, 0), (1 << 30, 0), (1 << 30, 8)])
+def test_fused_tail_does_not_change_bits(gpu_available, tail):
+    """The last colour classes fused into one launch, where a body waits for
+    its own lower-class partners only, give the oracle's colour-by-colour
+    bits: no fusion (the default), a small tail, 4,096 bodies, the whole
+    sweep in one launch, and the whole sweep on eight wavefronts (each
+    solving many batches in list order); a random drop, the plate scene (rolling and torsion
+    rows), and jointed chains."""
+    rows = []
+    from test_gpu_joints import mixed_chains
+    for s in (scenes.box_drop_scene(600, np.random.Generator(np.random.PCG64(29))), _plate_scene(),
+              mixed_chains(n_chains=6, links=8, seed=5)):
+        g = AvbdEngine(scene=s)
+        g.set_primal_tail(*tail)
+        o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+        fused = 0
+        for k in range(6):
+            g.step_n(10)
+            o.step_inner(10)
+            fused = max(fused, g.stats()["tail_colours"])
+            assert np.array_equal(g.state().view(np.uint32), o.state().view(np.uint32)), f"step {10 * k + 9}"
+        assert np.array_equal(g.points().view(np.uint32), o.points().view(np.uint32))
+        if s.get("joints") is not None and len(s["joints"]):
+            nj = len(s["joints"])
+            assert np.array_equal(g.joints().view(np.uint32), o.joints(nj).view(np.uint32))
+        rows.append(fused)
+    if tail[0] == 0:
+        assert max(rows) == 0
+    else:
+        assert max(rows) >= 2
 
 
 def test_autotune_restores_state_and_keeps_bits(gpu_available):
