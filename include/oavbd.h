@@ -70,6 +70,8 @@ typedef struct oavbd_stats {
   uint32_t toi_casts;   /* (mover, target) casts tested by the TOI clamp     */
   uint32_t joints;      /* joints held by the context                        */
   uint32_t tail_colours; /* last colour classes swept by one fused launch    */
+  uint32_t replays;     /* unwaited batches repeated after an overflow (total) */
+  uint32_t unwaited;    /* 1 if the last oavbd_step_n ran unwaited            */
 } oavbd_stats;
 
 /* Kernel time and algorithmic bytes accumulated since the last reset
@@ -304,6 +306,22 @@ int oavbd_get_primal_table(oavbd_ctx* ctx, int32_t* table, uint32_t n);
  * measures the per-class variants.  No reference counterpart (the
  * reference sweeps bodies serially, mod.rs:1052-1059). */
 int oavbd_set_primal_tail(oavbd_ctx* ctx, uint32_t max_bodies, uint32_t max_waves);
+
+/* Unwaited stepping (off by default: one unwaited batch that outgrew its
+ * buffers ended in a GPU memory fault whose cause has not been found, so
+ * the path stays disabled until it is).  Once one step's counts have been
+ * read back, the steps of an oavbd_step_n call run without the host
+ * reading any count mid-step (TOI casts, candidates, pairs, points, colour
+ * classes): launches are sized from buffers kept a quarter above the last
+ * counts and from the last colour count plus `spare_colours` extra class
+ * launches per sweep, and every kernel takes its count from the device.  A
+ * batch in which a count outgrew its buffer is restored from a device
+ * checkpoint taken at its start and repeated with the host reading every
+ * count (oavbd_stats.replays), so the results are the same bits either
+ * way.  Recorded (differentiable), timed and autotuning steps always wait.
+ * No reference counterpart (the reference's CPU step has no host/device
+ * split). */
+int oavbd_set_unwaited(oavbd_ctx* ctx, int enable, uint32_t spare_colours);
 
 /* Rung-1 staged bodies: n GpuAvbdMass records (16 bytes: inertia[3],
  * inv_mass) and n GpuBodyState residuals (32 bytes: lin[3], pad, ang[3],

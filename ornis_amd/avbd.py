@@ -36,7 +36,7 @@ class OavbdStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_uint32) for k in (
         "pairs", "points", "candidates", "colours", "max_colour", "sleeping", "breakdowns", "toi_hits",
         "colour_rounds", "solved", "point_sides", "pair_sides", "joint_sides", "toi_casts", "joints",
-        "tail_colours")]
+        "tail_colours", "replays", "unwaited")]
 
 
 class OavbdKernelTimes(ctypes.Structure):
@@ -95,6 +95,7 @@ SIGNATURES = {
     "oavbd_set_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_get_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_set_primal_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "oavbd_set_unwaited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]),
     "oavbd_solve_diag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_void_p]),
     "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
@@ -184,6 +185,7 @@ class AvbdEngine:
         self.n = 0
         self.njoints = 0
         self._tail = None
+        self._unwaited = None
         if scene is not None:
             self.load(scene)
 
@@ -205,6 +207,8 @@ class AvbdEngine:
         self._check(self._L.oavbd_create(ctypes.byref(prm), mb, mj, self._device, ctypes.byref(self._ctx)))
         if self._tail is not None:
             self._check(self._L.oavbd_set_primal_tail(self._ctx, *self._tail))
+        if self._unwaited is not None:
+            self._check(self._L.oavbd_set_unwaited(self._ctx, *self._unwaited))
 
     @staticmethod
     def _body_arrays(s, scene_ids=None):
@@ -463,6 +467,15 @@ class AvbdEngine:
         self._tail = (int(max_bodies), int(max_waves))
         if self._ctx:
             self._check(self._L.oavbd_set_primal_tail(self._ctx, *self._tail))
+
+    def set_unwaited(self, on=True, spare_colours=1):
+        """Unwaited stepping (default on): once a step's counts have been read
+        back, step_n runs its steps without the host reading any count,
+        repeating a batch whose counts outgrew their buffers; the same bits
+        either way.  Kept for the context a later load creates."""
+        self._unwaited = (1 if on else 0, int(spare_colours))
+        if self._ctx:
+            self._check(self._L.oavbd_set_unwaited(self._ctx, *self._unwaited))
 
     def set_timing(self, on=True):
         self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))

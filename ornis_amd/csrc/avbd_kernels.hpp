@@ -45,6 +45,28 @@ __device__ __forceinline__ void block_add(uint32_t* ctr, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
+// counts the device derives (a step never needs the host to read them)
+// ---------------------------------------------------------------------------
+// TOI casts and broadphase candidates: their scan totals, or zero when the
+// total exceeds the buffer's capacity (the fill kernels then record the
+// overflow and write nothing, so every later kernel stays in bounds).
+__device__ __forceinline__ uint32_t dev_ncast(const Dev& d) {
+  const uint32_t t = d.toi_off[d.n];
+  return t <= d.cap_toi ? t : 0u;
+}
+__device__ __forceinline__ uint32_t dev_ncand(const Dev& d) {
+  const uint32_t t = d.cand_off[d.n];
+  return t <= d.cap_cand ? t : 0u;
+}
+// This step's pairs and points (written by k_pair_offsets).
+__device__ __forceinline__ uint32_t dev_npairs(const Dev& d) { return d.dc[DC_NPAIRS]; }
+__device__ __forceinline__ uint32_t dev_npts(const Dev& d) { return d.dc[DC_NPTS]; }
+__device__ __forceinline__ void note_overflow(const Dev& d, uint32_t word, uint32_t need) {
+  atomicOr(d.ovf + OVF_FLAGS, 1u << word);
+  atomicMax(d.ovf + word, need);
+}
+
+// ---------------------------------------------------------------------------
 // body helpers
 // ---------------------------------------------------------------------------
 
@@ -153,16 +175,18 @@ __device__ __forceinline__ uint32_t bucket_of(int x, int y, int z, uint32_t sc, 
   return h & (nb - 1u);
 }
 
-__global__ void k_grid_clear(Dev d) {
+// `after_toi`: the rebuild after the TOI clamps, which runs only when
+// there were casts.
+__global__ void k_grid_clear(Dev d, uint32_t after_toi) {
   uint32_t i = gid();
-  if (i >= d.g.nbuckets) return;
+  if (i >= d.g.nbuckets || (after_toi && dev_ncast(d) == 0u)) return;
   d.g.cnt[i] = 0u;
   d.g.cursor[i] = 0u;
 }
 
-__global__ void k_grid_count(Dev d) {
+__global__ void k_grid_count(Dev d, uint32_t after_toi) {
   uint32_t h = gid();
-  if (h >= d.n) return;
+  if (h >= d.n || (after_toi && dev_ncast(d) == 0u)) return;
   if (d.g.is_large[h]) {
     d.g.bucket[h] = 0xffffffffu;
     return;
@@ -176,9 +200,9 @@ __global__ void k_grid_count(Dev d) {
   atomicAdd(&d.g.cnt[bk], 1u);
 }
 
-__global__ void k_grid_fill(Dev d) {
+__global__ void k_grid_fill(Dev d, uint32_t after_toi) {
   uint32_t h = gid();
-  if (h >= d.n) return;
+  if (h >= d.n || (after_toi && dev_ncast(d) == 0u)) return;
   uint32_t bk = d.g.bucket[h];
   if (bk == 0xffffffffu) return;
   uint32_t at = d.g.start[bk] + atomicAdd(&d.g.cursor[bk], 1u);
@@ -402,6 +426,11 @@ __global__ void __launch_bounds__(TPB) k_toi_count(Dev d) {
 
 __global__ void k_toi_fill(Dev d) {
   uint32_t h = gid();
+  const uint32_t total = d.toi_off[d.n];
+  if (total > d.cap_toi) {  // recorded; the casts read as none
+    if (h == 0) note_overflow(d, OVF_TOI, total);
+    return;
+  }
   if (h >= d.n || !d.fast[h]) return;
   uint2* out = d.toi_pairs + d.toi_off[h];
   uint32_t m = d.toi_cnt[h];
@@ -482,8 +511,9 @@ __device__ __forceinline__ void toi_result(const Dev& d, uint32_t i, uint2 ho, f
 // — most casts, since movers mostly pass their neighbours.  The rest are
 // listed for k_toi_pair.
 template <int G>
-__global__ void __launch_bounds__(TPB) k_toi_gate(Dev d, uint32_t npairs) {
+__global__ void __launch_bounds__(TPB) k_toi_gate(Dev d) {
   __shared__ uint32_t s_n, s_base;
+  const uint32_t npairs = dev_ncast(d);
   if (threadIdx.x == 0) s_n = 0u;
   __syncthreads();
   const uint32_t i = gid() / G;
@@ -535,9 +565,9 @@ __global__ void __launch_bounds__(TPB) k_toi_pair(Dev d) {
 
 // The mover's hit is the cast holding the winning key (a repeated
 // (mover, target) entry carries identical values).
-__global__ void __launch_bounds__(TPB) k_toi_win(Dev d, uint32_t npairs) {
+__global__ void __launch_bounds__(TPB) k_toi_win(Dev d) {
   uint32_t i = gid();
-  if (i >= npairs) return;
+  if (i >= dev_ncast(d)) return;
   float4 r = d.toi_res[i];
   if (!(r.w >= 0.0f)) return;
   uint2 ho = d.toi_pairs[i];
@@ -569,6 +599,7 @@ __device__ __forceinline__ bool toi_apply_body(const Dev& d, uint32_t h) {
   return true;
 }
 __global__ void __launch_bounds__(TPB) k_toi_apply(Dev d) {
+  if (dev_ncast(d) == 0u) return;  // whole launch
   uint32_t h = gid();
   bool hit = false;
   if (h < d.n) hit = toi_apply_body(d, h);
@@ -582,9 +613,9 @@ __device__ __forceinline__ void apply_wake_body(const Dev& d, uint32_t h) {
     wake_body_dev(d, h);
   }
 }
-__global__ void k_apply_wake(Dev d) {
+__global__ void k_apply_wake(Dev d, uint32_t after_toi) {
   uint32_t h = gid();
-  if (h >= d.n) return;
+  if (h >= d.n || (after_toi && dev_ncast(d) == 0u)) return;
   apply_wake_body(d, h);
 }
 
@@ -746,6 +777,11 @@ __global__ void k_bp_large_count(Dev d, uint32_t* cnt) {
 __global__ void __launch_bounds__(TPB) k_bp_fill(Dev d) {
   __shared__ uint32_t srt[BP_CACHE][TPB];  // column per thread: no bank conflicts between lanes
   uint32_t a = gid();
+  const uint32_t total = d.cand_off[d.n];
+  if (total > d.cap_cand) {  // recorded; the candidates read as none
+    if (a == 0) note_overflow(d, OVF_CAND, total);
+    return;
+  }
   if (a >= d.n || d.g.is_large[a]) return;
   uint2* out = d.C.ab + d.cand_off[a];
   const uint32_t m = d.cand_off[a + 1] - d.cand_off[a];
@@ -778,7 +814,7 @@ __global__ void __launch_bounds__(TPB) k_bp_fill(Dev d) {
 
 __global__ void k_bp_large_fill(Dev d) {
   uint32_t t = gid();
-  if (t >= d.g.lflat || !d.lflag[t]) return;
+  if (t >= d.g.lflat || !d.lflag[t] || d.cand_off[d.n] > d.cap_cand) return;
   uint4 r = d.g.lrange[large_row(d, t)];
   uint32_t b = d.g.sitems[r.x + (t - r.z)];
   d.C.ab[d.cand_off[r.w] + d.lrank[t] - d.lrank[r.z]] = make_uint2(r.w, b);
@@ -804,9 +840,21 @@ __device__ __forceinline__ int find_old_pair(const Dev& d, uint32_t a, uint32_t 
 
 // First pair of every body's a-run: pairs are grouped by their first body
 // in body order, so the run of h starts where h's accepted candidates do.
+// Every thread also derives this step's pair and point totals (the
+// acceptance scan's); when either exceeds its table the overflow is
+// recorded and the step has no pairs (all a-runs empty).
 __global__ void k_pair_offsets(Dev d) {
   uint32_t h = gid();
-  if (h <= d.n) d.poff[h] = d.acc_off[d.cand_off[h]];
+  const uint32_t nc = dev_ncand(d);
+  const uint32_t np = d.acc_off[nc], nq = d.pt_off[nc];
+  const bool fits = np <= d.cap_pairs && nq <= d.cap_pts;
+  if (h <= d.n) d.poff[h] = (fits && nc) ? d.acc_off[d.cand_off[h]] : 0u;
+  if (h == 0) {
+    d.dc[DC_NPAIRS] = fits ? np : 0u;
+    d.dc[DC_NPTS] = fits ? nq : 0u;
+    if (np > d.cap_pairs) note_overflow(d, OVF_PAIRS, np);
+    if (nq > d.cap_pts) note_overflow(d, OVF_PTS, nq);
+  }
 }
 
 // Narrowphase gate: filters, the old pair, and for two boxes the SAT gap
@@ -816,8 +864,9 @@ __global__ void k_pair_offsets(Dev d) {
 // wavefronts: overlapping box pairs (whose feature scans are the longest)
 // from the back of the candidate range, the rest from the front, so that
 // wavefronts hold work of one kind.
-__global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
+__global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d) {
   __shared__ uint32_t s_n, s_base, s_p, s_pbase;  // front and back lists
+  const uint32_t ncand = dev_ncand(d);
   if (threadIdx.x == 0) {
     s_n = 0u;
     s_p = 0u;
@@ -865,7 +914,8 @@ __global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d, uint32_t ncand) {
 
 // Entry t of the gate's list: the first counters[CTR_NARROW] entries of
 // nlist and the last counters[CTR_NARROW_PEN]; false between them.
-__device__ __forceinline__ bool narrow_entry(const Dev& d, uint32_t t, uint32_t ncand, uint32_t* c) {
+__device__ __forceinline__ bool narrow_entry(const Dev& d, uint32_t t, uint32_t* c) {
+  const uint32_t ncand = dev_ncand(d);
   if (t >= ncand) return false;
   if (t >= d.counters[CTR_NARROW] && t < ncand - d.counters[CTR_NARROW_PEN]) return false;
   *c = d.nlist[t];
@@ -877,11 +927,11 @@ __device__ __forceinline__ bool narrow_entry(const Dev& d, uint32_t t, uint32_t 
 // stashed in the candidate's first two point slots for k_narrow, which
 // reads them before it writes its points.
 template <int G>
-__global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
+__global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d) {
   const uint32_t t = gid() / G;
   const int s = (int)(threadIdx.x % G);
   uint32_t c;
-  if (!narrow_entry(d, t, ncand, &c)) return;  // whole groups
+  if (!narrow_entry(d, t, &c)) return;  // whole groups
   const uint2 ab = d.C.ab[c];
   if (!(d.b.kind[ab.x] == SH_BOX && d.b.kind[ab.y] == SH_BOX)) return;
   const shp A = body_shape(d, ab.x), B = body_shape(d, ab.y);
@@ -894,13 +944,13 @@ __global__ void __launch_bounds__(TPB) k_narrow_dist(Dev d, uint32_t ncand) {
 
 // Narrowphase of the gate's survivors (thread t takes list entry t; threads
 // off the list return at once).
-__global__ void __launch_bounds__(TPB) k_narrow(Dev d, uint32_t ncand) {
+__global__ void __launch_bounds__(TPB) k_narrow(Dev d) {
   // the manifold points (local offsets on a and on b) of each thread, one
   // LDS column per thread: the point loops index them at run time
   __shared__ float s_pts[K_MAX_POINTS * 6][TPB];
   const uint32_t t = gid();
   uint32_t c;
-  if (!narrow_entry(d, t, ncand, &c)) return;
+  if (!narrow_entry(d, t, &c)) return;
   uint2 ab = d.C.ab[c];
   uint32_t ia = ab.x, ib = ab.y;
   shp A = body_shape(d, ia), B = body_shape(d, ib);
@@ -1047,7 +1097,7 @@ __global__ void __launch_bounds__(TPB) k_merge(Dev d) {
   constexpr int MG = K_MAX_POINTS;
   const uint32_t p = gid() / MG;
   const uint32_t k = gid() % MG;
-  if (p >= d.npairs) return;  // whole groups
+  if (p >= dev_npairs(d)) return;  // whole groups
   const uint32_t c = d.pair_cand[p];
   uint32_t start = d.pt_off[c];
   uint32_t cnt = d.C.npts[c];
@@ -1330,7 +1380,7 @@ __global__ void k_joint_prep(Dev d) {
 __global__ void __launch_bounds__(TPB) k_pair_prep(Dev d) {
   const uint32_t p = gid() / PREP_LANES;
   const uint32_t s = threadIdx.x % PREP_LANES;
-  if (p >= d.npairs) return;
+  if (p >= dev_npairs(d)) return;
   uint2 ab = d.P.ab[p];
   float4 n4 = d.P.n(p);
   f3 n = xyz(n4);
@@ -1388,7 +1438,7 @@ __global__ void __launch_bounds__(TPB) k_pair_prep(Dev d) {
 __global__ void __launch_bounds__(TPB) k_inc_count(Dev d) {
   uint32_t p = gid();
   uint32_t side_pairs = 0;
-  if (p < d.npairs) {
+  if (p < dev_npairs(d)) {
     uint2 ab = d.P.ab[p];
     uint32_t c = d.P.pts[p].y;
     bool sa = d.sw[ab.x], sb = d.sw[ab.y];
@@ -1433,7 +1483,7 @@ __device__ __forceinline__ void nbr_link(const Dev& d, uint2 ab) {
 }
 __global__ void k_inc_fill(Dev d) {
   uint32_t p = gid();
-  if (p >= d.npairs) return;
+  if (p >= dev_npairs(d)) return;
   uint2 ab = d.P.ab[p];
   bool sa = d.sw[ab.x], sb = d.sw[ab.y];
   if (sb) d.blist[d.boff[ab.y] + atomicAdd(&d.bcur[ab.y], 1u)] = p;
@@ -1684,10 +1734,13 @@ __global__ void __launch_bounds__(COLOUR_TAIL_TPB) k_colour_tail(Dev d, uint32_t
 // one atomic per body).
 // Exclusive offsets of the colour classes (colour_off[0..ncol], ncol the
 // colour count in cstat) from their sizes: one wavefront, 64 colours a round.
-__global__ void __launch_bounds__(64) k_colour_offsets(Dev d) {
+// More colours than the `slots` primal launches the host queued is
+// recorded as an overflow.
+__global__ void __launch_bounds__(64) k_colour_offsets(Dev d, uint32_t slots) {
   constexpr int WV = 64;
   const uint32_t ncol = d.cstat[CSTAT_MAX];
   const int lane = (int)threadIdx.x;
+  if (lane == 0 && ncol > slots) note_overflow(d, OVF_COLOURS, ncol);
   uint32_t base = 0;
   for (uint32_t c0 = 0; c0 < ncol; c0 += WV) {
     const uint32_t k = c0 + (uint32_t)lane;
@@ -2443,6 +2496,23 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restr
   primal_body<R, PF>(d, cr, valid, tid % BODY_LANES, NoWait{});
 }
 
+// The same solve of colour class `k` with the class's range read from the
+// device colour offsets (a step whose colour sizes the host has not read):
+// workgroups loop over the class's body groups, so any grid covers it.
+template <int W, int R, int PF = 0>
+__global__ void __launch_bounds__(WAVE* W) k_primal8_class(Dev d, uint32_t k) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
+  const int tid = threadIdx.x;
+  const uint32_t count = d.colour_cnt[k];
+  const uint4* __restrict__ list = d.colour_list + d.colour_off[k];
+  for (uint32_t g = blockIdx.x; g * NB < count; g += gridDim.x) {
+    const uint32_t idx = g * NB + tid / BODY_LANES;
+    const bool valid = idx < count;
+    const uint4 cr = valid ? list[idx] : make_uint4(0u, 0u, 0u, 0u);
+    primal_body<R, PF>(d, cr, valid, tid % BODY_LANES, NoWait{});
+  }
+}
+
 // The last colour classes of a sweep in one launch.  Each class's bodies
 // (its run of the colour list, from the device colour offsets) form
 // batches of four, a batch never spanning two classes; batches are numbered
@@ -2731,11 +2801,16 @@ __device__ __forceinline__ void store_inertial_block(const Dev& d, uint32_t h) {
 
 // Dual update of every point (threads below npts) and, when `next` is set,
 // the inertial blocks of the next iteration's sweep (threads above).
+// A grid-stride loop: the point count is the device's.
 __global__ void __launch_bounds__(TPB) k_dual(Dev d, uint32_t next) {
-  uint32_t g = gid();
-  if (g < d.npts) return dual_point(d, g);
-  g -= d.npts;
-  if (next && g < d.n) store_inertial_block(d, g);
+  const uint32_t npts = dev_npts(d);
+  const uint32_t total = npts + (next ? d.n : 0u);
+  for (uint32_t g = gid(); g < total; g += gridDim.x * TPB) {
+    if (g < npts)
+      dual_point(d, g);
+    else
+      store_inertial_block(d, g - npts);
+  }
 }
 
 __global__ void k_inertial_blocks(Dev d) {
@@ -2835,7 +2910,7 @@ __global__ void k_bdf1(Dev d) {
 
 __global__ void k_supported(Dev d) {
   uint32_t p = gid();
-  if (p < d.npairs) {
+  if (p < dev_npairs(d)) {
     if (d.P.n(p).w <= 0.0f && d.P.pts[p].y > 0u) {
       uint2 ab = d.P.ab[p];
       d.b.supported[ab.x] = 1;

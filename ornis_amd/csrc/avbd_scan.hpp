@@ -79,11 +79,33 @@ struct ScanJobs {
   uint32_t* post = nullptr;
   uint32_t* post_ticket = nullptr;
   uint32_t post_word = 0, post_flag = 0, post_seq = 0;
+  // optional: the length is n_dev[0] when that is at most the launch's n
+  // (the launch is sized for n), else zero -- a count only the device holds
+  const uint32_t* n_dev = nullptr;
+  // optional: the scan runs only when 0 < gate[0] <= gate_max
+  const uint32_t* gate = nullptr;
+  uint32_t gate_max = 0;
 };
+
+// The scan's length on the device (see ScanJobs::n_dev, ::gate); false
+// when the scan is skipped.
+__device__ __forceinline__ bool scan_length(const ScanJobs& jobs, uint32_t* n, uint32_t* ntiles) {
+  if (jobs.gate) {
+    const uint32_t g = *jobs.gate;
+    if (g == 0u || g > jobs.gate_max) return false;
+  }
+  if (jobs.n_dev) {
+    const uint32_t m = *jobs.n_dev;
+    *n = m <= *n ? m : 0u;
+    *ntiles = (*n + SCAN_TILE - 1) / SCAN_TILE;
+  }
+  return true;
+}
 
 __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_reduce(ScanJobs jobs, uint32_t n, uint32_t ntiles,
                                                             uint32_t* tiles, uint32_t* dones) {
   __shared__ uint32_t s[SCAN_BLOCK / 64];
+  if (!scan_length(jobs, &n, &ntiles) || blockIdx.x >= ntiles) return;  // whole workgroups
   const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
   uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
   uint32_t* done = dones + blockIdx.y;
@@ -127,6 +149,11 @@ __global__ void __launch_bounds__(SCAN_BLOCK) k_scan_apply(ScanJobs jobs, uint32
   __shared__ uint32_t s[SCAN_BLOCK / 64];
   const uint32_t* __restrict__ in = jobs.in[blockIdx.y];
   uint32_t* __restrict__ out = jobs.out[blockIdx.y];
+  if (!scan_length(jobs, &n, &ntiles)) return;
+  if (blockIdx.x >= ntiles) {  // whole workgroups; an empty scan still writes its total
+    if (n == 0u && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0u;
+    return;
+  }
   const uint32_t* tile = tiles + (size_t)blockIdx.y * (SCAN_MAX_TILES + 1);
   const uint32_t tid = threadIdx.x;
   const uint32_t base = blockIdx.x * SCAN_TILE + tid * SCAN_PER_THREAD;

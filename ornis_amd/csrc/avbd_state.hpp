@@ -251,7 +251,19 @@ struct Dev {
   uint32_t* toi_cache; // n x TOI_CACHE targets found by the counting pass
   unsigned long long* toi_best;  // per mover min (t bits << 32 | target)
   uint8_t* fast;
+  // Capacities of the buffers sized by step counts (TOI casts, candidates,
+  // and this step's pair and point tables), and the counts the device
+  // derives itself, so a step can run without the host reading them: a
+  // count past its capacity reads as zero downstream and is recorded in
+  // the overflow words (the host then repeats the steps with larger
+  // buffers).
+  uint32_t cap_toi, cap_cand, cap_pairs, cap_pts;
+  uint32_t* dc;   // DC_*: this step's pair and point counts (k_pair_offsets)
+  uint32_t* ovf;  // OVF_*: overflow bits and the largest counts that did not fit
 };
+
+enum { DC_NPAIRS = 0, DC_NPTS = 1, DC_WORDS = 4 };
+enum { OVF_FLAGS = 0, OVF_TOI = 1, OVF_CAND = 2, OVF_PAIRS = 3, OVF_PTS = 4, OVF_COLOURS = 5, OVF_WORDS = 8 };
 
 // CTR_NARROW and CTR_NARROW_PEN share one 64-bit word (even index) that the
 // narrowphase gate bumps with one atomic.

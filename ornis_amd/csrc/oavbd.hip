@@ -113,12 +113,19 @@ constexpr uint32_t TAILERR_H = CTR_COUNT + 2;  // copy of the fused tail's error
 // on this GPU invalidates / writes back the XCD's L2.
 constexpr uint32_t TAIL_DEFAULT = 0;
 constexpr uint32_t POST_FLAG = 63;  // word of the readback sequence flag
+constexpr uint32_t OVF_H = 40;      // copy of the overflow words (OVF_WORDS)
+constexpr uint32_t DC_H = 48;       // copy of the device counts (DC_WORDS)
+constexpr uint32_t TOTALS_H = 52;   // TOI cast and candidate totals of the last step
+static_assert(OVF_H >= TAILERR_H + 1 && DC_H >= OVF_H + OVF_WORDS && TOTALS_H >= DC_H + DC_WORDS &&
+                  TOTALS_H + 2 <= POST_FLAG,
+              "readback words overlap");
 
 // Device copy of everything a step reads from earlier steps (bodies, the
 // pair table with its points, joint duals): restoring one makes the next
 // steps repeat bit for bit.
 struct Snap {
   bool valid = false;
+  uint32_t alloc_n = 0, alloc_nj = 0;  // bodies and joints the per-body / per-joint copies hold
   size_t cap_pairs = 0, cap_pts = 0;
   uint32_t npairs = 0, npts = 0, sleeping = 0;
   float time_debt = 0.0f;
@@ -209,6 +216,16 @@ struct oavbd_ctx {
   uint32_t diff_steps = 0, diff_every = 0, diff_max_pts = 0, diff_max_pairs = 0;
   Adj adj{};
   size_t adj_cap_n = 0, adj_cap_pts = 0, adj_cap_pairs = 0;
+  // unwaited steps (step_inner with async): the counts the host last read,
+  // which size the buffers and grids of the next batch, the primal
+  // launches queued per sweep, and the batch's starting state
+  bool async_enabled = false;  // off: an unwaited batch faulted the GPU (cause not yet found)
+  bool sizes_known = false;  // a step's counts have been read back since the last upload
+  uint32_t last_ncast = 0, last_live = 0, last_ncand = 0, last_npairs = 0, last_npts = 0;
+  uint32_t slots = 0;        // colour classes launched per sweep by an unwaited step
+  uint32_t slot_spare = 1;   // classes queued beyond the last step's colour count
+  uint32_t replays = 0;      // batches repeated with exact sizes after an overflow
+  Snap bsnap;
   std::string err;
 };
 
@@ -248,6 +265,11 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   b.ji = nullptr;
   Dev& d = c->d;
   dalloc(&d.counters, CTR_COUNT);
+  dalloc(&d.dc, DC_WORDS);
+  dalloc(&d.ovf, OVF_WORDS);
+  HIPC(hipMemset(d.dc, 0, DC_WORDS * sizeof(uint32_t)));
+  HIPC(hipMemset(d.ovf, 0, OVF_WORDS * sizeof(uint32_t)));
+  d.cap_toi = d.cap_cand = d.cap_pairs = d.cap_pts = 0;
   dalloc(&d.tail_err, 1);
   HIPC(hipMemset(d.tail_err, 0, sizeof(uint32_t)));
   dalloc(&d.inc_off, n + 1); dalloc(&d.bcnt, n + 1); dalloc(&d.boff, n + 1); dalloc(&d.bcur, n + 1);
@@ -318,6 +340,7 @@ static void ensure_cand(oavbd_ctx* c, size_t need) {
   dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
   if (cap + 1 > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"candidate count exceeds the scan capacity"};
   c->cap_cand = cap;
+  c->d.cap_cand = (uint32_t)cap;
 }
 
 static void ensure_pairs(oavbd_ctx* c, size_t need) {
@@ -325,6 +348,7 @@ static void ensure_pairs(oavbd_ctx* c, size_t need) {
   free_pairs(c->d.P);
   c->cap_pairs = grow(need);
   alloc_pairs(c->d.P, c->cap_pairs);
+  c->d.cap_pairs = (uint32_t)c->cap_pairs;
 }
 
 static void ensure_points(oavbd_ctx* c, size_t need) {
@@ -332,6 +356,21 @@ static void ensure_points(oavbd_ctx* c, size_t need) {
   free_points(c->d.Q);
   c->cap_pts = grow(need);
   alloc_points(c->d.Q, c->cap_pts);
+  c->d.cap_pts = (uint32_t)c->cap_pts;
+}
+
+// TOI cast buffers for `need` casts.
+static void ensure_toi(oavbd_ctx* c, size_t need) {
+  if (need <= c->cap_toi && c->d.toi_pairs) return;
+  Dev& d = c->d;
+  dfree(d.toi_pairs);
+  dfree(d.toi_res);
+  dfree(d.toi_live);
+  c->cap_toi = grow(need);
+  dalloc(&d.toi_pairs, c->cap_toi);
+  dalloc(&d.toi_res, c->cap_toi);
+  dalloc(&d.toi_live, c->cap_toi);
+  d.cap_toi = (uint32_t)c->cap_toi;
 }
 
 // Per-step incidence scratch, sized for the current pair / point counts.
@@ -383,12 +422,13 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.g.is_large);
   Cands& C = d.C;
   dfree(C.ab); dfree(C.acc); dfree(C.npts); dfree(C.disc0); dfree(C.disc1); dfree(C.fresh);
-  dfree(d.counters); dfree(d.tail_err); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
+  dfree(d.counters); dfree(d.dc); dfree(d.ovf); dfree(d.tail_err); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.nlist); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
   dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
   adj_free(c);
+  snap_free(c->bsnap);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,10 +456,13 @@ static void scan(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint32_t n) {
   scan_tiles(c->ts, js, 1, n, c->st);
 }
 // Two or three arrays of the same length in one launch pair.
+// With n_dev, the length is the device word n_dev[0] (zero when it exceeds
+// n, for which the launch is sized).
 static void scan2(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint32_t* in1, uint32_t* out1,
-                  uint32_t n, uint32_t* pick0 = nullptr, uint32_t post_seq = 0) {
+                  uint32_t n, uint32_t* pick0 = nullptr, uint32_t post_seq = 0, const uint32_t* n_dev = nullptr) {
   if ((size_t)n > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"scan longer than the scan capacity"};
   ScanJobs js{{in0, in1, nullptr}, {out0, out1, nullptr}, pick0};
+  js.n_dev = n_dev;
   if (post_seq && n) {  // both totals to the pinned buffer from CTR_COUNT, then the flag
     js.post = c->counters_hd;
     js.post_ticket = c->ts.done + SCAN_JOBS;
@@ -436,12 +479,6 @@ static void scan3(oavbd_ctx* c, const uint32_t* in0, uint32_t* out0, const uint3
   scan_tiles(c->ts, js, 3, n, c->st);
 }
 
-static void read_counters(oavbd_ctx* c) {
-  HIPC(hipMemcpyAsync(c->counters_h, c->d.counters, CTR_COUNT * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipMemcpyAsync(c->counters_h + TAILERR_H, c->d.tail_err, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
-  HIPC(hipStreamSynchronize(c->st));
-  if (c->counters_h[TAILERR_H]) throw HipError{"a fused primal tail launch timed out waiting for a partner"};
-}
 
 // Readbacks the step waits for mid-stream: a one-workgroup kernel writes the
 // words straight into the pinned buffer and then raises a sequence flag
@@ -627,12 +664,16 @@ static void snap_free(Snap& s) {
   dfree(s.poff); dfree(s.qrec); dfree(s.qroll); dfree(s.qpair); dfree(s.qsrc); dfree(s.jlam); dfree(s.jpen); dfree(s.jlama); dfree(s.jpena);
   s.valid = false;
   s.cap_pairs = s.cap_pts = 0;
+  s.alloc_n = s.alloc_nj = 0;
 }
 
 static void snap_save(oavbd_ctx* c, Snap& s) {
   Dev& d = c->d;
   const uint32_t n = d.n, nj = d.njoints;
+  if (s.pos && (s.alloc_n < n || s.alloc_nj < nj)) snap_free(s);  // the body or joint table grew
   if (!s.pos) {
+    s.alloc_n = n;
+    s.alloc_nj = nj;
     dalloc(&s.pos, n); dalloc(&s.rot, n); dalloc(&s.vel, n); dalloc(&s.angvel, n); dalloc(&s.torque, n);
     dalloc(&s.inertia, n); dalloc(&s.prev_pos, n); dalloc(&s.prev_rot, n); dalloc(&s.asleep, n); dalloc(&s.flag_a, n);
     dalloc(&s.timer, n); dalloc(&s.poff, n + 1); dalloc(&s.jlam, nj); dalloc(&s.jpen, nj);
@@ -783,15 +824,43 @@ static void wake_joint_motion(oavbd_ctx* c) {
   k_awake_apply<<<blocks(n), TPB, 0, c->st>>>(c->d);
 }
 
-static void build_grid(oavbd_ctx* c) {
+// `after_toi`: the rebuild after the TOI clamps, skipped on the device
+// when there were no casts.
+static void build_grid(oavbd_ctx* c, bool after_toi = false) {
   Dev& d = c->d;
-  k_grid_clear<<<blocks(d.g.nbuckets), TPB, 0, c->st>>>(d);
-  k_grid_count<<<blocks(d.n), TPB, 0, c->st>>>(d);
-  scan(c, d.g.cnt, d.g.start, d.g.nbuckets);
-  k_grid_fill<<<blocks(d.n), TPB, 0, c->st>>>(d);
+  const uint32_t g = after_toi ? 1u : 0u;
+  k_grid_clear<<<blocks(d.g.nbuckets), TPB, 0, c->st>>>(d, g);
+  k_grid_count<<<blocks(d.n), TPB, 0, c->st>>>(d, g);
+  ScanJobs js{};
+  js.in[0] = d.g.cnt;
+  js.out[0] = d.g.start;
+  if (after_toi) {
+    js.gate = d.toi_off + d.n;
+    js.gate_max = d.cap_toi;
+  }
+  scan_tiles(c->ts, js, 1, d.g.nbuckets, c->st);
+  k_grid_fill<<<blocks(d.n), TPB, 0, c->st>>>(d, g);
 }
 
-static void broadphase(oavbd_ctx* c) {
+// The TOI clamp (mod.rs:882-978) for `m` casts: the exact count when the
+// host waited for it, else the buffer capacity (the device's count rules;
+// launches are sized for the capacity, the full casts from the last count
+// seen).
+static void toi_phase(oavbd_ctx* c, uint32_t m, uint32_t m_pair) {
+  Dev& d = c->d;
+  const uint32_t n = d.n;
+  k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
+  // the gate settles most casts; the survivors' count stays on the device,
+  // so the full casts run a grid-stride loop over a quarter-size grid
+  k_toi_gate<TOI_GATE_LANES><<<blocks((size_t)m * TOI_GATE_LANES), TPB, 0, c->st>>>(d);
+  k_toi_pair<TOI_LANES><<<std::max<uint32_t>(blocks((size_t)(m_pair + 3) / 4 * TOI_LANES), 1u), TPB, 0, c->st>>>(d);
+  k_toi_win<<<blocks(m), TPB, 0, c->st>>>(d);
+  k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
+  k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d, 1u);
+  build_grid(c, true);
+}
+
+static void broadphase(oavbd_ctx* c, bool async) {
   Dev& d = c->d;
   uint32_t n = d.n;
   k_bp_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d, c->cand_cnt);
@@ -800,32 +869,43 @@ static void broadphase(oavbd_ctx* c) {
     scan(c, d.lflag, d.lrank, d.g.lflat);  // one scan over all large bodies' flag rows
     k_bp_large_count<<<blocks(d.g.nlarge), TPB, 0, c->st>>>(d, c->cand_cnt);
   }
-  c->ncand = scan_total(c, c->cand_cnt, d.cand_off, n);
-  ensure_cand(c, c->ncand);
+  if (async) {
+    scan(c, c->cand_cnt, d.cand_off, n);
+  } else {
+    c->ncand = scan_total(c, c->cand_cnt, d.cand_off, n);
+    ensure_cand(c, c->ncand);
+  }
   k_bp_fill<<<blocks(n), TPB, 0, c->st>>>(d);
   if (d.g.nlarge) k_bp_large_fill<<<blocks(d.g.lflat), TPB, 0, c->st>>>(d);
 }
 
-static void narrow_and_merge(oavbd_ctx* c) {
+static void narrow_and_merge(oavbd_ctx* c, bool async) {
   Dev& d = c->d;
   std::swap(d.P, d.Pold);
   std::swap(d.poff, d.poff_old);
   std::swap(d.Q, d.Qold);
   std::swap(c->cap_pairs, c->cap_pairs_old);
   std::swap(c->cap_pts, c->cap_pts_old);
+  d.cap_pairs = (uint32_t)c->cap_pairs;
+  d.cap_pts = (uint32_t)c->cap_pts;
   d.npairs_old = d.npairs;
   d.npts_old = d.npts;
-  uint32_t nc = c->ncand;
+  const uint32_t nc = async ? (uint32_t)c->cap_cand : c->ncand;  // launch size
   if (nc) {
-    k_narrow_gate<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
-    k_narrow_dist<NARROW_LANES><<<blocks((size_t)nc * NARROW_LANES), TPB, 0, c->st>>>(d, nc);
-    k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d, nc);
+    k_narrow_gate<<<blocks(nc), TPB, 0, c->st>>>(d);
+    k_narrow_dist<NARROW_LANES><<<blocks((size_t)nc * NARROW_LANES), TPB, 0, c->st>>>(d);
+    k_narrow<<<blocks(nc), TPB, 0, c->st>>>(d);
   }
   // the accepted candidates' flags scan to pair indices; the pair -> candidate
   // map they also give lets the merge run over pairs only
+  if (async) {
+    scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc, d.pair_cand, 0, d.cand_off + d.n);
+    k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
+    k_merge<<<blocks((size_t)c->cap_pairs * K_MAX_POINTS), TPB, 0, c->st>>>(d);
+    return;
+  }
   const uint32_t seq = nc ? ++c->post_seq : 0u;
   scan2(c, d.C.acc, d.acc_off, d.C.npts, d.pt_off, nc, d.pair_cand, seq);
-  k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);
   if (nc) {
     post_wait(c, seq);  // the scan posted both totals
   } else {
@@ -843,17 +923,20 @@ static void narrow_and_merge(oavbd_ctx* c) {
   ensure_pairs(c, d.npairs);
   ensure_points(c, d.npts);
   ensure_incidence(c, d.npairs, d.npts);
+  k_pair_offsets<<<blocks(d.n + 1), TPB, 0, c->st>>>(d);  // sized tables: the totals fit
   if (d.npairs) k_merge<<<blocks((size_t)d.npairs * K_MAX_POINTS), TPB, 0, c->st>>>(d);
 }
 
-static void incidence_and_colour(oavbd_ctx* c) {
+// `gp`: the launch size of the per-pair kernels (the pair count, or the
+// table's capacity when the host has not read the count).
+static void incidence_and_colour(oavbd_ctx* c, uint32_t gp, uint32_t slots) {
   Dev& d = c->d;
   uint32_t n = d.n;
   k_colour_init<<<blocks(std::max<uint32_t>(n + 1, CSTAT_HEAD + 65536)), TPB, 0, c->st>>>(d);
-  if (d.npairs) k_inc_count<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  if (gp) k_inc_count<<<blocks(gp), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_count<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   scan3(c, d.ptcnt, d.inc_off, d.bcnt, d.boff, d.nbr_cnt, d.nbr_off, n);
-  if (d.npairs) k_inc_fill<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+  if (gp) k_inc_fill<<<blocks(gp), TPB, 0, c->st>>>(d);
   if (d.njoints) k_nbr_joint_fill<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
   k_inc_build<<<blocks((size_t)n * INC_LANES), TPB, 0, c->st>>>(d);
   // Jones-Plassmann colouring in passes.  The first `grid` passes run over
@@ -884,17 +967,17 @@ static void incidence_and_colour(oavbd_ctx* c) {
     j.nrun = CSTAT_HEAD + CSTAT_EAGER;
     c->colour_seq = post(c, j);
   }
-  k_colour_offsets<<<1, 64, 0, c->st>>>(d);
+  k_colour_offsets<<<1, 64, 0, c->st>>>(d, slots);
   k_colour_fill<<<blocks(n), TPB, 0, c->st>>>(d);
 }
 
-// The colouring check: pass and colour counts, colour sizes and offsets on
-// the host (the device made the same offsets).
-static void colour_wait(oavbd_ctx* c) {
+// The colouring check (the posted head of the colouring status): pass and
+// colour counts, colour sizes and offsets on the host (the device made the
+// same offsets).
+static void colour_check(oavbd_ctx* c) {
   Dev& d = c->d;
   const uint32_t* head = c->counters_h + HEAD_H;
   const int grid = c->grid_rounds;
-  post_wait(c, c->colour_seq);
   int rounds = -1, next_grid = -1;
   for (int r = 0; r < grid; ++r) {
     if (rounds < 0 && head[r] == 0) rounds = r + 1;
@@ -922,6 +1005,10 @@ static void colour_wait(oavbd_ctx* c) {
     c->stats.max_colour = std::max(c->stats.max_colour, c->colour_cnt_h[k]);
     c->solved += c->colour_cnt_h[k];
   }
+}
+static void colour_wait(oavbd_ctx* c) {
+  post_wait(c, c->colour_seq);
+  colour_check(c);
 }
 
 // A kernel launch on the context stream; with events (timing), the start
@@ -1000,6 +1087,35 @@ static int launch_primal(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent
   return v;
 }
 
+// Primal launch of colour class k whose range only the device holds
+// (k_primal8_class): the variant and the grid follow `est`, the class's
+// size at the last readback (the grid loops over whatever the class holds).
+// The LDS-staged variants map to the prefetching ones of the same
+// wavefront count.
+template <int W, int PF>
+static void launch_class_w(oavbd_ctx* c, uint32_t k, uint32_t est) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;
+  const uint32_t grid = std::max<uint32_t>(blocks(est + est / 8u + 1u, NB), 1u);
+  if (c->rows6)
+    k_primal8_class<W, 6, PF><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+  else
+    k_primal8_class<W, 4, PF><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+}
+static void launch_primal_class(oavbd_ctx* c, uint32_t k, uint32_t est) {
+  int v = c->force_variant;
+  if (v < 0) v = c->ptable[size_bucket(est)];
+  if (v < 0) v = est < PRIMAL_TAIL ? PV_P1 : (c->waves == 1 ? PV_W1 : c->waves == 2 ? PV_W2 : c->waves == 8 ? PV_W8 : PV_W4);
+  switch (v) {
+    case PV_W1: launch_class_w<1, 0>(c, k, est); break;
+    case PV_W2: launch_class_w<2, 0>(c, k, est); break;
+    case PV_W4: launch_class_w<4, 0>(c, k, est); break;
+    case PV_W8: launch_class_w<8, 0>(c, k, est); break;
+    case PV_L1: case PV_P1: launch_class_w<1, 1>(c, k, est); break;
+    case PV_L2: case PV_P2: launch_class_w<2, 1>(c, k, est); break;
+    default: launch_class_w<4, 1>(c, k, est); break;
+  }
+}
+
 // Wavefronts of k_primal_tail that fit on the device at once (the fused
 // tail launch is sized within this, so all its wavefronts are resident).
 static uint32_t tail_capacity(oavbd_ctx* c) {
@@ -1052,9 +1168,38 @@ static void record(oavbd_ctx* c, int i) {
   if (c->timing) HIPC(hipEventRecord(c->ev[i], c->st));
 }
 
+// The words a finished step reads back, gathered into the pinned buffer by
+// one workgroup: counters, the fused tail's error word, overflow words,
+// device counts and the TOI / candidate totals.
+struct Readback {
+  const uint32_t* src[6];
+  uint32_t nwords[6];
+  uint32_t dst[6];
+};
+__global__ void __launch_bounds__(64) k_readback(Readback r, uint32_t* buf) {
+  for (int j = 0; j < 6; ++j)
+    for (uint32_t i = threadIdx.x; i < r.nwords[j]; i += 64) buf[r.dst[j] + i] = r.src[j][i];
+}
+
+static void read_counters(oavbd_ctx* c) {
+  Dev& d = c->d;
+  Readback r{{d.counters, d.tail_err, d.ovf, d.dc, d.toi_off + d.n, d.cand_off + d.n},
+             {CTR_COUNT, 1u, OVF_WORDS, DC_WORDS, 1u, 1u},
+             {0u, TAILERR_H, OVF_H, DC_H, TOTALS_H, TOTALS_H + 1}};
+  k_readback<<<1, 64, 0, c->st>>>(r, c->counters_hd);
+  HIPC(hipStreamSynchronize(c->st));
+  if (c->counters_h[TAILERR_H]) throw HipError{"a fused primal tail launch timed out waiting for a partner"};
+}
+
 // One fixed substep.  `finish` reads the step's counters back (stats,
 // sleep count); other steps leave the stream running into the next one.
-static void step_inner(oavbd_ctx* c, bool finish) {
+// `async`: the host waits for no count inside the step -- launches are
+// sized from the buffer capacities and the last counts read back, every
+// count comes from the device, and a count past its buffer is recorded in
+// the overflow words (the caller repeats the steps); otherwise the host
+// reads each count as soon as it exists and sizes buffers and launches
+// exactly.
+static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
   Dev& d = c->d;
   uint32_t n = d.n;
   if (c->joints_dirty) upload_joints(c);
@@ -1071,34 +1216,26 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
   k_toi_count<<<blocks((size_t)n * BP_LANES), TPB, 0, c->st>>>(d);
-  uint32_t ncast = scan_total(c, d.toi_cnt, d.toi_off, n);
-  c->stats.toi_casts = ncast;
-  if (ncast) {
-    if (ncast > c->cap_toi) {
-      dfree(d.toi_pairs);
-      dfree(d.toi_res);
-      dfree(d.toi_live);
-      c->cap_toi = grow(ncast);
-      dalloc(&d.toi_pairs, c->cap_toi);
-      dalloc(&d.toi_res, c->cap_toi);
-      dalloc(&d.toi_live, c->cap_toi);
+  if (async) {
+    scan(c, d.toi_cnt, d.toi_off, n);
+    toi_phase(c, (uint32_t)c->cap_toi, std::min<uint32_t>(c->last_ncast + c->last_ncast / 4u + 256u, (uint32_t)c->cap_toi));
+  } else {
+    const uint32_t ncast = scan_total(c, d.toi_cnt, d.toi_off, n);
+    c->stats.toi_casts = ncast;
+    c->last_ncast = ncast;
+    if (ncast) {
+      ensure_toi(c, ncast);
+      toi_phase(c, ncast, ncast);
     }
-    k_toi_fill<<<blocks(n), TPB, 0, c->st>>>(d);
-    // the gate settles most casts; the survivors' count stays on the device,
-    // so the full casts run a grid-stride loop over a quarter-size grid
-    k_toi_gate<TOI_GATE_LANES><<<blocks((size_t)ncast * TOI_GATE_LANES), TPB, 0, c->st>>>(d, ncast);
-    k_toi_pair<TOI_LANES><<<blocks((size_t)(ncast + 3) / 4 * TOI_LANES), TPB, 0, c->st>>>(d);
-    k_toi_win<<<blocks(ncast), TPB, 0, c->st>>>(d, ncast);
-    k_toi_apply<<<blocks(n), TPB, 0, c->st>>>(d);
-    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
-    build_grid(c);
   }
-  broadphase(c);
-  narrow_and_merge(c);
+  broadphase(c, async);
+  narrow_and_merge(c, async);
+  // per-pair launches: the pair count, or the table's capacity
+  const uint32_t gp = async ? (uint32_t)c->cap_pairs : d.npairs;
   if (d.njoints == 0) {
     k_wake_inertial<<<blocks(n), TPB, 0, c->st>>>(d, may_sleep);
   } else {
-    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d);
+    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(d, 0u);
     if (may_sleep) {
       wake_joint_motion(c);
       k_external_wake<<<blocks(n), TPB, 0, c->st>>>(d);
@@ -1106,13 +1243,18 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     k_inertial<<<blocks(n), TPB, 0, c->st>>>(d);
   }
   if (d.njoints) k_joint_prep<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
-  if (d.npairs) k_pair_prep<<<blocks((size_t)d.npairs * PREP_LANES), TPB, 0, c->st>>>(d);
-  incidence_and_colour(c);
+  if (gp) k_pair_prep<<<blocks((size_t)gp * PREP_LANES), TPB, 0, c->st>>>(d);
+  incidence_and_colour(c, gp, async ? c->slots : 0xffffffffu);
   k_sb_pack<<<blocks(n), TPB, 0, c->st>>>(d);
   k_inertial_blocks<<<blocks(n), TPB, 0, c->st>>>(d);
   record(c, 1);
-  colour_wait(c);
-  const uint32_t ncol = c->stats.colours;
+  uint32_t ncol = 0;
+  if (async) {
+    ncol = c->slots;  // classes past the last one are empty launches
+  } else {
+    colour_wait(c);
+    ncol = c->stats.colours;
+  }
   // In timing mode every primal / dual launch is bracketed by pool events;
   // they are resolved after the step's final synchronisation.
   size_t np_ev = 0, nd_ev = 0;
@@ -1125,11 +1267,18 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     return pool[i];
   };
   if (c->rec) tape_begin(c);
-  const uint32_t ktail = tail_first(c, ncol);
-  c->stats.tail_colours = ncol - ktail;
+  const uint32_t ktail = async ? ncol : tail_first(c, ncol);
+  if (!async) c->stats.tail_colours = ncol - ktail;
+  // the dual's launch size: the point count, or the last one seen (its loop
+  // covers the device's count)
+  const uint32_t gq = async ? c->last_npts + c->last_npts / 8u : d.npts;
   for (uint32_t it = 0; it < c->prm.iters; ++it) {
     if (c->rec) tape_iter_start(c, it);
     for (uint32_t k = 0; k <= ktail && k < ncol; ++k) {
+      if (async) {
+        launch_primal_class(c, k, k < c->colour_cnt_h.size() ? c->colour_cnt_h[k] : 64u);
+        continue;
+      }
       const uint32_t cnt = k < ktail ? c->colour_cnt_h[k] : c->colour_off_h[ncol] - c->colour_off_h[k];
       if (!cnt) continue;
       hipEvent_t e0 = c->timing ? pool_event(c->pev, np_ev) : nullptr;
@@ -1149,11 +1298,12 @@ static void step_inner(oavbd_ctx* c, bool finish) {
     // the dual update, and the next sweep's inertial blocks (timed from the
     // first dual kernel's start to the last one's end)
     const uint32_t next = it + 1 < c->prm.iters ? 1u : 0u;
-    const bool dual_pts = d.npts || next, dual_j = d.njoints != 0;
+    const bool dual_pts = async || d.npts || next, dual_j = d.njoints != 0;
     hipEvent_t de0 = c->timing ? pool_event(c->dev_ev, nd_ev) : nullptr;
     hipEvent_t de1 = c->timing ? pool_event(c->dev_ev, nd_ev + 1) : nullptr;
     if (dual_pts)
-      launch_k(c, k_dual, blocks((size_t)d.npts + (next ? n : 0u)), TPB, de0, dual_j ? nullptr : de1, d, next);
+      launch_k(c, k_dual, std::max<uint32_t>(blocks((size_t)gq + (next ? n : 0u)), 1u), TPB, de0,
+               dual_j ? nullptr : de1, d, next);
     if (dual_j) launch_k(c, k_dual_joint, blocks(d.njoints), TPB, dual_pts ? nullptr : de0, de1, d);
     if (c->timing && !dual_pts && !dual_j) {
       HIPC(hipEventRecord(de0, c->st));
@@ -1164,12 +1314,12 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   record(c, 3);
   if (d.njoints == 0) {
     k_finish_bodies<<<blocks(n), TPB, 0, c->st>>>(d);
-    if (d.npairs) k_supported<<<blocks(d.npairs), TPB, 0, c->st>>>(d);
+    if (gp) k_supported<<<blocks(gp), TPB, 0, c->st>>>(d);
     k_sleep_bodies<<<blocks(n), TPB, 0, c->st>>>(d);
   } else {
     k_sb_unpack<<<blocks(n), TPB, 0, c->st>>>(d);
     k_bdf1<<<blocks(n), TPB, 0, c->st>>>(d);
-    uint32_t ns = std::max(d.npairs, d.njoints);
+    uint32_t ns = std::max(gp, d.njoints);
     if (ns) k_supported<<<blocks(ns), TPB, 0, c->st>>>(d);
     k_ready_init<<<blocks(n), TPB, 0, c->st>>>(d);
     k_ready_joints<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
@@ -1180,15 +1330,39 @@ static void step_inner(oavbd_ctx* c, bool finish) {
   }
   record(c, 4);
   if (c->rec) tape_end(c);
-  c->stats.pairs = d.npairs;
-  c->stats.points = d.npts;
-  c->stats.candidates = c->ncand;
-  c->stats.solved = c->solved;
+  if (!async) {
+    c->stats.pairs = d.npairs;
+    c->stats.points = d.npts;
+    c->stats.candidates = c->ncand;
+    c->stats.solved = c->solved;
+    c->last_ncand = c->ncand;
+    c->last_npairs = d.npairs;
+    c->last_npts = d.npts;
+  }
   if (!finish && !c->timing) {
     c->may_sleep = true;  // the sleep count is not read back: wake passes run next step
     return;
   }
   read_counters(c);
+  if (async) {
+    // the last step's counts (the caller checks the overflow words first)
+    const uint32_t* h = c->counters_h;
+    d.npairs = h[DC_H + DC_NPAIRS];
+    d.npts = h[DC_H + DC_NPTS];
+    c->ncand = h[TOTALS_H + 1];
+    c->last_ncast = h[TOTALS_H];
+    c->last_ncand = c->ncand;
+    c->last_npairs = d.npairs;
+    c->last_npts = d.npts;
+    c->stats.toi_casts = h[TOTALS_H];
+    colour_check(c);  // the posted colouring status has landed
+    c->stats.tail_colours = 0;
+    c->stats.pairs = d.npairs;
+    c->stats.points = d.npts;
+    c->stats.candidates = c->ncand;
+    c->stats.solved = c->solved;
+  }
+  c->sizes_known = true;
   c->sleeping = c->counters_h[CTR_SLEEPING];
   c->may_sleep = c->sleeping != 0;
   c->stats.sleeping = c->sleeping;
@@ -1543,6 +1717,7 @@ static void set_bodies_impl(oavbd_ctx* c, const BodyInputs& in) {
     c->may_sleep = false;
     c->time_debt = 0.0f;
     c->stats = oavbd_stats{};
+    c->sizes_known = false;  // the next steps wait for their counts until one has been read back
     ensure_pairs(c, 1024);
     ensure_points(c, 4096);
     ensure_incidence(c, 1024, 4096);
@@ -1851,7 +2026,7 @@ int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
       std::vector<uint8_t> flags(hb.asleep);
       h2d(d.b.flag_a, flags, flags.size());
       HIPC(hipDeviceSynchronize());
-      k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d);
+      k_apply_wake<<<blocks(d.n), TPB, 0, c->st>>>(d, 0u);
       HIPC(hipStreamSynchronize(c->st));
       c->sleeping = 0;
     }
@@ -1981,7 +2156,7 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
     if (a < n) HIPC(hipMemcpy(c->d.b.flag_a + a, &one, 1, hipMemcpyHostToDevice));
     if (b < n) HIPC(hipMemcpy(c->d.b.flag_a + b, &one, 1, hipMemcpyHostToDevice));
     HIPC(hipDeviceSynchronize());
-    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(c->d);
+    k_apply_wake<<<blocks(n), TPB, 0, c->st>>>(c->d, 0u);
     HIPC(hipStreamSynchronize(c->st));
   });
   if (a == b || a >= n || b >= n) return fail(c, OAVBD_EINVAL, "bad joint bodies");
@@ -2027,11 +2202,103 @@ int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3]
   return oavbd_add_joint(c, JK_BALL, a, b, la, lb, nullptr, nullptr, out_index);
 }
 
+// Both tables of a double-buffered pair (or point) table hold at least
+// `need` entries: the stale one is reallocated, the current one (the last
+// step's, the next step's merge source) moved into a larger allocation.
+static void ensure_pair_tables(oavbd_ctx* c, size_t need) {
+  Dev& d = c->d;
+  if (need > c->cap_pairs_old || !d.Pold.ab) {
+    free_pairs(d.Pold);
+    c->cap_pairs_old = grow(need);
+    alloc_pairs(d.Pold, c->cap_pairs_old);
+  }
+  if (need > c->cap_pairs || !d.P.ab) {
+    Pairs P{};
+    const size_t cap = grow(need);
+    alloc_pairs(P, cap);
+    if (d.P.ab) {
+      dcopy(c, P.ab, d.P.ab, d.npairs);
+      dcopy(c, P.rec, d.P.rec, 4 * (size_t)d.npairs);
+      dcopy(c, P.pts, d.P.pts, d.npairs);
+      HIPC(hipStreamSynchronize(c->st));
+    }
+    free_pairs(d.P);
+    d.P = P;
+    c->cap_pairs = cap;
+  }
+  d.cap_pairs = (uint32_t)c->cap_pairs;
+}
+static void ensure_point_tables(oavbd_ctx* c, size_t need) {
+  Dev& d = c->d;
+  if (need > c->cap_pts_old || !d.Qold.rec) {
+    free_points(d.Qold);
+    c->cap_pts_old = grow(need);
+    alloc_points(d.Qold, c->cap_pts_old);
+  }
+  if (need > c->cap_pts || !d.Q.rec) {
+    Points Q{};
+    const size_t cap = grow(need);
+    alloc_points(Q, cap);
+    if (d.Q.rec) {
+      dcopy(c, Q.rec, d.Q.rec, 4 * (size_t)d.npts);
+      dcopy(c, Q.roll, d.Q.roll, d.npts);
+      dcopy(c, Q.pair, d.Q.pair, d.npts);
+      dcopy(c, Q.src, d.Q.src, d.npts);
+      HIPC(hipStreamSynchronize(c->st));
+    }
+    free_points(d.Q);
+    d.Q = Q;
+    c->cap_pts = cap;
+  }
+  d.cap_pts = (uint32_t)c->cap_pts;
+}
+
+// Buffers of an unwaited batch: every count-sized buffer a quarter above
+// the last counts read back (steps change them slowly; a batch that still
+// outgrows one is repeated), the colour classes queued per sweep one above
+// the last colour count, the overflow words cleared.
+static void async_prepare(oavbd_ctx* c) {
+  Dev& d = c->d;
+  auto room = [](uint32_t x) { return (size_t)x + x / 4u + 1024u; };
+  ensure_toi(c, room(c->last_ncast));
+  ensure_cand(c, room(c->last_ncand));
+  ensure_pair_tables(c, room(c->last_npairs));
+  ensure_point_tables(c, room(c->last_npts));
+  ensure_incidence(c, std::max(c->cap_pairs, c->cap_pairs_old), std::max(c->cap_pts, c->cap_pts_old));
+  c->slots = (uint32_t)c->colour_cnt_h.size() + c->slot_spare;
+  zero_u32(c, d.ovf, OVF_WORDS);
+}
+
+// `nsteps` substeps.  Once a step's counts have been read back (the first
+// step after an upload waits for them), the steps of a call run unwaited
+// from a device checkpoint; a batch in which some count outgrew its buffer
+// is restored and repeated with the host reading every count, which gives
+// the same bits (the checkpoint holds everything a step reads).  Recorded
+// (differentiable), timed and tuning steps always wait for their counts.
+static void step_batch(oavbd_ctx* c, uint32_t nsteps) {
+  if (!nsteps) return;
+  if (c->joints_dirty) upload_joints(c);
+  const bool async = c->async_enabled && c->sizes_known && !c->rec && !c->timing && !c->tuning &&
+                     c->force_variant < 0;
+  c->stats.unwaited = async ? 1u : 0u;
+  if (!async) {
+    for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps);
+    return;
+  }
+  async_prepare(c);
+  snap_save(c, c->bsnap);
+  for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps, true);
+  if (c->counters_h[OVF_H + OVF_FLAGS] == 0u) return;
+  ++c->replays;
+  snap_load(c, c->bsnap);
+  for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps);
+}
+
 int oavbd_step_n(oavbd_ctx* c, uint32_t nsteps) {
   if (!c) return OAVBD_EINVAL;
   GUARD(c, {
     HIPC(hipSetDevice(c->device));
-    for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps);
+    step_batch(c, nsteps);
   });
   return OAVBD_OK;
 }
@@ -2203,6 +2470,15 @@ int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   if (!c || !out) return OAVBD_EINVAL;
   *out = c->stats;
   out->joints = (uint32_t)c->joints.size();
+  out->replays = c->replays;
+  return OAVBD_OK;
+}
+
+int oavbd_set_unwaited(oavbd_ctx* c, int enable, uint32_t spare_colours) {
+  if (!c) return OAVBD_EINVAL;
+  if (spare_colours > 64u) return fail(c, OAVBD_EINVAL, "spare_colours above 64");
+  c->async_enabled = enable != 0;
+  c->slot_spare = spare_colours;
   return OAVBD_OK;
 }
 
