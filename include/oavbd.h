@@ -307,6 +307,49 @@ int oavbd_get_primal_table(oavbd_ctx* ctx, int32_t* table, uint32_t n);
  * reference sweeps bodies serially, mod.rs:1052-1059). */
 int oavbd_set_primal_tail(oavbd_ctx* ctx, uint32_t max_bodies, uint32_t max_waves);
 
+/* Contact and trigger events.  Replaces: PhysicsEngine::drain_contact_events
+ * / drain_trigger_events (crates/physics/src/avbd/mod.rs:1591-1597) with
+ * the events emit_events produces (mod.rs:1606-1669) and the TOI clamp's
+ * impact events (mod.rs:943-960, :1126-1136).  Off by default (each
+ * reported step then waits for its counts and copies the touching pairs
+ * back); oavbd_set_events(ctx, 1) turns them on.  The last substep of each
+ * oavbd_step / oavbd_step_n call reports: trigger pairs whose overlap
+ * began or ended since the last report (Entered / Exited, pairs ascending),
+ * contact pairs that started or stopped touching by more than 5 mm (Begin /
+ * End, ascending), then Hit for newly touching pairs closing faster than
+ * 1 m/s (pair order: ascending (a, b)); every substep adds the TOI clamp's
+ * impacts above 1 m/s (movers ascending), those of a pair already reported
+ * as a hit in the same substep dropped.  Hit carries the contact point,
+ * the normal from body a towards body b and the closing speed. */
+typedef struct oavbd_event {
+  uint32_t kind;      /* OAVBD_EVENT_*                                      */
+  uint32_t a, b;      /* body handles, a < b                                */
+  float point[3];     /* Hit: contact point (world)                         */
+  float normal[3];    /* Hit: unit normal from a towards b                  */
+  float approach;     /* Hit: closing speed along the normal (m/s, > 1)    */
+} oavbd_event;
+enum {
+  OAVBD_EVENT_BEGIN = 0,
+  OAVBD_EVENT_END = 1,
+  OAVBD_EVENT_HIT = 2,
+  OAVBD_EVENT_ENTERED = 3, /* trigger events */
+  OAVBD_EVENT_EXITED = 4
+};
+int oavbd_set_events(oavbd_ctx* ctx, int enable);
+/* which = 0: contact events, 1: trigger events.  With out = NULL, *n is the
+ * number queued; otherwise the oldest min(cap, queued) are copied to out,
+ * removed from the queue, and *n is their number. */
+int oavbd_drain_events(oavbd_ctx* ctx, int which, oavbd_event* out, uint32_t cap, uint32_t* n);
+/* The last report's touching (which = 0) or overlapping trigger (1) pairs,
+ * ascending (a, b) pairs: the solver-migration event baseline.  Replaces:
+ * AvbdEngine::event_state / restore_event_state (mod.rs:516-556); a host
+ * that rebuilds a context restores it so no Begin / Entered repeats.
+ * oavbd_remove_body reports Exited for h's trigger pairs, renumbers the
+ * rest and drops the touch baseline and the undrained contact events
+ * (mod.rs:1240-1290). */
+int oavbd_get_event_state(oavbd_ctx* ctx, int which, uint32_t* ab, uint32_t cap, uint32_t* n);
+int oavbd_set_event_state(oavbd_ctx* ctx, int which, const uint32_t* ab, uint32_t n);
+
 /* Unwaited stepping (off by default: one unwaited batch that outgrew its
  * buffers ended in a GPU memory fault whose cause has not been found, so
  * the path stays disabled until it is).  Once one step's counts have been

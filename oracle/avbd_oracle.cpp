@@ -144,6 +144,16 @@ struct Point {
   int src_pair = -1, src_pt = -1;  // point of the previous step's pair vector inherited from
 };
 
+// A contact or trigger event (ContactEvent / TriggerEvent, trigger.rs:29-81):
+// kind EV_*, the pair, and for a hit the point, normal and approach speed.
+enum EventKind : int { EV_BEGIN = 0, EV_END = 1, EV_HIT = 2, EV_ENTERED = 3, EV_EXITED = 4 };
+template <class T>
+struct Event {
+  int kind, a, b;
+  V3<T> point, normal;
+  T approach;
+};
+
 template <class T>
 struct Pair {
   int a, b;
@@ -612,6 +622,11 @@ struct Engine {
   // last step's TOI clamps: distance travelled to the hit (-1: none), target
   std::vector<T> toi_t;
   std::vector<int> toi_tgt;
+  // events (mod.rs:424-464 prev_touch / prev_trigger / pre_vel, the drained
+  // queues; emit_events mod.rs:1606-1669)
+  std::vector<V3<T>> pre_vel;
+  std::vector<Event<T>> contact_events, trigger_events, ccd_hits;
+  std::vector<std::pair<int, int>> prev_touch, prev_trigger, trigger_now;
 
   // Frozen active set (test support for the differentiable GPU step): a
   // recording run keeps, per step, the pair vector (normals, gaps, anchors,
@@ -708,6 +723,24 @@ struct Engine {
     const int n = (int)bodies.size();
     if (hi < 0 || hi >= n) return;
     const int last = n - 1;
+    // exiting trigger pairs report Exited (mod.rs:1246-1262); touch state
+    // and undrained contact events are dropped, trigger state remapped
+    // (mod.rs:1278-1286)
+    for (const auto& ab : prev_trigger)
+      if (ab.first == hi || ab.second == hi)
+        trigger_events.push_back(Event<T>{EV_EXITED, ab.first, ab.second, {}, {}, T(0)});
+    {
+      auto mp = [&](int h) { return h == last ? hi : h; };
+      std::vector<std::pair<int, int>> kept;
+      for (const auto& ab : prev_trigger)
+        if (ab.first != hi && ab.second != hi)
+          kept.push_back({std::min(mp(ab.first), mp(ab.second)), std::max(mp(ab.first), mp(ab.second))});
+      std::sort(kept.begin(), kept.end());
+      kept.erase(std::unique(kept.begin(), kept.end()), kept.end());
+      prev_trigger = std::move(kept);
+      prev_touch.clear();
+      contact_events.clear();
+    }
     auto swap_rm = [&](auto& v) {
       if (hi < (int)v.size()) {
         v[hi] = v.back();
@@ -1036,6 +1069,14 @@ struct Engine {
   }
 
   void apply_toi_hit(int h, const Hit& hit) {
+    {  // the clamp's impact event (mod.rs:947-960)
+      const T approach = -dot(pre_vel[h] - pre_vel[hit.handle], hit.normal);
+      if (approach > T(1)) {
+        const bool lo = h < hit.handle;
+        ccd_hits.push_back(Event<T>{EV_HIT, lo ? h : hit.handle, lo ? hit.handle : h, hit.point,
+                                    lo ? -hit.normal : hit.normal, approach});
+      }
+    }
     Body<T>& b = bodies[h];
     V3<T> disp = b.vel * K<T>::DT();
     T len = fmax_(length(disp), K<T>::MIN_TOI_LEN());
@@ -1341,9 +1382,13 @@ struct Engine {
                      [](const Disc& x, const Disc& y) { return x.ia != y.ia ? x.ia < y.ia : x.ib < y.ib; });
     std::vector<uint8_t> seen(pairs.size(), 0);
     std::vector<std::pair<int, int>> wake;
+    trigger_now.clear();
     for (auto& dd : disc) {
       int ia = dd.ia, ib = dd.ib;
-      if (dd.trigger) continue;
+      if (dd.trigger) {  // rows.rs:803-807: overlapping trigger pairs
+        if (dd.signed_ <= T(0)) trigger_now.push_back({ia, ib});
+        continue;
+      }
       if (dd.exists) {
         int idx = -1;
         if (use_index) {
@@ -2149,10 +2194,67 @@ struct Engine {
     }
   }
 
-  void step_inner() {
+  // emit_events (mod.rs:1606-1669): trigger transitions, then contact
+  // Begin / End (both in pair order), then hits of newly touching pairs in
+  // pair-vector order.
+  static void set_transitions(const std::vector<std::pair<int, int>>& now, const std::vector<std::pair<int, int>>& prev,
+                              int k_in, int k_out, std::vector<Event<T>>& out) {
+    size_t i = 0, j = 0;
+    while (i < now.size() || j < prev.size()) {
+      if (j == prev.size() || (i < now.size() && now[i] < prev[j])) {
+        out.push_back(Event<T>{k_in, now[i].first, now[i].second, {}, {}, T(0)});
+        ++i;
+      } else if (i == now.size() || prev[j] < now[i]) {
+        out.push_back(Event<T>{k_out, prev[j].first, prev[j].second, {}, {}, T(0)});
+        ++j;
+      } else {
+        ++i;
+        ++j;
+      }
+    }
+  }
+  void emit_events() {
+    set_transitions(trigger_now, prev_trigger, EV_ENTERED, EV_EXITED, trigger_events);
+    prev_trigger = trigger_now;
+    std::vector<std::pair<int, int>> touch_now;
+    std::vector<Event<T>> hits;
+    for (const auto& p : pairs) {
+      const Body<T>& a = bodies[p.a];
+      const Body<T>& b = bodies[p.b];
+      T deepest = T(0);
+      V3<T> witness = (a.pos + b.pos) * T(0.5);
+      for (const auto& pt : p.pts) {
+        const V3<T> pa = a.pos + rotate(a.rot, pt.ra);
+        const V3<T> pb = b.pos + rotate(b.rot, pt.rb);
+        const T gap = dot(p.n, pa - pb);
+        if (-gap > deepest) {
+          deepest = -gap;
+          witness = pa;
+        }
+      }
+      if (deepest > T(0.005f)) {  // CONTACT_BEGIN_SLOP (trigger.rs:68)
+        touch_now.push_back({p.a, p.b});
+        if (!std::binary_search(prev_touch.begin(), prev_touch.end(), std::make_pair(p.a, p.b))) {
+          const T approach = -dot(pre_vel[p.a] - pre_vel[p.b], p.n);
+          if (approach > T(1)) hits.push_back(Event<T>{EV_HIT, p.a, p.b, witness, -p.n, approach});
+        }
+      }
+    }
+    std::sort(touch_now.begin(), touch_now.end());
+    set_transitions(touch_now, prev_touch, EV_BEGIN, EV_END, contact_events);
+    for (auto& e : hits) contact_events.push_back(e);
+    prev_touch = std::move(touch_now);
+  }
+
+  // One substep; `emit`: the last substep of a step call, which reports
+  // contact and trigger transitions (mod.rs:1123-1125); TOI impacts are
+  // reported by every substep (mod.rs:1126-1136).
+  void step_inner(bool emit = true) {
     const T DT = K<T>::DT();
     int n = (int)bodies.size();
     stats = Stats{};
+    const size_t event_start = contact_events.size();
+    ccd_hits.clear();
     wake_joint_motion();
     for (int h = 0; h < n; ++h) {
       if (!solvable(h)) continue;
@@ -2166,6 +2268,8 @@ struct Engine {
       b.torque = V3<T>{T(0), T(0), T(0)};
       b.angvel = b.angvel + mat3_vec(iwi, torque) * DT;
     }
+    pre_vel.resize(n);
+    for (int h = 0; h < n; ++h) pre_vel[h] = bodies[h].vel;  // mod.rs:874-876
     toi_clamp();
     generate_pairs();
     wake_joint_motion();
@@ -2283,6 +2387,15 @@ struct Engine {
     stats.pairs = (int)pairs.size();
     for (auto& p : pairs) stats.points += (int)p.pts.size();
     for (int h = 0; h < n; ++h) stats.sleeping += asleep[h] ? 1 : 0;
+    if (emit) emit_events();
+    for (const auto& hit : ccd_hits) {
+      bool dup = false;
+      for (size_t i = event_start; i < contact_events.size(); ++i) {
+        const auto& e = contact_events[i];
+        dup = dup || (e.kind == EV_HIT && e.a == hit.a && e.b == hit.b);
+      }
+      if (!dup) contact_events.push_back(hit);
+    }
   }
 
   // PhysicsEngine::step (mod.rs:1212-1235).
@@ -2294,7 +2407,7 @@ struct Engine {
     while (time_debt >= DT && k < MAX_SUBSTEPS) {
       time_debt -= DT;
       ++k;
-      step_inner();
+      step_inner(time_debt < DT || k == MAX_SUBSTEPS);
     }
   }
 };
@@ -2455,11 +2568,12 @@ void orc_step(void* h, float dt, int nsteps) {
 
 void orc_step_inner(void* h, int nsteps) {
   OracleCtx* c = (OracleCtx*)h;
+  // like a step call of nsteps substeps: the last one reports transitions
   for (int i = 0; i < nsteps; ++i) DISPATCH(c, {
       if (E->freeze_mode == 2)
         E->step_frozen();
       else
-        E->step_inner();
+        E->step_inner(i + 1 == nsteps);
     });
 }
 
@@ -2551,6 +2665,57 @@ void orc_get_sleep(void* h, uint8_t* asleep, float* timer) {
       if (asleep) asleep[i] = E->asleep[i];
       if (timer) timer[i] = (float)E->sleep_timer[i];
     }
+  });
+}
+
+// Events queued since the last drain: which = 0 contact, 1 trigger.  With
+// out arrays, copies all of them (ints: kind, a, b; floats: point3,
+// normal3, approach) and empties the queue; returns the count.
+int orc_drain_events(void* h, int which, int32_t* out_i, float* out_f) {
+  OracleCtx* c = (OracleCtx*)h;
+  int cnt = 0;
+  DISPATCH(c, {
+    auto& q = which ? E->trigger_events : E->contact_events;
+    cnt = (int)q.size();
+    if (out_i) {
+      for (int k = 0; k < cnt; ++k) {
+        const auto& e = q[k];
+        out_i[3 * k] = e.kind;
+        out_i[3 * k + 1] = e.a;
+        out_i[3 * k + 2] = e.b;
+        const float f[7] = {float(e.point.x), float(e.point.y), float(e.point.z), float(e.normal.x),
+                            float(e.normal.y), float(e.normal.z), float(e.approach)};
+        for (int m = 0; m < 7; ++m) out_f[7 * k + m] = f[m];
+      }
+      q.clear();
+    }
+  });
+  return cnt;
+}
+
+// The completed-step event baseline (event_state / restore_event_state,
+// mod.rs:516-556): which = 0 touching pairs, 1 overlapping trigger pairs.
+int orc_get_event_state(void* h, int which, uint32_t* ab) {
+  OracleCtx* c = (OracleCtx*)h;
+  int cnt = 0;
+  DISPATCH(c, {
+    const auto& v = which ? E->prev_trigger : E->prev_touch;
+    cnt = (int)v.size();
+    if (ab)
+      for (int k = 0; k < cnt; ++k) {
+        ab[2 * k] = (uint32_t)v[k].first;
+        ab[2 * k + 1] = (uint32_t)v[k].second;
+      }
+  });
+  return cnt;
+}
+void orc_set_event_state(void* h, int which, const uint32_t* ab, int cnt) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    auto& v = which ? E->prev_trigger : E->prev_touch;
+    v.clear();
+    for (int k = 0; k < cnt; ++k) v.push_back({(int)ab[2 * k], (int)ab[2 * k + 1]});
+    std::sort(v.begin(), v.end());
   });
 }
 

@@ -78,6 +78,11 @@ def lib():
         L.orc_tangent_basis_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_shape_distance_f32.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_drain_events.restype = ctypes.c_int
+        L.orc_drain_events.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _f32p]
+        L.orc_get_event_state.restype = ctypes.c_int
+        L.orc_get_event_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p]
+        L.orc_set_event_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_int]
         L.orc_set_freeze.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_frozen_steps.restype = ctypes.c_int
         L.orc_frozen_steps.argtypes = [ctypes.c_void_p]
@@ -325,6 +330,30 @@ class OracleEngine:
         tg = np.zeros(self.n, np.int32)
         self._L.orc_get_toi(self.h, _p(t, _f32p), _p(tg, _i32p))
         return t, tg
+
+    def drain_events(self, trigger=False):
+        """Queued contact (or trigger) events, oldest first: (kinds, ab
+        (k, 2) int32, point (k, 3), normal (k, 3), approach (k,)) float32;
+        the queue is emptied."""
+        w = 1 if trigger else 0
+        k = self._L.orc_drain_events(self.h, w, None, None)
+        ii = np.zeros((k, 3), np.int32)
+        ff = np.zeros((k, 7), np.float32)
+        if k:
+            self._L.orc_drain_events(self.h, w, _p(ii, _i32p), _p(ff, _f32p))
+        return ii[:, 0].copy(), ii[:, 1:3].copy(), ff[:, 0:3].copy(), ff[:, 3:6].copy(), ff[:, 6].copy()
+
+    def event_state(self, trigger=False):
+        w = 1 if trigger else 0
+        k = self._L.orc_get_event_state(self.h, w, None)
+        ab = np.zeros((k, 2), np.uint32)
+        if k:
+            self._L.orc_get_event_state(self.h, w, _p(ab, _u32p))
+        return ab
+
+    def set_event_state(self, ab, trigger=False):
+        ab = np.ascontiguousarray(ab, dtype=np.uint32).reshape(-1, 2)
+        self._L.orc_set_event_state(self.h, 1 if trigger else 0, _p(ab, _u32p), len(ab))
 
     def stats(self):
         out = np.zeros(7, np.int32)

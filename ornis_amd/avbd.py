@@ -39,6 +39,14 @@ class OavbdStats(ctypes.Structure):
         "tail_colours", "replays", "unwaited")]
 
 
+class OavbdEvent(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32),
+                ("point", ctypes.c_float * 3), ("normal", ctypes.c_float * 3), ("approach", ctypes.c_float)]
+
+
+EVENT_BEGIN, EVENT_END, EVENT_HIT, EVENT_ENTERED, EVENT_EXITED = 0, 1, 2, 3, 4
+
+
 class OavbdKernelTimes(ctypes.Structure):
     _fields_ = [("primal_ms", ctypes.c_double), ("dual_ms", ctypes.c_double), ("primal_bytes", ctypes.c_double),
                 ("dual_bytes", ctypes.c_double), ("primal_launches", ctypes.c_uint64),
@@ -96,6 +104,10 @@ SIGNATURES = {
     "oavbd_get_primal_table": (ctypes.c_int, [ctypes.c_void_p, _i32p, ctypes.c_uint32]),
     "oavbd_set_primal_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_set_unwaited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]),
+    "oavbd_set_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "oavbd_drain_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, _u32p]),
+    "oavbd_get_event_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32, _u32p]),
+    "oavbd_set_event_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32]),
     "oavbd_solve_diag": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_void_p]),
     "oavbd_solve_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
@@ -476,6 +488,43 @@ class AvbdEngine:
         self._unwaited = (1 if on else 0, int(spare_colours))
         if self._ctx:
             self._check(self._L.oavbd_set_unwaited(self._ctx, *self._unwaited))
+
+    def set_events(self, on=True):
+        """Contact and trigger events (off by default)."""
+        self._check(self._L.oavbd_set_events(self._ctx, 1 if on else 0))
+
+    def drain_events(self, trigger=False):
+        """Queued contact (or trigger) events, oldest first, as arrays:
+        (kinds, ab (k, 2), point (k, 3), normal (k, 3), approach (k,)); the
+        queue is emptied."""
+        w = 1 if trigger else 0
+        n = ctypes.c_uint32(0)
+        self._check(self._L.oavbd_drain_events(self._ctx, w, None, 0, ctypes.byref(n)))
+        k = n.value
+        buf = (OavbdEvent * max(k, 1))()
+        if k:
+            self._check(self._L.oavbd_drain_events(self._ctx, w, ctypes.cast(buf, ctypes.c_void_p), k,
+                                                   ctypes.byref(n)))
+        ev = buf[:k]
+        kinds = np.array([e.kind for e in ev], np.uint32)
+        ab = np.array([(e.a, e.b) for e in ev], np.uint32).reshape(-1, 2)
+        pt = np.array([list(e.point) for e in ev], np.float32).reshape(-1, 3)
+        nrm = np.array([list(e.normal) for e in ev], np.float32).reshape(-1, 3)
+        app = np.array([e.approach for e in ev], np.float32)
+        return kinds, ab, pt, nrm, app
+
+    def event_state(self, trigger=False):
+        w = 1 if trigger else 0
+        n = ctypes.c_uint32(0)
+        self._check(self._L.oavbd_get_event_state(self._ctx, w, None, 0, ctypes.byref(n)))
+        ab = np.zeros((n.value, 2), np.uint32)
+        if n.value:
+            self._check(self._L.oavbd_get_event_state(self._ctx, w, _p(ab, _u32p), n.value, ctypes.byref(n)))
+        return ab
+
+    def set_event_state(self, ab, trigger=False):
+        ab = np.ascontiguousarray(ab, dtype=np.uint32).reshape(-1, 2)
+        self._check(self._L.oavbd_set_event_state(self._ctx, 1 if trigger else 0, _p(ab, _u32p), len(ab)))
 
     def set_timing(self, on=True):
         self._check(self._L.oavbd_set_timing(self._ctx, 1 if on else 0))

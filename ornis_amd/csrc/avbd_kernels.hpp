@@ -258,7 +258,7 @@ __device__ __forceinline__ bool toi_target_ok(const Dev& d, uint32_t h, uint32_t
 // distance: the expensive feature-distance pass is skipped and the outcome
 // is unchanged.
 template <bool BOXES = true>
-HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* normal) {
+HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* normal, f3* point) {
   const bool bb = BOXES && mover.kind == SH_BOX && target.kind == SH_BOX;
   float slack = 0.0f;
   if (bb) {
@@ -285,6 +285,7 @@ HD float cast_one(const shp& mover, f3 dir, float len, const shp& target, f3* no
     if (r.dist <= K_SHAPE_TOUCH) {
       if (t > 0.0f) {
         *normal = normalize_or(sub(r.pa, r.pb), neg(dir));
+        *point = r.pb;
         return t;
       }
       return -1.0f;
@@ -374,6 +375,7 @@ __global__ void __launch_bounds__(TPB) k_toi_count(Dev d) {
   if (h < d.n && s == 0) {
     d.toi_hit[h] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);  // w = travel to the hit; < 0: none
     d.toi_best[h] = ~0ull;
+    if (d.ev_on) d.pre_vel[h] = d.b.vel[h];  // mod.rs:874-876: after the torque, before the clamp
   }
   bool moving = false;
   if (h < d.n && d.fast[h]) {
@@ -474,8 +476,8 @@ __device__ __forceinline__ bool cast_bound(const shp& mover, f3 dir, float len, 
 }
 
 template <int G>
-__device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, int s) {
-  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one<false>(mover, dir, len, target, normal);
+__device__ float cast_group(const shp& mover, f3 dir, float len, const shp& target, f3* normal, f3* point, int s) {
+  if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) return cast_one<false>(mover, dir, len, target, normal, point);
   const float slack = cast_slack(mover, len, target);
   float t = 0.0f;
   for (int it = 0; it < 24; ++it) {
@@ -487,6 +489,7 @@ __device__ float cast_group(const shp& mover, f3 dir, float len, const shp& targ
     if (r.dist <= K_SHAPE_TOUCH) {
       if (t > 0.0f) {
         *normal = normalize_or(sub(r.pa, r.pb), neg(dir));
+        *point = r.pb;
         return t;
       }
       return -1.0f;
@@ -499,8 +502,9 @@ __device__ float cast_group(const shp& mover, f3 dir, float len, const shp& targ
 
 // A cast's outcome: its (normal, t) is kept and the mover keeps the
 // smallest (t, target) key: t > 0, so its IEEE bits order like the value.
-__device__ __forceinline__ void toi_result(const Dev& d, uint32_t i, uint2 ho, f3 nrm, float t) {
+__device__ __forceinline__ void toi_result(const Dev& d, uint32_t i, uint2 ho, f3 nrm, f3 pt, float t) {
   d.toi_res[i] = pk(nrm, t);
+  d.toi_pt[i] = pk(pt, 0.0f);
   if (t < 0.0f) return;
   atomicMin(&d.toi_best[ho.x], toi_key(t, ho.y));
 }
@@ -525,17 +529,17 @@ __global__ void __launch_bounds__(TPB) k_toi_gate(Dev d) {
     const float len = len3(delta);
     const f3 dir = divs(delta, len);
     const shp mover = body_shape(d, ho.x), target = body_shape(d, ho.y);
-    f3 nrm = zero3();
+    f3 nrm = zero3(), pt = zero3();
     float t = -1.0f;
     if (!(mover.kind == SH_BOX && target.kind == SH_BOX)) {
-      t = cast_one<false>(mover, dir, len, target, &nrm);
+      t = cast_one<false>(mover, dir, len, target, &nrm, &pt);
     } else {
       shp m;
       f3 sn;
       float gap;
       live = !cast_bound<G>(mover, dir, len, target, 0.0f, cast_slack(mover, len, target), s, &m, &sn, &gap);
     }
-    if (s == 0 && !live) toi_result(d, i, ho, nrm, t);
+    if (s == 0 && !live) toi_result(d, i, ho, nrm, pt, t);
   }
   const uint32_t slot = (live && s == 0) ? atomicAdd(&s_n, 1u) : 0u;
   __syncthreads();
@@ -557,9 +561,9 @@ __global__ void __launch_bounds__(TPB) k_toi_pair(Dev d) {
     const f3 delta = mul(xyz(d.b.vel[ho.x]), K_DT);
     const float len = len3(delta);
     const f3 dir = divs(delta, len);
-    f3 nrm = zero3();
-    const float t = cast_group<G>(body_shape(d, ho.x), dir, len, body_shape(d, ho.y), &nrm, s);
-    if (s == 0) toi_result(d, i, ho, nrm, t);
+    f3 nrm = zero3(), pt = zero3();
+    const float t = cast_group<G>(body_shape(d, ho.x), dir, len, body_shape(d, ho.y), &nrm, &pt, s);
+    if (s == 0) toi_result(d, i, ho, nrm, pt, t);
   }
 }
 
@@ -574,6 +578,7 @@ __global__ void __launch_bounds__(TPB) k_toi_win(Dev d) {
   if (d.toi_best[ho.x] != toi_key(r.w, ho.y)) return;
   d.toi_hit[ho.x] = r;
   d.toi_tgt[ho.x] = ho.y;
+  d.toi_hpt[ho.x] = d.toi_pt[i];
 }
 
 __device__ __forceinline__ bool toi_apply_body(const Dev& d, uint32_t h) {
@@ -598,10 +603,34 @@ __device__ __forceinline__ bool toi_apply_body(const Dev& d, uint32_t h) {
   d.b.flag_a[o] = 1;  // wake the target
   return true;
 }
+// The clamp's impact event of mover h (mod.rs:947-960): the closing speed
+// from the velocities before the clamp; (a, b) in handle order, the normal
+// as the clamp found it when the mover is b, negated when it is a.
+__device__ __forceinline__ void toi_impact_event(const Dev& d, uint32_t h) {
+  uint32_t flag = 0u;
+  const float4 hit = d.toi_hit[h];
+  if (hit.w >= 0.0f && d.fast[h]) {
+    const uint32_t o = d.toi_tgt[h];
+    const f3 nrm = xyz(hit);
+    const float approach = -dot3(sub(xyz(d.pre_vel[h]), xyz(d.pre_vel[o])), nrm);
+    if (approach > 1.0f) {  // CONTACT_HIT_THRESHOLD (trigger.rs:63)
+      const bool lo = h < o;
+      const f3 n = lo ? neg(nrm) : nrm;
+      const float4 pt = d.toi_hpt[h];
+      d.ev_crec[3 * (size_t)h] = make_uint4(EVK_HIT, lo ? h : o, lo ? o : h, 0u);
+      d.ev_crec[3 * (size_t)h + 1] =
+          make_uint4(__float_as_uint(pt.x), __float_as_uint(pt.y), __float_as_uint(pt.z), __float_as_uint(approach));
+      d.ev_crec[3 * (size_t)h + 2] = make_uint4(__float_as_uint(n.x), __float_as_uint(n.y), __float_as_uint(n.z), 0u);
+      flag = 1u;
+    }
+  }
+  d.ev_cflag[h] = flag;
+}
 __global__ void __launch_bounds__(TPB) k_toi_apply(Dev d) {
   if (dev_ncast(d) == 0u) return;  // whole launch
   uint32_t h = gid();
   bool hit = false;
+  if (h < d.n && d.ev_on) toi_impact_event(d, h);  // before the clamp changes the velocity
   if (h < d.n) hit = toi_apply_body(d, h);
   block_add(&d.counters[CTR_TOIHITS], hit ? 1u : 0u);
 }
@@ -879,6 +908,22 @@ __global__ void __launch_bounds__(TPB) k_narrow_gate(Dev d) {
     d.C.npts[c] = 0u;
     const uint2 ab = d.C.ab[c];
     const uint32_t ia = ab.x, ib = ab.y;
+    if (d.ev_emit) {  // trigger pairs overlap at a signed gap <= 0 (rows.rs:496-523, :803-807)
+      uint32_t g = 0u;
+      if (d.b.trigger[ia] || d.b.trigger[ib]) {
+        const shp A = body_shape(d, ia), B = body_shape(d, ib);
+        float sgn;
+        if (A.kind == SH_BOX && B.kind == SH_BOX) {
+          f3 sn;
+          sgn = separating_axis(A, A.p, B, B.p, &sn);
+        } else {
+          float sat;
+          sgn = shape_distance<false>(A, B, &sat).dist;
+        }
+        g = sgn <= 0.0f ? 1u : 0u;
+      }
+      d.ev_gflag[c] = g;
+    }
     if (!d.b.trigger[ia] && !d.b.trigger[ib]) {  // trigger overlaps form no pair
       const int old = find_old_pair(d, ia, ib);
       live = true;
@@ -3005,4 +3050,67 @@ __global__ void __launch_bounds__(TPB) k_sleep_bodies(Dev d) {
     asleep = sleep_apply_body(d, h);
   }
   block_add(&d.counters[CTR_SLEEPING], asleep ? 1u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// contact and trigger events (emit_events, mod.rs:1606-1669)
+// ---------------------------------------------------------------------------
+// Per pair at the end of the step: its deepest anchor pair under the final
+// poses (the witness is the anchor on body a), touching when deeper than
+// CONTACT_BEGIN_SLOP (trigger.rs:68); a touching pair closing faster than
+// CONTACT_HIT_THRESHOLD (trigger.rs:63), from the velocities before the
+// step's clamp, gets a hit record -- reported when the pair did not touch
+// at the last report (the host holds that set).
+__global__ void __launch_bounds__(TPB) k_ev_touch(Dev d, uint32_t* tflag, uint32_t* hflag, uint4* hrec) {
+  const uint32_t p = gid();
+  if (p >= dev_npairs(d)) return;
+  const uint2 ab = d.P.ab[p];
+  const f3 pa = xyz(d.b.pos[ab.x]), pb = xyz(d.b.pos[ab.y]);
+  const qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  const f3 n = xyz(d.P.n(p));
+  float deepest = 0.0f;
+  f3 witness = mul(add(pa, pb), 0.5f);
+  const uint2 pts = d.P.pts[p];
+  for (uint32_t k = 0; k < pts.y; ++k) {
+    const uint32_t q = pts.x + k;
+    const f3 wa = add(pa, qrot(qa, xyz(d.Q.ra(q))));
+    const f3 wb = add(pb, qrot(qb, xyz(d.Q.rb(q))));
+    const float gap = dot3(n, sub(wa, wb));
+    if (-gap > deepest) {
+      deepest = -gap;
+      witness = wa;
+    }
+  }
+  const bool touch = deepest > 0.005f;
+  uint32_t hit = 0u;
+  if (touch) {
+    const float approach = -dot3(sub(xyz(d.pre_vel[ab.x]), xyz(d.pre_vel[ab.y])), n);
+    if (approach > 1.0f) {
+      const f3 nn = neg(n);
+      hrec[3 * (size_t)p] = make_uint4(EVK_HIT, ab.x, ab.y, 0u);
+      hrec[3 * (size_t)p + 1] = make_uint4(__float_as_uint(witness.x), __float_as_uint(witness.y),
+                                           __float_as_uint(witness.z), __float_as_uint(approach));
+      hrec[3 * (size_t)p + 2] = make_uint4(__float_as_uint(nn.x), __float_as_uint(nn.y), __float_as_uint(nn.z), 0u);
+      hit = 1u;
+    }
+  }
+  tflag[p] = touch ? 1u : 0u;
+  hflag[p] = hit;
+}
+
+// Stream compaction of flagged entries (exclusive offsets from a scan):
+// body-index pairs, and three-word event records.
+__global__ void k_ev_compact_ab(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                                const uint2* __restrict__ ab, uint2* __restrict__ out, uint32_t n) {
+  const uint32_t i = gid();
+  if (i < n && flag[i]) out[off[i]] = ab[i];
+}
+__global__ void k_ev_compact_rec(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ off,
+                                 const uint4* __restrict__ rec, uint4* __restrict__ out, uint32_t n) {
+  const uint32_t i = gid();
+  if (i >= n || !flag[i]) return;
+  const size_t o = 3 * (size_t)off[i];
+  out[o] = rec[3 * (size_t)i];
+  out[o + 1] = rec[3 * (size_t)i + 1];
+  out[o + 2] = rec[3 * (size_t)i + 2];
 }

@@ -260,7 +260,20 @@ struct Dev {
   uint32_t cap_toi, cap_cand, cap_pairs, cap_pts;
   uint32_t* dc;   // DC_*: this step's pair and point counts (k_pair_offsets)
   uint32_t* ovf;  // OVF_*: overflow bits and the largest counts that did not fit
+  // contact / trigger events (emit_events, mod.rs:1606-1669)
+  uint32_t ev_on;      // events enabled
+  uint32_t ev_emit;    // this substep reports transitions (the last of a step call)
+  float4* pre_vel;     // per body: velocity before the TOI clamp (mod.rs:874-876)
+  float4* toi_pt;      // per cast: the target's witness at the hit
+  float4* toi_hpt;     // per body: its clamp's witness
+  uint32_t* ev_cflag;  // per body: its clamp reports an impact
+  uint4* ev_crec;      // per body, 3 words: that impact event
+  uint32_t* ev_gflag;  // per candidate: an overlapping trigger pair
 };
+
+// A device event record: (kind, a, b, 0), (point xyz, approach speed),
+// (normal xyz, 0).
+enum { EVK_BEGIN = 0, EVK_END = 1, EVK_HIT = 2, EVK_ENTERED = 3, EVK_EXITED = 4 };
 
 enum { DC_NPAIRS = 0, DC_NPTS = 1, DC_WORDS = 4 };
 enum { OVF_FLAGS = 0, OVF_TOI = 1, OVF_CAND = 2, OVF_PAIRS = 3, OVF_PTS = 4, OVF_COLOURS = 5, OVF_WORDS = 8 };

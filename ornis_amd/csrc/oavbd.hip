@@ -226,6 +226,17 @@ struct oavbd_ctx {
   uint32_t slot_spare = 1;   // classes queued beyond the last step's colour count
   uint32_t replays = 0;      // batches repeated with exact sizes after an overflow
   Snap bsnap;
+  // contact / trigger events: the queues the host drains, the sets of the
+  // last report (pairs a << 32 | b, ascending), and the compaction scratch
+  bool ev_on = false;
+  std::vector<oavbd_event> ev_contact, ev_trigger;
+  std::vector<uint64_t> prev_touch, prev_trigger;
+  struct EvBuf {
+    size_t cap = 0;
+    uint32_t *fa = nullptr, *fb = nullptr, *oa = nullptr, *ob = nullptr;
+    uint4 *rec = nullptr, *rec_out = nullptr;
+    uint2* ab_out = nullptr;
+  } evb;
   std::string err;
 };
 
@@ -281,6 +292,10 @@ static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   dalloc(&c->cand_cnt, n + 1);
   dalloc(&d.colour_list, n); dalloc(&d.cstat, CSTAT_HEAD + 65536); d.remain = d.cstat; d.cmax = d.cstat + CSTAT_MAX; d.fcnt = d.cstat + CSTAT_FRONT; dalloc(&d.front, 2 * (size_t)n); d.colour_cnt = d.cstat + CSTAT_HEAD; dalloc(&d.colour_off, 65537); dalloc(&d.colour_cur, 65536); dalloc(&d.ccls, 2 * CCLS);
   dalloc(&d.toi_hit, n); dalloc(&d.toi_tgt, n); dalloc(&d.fast, n);
+  dalloc(&d.pre_vel, n); dalloc(&d.toi_hpt, n); dalloc(&d.ev_cflag, n); dalloc(&d.ev_crec, 3 * (size_t)n);
+  d.toi_pt = nullptr;
+  d.ev_gflag = nullptr;
+  d.ev_on = d.ev_emit = 0u;
   dalloc(&d.toi_cnt, n + 1); dalloc(&d.toi_off, n + 1); dalloc(&d.toi_best, n);
   d.toi_pairs = nullptr;
   d.toi_res = nullptr;
@@ -337,6 +352,8 @@ static void ensure_cand(oavbd_ctx* c, size_t need) {
   dalloc(&C.ab, cap); dalloc(&C.acc, cap); dalloc(&C.npts, cap); dalloc(&C.disc0, cap); dalloc(&C.disc1, cap);
   dalloc(&c->d.nlist, cap);
   dalloc(&c->d.pair_cand, cap);
+  dfree(c->d.ev_gflag);
+  dalloc(&c->d.ev_gflag, cap);
   dalloc(&C.fresh, cap * 16); dalloc(&c->d.acc_off, cap + 1); dalloc(&c->d.pt_off, cap + 1);
   if (cap + 1 > (size_t)SCAN_MAX_TILES * SCAN_TILE) throw HipError{"candidate count exceeds the scan capacity"};
   c->cap_cand = cap;
@@ -366,10 +383,12 @@ static void ensure_toi(oavbd_ctx* c, size_t need) {
   dfree(d.toi_pairs);
   dfree(d.toi_res);
   dfree(d.toi_live);
+  dfree(d.toi_pt);
   c->cap_toi = grow(need);
   dalloc(&d.toi_pairs, c->cap_toi);
   dalloc(&d.toi_res, c->cap_toi);
   dalloc(&d.toi_live, c->cap_toi);
+  dalloc(&d.toi_pt, c->cap_toi);
   d.cap_toi = (uint32_t)c->cap_toi;
 }
 
@@ -425,7 +444,13 @@ static void free_all(oavbd_ctx* c) {
   dfree(d.counters); dfree(d.dc); dfree(d.ovf); dfree(d.tail_err); dfree(d.inc_off); dfree(d.inc); dfree(d.bcnt); dfree(d.boff); dfree(d.bcur); dfree(d.blist);
   dfree(d.cand_off); dfree(d.poff); dfree(d.poff_old); dfree(d.sb); dfree(d.inb); dfree(d.acc_off); dfree(d.pt_off); dfree(d.nlist); dfree(d.ptcnt); dfree(d.lflag); dfree(d.lrank);
   dfree(d.cstat); dfree(d.front); dfree(d.colour_off); dfree(d.colour_cur); dfree(d.ccls); dfree(d.colour_list); dfree(d.toi_hit);
-  dfree(d.toi_tgt); dfree(d.fast); dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
+  dfree(d.toi_tgt); dfree(d.fast); dfree(d.pre_vel); dfree(d.toi_hpt); dfree(d.ev_cflag); dfree(d.ev_crec);
+  dfree(d.toi_pt); dfree(d.ev_gflag);
+  {
+    auto& e = c->evb;
+    dfree(e.fa); dfree(e.fb); dfree(e.oa); dfree(e.ob); dfree(e.rec); dfree(e.rec_out); dfree(e.ab_out);
+    e.cap = 0;
+  } dfree(d.toi_cnt); dfree(d.toi_off); dfree(d.toi_best); dfree(d.toi_pairs); dfree(d.toi_res); dfree(d.toi_live);
   dfree(d.sw); dfree(d.ckey); dfree(d.nbr_cnt); dfree(d.nbr_off); dfree(d.nbr_cur); dfree(d.nbr); dfree(c->cand_cnt); dfree(c->ts.tile); dfree(c->ts.done);
   adj_free(c);
   snap_free(c->bsnap);
@@ -1191,6 +1216,122 @@ static void read_counters(oavbd_ctx* c) {
   if (c->counters_h[TAILERR_H]) throw HipError{"a fused primal tail launch timed out waiting for a partner"};
 }
 
+// ---------------------------------------------------------------------------
+// events: the device flags and compacts, the host keeps the sets of the
+// last report and builds the queues (emit_events, mod.rs:1606-1669)
+// ---------------------------------------------------------------------------
+static void ensure_evbuf(oavbd_ctx* c, size_t need) {
+  auto& e = c->evb;
+  if (need <= e.cap && e.fa) return;
+  dfree(e.fa); dfree(e.fb); dfree(e.oa); dfree(e.ob); dfree(e.rec); dfree(e.rec_out); dfree(e.ab_out);
+  e.cap = grow(need);
+  dalloc(&e.fa, e.cap); dalloc(&e.fb, e.cap); dalloc(&e.oa, e.cap + 1); dalloc(&e.ob, e.cap + 1);
+  dalloc(&e.rec, 3 * e.cap); dalloc(&e.rec_out, 3 * e.cap); dalloc(&e.ab_out, e.cap);
+}
+static uint64_t ab_key(uint32_t a, uint32_t b) { return ((uint64_t)a << 32) | b; }
+// The flagged body pairs of `ab[0..n)`, ascending.
+static void ev_pairs(oavbd_ctx* c, const uint32_t* flag, const uint2* ab, uint32_t n, std::vector<uint64_t>& out) {
+  out.clear();
+  if (!n) return;
+  const uint32_t total = scan_total(c, flag, c->evb.oa, n);
+  if (!total) return;
+  k_ev_compact_ab<<<blocks(n), TPB, 0, c->st>>>(flag, c->evb.oa, ab, c->evb.ab_out, n);
+  std::vector<uint2> h(total);
+  HIPC(hipMemcpyAsync(h.data(), c->evb.ab_out, total * sizeof(uint2), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+  for (const uint2& v : h) out.push_back(ab_key(v.x, v.y));
+  std::sort(out.begin(), out.end());
+}
+// The flagged event records of `rec[0..n)` in index order.
+static void ev_records(oavbd_ctx* c, const uint32_t* flag, const uint4* rec, uint32_t n,
+                       std::vector<oavbd_event>& out) {
+  out.clear();
+  if (!n) return;
+  const uint32_t total = scan_total(c, flag, c->evb.ob, n);
+  if (!total) return;
+  k_ev_compact_rec<<<blocks(n), TPB, 0, c->st>>>(flag, c->evb.ob, rec, c->evb.rec_out, n);
+  std::vector<uint4> h(3 * (size_t)total);
+  HIPC(hipMemcpyAsync(h.data(), c->evb.rec_out, h.size() * sizeof(uint4), hipMemcpyDeviceToHost, c->st));
+  HIPC(hipStreamSynchronize(c->st));
+  auto bits = [](uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, sizeof(f));
+    return f;
+  };
+  for (uint32_t i = 0; i < total; ++i) {
+    const uint4 w0 = h[3 * i], w1 = h[3 * i + 1], w2 = h[3 * i + 2];
+    oavbd_event e{};
+    e.kind = w0.x;
+    e.a = w0.y;
+    e.b = w0.z;
+    e.point[0] = bits(w1.x);
+    e.point[1] = bits(w1.y);
+    e.point[2] = bits(w1.z);
+    e.approach = bits(w1.w);
+    e.normal[0] = bits(w2.x);
+    e.normal[1] = bits(w2.y);
+    e.normal[2] = bits(w2.z);
+    out.push_back(e);
+  }
+}
+// Pairs in exactly one of two ascending sets, in ascending order: `in` for
+// those of `now`, `out` for those of `prev` (BTreeSet::symmetric_difference).
+static void ev_transitions(const std::vector<uint64_t>& now, const std::vector<uint64_t>& prev, uint32_t k_in,
+                           uint32_t k_out, std::vector<oavbd_event>& q) {
+  size_t i = 0, j = 0;
+  auto push = [&](uint64_t key, uint32_t kind) {
+    oavbd_event e{};
+    e.kind = kind;
+    e.a = (uint32_t)(key >> 32);
+    e.b = (uint32_t)key;
+    q.push_back(e);
+  };
+  while (i < now.size() || j < prev.size()) {
+    if (j == prev.size() || (i < now.size() && now[i] < prev[j]))
+      push(now[i++], k_in);
+    else if (i == now.size() || prev[j] < now[i])
+      push(prev[j++], k_out);
+    else
+      ++i, ++j;
+  }
+}
+// After a waited substep: its clamp impacts, and on the substep that
+// reports (the last of a step call) the trigger and contact transitions
+// and the hits of newly touching pairs; clamp impacts already reported as
+// hits of the same pair in this substep are dropped (mod.rs:1123-1136).
+static void collect_events(oavbd_ctx* c, bool emit, uint32_t ncast) {
+  Dev& d = c->d;
+  std::vector<oavbd_event> ccd;
+  ensure_evbuf(c, std::max<size_t>({(size_t)d.n, (size_t)d.npairs, (size_t)c->ncand, 1}));
+  if (ncast) ev_records(c, d.ev_cflag, d.ev_crec, d.n, ccd);
+  const size_t start = c->ev_contact.size();
+  if (emit) {
+    std::vector<uint64_t> trig_now, touch_now;
+    if (c->ncand) ev_pairs(c, d.ev_gflag, d.C.ab, c->ncand, trig_now);
+    ev_transitions(trig_now, c->prev_trigger, OAVBD_EVENT_ENTERED, OAVBD_EVENT_EXITED, c->ev_trigger);
+    c->prev_trigger = std::move(trig_now);
+    std::vector<oavbd_event> hits;
+    if (d.npairs) {
+      k_ev_touch<<<blocks(d.npairs), TPB, 0, c->st>>>(d, c->evb.fa, c->evb.fb, c->evb.rec);
+      ev_pairs(c, c->evb.fa, d.P.ab, d.npairs, touch_now);
+      ev_records(c, c->evb.fb, c->evb.rec, d.npairs, hits);
+    }
+    ev_transitions(touch_now, c->prev_touch, OAVBD_EVENT_BEGIN, OAVBD_EVENT_END, c->ev_contact);
+    for (const auto& e : hits)
+      if (!std::binary_search(c->prev_touch.begin(), c->prev_touch.end(), ab_key(e.a, e.b)))
+        c->ev_contact.push_back(e);
+    c->prev_touch = std::move(touch_now);
+  }
+  for (const auto& e : ccd) {
+    bool dup = false;
+    for (size_t i = start; i < c->ev_contact.size(); ++i) {
+      const auto& f = c->ev_contact[i];
+      dup = dup || (f.kind == OAVBD_EVENT_HIT && f.a == e.a && f.b == e.b);
+    }
+    if (!dup) c->ev_contact.push_back(e);
+  }
+}
+
 // One fixed substep.  `finish` reads the step's counters back (stats,
 // sleep count); other steps leave the stream running into the next one.
 // `async`: the host waits for no count inside the step -- launches are
@@ -1203,6 +1344,8 @@ static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
   Dev& d = c->d;
   uint32_t n = d.n;
   if (c->joints_dirty) upload_joints(c);
+  d.ev_on = c->ev_on ? 1u : 0u;
+  d.ev_emit = (c->ev_on && finish) ? 1u : 0u;
   record(c, 0);
   zero_u32(c, d.counters, CTR_COUNT);
   // per-body passes are fused where no joint propagation sits between them
@@ -1330,6 +1473,7 @@ static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
   }
   record(c, 4);
   if (c->rec) tape_end(c);
+  if (c->ev_on && !async) collect_events(c, finish, c->stats.toi_casts);
   if (!async) {
     c->stats.pairs = d.npairs;
     c->stats.points = d.npts;
@@ -1781,6 +1925,11 @@ int oavbd_set_bodies(oavbd_ctx* c, uint32_t n, const int32_t* shape, const float
   GUARD(c, {
     set_bodies_impl(c, in);
     c->inputs = std::move(in);
+    // a new body table: no event baseline, nothing queued
+    c->ev_contact.clear();
+    c->ev_trigger.clear();
+    c->prev_touch.clear();
+    c->prev_trigger.clear();
   });
   return OAVBD_OK;
 }
@@ -1974,6 +2123,28 @@ int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
     HIPC(hipStreamSynchronize(c->st));
     if (c->joints_dirty) upload_joints(c);
     const uint32_t last = c->inputs.n - 1;
+    {  // event state (mod.rs:1246-1286): trigger pairs with h exit, the rest
+       // follow the renumbering; touches and undrained contact events go
+      auto mp = [h, last](uint32_t x) { return x == last ? h : x; };
+      std::vector<uint64_t> kept;
+      for (uint64_t k : c->prev_trigger) {
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+        if (a == h || b == h) {
+          oavbd_event e{};
+          e.kind = OAVBD_EVENT_EXITED;
+          e.a = a;
+          e.b = b;
+          c->ev_trigger.push_back(e);
+        } else {
+          kept.push_back(ab_key(std::min(mp(a), mp(b)), std::max(mp(a), mp(b))));
+        }
+      }
+      std::sort(kept.begin(), kept.end());
+      kept.erase(std::unique(kept.begin(), kept.end()), kept.end());
+      c->prev_trigger = std::move(kept);
+      c->prev_touch.clear();
+      c->ev_contact.clear();
+    }
     BodyInputs in = c->inputs;
     auto rm = [h, last](auto& v, size_t per) {
       for (size_t k = 0; k < per; ++k) v[per * h + k] = v[per * last + k];
@@ -2279,7 +2450,7 @@ static void step_batch(oavbd_ctx* c, uint32_t nsteps) {
   if (!nsteps) return;
   if (c->joints_dirty) upload_joints(c);
   const bool async = c->async_enabled && c->sizes_known && !c->rec && !c->timing && !c->tuning &&
-                     c->force_variant < 0;
+                     c->force_variant < 0 && !c->ev_on;
   c->stats.unwaited = async ? 1u : 0u;
   if (!async) {
     for (uint32_t s = 0; s < nsteps; ++s) step_inner(c, s + 1 == nsteps);
@@ -2471,6 +2642,53 @@ int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   *out = c->stats;
   out->joints = (uint32_t)c->joints.size();
   out->replays = c->replays;
+  return OAVBD_OK;
+}
+
+int oavbd_set_events(oavbd_ctx* c, int enable) {
+  if (!c) return OAVBD_EINVAL;
+  c->ev_on = enable != 0;
+  return OAVBD_OK;
+}
+
+int oavbd_drain_events(oavbd_ctx* c, int which, oavbd_event* out, uint32_t cap, uint32_t* n) {
+  if (!c || !n || (which != 0 && which != 1)) return OAVBD_EINVAL;
+  auto& q = which ? c->ev_trigger : c->ev_contact;
+  if (!out) {
+    *n = (uint32_t)q.size();
+    return OAVBD_OK;
+  }
+  const uint32_t k = std::min<uint32_t>(cap, (uint32_t)q.size());
+  std::copy(q.begin(), q.begin() + k, out);
+  q.erase(q.begin(), q.begin() + k);
+  *n = k;
+  return OAVBD_OK;
+}
+
+int oavbd_get_event_state(oavbd_ctx* c, int which, uint32_t* ab, uint32_t cap, uint32_t* n) {
+  if (!c || !n || (which != 0 && which != 1)) return OAVBD_EINVAL;
+  const auto& v = which ? c->prev_trigger : c->prev_touch;
+  *n = (uint32_t)v.size();
+  if (!ab) return OAVBD_OK;
+  if (cap < v.size()) return fail(c, OAVBD_ECAPACITY, "event state larger than the buffer");
+  for (size_t i = 0; i < v.size(); ++i) {
+    ab[2 * i] = (uint32_t)(v[i] >> 32);
+    ab[2 * i + 1] = (uint32_t)v[i];
+  }
+  return OAVBD_OK;
+}
+
+int oavbd_set_event_state(oavbd_ctx* c, int which, const uint32_t* ab, uint32_t n) {
+  if (!c || (which != 0 && which != 1) || (n && !ab)) return OAVBD_EINVAL;
+  std::vector<uint64_t> v;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t a = ab[2 * i], b = ab[2 * i + 1];
+    if (a >= b || b >= c->inputs.n) return fail(c, OAVBD_EINVAL, "event state pair not (a < b) of live bodies");
+    v.push_back(ab_key(a, b));
+  }
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  (which ? c->prev_trigger : c->prev_touch) = std::move(v);
   return OAVBD_OK;
 }
 
