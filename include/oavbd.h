@@ -17,7 +17,9 @@
  * Bodies are dense indices 0..n-1 (the reference's BodyHandle,
  * crates/physics/src/body.rs).  Shapes: 0 sphere (param[0] = radius),
  * 1 box (param = half extents), 2 half-space (param = outward unit normal,
- * static only).  Body types: 0 static, 1 dynamic, 2 kinematic.  Mass 0
+ * static only), 3 capsule (param = radius, half height of the core segment,
+ * which runs along the body's y axis; Shape::Capsule,
+ * crates/physics/src/collision/shape.rs:157).  Body types: 0 static, 1 dynamic, 2 kinematic.  Mass 0
  * makes a dynamic body static (RigidBody::build, body.rs:346-376).
  *
  * Scenes: every body carries a scene id; bodies of different scenes never
@@ -306,6 +308,21 @@ int oavbd_get_primal_table(oavbd_ctx* ctx, int32_t* table, uint32_t n);
  * measures the per-class variants.  No reference counterpart (the
  * reference sweeps bodies serially, mod.rs:1052-1059). */
 int oavbd_set_primal_tail(oavbd_ctx* ctx, uint32_t max_bodies, uint32_t max_waves);
+
+/* Anisotropic friction of the first n bodies.  Replaces:
+ * RigidBody::set_friction_axis / clear_friction_axis and
+ * friction_transverse (crates/physics/src/body.rs:276-286, :444-466) as
+ * friction_frame reads them (crates/physics/src/avbd/rows.rs:32-52):
+ * axis3 is a body-frame direction (normalised here; zero: isotropic, a
+ * non-normalizable axis is rejected), transverse the friction across it.
+ * A contact's first friction direction is body a's axis (else b's) turned
+ * with the body's current orientation and projected on the contact plane,
+ * re-evaluated at every solve and dual update; the coefficients along and
+ * across it are the larger of the two bodies' (a body without an axis
+ * counts its friction both ways).  Axes persist through body edits; new
+ * bodies are isotropic; oavbd_set_bodies resets them.  Differentiable
+ * stepping rejects a context with axes. */
+int oavbd_set_friction_axes(oavbd_ctx* ctx, const float* axis3, const float* transverse, uint32_t n);
 
 /* Contact and trigger events.  Replaces: PhysicsEngine::drain_contact_events
  * / drain_trigger_events (crates/physics/src/avbd/mod.rs:1591-1597) with

@@ -62,6 +62,7 @@
 #include <cstring>
 #include <limits>
 #include <unordered_map>
+#include <unordered_set>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -70,7 +71,7 @@
 
 namespace orc {
 
-enum Shape : int { SH_SPHERE = 0, SH_BOX = 1, SH_HALFSPACE = 2 };
+enum Shape : int { SH_SPHERE = 0, SH_BOX = 1, SH_HALFSPACE = 2, SH_CAPSULE = 3 };  // capsule sp: (radius, half height)
 enum BodyType : int { BT_STATIC = 0, BT_DYNAMIC = 1, BT_KINEMATIC = 2 };
 enum Order : int { ORDER_REFERENCE = 0, ORDER_COLOUR = 1 };
 
@@ -133,6 +134,11 @@ struct Body {
   uint32_t layer, mask;
   bool trigger;
   int type;
+  // anisotropic friction (body.rs:276-286): unit axis in the body frame
+  // (friction_dir; zero: isotropic) and the friction across it
+  V3<T> fdir{T(0), T(0), T(0)};
+  T ftrans = T(0);
+  bool has_fdir() const { return fdir.x != T(0) || fdir.y != T(0) || fdir.z != T(0); }
 };
 
 template <class T>
@@ -212,6 +218,12 @@ V3<T> shape_inertia(int shape, V3<T> sp, T mass) {
     T k = mass / K<T>::BOX_I_DIV();
     return V3<T>{k * (y + z), k * (z + x), k * (x + y)};
   }
+  if (shape == SH_CAPSULE) {  // shape.rs:769-781, along the body's y axis
+    const T r = sp.x, h = sp.y;
+    const T iy = T(0.5) * mass * r * r;
+    const T ixz = T(0.25) * mass * (r * r) + (mass / T(3)) * h * r + T(0.25) * mass * h * h;
+    return V3<T>{ixz, iy, ixz};
+  }
   return V3<T>{T(0), T(0), T(0)};
 }
 
@@ -220,6 +232,7 @@ template <class T>
 T bound_radius(int shape, V3<T> sp) {
   if (shape == SH_SPHERE) return sp.x;
   if (shape == SH_BOX) return length(sp);
+  if (shape == SH_CAPSULE) return sp.y + sp.x;
   return std::numeric_limits<T>::infinity();
 }
 
@@ -228,6 +241,7 @@ template <class T>
 T shape_min_dim(int shape, V3<T> sp) {
   if (shape == SH_SPHERE) return sp.x;
   if (shape == SH_BOX) return min_element(sp);
+  if (shape == SH_CAPSULE) return sp.x;
   return std::numeric_limits<T>::infinity();
 }
 
@@ -237,6 +251,36 @@ void tangent_basis(V3<T> n, V3<T>* t1, V3<T>* t2) {
   V3<T> axis = std::fabs(n.x) < K<T>::TANGENT_DOT() ? V3<T>{T(1), T(0), T(0)} : V3<T>{T(0), T(1), T(0)};
   *t1 = normalize_or(cross(n, axis), V3<T>{T(0), T(0), T(1)});
   *t2 = cross(*t1, n);
+}
+
+// friction_frame (rows.rs:32-52): the first friction direction from body
+// a's axis, else b's, projected on the contact plane and normalised
+// (UnitVec3::normalize_checked, invariants.rs:55-60: none when not finite
+// or shorter than DEGENERATE_LEN2), else tangent_basis(n); the friction
+// coefficients along it and across it.
+template <class T>
+V3<T> friction_t1(const Body<T>& a, const Body<T>& b, V3<T> n) {
+  auto pick = [&](const Body<T>& bd, V3<T>* out) {
+    if (!bd.has_fdir()) return false;
+    const V3<T> world = rotate(bd.rot, bd.fdir);
+    const V3<T> proj = world - n * dot(world, n);
+    if (!(std::isfinite((double)proj.x) && std::isfinite((double)proj.y) && std::isfinite((double)proj.z)) ||
+        dot(proj, proj) < T(1e-12f))
+      return false;
+    *out = normalize(proj);
+    return true;
+  };
+  V3<T> t1;
+  if (pick(a, &t1) || pick(b, &t1)) return t1;
+  V3<T> t2;
+  tangent_basis(n, &t1, &t2);
+  return t1;
+}
+template <class T>
+void friction_mu(const Body<T>& a, const Body<T>& b, T* mu1, T* mu2) {
+  const T a2 = a.has_fdir() ? a.ftrans : a.friction, b2 = b.has_fdir() ? b.ftrans : b.friction;
+  *mu1 = fmax_(a.friction, b.friction);
+  *mu2 = fmax_(a2, b2);
 }
 
 // quat_mul (rows.rs:91-98).
@@ -446,10 +490,15 @@ Dist<T> sphere_obb(const SRef<T>& a, T r, const SRef<T>& b, V3<T> half) {
   return Dist<T>{inside ? -core - r : core - r, a.pos - n * r, pb};
 }
 
-// gjk::support, box/sphere arms (gjk.rs:68-79).
+// gjk::support, box/sphere/capsule arms (gjk.rs:68-79).
 template <class T>
 V3<T> support(const SRef<T>& s, V3<T> dir) {
   if (s.shape == SH_SPHERE) return s.pos + normalize_or(dir, V3<T>{T(1), T(0), T(0)}) * s.sp.x;
+  if (s.shape == SH_CAPSULE) {
+    const V3<T> axis = rotate(s.rot, V3<T>{T(0), T(1), T(0)});
+    const V3<T> tip = (dot(dir, axis) >= T(0) ? axis : -axis) * s.sp.y;
+    return s.pos + tip + normalize_or(dir, V3<T>{T(1), T(0), T(0)}) * s.sp.x;
+  }
   V3<T> local = rotate(conj(s.rot), dir);
   V3<T> sg{local.x >= T(0) ? T(1) : T(-1), local.y >= T(0) ? T(1) : T(-1), local.z >= T(0) ? T(1) : T(-1)};
   return s.pos + rotate(s.rot, cmul(s.sp, sg));
@@ -563,6 +612,158 @@ Dist<T> box_box(const SRef<T>& a, V3<T> ha, const SRef<T>& b, V3<T> hb, T* sat) 
   return r;
 }
 
+// Capsule core segment (distance.rs:174-178): the capsule's y axis.
+template <class T>
+void capsule_segment(V3<T> pos, T hh, Q<T> rot, V3<T>* s0, V3<T>* s1) {
+  const V3<T> axis = rotate(rot, V3<T>{T(0), T(1), T(0)});
+  *s0 = pos - axis * hh;
+  *s1 = pos + axis * hh;
+}
+// point_segment_closest (distance.rs:63-71).
+template <class T>
+V3<T> point_segment_closest(V3<T> p, V3<T> a, V3<T> b) {
+  const V3<T> ab = b - a;
+  const T len_sq = length_sq(ab);
+  if (len_sq < CF(1e-12)) return a;
+  const T t = clampf(dot(p - a, ab) / len_sq, T(0), T(1));
+  return a + ab * t;
+}
+// sphere_capsule (distance.rs:218-230).
+template <class T>
+Dist<T> sphere_capsule(const SRef<T>& a, T r, const SRef<T>& b, T cr, T hh) {
+  V3<T> s0, s1;
+  capsule_segment(b.pos, hh, b.rot, &s0, &s1);
+  const V3<T> core_b = point_segment_closest(a.pos, s0, s1);
+  const V3<T> d_vec = a.pos - core_b;
+  const T core = length(d_vec);
+  const V3<T> n = normalize_or(d_vec, V3<T>{T(1), T(0), T(0)});
+  return Dist<T>{core - r - cr, a.pos - n * r, core_b + n * cr};
+}
+// capsule_capsule (distance.rs:232-245).
+template <class T>
+Dist<T> capsule_capsule(const SRef<T>& a, T ra, T ha, const SRef<T>& b, T rb, T hb) {
+  V3<T> a0, a1, b0, b1, ca, cb;
+  capsule_segment(a.pos, ha, a.rot, &a0, &a1);
+  capsule_segment(b.pos, hb, b.rot, &b0, &b1);
+  seg_seg_closest(a0, a1, b0, b1, &ca, &cb);
+  const V3<T> d_vec = cb - ca;
+  const T core = length(d_vec);
+  const V3<T> n = normalize_or(d_vec, V3<T>{T(1), T(0), T(0)});
+  return Dist<T>{core - ra - rb, ca + n * ra, cb - n * rb};
+}
+// segment_box (distance_box.rs:148-197): closest points between the segment
+// [start, end] and the solid box [lo, hi], and their squared distance:
+// the segment is cut at the box's slab boundaries, and each piece's
+// stationary point (and ends) tested.
+template <class T>
+void segment_box(V3<T> start, V3<T> end, V3<T> lo, V3<T> hi, V3<T>* core, V3<T>* surface, T* squared) {
+  const V3<T> delta = end - start;
+  T cuts[8];
+  int nc = 0;
+  cuts[nc++] = T(0);
+  cuts[nc++] = T(1);
+  for (int i = 0; i < 3; ++i) {
+    if (std::fabs(delta[i]) > CF(1e-12)) {
+      const T bounds[2] = {lo[i], hi[i]};
+      for (T boundary : bounds) {
+        const T t = (boundary - start[i]) / delta[i];
+        if (t > T(0) && t < T(1)) cuts[nc++] = t;
+      }
+    }
+  }
+  std::sort(cuts, cuts + nc);
+  *core = start;
+  *surface = vclamp(start, lo, hi);
+  *squared = std::numeric_limits<T>::infinity();
+  for (int w = 0; w + 1 < nc; ++w) {
+    const T r0 = cuts[w], r1 = cuts[w + 1];
+    const T middle = (r0 + r1) * T(0.5);
+    const V3<T> p = start + delta * middle;
+    T num = T(0), den = T(0);
+    for (int i = 0; i < 3; ++i) {
+      T boundary;
+      if (p[i] < lo[i])
+        boundary = lo[i];
+      else if (p[i] > hi[i])
+        boundary = hi[i];
+      else
+        continue;
+      num += delta[i] * (start[i] - boundary);
+      den += delta[i] * delta[i];
+    }
+    const T stationary = den > T(0) ? clampf(-num / den, r0, r1) : middle;
+    const T ts[3] = {stationary, r0, r1};
+    for (T t : ts) {
+      const V3<T> point = start + delta * t;
+      const V3<T> closest = vclamp(point, lo, hi);
+      const T sq = length_sq(point - closest);
+      if (sq < *squared) {
+        *core = point;
+        *surface = closest;
+        *squared = sq;
+      }
+    }
+  }
+}
+// spine_penetration (distance_box.rs:244-265): the deepest separating axis
+// between a box and a capsule whose spine meets it (box frame).
+template <class T>
+T spine_penetration(V3<T> center, V3<T> axis, V3<T> half, T radius, T height, V3<T>* normal) {
+  T best = -std::numeric_limits<T>::infinity();
+  *normal = V3<T>{T(1), T(0), T(0)};
+  const V3<T> E[3] = {{T(1), T(0), T(0)}, {T(0), T(1), T(0)}, {T(0), T(0), T(1)}};
+  for (const V3<T>& e : E) {
+    const V3<T> cand[2] = {e, normalize_or(cross(axis, e), V3<T>{T(0), T(0), T(0)})};
+    for (const V3<T>& n : cand) {
+      if (n == V3<T>{T(0), T(0), T(0)}) continue;
+      const T projection = dot(center, n);
+#ifdef ORC_PLANT_CAPSULE  // a planted mistake for the pin tests: the spine's extent dropped
+      const T gap = std::fabs(projection) - dot(half, vabs(n)) - radius;
+#else
+      const T gap = std::fabs(projection) - dot(half, vabs(n)) - height * std::fabs(dot(axis, n)) - radius;
+#endif
+      if (gap > best) {
+        best = gap;
+        *normal = projection >= T(0) ? n : -n;
+      }
+    }
+  }
+  return best;
+}
+// box_capsule (distance_box.rs:200-240): box a, capsule b.
+template <class T>
+Dist<T> box_capsule(const SRef<T>& a, V3<T> half, const SRef<T>& b, T radius, T height) {
+  const Q<T> inv = conj(a.rot);
+  const V3<T> center = rotate(inv, b.pos - a.pos);
+  const V3<T> axis = rotate(inv, rotate(b.rot, V3<T>{T(0), T(1), T(0)}));
+  const V3<T> start = center - axis * height, end = center + axis * height;
+  V3<T> core, surface, normal;
+  T squared, distance;
+  segment_box(start, end, -half, half, &core, &surface, &squared);
+  if (squared > CF(1e-12)) {
+    const T len = std::sqrt(squared);
+    distance = len - radius;
+    normal = (core - surface) / len;
+  } else {
+    distance = spine_penetration(center, axis, half, radius, height, &normal);
+    V3<T> lo = -half, hi = half;
+    for (int i = 0; i < 3; ++i)
+      if (std::fabs(normal[i]) > CF(1e-6)) {
+        lo[i] = signum(normal[i]) * half[i];
+        hi[i] = lo[i];
+      }
+    const T projection = dot(axis, normal);
+    V3<T> e0 = start, e1 = end;
+    if (projection > CF(1e-6))
+      e1 = start;
+    else if (projection < -CF(1e-6))
+      e0 = end;
+    T sq;
+    segment_box(e0, e1, lo, hi, &core, &surface, &sq);
+  }
+  return Dist<T>{distance, a.pos + rotate(a.rot, surface), a.pos + rotate(a.rot, core - normal * radius)};
+}
+
 // shape_distance (distance.rs:654-823) for box/sphere/half-space; *sat
 // receives the box-box SAT gap (box_box_signed_gap, distance_box.rs:54-79).
 template <class T>
@@ -576,6 +777,18 @@ Dist<T> shape_distance(const SRef<T>& a, const SRef<T>& b, T* sat) {
     Dist<T> d = sphere_obb(b, b.sp.x, a, a.sp);
     return Dist<T>{d.dist, d.pb, d.pa};
   }
+  if (a.shape == SH_SPHERE && b.shape == SH_CAPSULE) return sphere_capsule(a, a.sp.x, b, b.sp.x, b.sp.y);
+  if (a.shape == SH_CAPSULE && b.shape == SH_SPHERE) {
+    Dist<T> d = sphere_capsule(b, b.sp.x, a, a.sp.x, a.sp.y);
+    return Dist<T>{d.dist, d.pb, d.pa};
+  }
+  if (a.shape == SH_BOX && b.shape == SH_CAPSULE) return box_capsule(a, a.sp, b, b.sp.x, b.sp.y);
+  if (a.shape == SH_CAPSULE && b.shape == SH_BOX) {
+    Dist<T> d = box_capsule(b, b.sp, a, a.sp.x, a.sp.y);
+    return Dist<T>{d.dist, d.pb, d.pa};
+  }
+  if (a.shape == SH_CAPSULE && b.shape == SH_CAPSULE)
+    return capsule_capsule(a, a.sp.x, a.sp.y, b, b.sp.x, b.sp.y);
   return box_box(a, a.sp, b, b.sp, sat);
 }
 
@@ -847,12 +1060,21 @@ struct Engine {
   static bool pair_allowed(const Body<T>& a, const Body<T>& b) {
     return (a.layer & b.mask) != 0 && (b.layer & a.mask) != 0;
   }
-  bool ball_jointed(int ia, int ib) const {
-    // every kind here but Fixed disables contact between its bodies
-    for (const auto& j : joints)
-      if (j.kind != JK_FIXED && ((j.a == ia && j.b == ib) || (j.a == ib && j.b == ia))) return true;
-    return false;
+  // Every joint kind here but Fixed disables contact between its bodies.
+  // The pairs are looked up in `jointed`, which generate_pairs fills from
+  // the joint list before discovery (a scan of every joint per candidate
+  // pair is quadratic on chain scenes).
+  std::unordered_set<uint64_t> jointed;
+  static uint64_t jkey(int ia, int ib) {
+    const uint32_t lo = (uint32_t)std::min(ia, ib), hi = (uint32_t)std::max(ia, ib);
+    return (uint64_t(lo) << 32) | hi;
   }
+  void index_joints() {
+    jointed.clear();
+    for (const auto& j : joints)
+      if (j.kind != JK_FIXED) jointed.insert(jkey(j.a, j.b));
+  }
+  bool ball_jointed(int ia, int ib) const { return jointed.count(jkey(ia, ib)) != 0; }
 
   // ---- joints helpers (sleep.rs) ----
   T joint_rest_error(const Joint<T>& j) const {
@@ -1205,10 +1427,11 @@ struct Engine {
     T reach = K<T>::GEN_MARGIN() + approach * K<T>::DT();
     if (sgn > reach && !exists) return false;
     bool separated = sgn > reach;
-    // friction_frame, isotropic arm (rows.rs:32-52)
-    T mu1 = fmax_(a.friction, b.friction);
+    // friction_frame's coefficients (rows.rs:32-52, :567-570)
+    T mu1, mu2;
+    friction_mu(a, b, &mu1, &mu2);
     out.mu[0] = fmax_(mu1, T(0));
-    out.mu[1] = fmax_(mu1, T(0));
+    out.mu[1] = fmax_(mu2, T(0));
     T patch = fmin_(ra, rb) + K<T>::MARGIN();
     V3<T> pp = (d.pa + d.pb) * K<T>::HALF();
     V3<T> pa_c = a.pos - normal * dot(a.pos - pp, normal);
@@ -1347,6 +1570,7 @@ struct Engine {
 
   // generate_pairs (rows.rs:767-943).
   void generate_pairs() {
+    index_joints();
     std::unordered_map<uint64_t, int> index;
     bool use_index = !brute;
     if (use_index)
@@ -1729,8 +1953,8 @@ struct Engine {
           }
           continue;
         }
-        V3<T> t1, t2u;
-        tangent_basis(p.n, &t1, &t2u);
+        // the friction frame from the live orientations (assembly.rs:176-181)
+        V3<T> t1 = friction_t1(bodies[p.a], bodies[p.b], p.n);
         V3<T> t2 = cross(t1, p.n);
         V3<T> ra_f, rb_f;
         friction_levers(p, pt, &ra_f, &rb_f);
@@ -1878,8 +2102,7 @@ struct Engine {
   // dual_update (joints_ext.rs:464-598), contact + Ball arms.
   void dual_update() {
     for (auto& p : pairs) {
-      V3<T> t1, t2u;
-      tangent_basis(p.n, &t1, &t2u);
+      V3<T> t1 = friction_t1(bodies[p.a], bodies[p.b], p.n);  // joints_ext.rs:470-474
       V3<T> t2 = cross(t1, p.n);
       bool touching = p.gap <= T(0);
       for (auto& pt : p.pts) {
@@ -2664,6 +2887,25 @@ void orc_get_sleep(void* h, uint8_t* asleep, float* timer) {
     for (size_t i = 0; i < E->bodies.size(); ++i) {
       if (asleep) asleep[i] = E->asleep[i];
       if (timer) timer[i] = (float)E->sleep_timer[i];
+    }
+  });
+}
+
+// Anisotropic friction of the first n bodies: axis3 (body frame, normalised
+// here; zero: isotropic) and the friction across it.
+void orc_set_friction_axes(void* h, const float* axis3, const float* trans, int n) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    for (int i = 0; i < n && i < (int)E->bodies.size(); ++i) {
+      // RigidBody::set_friction_axis (body.rs:452-460): normalize_checked in f32
+      float v[3] = {axis3[3 * i], axis3[3 * i + 1], axis3[3 * i + 2]};
+      if (v[0] != 0.0f || v[1] != 0.0f || v[2] != 0.0f) {
+        const float r = 1.0f / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (float& x : v) x *= r;
+      }
+      E->bodies[i].fdir = V3<TT>{TT(v[0]), TT(v[1]), TT(v[2])};
+      E->bodies[i].ftrans = TT(trans[i]);
     }
   });
 }

@@ -78,6 +78,7 @@ def lib():
         L.orc_tangent_basis_f32.argtypes = [_f32p, _f32p, _f32p]
         L.orc_shape_distance_f32.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_friction_axes.argtypes = [ctypes.c_void_p, _f32p, _f32p, ctypes.c_int]
         L.orc_drain_events.restype = ctypes.c_int
         L.orc_drain_events.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _f32p]
         L.orc_get_event_state.restype = ctypes.c_int
@@ -330,6 +331,13 @@ class OracleEngine:
         tg = np.zeros(self.n, np.int32)
         self._L.orc_get_toi(self.h, _p(t, _f32p), _p(tg, _i32p))
         return t, tg
+
+    def set_friction_axes(self, axis, transverse):
+        """Anisotropic friction of the bodies: (n, 3) body-frame unit axes
+        (zero rows: isotropic) and (n,) frictions across them."""
+        ax = np.ascontiguousarray(axis, dtype=np.float32).reshape(-1, 3)
+        tr = np.ascontiguousarray(transverse, dtype=np.float32).reshape(-1)
+        self._L.orc_set_friction_axes(self.h, _p(ax, _f32p), _p(tr, _f32p), len(tr))
 
     def drain_events(self, trigger=False):
         """Queued contact (or trigger) events, oldest first: (kinds, ab

@@ -105,6 +105,7 @@ SIGNATURES = {
     "oavbd_set_primal_tail": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "oavbd_set_unwaited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]),
     "oavbd_set_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "oavbd_set_friction_axes": (ctypes.c_int, [ctypes.c_void_p, _f32p, _f32p, ctypes.c_uint32]),
     "oavbd_drain_events": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, _u32p]),
     "oavbd_get_event_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32, _u32p]),
     "oavbd_set_event_state": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _u32p, ctypes.c_uint32]),
@@ -488,6 +489,13 @@ class AvbdEngine:
         self._unwaited = (1 if on else 0, int(spare_colours))
         if self._ctx:
             self._check(self._L.oavbd_set_unwaited(self._ctx, *self._unwaited))
+
+    def set_friction_axes(self, axis, transverse):
+        """Anisotropic friction: (n, 3) body-frame axes (zero rows:
+        isotropic) and (n,) frictions across them, for the first n bodies."""
+        ax = np.ascontiguousarray(axis, dtype=np.float32).reshape(-1, 3)
+        tr = np.ascontiguousarray(transverse, dtype=np.float32).reshape(-1)
+        self._check(self._L.oavbd_set_friction_axes(self._ctx, _p(ax, _f32p), _p(tr, _f32p), len(tr)))
 
     def set_events(self, on=True):
         """Contact and trigger events (off by default)."""

@@ -1,5 +1,5 @@
-// avbd_geom.hpp — exact distance queries between placed boxes, spheres and
-// half-spaces (the narrowphase of the AVBD step), device + host.
+// avbd_geom.hpp — exact distance queries between placed boxes, spheres,
+// capsules and half-spaces (the narrowphase of the AVBD step), device + host.
 //
 // Each function follows the operation order of its reference counterpart
 // so that witnesses and gaps are bit-identical:
@@ -13,16 +13,23 @@
 //   support_face         distance_box.rs:81-98
 //   feature_distance     distance_box.rs:100-133
 //   box_box              distance_box.rs:135-147
-//   shape_distance       distance.rs:654-823 (box/sphere/half-space arms)
+//   capsule_segment      distance.rs:174-178
+//   point_segment_closest distance.rs:63-71
+//   sphere_capsule       distance.rs:218-230
+//   capsule_capsule      distance.rs:232-245
+//   segment_box          distance_box.rs:150-197
+//   box_capsule          distance_box.rs:199-240
+//   spine_penetration    distance_box.rs:244-265
+//   shape_distance       distance.rs:654-823 (box/sphere/capsule/half-space arms)
 #pragma once
 
 #include "avbd_math.hpp"
 
-enum { SH_SPHERE = 0, SH_BOX = 1, SH_HALFSPACE = 2 };
+enum { SH_SPHERE = 0, SH_BOX = 1, SH_HALFSPACE = 2, SH_CAPSULE = 3 };
 
 struct shp {
   int kind;
-  f3 p;  // sphere (r,0,0) / box half extents / half-space unit normal
+  f3 p;  // sphere (r,0,0) / box half extents / half-space unit normal / capsule (r, half height, 0)
   f3 pos;
   qf rot;
 };
@@ -35,13 +42,24 @@ struct dist_t {
 HD float bound_radius(int kind, f3 p) {
   if (kind == SH_SPHERE) return p.x;
   if (kind == SH_BOX) return len3(p);
+  if (kind == SH_CAPSULE) return p.y + p.x;
   return __builtin_inff();
 }
 
 HD float shape_min_dim(int kind, f3 p) {
   if (kind == SH_SPHERE) return p.x;
   if (kind == SH_BOX) return fmin_(p.x, fmin_(p.y, p.z));
+  if (kind == SH_CAPSULE) return p.x;
   return __builtin_inff();
+}
+
+// Shape::inertia's capsule arm (crates/physics/src/collision/shape.rs:769-781):
+// about the core axis (body y) the disk term, across it the disk, stem
+// coupling and stem terms.
+HD f3 capsule_inertia(float r, float h, float mass) {
+  const float iy = 0.5f * mass * r * r;
+  const float ixz = 0.25f * mass * (r * r) + (mass / 3.0f) * h * r + 0.25f * mass * h * h;
+  return f3{ixz, iy, ixz};
 }
 
 HD void seg_seg_closest(f3 a0, f3 a1, f3 b0, f3 b1, f3* x, f3* y) {
@@ -134,6 +152,11 @@ HD dist_t sphere_obb(const shp& a, float r, const shp& b, f3 half) {
 
 HD f3 support_pt(const shp& s, f3 dir) {
   if (s.kind == SH_SPHERE) return add(s.pos, mul(normalize_or(dir, f3{1.0f, 0.0f, 0.0f}), s.p.x));
+  if (s.kind == SH_CAPSULE) {
+    const f3 axis = qrot(s.rot, f3{0.0f, 1.0f, 0.0f});
+    const f3 tip = mul(dot3(dir, axis) >= 0.0f ? axis : neg(axis), s.p.y);
+    return add(add(s.pos, tip), mul(normalize_or(dir, f3{1.0f, 0.0f, 0.0f}), s.p.x));
+  }
   f3 local = qrot(qconj(s.rot), dir);
   f3 sg{local.x >= 0.0f ? 1.0f : -1.0f, local.y >= 0.0f ? 1.0f : -1.0f, local.z >= 0.0f ? 1.0f : -1.0f};
   return add(s.pos, qrot(s.rot, cmul(s.p, sg)));
@@ -333,6 +356,164 @@ HD dist_t box_box_from_sat(const shp& a, f3 ha, const shp& b, f3 hb, float gap, 
   return r;
 }
 
+// The capsule's core segment: its y axis, half height hh each way.
+HD void capsule_segment(f3 pos, float hh, qf rot, f3* s0, f3* s1) {
+  const f3 axis = qrot(rot, f3{0.0f, 1.0f, 0.0f});
+  *s0 = sub(pos, mul(axis, hh));
+  *s1 = add(pos, mul(axis, hh));
+}
+
+HD f3 point_segment_closest(f3 p, f3 a, f3 b) {
+  const f3 ab = sub(b, a);
+  const float len_sq = len2(ab);
+  if (len_sq < 1e-12f) return a;
+  const float t = clampf_(dot3(sub(p, a), ab) / len_sq, 0.0f, 1.0f);
+  return add(a, mul(ab, t));
+}
+
+HD dist_t sphere_capsule(const shp& a, float r, const shp& b, float cr, float hh) {
+  f3 s0, s1;
+  capsule_segment(b.pos, hh, b.rot, &s0, &s1);
+  const f3 core_b = point_segment_closest(a.pos, s0, s1);
+  const f3 dv = sub(a.pos, core_b);
+  const float core = len3(dv);
+  const f3 n = normalize_or(dv, f3{1.0f, 0.0f, 0.0f});
+  return dist_t{core - r - cr, sub(a.pos, mul(n, r)), add(core_b, mul(n, cr))};
+}
+
+HD dist_t capsule_capsule(const shp& a, float ra, float ha, const shp& b, float rb, float hb) {
+  f3 a0, a1, b0, b1, ca, cb;
+  capsule_segment(a.pos, ha, a.rot, &a0, &a1);
+  capsule_segment(b.pos, hb, b.rot, &b0, &b1);
+  seg_seg_closest(a0, a1, b0, b1, &ca, &cb);
+  const f3 dv = sub(cb, ca);
+  const float core = len3(dv);
+  const f3 n = normalize_or(dv, f3{1.0f, 0.0f, 0.0f});
+  return dist_t{core - ra - rb, add(ca, mul(n, ra)), sub(cb, mul(n, rb))};
+}
+
+// Closest points between the segment [start, end] and the solid box
+// [lo, hi] (box frame) and their squared distance.  The segment is cut
+// where it crosses the box's slab planes; on each piece the squared
+// distance is one quadratic, whose clamped stationary point and the piece's
+// ends are tried in turn (first strict minimum kept).  At most 6 interior
+// cuts: the cut list is kept sorted by insertion.
+HD void segment_box(f3 start, f3 end, f3 lo, f3 hi, f3* core, f3* surface, float* squared) {
+  const f3 delta = sub(end, start);
+  float cuts[8];
+  int nc = 2;
+  cuts[0] = 0.0f;
+  cuts[1] = 1.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float di = comp(delta, i);
+    if (fabsf(di) > 1e-12f) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const float t = ((k ? comp(hi, i) : comp(lo, i)) - comp(start, i)) / di;
+        if (t > 0.0f && t < 1.0f) {
+          int j = nc++;
+          while (j > 0 && cuts[j - 1] > t) {
+            cuts[j] = cuts[j - 1];
+            --j;
+          }
+          cuts[j] = t;
+        }
+      }
+    }
+  }
+  *core = start;
+  *surface = min3(max3(start, lo), hi);
+  *squared = __builtin_inff();
+  for (int w = 0; w + 1 < nc; ++w) {
+    const float r0 = cuts[w], r1 = cuts[w + 1];
+    const float middle = (r0 + r1) * 0.5f;
+    const f3 p = add(start, mul(delta, middle));
+    float num = 0.0f, den = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const float pi = comp(p, i), li = comp(lo, i), hi_ = comp(hi, i);
+      if (!(pi < li) && !(pi > hi_)) continue;
+      const float boundary = pi < li ? li : hi_;
+      const float di = comp(delta, i);
+      num += di * (comp(start, i) - boundary);
+      den += di * di;
+    }
+    const float stationary = den > 0.0f ? clampf_(-num / den, r0, r1) : middle;
+    const float ts[3] = {stationary, r0, r1};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const f3 point = add(start, mul(delta, ts[k]));
+      const f3 closest = min3(max3(point, lo), hi);
+      const float sq = len2(sub(point, closest));
+      if (sq < *squared) {
+        *core = point;
+        *surface = closest;
+        *squared = sq;
+      }
+    }
+  }
+}
+
+// The deepest separating axis between a box (half extents `half`, at the
+// origin) and a capsule whose spine meets it: the box's face normals and
+// the spine's cross products with them, each face offset by the radius.
+HD float spine_penetration(f3 center, f3 axis, f3 half, float radius, float height, f3* normal) {
+  float best = -__builtin_inff();
+  *normal = f3{1.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const f3 ev{e == 0 ? 1.0f : 0.0f, e == 1 ? 1.0f : 0.0f, e == 2 ? 1.0f : 0.0f};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const f3 n = k ? normalize_or(cross3(axis, ev), zero3()) : ev;
+      if (eq3(n, zero3())) continue;
+      const float projection = dot3(center, n);
+      const float gap = fabsf(projection) - dot3(half, abs3(n)) - height * fabsf(dot3(axis, n)) - radius;
+      if (gap > best) {
+        best = gap;
+        *normal = projection >= 0.0f ? n : neg(n);
+      }
+    }
+  }
+  return best;
+}
+
+// Box a (half extents `half`) against capsule b.
+HD dist_t box_capsule(const shp& a, f3 half, const shp& b, float radius, float height) {
+  const qf inv = qconj(a.rot);
+  const f3 center = qrot(inv, sub(b.pos, a.pos));
+  const f3 axis = qrot(inv, qrot(b.rot, f3{0.0f, 1.0f, 0.0f}));
+  const f3 start = sub(center, mul(axis, height)), end = add(center, mul(axis, height));
+  f3 core, surface, normal;
+  float squared, distance;
+  segment_box(start, end, neg(half), half, &core, &surface, &squared);
+  if (squared > 1e-12f) {
+    const float len = sqrtf(squared);
+    distance = len - radius;
+    normal = divs(sub(core, surface), len);
+  } else {
+    distance = spine_penetration(center, axis, half, radius, height, &normal);
+    f3 lo = neg(half), hi = half;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (fabsf(comp(normal, i)) > 1e-6f) {
+        const float v = signumf(comp(normal, i)) * comp(half, i);
+        setc(lo, i, v);
+        setc(hi, i, v);
+      }
+    const float projection = dot3(axis, normal);
+    f3 e0 = start, e1 = end;
+    if (projection > 1e-6f)
+      e1 = start;
+    else if (projection < -1e-6f)
+      e0 = end;
+    float sq;
+    segment_box(e0, e1, lo, hi, &core, &surface, &sq);
+  }
+  return dist_t{distance, add(a.pos, qrot(a.rot, surface)), add(a.pos, qrot(a.rot, sub(core, mul(normal, radius))))};
+}
+
 // shape_distance; for box-box pairs *sat receives the SAT gap
 // (box_box_signed_gap) the discovery reads as the signed gap.  BOXES =
 // false drops the box-box arm for callers that never pass two boxes.
@@ -342,6 +523,19 @@ HD dist_t shape_distance(const shp& a, const shp& b, float* sat) {
   if (a.kind == SH_HALFSPACE) return halfspace_distance(a, b, true);
   if (b.kind == SH_HALFSPACE) return halfspace_distance(b, a, false);
   if (a.kind == SH_SPHERE && b.kind == SH_SPHERE) return sphere_sphere(a, a.p.x, b, b.p.x);
+  if (a.kind == SH_CAPSULE || b.kind == SH_CAPSULE) {
+    if (a.kind == SH_SPHERE) return sphere_capsule(a, a.p.x, b, b.p.x, b.p.y);
+    if (b.kind == SH_SPHERE) {
+      dist_t d = sphere_capsule(b, b.p.x, a, a.p.x, a.p.y);
+      return dist_t{d.dist, d.pb, d.pa};
+    }
+    if (a.kind == SH_BOX) return box_capsule(a, a.p, b, b.p.x, b.p.y);
+    if (b.kind == SH_BOX) {
+      dist_t d = box_capsule(b, b.p, a, a.p.x, a.p.y);
+      return dist_t{d.dist, d.pb, d.pa};
+    }
+    return capsule_capsule(a, a.p.x, a.p.y, b, b.p.x, b.p.y);
+  }
   if (a.kind == SH_SPHERE) return sphere_obb(a, a.p.x, b, b.p);
   if (b.kind == SH_SPHERE) {
     dist_t d = sphere_obb(b, b.p.x, a, a.p);

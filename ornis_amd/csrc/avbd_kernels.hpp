@@ -69,6 +69,7 @@ __device__ __forceinline__ void note_overflow(const Dev& d, uint32_t word, uint3
 // ---------------------------------------------------------------------------
 // body helpers
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool has_axis(float4 fd) { return fd.x != 0.0f || fd.y != 0.0f || fd.z != 0.0f; }
 
 __device__ __forceinline__ f3 shape_inertia_dev(int kind, f3 p, float mass) {
   if (!(mass > 0.0f) || !isfinite(mass)) return zero3();
@@ -82,6 +83,7 @@ __device__ __forceinline__ f3 shape_inertia_dev(int kind, f3 p, float mass) {
     float k = mass / 12.0f;
     return f3{k * (y + z), k * (z + x), k * (x + y)};
   }
+  if (kind == SH_CAPSULE) return capsule_inertia(p.x, p.y, mass);
   return zero3();
 }
 
@@ -1034,8 +1036,14 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d) {
   if (sgn > reach && old < 0) return;
   bool separated = sgn > reach;
   float4 ma = d.b.mat[ia], mb = d.b.mat[ib];
+  // friction_frame's coefficients (rows.rs:32-52, :567-570): along the
+  // axis and across it for a body with an axis, the friction both ways else
+  const float4 fa = d.b.fdir[ia], fb = d.b.fdir[ib];
+  const float a2 = has_axis(fa) ? fa.w : ma.x, b2 = has_axis(fb) ? fb.w : mb.x;
   float mu1 = fmax_(ma.x, mb.x);
   mu1 = fmax_(mu1, 0.0f);
+  float mu2 = fmax_(a2, b2);
+  mu2 = fmax_(mu2, 0.0f);
   float ra = d.b.shape[ia].w, rb = d.b.shape[ib].w;
   float patch = fmin_(ra, rb) + K_MARGIN;
   f3 pp = mul(add(dd.pa, dd.pb), 0.5f);
@@ -1126,7 +1134,7 @@ __global__ void __launch_bounds__(TPB) k_narrow(Dev d) {
   d.C.acc[c] = 1u;
   d.C.npts[c] = (uint32_t)nf;
   d.C.disc0[c] = pk(normal, sgn);
-  d.C.disc1[c] = make_float4(mu1, mu1, __int_as_float(old), wake ? 1.0f : 0.0f);
+  d.C.disc1[c] = make_float4(mu1, mu2, __int_as_float(old), wake ? 1.0f : 0.0f);
   for (int k = 0; k < nf; ++k) {
     const f3 x = fra(k), y = frb(k);
     d.C.fresh[(size_t)c * 16 + 2 * k] = make_float4(x.x, x.y, x.z, y.x);
@@ -1422,6 +1430,28 @@ __global__ void k_joint_prep(Dev d) {
 // Per-pair step constants: friction frame, separated-damper stiffness
 // (assembly.rs:129-174), step-start violation c0 per point (joints_ext.rs:21-32).
 // PREP_LANES lanes per pair, lane s taking points s, s + PREP_LANES, ...
+// friction_frame's first direction (rows.rs:32-52) from the live rotations
+// qa, qb of the pair's bodies: body a's axis, else b's, projected on the
+// contact plane and normalised (UnitVec3::normalize_checked,
+// invariants.rs:55-60: none when not finite or shorter than
+// DEGENERATE_LEN2), else tangent_basis(n).
+__device__ __forceinline__ f3 friction_t1(const Dev& d, uint32_t a, uint32_t b, qf qa, qf qb, f3 n) {
+  const float4 fd[2] = {d.b.fdir[a], d.b.fdir[b]};
+  const qf q[2] = {qa, qb};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!has_axis(fd[k])) continue;
+    const f3 world = qrot(q[k], xyz(fd[k]));
+    const f3 proj = sub(world, mul(n, dot3(world, n)));
+    const float l2 = dot3(proj, proj);
+    if (!(isfinite(proj.x) && isfinite(proj.y) && isfinite(proj.z)) || l2 < 1e-12f) continue;
+    return mul(proj, 1.0f / sqrtf(dot3(proj, proj)));
+  }
+  f3 t1, t2;
+  tangent_basis(n, &t1, &t2);
+  return t1;
+}
+
 __global__ void __launch_bounds__(TPB) k_pair_prep(Dev d) {
   const uint32_t p = gid() / PREP_LANES;
   const uint32_t s = threadIdx.x % PREP_LANES;
@@ -1446,8 +1476,11 @@ __global__ void __launch_bounds__(TPB) k_pair_prep(Dev d) {
     }
   }
   if (s == 0) {
+    // t2.w marks a pair whose friction frame follows its bodies' axes: the
+    // solves and the dual then take it from the live rotations
+    const bool aniso = has_axis(d.b.fdir[ab.x]) || has_axis(d.b.fdir[ab.y]);
     d.P.t1(p) = pk(t1, damp);
-    d.P.t2(p) = pk(t2, 0.0f);
+    d.P.t2(p) = pk(t2, aniso ? 1.0f : 0.0f);
   }
   if (s >= pts.y) return;
   bool sphere = d.b.kind[ab.x] == SH_SPHERE || d.b.kind[ab.y] == SH_SPHERE;
@@ -1984,6 +2017,10 @@ __device__ __forceinline__ void item_rows(const Dev& d, uint4 it, const ItemRec&
     return;
   }
   f3 t1 = xyz(t14), t2 = xyz(t24);
+  if (t24.w != 0.0f) {  // the frame from the live orientations (assembly.rs:176-181)
+    t1 = friction_t1(d, it.z, it.w, qa, qb, n);
+    t2 = cross3(t1, n);
+  }
   // friction_levers (rows.rs:987-996)
   float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
   f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
@@ -2766,7 +2803,12 @@ __device__ __forceinline__ void dual_point(const Dev& d, uint32_t q) {
     return cc * (1.0f - K_ALPHA) + dot3(u, sub(d_a, d_b));
   };
   float cn = row_c(n, c0, ra_w, rb_w);
-  f3 t1 = xyz(d.P.t1(p)), t2 = xyz(d.P.t2(p));
+  const float4 t24 = d.P.t2(p);
+  f3 t1 = xyz(d.P.t1(p)), t2 = xyz(t24);
+  if (t24.w != 0.0f) {  // joints_ext.rs:470-474, live orientations
+    t1 = friction_t1(d, ab.x, ab.y, qa, qb, n);
+    t2 = cross3(t1, n);
+  }
   float sep = dot3(sub(add(xyz(pa4), ra_w), add(xyz(pb4), rb_w)), n);
   f3 ra_f = sub(ra_w, mul(n, sep * 0.5f));
   f3 rb_f = add(rb_w, mul(n, sep * 0.5f));

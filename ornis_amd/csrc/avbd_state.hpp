@@ -76,6 +76,7 @@ struct Bodies {
   float4* inertia;   // body-frame diagonal xyz (0 while asleep), w = mass
   float4* shape;     // param xyz, w = bound radius
   float4* mat;       // friction, restitution, rolling, torsion
+  float4* fdir;      // anisotropic friction: unit axis in the body frame (zero: isotropic), w = friction across it
   int* kind;         // shape kind
   int* type;         // body type
   uint32_t* layer;

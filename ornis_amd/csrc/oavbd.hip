@@ -149,6 +149,7 @@ struct BodyInputs {
   uint32_t n = 0;
   std::vector<int32_t> shape, type;
   std::vector<float> sp3, pos3, quat4, vel3, ang3, mass, friction, restitution, rolling, torsion;
+  std::vector<float> fdir3, ftrans;  // anisotropic friction: body-frame unit axis (zero: none), friction across it
   std::vector<uint32_t> layer, mask, scene;
   std::vector<uint8_t> trigger;
 };
@@ -228,6 +229,7 @@ struct oavbd_ctx {
   Snap bsnap;
   // contact / trigger events: the queues the host drains, the sets of the
   // last report (pairs a << 32 | b, ascending), and the compaction scratch
+  bool any_axis = false;  // some body has an anisotropic friction axis
   bool ev_on = false;
   std::vector<oavbd_event> ev_contact, ev_trigger;
   std::vector<uint64_t> prev_touch, prev_trigger;
@@ -264,7 +266,7 @@ static int fail(oavbd_ctx* c, int code, const std::string& m) {
 static void alloc_bodies(oavbd_ctx* c, uint32_t n) {
   Bodies& b = c->d.b;
   dalloc(&b.pos, n); dalloc(&b.rot, n); dalloc(&b.vel, n); dalloc(&b.angvel, n); dalloc(&b.torque, n);
-  dalloc(&b.inertia, n); dalloc(&b.shape, n); dalloc(&b.mat, n); dalloc(&b.kind, n); dalloc(&b.type, n);
+  dalloc(&b.inertia, n); dalloc(&b.shape, n); dalloc(&b.mat, n); dalloc(&b.fdir, n); dalloc(&b.kind, n); dalloc(&b.type, n);
   dalloc(&b.layer, n); dalloc(&b.mask, n); dalloc(&b.scene, n); dalloc(&b.prio, n); dalloc(&b.local, n);
   dalloc(&b.trigger, n); dalloc(&b.pos0, n); dalloc(&b.rot0, n); dalloc(&b.inertial, n);
   dalloc(&b.inertial_rot, n); dalloc(&b.drot, n); dalloc(&b.prev_pos, n); dalloc(&b.prev_rot, n);
@@ -428,7 +430,7 @@ static void adj_free(oavbd_ctx* c) {
 static void free_all(oavbd_ctx* c) {
   Bodies& b = c->d.b;
   dfree(b.pos); dfree(b.rot); dfree(b.vel); dfree(b.angvel); dfree(b.torque); dfree(b.inertia); dfree(b.shape);
-  dfree(b.mat); dfree(b.kind); dfree(b.type); dfree(b.layer); dfree(b.mask); dfree(b.scene); dfree(b.prio);
+  dfree(b.mat); dfree(b.fdir); dfree(b.kind); dfree(b.type); dfree(b.layer); dfree(b.mask); dfree(b.scene); dfree(b.prio);
   dfree(b.local); dfree(b.trigger); dfree(b.pos0); dfree(b.rot0); dfree(b.inertial); dfree(b.inertial_rot);
   dfree(b.drot); dfree(b.prev_pos); dfree(b.prev_rot); dfree(b.asleep); dfree(b.timer); dfree(b.colour); dfree(b.done);
   dfree(b.flag_a); dfree(b.flag_b); dfree(b.supported); dfree(b.comp); dfree(b.comp_any);
@@ -1697,6 +1699,8 @@ static void awake_mass(int shape, f3 p, int type, float m, float* inv, f3* I) {
     float x = s2.x * s2.x, y = s2.y * s2.y, z = s2.z * s2.z;
     float k = m / 12.0f;
     *I = f3{k * (y + z), k * (z + x), k * (x + y)};
+  } else if (shape == SH_CAPSULE) {
+    *I = capsule_inertia(p.x, p.y, m);
   }
 }
 
@@ -1732,7 +1736,7 @@ static void set_bodies_impl(oavbd_ctx* c, const BodyInputs& in) {
       float inv = 0.0f, mm = t == BT_DYNAMIC ? m : 0.0f;
       f3 I = zero3();
       awake_mass(shape[i], p, t, m, &inv, &I);
-      float br = shape[i] == SH_SPHERE ? p.x : (shape[i] == SH_BOX ? sqrtf(p.x * p.x + p.y * p.y + p.z * p.z) : INFINITY);
+      float br = bound_radius(shape[i], p);
       brad[i] = br;
       pos[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], inv);
       rot[i] = make_float4(quat4[4 * i], quat4[4 * i + 1], quat4[4 * i + 2], quat4[4 * i + 3]);
@@ -1831,6 +1835,15 @@ static void set_bodies_impl(oavbd_ctx* c, const BodyInputs& in) {
       HIPC(hipMemcpy(dst, v.data(), n * sizeof(float4), hipMemcpyHostToDevice));
     };
     up4(d.b.pos, pos); up4(d.b.rot, rot); up4(d.b.vel, vel); up4(d.b.angvel, ang); up4(d.b.torque, zero);
+    {
+      std::vector<float4> fd(n);
+      c->any_axis = false;
+      for (uint32_t i = 0; i < n; ++i) {
+        fd[i] = make_float4(in.fdir3[3 * i], in.fdir3[3 * i + 1], in.fdir3[3 * i + 2], in.ftrans[i]);
+        c->any_axis = c->any_axis || fd[i].x != 0.0f || fd[i].y != 0.0f || fd[i].z != 0.0f;
+      }
+      up4(d.b.fdir, fd);
+    }
     up4(d.b.inertia, inert); up4(d.b.shape, shp4); up4(d.b.mat, mat); up4(d.b.prev_pos, pos); up4(d.b.prev_rot, rot);
     up4(d.b.pos0, pos); up4(d.b.rot0, rot); up4(d.b.drot, zero);
     HIPC(hipMemcpy(d.b.kind, shape, n * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -1883,10 +1896,13 @@ static int append_inputs(oavbd_ctx* c, BodyInputs& in, uint32_t n, const int32_t
             !body_type))
     return fail(c, OAVBD_EINVAL, "null body array");
   for (uint32_t i = 0; i < n; ++i) {
-    if (shape[i] < 0 || shape[i] > 2) return fail(c, OAVBD_EINVAL, "bad shape kind");
+    if (shape[i] < 0 || shape[i] > 3) return fail(c, OAVBD_EINVAL, "bad shape kind");
     if (body_type[i] < 0 || body_type[i] > 2) return fail(c, OAVBD_EINVAL, "bad body type");
     if (shape[i] == SH_HALFSPACE && body_type[i] == BT_DYNAMIC && mass[i] > 0.0f)
       return fail(c, OAVBD_EINVAL, "dynamic half-space");
+    if (shape[i] == SH_CAPSULE && !(sp3[3 * i] > 0.0f && sp3[3 * i] < INFINITY && sp3[3 * i + 1] >= 0.0f &&
+                                    sp3[3 * i + 1] < INFINITY))
+      return fail(c, OAVBD_EINVAL, "capsule needs a finite radius > 0 and half height >= 0");
   }
   auto add = [n](auto& v, const auto* src, size_t per, auto dflt) {
     for (uint32_t i = 0; i < n; ++i)
@@ -1908,6 +1924,10 @@ static int append_inputs(oavbd_ctx* c, BodyInputs& in, uint32_t n, const int32_t
   add(in.mask, mask, 1, 0xffffffffu);
   add(in.scene, scene, 1, 0u);
   add(in.trigger, trigger, 1, (uint8_t)0);
+  for (uint32_t i = 0; i < n; ++i) {  // isotropic until oavbd_set_friction_axes
+    for (int k = 0; k < 3; ++k) in.fdir3.push_back(0.0f);
+    in.ftrans.push_back(friction[i]);
+  }
   in.n += n;
   return OAVBD_OK;
 }
@@ -2153,6 +2173,7 @@ int oavbd_remove_body(oavbd_ctx* c, uint32_t h) {
     rm(in.shape, 1); rm(in.type, 1); rm(in.sp3, 3); rm(in.pos3, 3); rm(in.quat4, 4); rm(in.vel3, 3);
     rm(in.ang3, 3); rm(in.mass, 1); rm(in.friction, 1); rm(in.restitution, 1); rm(in.rolling, 1);
     rm(in.torsion, 1); rm(in.layer, 1); rm(in.mask, 1); rm(in.scene, 1); rm(in.trigger, 1);
+    rm(in.fdir3, 3); rm(in.ftrans, 1);
     in.n = last;
     HostBodies hb = download_bodies(c);
     rm(hb.pos, 1); rm(hb.rot, 1); rm(hb.vel, 1); rm(hb.ang, 1); rm(hb.torque, 1); rm(hb.inertia, 1);
@@ -2645,6 +2666,41 @@ int oavbd_get_stats(oavbd_ctx* c, oavbd_stats* out) {
   return OAVBD_OK;
 }
 
+int oavbd_set_friction_axes(oavbd_ctx* c, const float* axis3, const float* transverse, uint32_t n) {
+  if (!c || (n && (!axis3 || !transverse))) return OAVBD_EINVAL;
+  if (n > c->inputs.n) return fail(c, OAVBD_EINVAL, "more friction axes than bodies");
+  std::vector<float> ax(axis3, axis3 + 3 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i) {
+    float* v = &ax[3 * i];
+    if (!std::isfinite(transverse[i])) return fail(c, OAVBD_EINVAL, "transverse friction not finite");
+    if (v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f) continue;  // isotropic
+    // RigidBody::set_friction_axis (body.rs:452-460): normalize_checked
+    const float l2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])) || l2 < 1e-12f)
+      return fail(c, OAVBD_EINVAL, "friction axis not normalizable");
+    const float r = 1.0f / std::sqrt(l2);
+    v[0] *= r;
+    v[1] *= r;
+    v[2] *= r;
+  }
+  GUARD(c, {
+    HIPC(hipSetDevice(c->device));
+    HIPC(hipStreamSynchronize(c->st));
+    BodyInputs& in = c->inputs;
+    std::copy(ax.begin(), ax.end(), in.fdir3.begin());
+    std::copy(transverse, transverse + n, in.ftrans.begin());
+    std::vector<float4> fd(in.n);
+    c->any_axis = false;
+    for (uint32_t i = 0; i < in.n; ++i) {
+      fd[i] = make_float4(in.fdir3[3 * i], in.fdir3[3 * i + 1], in.fdir3[3 * i + 2], in.ftrans[i]);
+      c->any_axis = c->any_axis || fd[i].x != 0.0f || fd[i].y != 0.0f || fd[i].z != 0.0f;
+    }
+    HIPC(hipMemcpy(c->d.b.fdir, fd.data(), in.n * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(hipDeviceSynchronize());  // the null-stream upload lands before the context stream reads it
+  });
+  return OAVBD_OK;
+}
+
 int oavbd_set_events(oavbd_ctx* c, int enable) {
   if (!c) return OAVBD_EINVAL;
   c->ev_on = enable != 0;
@@ -2798,6 +2854,7 @@ int oavbd_diff_forward(oavbd_ctx* c, uint32_t nsteps, uint32_t every) {
   if (!c || every == 0) return OAVBD_EINVAL;
   if (c->d.njoints) return fail(c, OAVBD_EINVAL, "differentiable stepping covers contact scenes (no joints)");
   if (c->rows6) return fail(c, OAVBD_EINVAL, "differentiable stepping covers sliding friction (no rolling/torsion)");
+  if (c->any_axis) return fail(c, OAVBD_EINVAL, "differentiable stepping covers isotropic friction (no axes)");
   for (int t : c->type_h)
     if (t == BT_KINEMATIC) return fail(c, OAVBD_EINVAL, "differentiable stepping has no kinematic bodies");
   GUARD(c, {

@@ -17,6 +17,7 @@ import numpy as np
 SHAPE_SPHERE = 0
 SHAPE_BOX = 1
 SHAPE_HALFSPACE = 2
+SHAPE_CAPSULE = 3  # shape_param (radius, half height, 0), along the body's y axis
 
 BODY_STATIC = 0
 BODY_DYNAMIC = 1
@@ -54,6 +55,13 @@ class SceneBuilder:
         kw.setdefault("restitution", 0.5)
         kw.setdefault("friction", 0.3)
         return self._add(SHAPE_SPHERE, (float(radius), 0.0, 0.0), tuple(pos), float(mass), **kw)
+
+    def capsule(self, pos, radius, half_height, mass, **kw):
+        """RigidBody::new_capsule (crates/physics/src/body.rs:562-575): the
+        core segment runs along the body's y axis, capsule material."""
+        kw.setdefault("restitution", 0.4)
+        kw.setdefault("friction", 0.4)
+        return self._add(SHAPE_CAPSULE, (float(radius), float(half_height), 0.0), tuple(pos), float(mass), **kw)
 
     def halfspace(self, pos, normal, **kw):
         kw.setdefault("restitution", 0.3)

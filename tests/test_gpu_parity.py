@@ -419,3 +419,108 @@ def test_thin_wall_and_bounced_retouch_bitwise(gpu_available):
         assert_same(g, o, 299)
         b = g.state()[1]
         assert min_y > 0.0 and abs(b[1] - 0.4) < 0.05 and abs(b[0] - x0) < 0.15 and np.linalg.norm(b[7:10]) < 0.5
+
+
+def test_anisotropic_friction_bitwise(gpu_available):
+    """Anisotropic friction axes (friction_frame, rows.rs:32-52): boxes with
+    random body-frame axes and transverse frictions, randomly turned, slide
+    and tumble over a floor and over each other; the friction frame follows
+    the live orientations in every solve and dual update.  Bodies, pairs and
+    TOI clamps equal the oracle's bit for bit; the reference's channel test
+    (avbd_engine.rs:466-495) holds on the GPU; a zero-length axis is
+    rejected."""
+    rng = _rng(77)
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (12.0, 1.0, 12.0), 0.0, friction=0.2)
+    n = 48
+    for i in range(n):
+        q = rng.normal(size=4)
+        q = (q / np.linalg.norm(q)).astype(np.float32)
+        sb.box((rng.uniform(-6, 6), 0.6 + 1.1 * (i // 16), rng.uniform(-6, 6)), tuple(rng.uniform(0.2, 0.5, 3)),
+               2.0, friction=float(rng.uniform(0.05, 0.3)), quat=tuple(float(x) for x in q),
+               vel=(float(rng.uniform(-4, 4)), 0.0, float(rng.uniform(-4, 4))))
+    s = sb.build()
+    axis = np.zeros((n + 1, 3), np.float32)
+    trans = s["friction"].copy()
+    for i in range(1, n + 1):
+        if i % 4:
+            axis[i] = rng.normal(size=3) * rng.uniform(0.5, 2.0)
+            trans[i] = rng.uniform(0.5, 1.2)
+    g = AvbdEngine(scene=s)
+    g.set_friction_axes(axis, trans)
+    o = po.OracleEngine(s, order=po.ORDER_COLOUR, math=po.MATH_PORTABLE)
+    o.set_friction_axes(axis, trans)
+    for step in range(120):
+        g.step_n(1)
+        o.step_inner(1)
+        if step % 10 == 0 or step == 119:
+            assert_same(g, o, step)
+
+    def channel(direction):
+        sb = scenes.SceneBuilder()
+        sb.box((0.0, -1.0, 0.0), (5.0, 1.0, 5.0), 0.0, friction=0.1)
+        d = np.asarray(direction, np.float32)
+        sb.box((0.0, 0.5, 0.0), (0.5, 0.5, 0.5), 1.0, friction=0.1, vel=tuple(5.0 * d))
+        e = AvbdEngine(scene=sb.build())
+        e.set_friction_axes(np.array([[0, 0, 0], [1, 0, 0]], np.float32), np.array([0.1, 1.0], np.float32))
+        e.step_n(10)
+        return 5.0 - float(e.state()[1, 7:10] @ d)
+
+    assert channel((0, 0, 1)) > 4.0 * channel((1, 0, 0))
+    with pytest.raises(Exception):
+        g.set_friction_axes(np.array([[0, 0, 0], [1e-9, 0, 0]], np.float32), np.array([0.1, 0.1], np.float32))
+
+
+def _capsule_scene(seed):
+    """Capsules of every proportion (squat to long, radius 0.08-0.4) with
+    boxes and spheres, randomly turned, dropped on a half-space floor beside
+    a static box ledge and a static capsule rail; four fast capsules thrown
+    at a thin wall take the TOI clamp."""
+    rng = _rng(seed)
+    sb = scenes.SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    sb.box((-4.0, 0.5, 0.0), (1.5, 0.5, 3.0), 0.0)
+    sb.capsule((3.0, 0.4, 0.0), 0.3, 2.5, 0.0, quat=(0.0, 0.0, float(np.sin(np.pi / 4)), float(np.cos(np.pi / 4))))
+    sb.box((9.0, 2.0, 0.0), (0.03, 2.0, 3.0), 0.0)
+    for i in range(72):
+        q = rng.normal(size=4)
+        q = tuple(float(x) for x in (q / np.linalg.norm(q)).astype(np.float32))
+        p = (float(rng.uniform(-5, 5)), 1.2 + 0.8 * (i // 12), float(rng.uniform(-3, 3)))
+        if i % 4 == 3:
+            sb.box(p, tuple(rng.uniform(0.15, 0.4, 3)), 1.5, quat=q)
+        elif i % 8 == 1:
+            sb.sphere(p, float(rng.uniform(0.15, 0.4)), 1.0)
+        else:
+            sb.capsule(p, float(rng.uniform(0.08, 0.4)), float(rng.uniform(0.0, 0.9)), float(rng.uniform(0.5, 3.0)),
+                       quat=q, angvel=tuple(rng.uniform(-3, 3, 3)))
+    for k in range(4):
+        sb.capsule((2.0, 1.0 + 0.9 * k, -1.5 + k), 0.1, 0.3, 1.0, vel=(40.0 + 5.0 * k, 0.0, 0.0),
+                   quat=(float(np.sin(0.3 * k)), 0.0, 0.0, float(np.cos(0.3 * k))))
+    return sb.build()
+
+
+def test_capsules_bitwise(gpu_available):
+    """Capsule pairs with every other shape (sphere_capsule,
+    capsule_capsule, box_capsule with its spine path, half-space support)
+    through 180 steps: bodies, pairs, colourings and TOI clamps bit for bit;
+    the contained / face-crossing trigger overlaps of
+    solver_lifecycle.rs:465-492 enter once each on the GPU; bad capsule
+    parameters are rejected."""
+    g, o = run_pair(_capsule_scene(91), 180, check_every=15)
+    assert g.stats()["toi_hits"] >= 0 and g.stats()["breakdowns"] == 0
+    for sp in ((0.1, 0.2), (0.1, 3.0)):
+        sb = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+        t = sb.box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.0)
+        sb.capsule((0.0, 0.0, 0.0), sp[0], sp[1], 1.0)
+        sc = sb.build()
+        sc["trigger"][t] = 1
+        e = AvbdEngine(scene=sc)
+        e.set_events(True)
+        e.step(1.0 / 60.0)
+        kinds, ab, *_ = e.drain_events(trigger=True)
+        assert kinds.tolist() == [3] and ab.tolist() == [[0, 1]], (sp, kinds)
+    for bad in ((0.0, 1.0), (0.2, -0.1), (float("inf"), 1.0)):
+        sb = scenes.SceneBuilder()
+        sb.capsule((0.0, 0.0, 0.0), bad[0], bad[1], 1.0)
+        with pytest.raises(Exception):
+            AvbdEngine(scene=sb.build())

@@ -183,3 +183,33 @@ def test_heavy_column_sinks_under_the_penalty_ramp():
     assert 39.8 - (y_ref[-1] + 0.5) > 3.0  # the column's total compression
     assert abs(y_col[0] - y_ref[0]) < 0.01
     assert np.isfinite(y_ref).all() and np.isfinite(y_col).all()
+
+
+def _aniso_slide(direction, quat=(0.0, 0.0, 0.0, 1.0), order=po.ORDER_REFERENCE):
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, -1.0, 0.0), (5.0, 1.0, 5.0), 0.0, friction=0.1)
+    d = np.asarray(direction, np.float32)
+    sb.box((0.0, 0.5, 0.0), (0.5, 0.5, 0.5), 1.0, friction=0.1, vel=tuple(5.0 * d), quat=quat)
+    e = po.OracleEngine(sb.build(), order=order)
+    e.set_friction_axes(np.array([[0, 0, 0], [1, 0, 0]], np.float32), np.array([0.1, 1.0], np.float32))
+    for _ in range(10):
+        e.step(DT)
+    return 5.0 - float(e.state()[1, 7:10] @ d)
+
+
+def test_anisotropic_friction_channels():
+    """avbd_anisotropic_friction_channels (avbd_engine.rs:466-495): a box
+    with friction 0.1 along its x axis and 1.0 across it slides 10 steps at
+    5 m/s; the transverse slide decays more than 4x faster.  Each channel
+    loses mu g t of speed (Coulomb: 0.164 and 1.64 m/s) within 5 %, in both
+    sweep orders, and a box turned 90 degrees about y swaps the channels:
+    the axis turns with the body (friction_frame, rows.rs:32-52)."""
+    for order in (po.ORDER_REFERENCE, po.ORDER_COLOUR):
+        dx, dz = _aniso_slide((1, 0, 0), order=order), _aniso_slide((0, 0, 1), order=order)
+        assert dz > 4.0 * max(dx, 1e-6)
+        assert abs(dx - 0.1 * 9.81 * 10 * DT) < 0.05 * 0.1 * 9.81 * 10 * DT
+        assert abs(dz - 1.0 * 9.81 * 10 * DT) < 0.05 * 1.0 * 9.81 * 10 * DT
+    c = float(np.cos(np.pi / 4))
+    turned = (0.0, c, 0.0, c)
+    assert abs(_aniso_slide((1, 0, 0), turned) - dz) < 1e-3
+    assert abs(_aniso_slide((0, 0, 1), turned) - dx) < 1e-3
