@@ -19,8 +19,9 @@
  * 1 box (param = half extents), 2 half-space (param = outward unit normal,
  * static only), 3 capsule (param = radius, half height of the core segment,
  * which runs along the body's y axis; Shape::Capsule,
- * crates/physics/src/collision/shape.rs:157).  Body types: 0 static, 1 dynamic, 2 kinematic.  Mass 0
- * makes a dynamic body static (RigidBody::build, body.rs:346-376).
+ * crates/physics/src/collision/shape.rs:157).  Body types: 0 static,
+ * 1 dynamic, 2 kinematic.  Mass 0 makes a dynamic body static
+ * (RigidBody::build, body.rs:346-376).
  *
  * Scenes: every body carries a scene id; bodies of different scenes never
  * collide, so one context steps thousands of independent scenes in one
@@ -190,21 +191,47 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
  * oavbd_get_state, oavbd_get_sleep and oavbd_get_contacts. */
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
-/* Joint kinds of oavbd_add_joint (the reference's JointKind arms without
- * limit or motor; the other kinds are not built). */
-enum { OAVBD_JOINT_BALL = 0, OAVBD_JOINT_REVOLUTE = 1, OAVBD_JOINT_PRISMATIC = 2, OAVBD_JOINT_FIXED = 3 };
+/* Joint kinds (the reference's JointKind arms; Revolute and Prismatic
+ * without limit or motor; Wheel, Gear, SixDof and Motor are not built).
+ * Rope and Spring are added with oavbd_add_rope_joint /
+ * oavbd_add_spring_joint, the others with oavbd_add_joint. */
+enum {
+  OAVBD_JOINT_BALL = 0,
+  OAVBD_JOINT_REVOLUTE = 1,
+  OAVBD_JOINT_PRISMATIC = 2,
+  OAVBD_JOINT_FIXED = 3,
+  OAVBD_JOINT_DISTANCE = 4,
+  OAVBD_JOINT_ROPE = 5,
+  OAVBD_JOINT_SPRING = 6
+};
+
+/* A joint motor as a Spring joint carries it (JointMotor,
+ * crates/physics/src/joint.rs:962-1126): kind (MotorKind), model
+ * (MotorModel: acceleration-based scales stiffness and damping by the
+ * bodies' reduced mass), rest length (target_position), stiffness, damping,
+ * force budget and target velocity.  The step reads rest length, stiffness,
+ * damping and model; kind, budget and target velocity decide whether the
+ * spring keeps its bodies awake (JointMotor::keeps_awake). */
+enum { OAVBD_MOTOR_VELOCITY = 0, OAVBD_MOTOR_POSITION = 1, OAVBD_MOTOR_SERVO = 2 };
+enum { OAVBD_MOTOR_ACCELERATION_BASED = 0, OAVBD_MOTOR_FORCE_BASED = 1 };
+typedef struct oavbd_motor {
+  uint32_t kind;  /* OAVBD_MOTOR_VELOCITY / _POSITION / _SERVO           */
+  uint32_t model; /* OAVBD_MOTOR_ACCELERATION_BASED / _FORCE_BASED       */
+  float target_position, target_velocity, stiffness, damping, max_force;
+} oavbd_motor;
 
 /* Joint of `kind` between bodies a and b: body-local anchors la, lb and,
  * for Revolute and Prismatic, body-local hinge / slide axes axis_a, axis_b
  * (normalised here; NULL for the other kinds).  A Fixed joint captures the
- * bodies' current relative rotation and anchor offset as its assembly pose.
+ * bodies' current relative rotation and anchor offset as its assembly pose,
+ * a Distance joint the anchors' current distance as its rod length.
  * Both bodies wake first, as in the reference, even when the spec is then
  * rejected: EINVAL for a == b, an out-of-range body, an unknown kind or a
  * degenerate axis.  Jointed bodies do not collide, except through a Fixed
  * joint.  Returns the joint index through *out_index (may be NULL).
  * Replaces: PhysicsEngine::add_joint for JointKind::{Ball, Revolute,
- * Prismatic, Fixed} with limit and motor None (mod.rs:1308-1510; frames of
- * resolve_joint, crates/physics/src/joint.rs:1251-1323). */
+ * Prismatic, Fixed, Distance} with limit and motor None (mod.rs:1308-1510;
+ * frames of resolve_joint, crates/physics/src/joint.rs:1251-1332). */
 int oavbd_add_joint(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                     const float axis_a[3], const float axis_b[3], uint32_t* out_index);
 
@@ -213,6 +240,24 @@ int oavbd_add_joint(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const
  * 1273-1279). */
 int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                          uint32_t* out_index);
+
+/* Rope between a and b: anchors la, lb may come closer freely but never
+ * part beyond max_distance (one-sided row, a velocity catch each step).
+ * EINVAL, before any wake, unless max_distance is finite and positive.
+ * Replaces: PhysicsEngine::add_joint with JointKind::Rope (mod.rs:1483-1486;
+ * JointKind::rope_checked, crates/physics/src/joint.rs:539-553; rows
+ * assembly.rs:344-362, dual joints_ext.rs:618-641, catch joints_ext.rs:
+ * 173-223). */
+int oavbd_add_rope_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                         float max_distance, uint32_t* out_index);
+
+/* Spring between anchors la, lb about the motor's rest length (a
+ * position-level row with a fixed penalty, no dual state).  EINVAL, before
+ * any wake, for a motor failing JointMotor::check (joint.rs:1101-1110).
+ * Replaces: PhysicsEngine::add_joint with JointKind::Spring (mod.rs:
+ * 1487-1493; rows assembly.rs:363-425). */
+int oavbd_add_spring_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                           const oavbd_motor* motor, uint32_t* out_index);
 
 /* Advance by dt seconds through the reference's fixed-step accumulator
  * (1/60 s substeps, at most 4 per call; non-finite or non-positive dt is

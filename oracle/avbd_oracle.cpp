@@ -90,6 +90,8 @@ struct K {
   static T PEN_MAX() { return CF(1.0e10); }
   static T PEN_INIT() { return CF(1.0); }
   static T JOINT_PEN_INIT() { return CF(1.0); }
+  static T LIM_PEN_MAX() { return CF(1e4); }
+  static T MIN_EFF_MASS() { return CF(1e-9); }
   static T GEN_MARGIN() { return CF(0.05); }
   static T EXPAND_SLOP() { return CF(0.005); }
   static T POINT_MATCH() { return CF(0.03); }
@@ -170,8 +172,14 @@ struct Pair {
 };
 
 // Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300): the
-// remaining kinds are out of its scope.
-enum JointKindC : int { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3 };
+// remaining kinds (Wheel, Gear, SixDof, Motor) are out of its scope.
+enum JointKindC : int {
+  JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6
+};
+// JointMotor (joint.rs:962-1126) as a Spring joint carries it: kind
+// (Velocity, Position, Servo), model (AccelerationBased, ForceBased).
+enum MotorKindC : int { MK_VELOCITY = 0, MK_POSITION = 1, MK_SERVO = 2 };
+enum MotorModelC : int { MM_ACCELERATION = 0, MM_FORCE = 1 };
 
 template <class T>
 struct Joint {
@@ -181,6 +189,12 @@ struct Joint {
   V3<T> ax_a{T(1), T(0), T(0)}, ax_b{T(1), T(0), T(0)};  // unit hinge / slide axes (body frames)
   V3<T> dref{T(0), T(0), T(0)};                          // Fixed: assembly anchor offset in A's frame
   Q<T> q_ref{T(0), T(0), T(0), T(1)};                    // Fixed: assembly relative rotation conj(qa) qb
+  // Distance: assembly anchor distance; Rope: maximum; Spring: the motor's
+  // target position (rest length)
+  T ref_val = T(0);
+  // Spring: the motor's stiffness, damping, force budget, target velocity
+  T m_stiff = T(0), m_damp = T(0), m_max = T(0), m_tvel = T(0);
+  int m_kind = MK_POSITION, m_model = MM_ACCELERATION;
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -998,11 +1012,27 @@ struct Engine {
   // motors (mod.rs:1308-1510) with the frames of resolve_joint
   // (joint.rs:1251-1323): hinge / slide axes normalised (a degenerate axis
   // rejects), Fixed captures the assembly rotation and anchor offset.
-  int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb) {
+  //
+  // Distance, Rope and Spring (mod.rs:1479-1493; resolve_joint,
+  // joint.rs:1325-1351): Distance captures the anchors' assembly distance,
+  // Rope takes its maximum, Spring its motor (prm: max distance | target
+  // position, stiffness, damping, max force, target velocity, motor kind,
+  // motor model).  A rope maximum that is not finite and positive, or a
+  // motor failing JointMotor::check (joint.rs:1101-1110), rejects before
+  // the bodies wake (validate_joint runs first, mod.rs:1315).
+  int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
+    if (kind < JK_BALL || kind > JK_SPRING) return -1;
+    if (kind == JK_ROPE && !(prm && std::isfinite(prm[0]) && prm[0] > 0.0f)) return -1;
+    if (kind == JK_SPRING) {
+      if (!prm) return -1;
+      for (int k = 0; k < 5; ++k)
+        if (!std::isfinite(prm[k])) return -1;
+      if (!(prm[1] >= 0.0f && prm[2] >= 0.0f && prm[3] >= 0.0f)) return -1;
+      if (!(prm[5] >= 0.0f && prm[5] <= 2.0f && prm[6] >= 0.0f && prm[6] <= 1.0f)) return -1;
+    }
     wake_body(a);
     wake_body(b);
     if (a == b || a < 0 || b < 0 || a >= (int)bodies.size() || b >= (int)bodies.size()) return -1;
-    if (kind < JK_BALL || kind > JK_FIXED) return -1;
     Joint<T> j;
     j.a = a;
     j.b = b;
@@ -1021,6 +1051,21 @@ struct Engine {
       j.q_ref = quat_mul(conj(A.rot), B.rot);
       V3<T> ra = rotate(A.rot, la), rb = rotate(B.rot, lb);
       j.dref = rotate(conj(A.rot), (B.pos + rb) - (A.pos + ra));
+    }
+    if (kind == JK_DISTANCE) {
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      j.ref_val = length((B.pos + rotate(B.rot, lb)) - (A.pos + rotate(A.rot, la)));
+    } else if (kind == JK_ROPE) {
+      j.ref_val = T(prm[0]);
+    } else if (kind == JK_SPRING) {
+      j.ref_val = T(prm[0]);
+      j.m_stiff = T(prm[1]);
+      j.m_damp = T(prm[2]);
+      j.m_max = T(prm[3]);
+      j.m_tvel = T(prm[4]);
+      j.m_kind = (int)prm[5];
+      j.m_model = (int)prm[6];
     }
     for (int k = 0; k < 3; ++k) {
       j.lam_l[k] = T(0);
@@ -1092,11 +1137,24 @@ struct Engine {
         V3<T> perp = delta - axis * dot(delta, axis);
         return fmax_(length(perp), length(axis - rotate(b.rot, j.ax_b)));
       }
+      case JK_DISTANCE:
+        return std::fabs(length(delta) - j.ref_val);
+      case JK_ROPE:  // slack is rest
+        return fmax_(length(delta) - j.ref_val, T(0));
+      case JK_SPRING:
+        return std::fabs(length(delta) - j.ref_val);
       default:
         return length(delta);
     }
   }
+  // JointMotor::keeps_awake (joint.rs:1116-1126) of a Spring's motor.
+  static bool spring_keeps_awake(const Joint<T>& j) {
+    if (j.kind != JK_SPRING || !(j.m_max > T(0))) return false;
+    if (j.m_kind == MK_VELOCITY) return j.m_tvel != T(0);
+    return j.m_stiff > T(0) || j.m_tvel != T(0);
+  }
   bool joint_driven(const Joint<T>& j) const {
+    if (spring_keeps_awake(j)) return true;
     int m[2] = {j.a, j.b};
     for (int h : m) {
       const Body<T>& b = bodies[h];
@@ -1318,6 +1376,51 @@ struct Engine {
   }
 
   // mod.rs:882-978.
+  // The inverse world inertia applied to w (ang_inv_wa, joints_ext.rs:423-438).
+  V3<T> ang_inv_wa(int h, V3<T> w) const {
+    if (!solvable(h)) return V3<T>{T(0), T(0), T(0)};
+    const Body<T>& b = bodies[h];
+    T iw[3][3], iwi[3][3];
+    world_inertia(b.inertia, b.rot, iw);
+    inverse_symmetric(iw, V3<T>{fmax_(b.inertia.x, T(0)), fmax_(b.inertia.y, T(0)), fmax_(b.inertia.z, T(0))}, iwi);
+    return mat3_vec(iwi, w);
+  }
+  // rope_impulse (joints_ext.rs:173-223): once per step, in joint order, a
+  // stretched rope whose ends separate has the separating anchor velocity
+  // removed through the pair's effective mass.
+  void rope_impulse() {
+    for (const auto& j : joints) {
+      if (j.kind != JK_ROPE) continue;
+      const int ia = j.a, ib = j.b;
+      const V3<T> pa = bodies[ia].pos + rotate(bodies[ia].rot, j.la);
+      const V3<T> pb = bodies[ib].pos + rotate(bodies[ib].rot, j.lb);
+      const V3<T> delta = pb - pa;
+      const T len = length(delta);
+      if (len <= j.ref_val) continue;
+      const V3<T> nrm = delta / len;
+      const V3<T> ra = pa - bodies[ia].pos, rb = pb - bodies[ib].pos;
+      const T ka = eff_inv_mass(ia), kb = eff_inv_mass(ib);
+      const V3<T> ta = cross(ra, nrm), tb = cross(rb, nrm);
+      const T k = ka + kb + dot(ang_inv_wa(ia, ta), ta) + dot(ang_inv_wa(ib, tb), tb);
+      if (k < K<T>::MIN_EFF_MASS()) continue;
+      const V3<T> va = bodies[ia].vel + cross(bodies[ia].angvel, ra);
+      const V3<T> vb = bodies[ib].vel + cross(bodies[ib].angvel, rb);
+      const T vsep = dot(vb - va, nrm);
+      if (vsep <= T(0)) continue;
+      const T dj = -vsep / k;
+      if (solvable(ia)) {
+        const V3<T> wa = ang_inv_wa(ia, ta);
+        bodies[ia].vel = bodies[ia].vel - nrm * (dj * ka);
+        bodies[ia].angvel = bodies[ia].angvel - wa * dj;
+      }
+      if (solvable(ib)) {
+        const V3<T> wb = ang_inv_wa(ib, tb);
+        bodies[ib].vel = bodies[ib].vel + nrm * (dj * kb);
+        bodies[ib].angvel = bodies[ib].angvel + wb * dj;
+      }
+    }
+  }
+
   void toi_clamp() {
     int n = (int)bodies.size();
     toi_t.assign(n, T(-1));
@@ -2011,7 +2114,46 @@ struct Engine {
       const bool fixed = j.kind == JK_FIXED;
       // live slide / hinge axis, from A's side (assembly.rs:273-275)
       const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
-      if (j.kind != JK_PRISMATIC) {
+      if (j.kind >= JK_DISTANCE) {
+        // Distance / Rope / Spring: one row along the live anchor delta
+        // (assembly.rs:328-425)
+        const T len = length(live);
+        const V3<T> dir = len > K<T>::NEAR_ZERO() ? live / len : V3<T>{T(0), T(1), T(0)};
+        const V3<T> r_side = is_a ? rotate(a.rot, j.la) : rotate(b.rot, j.lb);
+        if (j.kind != JK_SPRING) {
+          // a rope stamps only while stretched
+          if (j.kind == JK_DISTANCE || len > j.ref_val) {
+            T c = (len - j.ref_val) - K<T>::ALPHA() * (length(c0v) - j.ref_val);
+            T f = j.pen_l[0] * c + j.lam_l[0];
+            if (row_live(c, f)) {
+              acc.stamp(dir, j.pen_l[0], f, r_side, sign);
+              fl = dir * f;
+            }
+          }
+        } else {
+          // position-level spring about the rest length: a fixed penalty
+          // from the stiffness (scaled by the reduced linear mass for the
+          // acceleration model), damping from the step delta, no dual state
+          const T sp = len - j.ref_val;
+          const T s0 = length(c0v) - j.ref_val;
+          const T ka = a.type == BT_DYNAMIC ? a.inv_mass : T(0);
+          const T kb = b.type == BT_DYNAMIC ? b.inv_mass : T(0);
+          const T m = ka + kb > K<T>::NEAR_ZERO() ? T(1) / (ka + kb) : T(0);
+          if (m <= T(0)) continue;
+          const bool force = j.m_model == MM_FORCE;
+          const T stiff = force ? j.m_stiff : j.m_stiff * m;
+          const T dampc = force ? j.m_damp : j.m_damp * m;
+          if (stiff <= T(0) && dampc <= T(0)) continue;
+          if (stiff > T(0)) {
+            const T f = stiff * sp + dampc * (sp - s0) / K<T>::DT();
+            acc.stamp(dir, stiff, f, r_side, sign);
+          } else {  // a pure damper
+            const T pen_v = dampc / K<T>::DT();
+            const T f = dampc * (sp - s0) / K<T>::DT();
+            acc.stamp(dir, pen_v, f, r_side, sign);
+          }
+        }
+      } else if (j.kind != JK_PRISMATIC) {
         // Ball / Revolute / Fixed: three world-axis rows at the anchors
         // (assembly.rs:279-310), Fixed offset by the assembly delta
         const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;
@@ -2178,7 +2320,29 @@ struct Engine {
       V3<T> pb0 = pos0[j.b] + rotate(rot0[j.b], j.lb);
       V3<T> live = pa - pb, c0v = pa0 - pb0;
       const bool fixed = j.kind == JK_FIXED;
-      if (j.kind != JK_PRISMATIC) {
+      if (j.kind == JK_DISTANCE) {
+        const T len = length(live);
+        T c = (len - j.ref_val) - K<T>::ALPHA() * (length(c0v) - j.ref_val);
+        if (std::fabs(c) >= K<T>::C_EPS()) {
+          j.lam_l[0] += j.pen_l[0] * c;
+          j.pen_l[0] = fmin_(j.pen_l[0] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+        }
+      } else if (j.kind == JK_ROPE) {
+        // one-sided: tension only, forgotten when slack; the penalty ramp
+        // capped like a limit row's (joints_ext.rs:618-641)
+        const T len = length(live);
+        if (len > j.ref_val) {
+          T c = (len - j.ref_val) - K<T>::ALPHA() * (length(c0v) - j.ref_val);
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            j.lam_l[0] = fmax_(j.lam_l[0] + j.pen_l[0] * c, T(0));
+            j.pen_l[0] = fmin_(j.pen_l[0] + K<T>::BETA() * std::fabs(c), K<T>::LIM_PEN_MAX());
+          }
+        } else {
+          j.lam_l[0] = T(0);
+        }
+      } else if (j.kind == JK_SPRING) {
+        // no dual state
+      } else if (j.kind != JK_PRISMATIC) {
         const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;
         const V3<T> in0 = fixed ? c0v + rotate(rot0[j.a], j.dref) : c0v;
         for (int k = 0; k < 3; ++k) {
@@ -2491,6 +2655,7 @@ struct Engine {
       b.torque = V3<T>{T(0), T(0), T(0)};
       b.angvel = b.angvel + mat3_vec(iwi, torque) * DT;
     }
+    rope_impulse();
     pre_vel.resize(n);
     for (int h = 0; h < n; ++h) pre_vel[h] = bodies[h].vel;  // mod.rs:874-876
     toi_clamp();
@@ -2770,6 +2935,21 @@ int orc_add_joint(void* h, int kind, int a, int b, const float* la, const float*
     using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
     r = E->add_joint(kind, a, b, V3<TT>{TT(la[0]), TT(la[1]), TT(la[2])}, V3<TT>{TT(lb[0]), TT(lb[1]), TT(lb[2])},
                      V3<TT>{TT(axa[0]), TT(axa[1]), TT(axa[2])}, V3<TT>{TT(axb[0]), TT(axb[1]), TT(axb[2])});
+  });
+  return r;
+}
+
+// add_joint with a parameter block (Rope: prm[0] = maximum; Spring:
+// target position, stiffness, damping, max force, target velocity, motor
+// kind, motor model); axa / axb unused by those kinds.
+int orc_add_joint_prm(void* h, int kind, int a, int b, const float* la, const float* lb, const float* axa,
+                      const float* axb, const float* prm) {
+  OracleCtx* c = (OracleCtx*)h;
+  int r = -1;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    r = E->add_joint(kind, a, b, V3<TT>{TT(la[0]), TT(la[1]), TT(la[2])}, V3<TT>{TT(lb[0]), TT(lb[1]), TT(lb[2])},
+                     V3<TT>{TT(axa[0]), TT(axa[1]), TT(axa[2])}, V3<TT>{TT(axb[0]), TT(axb[1]), TT(axb[2])}, prm);
   });
   return r;
 }

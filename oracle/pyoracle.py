@@ -50,6 +50,9 @@ def lib():
         L.orc_add_joint.restype = ctypes.c_int
         L.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p,
                                     _f32p]
+        L.orc_add_joint_prm.restype = ctypes.c_int
+        L.orc_add_joint_prm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p,
+                                        _f32p, _f32p, _f32p]
         L.orc_add_ball_joint.restype = ctypes.c_int
         L.orc_add_ball_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _f32p, _f32p]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_int]
@@ -311,11 +314,21 @@ class OracleEngine:
         self._L.orc_get_sleep(self.h, _p(a, _u8p), _p(t, _f32p))
         return a, t
 
-    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
-        """add_joint: kind 0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed (local
-        anchors la, lb; local hinge / slide axes for kinds 1 and 2)."""
-        r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
-                                  _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))
+    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0), prm=None):
+        """add_joint: kind 0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed, 4 Distance,
+        5 Rope, 6 Spring (local anchors la, lb; local hinge / slide axes for
+        kinds 1 and 2; prm for Rope (max distance) and Spring (target
+        position, stiffness, damping, max force, target velocity, motor
+        kind, motor model))."""
+        if prm is None:
+            r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
+                                      _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))
+        else:
+            pr = np.zeros(8, np.float32)
+            pr[:len(prm)] = prm
+            r = self._L.orc_add_joint_prm(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p),
+                                          _p(f32(lb), _f32p), _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p),
+                                          _p(pr, _f32p))
         if r < 0:
             raise ValueError("bad joint")
         return r

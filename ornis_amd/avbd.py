@@ -39,6 +39,13 @@ class OavbdStats(ctypes.Structure):
         "tail_colours", "replays", "unwaited")]
 
 
+class OavbdMotor(ctypes.Structure):
+    """oavbd_motor: a Spring joint's motor (JointMotor, joint.rs:962-1126)."""
+    _fields_ = [("kind", ctypes.c_uint32), ("model", ctypes.c_uint32), ("target_position", ctypes.c_float),
+                ("target_velocity", ctypes.c_float), ("stiffness", ctypes.c_float), ("damping", ctypes.c_float),
+                ("max_force", ctypes.c_float)]
+
+
 class OavbdEvent(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("a", ctypes.c_uint32), ("b", ctypes.c_uint32),
                 ("point", ctypes.c_float * 3), ("normal", ctypes.c_float * 3), ("approach", ctypes.c_float)]
@@ -93,6 +100,10 @@ SIGNATURES = {
     "oavbd_get_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
     "oavbd_get_toi": (ctypes.c_int, [ctypes.c_void_p, _f32p, _i32p, ctypes.c_uint32]),
     "oavbd_get_joints": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
+    "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                            ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
+    "oavbd_add_spring_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                              ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdStats)]),
     "oavbd_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "oavbd_get_kernel_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OavbdKernelTimes), ctypes.c_int]),
@@ -301,13 +312,28 @@ class AvbdEngine:
         t = np.ascontiguousarray(timer, dtype=np.float32)
         self._check(self._L.oavbd_set_sleep(self._ctx, _p(a, _u8p), _p(t, _f32p), self.n))
 
-    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
-        """Joint of `kind` (0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed)."""
+    def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0), prm=None):
+        """Joint of `kind` (0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed, 4 Distance,
+        5 Rope with prm = (max distance,), 6 Spring with prm = (rest length,
+        stiffness, damping, max force, target velocity, motor kind, motor
+        model))."""
         la = np.ascontiguousarray(la, dtype=np.float32)
         lb = np.ascontiguousarray(lb, dtype=np.float32)
+        idx = ctypes.c_uint32()
+        if int(kind) == 5:
+            self._check(self._L.oavbd_add_rope_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
+                                                     ctypes.c_float(float(prm[0])), ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
+        if int(kind) == 6:
+            m = OavbdMotor(kind=int(prm[5]), model=int(prm[6]), target_position=prm[0], target_velocity=prm[4],
+                           stiffness=prm[1], damping=prm[2], max_force=prm[3])
+            self._check(self._L.oavbd_add_spring_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
+                                                       ctypes.byref(m), ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
         xa = np.ascontiguousarray(axis_a, dtype=np.float32)
         xb = np.ascontiguousarray(axis_b, dtype=np.float32)
-        idx = ctypes.c_uint32()
         self._check(self._L.oavbd_add_joint(self._ctx, int(kind), int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
                                             _p(xa, _f32p), _p(xb, _f32p), ctypes.byref(idx)))
         self.njoints += 1

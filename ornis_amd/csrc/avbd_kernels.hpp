@@ -121,6 +121,15 @@ __device__ __forceinline__ bool jointed_dev(const Dev& d, uint32_t a, uint32_t b
 // ---------------------------------------------------------------------------
 // phase 1: torque and TOI mover flags (mod.rs:855-871, :882-895)
 // ---------------------------------------------------------------------------
+// The TOI gate (mod.rs:882-895) of a solvable body: awake, not a trigger,
+// and its step displacement above K_CCD_GATE of its smallest dimension.
+__device__ __forceinline__ uint8_t fast_mover(const Dev& d, uint32_t h) {
+  if (d.b.asleep[h] || d.b.trigger[h]) return 0;
+  f3 disp = mul(xyz(d.b.vel[h]), K_DT);
+  float4 sh = d.b.shape[h];
+  return !(len3(disp) <= K_CCD_GATE * shape_min_dim(d.b.kind[h], xyz(sh))) ? 1 : 0;
+}
+
 __device__ __forceinline__ uint8_t begin_body(const Dev& d, uint32_t h) {
   float4 p = d.b.pos[h];
   bool solvable = d.b.type[h] == BT_DYNAMIC && p.w > 0.0f;
@@ -138,11 +147,7 @@ __device__ __forceinline__ uint8_t begin_body(const Dev& d, uint32_t h) {
     float4 w = d.b.angvel[h];
     f3 nw = add(xyz(w), mul(mat3_vec(iwi, xyz(tq)), K_DT));
     d.b.angvel[h] = pk(nw, w.w);
-    if (!d.b.asleep[h] && !d.b.trigger[h]) {
-      f3 disp = mul(xyz(d.b.vel[h]), K_DT);
-      float4 sh = d.b.shape[h];
-      if (!(len3(disp) <= K_CCD_GATE * shape_min_dim(d.b.kind[h], xyz(sh)))) fast = 1;
-    }
+    fast = fast_mover(d, h);
   }
   d.fast[h] = fast;
   return fast;
@@ -152,6 +157,68 @@ __global__ void __launch_bounds__(TPB) k_begin(Dev d) {
   uint8_t fast = 0;
   if (h < d.n) fast = begin_body(d, h);
   block_add(&d.counters[CTR_FAST], fast);
+}
+
+// The inverse world inertia of body h applied to w (ang_inv_wa,
+// joints_ext.rs:423-438); zero unless the body is solvable.
+__device__ __forceinline__ f3 ang_inv_wa_dev(const Dev& d, uint32_t h, f3 w) {
+  if (!(d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f)) return zero3();
+  const f3 I = xyz(d.b.inertia[h]);
+  float iw[3][3], iwi[3][3];
+  world_inertia(I, asq(d.b.rot[h]), iw);
+  inverse_symmetric(iw, f3{fmax_(I.x, 0.0f), fmax_(I.y, 0.0f), fmax_(I.z, 0.0f)}, iwi);
+  return mat3_vec(iwi, w);
+}
+
+// rope_impulse (joints_ext.rs:173-223): once per step, after the torques,
+// every stretched rope whose ends separate has the separating anchor
+// velocity removed through the pair's effective mass.  Ropes sharing a body
+// see each other's updates, so one lane walks them in joint order.
+__global__ void k_rope_impulse(Dev d) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint32_t r = 0; r < d.nropes; ++r) {
+    const uint32_t j = d.ropes[r];
+    const uint2 ab = d.J.ab[j];
+    const float max = d.J.dref[j].x;
+    const float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+    const f3 pa = add(xyz(pa4), qrot(asq(d.b.rot[ab.x]), xyz(d.J.la[j])));
+    const f3 pb = add(xyz(pb4), qrot(asq(d.b.rot[ab.y]), xyz(d.J.lb[j])));
+    const f3 delta = sub(pb, pa);
+    const float len = len3(delta);
+    if (len <= max) continue;
+    const f3 nrm = divs(delta, len);
+    const f3 ra = sub(pa, xyz(pa4)), rb = sub(pb, xyz(pb4));
+    const float ka = d.b.type[ab.x] == BT_DYNAMIC ? pa4.w : 0.0f;
+    const float kb = d.b.type[ab.y] == BT_DYNAMIC ? pb4.w : 0.0f;
+    const f3 ta = cross3(ra, nrm), tb = cross3(rb, nrm);
+    const float k = ka + kb + dot3(ang_inv_wa_dev(d, ab.x, ta), ta) + dot3(ang_inv_wa_dev(d, ab.y, tb), tb);
+    if (k < K_MIN_EFF_MASS) continue;
+    const float4 va4 = d.b.vel[ab.x], vb4 = d.b.vel[ab.y], wa4 = d.b.angvel[ab.x], wb4 = d.b.angvel[ab.y];
+    const f3 va = add(xyz(va4), cross3(xyz(wa4), ra));
+    const f3 vb = add(xyz(vb4), cross3(xyz(wb4), rb));
+    const float vsep = dot3(sub(vb, va), nrm);
+    if (vsep <= 0.0f) continue;
+    const float dj = -vsep / k;
+    if (d.b.type[ab.x] == BT_DYNAMIC && pa4.w > 0.0f) {
+      const f3 ia = ang_inv_wa_dev(d, ab.x, ta);
+      d.b.vel[ab.x] = pk(sub(xyz(va4), mul(nrm, dj * ka)), va4.w);
+      d.b.angvel[ab.x] = pk(sub(xyz(wa4), mul(ia, dj)), wa4.w);
+    }
+    if (d.b.type[ab.y] == BT_DYNAMIC && pb4.w > 0.0f) {
+      const f3 ib = ang_inv_wa_dev(d, ab.y, tb);
+      const float4 vb2 = d.b.vel[ab.y], wb2 = d.b.angvel[ab.y];  // body a's update above may be this body's
+      d.b.vel[ab.y] = pk(add(xyz(vb2), mul(nrm, dj * kb)), vb2.w);
+      d.b.angvel[ab.y] = pk(add(xyz(wb2), mul(ib, dj)), wb2.w);
+    }
+  }
+}
+
+// The TOI gate again after the rope catch changed velocities.
+__global__ void __launch_bounds__(TPB) k_refast(Dev d) {
+  uint32_t h = gid();
+  if (h >= d.n) return;
+  const bool solvable = d.b.type[h] == BT_DYNAMIC && d.b.pos[h].w > 0.0f;
+  d.fast[h] = solvable ? fast_mover(d, h) : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1247,14 +1314,18 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
         return true;
     }
   }
+  const int kind = d.J.kind[j];
+  if (kind == JK_SPRING && ((uint32_t)d.J.dref[j].w & SPRING_KEEPS_AWAKE)) return true;
   const qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
   f3 pa = add(xyz(d.b.pos[ab.x]), qrot(qa, xyz(d.J.la[j])));
   f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, xyz(d.J.lb[j])));
   const f3 delta = sub(pb, pa);
   float err;
-  const int kind = d.J.kind[j];
   if (kind == JK_BALL) {
     err = len3(delta);
+  } else if (kind >= JK_DISTANCE) {  // a slack rope is at rest
+    const float e = len3(delta) - d.J.dref[j].x;
+    err = kind == JK_ROPE ? fmax_(e, 0.0f) : fabsf(e);
   } else if (kind == JK_FIXED) {  // joint_rest_error (sleep.rs:30-66)
     err = fmax_(len3(sub(delta, qrot(qa, xyz(d.J.dref[j])))), len3(quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]))));
   } else {
@@ -2124,7 +2195,46 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   f3 r_b = qrot(qb, lb);
   float fl[3] = {0.0f, 0.0f, 0.0f};
   Row row;
-  if (kind != JK_PRISMATIC) {
+  if (kind >= JK_DISTANCE) {
+    // Distance / Rope / Spring: one row along the live anchor delta
+    // (assembly.rs:328-425)
+    const float4 pr = d.J.dref[j];
+    const float len = len3(live);
+    const f3 dir = len > K_NEAR_ZERO ? divs(live, len) : f3{0.0f, 1.0f, 0.0f};
+    const f3 r_side = is_a ? qrot(qa, la) : r_b;
+    if (kind != JK_SPRING) {
+      if (kind == JK_DISTANCE || len > pr.x) {  // a rope stamps only while stretched
+        const float c = (len - pr.x) - K_ALPHA * (len3(c0v) - pr.x);
+        const float f = penv[0] * c + lamv[0];
+        if (fabsf(c) >= K_C_EPS || f != 0.0f) {
+          make_stamp(row, dir, penv[0], f, r_side, sign);
+          emit(0, row);
+          fl[0] = dir.x * f;
+          fl[1] = dir.y * f;
+          fl[2] = dir.z * f;
+        }
+      }
+    } else {
+      // position-level spring about the rest length: fixed penalty, no dual
+      const float sp = len - pr.x;
+      const float s0 = len3(c0v) - pr.x;
+      const float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+      const float ka = d.b.type[ab.x] == BT_DYNAMIC ? pa4.w : 0.0f;
+      const float kb = d.b.type[ab.y] == BT_DYNAMIC ? pb4.w : 0.0f;
+      const float m = ka + kb > K_NEAR_ZERO ? 1.0f / (ka + kb) : 0.0f;
+      if (m <= 0.0f) return;
+      const bool force = ((uint32_t)pr.w & SPRING_FORCE_MODEL) != 0u;
+      const float stiff = force ? pr.y : pr.y * m;
+      const float dampc = force ? pr.z : pr.z * m;
+      if (stiff <= 0.0f && dampc <= 0.0f) return;
+      if (stiff > 0.0f) {
+        make_stamp(row, dir, stiff, stiff * sp + dampc * (sp - s0) / K_DT, r_side, sign);
+      } else {  // a pure damper
+        make_stamp(row, dir, dampc / K_DT, dampc * (sp - s0) / K_DT, r_side, sign);
+      }
+      emit(0, row);
+    }
+  } else if (kind != JK_PRISMATIC) {
     if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -2177,7 +2287,7 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   row.pen = 0.0f;
   row.f = 0.0f;
   emit(2, row);
-  if (kind == JK_BALL) return;
+  if (kind == JK_BALL || kind >= JK_DISTANCE) return;
   const float4 lama = d.J.lama[j], pena = d.J.pena[j];
   const float lav[3] = {lama.x, lama.y, lama.z}, pav[3] = {pena.x, pena.y, pena.z};
   auto angular = [&](f3 g, float pn, float f) {
@@ -2923,6 +3033,29 @@ __global__ void k_dual_joint(Dev d) {
   f3 c0v = xyz(d.J.c0v[j]);
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
   float lv[3] = {lam.x, lam.y, lam.z}, pv[3] = {pen.x, pen.y, pen.z};
+  if (kind >= JK_DISTANCE) {
+    if (kind == JK_SPRING) return;  // no dual state
+    const float ref = d.J.dref[j].x;
+    const float len = len3(live);
+    if (kind == JK_DISTANCE) {
+      const float c = (len - ref) - K_ALPHA * (len3(c0v) - ref);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[0] += pv[0] * c;
+        pv[0] = fmin_(pv[0] + K_BETA * fabsf(c), K_PEN_MAX);
+      }
+    } else if (len > ref) {  // rope: tension only, capped like a limit row
+      const float c = (len - ref) - K_ALPHA * (len3(c0v) - ref);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[0] = fmax_(lv[0] + pv[0] * c, 0.0f);
+        pv[0] = fmin_(pv[0] + K_BETA * fabsf(c), K_LIM_PEN_MAX);
+      }
+    } else {
+      lv[0] = 0.0f;  // slack forgets the force
+    }
+    d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], 0.0f);
+    d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+    return;
+  }
   if (kind != JK_PRISMATIC) {
     if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
     for (int k = 0; k < 3; ++k) {

@@ -40,6 +40,8 @@
 #define K_SHAPE_TOUCH 1e-3f
 #define K_CCD_GATE 0.5f
 #define K_JOINT_REST 0.005f
+#define K_LIM_PEN_MAX 1e4f
+#define K_MIN_EFF_MASS 1e-9f
 #define K_MAX_POINTS 8
 #define COLOUR_RING 64   // frontier sizes kept per round number modulo this (a batch is shorter)
 #define CSTAT_FRONT 96   // words of the three frontier counters (own cache line)
@@ -138,7 +140,10 @@ struct Points {
 };
 
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
-enum { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3 };
+enum { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6 };
+// Flags of a Spring joint (the w of its parameter record): the motor's
+// force-based model; the motor keeps its bodies awake (JointMotor::keeps_awake)
+enum { SPRING_FORCE_MODEL = 1, SPRING_KEEPS_AWAKE = 2 };
 
 struct Joints {
   uint2* ab;
@@ -147,7 +152,8 @@ struct Joints {
   float4* lb;        // local anchor on B
   float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
   float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames
-  float4* dref;      // Fixed: assembly anchor offset in A's frame
+  float4* dref;      // Fixed: assembly anchor offset in A's frame; Distance / Rope / Spring:
+                     // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags)
   float4* qref;      // Fixed: assembly relative rotation conj(qa) qb
   float4* lam;       // lam_l xyz
   float4* pen;       // pen_l xyz
@@ -201,6 +207,8 @@ struct Dev {
   uint32_t npairs, npts, npairs_old, npts_old;
   Joints J;
   uint32_t njoints;
+  uint32_t* ropes;   // Rope joint indices, ascending (the velocity catch runs in joint order)
+  uint32_t nropes;
   Grid g;
   Cands C;
   float gravity[3];

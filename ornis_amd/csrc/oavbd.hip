@@ -71,6 +71,9 @@ struct HostJoint {
   float axb[3] = {1.0f, 0.0f, 0.0f};
   float dref[3] = {0.0f, 0.0f, 0.0f};  // Fixed: assembly anchor offset in A's frame
   float qref[4] = {0.0f, 0.0f, 0.0f, 1.0f};  // Fixed: assembly relative rotation
+  // Distance: assembly distance; Rope: maximum; Spring: rest length,
+  // stiffness, damping, SPRING_* flags
+  float prm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 };
 
 }  // namespace
@@ -439,6 +442,7 @@ static void free_all(oavbd_ctx* c) {
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.kind); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.ax); dfree(d.J.dref);
   dfree(d.J.qref); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.lama); dfree(d.J.pena); dfree(d.J.c0v); dfree(d.J.a0);
+  dfree(d.ropes);
   dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
   dfree(d.g.is_large);
   Cands& C = d.C;
@@ -596,12 +600,13 @@ static void upload_joints(oavbd_ctx* c) {
   Joints& J = d.J;
   dfree(J.ab); dfree(J.kind); dfree(J.la); dfree(J.lb); dfree(J.la0); dfree(J.ax); dfree(J.dref); dfree(J.qref);
   dfree(J.lam); dfree(J.pen); dfree(J.lama); dfree(J.pena); dfree(J.c0v); dfree(J.a0);
-  dfree(d.b.jp); dfree(d.b.ji);
+  dfree(d.b.jp); dfree(d.b.ji); dfree(d.ropes);
   dalloc(&J.ab, nj); dalloc(&J.kind, nj); dalloc(&J.la, nj); dalloc(&J.lb, nj); dalloc(&J.la0, nj);
   dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
   dalloc(&J.lama, nj); dalloc(&J.pena, nj); dalloc(&J.c0v, nj); dalloc(&J.a0, 2 * (size_t)nj);
   std::vector<uint2> ab(nj);
   std::vector<int> kind(nj);
+  std::vector<uint32_t> ropes;
   std::vector<float4> la(nj), lb(nj), la0(nj), ax(2 * (size_t)nj), dref(nj), qref(nj), lam(nj), pen(nj), lama(nj),
       pena(nj);
   // no-collide partners: every kind but Fixed (rows.rs:437-450); the
@@ -620,7 +625,9 @@ static void upload_joints(oavbd_ctx* c) {
                          h.la[2] + (fixed ? h.dref[2] : 0.0f), 0.0f);
     ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], 0.0f);
     ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], 0.0f);
-    dref[j] = make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    dref[j] = h.kind >= JK_DISTANCE ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
+                                    : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    if (h.kind == JK_ROPE) ropes.push_back(j);
     qref[j] = make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
     lam[j] = j < kept ? kept_lam[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -672,6 +679,9 @@ static void upload_joints(oavbd_ctx* c) {
   up(J.ab, ab); up(J.kind, kind); up(J.la, la); up(J.lb, lb); up(J.la0, la0); up(J.ax, ax); up(J.dref, dref);
   up(J.qref, qref); up(J.lam, lam); up(J.pen, pen); up(J.lama, lama); up(J.pena, pena);
   up(d.b.jp_off, jp_off); up(d.b.ji_off, ji_off); up(d.b.jp, jp); up(d.b.ji, ji); up(d.b.comp, comp);
+  if (!ropes.empty()) dalloc(&d.ropes, ropes.size());
+  up(d.ropes, ropes);
+  d.nropes = (uint32_t)ropes.size();
   HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   d.njoints = nj;
   c->joints_dirty = false;
@@ -1357,6 +1367,10 @@ static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
   } else {
     if (may_sleep) wake_joint_motion(c);
     k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
+    if (d.nropes) {  // the rope catch runs after the torques, before the TOI gate
+      k_rope_impulse<<<1, 64, 0, c->st>>>(d);
+      k_refast<<<blocks(n), TPB, 0, c->st>>>(d);
+    }
   }
   build_grid(c);
   // TOI: every body resets its clamp; fast movers count their targets
@@ -2335,9 +2349,10 @@ int oavbd_set_sleep(oavbd_ctx* c, const uint8_t* asleep, const float* timer, uin
   return OAVBD_OK;
 }
 
-int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
-                    const float axis_a[3], const float axis_b[3], uint32_t* out_index) {
-  if (!c || !la || !lb) return OAVBD_EINVAL;
+// The common body of the add_joint entry points: `prm` carries a Rope's
+// maximum or a Spring's motor (already validated), as HostJoint::prm.
+static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                          const float axis_a[3], const float axis_b[3], const float prm[4], uint32_t* out_index) {
   const uint32_t n = c->d.n;
   // the reference wakes both members before it validates the spec
   // (mod.rs:1317-1318)
@@ -2352,7 +2367,6 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
     HIPC(hipStreamSynchronize(c->st));
   });
   if (a == b || a >= n || b >= n) return fail(c, OAVBD_EINVAL, "bad joint bodies");
-  if (kind > JK_FIXED) return fail(c, OAVBD_EINVAL, "unsupported joint kind");
   if (c->joints.size() >= c->max_joints) return fail(c, OAVBD_ECAPACITY, "more joints than max_joints");
   HostJoint j{a, b, {la[0], la[1], la[2]}, {lb[0], lb[1], lb[2]}};
   j.kind = (int)kind;
@@ -2366,9 +2380,11 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
     j.axa[0] = na.x; j.axa[1] = na.y; j.axa[2] = na.z;
     j.axb[0] = nb.x; j.axb[1] = nb.y; j.axb[2] = nb.z;
   }
-  if (kind == JK_FIXED) {
-    // resolve_joint (joint.rs:1313-1323): the assembly rotation and anchor
-    // offset from the bodies' current poses
+  if (prm)
+    for (int k = 0; k < 4; ++k) j.prm[k] = prm[k];
+  if (kind == JK_FIXED || kind == JK_DISTANCE) {
+    // resolve_joint (joint.rs:1313-1332): the assembly rotation and anchor
+    // offset (Fixed) or anchor distance (Distance) from the current poses
     float4 p[2], q[2];
     GUARD(c, {
       HIPC(hipMemcpy(&p[0], c->d.b.pos + a, sizeof(float4), hipMemcpyDeviceToHost));
@@ -2380,8 +2396,12 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
     const qf qr = qmul(qconj(qa), qb);
     const f3 ra = qrot(qa, f3{la[0], la[1], la[2]}), rb = qrot(qb, f3{lb[0], lb[1], lb[2]});
     const f3 dr = qrot(qconj(qa), sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
-    j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
-    j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
+    if (kind == JK_FIXED) {
+      j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
+      j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
+    } else {
+      j.prm[0] = len3(sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
+    }
   }
   c->joints.push_back(j);
   c->joints_dirty = true;
@@ -2389,9 +2409,47 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
   return OAVBD_OK;
 }
 
+int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                    const float axis_a[3], const float axis_b[3], uint32_t* out_index) {
+  if (!c || !la || !lb) return OAVBD_EINVAL;
+  if (kind == JK_ROPE || kind == JK_SPRING)
+    return fail(c, OAVBD_EINVAL, "rope and spring joints take oavbd_add_rope_joint / oavbd_add_spring_joint");
+  if (kind > JK_SPRING) return fail(c, OAVBD_EINVAL, "unsupported joint kind");
+  return add_joint_impl(c, kind, a, b, la, lb, axis_a, axis_b, nullptr, out_index);
+}
+
 int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                          uint32_t* out_index) {
   return oavbd_add_joint(c, JK_BALL, a, b, la, lb, nullptr, nullptr, out_index);
+}
+
+int oavbd_add_rope_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                         float max_distance, uint32_t* out_index) {
+  if (!c || !la || !lb) return OAVBD_EINVAL;
+  // JointKind::rope_checked (joint.rs:539-553), before any wake
+  if (!(std::isfinite(max_distance) && max_distance > 0.0f))
+    return fail(c, OAVBD_EINVAL, "rope maximum must be finite and positive");
+  const float prm[4] = {max_distance, 0.0f, 0.0f, 0.0f};
+  return add_joint_impl(c, JK_ROPE, a, b, la, lb, nullptr, nullptr, prm, out_index);
+}
+
+int oavbd_add_spring_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                           const oavbd_motor* m, uint32_t* out_index) {
+  if (!c || !la || !lb || !m) return OAVBD_EINVAL;
+  // JointMotor::check (joint.rs:1101-1110), before any wake
+  const bool ok = std::isfinite(m->target_position) && std::isfinite(m->target_velocity) &&
+                  std::isfinite(m->stiffness) && m->stiffness >= 0.0f && std::isfinite(m->damping) &&
+                  m->damping >= 0.0f && std::isfinite(m->max_force) && m->max_force >= 0.0f &&
+                  m->kind <= OAVBD_MOTOR_SERVO && m->model <= OAVBD_MOTOR_FORCE_BASED;
+  if (!ok) return fail(c, OAVBD_EINVAL, "bad spring motor");
+  // JointMotor::keeps_awake (joint.rs:1116-1126)
+  const bool awake = m->max_force > 0.0f && (m->kind == OAVBD_MOTOR_VELOCITY
+                                                 ? m->target_velocity != 0.0f
+                                                 : (m->stiffness > 0.0f || m->target_velocity != 0.0f));
+  const float flags = (float)((m->model == OAVBD_MOTOR_FORCE_BASED ? SPRING_FORCE_MODEL : 0) |
+                              (awake ? SPRING_KEEPS_AWAKE : 0));
+  const float prm[4] = {m->target_position, m->stiffness, m->damping, flags};
+  return add_joint_impl(c, JK_SPRING, a, b, la, lb, nullptr, nullptr, prm, out_index);
 }
 
 // Both tables of a double-buffered pair (or point) table hold at least

@@ -28,6 +28,10 @@ _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 
 # joint kinds (include/oavbd.h)
 JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
+JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING = 4, 5, 6
+# motor kinds and models of a Spring joint (oavbd_motor)
+MOTOR_VELOCITY, MOTOR_POSITION, MOTOR_SERVO = 0, 1, 2
+MOTOR_ACCELERATION_BASED, MOTOR_FORCE_BASED = 0, 1
 
 
 class SceneBuilder:
@@ -79,6 +83,20 @@ class SceneBuilder:
         JOINT_FIXED between bodies a and b: local anchors la, lb and (hinge /
         slide kinds) local axes."""
         self.joints.append((int(a), int(b), tuple(la), tuple(lb), int(kind), tuple(axis_a), tuple(axis_b)))
+        return len(self.joints) - 1
+
+    def rope(self, a, b, la, lb, max_distance):
+        """Rope joint: the anchors part at most max_distance."""
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_ROPE, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
+                            (float(max_distance),)))
+        return len(self.joints) - 1
+
+    def spring(self, a, b, la, lb, rest, stiffness, damping, max_force=1e6, target_velocity=0.0,
+               motor_kind=MOTOR_POSITION, model=MOTOR_ACCELERATION_BASED):
+        """Spring joint about `rest` (a JointMotor of `motor_kind` / `model`)."""
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_SPRING, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
+                            (float(rest), float(stiffness), float(damping), float(max_force), float(target_velocity),
+                             float(motor_kind), float(model))))
         return len(self.joints) - 1
 
     def build(self):

@@ -1,10 +1,13 @@
-"""Revolute, Prismatic and Fixed joints on the GPU against the oracle
-(float32, colour order), bit for bit: bodies, pairs, colourings and the
-joints' linear and angular duals.  The scenes are the reference's own joint
-tests (crates/physics/tests/avbd_engine.rs: hinge :139-183, weld :891-935,
-slider :598-646 without its limit) and chains whose links cycle through
-the four kinds with tilted axes, falling into a field of spheres and onto
-each other, with a body removed mid-run."""
+"""Revolute, Prismatic, Fixed, Distance, Rope and Spring joints on the GPU
+against the oracle (float32, colour order), bit for bit: bodies, pairs,
+colourings and the joints' linear and angular duals.  The scenes are the
+reference's own joint tests (crates/physics/tests/avbd_engine.rs: hinge
+:139-183, weld :891-935, slider :598-646 without its limit, rod :939-966;
+crates/physics/tests/p6_rope_spring_motor.rs: rope :45-105, springs
+:124-216), chains whose links cycle through the first four kinds with
+tilted axes, falling into a field of spheres and onto each other, with a
+body removed mid-run, and a field of tethered bodies (ropes sharing bodies,
+rods, springs of both motor models and a damper) over a floor."""
 import numpy as np
 import pytest
 
@@ -128,4 +131,108 @@ def test_joint_spec_validation(gpu_available):
                  (0, 1, (0, 0, 0), (0, 0, 0), 2, (0, 0, 1), (1e-7, 0, 0))):  # degenerate slide axis
         with pytest.raises(AvbdError):
             g.add_joint(*args)
+    assert g.stats()["joints"] == 0
+
+
+def test_reference_rod_rope_spring_scenes_bitwise(gpu_available):
+    """The reference's rod pendulum, hanging rope and spring oscillators
+    (both motor models, and a pure damper) over their whole runs."""
+    sb = scenes.SceneBuilder()
+    a = sb.sphere((0.0, 0.0, 0.0), 0.1, 0.0)
+    b = sb.sphere((2.0, 0.0, 0.0), 0.25, 1.0)
+    sb.joint(scenes.JOINT_DISTANCE, a, b, (0, 0, 0), (0, 0, 0))
+    g, _ = _run(sb.build(), 300, every=20)
+    assert abs(np.linalg.norm(g.state()[b, 0:3]) - 2.0) < 0.25
+    sb = scenes.SceneBuilder()
+    a = sb.sphere((0.0, 0.0, 0.0), 0.1, 0.0)
+    b = sb.sphere((0.0, -1.0, 0.0), 0.2, 2.0)
+    sb.rope(a, b, (0, 0, 0), (0, 1, 0), 2.0)
+    g, _ = _run(sb.build(), 240, every=10)
+    assert abs(np.linalg.norm(g.state()[b, 0:3] + np.float32([0, 1, 0])) - 2.0) < 0.2
+    for model, kind, k in ((scenes.MOTOR_ACCELERATION_BASED, scenes.MOTOR_POSITION, 30.0),
+                           (scenes.MOTOR_FORCE_BASED, scenes.MOTOR_POSITION, 30.0),
+                           (scenes.MOTOR_FORCE_BASED, scenes.MOTOR_VELOCITY, 0.0)):
+        sb = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+        a = sb.sphere((0.0, 0.0, 0.0), 0.1, 0.0)
+        b = sb.sphere((0.0, -2.5, 0.0), 0.2, 1.0, vel=(0.0, -1.0, 0.0))
+        sb.spring(a, b, (0, 0, 0), (0, 0.5, 0), 1.0, k, 6.0, motor_kind=kind, model=model)
+        g, _ = _run(sb.build(), 600 if k else 120, every=30)
+
+
+def tether_field(seed=5, n=40):
+    """Boxes and spheres hung from static anchors and from each other by
+    ropes (several on one body, so the catch order matters), rods and
+    springs (both motor models, one a damper), dropped over a floor with
+    static spheres."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = scenes.SceneBuilder()
+    b.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=0.5, restitution=0.0)
+    for i in range(8):
+        b.sphere((rng.uniform(-4, 4), 0.4, rng.uniform(-4, 4)), 0.4, 0.0)
+    anchors = [b.box((x, 6.0, z), (0.1, 0.1, 0.1), 0.0) for x in (-3.0, 0.0, 3.0) for z in (-2.0, 2.0)]
+    bodies = []
+    for i in range(n):
+        q = rng.normal(size=4)
+        q = tuple(float(v) for v in (q / np.linalg.norm(q)).astype(np.float32))
+        p = (float(rng.uniform(-4, 4)), float(rng.uniform(2.5, 5.5)), float(rng.uniform(-3, 3)))
+        v = tuple(float(x) for x in rng.uniform(-3, 3, 3))
+        h = b.box(p, tuple(rng.uniform(0.1, 0.3, 3)), float(rng.uniform(0.5, 3.0)), quat=q, vel=v) if i % 2 else \
+            b.sphere(p, float(rng.uniform(0.1, 0.3)), float(rng.uniform(0.5, 3.0)), vel=v)
+        bodies.append(h)
+    for i, h in enumerate(bodies):
+        other = anchors[i % len(anchors)] if i % 3 == 0 else bodies[(i * 7 + 3) % n]
+        if other == h:
+            other = anchors[0]
+        la = tuple(float(x) for x in rng.uniform(-0.1, 0.1, 3))
+        lb = tuple(float(x) for x in rng.uniform(-0.1, 0.1, 3))
+        kind = i % 5
+        if kind in (0, 1):
+            b.rope(other, h, la, lb, float(rng.uniform(1.0, 3.0)))
+        elif kind == 2:
+            b.joint(scenes.JOINT_DISTANCE, other, h, la, lb)
+        elif kind == 3:
+            b.spring(other, h, la, lb, float(rng.uniform(0.5, 2.0)), float(rng.uniform(5, 60)),
+                     float(rng.uniform(0.5, 5)), model=i % 2)
+        else:
+            b.spring(other, h, la, lb, 1.0, 0.0, 3.0, motor_kind=scenes.MOTOR_VELOCITY,
+                     model=scenes.MOTOR_FORCE_BASED)
+    # a second rope on three bodies that already hang from one
+    for i in (0, 5, 10):
+        b.rope(anchors[(i + 1) % len(anchors)], bodies[i], (0, 0, 0), (0, 0, 0), 4.0)
+    return b.build()
+
+
+def test_tether_field_bitwise(gpu_available):
+    s = tether_field()
+    kinds = {j[4] for j in s["joints"]}
+    assert {scenes.JOINT_DISTANCE, scenes.JOINT_ROPE, scenes.JOINT_SPRING} <= kinds
+    g, o = _run(s, 200, every=10)
+    assert g.stats()["breakdowns"] == 0 and np.isfinite(g.state()).all()
+    victim = int(s["joints"][7][1])
+    g.remove_body(victim)
+    o.remove_body(victim)
+    for s_ in range(40):
+        g.step_n(1)
+        o.step_inner(1)
+    _same(g, o, g.njoints, "after a removal")
+
+
+def test_rope_spring_spec_validation(gpu_available):
+    b = scenes.SceneBuilder()
+    b.box((0.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    b.box((1.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    g = AvbdEngine(scene=b.build())
+    from ornis_amd.avbd import AvbdError
+    for m in (0.0, -1.0, float("inf")):
+        with pytest.raises(AvbdError):
+            g.add_joint(0, 1, (0, 0, 0), (0, 0, 0), scenes.JOINT_ROPE, prm=(m,))
+    for prm in ((1.0, -1.0, 1.0, 1.0, 0.0, 1, 0), (1.0, 1.0, 1.0, -1.0, 0.0, 1, 0),
+                (float("nan"), 1.0, 1.0, 1.0, 0.0, 1, 0), (1.0, 1.0, 1.0, 1.0, 0.0, 3, 0)):
+        with pytest.raises(AvbdError):
+            g.add_joint(0, 1, (0, 0, 0), (0, 0, 0), scenes.JOINT_SPRING, prm=prm)
+    from ornis_amd.avbd import _f32p
+    z = np.zeros(3, np.float32)
+    zp = z.ctypes.data_as(_f32p)
+    for kind in (scenes.JOINT_ROPE, scenes.JOINT_SPRING):  # only through their own entry points
+        assert g._L.oavbd_add_joint(g._ctx, kind, 0, 1, zp, zp, None, None, None) != 0
     assert g.stats()["joints"] == 0
