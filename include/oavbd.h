@@ -191,10 +191,11 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
  * oavbd_get_state, oavbd_get_sleep and oavbd_get_contacts. */
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
-/* Joint kinds (the reference's JointKind arms; Revolute and Prismatic
- * without limit or motor; Wheel, Gear, SixDof and Motor are not built).
- * Rope and Spring are added with oavbd_add_rope_joint /
- * oavbd_add_spring_joint, the others with oavbd_add_joint. */
+/* Joint kinds (the reference's JointKind arms; Wheel, Gear, SixDof and
+ * Motor are not built).  Rope and Spring are added with
+ * oavbd_add_rope_joint / oavbd_add_spring_joint, Revolute and Prismatic
+ * with a limit or motor with oavbd_add_joint_limited, the others with
+ * oavbd_add_joint. */
 enum {
   OAVBD_JOINT_BALL = 0,
   OAVBD_JOINT_REVOLUTE = 1,
@@ -240,6 +241,22 @@ int oavbd_add_joint(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const
  * 1273-1279). */
 int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                          uint32_t* out_index);
+
+/* A Revolute or Prismatic joint (as oavbd_add_joint) with an optional
+ * travel window limit[0] <= limit[1] (radians of twist about the hinge
+ * axis from the assembly angle, or metres along the slide axis from the
+ * assembly length; a one-sided row whose force the dual accumulates) and
+ * an optional velocity motor (motor[0] target speed in rad/s or m/s,
+ * motor[1] torque or force budget: a budget-clamped velocity impulse each
+ * step, paused while the limit is violated).  NULL for none.  EINVAL,
+ * before any wake, for a bound that is not finite or lo > hi, or a motor
+ * with a non-finite value or a negative budget.  Replaces:
+ * PhysicsEngine::add_joint for JointKind::{Revolute, Prismatic} with
+ * limit / motor Some (mod.rs:1437-1459; rows assembly.rs:809-874, dual
+ * joints_ext.rs:820-890, motor joints_ext.rs:237-356). */
+int oavbd_add_joint_limited(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const float la[3],
+                            const float lb[3], const float axis_a[3], const float axis_b[3], const float limit[2],
+                            const float motor[2], uint32_t* out_index);
 
 /* Rope between a and b: anchors la, lb may come closer freely but never
  * part beyond max_distance (one-sided row, a velocity catch each step).

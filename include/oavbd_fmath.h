@@ -1,5 +1,5 @@
 /*
- * oavbd_fmath.h — portable single-precision sin/cos/asin for the AVBD step.
+ * oavbd_fmath.h — portable single-precision sin/cos/asin/atan2 for the AVBD step.
  *
  * The AVBD rotation helpers need sin/cos (exponential-map integration,
  * crates/physics/src/avbd/rows.rs:110-119) and asin (angle-axis recovery,
@@ -21,6 +21,15 @@
  *   - asin: Maclaurin series of asin (coefficients (2k)!/(4^k (k!)^2 (2k+1)))
  *     through x^23 on |x| <= 1/2, and asin(x) = pi/2 - 2 asin(sqrt((1-x)/2))
  *     above, so the series argument never exceeds 1/2 (truncation < 3e-9).
+ *
+ *   - atan2: the quadrant and octant are reduced away exactly (|y| <= |x|
+ *     after a swap), then t = |y|/|x| in [0, 1] is moved next to zero with
+ *     the addition formula atan(t) = atan(c) + atan((t - c) / (1 + t c)):
+ *     c = 0 below 7/16, c = 1/2 up to 11/16 (u = (2t - 1) / (2 + t), whose
+ *     numerator is exact there), c = 1 above (u = (t - 1) / (t + 1), exact
+ *     numerator), so |u| <= 7/16; the Maclaurin series of atan through
+ *     u^23 (truncation below 1e-9 relative); atan(1/2), pi/4, pi/2 and pi
+ *     are carried as float high + low parts.
  *
  * Measured accuracy against double-precision libm is pinned by the test
  * suite (tests/test_fmath.py).  Compile every consumer with
@@ -148,6 +157,75 @@ OAVBD_INLINE OAVBD_HD float oavbd_asinf(float x) {
     y = (OAVBD_PIO2_HI - 2.0f * u) + OAVBD_PIO2_LO;
   }
   return x < 0.0f ? -y : y;
+}
+
+/* pi/4 and pi as float high + low parts. */
+#define OAVBD_PIO4_HI 0.785398185253143310546875f
+#define OAVBD_PIO4_LO (-2.1855694143368964e-08f)
+#define OAVBD_PI_HI 3.1415927410125732421875f
+#define OAVBD_PI_LO (-8.742277657347586e-08f)
+
+/* atan(u) for |u| <= 7/16 (Maclaurin through u^23). */
+OAVBD_INLINE OAVBD_HD float oavbd_atan_series(float u) {
+  const float a1 = (float)(-1.0 / 3.0);
+  const float a2 = (float)(1.0 / 5.0);
+  const float a3 = (float)(-1.0 / 7.0);
+  const float a4 = (float)(1.0 / 9.0);
+  const float a5 = (float)(-1.0 / 11.0);
+  const float a6 = (float)(1.0 / 13.0);
+  const float a7 = (float)(-1.0 / 15.0);
+  const float a8 = (float)(1.0 / 17.0);
+  const float a9 = (float)(-1.0 / 19.0);
+  const float a10 = (float)(1.0 / 21.0);
+  const float a11 = (float)(-1.0 / 23.0);
+  float z = u * u;
+  float p = a11;
+  p = p * z + a10;
+  p = p * z + a9;
+  p = p * z + a8;
+  p = p * z + a7;
+  p = p * z + a6;
+  p = p * z + a5;
+  p = p * z + a4;
+  p = p * z + a3;
+  p = p * z + a2;
+  p = p * z + a1;
+  return u + (u * z) * p;
+}
+
+/* atan(1/2) as float high + low parts. */
+#define OAVBD_ATANHALF_HI 0.46364760398864746f
+#define OAVBD_ATANHALF_LO 5.01215868808913e-09f
+
+/* atan(t) for t in [0, 1]. */
+OAVBD_INLINE OAVBD_HD float oavbd_atan01(float t) {
+  if (t < 0.4375f) return oavbd_atan_series(t);
+  if (t < 0.6875f) {
+    float u = (2.0f * t - 1.0f) / (2.0f + t);
+    return OAVBD_ATANHALF_HI + (oavbd_atan_series(u) + OAVBD_ATANHALF_LO);
+  }
+  float u = (t - 1.0f) / (t + 1.0f);
+  return OAVBD_PIO4_HI + (oavbd_atan_series(u) + OAVBD_PIO4_LO);
+}
+
+/* atan2(y, x) with the IEEE signed-zero and infinity conventions of C's
+ * atan2f; NaN in, NaN out. */
+OAVBD_INLINE OAVBD_HD float oavbd_atan2f(float y, float x) {
+  if (x != x || y != y) return x + y;
+  float ax = fabsf(x), ay = fabsf(y);
+  float a;
+  if (ax == 0.0f && ay == 0.0f) {
+    a = 0.0f;
+  } else if (isinf(ax) && isinf(ay)) {
+    a = OAVBD_PIO4_HI;
+  } else if (ay <= ax) {
+    a = oavbd_atan01(ay / ax);
+  } else {
+    a = (OAVBD_PIO2_HI - oavbd_atan01(ax / ay)) + OAVBD_PIO2_LO;
+  }
+  /* x < 0 reflects about pi/2; of the zeros only atan2(+-0, -0) does */
+  if (x < 0.0f || (a == 0.0f && signbit(x))) a = (OAVBD_PI_HI - a) + OAVBD_PI_LO;
+  return copysignf(a, y);
 }
 
 #endif /* OAVBD_FMATH_H */

@@ -91,6 +91,10 @@ struct K {
   static T PEN_INIT() { return CF(1.0); }
   static T JOINT_PEN_INIT() { return CF(1.0); }
   static T LIM_PEN_MAX() { return CF(1e4); }
+  static T PI() { return CF(3.14159265358979323846); }
+  static T TAU() { return CF(6.28318530717958647692); }
+  static T LIMIT_SLOP_ANG() { return CF(0.005); }
+  static T LIMIT_SLOP_LIN() { return CF(0.002); }
   static T MIN_EFF_MASS() { return CF(1e-9); }
   static T GEN_MARGIN() { return CF(0.05); }
   static T EXPAND_SLOP() { return CF(0.005); }
@@ -195,6 +199,12 @@ struct Joint {
   // Spring: the motor's stiffness, damping, force budget, target velocity
   T m_stiff = T(0), m_damp = T(0), m_max = T(0), m_tvel = T(0);
   int m_kind = MK_POSITION, m_model = MM_ACCELERATION;
+  // Revolute / Prismatic: travel window about the assembly angle / length
+  // (ref_val), a velocity motor [target speed, budget], and the limit's
+  // one-sided force accumulators (acc_lim: revolute, lim_dual: prismatic)
+  bool has_lim = false, has_mot = false;
+  T lim_lo = T(0), lim_hi = T(0), mot_speed = T(0), mot_max = T(0);
+  T acc_lim = T(0), lim_dual = T(0);
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -1023,6 +1033,10 @@ struct Engine {
   int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
     if (kind < JK_BALL || kind > JK_SPRING) return -1;
     if (kind == JK_ROPE && !(prm && std::isfinite(prm[0]) && prm[0] > 0.0f)) return -1;
+    if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && prm) {  // validate_joint (migration.rs:160-190)
+      if (prm[4] != 0.0f && !(std::isfinite(prm[0]) && std::isfinite(prm[1]) && prm[0] <= prm[1])) return -1;
+      if (prm[5] != 0.0f && !(std::isfinite(prm[2]) && std::isfinite(prm[3]) && prm[3] >= 0.0f)) return -1;
+    }
     if (kind == JK_SPRING) {
       if (!prm) return -1;
       for (int k = 0; k < 5; ++k)
@@ -1044,6 +1058,23 @@ struct Engine {
       if (dot(axa, axa) < K<T>::DEGENERATE() || dot(axb, axb) < K<T>::DEGENERATE()) return -1;
       j.ax_a = normalize_or(axa, Z);
       j.ax_b = normalize_or(axb, Z);
+      // resolve_joint's assembly twist / slide (joint.rs:1280-1312)
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      if (kind == JK_REVOLUTE) {
+        j.ref_val = hinge_twist(A.rot, B.rot, j.ax_a);
+      } else {
+        const V3<T> wa = normalize_or(rotate(A.rot, j.ax_a), Z);
+        j.ref_val = dot((B.pos + rotate(B.rot, lb)) - (A.pos + rotate(A.rot, la)), wa);
+      }
+      if (prm) {  // [lo, hi, target speed, budget, has limit, has motor]
+        j.has_lim = prm[4] != 0.0f;
+        j.has_mot = prm[5] != 0.0f;
+        j.lim_lo = T(prm[0]);
+        j.lim_hi = T(prm[1]);
+        j.mot_speed = T(prm[2]);
+        j.mot_max = T(prm[3]);
+      }
     }
     if (kind == JK_FIXED) {
       const Body<T>& A = bodies[a];
@@ -1122,6 +1153,84 @@ struct Engine {
   bool ball_jointed(int ia, int ib) const { return jointed.count(jkey(ia, ib)) != 0; }
 
   // ---- joints helpers (sleep.rs) ----
+  // quat_twist / hinge_twist (crates/physics/src/sequential_impulse/
+  // joints.rs:38-48): the twist of a relative rotation about an axis,
+  // wrapped to [-pi, pi]; wrap_pi (rows.rs:149-151).
+  T wrap_pi(T x) const { return rem_euclid(x + K<T>::PI(), K<T>::TAU()) - K<T>::PI(); }
+  T quat_twist(Q<T> q, V3<T> axis) const {
+    return wrap_pi(T(2) * atan2_t(dot(V3<T>{q.x, q.y, q.z}, axis), q.w, math));
+  }
+  T hinge_twist(Q<T> qa, Q<T> qb, V3<T> axis) const { return quat_twist(quat_mul(conj(qa), qb), axis); }
+  // limit_state / warm_limit_state / regularized_limit (rows.rs:156-191):
+  // 1 = lower bound, 0 = upper bound, -1 = free inside the window.
+  static int limit_state(T v, T lo, T hi, T slop) {
+    if (v <= lo + slop) return 1;
+    if (v >= hi - slop) return 0;
+    return -1;
+  }
+  static int warm_limit_state(T v, T lo, T hi, T slop, T force) {
+    const int st = limit_state(v, lo, hi, slop);
+    if (st >= 0) return st;
+    if (force < T(0)) return 1;
+    if (force > T(0)) return 0;
+    return -1;
+  }
+  static T regularized_limit(T value, T initial, T bound, bool lower) {
+    const T old = initial - bound;
+    const T violation = lower ? fmin_(old, T(0)) : fmax_(old, T(0));
+    return value - bound - K<T>::ALPHA() * violation;
+  }
+  // The joint's live angle (Revolute) or slide (Prismatic) from the
+  // assembly value, for its limit.
+  T limit_value(const Joint<T>& j, Q<T> ra, Q<T> rb, V3<T> pa_w, V3<T> pb_w, V3<T> wa) const {
+    if (j.kind == JK_REVOLUTE) return wrap_pi(hinge_twist(ra, rb, j.ax_a) - j.ref_val);
+    return dot(pb_w - pa_w, wa) - j.ref_val;
+  }
+  // joint_limit_violated (joints_ext.rs:440-462), the motor pause rule.
+  bool joint_limit_violated(const Joint<T>& j) const {
+    if (!j.has_lim || (j.kind != JK_REVOLUTE && j.kind != JK_PRISMATIC)) return false;
+    const Body<T>& a = bodies[j.a];
+    const Body<T>& b = bodies[j.b];
+    const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
+    const V3<T> pa = a.pos + rotate(a.rot, j.la), pb = b.pos + rotate(b.rot, j.lb);
+    const T v = limit_value(j, a.rot, b.rot, pa, pb, wa);
+    return limit_state(v, j.lim_lo, j.lim_hi, j.kind == JK_REVOLUTE ? K<T>::LIMIT_SLOP_ANG() : K<T>::LIMIT_SLOP_LIN()) >= 0;
+  }
+  // motor_impulse (joints_ext.rs:237-356) for Revolute / Prismatic velocity
+  // motors: the budget-clamped deadbeat impulse driving the axis rate to
+  // the target (JointMotor::velocity_impulse, joint.rs:1172-1177), paused
+  // while the joint's limit is violated; joint order.
+  void motor_impulse() {
+    const V3<T> Z{T(0), T(0), T(1)};
+    for (const auto& j : joints) {
+      if (!j.has_mot || (j.kind != JK_REVOLUTE && j.kind != JK_PRISMATIC)) continue;
+      if (j.mot_max <= T(0) || joint_limit_violated(j)) continue;
+      const int ia = j.a, ib = j.b;
+      const V3<T> wa = normalize_or(rotate(bodies[ia].rot, j.ax_a), Z);
+      auto impulse = [&](T err, T k) -> T {
+        if (k <= T(0)) return T(0);
+        return clampf(err / k, -j.mot_max * K<T>::DT(), j.mot_max * K<T>::DT());
+      };
+      if (j.kind == JK_REVOLUTE) {
+        const T w = dot(bodies[ib].angvel - bodies[ia].angvel, wa);
+        const V3<T> wia = ang_inv_wa(ia, wa), wib = ang_inv_wa(ib, wa);
+        const T k = dot(wia, wa) + dot(wib, wa);
+        if (k < K<T>::MIN_EFF_MASS()) continue;
+        const T dj = impulse(j.mot_speed - w, k);
+        if (solvable(ia)) bodies[ia].angvel = bodies[ia].angvel + (-dj) * wia;
+        if (solvable(ib)) bodies[ib].angvel = bodies[ib].angvel + dj * wib;
+      } else {
+        const T v = dot(bodies[ib].vel - bodies[ia].vel, wa);
+        const T ka = eff_inv_mass(ia), kb = eff_inv_mass(ib);
+        const T k = ka + kb;
+        if (k < K<T>::MIN_EFF_MASS()) continue;
+        const T dj = impulse(j.mot_speed - v, k);
+        if (solvable(ia)) bodies[ia].vel = bodies[ia].vel + ((-dj) * ka) * wa;
+        if (solvable(ib)) bodies[ib].vel = bodies[ib].vel + (dj * kb) * wa;
+      }
+    }
+  }
+
   T joint_rest_error(const Joint<T>& j) const {
     const Body<T>& a = bodies[j.a];
     const Body<T>& b = bodies[j.b];
@@ -1135,7 +1244,12 @@ struct Engine {
                      length(quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math)));
       case JK_PRISMATIC: {
         V3<T> perp = delta - axis * dot(delta, axis);
-        return fmax_(length(perp), length(axis - rotate(b.rot, j.ax_b)));
+        T error = length(perp);
+        if (j.has_lim) {
+          const T value = dot(delta, axis) - j.ref_val;
+          error = fmax_(error, fmax_(j.lim_lo - value, value - j.lim_hi));
+        }
+        return fmax_(error, length(axis - rotate(b.rot, j.ax_b)));
       }
       case JK_DISTANCE:
         return std::fabs(length(delta) - j.ref_val);
@@ -1155,6 +1269,7 @@ struct Engine {
   }
   bool joint_driven(const Joint<T>& j) const {
     if (spring_keeps_awake(j)) return true;
+    if (j.has_mot && j.mot_speed != T(0) && j.mot_max > T(0)) return true;
     int m[2] = {j.a, j.b};
     for (int h : m) {
       const Body<T>& b = bodies[h];
@@ -2228,6 +2343,31 @@ struct Engine {
           acc.couple(is_a ? -dir : dir, j.pen_a[k], f);
         }
       }
+      // the travel limit (assembly.rs:809-874): a one-sided row warmed by
+      // the dual's accumulator, never written here
+      if (j.has_lim && (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC)) {
+        const bool rev = j.kind == JK_REVOLUTE;
+        const T v = limit_value(j, a.rot, b.rot, pa, pb, wa);
+        const T acc_f = rev ? j.acc_lim : j.lim_dual;
+        const int st = warm_limit_state(v, j.lim_lo, j.lim_hi, rev ? K<T>::LIMIT_SLOP_ANG() : K<T>::LIMIT_SLOP_LIN(), acc_f);
+        if (st >= 0) {
+          const bool lower = st == 1;
+          const V3<T> wa0 = normalize_or(rotate(rot0[j.a], j.ax_a), V3<T>{T(0), T(0), T(1)});
+          const T initial = limit_value(j, rot0[j.a], rot0[j.b], pa0, pb0, wa0);
+          const T c = regularized_limit(v, initial, lower ? j.lim_lo : j.lim_hi, lower);
+          const T pen = rev ? j.pen_a[2] : j.pen_l[2];
+          const T f_raw = pen * c + acc_f;
+          const T f = lower ? fmin_(f_raw, T(0)) : fmax_(f_raw, T(0));
+          if (row_live(c, f)) {
+            if (rev) {
+              acc.couple(is_a ? -wa : wa, pen, f);
+            } else {
+              const V3<T> r_side = is_a ? rotate(a.rot, j.la) : rotate(b.rot, j.lb);
+              acc.stamp(wa, pen, f, r_side, is_a ? T(-1) : T(1));  // C = (B - A).wa
+            }
+          }
+        }
+      }
     }
     acc.finish();
 
@@ -2386,6 +2526,36 @@ struct Engine {
           if (std::fabs(c) < K<T>::C_EPS()) continue;
           j.lam_a[k] += j.pen_a[k] * c;
           j.pen_a[k] = fmin_(j.pen_a[k] + K<T>::BETA_ANG() * std::fabs(c), K<T>::PEN_MAX());
+        }
+      }
+      // the limit's one-sided accumulator (joints_ext.rs:820-890), zeroed
+      // when the joint is clear of both bounds
+      if (j.has_lim && (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC)) {
+        const bool rev = j.kind == JK_REVOLUTE;
+        const V3<T> Zv{T(0), T(0), T(1)};
+        const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), Zv);
+        const T v = limit_value(j, a.rot, b.rot, pa, pb, wa);
+        const int st = warm_limit_state(v, j.lim_lo, j.lim_hi, rev ? K<T>::LIMIT_SLOP_ANG() : K<T>::LIMIT_SLOP_LIN(),
+                                        rev ? j.acc_lim : j.lim_dual);
+        if (st >= 0) {
+          const bool lower = st == 1;
+          const V3<T> wa0 = normalize_or(rotate(rot0[j.a], j.ax_a), Zv);
+          const T initial = limit_value(j, rot0[j.a], rot0[j.b], pa0, pb0, wa0);
+          const T c = regularized_limit(v, initial, lower ? j.lim_lo : j.lim_hi, lower);
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            if (rev) {
+              const T f = j.pen_a[2] * c + j.acc_lim;
+              j.acc_lim = lower ? fmin_(f, T(0)) : fmax_(f, T(0));
+              j.pen_a[2] = fmin_(j.pen_a[2] + K<T>::BETA_ANG() * std::fabs(c), K<T>::LIM_PEN_MAX());
+            } else {
+              const T f_raw = j.pen_l[2] * c + j.lim_dual;
+              j.lim_dual = lower ? fmin_(f_raw, T(0)) : fmax_(f_raw, T(0));
+              j.pen_l[2] = fmin_(j.pen_l[2] + K<T>::BETA() * std::fabs(c), K<T>::LIM_PEN_MAX());
+            }
+          }
+        } else {
+          j.acc_lim = T(0);
+          if (!rev) j.lim_dual = T(0);
         }
       }
     }
@@ -2655,6 +2825,7 @@ struct Engine {
       b.torque = V3<T>{T(0), T(0), T(0)};
       b.angvel = b.angvel + mat3_vec(iwi, torque) * DT;
     }
+    motor_impulse();
     rope_impulse();
     pre_vel.resize(n);
     for (int h = 0; h < n; ++h) pre_vel[h] = bodies[h].vel;  // mod.rs:874-876
@@ -3383,6 +3554,37 @@ void orc_set_pose(void* h, int i, const float* pos, const float* quat) {
   });
 }
 
+// A Revolute / Prismatic joint's limit accumulators: out2 = acc_lim, lim_dual.
+void orc_get_joint_limit_state(void* h, int j, float* out2) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    out2[0] = (float)E->joints[j].acc_lim;
+    out2[1] = (float)E->joints[j].lim_dual;
+  });
+}
+void orc_set_joint_limit_state(void* h, int j, float acc_lim, float lim_dual) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    E->joints[j].acc_lim = acc_lim;
+    E->joints[j].lim_dual = lim_dual;
+  });
+}
+// wake_body plus a zero sleep timer (a test's forced wake each step).
+void orc_wake_body(void* h, int i) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    E->wake_body(i);
+    E->sleep_timer[i] = 0;
+  });
+}
+void orc_set_gravity(void* h, const float* g) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    using TT = std::remove_reference_t<decltype(E->bodies[0].pos.x)>;
+    E->gravity = V3<TT>{TT(g[0]), TT(g[1]), TT(g[2])};
+  });
+}
+
 // Overwrite body i's velocities.
 void orc_set_vel(void* h, int i, const float* vel, const float* angvel) {
   OracleCtx* c = (OracleCtx*)h;
@@ -3615,5 +3817,6 @@ void orc_set_params_f64(void* h, int i, const double* vel3, double mass, double 
 
 void orc_fmath_sincos(float x, float* s, float* c) { oavbd_sincosf(x, s, c); }
 float orc_fmath_asin(float x) { return oavbd_asinf(x); }
+float orc_fmath_atan2(float y, float x) { return oavbd_atan2f(y, x); }
 
 }  // extern "C"

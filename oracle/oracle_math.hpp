@@ -138,5 +138,12 @@ inline void sincos_t(double x, double* s, double* c, int) {
 }
 inline float asin_t(float x, int mode) { return mode == MATH_PORTABLE ? oavbd_asinf(x) : ::asinf(x); }
 inline double asin_t(double x, int) { return std::asin(x); }
+inline float atan2_t(float y, float x, int mode) { return mode == MATH_PORTABLE ? oavbd_atan2f(y, x) : ::atan2f(y, x); }
+inline double atan2_t(double y, double x, int) { return std::atan2(y, x); }
+// Rust f32::rem_euclid: the remainder of fmod, moved into [0, |m|).
+template <class T> inline T rem_euclid(T x, T m) {
+  T r = std::fmod(x, m);
+  return r < T(0) ? r + std::fabs(m) : r;
+}
 
 }  // namespace orc

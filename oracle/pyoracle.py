@@ -50,6 +50,11 @@ def lib():
         L.orc_add_joint.restype = ctypes.c_int
         L.orc_add_joint.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p,
                                     _f32p]
+        L.orc_get_joint_limit_state.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p]
+        L.orc_set_joint_limit_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.orc_set_gravity.argtypes = [ctypes.c_void_p, _f32p]
+        L.orc_wake_body.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_vel.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
         L.orc_add_joint_prm.restype = ctypes.c_int
         L.orc_add_joint_prm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p,
                                         _f32p, _f32p, _f32p]
@@ -109,6 +114,8 @@ def lib():
         L.orc_fmath_sincos.argtypes = [ctypes.c_float, _f32p, _f32p]
         L.orc_fmath_asin.restype = ctypes.c_float
         L.orc_fmath_asin.argtypes = [ctypes.c_float]
+        L.orc_fmath_atan2.restype = ctypes.c_float
+        L.orc_fmath_atan2.argtypes = [ctypes.c_float, ctypes.c_float]
         _lib = L
     return _lib
 
@@ -190,6 +197,24 @@ class OracleEngine:
 
     def set_joint_duals(self, j, d12):
         self._L.orc_set_joint_duals(self.h, int(j), _p(f32(d12), _f32p))
+
+    def joint_limit_state(self, j):
+        """(acc_lim, lim_dual) of a Revolute / Prismatic joint."""
+        out = np.zeros(2, np.float32)
+        self._L.orc_get_joint_limit_state(self.h, int(j), _p(out, _f32p))
+        return out
+
+    def set_joint_limit_state(self, j, acc_lim, lim_dual):
+        self._L.orc_set_joint_limit_state(self.h, int(j), ctypes.c_float(acc_lim), ctypes.c_float(lim_dual))
+
+    def wake_body(self, i):
+        self._L.orc_wake_body(self.h, int(i))
+
+    def set_gravity(self, g):
+        self._L.orc_set_gravity(self.h, _p(f32(g), _f32p))
+
+    def set_vel(self, i, vel, angvel=(0.0, 0.0, 0.0)):
+        self._L.orc_set_vel(self.h, int(i), _p(f32(vel), _f32p), _p(f32(angvel), _f32p))
 
     def solve_body(self, i):
         self._L.orc_solve_body(self.h, int(i))
@@ -317,9 +342,10 @@ class OracleEngine:
     def add_joint(self, a, b, la, lb, kind=0, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0), prm=None):
         """add_joint: kind 0 Ball, 1 Revolute, 2 Prismatic, 3 Fixed, 4 Distance,
         5 Rope, 6 Spring (local anchors la, lb; local hinge / slide axes for
-        kinds 1 and 2; prm for Rope (max distance) and Spring (target
-        position, stiffness, damping, max force, target velocity, motor
-        kind, motor model))."""
+        kinds 1 and 2; prm for Revolute / Prismatic (limit lo, hi, motor
+        speed, budget, has limit, has motor), Rope (max distance) and
+        Spring (target position, stiffness, damping, max force, target
+        velocity, motor kind, motor model))."""
         if prm is None:
             r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
                                       _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))

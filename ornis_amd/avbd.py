@@ -100,6 +100,9 @@ SIGNATURES = {
     "oavbd_get_sleep": (ctypes.c_int, [ctypes.c_void_p, _u8p, _f32p, ctypes.c_uint32]),
     "oavbd_get_toi": (ctypes.c_int, [ctypes.c_void_p, _f32p, _i32p, ctypes.c_uint32]),
     "oavbd_get_joints": (ctypes.c_int, [ctypes.c_void_p, _f32p, ctypes.c_uint32]),
+    "oavbd_add_joint_limited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                               _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                               ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                             ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_spring_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
@@ -334,6 +337,14 @@ class AvbdEngine:
             return idx.value
         xa = np.ascontiguousarray(axis_a, dtype=np.float32)
         xb = np.ascontiguousarray(axis_b, dtype=np.float32)
+        if prm is not None and int(kind) in (1, 2):  # (lo, hi, speed, budget, has limit, has motor)
+            lim = np.ascontiguousarray(prm[0:2], dtype=np.float32)
+            mot = np.ascontiguousarray(prm[2:4], dtype=np.float32)
+            self._check(self._L.oavbd_add_joint_limited(
+                self._ctx, int(kind), int(a), int(b), _p(la, _f32p), _p(lb, _f32p), _p(xa, _f32p), _p(xb, _f32p),
+                _p(lim, _f32p) if prm[4] else None, _p(mot, _f32p) if prm[5] else None, ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
         self._check(self._L.oavbd_add_joint(self._ctx, int(kind), int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
                                             _p(xa, _f32p), _p(xb, _f32p), ctypes.byref(idx)))
         self.njoints += 1

@@ -170,12 +170,78 @@ __device__ __forceinline__ f3 ang_inv_wa_dev(const Dev& d, uint32_t h, f3 w) {
   return mat3_vec(iwi, w);
 }
 
+// A Revolute / Prismatic joint's live angle / slide from its assembly value
+// (rows.rs:149-151; assembly.rs:811-849).
+__device__ __forceinline__ float limit_value(const Dev& d, uint32_t j, int kind, qf qa, qf qb, f3 pa, f3 pb, f3 wa) {
+  if (kind == JK_REVOLUTE) return wrap_pi(hinge_twist(qa, qb, xyz(d.J.ax[2 * j])) - d.J.dref[j].x);
+  return dot3(sub(pb, pa), wa) - d.J.dref[j].x;
+}
+// limit_state / warm_limit_state (rows.rs:156-182): 1 lower, 0 upper, -1 free.
+__device__ __forceinline__ int limit_state(float v, float lo, float hi, float slop) {
+  if (v <= lo + slop) return 1;
+  if (v >= hi - slop) return 0;
+  return -1;
+}
+__device__ __forceinline__ int warm_limit_state(float v, float lo, float hi, float slop, float force) {
+  const int st = limit_state(v, lo, hi, slop);
+  if (st >= 0) return st;
+  return force < 0.0f ? 1 : (force > 0.0f ? 0 : -1);
+}
+// regularized_limit (rows.rs:186-191).
+__device__ __forceinline__ float regularized_limit(float value, float initial, float bound, bool lower) {
+  const float old = initial - bound;
+  const float violation = lower ? fmin_(old, 0.0f) : fmax_(old, 0.0f);
+  return value - bound - K_ALPHA * violation;
+}
+
+// motor_impulse (joints_ext.rs:237-356) of one Revolute / Prismatic joint
+// with a velocity motor: the budget-clamped deadbeat impulse driving the
+// axis rate to its target (JointMotor::velocity_impulse, joint.rs:
+// 1172-1177), paused while the limit is violated (joints_ext.rs:440-462).
+__device__ void motor_impulse_one(const Dev& d, uint32_t j) {
+  const int kind = d.J.kind[j];
+  const uint2 ab = d.J.ab[j];
+  const float4 pr = d.J.dref[j], mo = d.J.qref[j];
+  if (!(mo.y > 0.0f)) return;
+  const float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+  const qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
+  const f3 wa = normalize_or(qrot(qa, xyz(d.J.ax[2 * j])), f3{0.0f, 0.0f, 1.0f});
+  if ((uint32_t)pr.w & HINGE_LIMIT) {
+    const f3 pa = add(xyz(pa4), qrot(qa, xyz(d.J.la[j]))), pb = add(xyz(pb4), qrot(qb, xyz(d.J.lb[j])));
+    const float v = limit_value(d, j, kind, qa, qb, pa, pb, wa);
+    if (limit_state(v, pr.y, pr.z, kind == JK_REVOLUTE ? K_LIMIT_SLOP_ANG : K_LIMIT_SLOP_LIN) >= 0) return;
+  }
+  const bool sa = d.b.type[ab.x] == BT_DYNAMIC && pa4.w > 0.0f, sbv = d.b.type[ab.y] == BT_DYNAMIC && pb4.w > 0.0f;
+  const float cap = mo.y * K_DT;
+  if (kind == JK_REVOLUTE) {
+    const float4 wa4 = d.b.angvel[ab.x], wb4 = d.b.angvel[ab.y];
+    const float w = dot3(sub(xyz(wb4), xyz(wa4)), wa);
+    const f3 ia = ang_inv_wa_dev(d, ab.x, wa), ib = ang_inv_wa_dev(d, ab.y, wa);
+    const float k = dot3(ia, wa) + dot3(ib, wa);
+    if (k < K_MIN_EFF_MASS) return;
+    const float dj = clampf_((mo.x - w) / k, -cap, cap);
+    if (sa) d.b.angvel[ab.x] = pk(add(xyz(wa4), mul(ia, -dj)), wa4.w);
+    if (sbv) d.b.angvel[ab.y] = pk(add(xyz(wb4), mul(ib, dj)), wb4.w);
+  } else {
+    const float4 va4 = d.b.vel[ab.x], vb4 = d.b.vel[ab.y];
+    const float v = dot3(sub(xyz(vb4), xyz(va4)), wa);
+    const float ka = d.b.type[ab.x] == BT_DYNAMIC ? pa4.w : 0.0f;
+    const float kb = d.b.type[ab.y] == BT_DYNAMIC ? pb4.w : 0.0f;
+    const float k = ka + kb;
+    if (k < K_MIN_EFF_MASS) return;
+    const float dj = clampf_((mo.x - v) / k, -cap, cap);
+    if (sa) d.b.vel[ab.x] = pk(add(xyz(va4), mul(wa, -dj * ka)), va4.w);
+    if (sbv) d.b.vel[ab.y] = pk(add(xyz(vb4), mul(wa, dj * kb)), vb4.w);
+  }
+}
+
 // rope_impulse (joints_ext.rs:173-223): once per step, after the torques,
 // every stretched rope whose ends separate has the separating anchor
 // velocity removed through the pair's effective mass.  Ropes sharing a body
 // see each other's updates, so one lane walks them in joint order.
-__global__ void k_rope_impulse(Dev d) {
+__global__ void k_joint_impulses(Dev d) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (uint32_t m = 0; m < d.nmotors; ++m) motor_impulse_one(d, d.motors[m]);  // all motors, then all ropes
   for (uint32_t r = 0; r < d.nropes; ++r) {
     const uint32_t j = d.ropes[r];
     const uint2 ab = d.J.ab[j];
@@ -1316,6 +1382,10 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   }
   const int kind = d.J.kind[j];
   if (kind == JK_SPRING && ((uint32_t)d.J.dref[j].w & SPRING_KEEPS_AWAKE)) return true;
+  if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && ((uint32_t)d.J.dref[j].w & HINGE_MOTOR)) {
+    const float4 mo = d.J.qref[j];  // a turning motor with a budget keeps its bodies awake
+    if (mo.x != 0.0f && mo.y > 0.0f) return true;
+  }
   const qf qa = asq(d.b.rot[ab.x]), qb = asq(d.b.rot[ab.y]);
   f3 pa = add(xyz(d.b.pos[ab.x]), qrot(qa, xyz(d.J.la[j])));
   f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, xyz(d.J.lb[j])));
@@ -1331,8 +1401,17 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   } else {
     const f3 axis = qrot(qa, xyz(d.J.ax[2 * j]));
     const float e_axis = len3(sub(axis, qrot(qb, xyz(d.J.ax[2 * j + 1]))));
-    err = kind == JK_REVOLUTE ? fmax_(len3(delta), e_axis)
-                              : fmax_(len3(sub(delta, mul(axis, dot3(delta, axis)))), e_axis);
+    if (kind == JK_REVOLUTE) {
+      err = fmax_(len3(delta), e_axis);
+    } else {
+      err = len3(sub(delta, mul(axis, dot3(delta, axis))));
+      const float4 pr = d.J.dref[j];
+      if ((uint32_t)pr.w & HINGE_LIMIT) {  // outside the travel window
+        const float value = dot3(delta, axis) - pr.x;
+        err = fmax_(err, fmax_(pr.y - value, value - pr.z));
+      }
+      err = fmax_(err, e_axis);
+    }
   }
   return err > K_JOINT_REST;
 }
@@ -1465,7 +1544,7 @@ __global__ void __launch_bounds__(TPB) k_wake_begin(Dev d, uint32_t may_sleep) {
 // error (:790).
 __device__ __forceinline__ float4 decay_lam(float4 l) {
   const float AG = K_ALPHA * K_GAMMA;
-  return make_float4(l.x * AG, l.y * AG, l.z * AG, 0.0f);
+  return make_float4(l.x * AG, l.y * AG, l.z * AG, l.w);  // w (a limit accumulator) does not decay
 }
 __device__ __forceinline__ float4 decay_pen(float4 pe) {
   return make_float4(clampf_(pe.x * K_GAMMA, K_PEN_MIN, K_PEN_MAX), clampf_(pe.y * K_GAMMA, K_PEN_MIN, K_PEN_MAX),
@@ -1486,6 +1565,17 @@ __global__ void k_joint_prep(Dev d) {
   f3 c0v = sub(pa0, pb0);
   if (kind == JK_FIXED) c0v = add(c0v, qrot(ra0, xyz(d.J.dref[j])));
   d.J.c0v[j] = pk(c0v, 0.0f);
+  if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && ((uint32_t)d.J.dref[j].w & HINGE_LIMIT)) {
+    const float ref = d.J.dref[j].x;
+    float v0;
+    if (kind == JK_REVOLUTE) {
+      v0 = wrap_pi(hinge_twist(ra0, rb0, xyz(d.J.ax[2 * j])) - ref);
+    } else {
+      const f3 wa0 = normalize_or(qrot(ra0, xyz(d.J.ax[2 * j])), f3{0.0f, 0.0f, 1.0f});
+      v0 = dot3(sub(pb0, pa0), wa0) - ref;
+    }
+    d.J.c0v[j] = pk(c0v, v0);
+  }
   if (kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
     const f3 axa0 = qrot(ra0, xyz(d.J.ax[2 * j])), axb0 = qrot(rb0, xyz(d.J.ax[2 * j + 1]));
     f3 t1, t2;
@@ -2323,6 +2413,28 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
     if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
     angular(is_a ? cross3(axa, t) : neg(cross3(axb, t)), pav[li], f);
   }
+  // the travel limit (assembly.rs:809-874): one-sided, warmed by the dual's
+  // accumulator (lama.w revolute, lam.w prismatic), never written here
+  const float4 pr = d.J.dref[j];
+  if (!((uint32_t)pr.w & HINGE_LIMIT)) return;
+  const bool rev = kind == JK_REVOLUTE;
+  const f3 wa = normalize_or(axa, f3{0.0f, 0.0f, 1.0f});
+  const float v = limit_value(d, j, kind, qa, qb, pa, pb, wa);
+  const float accf = rev ? lama.w : lam.w;
+  const int st = warm_limit_state(v, pr.y, pr.z, rev ? K_LIMIT_SLOP_ANG : K_LIMIT_SLOP_LIN, accf);
+  if (st < 0) return;
+  const bool lower = st == 1;
+  const float c = regularized_limit(v, d.J.c0v[j].w, lower ? pr.y : pr.z, lower);
+  const float lpen = rev ? pav[2] : penv[2];
+  const float f_raw = lpen * c + accf;
+  const float f = lower ? fmin_(f_raw, 0.0f) : fmax_(f_raw, 0.0f);
+  if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) return;
+  if (rev) {
+    angular(is_a ? neg(wa) : wa, lpen, f);
+  } else {
+    make_stamp(row, wa, lpen, f, is_a ? qrot(qa, la) : r_b, is_a ? -1.0f : 1.0f);  // C = (B - A).wa
+    emit(0, row);
+  }
 }
 
 // solve_6x6 (assembly.rs:25-69) on the 21-entry lower triangle.
@@ -3078,9 +3190,11 @@ __global__ void k_dual_joint(Dev d) {
       }
     }
   }
-  d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], 0.0f);
-  d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
-  if (kind == JK_BALL) return;
+  if (kind == JK_BALL) {
+    d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], lam.w);
+    d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+    return;
+  }
   float4 la4 = d.J.lama[j], pa4 = d.J.pena[j];
   float av[3] = {la4.x, la4.y, la4.z}, qv[3] = {pa4.x, pa4.y, pa4.z};
   if (kind == JK_FIXED) {
@@ -3102,7 +3216,38 @@ __global__ void k_dual_joint(Dev d) {
       qv[li] = fmin_(qv[li] + K_BETA_ANG * fabsf(c), K_PEN_MAX);
     }
   }
-  d.J.lama[j] = make_float4(av[0], av[1], av[2], 0.0f);
+  // the limit's one-sided accumulator (joints_ext.rs:820-890): acc_lim
+  // (lama.w) for a hinge, lim_dual (lam.w) for a slider, zeroed when the
+  // joint is clear of both bounds; penalties capped at K_LIM_PEN_MAX
+  float lim_dual = lam.w, acc_lim = la4.w;
+  const float4 pr = d.J.dref[j];
+  if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && ((uint32_t)pr.w & HINGE_LIMIT)) {
+    const bool rev = kind == JK_REVOLUTE;
+    const f3 wa = normalize_or(qrot(qa, xyz(d.J.ax[2 * j])), f3{0.0f, 0.0f, 1.0f});
+    const float v = limit_value(d, j, kind, qa, qb, pa, pb, wa);
+    const int st = warm_limit_state(v, pr.y, pr.z, rev ? K_LIMIT_SLOP_ANG : K_LIMIT_SLOP_LIN, rev ? acc_lim : lim_dual);
+    if (st >= 0) {
+      const bool lower = st == 1;
+      const float c = regularized_limit(v, d.J.c0v[j].w, lower ? pr.y : pr.z, lower);
+      if (fabsf(c) >= K_C_EPS) {
+        if (rev) {
+          const float f = qv[2] * c + acc_lim;
+          acc_lim = lower ? fmin_(f, 0.0f) : fmax_(f, 0.0f);
+          qv[2] = fmin_(qv[2] + K_BETA_ANG * fabsf(c), K_LIM_PEN_MAX);
+        } else {
+          const float f_raw = pv[2] * c + lim_dual;
+          lim_dual = lower ? fmin_(f_raw, 0.0f) : fmax_(f_raw, 0.0f);
+          pv[2] = fmin_(pv[2] + K_BETA * fabsf(c), K_LIM_PEN_MAX);
+        }
+      }
+    } else {
+      acc_lim = 0.0f;
+      if (!rev) lim_dual = 0.0f;
+    }
+  }
+  d.J.lam[j] = make_float4(lv[0], lv[1], lv[2], lim_dual);
+  d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
+  d.J.lama[j] = make_float4(av[0], av[1], av[2], acc_lim);
   d.J.pena[j] = make_float4(qv[0], qv[1], qv[2], 0.0f);
 }
 

@@ -74,6 +74,22 @@ HD float clampf_(float x, float lo, float hi) {
 HD float signumf(float x) { return isnan(x) ? x : copysignf(1.0f, x); }
 
 HD qf qid() { return qf{0.0f, 0.0f, 0.0f, 1.0f}; }
+
+// wrap_pi (crates/physics/src/avbd/rows.rs:149-151): (x + pi) rem_euclid
+// tau, minus pi.  The remainder is exact: below two periods it is one
+// exact subtraction (Sterbenz), which equals fmod there bit for bit.
+#define K_PI 3.14159265358979323846f
+#define K_TAU 6.28318530717958647692f
+HD float wrap_pi(float x) {
+  const float v = x + K_PI;
+  float r;
+  if (fabsf(v) < 2.0f * K_TAU)
+    r = fabsf(v) < K_TAU ? v : (v > 0.0f ? v - K_TAU : v + K_TAU);
+  else
+    r = fmodf(v, K_TAU);
+  if (r < 0.0f) r = r + K_TAU;
+  return r - K_PI;
+}
 HD qf qconj(qf q) { return qf{-q.x, -q.y, -q.z, q.w}; }
 // glam Quat * Vec3 (SSE2 mul_vec3a).
 HD f3 qrot(qf q, f3 v) {
@@ -95,6 +111,11 @@ HD qf qmul(qf a, qf b) {
   return qf{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
             a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
 }
+
+// quat_twist / hinge_twist (crates/physics/src/sequential_impulse/joints.rs:
+// 38-48): the twist of a relative rotation about an axis, in [-pi, pi].
+HD float quat_twist(qf q, f3 axis) { return wrap_pi(2.0f * oavbd_atan2f(dot3(f3{q.x, q.y, q.z}, axis), q.w)); }
+HD float hinge_twist(qf qa, qf qb, f3 axis) { return quat_twist(qmul(qconj(qa), qb), axis); }
 // quat_integrate, rows.rs:110-119.
 HD qf quat_integrate(qf q, f3 v) {
   float theta = len3(v);

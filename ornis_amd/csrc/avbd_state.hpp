@@ -144,6 +144,11 @@ enum { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE
 // Flags of a Spring joint (the w of its parameter record): the motor's
 // force-based model; the motor keeps its bodies awake (JointMotor::keeps_awake)
 enum { SPRING_FORCE_MODEL = 1, SPRING_KEEPS_AWAKE = 2 };
+// Flags of a Revolute / Prismatic joint (the w of its parameter record):
+// a travel limit, a velocity motor
+enum { HINGE_LIMIT = 1, HINGE_MOTOR = 2 };
+#define K_LIMIT_SLOP_ANG 0.005f
+#define K_LIMIT_SLOP_LIN 0.002f
 
 struct Joints {
   uint2* ab;
@@ -153,13 +158,16 @@ struct Joints {
   float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
   float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames
   float4* dref;      // Fixed: assembly anchor offset in A's frame; Distance / Rope / Spring:
-                     // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags)
-  float4* qref;      // Fixed: assembly relative rotation conj(qa) qb
-  float4* lam;       // lam_l xyz
+                     // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags);
+                     // Revolute / Prismatic: (assembly angle | length, limit lo, hi, HINGE_* flags)
+  float4* qref;      // Fixed: assembly relative rotation conj(qa) qb; Revolute / Prismatic:
+                     // the motor (target speed, budget, 0, 0)
+  float4* lam;       // lam_l xyz; w: a Prismatic limit's force accumulator
   float4* pen;       // pen_l xyz
-  float4* lama;      // lam_a xyz
+  float4* lama;      // lam_a xyz; w: a Revolute limit's force accumulator
   float4* pena;      // pen_a xyz
-  float4* c0v;       // per step: step-start anchor separation (Fixed: + rot0_a dref)
+  float4* c0v;       // per step: step-start anchor separation (Fixed: + rot0_a dref); w: the
+                     // limit's step-start angle / slide
   float4* a0;        // per step, 2 per joint: hinge rows t1 | t.(axa0 - axb0), t2 | ...
                      // (tangent basis of the step-start axis); Fixed: step-start rotation error
 };
@@ -209,6 +217,8 @@ struct Dev {
   uint32_t njoints;
   uint32_t* ropes;   // Rope joint indices, ascending (the velocity catch runs in joint order)
   uint32_t nropes;
+  uint32_t* motors;  // Revolute / Prismatic joints with a motor, ascending
+  uint32_t nmotors;
   Grid g;
   Cands C;
   float gravity[3];

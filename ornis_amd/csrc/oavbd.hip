@@ -72,8 +72,10 @@ struct HostJoint {
   float dref[3] = {0.0f, 0.0f, 0.0f};  // Fixed: assembly anchor offset in A's frame
   float qref[4] = {0.0f, 0.0f, 0.0f, 1.0f};  // Fixed: assembly relative rotation
   // Distance: assembly distance; Rope: maximum; Spring: rest length,
-  // stiffness, damping, SPRING_* flags
+  // stiffness, damping, SPRING_* flags; Revolute / Prismatic: assembly
+  // angle / length, limit lo, hi, HINGE_* flags
   float prm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  float mot[2] = {0.0f, 0.0f};  // Revolute / Prismatic motor: target speed, budget
 };
 
 }  // namespace
@@ -442,7 +444,7 @@ static void free_all(oavbd_ctx* c) {
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.kind); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.ax); dfree(d.J.dref);
   dfree(d.J.qref); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.lama); dfree(d.J.pena); dfree(d.J.c0v); dfree(d.J.a0);
-  dfree(d.ropes);
+  dfree(d.ropes); dfree(d.motors);
   dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
   dfree(d.g.is_large);
   Cands& C = d.C;
@@ -600,13 +602,13 @@ static void upload_joints(oavbd_ctx* c) {
   Joints& J = d.J;
   dfree(J.ab); dfree(J.kind); dfree(J.la); dfree(J.lb); dfree(J.la0); dfree(J.ax); dfree(J.dref); dfree(J.qref);
   dfree(J.lam); dfree(J.pen); dfree(J.lama); dfree(J.pena); dfree(J.c0v); dfree(J.a0);
-  dfree(d.b.jp); dfree(d.b.ji); dfree(d.ropes);
+  dfree(d.b.jp); dfree(d.b.ji); dfree(d.ropes); dfree(d.motors);
   dalloc(&J.ab, nj); dalloc(&J.kind, nj); dalloc(&J.la, nj); dalloc(&J.lb, nj); dalloc(&J.la0, nj);
   dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
   dalloc(&J.lama, nj); dalloc(&J.pena, nj); dalloc(&J.c0v, nj); dalloc(&J.a0, 2 * (size_t)nj);
   std::vector<uint2> ab(nj);
   std::vector<int> kind(nj);
-  std::vector<uint32_t> ropes;
+  std::vector<uint32_t> ropes, motors;
   std::vector<float4> la(nj), lb(nj), la0(nj), ax(2 * (size_t)nj), dref(nj), qref(nj), lam(nj), pen(nj), lama(nj),
       pena(nj);
   // no-collide partners: every kind but Fixed (rows.rs:437-450); the
@@ -625,10 +627,12 @@ static void upload_joints(oavbd_ctx* c) {
                          h.la[2] + (fixed ? h.dref[2] : 0.0f), 0.0f);
     ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], 0.0f);
     ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], 0.0f);
-    dref[j] = h.kind >= JK_DISTANCE ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
-                                    : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    const bool hinge = h.kind == JK_REVOLUTE || h.kind == JK_PRISMATIC;
+    dref[j] = (h.kind >= JK_DISTANCE || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
+                                               : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
     if (h.kind == JK_ROPE) ropes.push_back(j);
-    qref[j] = make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
+    if (hinge && ((uint32_t)h.prm[3] & HINGE_MOTOR)) motors.push_back(j);
+    qref[j] = hinge ? make_float4(h.mot[0], h.mot[1], 0.0f, 0.0f) : make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
     lam[j] = j < kept ? kept_lam[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     lama[j] = j < kept ? kept_lama[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -682,6 +686,9 @@ static void upload_joints(oavbd_ctx* c) {
   if (!ropes.empty()) dalloc(&d.ropes, ropes.size());
   up(d.ropes, ropes);
   d.nropes = (uint32_t)ropes.size();
+  if (!motors.empty()) dalloc(&d.motors, motors.size());
+  up(d.motors, motors);
+  d.nmotors = (uint32_t)motors.size();
   HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   d.njoints = nj;
   c->joints_dirty = false;
@@ -1367,8 +1374,8 @@ static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
   } else {
     if (may_sleep) wake_joint_motion(c);
     k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
-    if (d.nropes) {  // the rope catch runs after the torques, before the TOI gate
-      k_rope_impulse<<<1, 64, 0, c->st>>>(d);
+    if (d.nmotors || d.nropes) {  // motors, then the rope catch: after the torques, before the TOI gate
+      k_joint_impulses<<<1, 64, 0, c->st>>>(d);
       k_refast<<<blocks(n), TPB, 0, c->st>>>(d);
     }
   }
@@ -2352,7 +2359,8 @@ int oavbd_set_sleep(oavbd_ctx* c, const uint8_t* asleep, const float* timer, uin
 // The common body of the add_joint entry points: `prm` carries a Rope's
 // maximum or a Spring's motor (already validated), as HostJoint::prm.
 static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
-                          const float axis_a[3], const float axis_b[3], const float prm[4], uint32_t* out_index) {
+                          const float axis_a[3], const float axis_b[3], const float prm[4], uint32_t* out_index,
+                          const float mot[2] = nullptr) {
   const uint32_t n = c->d.n;
   // the reference wakes both members before it validates the spec
   // (mod.rs:1317-1318)
@@ -2382,7 +2390,11 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
   }
   if (prm)
     for (int k = 0; k < 4; ++k) j.prm[k] = prm[k];
-  if (kind == JK_FIXED || kind == JK_DISTANCE) {
+  if (mot) {
+    j.mot[0] = mot[0];
+    j.mot[1] = mot[1];
+  }
+  if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
     // resolve_joint (joint.rs:1313-1332): the assembly rotation and anchor
     // offset (Fixed) or anchor distance (Distance) from the current poses
     float4 p[2], q[2];
@@ -2399,8 +2411,13 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     if (kind == JK_FIXED) {
       j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
       j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
-    } else {
+    } else if (kind == JK_DISTANCE) {
       j.prm[0] = len3(sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
+    } else if (kind == JK_REVOLUTE) {  // the assembly twist about A's hinge axis
+      j.prm[0] = hinge_twist(qa, qb, f3{j.axa[0], j.axa[1], j.axa[2]});
+    } else {  // the assembly slide along A's axis
+      const f3 wa = normalize_or(qrot(qa, f3{j.axa[0], j.axa[1], j.axa[2]}), f3{0.0f, 0.0f, 1.0f});
+      j.prm[0] = dot3(sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)), wa);
     }
   }
   c->joints.push_back(j);
@@ -2421,6 +2438,22 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
 int oavbd_add_ball_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                          uint32_t* out_index) {
   return oavbd_add_joint(c, JK_BALL, a, b, la, lb, nullptr, nullptr, out_index);
+}
+
+int oavbd_add_joint_limited(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3],
+                            const float lb[3], const float axis_a[3], const float axis_b[3], const float limit[2],
+                            const float motor[2], uint32_t* out_index) {
+  if (!c || !la || !lb) return OAVBD_EINVAL;
+  if (kind != JK_REVOLUTE && kind != JK_PRISMATIC)
+    return fail(c, OAVBD_EINVAL, "limits and motors belong to Revolute and Prismatic joints");
+  // validate_joint (migration.rs:160-178), before any wake
+  if (limit && !(std::isfinite(limit[0]) && std::isfinite(limit[1]) && limit[0] <= limit[1]))
+    return fail(c, OAVBD_EINVAL, "joint limit needs finite lo <= hi");
+  if (motor && !(std::isfinite(motor[0]) && std::isfinite(motor[1]) && motor[1] >= 0.0f))
+    return fail(c, OAVBD_EINVAL, "joint motor needs a finite speed and budget >= 0");
+  const float flags = (float)((limit ? HINGE_LIMIT : 0) | (motor ? HINGE_MOTOR : 0));
+  const float prm[4] = {0.0f, limit ? limit[0] : 0.0f, limit ? limit[1] : 0.0f, flags};
+  return add_joint_impl(c, kind, a, b, la, lb, axis_a, axis_b, prm, out_index, motor);
 }
 
 int oavbd_add_rope_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],

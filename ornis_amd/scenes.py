@@ -78,11 +78,18 @@ class SceneBuilder:
         self.joints.append((int(a), int(b), tuple(la), tuple(lb)))
         return len(self.joints) - 1
 
-    def joint(self, kind, a, b, la, lb, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0)):
+    def joint(self, kind, a, b, la, lb, axis_a=(1.0, 0.0, 0.0), axis_b=(1.0, 0.0, 0.0), limit=None, motor=None):
         """A joint of kind JOINT_BALL / JOINT_REVOLUTE / JOINT_PRISMATIC /
-        JOINT_FIXED between bodies a and b: local anchors la, lb and (hinge /
-        slide kinds) local axes."""
-        self.joints.append((int(a), int(b), tuple(la), tuple(lb), int(kind), tuple(axis_a), tuple(axis_b)))
+        JOINT_FIXED / JOINT_DISTANCE between bodies a and b: local anchors
+        la, lb and (hinge / slide kinds) local axes, with an optional travel
+        `limit` (lo, hi) about the assembly angle / length and an optional
+        velocity `motor` (target speed, force or torque budget)."""
+        row = (int(a), int(b), tuple(la), tuple(lb), int(kind), tuple(axis_a), tuple(axis_b))
+        if limit is not None or motor is not None:
+            lo, hi = limit if limit is not None else (0.0, 0.0)
+            sp, mx = motor if motor is not None else (0.0, 0.0)
+            row += ((float(lo), float(hi), float(sp), float(mx), float(limit is not None), float(motor is not None)),)
+        self.joints.append(row)
         return len(self.joints) - 1
 
     def rope(self, a, b, la, lb, max_distance):

@@ -1,4 +1,5 @@
-"""Revolute, Prismatic, Fixed, Distance, Rope and Spring joints on the GPU
+"""Revolute, Prismatic (with travel limits and velocity motors), Fixed,
+Distance, Rope and Spring joints on the GPU
 against the oracle (float32, colour order), bit for bit: bodies, pairs,
 colourings and the joints' linear and angular duals.  The scenes are the
 reference's own joint tests (crates/physics/tests/avbd_engine.rs: hinge
@@ -7,7 +8,9 @@ crates/physics/tests/p6_rope_spring_motor.rs: rope :45-105, springs
 :124-216), chains whose links cycle through the first four kinds with
 tilted axes, falling into a field of spheres and onto each other, with a
 body removed mid-run, and a field of tethered bodies (ropes sharing bodies,
-rods, springs of both motor models and a damper) over a floor."""
+rods, springs of both motor models and a damper) over a floor; the
+reference's hinge / slider limit and motor tests (avbd_engine.rs:507-815)
+and chains of limited and motorised hinges and sliders."""
 import numpy as np
 import pytest
 
@@ -235,4 +238,88 @@ def test_rope_spring_spec_validation(gpu_available):
     zp = z.ctypes.data_as(_f32p)
     for kind in (scenes.JOINT_ROPE, scenes.JOINT_SPRING):  # only through their own entry points
         assert g._L.oavbd_add_joint(g._ctx, kind, 0, 1, zp, zp, None, None, None) != 0
+    assert g.stats()["joints"] == 0
+
+
+def test_reference_limit_and_motor_scenes_bitwise(gpu_available):
+    """avbd_engine.rs:507-549 (hinge window), :552-595 (hinge motor, trigger
+    anchor), :598-646 (slider resting on its lower limit), :782-815 (slider
+    motor), and a slider hanging on an upper limit under 10 g."""
+    R, P = scenes.JOINT_REVOLUTE, scenes.JOINT_PRISMATIC
+    sb = scenes.SceneBuilder()
+    sb.sphere((0.0, 0.0, 0.0), 0.1, 0.0)
+    bob = sb.sphere((1.0, -1.0, 0.0), 0.25, 1.0)
+    sb.joint(R, 0, bob, (0, 0, 0), (-1.0, 1.0, 0.0), (0, 0, 1), (0, 0, 1), limit=(-0.5, 0.5))
+    _run(sb.build(), 300, every=20)
+    sb = scenes.SceneBuilder()
+    a = sb.sphere((0.0, 0.0, 0.0), 0.05, 0.0)
+    bob = sb.sphere((0.0, 0.0, 0.0), 0.1, 1.0)
+    sb.joint(R, a, bob, (0, 0, 0), (0, 0, 0), (0, 0, 1), (0, 0, 1), motor=(3.0, 20.0))
+    sc = sb.build()
+    sc["trigger"][a] = 1
+    g, _ = _run(sc, 360, every=30)
+    assert abs(g.state()[bob, 12] - 3.0) < 1.5
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 0.0)
+    bob = sb.sphere((0.0, -2.9, 0.0), 0.25, 1.0)
+    sb.joint(P, 0, bob, (0, 0, 0), (0, 1, 0), (0, 1, 0), (0, 1, 0), limit=(-2.0, 0.0))
+    g, _ = _run(sb.build(), 300, every=20)
+    assert -5.4 < g.state()[bob, 1] < -4.4
+    sb = scenes.SceneBuilder()
+    sb.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 0.0)
+    bob = sb.sphere((1.0, 0.0, 0.0), 0.25, 1.0)
+    sb.joint(P, 0, bob, (0, 0, 0), (-1, 0, 0), (1, 0, 0), (1, 0, 0), motor=(2.0, 50.0))
+    _run(sb.build(), 300, every=20)
+    sb = scenes.SceneBuilder(gravity=(0.0, 98.1, 0.0))
+    sb.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.25), 0.0)
+    bob = sb.sphere((0.0, 2.9, 0.0), 0.25, 1.0)
+    sb.joint(P, 0, bob, (0, 0, 0), (0, -1, 0), (0, 1, 0), (0, 1, 0), limit=(-2.0, 2.0))
+    _run(sb.build(), 400, every=20)
+
+
+def limited_chains(n_chains=6, links=10, seed=23):
+    """Chains of boxes joined by hinges and sliders with random tilted axes,
+    travel windows and velocity motors (some with no budget, some turning
+    the wrong way into their limit), the first link of every chain pinned
+    to a static anchor, swinging into static spheres and a floor."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = scenes.SceneBuilder()
+    b.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=0.5, restitution=0.0)
+    for i in range(10):
+        b.sphere((rng.uniform(-4, 4), 0.4, rng.uniform(-4, 4)), 0.4, 0.0)
+    hy = 0.2
+    for c in range(n_chains):
+        x0, z0 = -3.0 + 1.2 * c, rng.uniform(-1.0, 1.0)
+        prev = b.box((x0, 5.0 + hy, z0), (0.1, 0.1, 0.1), 0.0)
+        for k in range(links):
+            h = b.box((x0 + 0.3 * k, 5.0 - 2 * hy * k, z0), (0.08, hy, 0.08), float(rng.uniform(0.5, 2.0)),
+                      friction=0.5, restitution=0.0)
+            kind = scenes.JOINT_REVOLUTE if (c + k) % 2 else scenes.JOINT_PRISMATIC
+            axis = rng.normal(size=3)
+            axis = tuple((axis / np.linalg.norm(axis)).tolist())
+            lo = float(rng.uniform(-0.6, -0.05))
+            limit = (lo, lo + float(rng.uniform(0.1, 0.8))) if k % 3 != 2 else None
+            motor = (float(rng.uniform(-3, 3)), float(rng.choice([0.0, 5.0, 40.0]))) if k % 2 == 0 else None
+            b.joint(kind, prev, h, (0.0, -hy, 0.0), (0.0, hy, 0.0), axis, axis, limit=limit, motor=motor)
+            prev = h
+    return b.build()
+
+
+def test_limited_motor_chains_bitwise(gpu_available):
+    s = limited_chains()
+    assert any(len(j) == 8 and j[7][4] for j in s["joints"]) and any(len(j) == 8 and j[7][5] for j in s["joints"])
+    g, o = _run(s, 240, every=10)
+    assert np.isfinite(g.state()).all() and g.stats()["breakdowns"] == 0
+
+
+def test_limit_motor_spec_validation(gpu_available):
+    b = scenes.SceneBuilder()
+    b.box((0.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    b.box((1.0, 0.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+    g = AvbdEngine(scene=b.build())
+    from ornis_amd.avbd import AvbdError
+    for prm in ((1.0, -1.0, 0, 0, 1, 0), (float("nan"), 1.0, 0, 0, 1, 0), (0, 0, 1.0, -1.0, 0, 1),
+                (0, 0, float("inf"), 1.0, 0, 1)):
+        with pytest.raises(AvbdError):
+            g.add_joint(0, 1, (0, 0, 0), (0, 0, 0), scenes.JOINT_REVOLUTE, (0, 0, 1), (0, 0, 1), prm=prm)
     assert g.stats()["joints"] == 0

@@ -264,6 +264,29 @@ def test_portable_sincos_asin_accuracy():
     assert max(ea) <= 2.0
 
 
+def test_portable_atan2_accuracy():
+    """oavbd_atan2f within 2 ulp of the correctly rounded atan2 over every
+    octant and magnitude ratio, with C's signed-zero, infinity and NaN
+    conventions."""
+    L = po.lib()
+    rng = np.random.default_rng(3)
+    ang = rng.uniform(-np.pi, np.pi, 20000)
+    mag = np.exp(rng.uniform(-12, 4, 20000))
+    ys = (mag * np.sin(ang)).astype(F)
+    xs = (mag * np.cos(ang)).astype(F)
+    extra = np.array([(1.0, 1.0), (1.0, -1.0), (-1.0, -1.0), (1e-30, 1.0), (1.0, 1e-30), (0.41421356, 1.0),
+                      (0.4142136, 1.0), (3.0, -1e-7)], F)
+    ys, xs = np.concatenate([ys, extra[:, 0]]), np.concatenate([xs, extra[:, 1]])
+    err = max(_ulp_err(np.array([L.orc_fmath_atan2(F(y), F(x))], F),
+                       np.arctan2(np.float64(y), np.float64(x)))[0] for y, x in zip(ys, xs))
+    assert err <= 2.0, err
+    for y, x in ((0.0, 0.0), (-0.0, 0.0), (0.0, -0.0), (-0.0, -0.0), (0.0, -1.0), (-0.0, -1.0), (2.0, 0.0),
+                 (-2.0, -0.0), (np.inf, 1.0), (1.0, -np.inf), (np.inf, np.inf), (-np.inf, -np.inf)):
+        got, want = L.orc_fmath_atan2(F(y), F(x)), np.float32(np.arctan2(F(y), F(x)))
+        assert got == want and np.signbit(got) == np.signbit(want), (y, x, got, want)
+    assert np.isnan(L.orc_fmath_atan2(F(np.nan), F(1.0)))
+
+
 def test_fp32_tracks_fp64_on_ordered_scenes():
     """FP32-vs-FP64 drift of the step (the GPU step equals the FP32 oracle
     bit for bit, so this is the GPU path's drift too): within 1e-3 m on the
