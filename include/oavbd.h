@@ -191,9 +191,10 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
  * oavbd_get_state, oavbd_get_sleep and oavbd_get_contacts. */
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
-/* Joint kinds (the reference's JointKind arms; Wheel, Gear, SixDof and
- * Motor are not built).  Rope and Spring are added with
- * oavbd_add_rope_joint / oavbd_add_spring_joint, Revolute and Prismatic
+/* Joint kinds (the reference's JointKind arms, numbered as its
+ * AvbdJointKind; Wheel 7, Gear 8 and SixDof 9 are not built).  Rope,
+ * Spring and Motor are added with oavbd_add_rope_joint /
+ * oavbd_add_spring_joint / oavbd_add_motor_joint, Revolute and Prismatic
  * with a limit or motor with oavbd_add_joint_limited, the others with
  * oavbd_add_joint. */
 enum {
@@ -203,7 +204,8 @@ enum {
   OAVBD_JOINT_FIXED = 3,
   OAVBD_JOINT_DISTANCE = 4,
   OAVBD_JOINT_ROPE = 5,
-  OAVBD_JOINT_SPRING = 6
+  OAVBD_JOINT_SPRING = 6,
+  OAVBD_JOINT_MOTOR = 10
 };
 
 /* A joint motor as a Spring joint carries it (JointMotor,
@@ -257,6 +259,18 @@ int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[
 int oavbd_add_joint_limited(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const float la[3],
                             const float lb[3], const float axis_a[3], const float axis_b[3], const float limit[2],
                             const float motor[2], uint32_t* out_index);
+
+/* Free motor between a and b: each step begins with budget-clamped
+ * velocity impulses (per world axis, through the pair's effective mass at
+ * the centres) driving b's velocity relative to a's towards linear_target
+ * (m/s, within max_force) and angular_target (rad/s, within max_torque);
+ * no position rows, no dual state.  It captures the bodies' relative pose
+ * for its rest residual; a budgeted non-zero target keeps the bodies
+ * awake.  EINVAL, before any wake, for a non-finite target or a negative
+ * budget.  Replaces: PhysicsEngine::add_joint with JointKind::Motor
+ * (mod.rs:1517-1525; free_motor_impulse, joints_ext.rs:368-421). */
+int oavbd_add_motor_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float linear_target[3],
+                          const float angular_target[3], float max_force, float max_torque, uint32_t* out_index);
 
 /* Rope between a and b: anchors la, lb may come closer freely but never
  * part beyond max_distance (one-sided row, a velocity catch each step).

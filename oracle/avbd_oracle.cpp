@@ -175,11 +175,13 @@ struct Pair {
   std::vector<Point<T>> pts;
 };
 
-// Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300): the
-// remaining kinds (Wheel, Gear, SixDof, Motor) are out of its scope.
+// Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300, same
+// numbering): Wheel, Gear and SixDof are out of its scope.
 enum JointKindC : int {
-  JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6
+  JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
+  JK_MOTOR = 10
 };
+inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // JointMotor (joint.rs:962-1126) as a Spring joint carries it: kind
 // (Velocity, Position, Servo), model (AccelerationBased, ForceBased).
 enum MotorKindC : int { MK_VELOCITY = 0, MK_POSITION = 1, MK_SERVO = 2 };
@@ -205,6 +207,10 @@ struct Joint {
   bool has_lim = false, has_mot = false;
   T lim_lo = T(0), lim_hi = T(0), mot_speed = T(0), mot_max = T(0);
   T acc_lim = T(0), lim_dual = T(0);
+  // Motor (free drive): relative linear / angular velocity targets (world,
+  // B minus A) and their force / torque budgets
+  V3<T> lin_target{T(0), T(0), T(0)}, ang_target{T(0), T(0), T(0)};
+  T max_force = T(0), max_torque = T(0);
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -1031,7 +1037,13 @@ struct Engine {
   // motor failing JointMotor::check (joint.rs:1101-1110), rejects before
   // the bodies wake (validate_joint runs first, mod.rs:1315).
   int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
-    if (kind < JK_BALL || kind > JK_SPRING) return -1;
+    if (kind < JK_BALL || (kind > JK_SPRING && kind != JK_MOTOR)) return -1;
+    if (kind == JK_MOTOR) {  // validate_joint (migration.rs:335-350): finite targets, budgets >= 0
+      if (!prm) return -1;
+      for (int k = 0; k < 8; ++k)
+        if (!std::isfinite(prm[k])) return -1;
+      if (!(prm[6] >= 0.0f && prm[7] >= 0.0f)) return -1;
+    }
     if (kind == JK_ROPE && !(prm && std::isfinite(prm[0]) && prm[0] > 0.0f)) return -1;
     if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && prm) {  // validate_joint (migration.rs:160-190)
       if (prm[4] != 0.0f && !(std::isfinite(prm[0]) && std::isfinite(prm[1]) && prm[0] <= prm[1])) return -1;
@@ -1082,6 +1094,17 @@ struct Engine {
       j.q_ref = quat_mul(conj(A.rot), B.rot);
       V3<T> ra = rotate(A.rot, la), rb = rotate(B.rot, lb);
       j.dref = rotate(conj(A.rot), (B.pos + rb) - (A.pos + ra));
+    }
+    if (kind == JK_MOTOR) {  // resolve_joint (joint.rs:1393-1399): the assembly relative pose
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      j.la = j.lb = V3<T>{T(0), T(0), T(0)};
+      j.q_ref = quat_mul(conj(A.rot), B.rot);
+      j.dref = rotate(conj(A.rot), B.pos - A.pos);
+      j.lin_target = V3<T>{T(prm[0]), T(prm[1]), T(prm[2])};
+      j.ang_target = V3<T>{T(prm[3]), T(prm[4]), T(prm[5])};
+      j.max_force = T(prm[6]);
+      j.max_torque = T(prm[7]);
     }
     if (kind == JK_DISTANCE) {
       const Body<T>& A = bodies[a];
@@ -1196,6 +1219,37 @@ struct Engine {
     const T v = limit_value(j, a.rot, b.rot, pa, pb, wa);
     return limit_state(v, j.lim_lo, j.lim_hi, j.kind == JK_REVOLUTE ? K<T>::LIMIT_SLOP_ANG() : K<T>::LIMIT_SLOP_LIN()) >= 0;
   }
+  // free_motor_impulse (joints_ext.rs:368-421): per world axis, the
+  // budget-clamped deadbeat linear then angular impulse through the pair's
+  // effective mass at the centres (MotorDrive::linear_impulse /
+  // angular_impulse, joint.rs:485-499).
+  void free_motor_impulse(const Joint<T>& j) {
+    const int ia = j.a, ib = j.b;
+    auto clamp_imp = [](T err, T k, T budget) -> T {
+      if (budget <= T(0) || k <= T(0)) return T(0);
+      return clampf(err / k, -budget * K<T>::DT(), budget * K<T>::DT());
+    };
+    for (int e = 0; e < 3; ++e) {
+      V3<T> axis{T(0), T(0), T(0)};
+      axis[e] = T(1);
+      const T v = dot(bodies[ib].vel - bodies[ia].vel, axis);
+      const T ka_lin = eff_inv_mass(ia), kb_lin = eff_inv_mass(ib);
+      const T dj = clamp_imp(dot(j.lin_target, axis) - v, ka_lin + kb_lin, j.max_force);
+      if (dj != T(0)) {
+        if (solvable(ia)) bodies[ia].vel = bodies[ia].vel + ((-dj) * ka_lin) * axis;
+        if (solvable(ib)) bodies[ib].vel = bodies[ib].vel + (dj * kb_lin) * axis;
+      }
+      const T w = dot(bodies[ib].angvel - bodies[ia].angvel, axis);
+      const V3<T> wia = ang_inv_wa(ia, axis), wib = ang_inv_wa(ib, axis);
+      const T ka = dot(wia, axis) + dot(wib, axis);
+      const T dw = clamp_imp(dot(j.ang_target, axis) - w, ka, j.max_torque);
+      if (dw != T(0)) {
+        if (solvable(ia)) bodies[ia].angvel = bodies[ia].angvel + (-dw) * wia;
+        if (solvable(ib)) bodies[ib].angvel = bodies[ib].angvel + dw * wib;
+      }
+    }
+  }
+
   // motor_impulse (joints_ext.rs:237-356) for Revolute / Prismatic velocity
   // motors: the budget-clamped deadbeat impulse driving the axis rate to
   // the target (JointMotor::velocity_impulse, joint.rs:1172-1177), paused
@@ -1203,6 +1257,10 @@ struct Engine {
   void motor_impulse() {
     const V3<T> Z{T(0), T(0), T(1)};
     for (const auto& j : joints) {
+      if (j.kind == JK_MOTOR) {
+        free_motor_impulse(j);
+        continue;
+      }
       if (!j.has_mot || (j.kind != JK_REVOLUTE && j.kind != JK_PRISMATIC)) continue;
       if (j.mot_max <= T(0) || joint_limit_violated(j)) continue;
       const int ia = j.a, ib = j.b;
@@ -1257,6 +1315,9 @@ struct Engine {
         return fmax_(length(delta) - j.ref_val, T(0));
       case JK_SPRING:
         return std::fabs(length(delta) - j.ref_val);
+      case JK_MOTOR:
+        return fmax_(length(delta - rotate(a.rot, j.dref)),
+                     length(quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math)));
       default:
         return length(delta);
     }
@@ -1270,6 +1331,9 @@ struct Engine {
   bool joint_driven(const Joint<T>& j) const {
     if (spring_keeps_awake(j)) return true;
     if (j.has_mot && j.mot_speed != T(0) && j.mot_max > T(0)) return true;
+    if (j.kind == JK_MOTOR && ((j.max_force > T(0) && !(j.lin_target == V3<T>{T(0), T(0), T(0)})) ||
+                               (j.max_torque > T(0) && !(j.ang_target == V3<T>{T(0), T(0), T(0)}))))
+      return true;  // MotorDrive::keeps_awake (joint.rs:474-477)
     int m[2] = {j.a, j.b};
     for (int h : m) {
       const Body<T>& b = bodies[h];
@@ -2229,7 +2293,9 @@ struct Engine {
       const bool fixed = j.kind == JK_FIXED;
       // live slide / hinge axis, from A's side (assembly.rs:273-275)
       const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
-      if (j.kind >= JK_DISTANCE) {
+      if (j.kind == JK_MOTOR) {
+        // free drive: no position rows (its impulse ran at the step start)
+      } else if (is_tether(j.kind)) {
         // Distance / Rope / Spring: one row along the live anchor delta
         // (assembly.rs:328-425)
         const T len = length(live);
@@ -2480,7 +2546,7 @@ struct Engine {
         } else {
           j.lam_l[0] = T(0);
         }
-      } else if (j.kind == JK_SPRING) {
+      } else if (j.kind == JK_SPRING || j.kind == JK_MOTOR) {
         // no dual state
       } else if (j.kind != JK_PRISMATIC) {
         const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;

@@ -103,6 +103,8 @@ SIGNATURES = {
     "oavbd_add_joint_limited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                                ctypes.POINTER(ctypes.c_uint32)]),
+    "oavbd_add_motor_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                             ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                             ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_spring_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
@@ -323,6 +325,14 @@ class AvbdEngine:
         la = np.ascontiguousarray(la, dtype=np.float32)
         lb = np.ascontiguousarray(lb, dtype=np.float32)
         idx = ctypes.c_uint32()
+        if int(kind) == 10:  # Motor: prm = linear target 3, angular target 3, max force, max torque
+            lt = np.ascontiguousarray(prm[0:3], dtype=np.float32)
+            at = np.ascontiguousarray(prm[3:6], dtype=np.float32)
+            self._check(self._L.oavbd_add_motor_joint(self._ctx, int(a), int(b), _p(lt, _f32p), _p(at, _f32p),
+                                                      ctypes.c_float(float(prm[6])), ctypes.c_float(float(prm[7])),
+                                                      ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
         if int(kind) == 5:
             self._check(self._L.oavbd_add_rope_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
                                                      ctypes.c_float(float(prm[0])), ctypes.byref(idx)))

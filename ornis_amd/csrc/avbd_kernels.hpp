@@ -235,13 +235,55 @@ __device__ void motor_impulse_one(const Dev& d, uint32_t j) {
   }
 }
 
+// free_motor_impulse (joints_ext.rs:368-421) of a Motor joint: per world
+// axis, the budget-clamped deadbeat linear then angular impulse through the
+// pair's effective mass at the centres (MotorDrive::linear_impulse /
+// angular_impulse, joint.rs:485-499).
+__device__ void free_motor_impulse(const Dev& d, uint32_t j) {
+  const uint2 ab = d.J.ab[j];
+  const float4 lt = d.J.ax[2 * j], at = d.J.ax[2 * j + 1];
+  for (int e = 0; e < 3; ++e) {
+    f3 axis = zero3();
+    setc(axis, e, 1.0f);
+    const float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+    const bool sa = d.b.type[ab.x] == BT_DYNAMIC && pa4.w > 0.0f, sbv = d.b.type[ab.y] == BT_DYNAMIC && pb4.w > 0.0f;
+    const float4 va4 = d.b.vel[ab.x], vb4 = d.b.vel[ab.y];
+    const float v = dot3(sub(xyz(vb4), xyz(va4)), axis);
+    const float ka_lin = d.b.type[ab.x] == BT_DYNAMIC ? pa4.w : 0.0f;
+    const float kb_lin = d.b.type[ab.y] == BT_DYNAMIC ? pb4.w : 0.0f;
+    const float kl = ka_lin + kb_lin;
+    float dj = 0.0f;
+    if (lt.w > 0.0f && kl > 0.0f) dj = clampf_((dot3(xyz(lt), axis) - v) / kl, -lt.w * K_DT, lt.w * K_DT);
+    if (dj != 0.0f) {
+      if (sa) d.b.vel[ab.x] = pk(add(xyz(va4), mul(axis, -dj * ka_lin)), va4.w);
+      if (sbv) d.b.vel[ab.y] = pk(add(xyz(vb4), mul(axis, dj * kb_lin)), vb4.w);
+    }
+    const float4 wa4 = d.b.angvel[ab.x], wb4 = d.b.angvel[ab.y];
+    const float w = dot3(sub(xyz(wb4), xyz(wa4)), axis);
+    const f3 ia = ang_inv_wa_dev(d, ab.x, axis), ib = ang_inv_wa_dev(d, ab.y, axis);
+    const float ka = dot3(ia, axis) + dot3(ib, axis);
+    float dw = 0.0f;
+    if (at.w > 0.0f && ka > 0.0f) dw = clampf_((dot3(xyz(at), axis) - w) / ka, -at.w * K_DT, at.w * K_DT);
+    if (dw != 0.0f) {
+      if (sa) d.b.angvel[ab.x] = pk(add(xyz(wa4), mul(ia, -dw)), wa4.w);
+      if (sbv) d.b.angvel[ab.y] = pk(add(xyz(wb4), mul(ib, dw)), wb4.w);
+    }
+  }
+}
+
 // rope_impulse (joints_ext.rs:173-223): once per step, after the torques,
 // every stretched rope whose ends separate has the separating anchor
 // velocity removed through the pair's effective mass.  Ropes sharing a body
 // see each other's updates, so one lane walks them in joint order.
 __global__ void k_joint_impulses(Dev d) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (uint32_t m = 0; m < d.nmotors; ++m) motor_impulse_one(d, d.motors[m]);  // all motors, then all ropes
+  for (uint32_t m = 0; m < d.nmotors; ++m) {  // all motors, then all ropes
+    const uint32_t j = d.motors[m];
+    if (d.J.kind[j] == JK_MOTOR)
+      free_motor_impulse(d, j);
+    else
+      motor_impulse_one(d, j);
+  }
   for (uint32_t r = 0; r < d.nropes; ++r) {
     const uint32_t j = d.ropes[r];
     const uint2 ab = d.J.ab[j];
@@ -1391,12 +1433,18 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   f3 pb = add(xyz(d.b.pos[ab.y]), qrot(qb, xyz(d.J.lb[j])));
   const f3 delta = sub(pb, pa);
   float err;
+  if (kind == JK_MOTOR) {  // MotorDrive::keeps_awake (joint.rs:474-477)
+    const float4 lt = d.J.ax[2 * j], at = d.J.ax[2 * j + 1];
+    if ((lt.w > 0.0f && (lt.x != 0.0f || lt.y != 0.0f || lt.z != 0.0f)) ||
+        (at.w > 0.0f && (at.x != 0.0f || at.y != 0.0f || at.z != 0.0f)))
+      return true;
+  }
   if (kind == JK_BALL) {
     err = len3(delta);
-  } else if (kind >= JK_DISTANCE) {  // a slack rope is at rest
+  } else if (is_tether(kind)) {  // a slack rope is at rest
     const float e = len3(delta) - d.J.dref[j].x;
     err = kind == JK_ROPE ? fmax_(e, 0.0f) : fabsf(e);
-  } else if (kind == JK_FIXED) {  // joint_rest_error (sleep.rs:30-66)
+  } else if (kind == JK_FIXED || kind == JK_MOTOR) {  // joint_rest_error (sleep.rs:30-82)
     err = fmax_(len3(sub(delta, qrot(qa, xyz(d.J.dref[j])))), len3(quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]))));
   } else {
     const f3 axis = qrot(qa, xyz(d.J.ax[2 * j]));
@@ -2285,7 +2333,9 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   f3 r_b = qrot(qb, lb);
   float fl[3] = {0.0f, 0.0f, 0.0f};
   Row row;
-  if (kind >= JK_DISTANCE) {
+  if (kind == JK_MOTOR) {
+    // free drive: no position rows (its impulse ran at the step start)
+  } else if (is_tether(kind)) {
     // Distance / Rope / Spring: one row along the live anchor delta
     // (assembly.rs:328-425)
     const float4 pr = d.J.dref[j];
@@ -3146,7 +3196,7 @@ __global__ void k_dual_joint(Dev d) {
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
   float lv[3] = {lam.x, lam.y, lam.z}, pv[3] = {pen.x, pen.y, pen.z};
   if (kind >= JK_DISTANCE) {
-    if (kind == JK_SPRING) return;  // no dual state
+    if (kind == JK_SPRING || kind == JK_MOTOR) return;  // no dual state
     const float ref = d.J.dref[j].x;
     const float len = len3(live);
     if (kind == JK_DISTANCE) {

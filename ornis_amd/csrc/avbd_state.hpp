@@ -140,7 +140,11 @@ struct Points {
 };
 
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
-enum { JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6 };
+enum {
+  JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
+  JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Wheel 7, Gear 8, SixDof 9 are not built)
+};
+__host__ __device__ inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // Flags of a Spring joint (the w of its parameter record): the motor's
 // force-based model; the motor keeps its bodies awake (JointMotor::keeps_awake)
 enum { SPRING_FORCE_MODEL = 1, SPRING_KEEPS_AWAKE = 2 };
@@ -156,11 +160,12 @@ struct Joints {
   float4* la;        // local anchor on A
   float4* lb;        // local anchor on B
   float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
-  float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames
-  float4* dref;      // Fixed: assembly anchor offset in A's frame; Distance / Rope / Spring:
+  float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames; Motor: the
+                     // linear target and force budget, the angular target and torque budget
+  float4* dref;      // Fixed / Motor: assembly anchor offset in A's frame; Distance / Rope / Spring:
                      // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags);
                      // Revolute / Prismatic: (assembly angle | length, limit lo, hi, HINGE_* flags)
-  float4* qref;      // Fixed: assembly relative rotation conj(qa) qb; Revolute / Prismatic:
+  float4* qref;      // Fixed / Motor: assembly relative rotation conj(qa) qb; Revolute / Prismatic:
                      // the motor (target speed, budget, 0, 0)
   float4* lam;       // lam_l xyz; w: a Prismatic limit's force accumulator
   float4* pen;       // pen_l xyz
@@ -217,7 +222,7 @@ struct Dev {
   uint32_t njoints;
   uint32_t* ropes;   // Rope joint indices, ascending (the velocity catch runs in joint order)
   uint32_t nropes;
-  uint32_t* motors;  // Revolute / Prismatic joints with a motor, ascending
+  uint32_t* motors;  // Motor joints and Revolute / Prismatic joints with a motor, ascending
   uint32_t nmotors;
   Grid g;
   Cands C;

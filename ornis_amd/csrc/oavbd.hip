@@ -625,13 +625,14 @@ static void upload_joints(oavbd_ctx* c) {
     const bool fixed = h.kind == JK_FIXED;
     la0[j] = make_float4(h.la[0] + (fixed ? h.dref[0] : 0.0f), h.la[1] + (fixed ? h.dref[1] : 0.0f),
                          h.la[2] + (fixed ? h.dref[2] : 0.0f), 0.0f);
-    ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], 0.0f);
-    ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], 0.0f);
+    const bool motor = h.kind == JK_MOTOR;  // targets and budgets
+    ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], motor ? h.prm[0] : 0.0f);
+    ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], motor ? h.prm[1] : 0.0f);
     const bool hinge = h.kind == JK_REVOLUTE || h.kind == JK_PRISMATIC;
-    dref[j] = (h.kind >= JK_DISTANCE || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
-                                               : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    dref[j] = (is_tether(h.kind) || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
+                                           : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
     if (h.kind == JK_ROPE) ropes.push_back(j);
-    if (hinge && ((uint32_t)h.prm[3] & HINGE_MOTOR)) motors.push_back(j);
+    if ((hinge && ((uint32_t)h.prm[3] & HINGE_MOTOR)) || h.kind == JK_MOTOR) motors.push_back(j);
     qref[j] = hinge ? make_float4(h.mot[0], h.mot[1], 0.0f, 0.0f) : make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
     lam[j] = j < kept ? kept_lam[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -2394,7 +2395,7 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     j.mot[0] = mot[0];
     j.mot[1] = mot[1];
   }
-  if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
+  if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_MOTOR) {
     // resolve_joint (joint.rs:1313-1332): the assembly rotation and anchor
     // offset (Fixed) or anchor distance (Distance) from the current poses
     float4 p[2], q[2];
@@ -2408,7 +2409,7 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     const qf qr = qmul(qconj(qa), qb);
     const f3 ra = qrot(qa, f3{la[0], la[1], la[2]}), rb = qrot(qb, f3{lb[0], lb[1], lb[2]});
     const f3 dr = qrot(qconj(qa), sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
-    if (kind == JK_FIXED) {
+    if (kind == JK_FIXED || kind == JK_MOTOR) {  // Motor: la = lb = 0, so dr = conj(qa) (pb - pa)
       j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
       j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
     } else if (kind == JK_DISTANCE) {
@@ -2431,7 +2432,7 @@ int oavbd_add_joint(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const f
   if (!c || !la || !lb) return OAVBD_EINVAL;
   if (kind == JK_ROPE || kind == JK_SPRING)
     return fail(c, OAVBD_EINVAL, "rope and spring joints take oavbd_add_rope_joint / oavbd_add_spring_joint");
-  if (kind > JK_SPRING) return fail(c, OAVBD_EINVAL, "unsupported joint kind");
+  if (kind > JK_SPRING) return fail(c, OAVBD_EINVAL, "unsupported joint kind (Motor: oavbd_add_motor_joint)");
   return add_joint_impl(c, kind, a, b, la, lb, axis_a, axis_b, nullptr, out_index);
 }
 
@@ -2454,6 +2455,30 @@ int oavbd_add_joint_limited(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b,
   const float flags = (float)((limit ? HINGE_LIMIT : 0) | (motor ? HINGE_MOTOR : 0));
   const float prm[4] = {0.0f, limit ? limit[0] : 0.0f, limit ? limit[1] : 0.0f, flags};
   return add_joint_impl(c, kind, a, b, la, lb, axis_a, axis_b, prm, out_index, motor);
+}
+
+int oavbd_add_motor_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float linear_target[3],
+                          const float angular_target[3], float max_force, float max_torque, uint32_t* out_index) {
+  if (!c || !linear_target || !angular_target) return OAVBD_EINVAL;
+  // validate_joint (migration.rs:335-350), before any wake
+  bool ok = std::isfinite(max_force) && max_force >= 0.0f && std::isfinite(max_torque) && max_torque >= 0.0f;
+  for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(linear_target[k]) && std::isfinite(angular_target[k]);
+  if (!ok) return fail(c, OAVBD_EINVAL, "motor joint needs finite targets and budgets >= 0");
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  const float ax[3] = {linear_target[0], linear_target[1], linear_target[2]};
+  const float bx[3] = {angular_target[0], angular_target[1], angular_target[2]};
+  uint32_t idx = 0;
+  const int rc = add_joint_impl(c, JK_MOTOR, a, b, zero, zero, nullptr, nullptr, nullptr, &idx);
+  if (rc != OAVBD_OK) return rc;
+  HostJoint& j = c->joints[idx];  // the targets and budgets ride in the axis records
+  for (int k = 0; k < 3; ++k) {
+    j.axa[k] = ax[k];
+    j.axb[k] = bx[k];
+  }
+  j.prm[0] = max_force;
+  j.prm[1] = max_torque;
+  if (out_index) *out_index = idx;
+  return OAVBD_OK;
 }
 
 int oavbd_add_rope_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],

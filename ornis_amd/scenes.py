@@ -29,6 +29,7 @@ _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 # joint kinds (include/oavbd.h)
 JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
 JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING = 4, 5, 6
+JOINT_MOTOR = 10
 # motor kinds and models of a Spring joint (oavbd_motor)
 MOTOR_VELOCITY, MOTOR_POSITION, MOTOR_SERVO = 0, 1, 2
 MOTOR_ACCELERATION_BASED, MOTOR_FORCE_BASED = 0, 1
@@ -104,6 +105,14 @@ class SceneBuilder:
         self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_SPRING, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
                             (float(rest), float(stiffness), float(damping), float(max_force), float(target_velocity),
                              float(motor_kind), float(model))))
+        return len(self.joints) - 1
+
+    def motor(self, a, b, linear_target, angular_target, max_force, max_torque):
+        """Free Motor joint: drives b's velocity relative to a's (world frame)
+        towards the targets within the force / torque budgets."""
+        self.joints.append((int(a), int(b), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), JOINT_MOTOR, (1.0, 0.0, 0.0),
+                            (1.0, 0.0, 0.0), tuple(float(x) for x in tuple(linear_target) + tuple(angular_target)) +
+                            (float(max_force), float(max_torque))))
         return len(self.joints) - 1
 
     def build(self):

@@ -323,3 +323,41 @@ def test_limit_motor_spec_validation(gpu_available):
         with pytest.raises(AvbdError):
             g.add_joint(0, 1, (0, 0, 0), (0, 0, 0), scenes.JOINT_REVOLUTE, (0, 0, 1), (0, 0, 1), prm=prm)
     assert g.stats()["joints"] == 0
+
+
+def test_motor_joint_scenes_bitwise(gpu_available):
+    """Free Motor joints (joints_ext.rs:368-421): a box driven along x
+    against floor friction (motor_joint.rs:90-110), a budget below the
+    friction, a free ball spun up about z, and drives between pairs of
+    dynamic bodies tumbling onto a floor."""
+    for lin, ang, fmax, tmax in (((2.0, 0.0, 0.0), (0.0, 0.0, 0.0), 8.0, 10.0),
+                                 ((2.0, 0.0, 0.0), (0.0, 0.0, 0.0), 3.0, 0.0)):
+        sb = scenes.SceneBuilder()
+        sb.box((0.0, -0.5, 0.0), (10.0, 0.5, 10.0), 0.0)
+        block = sb.box((0.0, 0.5, 0.0), (0.5, 0.5, 0.5), 1.0)
+        anchor = sb.sphere((0.0, 5.0, 0.0), 0.1, 0.0)
+        sb.motor(anchor, block, lin, ang, fmax, tmax)
+        g, _ = _run(sb.build(), 300, every=20)
+        if fmax > 5.0:
+            assert abs(g.state()[block, 7] - 2.0) < 0.4
+    sb = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+    a = sb.sphere((0.0, 0.0, 0.0), 0.1, 0.0)
+    ball = sb.sphere((0.0, 2.0, 0.0), 0.3, 1.0)
+    sb.motor(a, ball, (0.0, 0.0, 0.0), (0.0, 0.0, 3.0), 0.0, 10.0)
+    _run(sb.build(), 120, every=10)
+    rng = np.random.Generator(np.random.PCG64(4))
+    sb = scenes.SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    for i in range(12):
+        q = rng.normal(size=4)
+        q = tuple(float(v) for v in (q / np.linalg.norm(q)).astype(np.float32))
+        p1 = (float(rng.uniform(-3, 3)), 1.0 + 0.8 * i, float(rng.uniform(-3, 3)))
+        h1 = sb.box(p1, tuple(rng.uniform(0.15, 0.35, 3)), float(rng.uniform(0.5, 2)), quat=q)
+        h2 = sb.sphere((p1[0] + 0.9, p1[1], p1[2]), float(rng.uniform(0.15, 0.3)), float(rng.uniform(0.5, 2)))
+        sb.motor(h1, h2, tuple(rng.uniform(-2, 2, 3)), tuple(rng.uniform(-4, 4, 3)), float(rng.uniform(0, 20)),
+                 float(rng.uniform(0, 5)))
+    g, o = _run(sb.build(), 200, every=10)
+    assert np.isfinite(g.state()).all()
+    with pytest.raises(Exception):
+        AvbdEngine(scene=sb.build()).add_joint(1, 2, (0, 0, 0), (0, 0, 0), 10,
+                                               prm=(float("nan"), 0, 0, 0, 0, 0, 1.0, 1.0))
