@@ -192,9 +192,9 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
 /* Joint kinds (the reference's JointKind arms, numbered as its
- * AvbdJointKind; Wheel 7, Gear 8 and SixDof 9 are not built).  Rope,
- * Spring and Motor are added with oavbd_add_rope_joint /
- * oavbd_add_spring_joint / oavbd_add_motor_joint, Revolute and Prismatic
+ * AvbdJointKind; Gear 8 and SixDof 9 are not built).  Rope, Spring, Wheel
+ * and Motor are added with oavbd_add_rope_joint / oavbd_add_spring_joint /
+ * oavbd_add_wheel_joint / oavbd_add_motor_joint, Revolute and Prismatic
  * with a limit or motor with oavbd_add_joint_limited, the others with
  * oavbd_add_joint. */
 enum {
@@ -205,6 +205,7 @@ enum {
   OAVBD_JOINT_DISTANCE = 4,
   OAVBD_JOINT_ROPE = 5,
   OAVBD_JOINT_SPRING = 6,
+  OAVBD_JOINT_WHEEL = 7,
   OAVBD_JOINT_MOTOR = 10
 };
 
@@ -259,6 +260,24 @@ int oavbd_add_ball_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[
 int oavbd_add_joint_limited(oavbd_ctx* ctx, uint32_t kind, uint32_t a, uint32_t b, const float la[3],
                             const float lb[3], const float axis_a[3], const float axis_b[3], const float limit[2],
                             const float motor[2], uint32_t* out_index);
+
+/* Wheel between a and b (Box2D's wheel formulation): a slide along the
+ * suspension axis (suspension_a / _b in each body's frame, normalised,
+ * +y when degenerate) held by a spring of frequency_hz and damping_ratio
+ * (a fixed penalty m w^2 and damping 2 m zeta w with m the bodies' reduced
+ * mass; a rigid slide when frequency_hz is 0), the two directions across
+ * the axle (axle_a / _b, orthogonalised against the suspension) locked,
+ * spin about it free, and an optional axle velocity motor (motor[0]
+ * rad/s, motor[1] torque budget; NULL for none).  EINVAL, before any wake,
+ * for a non-finite axis or anchor, a negative frequency or damping ratio,
+ * or a bad motor.  Replaces: PhysicsEngine::add_joint with JointKind::Wheel
+ * (mod.rs:1494-1501; resolve_joint, crates/physics/src/joint.rs:1352-1380;
+ * rows assembly.rs:521-580, :753-781; duals joints_ext.rs:738-805; motor
+ * joints_ext.rs:337-369). */
+int oavbd_add_wheel_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                          const float suspension_a[3], const float suspension_b[3], const float axle_a[3],
+                          const float axle_b[3], float frequency_hz, float damping_ratio, const float motor[2],
+                          uint32_t* out_index);
 
 /* Free motor between a and b: each step begins with budget-clamped
  * velocity impulses (per world axis, through the pair's effective mass at

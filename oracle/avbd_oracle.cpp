@@ -176,10 +176,10 @@ struct Pair {
 };
 
 // Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300, same
-// numbering): Wheel, Gear and SixDof are out of its scope.
+// numbering): Gear and SixDof are out of its scope.
 enum JointKindC : int {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_MOTOR = 10
+  JK_WHEEL = 7, JK_MOTOR = 10
 };
 inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // JointMotor (joint.rs:962-1126) as a Spring joint carries it: kind
@@ -211,6 +211,10 @@ struct Joint {
   // B minus A) and their force / torque budgets
   V3<T> lin_target{T(0), T(0), T(0)}, ang_target{T(0), T(0), T(0)};
   T max_force = T(0), max_torque = T(0);
+  // Wheel: the axle in each body's frame (ax_a / ax_b hold the suspension
+  // axes) and the suspension's frequency (Hz) and damping ratio
+  V3<T> bx_a{T(1), T(0), T(0)}, bx_b{T(1), T(0), T(0)};
+  T susp_hz = T(0), susp_zeta = T(0);
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -281,6 +285,20 @@ void tangent_basis(V3<T> n, V3<T>* t1, V3<T>* t2) {
   V3<T> axis = std::fabs(n.x) < K<T>::TANGENT_DOT() ? V3<T>{T(1), T(0), T(0)} : V3<T>{T(0), T(1), T(0)};
   *t1 = normalize_or(cross(n, axis), V3<T>{T(0), T(0), T(1)});
   *t2 = cross(*t1, n);
+}
+
+// orthogonalize_axle (crates/physics/src/math.rs:179-186): the axle with
+// its suspension component removed, normalised; the suspension's first
+// tangent when that leaves (almost) nothing.
+template <class T>
+V3<T> orthogonalize_axle(V3<T> suspension, V3<T> axle) {
+  const V3<T> a = axle - suspension * dot(axle, suspension);
+  if (length_sq(a) < CF(1e-6)) {
+    V3<T> t1, t2;
+    tangent_basis(suspension, &t1, &t2);
+    return t1;
+  }
+  return normalize(a);
 }
 
 // friction_frame (rows.rs:32-52): the first friction direction from body
@@ -1037,7 +1055,15 @@ struct Engine {
   // motor failing JointMotor::check (joint.rs:1101-1110), rejects before
   // the bodies wake (validate_joint runs first, mod.rs:1315).
   int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
-    if (kind < JK_BALL || (kind > JK_SPRING && kind != JK_MOTOR)) return -1;
+    if (kind < JK_BALL || (kind > JK_WHEEL && kind != JK_MOTOR)) return -1;
+    if (kind == JK_WHEEL) {  // validate_joint (migration.rs:288-315); prm = axle a 3, axle b 3, Hz, zeta,
+                             // motor speed, budget, has motor
+      if (!prm) return -1;
+      for (int k = 0; k < 10; ++k)
+        if (!std::isfinite(prm[k])) return -1;
+      if (!(prm[6] >= 0.0f && prm[7] >= 0.0f)) return -1;
+      if (prm[10] != 0.0f && !(prm[9] >= 0.0f)) return -1;
+    }
     if (kind == JK_MOTOR) {  // validate_joint (migration.rs:335-350): finite targets, budgets >= 0
       if (!prm) return -1;
       for (int k = 0; k < 8; ++k)
@@ -1094,6 +1120,22 @@ struct Engine {
       j.q_ref = quat_mul(conj(A.rot), B.rot);
       V3<T> ra = rotate(A.rot, la), rb = rotate(B.rot, lb);
       j.dref = rotate(conj(A.rot), (B.pos + rb) - (A.pos + ra));
+    }
+    if (kind == JK_WHEEL) {  // resolve_joint (joint.rs:1352-1380)
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      const V3<T> Y{T(0), T(1), T(0)};
+      j.ax_a = normalize_or(axa, Y);
+      j.ax_b = normalize_or(axb, Y);
+      j.bx_a = orthogonalize_axle(j.ax_a, V3<T>{T(prm[0]), T(prm[1]), T(prm[2])});
+      j.bx_b = orthogonalize_axle(j.ax_b, V3<T>{T(prm[3]), T(prm[4]), T(prm[5])});
+      const V3<T> wa = normalize_or(rotate(A.rot, j.ax_a), Z);
+      j.ref_val = dot((B.pos + rotate(B.rot, lb)) - (A.pos + rotate(A.rot, la)), wa);
+      j.susp_hz = T(prm[6]);
+      j.susp_zeta = T(prm[7]);
+      j.has_mot = prm[10] != 0.0f;
+      j.mot_speed = T(prm[8]);
+      j.mot_max = T(prm[9]);
     }
     if (kind == JK_MOTOR) {  // resolve_joint (joint.rs:1393-1399): the assembly relative pose
       const Body<T>& A = bodies[a];
@@ -1261,15 +1303,16 @@ struct Engine {
         free_motor_impulse(j);
         continue;
       }
-      if (!j.has_mot || (j.kind != JK_REVOLUTE && j.kind != JK_PRISMATIC)) continue;
+      if (!j.has_mot || (j.kind != JK_REVOLUTE && j.kind != JK_PRISMATIC && j.kind != JK_WHEEL)) continue;
       if (j.mot_max <= T(0) || joint_limit_violated(j)) continue;
       const int ia = j.a, ib = j.b;
-      const V3<T> wa = normalize_or(rotate(bodies[ia].rot, j.ax_a), Z);
+      // a wheel's motor turns about its axle (joints_ext.rs:337-369)
+      const V3<T> wa = normalize_or(rotate(bodies[ia].rot, j.kind == JK_WHEEL ? j.bx_a : j.ax_a), Z);
       auto impulse = [&](T err, T k) -> T {
         if (k <= T(0)) return T(0);
         return clampf(err / k, -j.mot_max * K<T>::DT(), j.mot_max * K<T>::DT());
       };
-      if (j.kind == JK_REVOLUTE) {
+      if (j.kind == JK_REVOLUTE || j.kind == JK_WHEEL) {
         const T w = dot(bodies[ib].angvel - bodies[ia].angvel, wa);
         const V3<T> wia = ang_inv_wa(ia, wa), wib = ang_inv_wa(ib, wa);
         const T k = dot(wia, wa) + dot(wib, wa);
@@ -1315,6 +1358,13 @@ struct Engine {
         return fmax_(length(delta) - j.ref_val, T(0));
       case JK_SPRING:
         return std::fabs(length(delta) - j.ref_val);
+      case JK_WHEEL: {
+        V3<T> perp = delta - axis * dot(delta, axis);
+        T error = length(perp);
+        error = fmax_(error, length(rotate(a.rot, j.bx_a) - rotate(b.rot, j.bx_b)));
+        if (j.susp_hz <= T(0)) error = fmax_(error, std::fabs(dot(delta, axis) - j.ref_val));
+        return error;
+      }
       case JK_MOTOR:
         return fmax_(length(delta - rotate(a.rot, j.dref)),
                      length(quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math)));
@@ -2334,6 +2384,43 @@ struct Engine {
             acc.stamp(dir, pen_v, f, r_side, sign);
           }
         }
+      } else if (j.kind == JK_WHEEL) {
+        // Wheel (assembly.rs:521-580): the two rows across the live
+        // suspension axis, then the suspension spring along it (a fixed
+        // penalty m w^2 from the frequency, damping 2 m zeta w, no dual),
+        // or a rigid row on slot 2 when the frequency is not positive
+        V3<T> u, v;
+        tangent_basis(wa, &u, &v);
+        const V3<T> axes[2] = {u, v};
+        const V3<T> r_side = is_a ? rotate(a.rot, j.la) : rotate(b.rot, j.lb);
+        for (int li = 0; li < 2; ++li) {
+          const V3<T> ax = axes[li];
+          T c = dot(live - c0v * K<T>::ALPHA(), ax);
+          T f = j.pen_l[li] * c + j.lam_l[li];
+          if (!row_live(c, f)) continue;
+          acc.stamp(ax, j.pen_l[li], f, r_side, sign);
+          fl = fl + ax * f;
+        }
+        const T sv = dot(pb - pa, wa) - j.ref_val;
+        const T lsign = is_a ? T(-1) : T(1);  // C = (B - A).wa
+        const V3<T> wa0 = normalize_or(rotate(rot0[j.a], j.ax_a), V3<T>{T(0), T(0), T(1)});
+        if (j.susp_hz > T(0)) {
+          const T ka = a.type == BT_DYNAMIC ? a.inv_mass : T(0);
+          const T kb = b.type == BT_DYNAMIC ? b.inv_mass : T(0);
+          const T m = ka + kb > K<T>::NEAR_ZERO() ? T(1) / (ka + kb) : T(0);
+          if (m > T(0)) {
+            const T omega = K<T>::TAU() * j.susp_hz;
+            const T stiff = m * omega * omega;
+            const T dampc = T(2) * m * j.susp_zeta * omega;
+            const T s0 = dot(pb0 - pa0, wa0) - j.ref_val;
+            const T f = stiff * sv + dampc * (sv - s0) / K<T>::DT();
+            acc.stamp(wa, stiff, f, r_side, lsign);
+          }
+        } else {
+          const T c = sv - K<T>::ALPHA() * (dot(pb0 - pa0, wa0) - j.ref_val);
+          const T f = j.pen_l[2] * c + j.lam_l[2];
+          if (row_live(c, f)) acc.stamp(wa, j.pen_l[2], f, r_side, lsign);
+        }
       } else if (j.kind != JK_PRISMATIC) {
         // Ball / Revolute / Fixed: three world-axis rows at the anchors
         // (assembly.rs:279-310), Fixed offset by the assembly delta
@@ -2382,9 +2469,11 @@ struct Engine {
                length(V3<T>{hm[0][2], hm[1][2], hm[2][2]})};
       acc.diag(hd);
       // angular equality rows (assembly.rs:620-648, :787-808)
-      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC) {
-        const V3<T> axa = rotate(a.rot, j.ax_a), axb = rotate(b.rot, j.ax_b);
-        const V3<T> axa0 = rotate(rot0[j.a], j.ax_a), axb0 = rotate(rot0[j.b], j.ax_b);
+      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC || j.kind == JK_WHEEL) {
+        // a wheel locks the two directions across its axle (assembly.rs:753-781)
+        const V3<T> ea = j.kind == JK_WHEEL ? j.bx_a : j.ax_a, eb = j.kind == JK_WHEEL ? j.bx_b : j.ax_b;
+        const V3<T> axa = rotate(a.rot, ea), axb = rotate(b.rot, eb);
+        const V3<T> axa0 = rotate(rot0[j.a], ea), axb0 = rotate(rot0[j.b], eb);
         V3<T> t1, t2;
         tangent_basis(axa0, &t1, &t2);
         const V3<T> ts[2] = {t1, t2};
@@ -2548,6 +2637,28 @@ struct Engine {
         }
       } else if (j.kind == JK_SPRING || j.kind == JK_MOTOR) {
         // no dual state
+      } else if (j.kind == JK_WHEEL) {
+        // the perp rows, and the rigid row on slot 2 when the suspension
+        // has no frequency (joints_ext.rs:738-766)
+        const V3<T> wa = normalize_or(rotate(a.rot, j.ax_a), V3<T>{T(0), T(0), T(1)});
+        V3<T> u, v;
+        tangent_basis(wa, &u, &v);
+        const V3<T> axes[2] = {u, v};
+        for (int li = 0; li < 2; ++li) {
+          T c = dot(live - c0v * K<T>::ALPHA(), axes[li]);
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            j.lam_l[li] += j.pen_l[li] * c;
+            j.pen_l[li] = fmin_(j.pen_l[li] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+          }
+        }
+        if (j.susp_hz <= T(0)) {
+          const V3<T> wa0 = normalize_or(rotate(rot0[j.a], j.ax_a), V3<T>{T(0), T(0), T(1)});
+          T c = (-dot(live, wa) - j.ref_val) - K<T>::ALPHA() * (-dot(pa0 - pb0, wa0) - j.ref_val);
+          if (std::fabs(c) >= K<T>::C_EPS()) {
+            j.lam_l[2] += j.pen_l[2] * c;
+            j.pen_l[2] = fmin_(j.pen_l[2] + K<T>::BETA() * std::fabs(c), K<T>::LIM_PEN_MAX());
+          }
+        }
       } else if (j.kind != JK_PRISMATIC) {
         const V3<T> lv = fixed ? live + rotate(a.rot, j.dref) : live;
         const V3<T> in0 = fixed ? c0v + rotate(rot0[j.a], j.dref) : c0v;
@@ -2571,10 +2682,11 @@ struct Engine {
           }
         }
       }
-      // angular equality rows (joints_ext.rs:767-822)
-      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC) {
-        const V3<T> axa = rotate(a.rot, j.ax_a), axb = rotate(b.rot, j.ax_b);
-        const V3<T> axa0 = rotate(rot0[j.a], j.ax_a), axb0 = rotate(rot0[j.b], j.ax_b);
+      // angular equality rows (joints_ext.rs:767-822), a wheel's about its axle
+      if (j.kind == JK_REVOLUTE || j.kind == JK_PRISMATIC || j.kind == JK_WHEEL) {
+        const V3<T> ea = j.kind == JK_WHEEL ? j.bx_a : j.ax_a, eb = j.kind == JK_WHEEL ? j.bx_b : j.ax_b;
+        const V3<T> axa = rotate(a.rot, ea), axb = rotate(b.rot, eb);
+        const V3<T> axa0 = rotate(rot0[j.a], ea), axb0 = rotate(rot0[j.b], eb);
         V3<T> t1, t2;
         tangent_basis(axa0, &t1, &t2);
         const V3<T> ts[2] = {t1, t2};

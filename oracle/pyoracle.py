@@ -350,7 +350,7 @@ class OracleEngine:
             r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
                                       _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))
         else:
-            pr = np.zeros(8, np.float32)
+            pr = np.zeros(12, np.float32)
             pr[:len(prm)] = prm
             r = self._L.orc_add_joint_prm(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p),
                                           _p(f32(lb), _f32p), _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p),

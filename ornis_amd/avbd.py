@@ -103,6 +103,9 @@ SIGNATURES = {
     "oavbd_add_joint_limited": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                                _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
                                                ctypes.POINTER(ctypes.c_uint32)]),
+    "oavbd_add_wheel_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p, _f32p,
+                                             _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p,
+                                             ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_motor_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                              ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
@@ -325,7 +328,17 @@ class AvbdEngine:
         la = np.ascontiguousarray(la, dtype=np.float32)
         lb = np.ascontiguousarray(lb, dtype=np.float32)
         idx = ctypes.c_uint32()
-        if int(kind) == 10:  # Motor: prm = linear target 3, angular target 3, max force, max torque
+        if int(kind) == 7 and prm is not None:  # Wheel: axes = suspensions; prm = axle a 3, axle b 3, Hz, zeta, speed, budget, has motor
+            f = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+            sa, sb, xa_, xb_ = f(axis_a), f(axis_b), f(prm[0:3]), f(prm[3:6])
+            mot = f(prm[8:10])
+            self._check(self._L.oavbd_add_wheel_joint(
+                self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p), _p(sa, _f32p), _p(sb, _f32p),
+                _p(xa_, _f32p), _p(xb_, _f32p), ctypes.c_float(float(prm[6])), ctypes.c_float(float(prm[7])),
+                _p(mot, _f32p) if prm[10] else None, ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
+        if int(kind) == 10 and prm is not None:  # Motor: prm = linear target 3, angular target 3, max force, max torque
             lt = np.ascontiguousarray(prm[0:3], dtype=np.float32)
             at = np.ascontiguousarray(prm[3:6], dtype=np.float32)
             self._check(self._L.oavbd_add_motor_joint(self._ctx, int(a), int(b), _p(lt, _f32p), _p(at, _f32p),
@@ -333,12 +346,12 @@ class AvbdEngine:
                                                       ctypes.byref(idx)))
             self.njoints += 1
             return idx.value
-        if int(kind) == 5:
+        if int(kind) == 5 and prm is not None:
             self._check(self._L.oavbd_add_rope_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
                                                      ctypes.c_float(float(prm[0])), ctypes.byref(idx)))
             self.njoints += 1
             return idx.value
-        if int(kind) == 6:
+        if int(kind) == 6 and prm is not None:
             m = OavbdMotor(kind=int(prm[5]), model=int(prm[6]), target_position=prm[0], target_velocity=prm[4],
                            stiffness=prm[1], damping=prm[2], max_force=prm[3])
             self._check(self._L.oavbd_add_spring_joint(self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),

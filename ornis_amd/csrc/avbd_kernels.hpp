@@ -213,7 +213,7 @@ __device__ void motor_impulse_one(const Dev& d, uint32_t j) {
   }
   const bool sa = d.b.type[ab.x] == BT_DYNAMIC && pa4.w > 0.0f, sbv = d.b.type[ab.y] == BT_DYNAMIC && pb4.w > 0.0f;
   const float cap = mo.y * K_DT;
-  if (kind == JK_REVOLUTE) {
+  if (kind == JK_REVOLUTE || kind == JK_WHEEL) {  // a wheel turns about its axle (ax holds it)
     const float4 wa4 = d.b.angvel[ab.x], wb4 = d.b.angvel[ab.y];
     const float w = dot3(sub(xyz(wb4), xyz(wa4)), wa);
     const f3 ia = ang_inv_wa_dev(d, ab.x, wa), ib = ang_inv_wa_dev(d, ab.y, wa);
@@ -1424,7 +1424,7 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   }
   const int kind = d.J.kind[j];
   if (kind == JK_SPRING && ((uint32_t)d.J.dref[j].w & SPRING_KEEPS_AWAKE)) return true;
-  if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC) && ((uint32_t)d.J.dref[j].w & HINGE_MOTOR)) {
+  if ((kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_WHEEL) && ((uint32_t)d.J.dref[j].w & HINGE_MOTOR)) {
     const float4 mo = d.J.qref[j];  // a turning motor with a budget keeps its bodies awake
     if (mo.x != 0.0f && mo.y > 0.0f) return true;
   }
@@ -1444,6 +1444,12 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   } else if (is_tether(kind)) {  // a slack rope is at rest
     const float e = len3(delta) - d.J.dref[j].x;
     err = kind == JK_ROPE ? fmax_(e, 0.0f) : fabsf(e);
+  } else if (kind == JK_WHEEL) {  // joint_rest_error (sleep.rs:61-76)
+    const f3 axis = qrot(qa, xyz(d.J.sus[j]));
+    const float4 pr = d.J.dref[j];
+    err = len3(sub(delta, mul(axis, dot3(delta, axis))));
+    err = fmax_(err, len3(sub(qrot(qa, xyz(d.J.ax[2 * j])), qrot(qb, xyz(d.J.ax[2 * j + 1])))));
+    if (!(pr.y > 0.0f)) err = fmax_(err, fabsf(dot3(delta, axis) - pr.x));
   } else if (kind == JK_FIXED || kind == JK_MOTOR) {  // joint_rest_error (sleep.rs:30-82)
     err = fmax_(len3(sub(delta, qrot(qa, xyz(d.J.dref[j])))), len3(quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]))));
   } else {
@@ -1624,7 +1630,7 @@ __global__ void k_joint_prep(Dev d) {
     }
     d.J.c0v[j] = pk(c0v, v0);
   }
-  if (kind == JK_REVOLUTE || kind == JK_PRISMATIC) {
+  if (kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_WHEEL) {  // (a wheel: its axle)
     const f3 axa0 = qrot(ra0, xyz(d.J.ax[2 * j])), axb0 = qrot(rb0, xyz(d.J.ax[2 * j + 1]));
     f3 t1, t2;
     tangent_basis(axa0, &t1, &t2);
@@ -2374,6 +2380,54 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
       }
       emit(0, row);
     }
+  } else if (kind == JK_WHEEL) {
+    // Wheel (assembly.rs:521-580): two rows across the live suspension
+    // axis, then the suspension spring along it (penalty m w^2, damping
+    // 2 m zeta w, no dual) or, without a frequency, a rigid row on slot 2
+    const float4 pr = d.J.dref[j];
+    const f3 wa = normalize_or(qrot(qa, xyz(d.J.sus[j])), f3{0.0f, 0.0f, 1.0f});
+    f3 u, v;
+    tangent_basis(wa, &u, &v);
+    const f3 dl = sub(live, mul(c0v, K_ALPHA));
+    const f3 r_side = is_a ? qrot(qa, la) : r_b;
+#pragma unroll
+    for (int li = 0; li < 2; ++li) {
+      const f3 ax = li == 0 ? u : v;
+      float c = dot3(dl, ax);
+      float f = penv[li] * c + lamv[li];
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      make_stamp(row, ax, penv[li], f, r_side, sign);
+      emit(0, row);
+      fl[0] = fl[0] + ax.x * f;
+      fl[1] = fl[1] + ax.y * f;
+      fl[2] = fl[2] + ax.z * f;
+    }
+    const float sv = dot3(sub(pb, pa), wa) - pr.x;
+    const float lsign = is_a ? -1.0f : 1.0f;  // C = (B - A).wa
+    const qf ra0 = asq(d.b.rot0[ab.x]), rb0 = asq(d.b.rot0[ab.y]);
+    const f3 wa0 = normalize_or(qrot(ra0, xyz(d.J.sus[j])), f3{0.0f, 0.0f, 1.0f});
+    const f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(ra0, la)), pb0 = add(xyz(d.b.pos0[ab.y]), qrot(rb0, lb));
+    const float s0 = dot3(sub(pb0, pa0), wa0) - pr.x;
+    if (pr.y > 0.0f) {
+      const float4 pa4 = d.b.pos[ab.x], pb4 = d.b.pos[ab.y];
+      const float ka = d.b.type[ab.x] == BT_DYNAMIC ? pa4.w : 0.0f;
+      const float kb = d.b.type[ab.y] == BT_DYNAMIC ? pb4.w : 0.0f;
+      const float m = ka + kb > K_NEAR_ZERO ? 1.0f / (ka + kb) : 0.0f;
+      if (m > 0.0f) {
+        const float omega = K_TAU * pr.y;
+        const float stiff = m * omega * omega;
+        const float dampc = 2.0f * m * pr.z * omega;
+        make_stamp(row, wa, stiff, stiff * sv + dampc * (sv - s0) / K_DT, r_side, lsign);
+        emit(0, row);
+      }
+    } else {
+      const float c = sv - K_ALPHA * s0;
+      const float f = penv[2] * c + lamv[2];
+      if (fabsf(c) >= K_C_EPS || f != 0.0f) {
+        make_stamp(row, wa, penv[2], f, r_side, lsign);
+        emit(0, row);
+      }
+    }
   } else if (kind != JK_PRISMATIC) {
     if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
 #pragma unroll
@@ -2427,7 +2481,7 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
   row.pen = 0.0f;
   row.f = 0.0f;
   emit(2, row);
-  if (kind == JK_BALL || kind >= JK_DISTANCE) return;
+  if (kind == JK_BALL || is_tether(kind) || kind == JK_MOTOR) return;
   const float4 lama = d.J.lama[j], pena = d.J.pena[j];
   const float lav[3] = {lama.x, lama.y, lama.z}, pav[3] = {pena.x, pena.y, pena.z};
   auto angular = [&](f3 g, float pn, float f) {
@@ -3195,7 +3249,7 @@ __global__ void k_dual_joint(Dev d) {
   f3 c0v = xyz(d.J.c0v[j]);
   float4 lam = d.J.lam[j], pen = d.J.pen[j];
   float lv[3] = {lam.x, lam.y, lam.z}, pv[3] = {pen.x, pen.y, pen.z};
-  if (kind >= JK_DISTANCE) {
+  if (is_tether(kind) || kind == JK_MOTOR) {
     if (kind == JK_SPRING || kind == JK_MOTOR) return;  // no dual state
     const float ref = d.J.dref[j].x;
     const float len = len3(live);
@@ -3218,7 +3272,30 @@ __global__ void k_dual_joint(Dev d) {
     d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
     return;
   }
-  if (kind != JK_PRISMATIC) {
+  if (kind == JK_WHEEL) {
+    // the perp rows, and the rigid row on slot 2 without a suspension
+    // frequency (joints_ext.rs:738-766)
+    const f3 wa = normalize_or(qrot(qa, xyz(d.J.sus[j])), f3{0.0f, 0.0f, 1.0f});
+    f3 u, v;
+    tangent_basis(wa, &u, &v);
+    const f3 dl = sub(live, mul(c0v, K_ALPHA));
+    for (int li = 0; li < 2; ++li) {
+      float c = dot3(dl, li == 0 ? u : v);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[li] += pv[li] * c;
+        pv[li] = fmin_(pv[li] + K_BETA * fabsf(c), K_PEN_MAX);
+      }
+    }
+    const float4 pr = d.J.dref[j];
+    if (!(pr.y > 0.0f)) {
+      const f3 wa0 = normalize_or(qrot(asq(d.b.rot0[ab.x]), xyz(d.J.sus[j])), f3{0.0f, 0.0f, 1.0f});
+      const float c = (-dot3(live, wa) - pr.x) - K_ALPHA * (-dot3(c0v, wa0) - pr.x);
+      if (fabsf(c) >= K_C_EPS) {
+        lv[2] += pv[2] * c;
+        pv[2] = fmin_(pv[2] + K_BETA * fabsf(c), K_LIM_PEN_MAX);
+      }
+    }
+  } else if (kind != JK_PRISMATIC) {
     if (kind == JK_FIXED) live = add(live, qrot(qa, xyz(d.J.dref[j])));
     for (int k = 0; k < 3; ++k) {
       float c = comp(live, k) - K_ALPHA * comp(c0v, k);

@@ -170,6 +170,19 @@ HD void tangent_basis(f3 n, f3* t1, f3* t2) {
 }
 
 // world_inertia (rows.rs:194-207), symmetric 3x3 row-major.
+// orthogonalize_axle (crates/physics/src/math.rs:179-186): the axle less
+// its suspension component, normalised; the suspension's first tangent
+// when that leaves (almost) nothing.
+HD f3 orthogonalize_axle(f3 suspension, f3 axle) {
+  const f3 a = sub(axle, mul(suspension, dot3(axle, suspension)));
+  if (len2(a) < 1e-6f) {
+    f3 t1, t2;
+    tangent_basis(suspension, &t1, &t2);
+    return t1;
+  }
+  return normalize3(a);
+}
+
 HD void world_inertia(f3 inertia, qf rot, float m[3][3]) {
   m3c r = mat3_from_quat(rot);
   for (int a = 0; a < 3; ++a)

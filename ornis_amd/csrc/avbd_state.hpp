@@ -142,7 +142,7 @@ struct Points {
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
 enum {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Wheel 7, Gear 8, SixDof 9 are not built)
+  JK_WHEEL = 7, JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Gear 8, SixDof 9 are not built)
 };
 __host__ __device__ inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // Flags of a Spring joint (the w of its parameter record): the motor's
@@ -160,12 +160,14 @@ struct Joints {
   float4* la;        // local anchor on A
   float4* lb;        // local anchor on B
   float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
-  float4* ax;        // 2 per joint: unit hinge / slide axis in A's and B's frames; Motor: the
+  float4* sus;       // Wheel: the unit suspension axis in A's frame
+  float4* ax;        // 2 per joint: unit hinge / slide axis (Wheel: axle) in A's and B's frames; Motor: the
                      // linear target and force budget, the angular target and torque budget
   float4* dref;      // Fixed / Motor: assembly anchor offset in A's frame; Distance / Rope / Spring:
                      // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags);
-                     // Revolute / Prismatic: (assembly angle | length, limit lo, hi, HINGE_* flags)
-  float4* qref;      // Fixed / Motor: assembly relative rotation conj(qa) qb; Revolute / Prismatic:
+                     // Revolute / Prismatic: (assembly angle | length, limit lo, hi, HINGE_* flags);
+                     // Wheel: (assembly slide, suspension Hz, damping ratio, HINGE_* flags)
+  float4* qref;      // Fixed / Motor: assembly relative rotation conj(qa) qb; Revolute / Prismatic / Wheel:
                      // the motor (target speed, budget, 0, 0)
   float4* lam;       // lam_l xyz; w: a Prismatic limit's force accumulator
   float4* pen;       // pen_l xyz

@@ -75,7 +75,8 @@ struct HostJoint {
   // stiffness, damping, SPRING_* flags; Revolute / Prismatic: assembly
   // angle / length, limit lo, hi, HINGE_* flags
   float prm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  float mot[2] = {0.0f, 0.0f};  // Revolute / Prismatic motor: target speed, budget
+  float mot[2] = {0.0f, 0.0f};  // Revolute / Prismatic / Wheel motor: target speed, budget
+  float sus[3] = {0.0f, 1.0f, 0.0f};  // Wheel: unit suspension axis in A's frame (axa / axb hold the axles)
 };
 
 }  // namespace
@@ -443,6 +444,7 @@ static void free_all(oavbd_ctx* c) {
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.kind); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.ax); dfree(d.J.dref);
+  dfree(d.J.sus);
   dfree(d.J.qref); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.lama); dfree(d.J.pena); dfree(d.J.c0v); dfree(d.J.a0);
   dfree(d.ropes); dfree(d.motors);
   dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
@@ -601,16 +603,17 @@ static void upload_joints(oavbd_ctx* c) {
   }
   Joints& J = d.J;
   dfree(J.ab); dfree(J.kind); dfree(J.la); dfree(J.lb); dfree(J.la0); dfree(J.ax); dfree(J.dref); dfree(J.qref);
+  dfree(J.sus);
   dfree(J.lam); dfree(J.pen); dfree(J.lama); dfree(J.pena); dfree(J.c0v); dfree(J.a0);
   dfree(d.b.jp); dfree(d.b.ji); dfree(d.ropes); dfree(d.motors);
   dalloc(&J.ab, nj); dalloc(&J.kind, nj); dalloc(&J.la, nj); dalloc(&J.lb, nj); dalloc(&J.la0, nj);
-  dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
+  dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.sus, nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
   dalloc(&J.lama, nj); dalloc(&J.pena, nj); dalloc(&J.c0v, nj); dalloc(&J.a0, 2 * (size_t)nj);
   std::vector<uint2> ab(nj);
   std::vector<int> kind(nj);
   std::vector<uint32_t> ropes, motors;
   std::vector<float4> la(nj), lb(nj), la0(nj), ax(2 * (size_t)nj), dref(nj), qref(nj), lam(nj), pen(nj), lama(nj),
-      pena(nj);
+      pena(nj), sus(nj);
   // no-collide partners: every kind but Fixed (rows.rs:437-450); the
   // incidence lists (rows, colouring) hold every joint
   std::vector<uint32_t> jp_cnt(n + 1, 0), ji_cnt(n + 1, 0);
@@ -628,7 +631,8 @@ static void upload_joints(oavbd_ctx* c) {
     const bool motor = h.kind == JK_MOTOR;  // targets and budgets
     ax[2 * j] = make_float4(h.axa[0], h.axa[1], h.axa[2], motor ? h.prm[0] : 0.0f);
     ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], motor ? h.prm[1] : 0.0f);
-    const bool hinge = h.kind == JK_REVOLUTE || h.kind == JK_PRISMATIC;
+    const bool hinge = h.kind == JK_REVOLUTE || h.kind == JK_PRISMATIC || h.kind == JK_WHEEL;
+    sus[j] = make_float4(h.sus[0], h.sus[1], h.sus[2], 0.0f);
     dref[j] = (is_tether(h.kind) || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
                                            : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
     if (h.kind == JK_ROPE) ropes.push_back(j);
@@ -682,6 +686,7 @@ static void upload_joints(oavbd_ctx* c) {
     if (!v.empty()) HIPC(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
   };
   up(J.ab, ab); up(J.kind, kind); up(J.la, la); up(J.lb, lb); up(J.la0, la0); up(J.ax, ax); up(J.dref, dref);
+  up(J.sus, sus);
   up(J.qref, qref); up(J.lam, lam); up(J.pen, pen); up(J.lama, lama); up(J.pena, pena);
   up(d.b.jp_off, jp_off); up(d.b.ji_off, ji_off); up(d.b.jp, jp); up(d.b.ji, ji); up(d.b.comp, comp);
   if (!ropes.empty()) dalloc(&d.ropes, ropes.size());
@@ -2361,7 +2366,7 @@ int oavbd_set_sleep(oavbd_ctx* c, const uint8_t* asleep, const float* timer, uin
 // maximum or a Spring's motor (already validated), as HostJoint::prm.
 static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, const float la[3], const float lb[3],
                           const float axis_a[3], const float axis_b[3], const float prm[4], uint32_t* out_index,
-                          const float mot[2] = nullptr) {
+                          const float mot[2] = nullptr, const float sus[3] = nullptr) {
   const uint32_t n = c->d.n;
   // the reference wakes both members before it validates the spec
   // (mod.rs:1317-1318)
@@ -2395,7 +2400,15 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     j.mot[0] = mot[0];
     j.mot[1] = mot[1];
   }
-  if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_MOTOR) {
+  if (kind == JK_WHEEL) {  // unit axles (already orthogonalised) and suspension
+    for (int k = 0; k < 3; ++k) {
+      j.axa[k] = axis_a[k];
+      j.axb[k] = axis_b[k];
+      j.sus[k] = sus[k];
+    }
+  }
+  if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_MOTOR ||
+      kind == JK_WHEEL) {
     // resolve_joint (joint.rs:1313-1332): the assembly rotation and anchor
     // offset (Fixed) or anchor distance (Distance) from the current poses
     float4 p[2], q[2];
@@ -2416,8 +2429,9 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
       j.prm[0] = len3(sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
     } else if (kind == JK_REVOLUTE) {  // the assembly twist about A's hinge axis
       j.prm[0] = hinge_twist(qa, qb, f3{j.axa[0], j.axa[1], j.axa[2]});
-    } else {  // the assembly slide along A's axis
-      const f3 wa = normalize_or(qrot(qa, f3{j.axa[0], j.axa[1], j.axa[2]}), f3{0.0f, 0.0f, 1.0f});
+    } else {  // the assembly slide along A's axis (a wheel's suspension)
+      const f3 sa = kind == JK_WHEEL ? f3{j.sus[0], j.sus[1], j.sus[2]} : f3{j.axa[0], j.axa[1], j.axa[2]};
+      const f3 wa = normalize_or(qrot(qa, sa), f3{0.0f, 0.0f, 1.0f});
       j.prm[0] = dot3(sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)), wa);
     }
   }
@@ -2479,6 +2493,31 @@ int oavbd_add_motor_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float line
   j.prm[1] = max_torque;
   if (out_index) *out_index = idx;
   return OAVBD_OK;
+}
+
+int oavbd_add_wheel_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                          const float suspension_a[3], const float suspension_b[3], const float axle_a[3],
+                          const float axle_b[3], float frequency_hz, float damping_ratio, const float motor[2],
+                          uint32_t* out_index) {
+  if (!c || !la || !lb || !suspension_a || !suspension_b || !axle_a || !axle_b) return OAVBD_EINVAL;
+  // validate_joint (migration.rs:288-315), before any wake
+  bool ok = std::isfinite(frequency_hz) && frequency_hz >= 0.0f && std::isfinite(damping_ratio) && damping_ratio >= 0.0f;
+  for (int k = 0; k < 3; ++k)
+    ok = ok && std::isfinite(suspension_a[k]) && std::isfinite(suspension_b[k]) && std::isfinite(axle_a[k]) &&
+         std::isfinite(axle_b[k]) && std::isfinite(la[k]) && std::isfinite(lb[k]);
+  if (motor) ok = ok && std::isfinite(motor[0]) && std::isfinite(motor[1]) && motor[1] >= 0.0f;
+  if (!ok) return fail(c, OAVBD_EINVAL, "wheel joint needs finite axes, a frequency and damping >= 0 and a valid motor");
+  // resolve_joint (joint.rs:1352-1380): unit suspensions (fallback +y), the
+  // axles orthogonalised against them
+  const f3 Y{0.0f, 1.0f, 0.0f};
+  const f3 sa = normalize_or(f3{suspension_a[0], suspension_a[1], suspension_a[2]}, Y);
+  const f3 sb = normalize_or(f3{suspension_b[0], suspension_b[1], suspension_b[2]}, Y);
+  const f3 xa = orthogonalize_axle(sa, f3{axle_a[0], axle_a[1], axle_a[2]});
+  const f3 xb = orthogonalize_axle(sb, f3{axle_b[0], axle_b[1], axle_b[2]});
+  // the wheel is stored as a hinge about its axle plus the suspension axis
+  const float prm[4] = {0.0f, frequency_hz, damping_ratio, (float)(motor ? HINGE_MOTOR : 0)};
+  const float axa[3] = {xa.x, xa.y, xa.z}, axb[3] = {xb.x, xb.y, xb.z}, sus[3] = {sa.x, sa.y, sa.z};
+  return add_joint_impl(c, JK_WHEEL, a, b, la, lb, axa, axb, prm, out_index, motor, sus);
 }
 
 int oavbd_add_rope_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],

@@ -28,7 +28,7 @@ _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 
 # joint kinds (include/oavbd.h)
 JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
-JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING = 4, 5, 6
+JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING, JOINT_WHEEL = 4, 5, 6, 7
 JOINT_MOTOR = 10
 # motor kinds and models of a Spring joint (oavbd_motor)
 MOTOR_VELOCITY, MOTOR_POSITION, MOTOR_SERVO = 0, 1, 2
@@ -105,6 +105,16 @@ class SceneBuilder:
         self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_SPRING, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
                             (float(rest), float(stiffness), float(damping), float(max_force), float(target_velocity),
                              float(motor_kind), float(model))))
+        return len(self.joints) - 1
+
+    def wheel(self, a, b, la, lb, suspension_a, suspension_b, axle_a, axle_b, hz, zeta, motor=None):
+        """Wheel joint: a slide along the suspension axis held by a spring of
+        frequency `hz` and damping ratio `zeta` (rigid when hz <= 0), free
+        spin about the axle, an optional axle motor (speed, torque budget)."""
+        sp, mx = motor if motor is not None else (0.0, 0.0)
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_WHEEL, tuple(suspension_a),
+                            tuple(suspension_b), tuple(float(x) for x in tuple(axle_a) + tuple(axle_b)) +
+                            (float(hz), float(zeta), float(sp), float(mx), float(motor is not None))))
         return len(self.joints) - 1
 
     def motor(self, a, b, linear_target, angular_target, max_force, max_torque):

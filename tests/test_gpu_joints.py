@@ -129,7 +129,8 @@ def test_joint_spec_validation(gpu_available):
     from ornis_amd.avbd import AvbdError
     for args in ((0, 0, (0, 0, 0), (0, 0, 0), 1, (0, 0, 1), (0, 0, 1)),       # self joint
                  (0, 5, (0, 0, 0), (0, 0, 0), 0),                          # body out of range
-                 (0, 1, (0, 0, 0), (0, 0, 0), 7),                          # unknown kind
+                 (0, 1, (0, 0, 0), (0, 0, 0), 9),                          # kind not built (SixDof)
+                 (0, 1, (0, 0, 0), (0, 0, 0), 11),                         # unknown kind
                  (0, 1, (0, 0, 0), (0, 0, 0), 1, (0, 0, 0), (0, 0, 1)),    # degenerate hinge axis
                  (0, 1, (0, 0, 0), (0, 0, 0), 2, (0, 0, 1), (1e-7, 0, 0))):  # degenerate slide axis
         with pytest.raises(AvbdError):
@@ -361,3 +362,37 @@ def test_motor_joint_scenes_bitwise(gpu_available):
     with pytest.raises(Exception):
         AvbdEngine(scene=sb.build()).add_joint(1, 2, (0, 0, 0), (0, 0, 0), 10,
                                                prm=(float("nan"), 0, 0, 0, 0, 0, 1.0, 1.0))
+
+
+def test_wheel_scenes_bitwise(gpu_available):
+    """Wheel joints: the reference's sprung chassis and axle motor
+    (avbd_engine.rs:966-1060), a hanging sprung wheel and a rigid one, an
+    oblique axle, and a four-wheeled cart with driven rear axles rolling on
+    a floor into static spheres."""
+    W = scenes.JOINT_WHEEL
+    sb = scenes.SceneBuilder()
+    chassis = sb.box((0.0, 1.0, 0.0), (0.5, 0.2, 0.3), 2.0)
+    wheel = sb.box((0.0, 0.0, 0.0), (0.25, 0.25, 0.15), 1.0)
+    sb.wheel(chassis, wheel, (0, 0, 0), (0, 0, 0), (0, 1, 0), (0, 1, 0), (0, 0, 1), (0, 0, 1), 3.0, 0.7)
+    _run(sb.build(), 300, every=20)
+    for hz, motor, axle in ((2.0, (6.0, 50.0), (0, 0, 1)), (0.0, None, (0, 0, 1)), (2.0, (4.0, 50.0), (0, 1, 1))):
+        sb = scenes.SceneBuilder(gravity=(0.0, -9.81 if motor is None else 0.0, 0.0))
+        anchor = sb.box((0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 0.0)
+        wheel = sb.box((0.0, -0.5, 0.0), (0.2, 0.2, 0.1), 1.0)
+        sb.wheel(anchor, wheel, (0, 0, 0), (0, 0, 0), (0, 1, 0), (0, 1, 0), axle, axle, hz, 0.5, motor=motor)
+        _run(sb.build(), 240, every=20)
+    sb = scenes.SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), friction=0.9, restitution=0.0)
+    for k in range(4):
+        sb.sphere((3.0 + 1.5 * k, 0.3, -1.0 + 0.7 * k), 0.3, 0.0)
+    body = sb.box((0.0, 0.9, 0.0), (1.0, 0.15, 0.5), 4.0)
+    for i, (x, z) in enumerate(((-0.8, -0.6), (-0.8, 0.6), (0.8, -0.6), (0.8, 0.6))):
+        w = sb.sphere((x, 0.35, z), 0.3, 0.8, friction=0.9, restitution=0.0)
+        sb.wheel(body, w, (x, -0.15, z), (0, 0, 0), (0, 1, 0), (0, 1, 0), (0, 0, 1), (0, 0, 1), 4.0, 0.6,
+                 motor=(-6.0, 20.0) if i < 2 else None)
+    g, o = _run(sb.build(), 240, every=10)
+    assert np.isfinite(g.state()).all() and g.stats()["breakdowns"] == 0
+    from ornis_amd.avbd import AvbdError
+    with pytest.raises(AvbdError):
+        g.add_joint(body, 0, (0, 0, 0), (0, 0, 0), W, (0, 1, 0), (0, 1, 0),
+                    prm=(0, 0, 1, 0, 0, 1, -1.0, 0.5, 0.0, 0.0, 0.0))
