@@ -2780,7 +2780,9 @@ __device__ __forceinline__ uint32_t joint_partner_id(const Dev& d, uint32_t e) {
 // PF = 1: the records of a slot's next item are requested before its
 // current item is evaluated (one item of records in flight behind the
 // arithmetic, 48 more registers); PF = 0: only the next descriptor.
-template <int R, int PF, class Wait>
+// JT = false compiles the joint rows out (a context without joints): their
+// code would otherwise set the register budget of every launch.
+template <int R, int PF, bool JT = true, class Wait>
 __device__ __forceinline__ void primal_body(const Dev& d, uint4 cr, bool valid, int slot, Wait&& wait) {
   uint32_t h = 0, i0 = 0, np = 0, j0 = 0, nj = 0;
   if (valid) {
@@ -2848,12 +2850,14 @@ __device__ __forceinline__ void primal_body(const Dev& d, uint4 cr, bool valid, 
     }
   }
   // joint sides continue the item numbering after the point-sides
-  const uint32_t total = np + nj;
-  for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
-       k += BODY_LANES) {
-    const uint32_t e = d.b.ji[j0 + (k - np)];
-    wait(joint_partner_id(d, e));
-    joint_rows(d, e, add_row);
+  if constexpr (JT) {
+    const uint32_t total = np + nj;
+    for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
+         k += BODY_LANES) {
+      const uint32_t e = d.b.ji[j0 + (k - np)];
+      wait(joint_partner_id(d, e));
+      joint_rows(d, e, add_row);
+    }
   }
   float L21[21], rhs[6];
   if constexpr (PF == 0) {
@@ -2894,20 +2898,20 @@ struct NoWait {
 
 // Primal solve of one colour class: W wavefronts per workgroup, four bodies
 // per wavefront.  Lanes past the list take an empty body.
-template <int W, int R, int PF = 0>
+template <int W, int R, int PF = 0, bool JT = true>
 __global__ void __launch_bounds__(WAVE* W) k_primal8(Dev d, const uint4* __restrict__ list, uint32_t count) {
   constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
   const int tid = threadIdx.x;
   const uint32_t idx = blockIdx.x * NB + tid / BODY_LANES;
   const bool valid = idx < count;
   const uint4 cr = valid ? list[idx] : make_uint4(0u, 0u, 0u, 0u);
-  primal_body<R, PF>(d, cr, valid, tid % BODY_LANES, NoWait{});
+  primal_body<R, PF, JT>(d, cr, valid, tid % BODY_LANES, NoWait{});
 }
 
 // The same solve of colour class `k` with the class's range read from the
 // device colour offsets (a step whose colour sizes the host has not read):
 // workgroups loop over the class's body groups, so any grid covers it.
-template <int W, int R, int PF = 0>
+template <int W, int R, int PF = 0, bool JT = true>
 __global__ void __launch_bounds__(WAVE* W) k_primal8_class(Dev d, uint32_t k) {
   constexpr int NB = (WAVE / BODY_LANES) * W;  // bodies per workgroup
   const int tid = threadIdx.x;
@@ -2917,7 +2921,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal8_class(Dev d, uint32_t k) {
     const uint32_t idx = g * NB + tid / BODY_LANES;
     const bool valid = idx < count;
     const uint4 cr = valid ? list[idx] : make_uint4(0u, 0u, 0u, 0u);
-    primal_body<R, PF>(d, cr, valid, tid % BODY_LANES, NoWait{});
+    primal_body<R, PF, JT>(d, cr, valid, tid % BODY_LANES, NoWait{});
   }
 }
 

@@ -1074,21 +1074,32 @@ static void launch_k(oavbd_ctx* c, K kernel, size_t grid, int threads, hipEvent_
   }
 }
 
+// The primal variants compile the joint rows in only for a context that
+// holds joints (JT): the rows' code sets the register budget of a launch.
+template <int W, int PF>
+static void launch_pf_any(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
+  constexpr int NB = (WAVE / BODY_LANES) * W;
+  const uint32_t g = blocks(cnt, NB);
+  const bool jt = c->d.njoints != 0;
+  if (c->rows6) {
+    if (jt)
+      launch_k(c, k_primal8<W, 6, PF, true>, g, WAVE * W, e0, e1, c->d, list, cnt);
+    else
+      launch_k(c, k_primal8<W, 6, PF, false>, g, WAVE * W, e0, e1, c->d, list, cnt);
+  } else {
+    if (jt)
+      launch_k(c, k_primal8<W, 4, PF, true>, g, WAVE * W, e0, e1, c->d, list, cnt);
+    else
+      launch_k(c, k_primal8<W, 4, PF, false>, g, WAVE * W, e0, e1, c->d, list, cnt);
+  }
+}
 template <int W>
 static void launch_w(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
-  constexpr int NB = (WAVE / BODY_LANES) * W;
-  if (c->rows6)
-    launch_k(c, k_primal8<W, 6>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
-  else
-    launch_k(c, k_primal8<W, 4>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
+  launch_pf_any<W, 0>(c, list, cnt, e0, e1);
 }
 template <int W>
 static void launch_pf(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
-  constexpr int NB = (WAVE / BODY_LANES) * W;
-  if (c->rows6)
-    launch_k(c, k_primal8<W, 6, 1>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
-  else
-    launch_k(c, k_primal8<W, 4, 1>, blocks(cnt, NB), WAVE * W, e0, e1, c->d, list, cnt);
+  launch_pf_any<W, 1>(c, list, cnt, e0, e1);
 }
 template <int W>
 static void launch_lds(oavbd_ctx* c, const uint4* list, uint32_t cnt, hipEvent_t e0, hipEvent_t e1) {
@@ -1146,10 +1157,18 @@ template <int W, int PF>
 static void launch_class_w(oavbd_ctx* c, uint32_t k, uint32_t est) {
   constexpr int NB = (WAVE / BODY_LANES) * W;
   const uint32_t grid = std::max<uint32_t>(blocks(est + est / 8u + 1u, NB), 1u);
-  if (c->rows6)
-    k_primal8_class<W, 6, PF><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
-  else
-    k_primal8_class<W, 4, PF><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+  const bool jt = c->d.njoints != 0;
+  if (c->rows6) {
+    if (jt)
+      k_primal8_class<W, 6, PF, true><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+    else
+      k_primal8_class<W, 6, PF, false><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+  } else {
+    if (jt)
+      k_primal8_class<W, 4, PF, true><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+    else
+      k_primal8_class<W, 4, PF, false><<<grid, WAVE * W, 0, c->st>>>(c->d, k);
+  }
 }
 static void launch_primal_class(oavbd_ctx* c, uint32_t k, uint32_t est) {
   int v = c->force_variant;
