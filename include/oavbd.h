@@ -192,9 +192,10 @@ int oavbd_set_contacts(oavbd_ctx* ctx, const oavbd_contact_pair* pairs, uint32_t
 int oavbd_set_sleep(oavbd_ctx* ctx, const uint8_t* asleep, const float* timer, uint32_t n);
 
 /* Joint kinds (the reference's JointKind arms, numbered as its
- * AvbdJointKind; Gear 8 and SixDof 9 are not built).  Rope, Spring, Wheel
- * and Motor are added with oavbd_add_rope_joint / oavbd_add_spring_joint /
- * oavbd_add_wheel_joint / oavbd_add_motor_joint, Revolute and Prismatic
+ * AvbdJointKind; Gear 8 is not built).  Rope, Spring, Wheel, SixDof and
+ * Motor are added with oavbd_add_rope_joint / oavbd_add_spring_joint /
+ * oavbd_add_wheel_joint / oavbd_add_sixdof_joint / oavbd_add_motor_joint,
+ * Revolute and Prismatic
  * with a limit or motor with oavbd_add_joint_limited, the others with
  * oavbd_add_joint. */
 enum {
@@ -206,6 +207,7 @@ enum {
   OAVBD_JOINT_ROPE = 5,
   OAVBD_JOINT_SPRING = 6,
   OAVBD_JOINT_WHEEL = 7,
+  OAVBD_JOINT_SIXDOF = 9,
   OAVBD_JOINT_MOTOR = 10
 };
 
@@ -278,6 +280,21 @@ int oavbd_add_wheel_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la
                           const float suspension_a[3], const float suspension_b[3], const float axle_a[3],
                           const float axle_b[3], float frequency_hz, float damping_ratio, const float motor[2],
                           uint32_t* out_index);
+
+/* SixDof between anchors la, lb: per axis of body A's live frame (axes[0..2]
+ * linear x y z, axes[3..5] angular x y z) OAVBD_AXIS_FREE, OAVBD_AXIS_LOCKED
+ * (an equality row on the drift from the assembly pose) or
+ * OAVBD_AXIS_LIMITED (a one-sided row keeping the travel from the assembly
+ * pose inside [lo[k], hi[k]]: metres along the axis, or radians of twist
+ * about it).  lo / hi may be NULL when no axis is limited.  Its bodies keep
+ * colliding (weld-like, as Fixed).  EINVAL, before any wake, for an
+ * unknown configuration or a limited axis without finite lo <= hi.
+ * Replaces: PhysicsEngine::add_joint with JointKind::SixDof (mod.rs:
+ * 1502-1511; rows assembly.rs:452-519, :682-740; duals joints_ext.rs:
+ * 646-714). */
+enum { OAVBD_AXIS_FREE = 0, OAVBD_AXIS_LOCKED = 1, OAVBD_AXIS_LIMITED = 2 };
+int oavbd_add_sixdof_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                           const uint32_t axes[6], const float lo[6], const float hi[6], uint32_t* out_index);
 
 /* Free motor between a and b: each step begins with budget-clamped
  * velocity impulses (per world axis, through the pair's effective mass at

@@ -176,11 +176,13 @@ struct Pair {
 };
 
 // Joint kinds of this restatement (AvbdJointKind, mod.rs:286-300, same
-// numbering): Gear and SixDof are out of its scope.
+// numbering): Gear is out of its scope.
 enum JointKindC : int {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_WHEEL = 7, JK_MOTOR = 10
+  JK_WHEEL = 7, JK_SIXDOF = 9, JK_MOTOR = 10
 };
+// AxisConfig (joint.rs:676-690) of a SixDof axis.
+enum AxisCfg : int { AX_FREE = 0, AX_LOCKED = 1, AX_LIMITED = 2 };
 inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // JointMotor (joint.rs:962-1126) as a Spring joint carries it: kind
 // (Velocity, Position, Servo), model (AccelerationBased, ForceBased).
@@ -215,6 +217,10 @@ struct Joint {
   // axes) and the suspension's frequency (Hz) and damping ratio
   V3<T> bx_a{T(1), T(0), T(0)}, bx_b{T(1), T(0), T(0)};
   T susp_hz = T(0), susp_zeta = T(0);
+  // SixDof: per axis of A's frame (linear x y z, then angular x y z) its
+  // AxisCfg and window, and the limited axes' force accumulators
+  int six[6] = {AX_FREE, AX_FREE, AX_FREE, AX_FREE, AX_FREE, AX_FREE};
+  T six_lo[6] = {}, six_hi[6] = {}, sacc[6] = {};
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -1055,7 +1061,15 @@ struct Engine {
   // motor failing JointMotor::check (joint.rs:1101-1110), rejects before
   // the bodies wake (validate_joint runs first, mod.rs:1315).
   int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
-    if (kind < JK_BALL || (kind > JK_WHEEL && kind != JK_MOTOR)) return -1;
+    if (kind < JK_BALL || kind == 8 || kind > JK_MOTOR) return -1;
+    if (kind == JK_SIXDOF) {  // validate_joint (migration.rs:188-191): limited windows finite, lo <= hi
+      if (!prm) return -1;
+      for (int k = 0; k < 6; ++k) {
+        if (!(prm[k] == 0.0f || prm[k] == 1.0f || prm[k] == 2.0f)) return -1;
+        if (prm[k] == 2.0f && !(std::isfinite(prm[6 + k]) && std::isfinite(prm[12 + k]) && prm[6 + k] <= prm[12 + k]))
+          return -1;
+      }
+    }
     if (kind == JK_WHEEL) {  // validate_joint (migration.rs:288-315); prm = axle a 3, axle b 3, Hz, zeta,
                              // motor speed, budget, has motor
       if (!prm) return -1;
@@ -1137,6 +1151,17 @@ struct Engine {
       j.mot_speed = T(prm[8]);
       j.mot_max = T(prm[9]);
     }
+    if (kind == JK_SIXDOF) {  // resolve_joint (joint.rs:1381-1391): the assembly relative pose
+      const Body<T>& A = bodies[a];
+      const Body<T>& B = bodies[b];
+      j.q_ref = quat_mul(conj(A.rot), B.rot);
+      j.dref = rotate(conj(A.rot), (B.pos + rotate(B.rot, lb)) - (A.pos + rotate(A.rot, la)));
+      for (int k = 0; k < 6; ++k) {
+        j.six[k] = (int)prm[k];
+        j.six_lo[k] = T(prm[6 + k]);
+        j.six_hi[k] = T(prm[12 + k]);
+      }
+    }
     if (kind == JK_MOTOR) {  // resolve_joint (joint.rs:1393-1399): the assembly relative pose
       const Body<T>& A = bodies[a];
       const Body<T>& B = bodies[b];
@@ -1213,7 +1238,7 @@ struct Engine {
   void index_joints() {
     jointed.clear();
     for (const auto& j : joints)
-      if (j.kind != JK_FIXED) jointed.insert(jkey(j.a, j.b));
+      if (j.kind != JK_FIXED && j.kind != JK_SIXDOF) jointed.insert(jkey(j.a, j.b));
   }
   bool ball_jointed(int ia, int ib) const { return jointed.count(jkey(ia, ib)) != 0; }
 
@@ -1363,6 +1388,23 @@ struct Engine {
         T error = length(perp);
         error = fmax_(error, length(rotate(a.rot, j.bx_a) - rotate(b.rot, j.bx_b)));
         if (j.susp_hz <= T(0)) error = fmax_(error, std::fabs(dot(delta, axis) - j.ref_val));
+        return error;
+      }
+      case JK_SIXDOF: {  // sixdof_rest_error (sleep.rs:86-103)
+        const V3<T> lin = rotate(conj(a.rot), delta) - j.dref;
+        const V3<T> ang = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
+        T error = T(0);
+        for (int i = 0; i < 3; ++i) {
+          const T vals[2] = {lin[i], ang[i]};
+          for (int s2 = 0; s2 < 2; ++s2) {
+            const int k = 3 * s2 + i;
+            const T v = vals[s2];
+            T r = T(0);
+            if (j.six[k] == AX_LOCKED) r = std::fabs(v);
+            if (j.six[k] == AX_LIMITED) r = fmax_(fmax_(j.six_lo[k] - v, v - j.six_hi[k]), T(0));
+            error = fmax_(error, r);
+          }
+        }
         return error;
       }
       case JK_MOTOR:
@@ -2384,6 +2426,36 @@ struct Engine {
             acc.stamp(dir, pen_v, f, r_side, sign);
           }
         }
+      } else if (j.kind == JK_SIXDOF) {
+        // SixDof (assembly.rs:452-519): per axis of A's live frame, a
+        // locked row on the drift from the assembly delta or a one-sided
+        // row on the raw separation inside its window (warmed by sacc)
+        const V3<T> r_side = is_a ? rotate(a.rot, j.la) : rotate(b.rot, j.lb);
+        const T lsign = is_a ? T(-1) : T(1);  // B minus A
+        const V3<T> evec = (pb - pa) - rotate(a.rot, j.dref);
+        for (int i = 0; i < 3; ++i) {
+          V3<T> e{T(0), T(0), T(0)};
+          e[i] = T(1);
+          const V3<T> dir = normalize_or(rotate(a.rot, e), e);
+          if (j.six[i] == AX_LOCKED) {
+            const T c = dot(evec, dir);
+            const T f = j.pen_l[i] * c + j.lam_l[i];
+            if (!row_live(c, f)) continue;
+            acc.stamp(dir, j.pen_l[i], f, r_side, lsign);
+          } else if (j.six[i] == AX_LIMITED) {
+            const T sep = dot(evec, dir);
+            const int st = warm_limit_state(sep, j.six_lo[i], j.six_hi[i], K<T>::LIMIT_SLOP_LIN(), j.sacc[i]);
+            if (st < 0) continue;
+            const bool lower = st == 1;
+            const V3<T> axis0 = normalize_or(rotate(rot0[j.a], e), e);
+            const T initial = dot((pb0 - pa0) - rotate(rot0[j.a], j.dref), axis0);
+            const T c = regularized_limit(sep, initial, lower ? j.six_lo[i] : j.six_hi[i], lower);
+            const T f_raw = j.pen_l[i] * c + j.sacc[i];
+            const T f = lower ? fmin_(f_raw, T(0)) : fmax_(f_raw, T(0));
+            if (!row_live(c, f)) continue;
+            acc.stamp(dir, j.pen_l[i], f, r_side, lsign);
+          }
+        }
       } else if (j.kind == JK_WHEEL) {
         // Wheel (assembly.rs:521-580): the two rows across the live
         // suspension axis, then the suspension spring along it (a fixed
@@ -2484,6 +2556,34 @@ struct Engine {
           if (!row_live(c, f)) continue;
           V3<T> g = is_a ? cross(axa, t) : -cross(axb, t);
           acc.couple(g, j.pen_a[li], f);
+        }
+      } else if (j.kind == JK_SIXDOF) {
+        // angular rows about A's live axes (assembly.rs:682-740): locked on
+        // the relative-rotation error, limited on the twist window
+        const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
+        for (int i = 0; i < 3; ++i) {
+          V3<T> e{T(0), T(0), T(0)};
+          e[i] = T(1);
+          const V3<T> dir = normalize_or(rotate(a.rot, e), e);
+          const int k = 3 + i;
+          T c, f;
+          if (j.six[k] == AX_LOCKED) {
+            c = dot(diff, e);
+            f = j.pen_a[i] * c + j.lam_a[i];
+          } else if (j.six[k] == AX_LIMITED) {
+            const T travel = hinge_twist(a.rot, b.rot, e) - quat_twist(j.q_ref, e);
+            const int st = warm_limit_state(travel, j.six_lo[k], j.six_hi[k], K<T>::LIMIT_SLOP_ANG(), j.sacc[k]);
+            if (st < 0) continue;
+            const bool lower = st == 1;
+            const T initial = hinge_twist(rot0[j.a], rot0[j.b], e) - quat_twist(j.q_ref, e);
+            c = regularized_limit(travel, initial, lower ? j.six_lo[k] : j.six_hi[k], lower);
+            const T f_raw = j.pen_a[i] * c + j.sacc[k];
+            f = lower ? fmin_(f_raw, T(0)) : fmax_(f_raw, T(0));
+          } else {
+            continue;
+          }
+          if (!row_live(c, f)) continue;
+          acc.couple(is_a ? -dir : dir, j.pen_a[i], f);
         }
       } else if (fixed) {
         const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
@@ -2637,6 +2737,38 @@ struct Engine {
         }
       } else if (j.kind == JK_SPRING || j.kind == JK_MOTOR) {
         // no dual state
+      } else if (j.kind == JK_SIXDOF) {
+        // per axis: locked accumulates, limited commits into sacc (zeroed
+        // when clear), penalties ramp capped (joints_ext.rs:646-686)
+        const V3<T> evec = (pb - pa) - rotate(a.rot, j.dref);
+        for (int i = 0; i < 3; ++i) {
+          V3<T> e{T(0), T(0), T(0)};
+          e[i] = T(1);
+          const V3<T> dir = normalize_or(rotate(a.rot, e), e);
+          if (j.six[i] == AX_LOCKED) {
+            const T c = dot(evec, dir);
+            if (std::fabs(c) >= K<T>::C_EPS()) {
+              j.lam_l[i] += j.pen_l[i] * c;
+              j.pen_l[i] = fmin_(j.pen_l[i] + K<T>::BETA() * std::fabs(c), K<T>::PEN_MAX());
+            }
+          } else if (j.six[i] == AX_LIMITED) {
+            const T sep = dot(evec, dir);
+            const int st = warm_limit_state(sep, j.six_lo[i], j.six_hi[i], K<T>::LIMIT_SLOP_LIN(), j.sacc[i]);
+            if (st >= 0) {
+              const bool lower = st == 1;
+              const V3<T> axis0 = normalize_or(rotate(rot0[j.a], e), e);
+              const T initial = dot((pb0 - pa0) - rotate(rot0[j.a], j.dref), axis0);
+              const T c = regularized_limit(sep, initial, lower ? j.six_lo[i] : j.six_hi[i], lower);
+              if (std::fabs(c) >= K<T>::C_EPS()) {
+                const T f = j.pen_l[i] * c + j.sacc[i];
+                j.sacc[i] = lower ? fmin_(f, T(0)) : fmax_(f, T(0));
+                j.pen_l[i] = fmin_(j.pen_l[i] + K<T>::BETA() * std::fabs(c), K<T>::LIM_PEN_MAX());
+              }
+            } else {
+              j.sacc[i] = T(0);
+            }
+          }
+        }
       } else if (j.kind == JK_WHEEL) {
         // the perp rows, and the rigid row on slot 2 when the suspension
         // has no frequency (joints_ext.rs:738-766)
@@ -2695,6 +2827,35 @@ struct Engine {
           if (std::fabs(c) < K<T>::C_EPS()) continue;
           j.lam_a[li] = j.pen_a[li] * c + j.lam_a[li];
           j.pen_a[li] = fmin_(j.pen_a[li] + K<T>::BETA_ANG() * std::fabs(c), K<T>::PEN_MAX());
+        }
+      } else if (j.kind == JK_SIXDOF) {
+        const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
+        for (int i = 0; i < 3; ++i) {
+          V3<T> e{T(0), T(0), T(0)};
+          e[i] = T(1);
+          const int k = 3 + i;
+          if (j.six[k] == AX_LOCKED) {
+            const T c = dot(diff, e);
+            if (std::fabs(c) >= K<T>::C_EPS()) {
+              j.lam_a[i] += j.pen_a[i] * c;
+              j.pen_a[i] = fmin_(j.pen_a[i] + K<T>::BETA_ANG() * std::fabs(c), K<T>::PEN_MAX());
+            }
+          } else if (j.six[k] == AX_LIMITED) {
+            const T travel = hinge_twist(a.rot, b.rot, e) - quat_twist(j.q_ref, e);
+            const int st = warm_limit_state(travel, j.six_lo[k], j.six_hi[k], K<T>::LIMIT_SLOP_ANG(), j.sacc[k]);
+            if (st >= 0) {
+              const bool lower = st == 1;
+              const T initial = hinge_twist(rot0[j.a], rot0[j.b], e) - quat_twist(j.q_ref, e);
+              const T c = regularized_limit(travel, initial, lower ? j.six_lo[k] : j.six_hi[k], lower);
+              if (std::fabs(c) >= K<T>::C_EPS()) {
+                const T f = j.pen_a[i] * c + j.sacc[k];
+                j.sacc[k] = lower ? fmin_(f, T(0)) : fmax_(f, T(0));
+                j.pen_a[i] = fmin_(j.pen_a[i] + K<T>::BETA_ANG() * std::fabs(c), K<T>::LIM_PEN_MAX());
+              }
+            } else {
+              j.sacc[k] = T(0);
+            }
+          }
         }
       } else if (fixed) {
         const V3<T> diff = quat_diff_vec(quat_mul(conj(a.rot), b.rot), j.q_ref, math);
@@ -3747,6 +3908,14 @@ void orc_set_joint_limit_state(void* h, int j, float acc_lim, float lim_dual) {
     E->joints[j].lim_dual = lim_dual;
   });
 }
+// A SixDof joint's limited-axis accumulators (linear xyz, angular xyz).
+void orc_get_joint_sacc(void* h, int j, float* out6) {
+  OracleCtx* c = (OracleCtx*)h;
+  DISPATCH(c, {
+    for (int k = 0; k < 6; ++k) out6[k] = (float)E->joints[j].sacc[k];
+  });
+}
+
 // wake_body plus a zero sleep timer (a test's forced wake each step).
 void orc_wake_body(void* h, int i) {
   OracleCtx* c = (OracleCtx*)h;

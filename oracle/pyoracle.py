@@ -54,6 +54,7 @@ def lib():
         L.orc_set_joint_limit_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_float]
         L.orc_set_gravity.argtypes = [ctypes.c_void_p, _f32p]
         L.orc_wake_body.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_get_joint_sacc.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p]
         L.orc_set_vel.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _f32p]
         L.orc_add_joint_prm.restype = ctypes.c_int
         L.orc_add_joint_prm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p,
@@ -207,6 +208,12 @@ class OracleEngine:
     def set_joint_limit_state(self, j, acc_lim, lim_dual):
         self._L.orc_set_joint_limit_state(self.h, int(j), ctypes.c_float(acc_lim), ctypes.c_float(lim_dual))
 
+    def joint_sacc(self, j):
+        """A SixDof joint's limited-axis accumulators (linear xyz, angular xyz)."""
+        out = np.zeros(6, np.float32)
+        self._L.orc_get_joint_sacc(self.h, int(j), _p(out, _f32p))
+        return out
+
     def wake_body(self, i):
         self._L.orc_wake_body(self.h, int(i))
 
@@ -350,7 +357,7 @@ class OracleEngine:
             r = self._L.orc_add_joint(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p), _p(f32(lb), _f32p),
                                       _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p))
         else:
-            pr = np.zeros(12, np.float32)
+            pr = np.zeros(24, np.float32)
             pr[:len(prm)] = prm
             r = self._L.orc_add_joint_prm(self.h, int(kind), int(a), int(b), _p(f32(la), _f32p),
                                           _p(f32(lb), _f32p), _p(f32(axis_a), _f32p), _p(f32(axis_b), _f32p),

@@ -106,6 +106,9 @@ SIGNATURES = {
     "oavbd_add_wheel_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p, _f32p,
                                              _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, _f32p,
                                              ctypes.POINTER(ctypes.c_uint32)]),
+    "oavbd_add_sixdof_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                              ctypes.POINTER(ctypes.c_uint32), _f32p, _f32p,
+                                              ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_motor_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                              ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
@@ -336,6 +339,15 @@ class AvbdEngine:
                 self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p), _p(sa, _f32p), _p(sb, _f32p),
                 _p(xa_, _f32p), _p(xb_, _f32p), ctypes.c_float(float(prm[6])), ctypes.c_float(float(prm[7])),
                 _p(mot, _f32p) if prm[10] else None, ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
+        if int(kind) == 9 and prm is not None:  # SixDof: prm = 6 axis configurations, lo 6, hi 6
+            cfg = np.ascontiguousarray(np.asarray(prm[0:6]).astype(np.uint32))
+            lo = np.ascontiguousarray(prm[6:12], dtype=np.float32)
+            hi = np.ascontiguousarray(prm[12:18], dtype=np.float32)
+            self._check(self._L.oavbd_add_sixdof_joint(
+                self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
+                cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(lo, _f32p), _p(hi, _f32p), ctypes.byref(idx)))
             self.njoints += 1
             return idx.value
         if int(kind) == 10 and prm is not None:  # Motor: prm = linear target 3, angular target 3, max force, max torque

@@ -1444,6 +1444,23 @@ __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
   } else if (is_tether(kind)) {  // a slack rope is at rest
     const float e = len3(delta) - d.J.dref[j].x;
     err = kind == JK_ROPE ? fmax_(e, 0.0f) : fabsf(e);
+  } else if (kind == JK_SIXDOF) {  // sixdof_rest_error (sleep.rs:86-103)
+    const f3 lin = sub(qrot(qconj(qa), delta), xyz(d.J.dref[j]));
+    const f3 ang = quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]));
+    const float4 w0 = d.J.six[4 * j], w1 = d.J.six[4 * j + 1], w2 = d.J.six[4 * j + 2], w3 = d.J.six[4 * j + 3];
+    err = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+      for (int s2 = 0; s2 < 2; ++s2) {
+        const int k = 3 * s2 + i;
+        const float v = s2 ? comp(ang, i) : comp(lin, i);
+        const float lo = s2 ? comp(xyz(w2), i) : comp(xyz(w0), i), hi = s2 ? comp(xyz(w3), i) : comp(xyz(w1), i);
+        const int cfg = six_cfg(w0.w, k);
+        float r = 0.0f;
+        if (cfg == AX_LOCKED) r = fabsf(v);
+        if (cfg == AX_LIMITED) r = fmax_(fmax_(lo - v, v - hi), 0.0f);
+        err = fmax_(err, r);
+      }
+    }
   } else if (kind == JK_WHEEL) {  // joint_rest_error (sleep.rs:61-76)
     const f3 axis = qrot(qa, xyz(d.J.sus[j]));
     const float4 pr = d.J.dref[j];
@@ -2310,6 +2327,19 @@ __device__ __forceinline__ void point_rows(const Dev& d, uint4 it, const float4 
   item_rows<R>(d, it, r, own, emit);
 }
 
+// A SixDof joint's axis k of A's step-start or live frame (linear: the
+// drift from the assembly delta along it; angular: the twist about it
+// less the assembly twist), and its window (assembly.rs:452-519, :682-740).
+__device__ __forceinline__ float six_lin_value(f3 pa, f3 pb, qf qa, f3 dref, f3 dir) {
+  return dot3(sub(sub(pb, pa), qrot(qa, dref)), dir);
+}
+__device__ __forceinline__ float six_ang_value(qf qa, qf qb, qf qref, f3 e) {
+  return hinge_twist(qa, qb, e) - quat_twist(qref, e);
+}
+__device__ __forceinline__ f3 unit_axis(int i) {
+  return f3{i == 0 ? 1.0f : 0.0f, i == 1 ? 1.0f : 0.0f, i == 2 ? 1.0f : 0.0f};
+}
+
 // Rows of one joint side, in the reference's order (assembly.rs:253-818):
 // the linear rows (Ball / Revolute / Fixed: three world-axis rows at the
 // anchors, Fixed offset by the assembly delta; Prismatic: two rows across
@@ -2378,6 +2408,42 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
       } else {  // a pure damper
         make_stamp(row, dir, dampc / K_DT, dampc * (sp - s0) / K_DT, r_side, sign);
       }
+      emit(0, row);
+    }
+  } else if (kind == JK_SIXDOF) {
+    // SixDof (assembly.rs:452-519): per axis of A's live frame a locked row
+    // on the drift from the assembly delta, or a one-sided row inside its
+    // window warmed by the accumulator; B minus A, so the gradient flips
+    const float4 w0 = d.J.six[4 * j], w1 = d.J.six[4 * j + 1];
+    const float4 s0 = d.J.sacc[2 * j];
+    const f3 dr = xyz(d.J.dref[j]);
+    const f3 r_side = is_a ? qrot(qa, la) : r_b;
+    const float lsign = is_a ? -1.0f : 1.0f;
+    const float lo3[3] = {w0.x, w0.y, w0.z}, hi3[3] = {w1.x, w1.y, w1.z}, sa3[3] = {s0.x, s0.y, s0.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int cfg = six_cfg(w0.w, i);
+      if (cfg == AX_FREE) continue;
+      const f3 e = unit_axis(i);
+      const f3 dir = normalize_or(qrot(qa, e), e);
+      const float v = six_lin_value(pa, pb, qa, dr, dir);
+      float c, f;
+      if (cfg == AX_LOCKED) {
+        c = v;
+        f = penv[i] * c + lamv[i];
+      } else {
+        const int st = warm_limit_state(v, lo3[i], hi3[i], K_LIMIT_SLOP_LIN, sa3[i]);
+        if (st < 0) continue;
+        const bool lower = st == 1;
+        const qf ra0 = asq(d.b.rot0[ab.x]), rb0 = asq(d.b.rot0[ab.y]);
+        const f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(ra0, la)), pb0 = add(xyz(d.b.pos0[ab.y]), qrot(rb0, lb));
+        const float initial = six_lin_value(pa0, pb0, ra0, dr, normalize_or(qrot(ra0, e), e));
+        c = regularized_limit(v, initial, lower ? lo3[i] : hi3[i], lower);
+        const float f_raw = penv[i] * c + sa3[i];
+        f = lower ? fmin_(f_raw, 0.0f) : fmax_(f_raw, 0.0f);
+      }
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      make_stamp(row, dir, penv[i], f, r_side, lsign);
       emit(0, row);
     }
   } else if (kind == JK_WHEEL) {
@@ -2491,6 +2557,39 @@ __device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit
     row.f = f;
     emit(1, row);
   };
+  if (kind == JK_SIXDOF) {
+    // angular rows about A's live axes (assembly.rs:682-740): locked on the
+    // relative-rotation error, limited on the twist window
+    const float4 w0 = d.J.six[4 * j], w2 = d.J.six[4 * j + 2], w3 = d.J.six[4 * j + 3];
+    const float4 s1 = d.J.sacc[2 * j + 1];
+    const qf qr = asq(d.J.qref[j]);
+    const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), qr);
+    const float lo3[3] = {w2.x, w2.y, w2.z}, hi3[3] = {w3.x, w3.y, w3.z}, sa3[3] = {s1.x, s1.y, s1.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int cfg = six_cfg(w0.w, 3 + i);
+      if (cfg == AX_FREE) continue;
+      const f3 e = unit_axis(i);
+      const f3 dir = normalize_or(qrot(qa, e), e);
+      float c, f;
+      if (cfg == AX_LOCKED) {
+        c = dot3(diff, e);
+        f = pav[i] * c + lav[i];
+      } else {
+        const float v = six_ang_value(qa, qb, qr, e);
+        const int st = warm_limit_state(v, lo3[i], hi3[i], K_LIMIT_SLOP_ANG, sa3[i]);
+        if (st < 0) continue;
+        const bool lower = st == 1;
+        const float initial = six_ang_value(asq(d.b.rot0[ab.x]), asq(d.b.rot0[ab.y]), qr, e);
+        c = regularized_limit(v, initial, lower ? lo3[i] : hi3[i], lower);
+        const float f_raw = pav[i] * c + sa3[i];
+        f = lower ? fmin_(f_raw, 0.0f) : fmax_(f_raw, 0.0f);
+      }
+      if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) continue;
+      angular(is_a ? neg(dir) : dir, pav[i], f);
+    }
+    return;
+  }
   if (kind == JK_FIXED) {
     const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]));
     const f3 diff0 = xyz(d.J.a0[2 * j]);
@@ -3276,7 +3375,48 @@ __global__ void k_dual_joint(Dev d) {
     d.J.pen[j] = make_float4(pv[0], pv[1], pv[2], 0.0f);
     return;
   }
-  if (kind == JK_WHEEL) {
+  float4 sacc0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sacc1 = sacc0;
+  if (kind == JK_SIXDOF) {
+    // per axis: locked accumulates, limited commits into its accumulator
+    // (zeroed when clear), penalties ramp capped (joints_ext.rs:646-686)
+    sacc0 = d.J.sacc[2 * j];
+    sacc1 = d.J.sacc[2 * j + 1];
+    const float4 w0 = d.J.six[4 * j], w1 = d.J.six[4 * j + 1];
+    const f3 dr = xyz(d.J.dref[j]);
+    float sa3[3] = {sacc0.x, sacc0.y, sacc0.z};
+    const float lo3[3] = {w0.x, w0.y, w0.z}, hi3[3] = {w1.x, w1.y, w1.z};
+    for (int i = 0; i < 3; ++i) {
+      const int cfg = six_cfg(w0.w, i);
+      if (cfg == AX_FREE) continue;
+      const f3 e = unit_axis(i);
+      const f3 dir = normalize_or(qrot(qa, e), e);
+      const float v = six_lin_value(pa, pb, qa, dr, dir);
+      if (cfg == AX_LOCKED) {
+        if (fabsf(v) >= K_C_EPS) {
+          lv[i] += pv[i] * v;
+          pv[i] = fmin_(pv[i] + K_BETA * fabsf(v), K_PEN_MAX);
+        }
+        continue;
+      }
+      const int st = warm_limit_state(v, lo3[i], hi3[i], K_LIMIT_SLOP_LIN, sa3[i]);
+      if (st < 0) {
+        sa3[i] = 0.0f;
+        continue;
+      }
+      const bool lower = st == 1;
+      const qf ra0 = asq(d.b.rot0[ab.x]), rb0 = asq(d.b.rot0[ab.y]);
+      const f3 pa0 = add(xyz(d.b.pos0[ab.x]), qrot(ra0, xyz(d.J.la[j])));
+      const f3 pb0 = add(xyz(d.b.pos0[ab.y]), qrot(rb0, xyz(d.J.lb[j])));
+      const float initial = six_lin_value(pa0, pb0, ra0, dr, normalize_or(qrot(ra0, e), e));
+      const float c = regularized_limit(v, initial, lower ? lo3[i] : hi3[i], lower);
+      if (fabsf(c) >= K_C_EPS) {
+        const float f = pv[i] * c + sa3[i];
+        sa3[i] = lower ? fmin_(f, 0.0f) : fmax_(f, 0.0f);
+        pv[i] = fmin_(pv[i] + K_BETA * fabsf(c), K_LIM_PEN_MAX);
+      }
+    }
+    sacc0 = make_float4(sa3[0], sa3[1], sa3[2], 0.0f);
+  } else if (kind == JK_WHEEL) {
     // the perp rows, and the rigid row on slot 2 without a suspension
     // frequency (joints_ext.rs:738-766)
     const f3 wa = normalize_or(qrot(qa, xyz(d.J.sus[j])), f3{0.0f, 0.0f, 1.0f});
@@ -3328,7 +3468,42 @@ __global__ void k_dual_joint(Dev d) {
   }
   float4 la4 = d.J.lama[j], pa4 = d.J.pena[j];
   float av[3] = {la4.x, la4.y, la4.z}, qv[3] = {pa4.x, pa4.y, pa4.z};
-  if (kind == JK_FIXED) {
+  if (kind == JK_SIXDOF) {
+    const float4 w0 = d.J.six[4 * j], w2 = d.J.six[4 * j + 2], w3 = d.J.six[4 * j + 3];
+    const qf qr = asq(d.J.qref[j]);
+    const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), qr);
+    float sa3[3] = {sacc1.x, sacc1.y, sacc1.z};
+    const float lo3[3] = {w2.x, w2.y, w2.z}, hi3[3] = {w3.x, w3.y, w3.z};
+    for (int i = 0; i < 3; ++i) {
+      const int cfg = six_cfg(w0.w, 3 + i);
+      if (cfg == AX_FREE) continue;
+      const f3 e = unit_axis(i);
+      if (cfg == AX_LOCKED) {
+        const float c = dot3(diff, e);
+        if (fabsf(c) >= K_C_EPS) {
+          av[i] += qv[i] * c;
+          qv[i] = fmin_(qv[i] + K_BETA_ANG * fabsf(c), K_PEN_MAX);
+        }
+        continue;
+      }
+      const float v = six_ang_value(qa, qb, qr, e);
+      const int st = warm_limit_state(v, lo3[i], hi3[i], K_LIMIT_SLOP_ANG, sa3[i]);
+      if (st < 0) {
+        sa3[i] = 0.0f;
+        continue;
+      }
+      const bool lower = st == 1;
+      const float initial = six_ang_value(asq(d.b.rot0[ab.x]), asq(d.b.rot0[ab.y]), qr, e);
+      const float c = regularized_limit(v, initial, lower ? lo3[i] : hi3[i], lower);
+      if (fabsf(c) >= K_C_EPS) {
+        const float f = qv[i] * c + sa3[i];
+        sa3[i] = lower ? fmin_(f, 0.0f) : fmax_(f, 0.0f);
+        qv[i] = fmin_(qv[i] + K_BETA_ANG * fabsf(c), K_LIM_PEN_MAX);
+      }
+    }
+    d.J.sacc[2 * j] = sacc0;
+    d.J.sacc[2 * j + 1] = make_float4(sa3[0], sa3[1], sa3[2], 0.0f);
+  } else if (kind == JK_FIXED) {
     const f3 diff = quat_diff_vec(qmul(qconj(qa), qb), asq(d.J.qref[j]));
     const f3 diff0 = xyz(d.J.a0[2 * j]);
     for (int k = 0; k < 3; ++k) {

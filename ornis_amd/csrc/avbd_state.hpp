@@ -142,9 +142,13 @@ struct Points {
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
 enum {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_WHEEL = 7, JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Gear 8, SixDof 9 are not built)
+  JK_WHEEL = 7, JK_SIXDOF = 9, JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Gear 8 is not built)
 };
 __host__ __device__ inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
+// SixDof axis configurations (AxisConfig), two bits per axis (linear x y z,
+// angular x y z) in the w of the joint's first six record
+enum { AX_FREE = 0, AX_LOCKED = 1, AX_LIMITED = 2 };
+__host__ __device__ inline int six_cfg(float packed, int k) { return ((uint32_t)packed >> (2 * k)) & 3; }
 // Flags of a Spring joint (the w of its parameter record): the motor's
 // force-based model; the motor keeps its bodies awake (JointMotor::keeps_awake)
 enum { SPRING_FORCE_MODEL = 1, SPRING_KEEPS_AWAKE = 2 };
@@ -161,13 +165,15 @@ struct Joints {
   float4* lb;        // local anchor on B
   float4* la0;       // A's lever as the reference stamps it: la + 0 (Fixed: la + dref)
   float4* sus;       // Wheel: the unit suspension axis in A's frame
+  float4* six;       // SixDof, 4 per joint: linear lo (w: packed AX_* per axis), linear hi, angular lo, hi
+  float4* sacc;      // SixDof, 2 per joint: the limited axes' force accumulators (linear, angular)
   float4* ax;        // 2 per joint: unit hinge / slide axis (Wheel: axle) in A's and B's frames; Motor: the
                      // linear target and force budget, the angular target and torque budget
-  float4* dref;      // Fixed / Motor: assembly anchor offset in A's frame; Distance / Rope / Spring:
+  float4* dref;      // Fixed / Motor / SixDof: assembly anchor offset in A's frame; Distance / Rope / Spring:
                      // (rest distance | maximum | rest length, stiffness, damping, SPRING_* flags);
                      // Revolute / Prismatic: (assembly angle | length, limit lo, hi, HINGE_* flags);
                      // Wheel: (assembly slide, suspension Hz, damping ratio, HINGE_* flags)
-  float4* qref;      // Fixed / Motor: assembly relative rotation conj(qa) qb; Revolute / Prismatic / Wheel:
+  float4* qref;      // Fixed / Motor / SixDof: assembly relative rotation conj(qa) qb; Revolute / Prismatic / Wheel:
                      // the motor (target speed, budget, 0, 0)
   float4* lam;       // lam_l xyz; w: a Prismatic limit's force accumulator
   float4* pen;       // pen_l xyz

@@ -77,6 +77,9 @@ struct HostJoint {
   float prm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   float mot[2] = {0.0f, 0.0f};  // Revolute / Prismatic / Wheel motor: target speed, budget
   float sus[3] = {0.0f, 1.0f, 0.0f};  // Wheel: unit suspension axis in A's frame (axa / axb hold the axles)
+  // SixDof: AX_* per axis (linear x y z, angular x y z) and the windows
+  int six[6] = {0, 0, 0, 0, 0, 0};
+  float six_lo[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, six_hi[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 };
 
 }  // namespace
@@ -444,7 +447,7 @@ static void free_all(oavbd_ctx* c) {
   Dev& d = c->d;
   free_pairs(d.P); free_pairs(d.Pold); free_points(d.Q); free_points(d.Qold);
   dfree(d.J.ab); dfree(d.J.kind); dfree(d.J.la); dfree(d.J.lb); dfree(d.J.la0); dfree(d.J.ax); dfree(d.J.dref);
-  dfree(d.J.sus);
+  dfree(d.J.sus); dfree(d.J.six); dfree(d.J.sacc);
   dfree(d.J.qref); dfree(d.J.lam); dfree(d.J.pen); dfree(d.J.lama); dfree(d.J.pena); dfree(d.J.c0v); dfree(d.J.a0);
   dfree(d.ropes); dfree(d.motors);
   dfree(d.g.cnt); dfree(d.g.start); dfree(d.g.cursor); dfree(d.g.items); dfree(d.g.ipos); dfree(d.g.ikey); dfree(d.g.hkey); dfree(d.bp_cache); dfree(d.toi_cache); dfree(d.g.bucket); dfree(d.g.large); dfree(d.g.lsc_off); dfree(d.g.lsc); dfree(d.g.sitems); dfree(d.g.soff); dfree(d.g.lrange);
@@ -593,27 +596,28 @@ static void upload_joints(oavbd_ctx* c) {
   // joints already on the device keep their duals (a new joint does not
   // reset the others' warm starts: add_joint only pushes, mod.rs:1308-1510)
   const uint32_t kept = std::min(d.njoints, nj);
-  std::vector<float4> kept_lam(kept), kept_pen(kept), kept_lama(kept), kept_pena(kept);
+  std::vector<float4> kept_lam(kept), kept_pen(kept), kept_lama(kept), kept_pena(kept), kept_sacc(2 * (size_t)kept);
   if (kept) {
     HIPC(hipStreamSynchronize(c->st));
     HIPC(hipMemcpy(kept_lam.data(), d.J.lam, kept * sizeof(float4), hipMemcpyDeviceToHost));
     HIPC(hipMemcpy(kept_pen.data(), d.J.pen, kept * sizeof(float4), hipMemcpyDeviceToHost));
     HIPC(hipMemcpy(kept_lama.data(), d.J.lama, kept * sizeof(float4), hipMemcpyDeviceToHost));
     HIPC(hipMemcpy(kept_pena.data(), d.J.pena, kept * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(kept_sacc.data(), d.J.sacc, 2 * kept * sizeof(float4), hipMemcpyDeviceToHost));
   }
   Joints& J = d.J;
   dfree(J.ab); dfree(J.kind); dfree(J.la); dfree(J.lb); dfree(J.la0); dfree(J.ax); dfree(J.dref); dfree(J.qref);
-  dfree(J.sus);
+  dfree(J.sus); dfree(J.six); dfree(J.sacc);
   dfree(J.lam); dfree(J.pen); dfree(J.lama); dfree(J.pena); dfree(J.c0v); dfree(J.a0);
   dfree(d.b.jp); dfree(d.b.ji); dfree(d.ropes); dfree(d.motors);
   dalloc(&J.ab, nj); dalloc(&J.kind, nj); dalloc(&J.la, nj); dalloc(&J.lb, nj); dalloc(&J.la0, nj);
-  dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.sus, nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
+  dalloc(&J.ax, 2 * (size_t)nj); dalloc(&J.sus, nj); dalloc(&J.six, 4 * (size_t)nj); dalloc(&J.sacc, 2 * (size_t)nj); dalloc(&J.dref, nj); dalloc(&J.qref, nj); dalloc(&J.lam, nj); dalloc(&J.pen, nj);
   dalloc(&J.lama, nj); dalloc(&J.pena, nj); dalloc(&J.c0v, nj); dalloc(&J.a0, 2 * (size_t)nj);
   std::vector<uint2> ab(nj);
   std::vector<int> kind(nj);
   std::vector<uint32_t> ropes, motors;
   std::vector<float4> la(nj), lb(nj), la0(nj), ax(2 * (size_t)nj), dref(nj), qref(nj), lam(nj), pen(nj), lama(nj),
-      pena(nj), sus(nj);
+      pena(nj), sus(nj), six(4 * (size_t)nj), sacc(2 * (size_t)nj);
   // no-collide partners: every kind but Fixed (rows.rs:437-450); the
   // incidence lists (rows, colouring) hold every joint
   std::vector<uint32_t> jp_cnt(n + 1, 0), ji_cnt(n + 1, 0);
@@ -633,6 +637,14 @@ static void upload_joints(oavbd_ctx* c) {
     ax[2 * j + 1] = make_float4(h.axb[0], h.axb[1], h.axb[2], motor ? h.prm[1] : 0.0f);
     const bool hinge = h.kind == JK_REVOLUTE || h.kind == JK_PRISMATIC || h.kind == JK_WHEEL;
     sus[j] = make_float4(h.sus[0], h.sus[1], h.sus[2], 0.0f);
+    uint32_t packed = 0;
+    for (int k = 0; k < 6; ++k) packed |= (uint32_t)h.six[k] << (2 * k);
+    six[4 * j] = make_float4(h.six_lo[0], h.six_lo[1], h.six_lo[2], (float)packed);
+    six[4 * j + 1] = make_float4(h.six_hi[0], h.six_hi[1], h.six_hi[2], 0.0f);
+    six[4 * j + 2] = make_float4(h.six_lo[3], h.six_lo[4], h.six_lo[5], 0.0f);
+    six[4 * j + 3] = make_float4(h.six_hi[3], h.six_hi[4], h.six_hi[5], 0.0f);
+    sacc[2 * j] = j < kept ? kept_sacc[2 * j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    sacc[2 * j + 1] = j < kept ? kept_sacc[2 * j + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     dref[j] = (is_tether(h.kind) || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
                                            : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
     if (h.kind == JK_ROPE) ropes.push_back(j);
@@ -642,7 +654,7 @@ static void upload_joints(oavbd_ctx* c) {
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     lama[j] = j < kept ? kept_lama[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pena[j] = j < kept ? kept_pena[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    if (!fixed) {
+    if (!fixed && h.kind != JK_SIXDOF) {  // weld-like: their contacts stay
       jp_cnt[h.a]++;
       jp_cnt[h.b]++;
     }
@@ -657,7 +669,7 @@ static void upload_joints(oavbd_ctx* c) {
   std::vector<uint32_t> jp(jp_off[n]), ji(ji_off[n]), pcur(n, 0), icur(n, 0);
   for (uint32_t j = 0; j < nj; ++j) {  // joint order per body = ascending joint index
     const HostJoint& h = c->joints[j];
-    if (h.kind != JK_FIXED) {
+    if (h.kind != JK_FIXED && h.kind != JK_SIXDOF) {
       jp[jp_off[h.a] + pcur[h.a]++] = h.b;
       jp[jp_off[h.b] + pcur[h.b]++] = h.a;
     }
@@ -686,7 +698,7 @@ static void upload_joints(oavbd_ctx* c) {
     if (!v.empty()) HIPC(hipMemcpy(dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
   };
   up(J.ab, ab); up(J.kind, kind); up(J.la, la); up(J.lb, lb); up(J.la0, la0); up(J.ax, ax); up(J.dref, dref);
-  up(J.sus, sus);
+  up(J.sus, sus); up(J.six, six); up(J.sacc, sacc);
   up(J.qref, qref); up(J.lam, lam); up(J.pen, pen); up(J.lama, lama); up(J.pena, pena);
   up(d.b.jp_off, jp_off); up(d.b.ji_off, ji_off); up(d.b.jp, jp); up(d.b.ji, ji); up(d.b.comp, comp);
   if (!ropes.empty()) dalloc(&d.ropes, ropes.size());
@@ -2427,7 +2439,7 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     }
   }
   if (kind == JK_FIXED || kind == JK_DISTANCE || kind == JK_REVOLUTE || kind == JK_PRISMATIC || kind == JK_MOTOR ||
-      kind == JK_WHEEL) {
+      kind == JK_WHEEL || kind == JK_SIXDOF) {
     // resolve_joint (joint.rs:1313-1332): the assembly rotation and anchor
     // offset (Fixed) or anchor distance (Distance) from the current poses
     float4 p[2], q[2];
@@ -2441,7 +2453,7 @@ static int add_joint_impl(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b, c
     const qf qr = qmul(qconj(qa), qb);
     const f3 ra = qrot(qa, f3{la[0], la[1], la[2]}), rb = qrot(qb, f3{lb[0], lb[1], lb[2]});
     const f3 dr = qrot(qconj(qa), sub(add(xyz(p[1]), rb), add(xyz(p[0]), ra)));
-    if (kind == JK_FIXED || kind == JK_MOTOR) {  // Motor: la = lb = 0, so dr = conj(qa) (pb - pa)
+    if (kind == JK_FIXED || kind == JK_MOTOR || kind == JK_SIXDOF) {  // Motor: la = lb = 0, so dr = conj(qa) (pb - pa)
       j.qref[0] = qr.x; j.qref[1] = qr.y; j.qref[2] = qr.z; j.qref[3] = qr.w;
       j.dref[0] = dr.x; j.dref[1] = dr.y; j.dref[2] = dr.z;
     } else if (kind == JK_DISTANCE) {
@@ -2537,6 +2549,29 @@ int oavbd_add_wheel_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3
   const float prm[4] = {0.0f, frequency_hz, damping_ratio, (float)(motor ? HINGE_MOTOR : 0)};
   const float axa[3] = {xa.x, xa.y, xa.z}, axb[3] = {xb.x, xb.y, xb.z}, sus[3] = {sa.x, sa.y, sa.z};
   return add_joint_impl(c, JK_WHEEL, a, b, la, lb, axa, axb, prm, out_index, motor, sus);
+}
+
+int oavbd_add_sixdof_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],
+                           const uint32_t axes[6], const float lo[6], const float hi[6], uint32_t* out_index) {
+  if (!c || !la || !lb || !axes) return OAVBD_EINVAL;
+  // validate_joint (migration.rs:188-191), before any wake
+  for (int k = 0; k < 6; ++k) {
+    if (axes[k] > (uint32_t)AX_LIMITED) return fail(c, OAVBD_EINVAL, "bad SixDof axis configuration");
+    if (axes[k] == (uint32_t)AX_LIMITED &&
+        !(lo && hi && std::isfinite(lo[k]) && std::isfinite(hi[k]) && lo[k] <= hi[k]))
+      return fail(c, OAVBD_EINVAL, "SixDof window needs finite lo <= hi");
+  }
+  uint32_t idx = 0;
+  const int rc = add_joint_impl(c, JK_SIXDOF, a, b, la, lb, nullptr, nullptr, nullptr, &idx);
+  if (rc != OAVBD_OK) return rc;
+  HostJoint& j = c->joints[idx];
+  for (int k = 0; k < 6; ++k) {
+    j.six[k] = (int)axes[k];
+    j.six_lo[k] = axes[k] == (uint32_t)AX_LIMITED ? lo[k] : 0.0f;
+    j.six_hi[k] = axes[k] == (uint32_t)AX_LIMITED ? hi[k] : 0.0f;
+  }
+  if (out_index) *out_index = idx;
+  return OAVBD_OK;
 }
 
 int oavbd_add_rope_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float la[3], const float lb[3],

@@ -29,7 +29,9 @@ _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 # joint kinds (include/oavbd.h)
 JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
 JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING, JOINT_WHEEL = 4, 5, 6, 7
-JOINT_MOTOR = 10
+JOINT_SIXDOF, JOINT_MOTOR = 9, 10
+# SixDof axis configurations
+AXIS_FREE, AXIS_LOCKED, AXIS_LIMITED = 0, 1, 2
 # motor kinds and models of a Spring joint (oavbd_motor)
 MOTOR_VELOCITY, MOTOR_POSITION, MOTOR_SERVO = 0, 1, 2
 MOTOR_ACCELERATION_BASED, MOTOR_FORCE_BASED = 0, 1
@@ -115,6 +117,15 @@ class SceneBuilder:
         self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_WHEEL, tuple(suspension_a),
                             tuple(suspension_b), tuple(float(x) for x in tuple(axle_a) + tuple(axle_b)) +
                             (float(hz), float(zeta), float(sp), float(mx), float(motor is not None))))
+        return len(self.joints) - 1
+
+    def sixdof(self, a, b, la, lb, kinds, lo=(0.0,) * 6, hi=(0.0,) * 6):
+        """SixDof joint: per axis of A's frame (linear x, y, z then angular
+        x, y, z) AXIS_FREE / AXIS_LOCKED / AXIS_LIMITED with window [lo, hi]
+        from the assembly pose."""
+        self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_SIXDOF, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
+                            tuple(float(k) for k in kinds) + tuple(float(x) for x in lo) +
+                            tuple(float(x) for x in hi)))
         return len(self.joints) - 1
 
     def motor(self, a, b, linear_target, angular_target, max_force, max_torque):

@@ -129,7 +129,7 @@ def test_joint_spec_validation(gpu_available):
     from ornis_amd.avbd import AvbdError
     for args in ((0, 0, (0, 0, 0), (0, 0, 0), 1, (0, 0, 1), (0, 0, 1)),       # self joint
                  (0, 5, (0, 0, 0), (0, 0, 0), 0),                          # body out of range
-                 (0, 1, (0, 0, 0), (0, 0, 0), 9),                          # kind not built (SixDof)
+                 (0, 1, (0, 0, 0), (0, 0, 0), 8),                          # kind not built (Gear)
                  (0, 1, (0, 0, 0), (0, 0, 0), 11),                         # unknown kind
                  (0, 1, (0, 0, 0), (0, 0, 0), 1, (0, 0, 0), (0, 0, 1)),    # degenerate hinge axis
                  (0, 1, (0, 0, 0), (0, 0, 0), 2, (0, 0, 1), (1e-7, 0, 0))):  # degenerate slide axis
@@ -396,3 +396,45 @@ def test_wheel_scenes_bitwise(gpu_available):
     with pytest.raises(AvbdError):
         g.add_joint(body, 0, (0, 0, 0), (0, 0, 0), W, (0, 1, 0), (0, 1, 0),
                     prm=(0, 0, 1, 0, 0, 1, -1.0, 0.5, 0.0, 0.0, 0.0))
+
+
+def test_sixdof_scenes_bitwise(gpu_available):
+    """SixDof joints: the reference's angular window against an 8 rad/s spin
+    (avbd_engine.rs:1514-1546), a locked weld dropped on a floor, a free one
+    whose boxes keep colliding, and chains of links mixing free, locked and
+    limited linear and angular axes tumbling onto spheres."""
+    F_, L_, M_ = scenes.AXIS_FREE, scenes.AXIS_LOCKED, scenes.AXIS_LIMITED
+    sb = scenes.SceneBuilder(gravity=(0.0, 0.0, 0.0))
+    a = sb.box((0.0, 0.0, 0.0), (0.5, 0.5, 0.5), 0.0)
+    b = sb.box((0.0, 0.0, 0.0), (0.4, 0.4, 0.4), 1.0, angvel=(0.0, 0.0, 8.0))
+    sb.sixdof(a, b, (0, 0, 0), (0, 0, 0), [L_] * 5 + [M_], (0,) * 5 + (-0.2,), (0,) * 5 + (0.2,))
+    _run(sb.build(), 120, every=10)
+    for kinds in ([L_] * 6, [F_] * 6):
+        sb = scenes.SceneBuilder()
+        sb.box((0.0, -1.0, 0.0), (5.0, 1.0, 5.0), 0.0)
+        a = sb.box((-0.5, 3.0, 0.0), (0.5, 0.5, 0.5), 1.0)
+        c = sb.box((0.45, 3.0, 0.0), (0.5, 0.5, 0.5), 1.0, vel=(0.0, 0.0, 1.0))
+        sb.sixdof(a, c, (0.5, 0, 0), (-0.5, 0, 0), kinds)
+        _run(sb.build(), 200, every=10)
+    rng = np.random.Generator(np.random.PCG64(9))
+    sb = scenes.SceneBuilder()
+    sb.halfspace((0.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+    for i in range(6):
+        sb.sphere((rng.uniform(-3, 3), 0.4, rng.uniform(-2, 2)), 0.4, 0.0)
+    for c in range(4):
+        prev = sb.box((-2.0 + 1.3 * c, 4.5, 0.0), (0.1, 0.1, 0.1), 0.0)
+        for k in range(8):
+            h = sb.box((-2.0 + 1.3 * c + 0.25 * k, 4.2 - 0.4 * k, 0.0), (0.08, 0.18, 0.08), float(rng.uniform(0.5, 2)))
+            # ragdoll-like: locked linear axes (one limited on every third
+            # link), random angular axes; fully random configurations of
+            # these chains blow up in the oracle and on the GPU alike (the
+            # two stay bitwise equal until NaNs appear)
+            kinds = [L_, L_, L_] + [int(x) for x in rng.integers(0, 3, 3)]
+            if k % 3 == 1:
+                kinds[int(rng.integers(0, 3))] = M_
+            lo = tuple(float(x) for x in rng.uniform(-0.4, -0.05, 6))
+            hi = tuple(float(x) for x in rng.uniform(0.05, 0.4, 6))
+            sb.sixdof(prev, h, (0.0, -0.18, 0.0), (0.0, 0.18, 0.0), kinds, lo, hi)
+            prev = h
+    g, o = _run(sb.build(), 200, every=10)
+    assert np.isfinite(g.state()).all()
