@@ -308,6 +308,20 @@ int oavbd_add_sixdof_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float l
 int oavbd_add_motor_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, const float linear_target[3],
                           const float angular_target[3], float max_force, float max_torque, uint32_t* out_index);
 
+/* Gear: holds coord(joint_a) + ratio * coord(joint_b) at its value when the
+ * gear is made, where coord is a Revolute joint's hinge twist past its
+ * assembly angle (unwrapped across +-pi step to step) or a Prismatic
+ * joint's slide past its assembly length.  One shared row over the bodies
+ * of both joints (up to four); a and b only name the gear's own pair (both
+ * are woken; their contacts stay).  EINVAL, before any wake, for a
+ * non-finite ratio, an unknown joint index or a joint that is neither
+ * Revolute nor Prismatic.  Replaces: PhysicsEngine::add_joint with
+ * JointKind::Gear (mod.rs:1329-1379; gear_sides / update_gear_mem,
+ * joints_ext.rs:38-161; rows assembly.rs:426-449, :650-679; dual
+ * joints_ext.rs:897-917). */
+int oavbd_add_gear_joint(oavbd_ctx* ctx, uint32_t a, uint32_t b, uint32_t joint_a, uint32_t joint_b, float ratio,
+                         uint32_t* out_index);
+
 /* Rope between a and b: anchors la, lb may come closer freely but never
  * part beyond max_distance (one-sided row, a velocity catch each step).
  * EINVAL, before any wake, unless max_distance is finite and positive.

@@ -179,7 +179,7 @@ struct Pair {
 // numbering): Gear is out of its scope.
 enum JointKindC : int {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_WHEEL = 7, JK_SIXDOF = 9, JK_MOTOR = 10
+  JK_WHEEL = 7, JK_GEAR = 8, JK_SIXDOF = 9, JK_MOTOR = 10
 };
 // AxisConfig (joint.rs:676-690) of a SixDof axis.
 enum AxisCfg : int { AX_FREE = 0, AX_LOCKED = 1, AX_LIMITED = 2 };
@@ -221,6 +221,13 @@ struct Joint {
   // AxisCfg and window, and the limited axes' force accumulators
   int six[6] = {AX_FREE, AX_FREE, AX_FREE, AX_FREE, AX_FREE, AX_FREE};
   T six_lo[6] = {}, six_hi[6] = {}, sacc[6] = {};
+  // Gear: the two Revolute / Prismatic joints it couples (ca + gratio cb =
+  // ref_val) and the step-start memory of their raw and continuous
+  // coordinates (joints_ext.rs:53-115), set once gmem
+  int gb[2] = {-1, -1};
+  T gratio = T(0);
+  bool gmem = false;
+  T graw[2] = {}, gcont[2] = {};
   T lam_l[3], pen_l[3];
   T lam_a[3], pen_a[3];
 };
@@ -1061,7 +1068,15 @@ struct Engine {
   // motor failing JointMotor::check (joint.rs:1101-1110), rejects before
   // the bodies wake (validate_joint runs first, mod.rs:1315).
   int add_joint(int kind, int a, int b, V3<T> la, V3<T> lb, V3<T> axa, V3<T> axb, const float* prm = nullptr) {
-    if (kind < JK_BALL || kind == 8 || kind > JK_MOTOR) return -1;
+    if (kind < JK_BALL || kind > JK_MOTOR) return -1;
+    if (kind == JK_GEAR) {  // mod.rs:1329-1365: a finite ratio over two existing Revolute / Prismatic joints
+      if (!prm || !std::isfinite(prm[2])) return -1;
+      for (int k = 0; k < 2; ++k) {
+        const int r = (int)prm[k];
+        if (!(prm[k] >= 0.0f) || r >= (int)joints.size()) return -1;
+        if (joints[r].kind != JK_REVOLUTE && joints[r].kind != JK_PRISMATIC) return -1;
+      }
+    }
     if (kind == JK_SIXDOF) {  // validate_joint (migration.rs:188-191): limited windows finite, lo <= hi
       if (!prm) return -1;
       for (int k = 0; k < 6; ++k) {
@@ -1177,6 +1192,11 @@ struct Engine {
       const Body<T>& A = bodies[a];
       const Body<T>& B = bodies[b];
       j.ref_val = length((B.pos + rotate(B.rot, lb)) - (A.pos + rotate(A.rot, la)));
+    } else if (kind == JK_GEAR) {  // the assembly constant from the current coordinates (mod.rs:1354-1378)
+      j.gb[0] = (int)prm[0];
+      j.gb[1] = (int)prm[1];
+      j.gratio = T(prm[2]);
+      j.ref_val = joint_coordinate(joints[j.gb[0]]) + j.gratio * joint_coordinate(joints[j.gb[1]]);
     } else if (kind == JK_ROPE) {
       j.ref_val = T(prm[0]);
     } else if (kind == JK_SPRING) {
@@ -1238,7 +1258,7 @@ struct Engine {
   void index_joints() {
     jointed.clear();
     for (const auto& j : joints)
-      if (j.kind != JK_FIXED && j.kind != JK_SIXDOF) jointed.insert(jkey(j.a, j.b));
+      if (j.kind != JK_FIXED && j.kind != JK_SIXDOF && j.kind != JK_GEAR) jointed.insert(jkey(j.a, j.b));
   }
   bool ball_jointed(int ia, int ib) const { return jointed.count(jkey(ia, ib)) != 0; }
 
@@ -1251,6 +1271,69 @@ struct Engine {
     return wrap_pi(T(2) * atan2_t(dot(V3<T>{q.x, q.y, q.z}, axis), q.w, math));
   }
   T hinge_twist(Q<T> qa, Q<T> qb, V3<T> axis) const { return quat_twist(quat_mul(conj(qa), qb), axis); }
+  // joint_coordinate (joints_ext.rs:38-51): a Revolute's twist past its
+  // assembly angle, a Prismatic's slide past its assembly length
+  T joint_coordinate(const Joint<T>& r) const {
+    const Body<T>& x = bodies[r.a];
+    const Body<T>& y = bodies[r.b];
+    if (r.kind == JK_REVOLUTE) return hinge_twist(x.rot, y.rot, r.ax_a) - r.ref_val;
+    const V3<T> wa = normalize_or(rotate(x.rot, r.ax_a), V3<T>{T(0), T(0), T(1)});
+    const V3<T> pa = x.pos + rotate(x.rot, r.la);
+    const V3<T> pb = y.pos + rotate(y.rot, r.lb);
+    return dot(pb - pa, wa) - r.ref_val;
+  }
+  // gear_sides (joints_ext.rs:120-161): side k of a gear, its coordinate
+  // unwrapped against the step-start memory on a hinge side
+  struct GearSide {
+    int a, b;
+    bool ang;
+    V3<T> axis, ra, rb;
+    T coord;
+  };
+  GearSide gear_side(const Joint<T>& g, int k) const {
+    const Joint<T>& r = joints[g.gb[k]];
+    const Body<T>& x = bodies[r.a];
+    const Body<T>& y = bodies[r.b];
+    GearSide s;
+    s.a = r.a;
+    s.b = r.b;
+    s.ang = r.kind == JK_REVOLUTE;
+    s.axis = normalize_or(rotate(x.rot, r.ax_a), V3<T>{T(0), T(0), T(1)});
+    s.ra = s.ang ? V3<T>{T(0), T(0), T(0)} : rotate(x.rot, r.la);
+    s.rb = s.ang ? V3<T>{T(0), T(0), T(0)} : rotate(y.rot, r.lb);
+    const T raw = joint_coordinate(r);
+    s.coord = (g.gmem && s.ang) ? g.gcont[k] + wrap_pi(raw - g.graw[k]) : raw;
+    return s;
+  }
+  T gear_error(const Joint<T>& g) const {
+    const GearSide sa = gear_side(g, 0), sb = gear_side(g, 1);
+    return sa.coord + g.gratio * sb.coord - g.ref_val;
+  }
+  // update_gear_mem (joints_ext.rs:56-115), once per step
+  void update_gear_mem() {
+    for (auto& g : joints) {
+      if (g.kind != JK_GEAR) continue;
+      for (int k = 0; k < 2; ++k) {
+        const Joint<T>& r = joints[g.gb[k]];
+        const T raw = joint_coordinate(r);
+        g.gcont[k] = (g.gmem && r.kind == JK_REVOLUTE) ? g.gcont[k] + wrap_pi(raw - g.graw[k]) : raw;
+        g.graw[k] = raw;
+      }
+      g.gmem = true;
+    }
+  }
+  // joint_members (sleep.rs:9-16): a gear's are its two joints' bodies
+  void joint_members(const Joint<T>& j, int m[4]) const {
+    if (j.kind == JK_GEAR) {
+      m[0] = joints[j.gb[0]].a;
+      m[1] = joints[j.gb[0]].b;
+      m[2] = joints[j.gb[1]].a;
+      m[3] = joints[j.gb[1]].b;
+    } else {
+      m[0] = m[2] = j.a;
+      m[1] = m[3] = j.b;
+    }
+  }
   // limit_state / warm_limit_state / regularized_limit (rows.rs:156-191):
   // 1 = lower bound, 0 = upper bound, -1 = free inside the window.
   static int limit_state(T v, T lo, T hi, T slop) {
@@ -1358,6 +1441,7 @@ struct Engine {
   }
 
   T joint_rest_error(const Joint<T>& j) const {
+    if (j.kind == JK_GEAR) return std::fabs(gear_error(j));
     const Body<T>& a = bodies[j.a];
     const Body<T>& b = bodies[j.b];
     V3<T> delta = (b.pos + rotate(b.rot, j.lb)) - (a.pos + rotate(a.rot, j.la));
@@ -1426,7 +1510,8 @@ struct Engine {
     if (j.kind == JK_MOTOR && ((j.max_force > T(0) && !(j.lin_target == V3<T>{T(0), T(0), T(0)})) ||
                                (j.max_torque > T(0) && !(j.ang_target == V3<T>{T(0), T(0), T(0)}))))
       return true;  // MotorDrive::keeps_awake (joint.rs:474-477)
-    int m[2] = {j.a, j.b};
+    int m[4];
+    joint_members(j, m);
     for (int h : m) {
       const Body<T>& b = bodies[h];
       if (b.type != BT_DYNAMIC && (b.pos != prev_pos[h] || b.rot != prev_rot[h])) return true;
@@ -1437,7 +1522,8 @@ struct Engine {
     for (;;) {
       bool changed = false;
       for (const auto& j : joints) {
-        int m[4] = {j.a, j.b, j.a, j.b};
+        int m[4];
+        joint_members(j, m);
         bool any = false;
         for (int h : m)
           if (bodies[h].type == BT_DYNAMIC && flags[h] == value) any = true;
@@ -1465,7 +1551,8 @@ struct Engine {
     }
     for (const auto& j : joints) {
       if (joint_driven(j) || joint_rest_error(j) > K<T>::JOINT_REST()) {
-        int m[4] = {j.a, j.b, j.a, j.b};
+        int m[4];
+        joint_members(j, m);
         for (int h : m)
           if (bodies[h].type == BT_DYNAMIC) awake[h] = 1;
       }
@@ -1535,6 +1622,7 @@ struct Engine {
   std::vector<int> joined_of(int h) const {
     std::vector<int> out;
     for (const auto& j : joints) {
+      if (j.kind == JK_GEAR) continue;  // mod.rs:903-905
       if (j.a == h)
         out.push_back(j.b);
       else if (j.b == h)
@@ -2370,6 +2458,23 @@ struct Engine {
         ~NextItem() { ++a.item; }
       } next_item{acc};
       const Joint<T>& j = joints[ji];
+      if (j.kind == JK_GEAR) {
+        // gear rows (assembly.rs:426-449, :650-679): C = ca + ratio cb -
+        // const, the slide sides stamped, then the hinge sides coupled
+        const GearSide sd[2] = {gear_side(j, 0), gear_side(j, 1)};
+        const T c = sd[0].coord + j.gratio * sd[1].coord - j.ref_val;
+        const T f = j.pen_l[0] * c + j.lam_l[0];
+        if (!row_live(c, f)) continue;
+        const T coef[2] = {T(1), j.gratio};
+        for (int k = 0; k < 2; ++k)
+          if (!sd[k].ang && (h == sd[k].a || h == sd[k].b))
+            acc.stamp(sd[k].axis, j.pen_l[0] * coef[k] * coef[k], f * coef[k], h == sd[k].a ? sd[k].ra : sd[k].rb,
+                      h == sd[k].a ? T(-1) : T(1));
+        for (int k = 0; k < 2; ++k)
+          if (sd[k].ang && (h == sd[k].a || h == sd[k].b))
+            acc.couple(h == sd[k].a ? -sd[k].axis : sd[k].axis, j.pen_l[0] * coef[k] * coef[k], f * coef[k]);
+        continue;
+      }
       bool is_a = j.a == h;
       T sign = is_a ? T(1) : T(-1);
       const Body<T>& a = bodies[j.a];
@@ -2707,6 +2812,16 @@ struct Engine {
       }
     }
     for (auto& j : joints) {
+      if (j.kind == JK_GEAR) {
+        // the gear pass (joints_ext.rs:897-917) reads only poses and the
+        // coupled joints' fixed data, so it may run in joint order here
+        const T c = gear_error(j);
+        if (std::fabs(c) >= K<T>::C_EPS()) {
+          j.lam_l[0] += j.pen_l[0] * c;
+          j.pen_l[0] = fmin_(j.pen_l[0] + K<T>::BETA() * std::fabs(c), K<T>::LIM_PEN_MAX());
+        }
+        continue;
+      }
       const Body<T>& a = bodies[j.a];
       const Body<T>& b = bodies[j.b];
       V3<T> pa = a.pos + rotate(a.rot, j.la);
@@ -2922,11 +3037,16 @@ struct Engine {
         adj[p.a].push_back(p.b);
         adj[p.b].push_back(p.a);
       }
-    for (auto& j : joints)
-      if (vert[j.a] && vert[j.b]) {
-        adj[j.a].push_back(j.b);
-        adj[j.b].push_back(j.a);
-      }
+    for (auto& j : joints) {
+      int m[4];
+      joint_members(j, m);  // a gear's rows read all four of its members
+      for (int x = 0; x < 4; ++x)
+        for (int y = x + 1; y < 4; ++y)
+          if (m[x] != m[y] && vert[m[x]] && vert[m[y]] && (j.kind == JK_GEAR || (x == 0 && y == 1))) {
+            adj[m[x]].push_back(m[y]);
+            adj[m[y]].push_back(m[x]);
+          }
+    }
     std::vector<int> orderv;
     for (int h = 0; h < n; ++h)
       if (vert[h]) orderv.push_back(h);
@@ -2963,8 +3083,12 @@ struct Engine {
       inc_pairs[pairs[i].b].push_back((int)i);
     }
     for (size_t i = 0; i < joints.size(); ++i) {
-      inc_joints[joints[i].a].push_back((int)i);
-      inc_joints[joints[i].b].push_back((int)i);
+      int m[4];
+      joint_members(joints[i], m);  // a gear joins each distinct member's rows
+      for (int x = 0; x < 4; ++x)
+        if ((x == 0 || m[x] != m[0]) && (x < 2 || m[x] != m[1]) && (x < 3 || m[x] != m[2]) &&
+            (joints[i].kind == JK_GEAR || x < 2))
+          inc_joints[m[x]].push_back((int)i);
     }
   }
 
@@ -3152,6 +3276,7 @@ struct Engine {
     const size_t event_start = contact_events.size();
     ccd_hits.clear();
     wake_joint_motion();
+    update_gear_mem();
     for (int h = 0; h < n; ++h) {
       if (!solvable(h)) continue;
       Body<T>& b = bodies[h];

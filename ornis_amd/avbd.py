@@ -109,6 +109,8 @@ SIGNATURES = {
     "oavbd_add_sixdof_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                               ctypes.POINTER(ctypes.c_uint32), _f32p, _f32p,
                                               ctypes.POINTER(ctypes.c_uint32)]),
+    "oavbd_add_gear_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_motor_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
                                              ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32)]),
     "oavbd_add_rope_joint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
@@ -348,6 +350,11 @@ class AvbdEngine:
             self._check(self._L.oavbd_add_sixdof_joint(
                 self._ctx, int(a), int(b), _p(la, _f32p), _p(lb, _f32p),
                 cfg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(lo, _f32p), _p(hi, _f32p), ctypes.byref(idx)))
+            self.njoints += 1
+            return idx.value
+        if int(kind) == 8 and prm is not None:  # Gear: prm = joint a, joint b, ratio
+            self._check(self._L.oavbd_add_gear_joint(self._ctx, int(a), int(b), int(prm[0]), int(prm[1]),
+                                                     ctypes.c_float(float(prm[2])), ctypes.byref(idx)))
             self.njoints += 1
             return idx.value
         if int(kind) == 10 and prm is not None:  # Motor: prm = linear target 3, angular target 3, max force, max torque

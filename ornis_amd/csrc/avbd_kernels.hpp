@@ -414,6 +414,7 @@ __device__ __forceinline__ int gather_buckets(const Dev& d, uint32_t sc, int lx,
 __device__ __forceinline__ bool joined_dev(const Dev& d, uint32_t h, uint32_t o) {
   for (uint32_t k = d.b.ji_off[h]; k < d.b.ji_off[h + 1]; ++k) {
     const uint32_t e = d.b.ji[k];
+    if (d.J.kind[e >> 1] == JK_GEAR) continue;  // mod.rs:903-905
     const uint2 ab = d.J.ab[e >> 1];
     if ((e & 1u) == 0u ? ab.y == o : ab.x == o) return true;
   }
@@ -1408,9 +1409,120 @@ __global__ void k_prop_clear(Dev d) {
   d.b.comp_any[h] = 0;
 }
 
+// Gear (joints_ext.rs:38-161): a gear j keeps its ratio and constant in
+// dref.xy and the indices of the two Revolute / Prismatic joints it couples
+// in dref.zw (bit patterns); sacc[2j] holds their step-start raw and
+// continuous coordinates, sacc[2j+1].x = 1 once those are set.
+struct GearSide {
+  uint32_t a, b;
+  bool ang;
+  f3 axis, ra, rb;
+  float coord;
+};
+struct PoseBodies {  // live poses from the body arrays
+  const Dev* d;
+  __device__ void operator()(uint32_t h, f3& p, qf& q) const {
+    p = xyz(d->b.pos[h]);
+    q = asq(d->b.rot[h]);
+  }
+};
+struct PoseRecords {  // live poses from the solver records
+  const float4* sb;
+  __device__ void operator()(uint32_t h, f3& p, qf& q) const {
+    p = xyz(sb[4 * (size_t)h]);
+    q = asq(sb[4 * (size_t)h + 1]);
+  }
+};
+__device__ __forceinline__ uint32_t gear_ref(const Dev& d, uint32_t j, int k) {
+  const float4 g = d.J.dref[j];
+  return __float_as_uint(k == 0 ? g.z : g.w);
+}
+// joint_members (sleep.rs:9-16): a gear's are its two joints' bodies
+__device__ __forceinline__ void joint_members_dev(const Dev& d, uint32_t j, uint32_t m[4]) {
+  if (d.J.kind[j] == JK_GEAR) {
+    const uint2 x = d.J.ab[gear_ref(d, j, 0)], y = d.J.ab[gear_ref(d, j, 1)];
+    m[0] = x.x; m[1] = x.y; m[2] = y.x; m[3] = y.y;
+  } else {
+    const uint2 ab = d.J.ab[j];
+    m[0] = m[2] = ab.x;
+    m[1] = m[3] = ab.y;
+  }
+}
+// joint_coordinate (joints_ext.rs:38-51) of joint r, with its gear-side
+// gradient data
+template <class Pose>
+__device__ __forceinline__ GearSide gear_side_raw(const Dev& d, uint32_t r, const Pose& pose) {
+  GearSide s;
+  const uint2 ab = d.J.ab[r];
+  s.a = ab.x;
+  s.b = ab.y;
+  s.ang = d.J.kind[r] == JK_REVOLUTE;
+  f3 pa, pb;
+  qf qa, qb;
+  pose(ab.x, pa, qa);
+  pose(ab.y, pb, qb);
+  const f3 ax = xyz(d.J.ax[2 * r]);
+  const float ref = d.J.dref[r].x;
+  s.axis = normalize_or(qrot(qa, ax), f3{0.0f, 0.0f, 1.0f});
+  if (s.ang) {
+    s.ra = s.rb = f3{0.0f, 0.0f, 0.0f};
+    s.coord = hinge_twist(qa, qb, ax) - ref;
+  } else {
+    s.ra = qrot(qa, xyz(d.J.la[r]));
+    s.rb = qrot(qb, xyz(d.J.lb[r]));
+    s.coord = dot3(sub(add(pb, s.rb), add(pa, s.ra)), s.axis) - ref;
+  }
+  return s;
+}
+// gear_sides (joints_ext.rs:120-161): a hinge side unwrapped against the
+// step-start memory
+template <class Pose>
+__device__ __forceinline__ GearSide gear_side(const Dev& d, uint32_t j, int k, const Pose& pose) {
+  GearSide s = gear_side_raw(d, gear_ref(d, j, k), pose);
+  if (s.ang && d.J.sacc[2 * j + 1].x != 0.0f) {
+    const float4 m = d.J.sacc[2 * j];
+    s.coord = (k == 0 ? m.z : m.w) + wrap_pi(s.coord - (k == 0 ? m.x : m.y));
+  }
+  return s;
+}
+template <class Pose>
+__device__ __forceinline__ float gear_error(const Dev& d, uint32_t j, const Pose& pose) {
+  const GearSide sa = gear_side(d, j, 0, pose), sb = gear_side(d, j, 1, pose);
+  const float4 g = d.J.dref[j];
+  return sa.coord + g.x * sb.coord - g.y;
+}
+// update_gear_mem (joints_ext.rs:56-115), once per step after the wake
+__global__ void k_gear_mem(Dev d) {
+  const uint32_t j = gid();
+  if (j >= d.njoints || d.J.kind[j] != JK_GEAR) return;
+  const PoseBodies pose{&d};
+  const GearSide s0 = gear_side_raw(d, gear_ref(d, j, 0), pose), s1 = gear_side_raw(d, gear_ref(d, j, 1), pose);
+  float4 m = d.J.sacc[2 * j];
+  const bool set = d.J.sacc[2 * j + 1].x != 0.0f;
+  m.z = (set && s0.ang) ? m.z + wrap_pi(s0.coord - m.x) : s0.coord;
+  m.w = (set && s1.ang) ? m.w + wrap_pi(s1.coord - m.y) : s1.coord;
+  m.x = s0.coord;
+  m.y = s1.coord;
+  d.J.sacc[2 * j] = m;
+  d.J.sacc[2 * j + 1] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ bool teleported(const Dev& d, uint32_t h) {  // a non-dynamic body moved by the host
+  if (d.b.type[h] == BT_DYNAMIC) return false;
+  const float4 p = d.b.pos[h], pp = d.b.prev_pos[h];
+  const float4 q = d.b.rot[h], pq = d.b.prev_rot[h];
+  return !(p.x == pp.x && p.y == pp.y && p.z == pp.z) || !(q.x == pq.x && q.y == pq.y && q.z == pq.z && q.w == pq.w);
+}
+
 // A joint driven (a non-dynamic member teleported) or off its rest pose
 // (joint_rest_error, sleep.rs:30-66).
 __device__ __forceinline__ bool joint_unsettled(const Dev& d, uint32_t j) {
+  if (d.J.kind[j] == JK_GEAR) {
+    uint32_t m[4];
+    joint_members_dev(d, j, m);
+    for (int i = 0; i < 4; ++i)
+      if (teleported(d, m[i])) return true;
+    return fabsf(gear_error(d, j, PoseBodies{&d})) > K_JOINT_REST;
+  }
   uint2 ab = d.J.ab[j];
   uint32_t m[2] = {ab.x, ab.y};
   for (int i = 0; i < 2; ++i) {
@@ -1510,9 +1622,10 @@ __global__ void k_awake_joints(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
   if (joint_unsettled(d, j)) {
-    uint2 ab = d.J.ab[j];
-    if (d.b.type[ab.x] == BT_DYNAMIC) d.b.flag_b[ab.x] = 1;
-    if (d.b.type[ab.y] == BT_DYNAMIC) d.b.flag_b[ab.y] = 1;
+    uint32_t m[4];
+    joint_members_dev(d, j, m);
+    for (int i = 0; i < 4; ++i)
+      if (d.b.type[m[i]] == BT_DYNAMIC) d.b.flag_b[m[i]] = 1;
   }
 }
 __device__ __forceinline__ void awake_apply_body(const Dev& d, uint32_t h) {
@@ -1777,10 +1890,18 @@ __global__ void __launch_bounds__(TPB) k_nbr_joint_count(Dev d) {
   uint32_t j = gid();
   uint32_t sides = 0;
   if (j < d.njoints) {
-    uint2 ab = d.J.ab[j];
-    bool sa = d.sw[ab.x], sb = d.sw[ab.y];
-    if (sa && sb) atomicAdd(&d.nbr_cnt[d.ckey[ab.y] > d.ckey[ab.x] ? ab.x : ab.y], 1u);
-    sides = (uint32_t)sa + (uint32_t)sb;
+    // a gear's rows read all four of its members: it links each two
+    uint32_t m[4];
+    joint_members_dev(d, j, m);
+    const bool gear = d.J.kind[j] == JK_GEAR;
+    for (int x = 0; x < 4; ++x) {
+      bool first = gear || x < 2;
+      for (int y = 0; y < x; ++y) first = first && m[y] != m[x];
+      sides += (first && d.sw[m[x]]) ? 1u : 0u;
+      for (int y = x + 1; y < 4; ++y)
+        if ((gear || (x == 0 && y == 1)) && m[x] != m[y] && d.sw[m[x]] && d.sw[m[y]])
+          atomicAdd(&d.nbr_cnt[d.ckey[m[y]] > d.ckey[m[x]] ? m[x] : m[y]], 1u);
+    }
   }
   block_add(&d.counters[CTR_JSIDES], sides);
 }
@@ -1802,8 +1923,12 @@ __global__ void k_inc_fill(Dev d) {
 __global__ void k_nbr_joint_fill(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
-  uint2 ab = d.J.ab[j];
-  if (d.sw[ab.x] && d.sw[ab.y]) nbr_link(d, ab);
+  uint32_t m[4];
+  joint_members_dev(d, j, m);
+  const bool gear = d.J.kind[j] == JK_GEAR;
+  for (int x = 0; x < 4; ++x)
+    for (int y = x + 1; y < 4; ++y)
+      if ((gear || (x == 0 && y == 1)) && m[x] != m[y] && d.sw[m[x]] && d.sw[m[y]]) nbr_link(d, make_uint2(m[x], m[y]));
 }
 // Body h's point-sides in ascending pair index: pairs where h is b (a < h),
 // then the contiguous run where h is a.  INC_LANES lanes per body: lane s
@@ -2349,11 +2474,41 @@ __device__ __forceinline__ f3 unit_axis(int i) {
 // (kind 0), the diagonal kind 2, angular rows couples (kind 1: x = (0, g),
 // its pen holding -K as the rolling rows do).
 template <class Emit>
-__device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit) {
+__device__ __forceinline__ void joint_rows(const Dev& d, uint32_t e, Emit&& emit, uint32_t h) {
   uint32_t j = e >> 1;
   bool is_a = (e & 1u) == 0u;
   float sign = is_a ? 1.0f : -1.0f;
   const int kind = d.J.kind[j];
+  if (kind == JK_GEAR) {
+    // Gear (assembly.rs:426-449, :650-679): C = ca + ratio cb - const over
+    // the coupled joints' bodies; body h's slide sides are stamped, then
+    // its hinge sides coupled
+    const PoseRecords pose{d.sb};
+    const GearSide sd[2] = {gear_side(d, j, 0, pose), gear_side(d, j, 1, pose)};
+    const float4 g = d.J.dref[j];
+    const float c = sd[0].coord + g.x * sd[1].coord - g.y;
+    const float pn = d.J.pen[j].x;
+    const float f = pn * c + d.J.lam[j].x;
+    if (!(fabsf(c) >= K_C_EPS || f != 0.0f)) return;
+    const float coef[2] = {1.0f, g.x};
+    Row row;
+    for (int k = 0; k < 2; ++k)
+      if (!sd[k].ang && (h == sd[k].a || h == sd[k].b)) {
+        make_stamp(row, sd[k].axis, pn * coef[k] * coef[k], f * coef[k], h == sd[k].a ? sd[k].ra : sd[k].rb,
+                   h == sd[k].a ? -1.0f : 1.0f);
+        emit(0, row);
+      }
+    for (int k = 0; k < 2; ++k)
+      if (sd[k].ang && (h == sd[k].a || h == sd[k].b)) {
+        const f3 gv = h == sd[k].a ? neg(sd[k].axis) : sd[k].axis;
+        row.x[0] = 0.0f; row.x[1] = 0.0f; row.x[2] = 0.0f;
+        row.x[3] = gv.x; row.x[4] = gv.y; row.x[5] = gv.z;
+        row.pen = -(pn * coef[k] * coef[k]);
+        row.f = f * coef[k];
+        emit(1, row);
+      }
+    return;
+  }
   uint2 ab = d.J.ab[j];
   const float4* sa = d.sb + 4 * (size_t)ab.x;
   const float4* sbb = d.sb + 4 * (size_t)ab.y;
@@ -2954,8 +3109,15 @@ __device__ __forceinline__ void primal_body(const Dev& d, uint4 cr, bool valid, 
     for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
          k += BODY_LANES) {
       const uint32_t e = d.b.ji[j0 + (k - np)];
-      wait(joint_partner_id(d, e));
-      joint_rows(d, e, add_row);
+      if (d.J.kind[e >> 1] == JK_GEAR) {
+        uint32_t m[4];
+        joint_members_dev(d, e >> 1, m);
+        for (int i = 0; i < 4; ++i)
+          if (m[i] != h) wait(m[i]);
+      } else {
+        wait(joint_partner_id(d, e));
+      }
+      joint_rows(d, e, add_row, h);
     }
   }
   float L21[21], rhs[6];
@@ -3174,7 +3336,7 @@ __global__ void __launch_bounds__(WAVE* W) k_primal_lds(Dev d, const uint4* __re
     const uint32_t total = np + nj;
     for (uint32_t k = np + (uint32_t)((slot - (int)(np % BODY_LANES) + BODY_LANES) % BODY_LANES); k < total;
          k += BODY_LANES)
-      joint_rows(d, d.b.ji[j0 + (k - np)], add_row);
+      joint_rows(d, d.b.ji[j0 + (k - np)], add_row, h);
   }
   float L21[21], rhs[6];
   const float4* inb = reinterpret_cast<const float4*>(d.inb + 28 * (size_t)h);
@@ -3342,6 +3504,17 @@ __global__ void k_dual_joint(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
   const int kind = d.J.kind[j];
+  if (kind == JK_GEAR) {  // the gear pass (joints_ext.rs:897-917)
+    const float c = gear_error(d, j, PoseRecords{d.sb});
+    if (fabsf(c) >= K_C_EPS) {
+      float4 lam = d.J.lam[j], pen = d.J.pen[j];
+      lam.x += pen.x * c;
+      pen.x = fmin_(pen.x + K_BETA * fabsf(c), K_LIM_PEN_MAX);
+      d.J.lam[j] = lam;
+      d.J.pen[j] = pen;
+    }
+    return;
+  }
   uint2 ab = d.J.ab[j];
   const float4* sa = d.sb + 4 * (size_t)ab.x;
   const float4* sbb = d.sb + 4 * (size_t)ab.y;
@@ -3615,9 +3788,9 @@ __global__ void k_ready_joints(Dev d) {
   uint32_t j = gid();
   if (j >= d.njoints) return;
   if (joint_unsettled(d, j)) {
-    uint2 ab = d.J.ab[j];
-    d.b.flag_b[ab.x] = 0;
-    d.b.flag_b[ab.y] = 0;
+    uint32_t m[4];
+    joint_members_dev(d, j, m);
+    for (int i = 0; i < 4; ++i) d.b.flag_b[m[i]] = 0;
   }
 }
 

@@ -142,7 +142,7 @@ struct Points {
 // Joint kinds (AvbdJointKind, mod.rs:286-300; the others are out of scope).
 enum {
   JK_BALL = 0, JK_REVOLUTE = 1, JK_PRISMATIC = 2, JK_FIXED = 3, JK_DISTANCE = 4, JK_ROPE = 5, JK_SPRING = 6,
-  JK_WHEEL = 7, JK_SIXDOF = 9, JK_MOTOR = 10  // the reference's AvbdJointKind numbering (Gear 8 is not built)
+  JK_WHEEL = 7, JK_GEAR = 8, JK_SIXDOF = 9, JK_MOTOR = 10  // the reference's AvbdJointKind numbering
 };
 __host__ __device__ inline bool is_tether(int kind) { return kind == JK_DISTANCE || kind == JK_ROPE || kind == JK_SPRING; }
 // SixDof axis configurations (AxisConfig), two bits per axis (linear x y z,

@@ -76,6 +76,7 @@ struct HostJoint {
   // angle / length, limit lo, hi, HINGE_* flags
   float prm[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   float mot[2] = {0.0f, 0.0f};  // Revolute / Prismatic / Wheel motor: target speed, budget
+  uint32_t gb[2] = {0u, 0u};    // Gear: the coupled joints (prm: ratio, constant)
   float sus[3] = {0.0f, 1.0f, 0.0f};  // Wheel: unit suspension axis in A's frame (axa / axb hold the axles)
   // SixDof: AX_* per axis (linear x y z, angular x y z) and the windows
   int six[6] = {0, 0, 0, 0, 0, 0};
@@ -191,6 +192,7 @@ struct oavbd_ctx {
   uint32_t post_seq = 0, colour_seq = 0;
   std::vector<HostJoint> joints;
   bool joints_dirty = false;
+  uint32_t ngears = 0;  // Gear joints in the table (their memory refreshes each step)
   BodyInputs inputs;  // the body table as last uploaded (set_bodies / add / remove)
   std::vector<int32_t> kind_h, type_h;
   float time_debt = 0.0f;
@@ -590,9 +592,32 @@ static uint32_t scan_total(oavbd_ctx* c, const uint32_t* in, uint32_t* out, uint
 // static joint structures: partner CSR (no-collide), incidence CSR (rows,
 // colouring), dynamic-joint components (sleep/wake propagation)
 // ---------------------------------------------------------------------------
+// The bodies whose rows joint j stamps, each once: a gear's are the
+// distinct bodies of the two joints it couples (joint_members,
+// sleep.rs:9-16), any other joint's its own two.
+static uint32_t joint_rows_members(const oavbd_ctx* c, uint32_t j, uint32_t m[4]) {
+  const HostJoint& h = c->joints[j];
+  if (h.kind != JK_GEAR) {
+    m[0] = h.a;
+    m[1] = h.b;
+    return 2;
+  }
+  const HostJoint& x = c->joints[h.gb[0]];
+  const HostJoint& y = c->joints[h.gb[1]];
+  const uint32_t all[4] = {x.a, x.b, y.a, y.b};
+  uint32_t nm = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    bool seen = false;
+    for (uint32_t l = 0; l < nm; ++l) seen = seen || m[l] == all[k];
+    if (!seen) m[nm++] = all[k];
+  }
+  return nm;
+}
+
 static void upload_joints(oavbd_ctx* c) {
   Dev& d = c->d;
   uint32_t n = d.n, nj = (uint32_t)c->joints.size();
+  uint32_t ngears = 0;
   // joints already on the device keep their duals (a new joint does not
   // reset the others' warm starts: add_joint only pushes, mod.rs:1308-1510)
   const uint32_t kept = std::min(d.njoints, nj);
@@ -647,6 +672,13 @@ static void upload_joints(oavbd_ctx* c) {
     sacc[2 * j + 1] = j < kept ? kept_sacc[2 * j + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     dref[j] = (is_tether(h.kind) || hinge) ? make_float4(h.prm[0], h.prm[1], h.prm[2], h.prm[3])
                                            : make_float4(h.dref[0], h.dref[1], h.dref[2], 0.0f);
+    if (h.kind == JK_GEAR) {  // ratio, constant, the coupled joints' indices as bit patterns
+      float g0, g1;
+      std::memcpy(&g0, &h.gb[0], sizeof g0);
+      std::memcpy(&g1, &h.gb[1], sizeof g1);
+      dref[j] = make_float4(h.prm[0], h.prm[1], g0, g1);
+      ++ngears;
+    }
     if (h.kind == JK_ROPE) ropes.push_back(j);
     if ((hinge && ((uint32_t)h.prm[3] & HINGE_MOTOR)) || h.kind == JK_MOTOR) motors.push_back(j);
     qref[j] = hinge ? make_float4(h.mot[0], h.mot[1], 0.0f, 0.0f) : make_float4(h.qref[0], h.qref[1], h.qref[2], h.qref[3]);
@@ -654,12 +686,13 @@ static void upload_joints(oavbd_ctx* c) {
     pen[j] = j < kept ? kept_pen[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     lama[j] = j < kept ? kept_lama[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     pena[j] = j < kept ? kept_pena[j] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    if (!fixed && h.kind != JK_SIXDOF) {  // weld-like: their contacts stay
+    if (!fixed && h.kind != JK_SIXDOF && h.kind != JK_GEAR) {  // weld-like and geared: their contacts stay
       jp_cnt[h.a]++;
       jp_cnt[h.b]++;
     }
-    ji_cnt[h.a]++;
-    ji_cnt[h.b]++;
+    uint32_t m[4];
+    const uint32_t nm = joint_rows_members(c, j, m);
+    for (uint32_t k = 0; k < nm; ++k) ji_cnt[m[k]]++;
   }
   std::vector<uint32_t> jp_off(n + 1, 0), ji_off(n + 1, 0);
   for (uint32_t i = 0; i < n; ++i) {
@@ -669,12 +702,13 @@ static void upload_joints(oavbd_ctx* c) {
   std::vector<uint32_t> jp(jp_off[n]), ji(ji_off[n]), pcur(n, 0), icur(n, 0);
   for (uint32_t j = 0; j < nj; ++j) {  // joint order per body = ascending joint index
     const HostJoint& h = c->joints[j];
-    if (h.kind != JK_FIXED && h.kind != JK_SIXDOF) {
+    if (h.kind != JK_FIXED && h.kind != JK_SIXDOF && h.kind != JK_GEAR) {
       jp[jp_off[h.a] + pcur[h.a]++] = h.b;
       jp[jp_off[h.b] + pcur[h.b]++] = h.a;
     }
-    ji[ji_off[h.a] + icur[h.a]++] = (j << 1) | 0u;
-    ji[ji_off[h.b] + icur[h.b]++] = (j << 1) | 1u;
+    uint32_t m[4];
+    const uint32_t nm = joint_rows_members(c, j, m);  // a gear's sides carry bit 0 clear
+    for (uint32_t k = 0; k < nm; ++k) ji[ji_off[m[k]] + icur[m[k]]++] = (j << 1) | (h.kind == JK_GEAR ? 0u : k);
   }
   // components of the graph of joints between two dynamic bodies
   std::vector<uint32_t> parent(n);
@@ -685,10 +719,14 @@ static void upload_joints(oavbd_ctx* c) {
   };
   for (uint32_t j = 0; j < nj; ++j) {
     const HostJoint& h = c->joints[j];
-    if (c->type_h[h.a] == BT_DYNAMIC && c->type_h[h.b] == BT_DYNAMIC) {
-      uint32_t x = find(h.a), y = find(h.b);
-      if (x != y) parent[std::max(x, y)] = std::min(x, y);
-    }
+    uint32_t m[4];
+    const uint32_t nm = joint_rows_members(c, j, m);  // a gear joins all its dynamic members
+    for (uint32_t k = 1; k < nm; ++k)
+      for (uint32_t l = 0; l < k; ++l)
+        if (c->type_h[m[k]] == BT_DYNAMIC && c->type_h[m[l]] == BT_DYNAMIC) {
+          uint32_t x = find(m[k]), y = find(m[l]);
+          if (x != y) parent[std::max(x, y)] = std::min(x, y);
+        }
   }
   std::vector<uint32_t> comp(n);
   for (uint32_t i = 0; i < n; ++i) comp[i] = find(i);
@@ -707,6 +745,7 @@ static void upload_joints(oavbd_ctx* c) {
   if (!motors.empty()) dalloc(&d.motors, motors.size());
   up(d.motors, motors);
   d.nmotors = (uint32_t)motors.size();
+  c->ngears = ngears;
   HIPC(hipDeviceSynchronize());  // null-stream uploads land before the context stream reads them
   d.njoints = nj;
   c->joints_dirty = false;
@@ -1410,6 +1449,7 @@ static void step_inner(oavbd_ctx* c, bool finish, bool async = false) {
     k_wake_begin<<<blocks(n), TPB, 0, c->st>>>(d, may_sleep);
   } else {
     if (may_sleep) wake_joint_motion(c);
+    if (c->ngears) k_gear_mem<<<blocks(d.njoints), TPB, 0, c->st>>>(d);
     k_begin<<<blocks(n), TPB, 0, c->st>>>(d);
     if (d.nmotors || d.nropes) {  // motors, then the rope catch: after the torques, before the TOI gate
       k_joint_impulses<<<1, 64, 0, c->st>>>(d);
@@ -2500,6 +2540,52 @@ int oavbd_add_joint_limited(oavbd_ctx* c, uint32_t kind, uint32_t a, uint32_t b,
   const float flags = (float)((limit ? HINGE_LIMIT : 0) | (motor ? HINGE_MOTOR : 0));
   const float prm[4] = {0.0f, limit ? limit[0] : 0.0f, limit ? limit[1] : 0.0f, flags};
   return add_joint_impl(c, kind, a, b, la, lb, axis_a, axis_b, prm, out_index, motor);
+}
+
+int oavbd_add_gear_joint(oavbd_ctx* c, uint32_t a, uint32_t b, uint32_t joint_a, uint32_t joint_b, float ratio,
+                         uint32_t* out_index) {
+  if (!c) return OAVBD_EINVAL;
+  // mod.rs:1329-1353, before any wake: a finite ratio over two existing
+  // Revolute / Prismatic joints
+  if (!std::isfinite(ratio)) return fail(c, OAVBD_EINVAL, "gear ratio must be finite");
+  const uint32_t nj = (uint32_t)c->joints.size();
+  if (joint_a >= nj || joint_b >= nj) return fail(c, OAVBD_EINVAL, "gear references an unknown joint");
+  for (uint32_t r : {joint_a, joint_b}) {
+    const int k = c->joints[r].kind;
+    if (k != JK_REVOLUTE && k != JK_PRISMATIC)
+      return fail(c, OAVBD_EINVAL, "gear must couple revolute / prismatic joints");
+  }
+  const float zero[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t idx = 0;
+  const int rc = add_joint_impl(c, JK_GEAR, a, b, zero, zero, nullptr, nullptr, nullptr, &idx);
+  if (rc != OAVBD_OK) return rc;
+  // the constant from the coupled joints' current coordinates
+  // (joint_coordinate, joints_ext.rs:38-51; mod.rs:1354-1378)
+  auto coordinate = [&](const HostJoint& r) -> float {
+    float4 p[2], q[2];
+    GUARD(c, {
+      HIPC(hipMemcpy(&p[0], c->d.b.pos + r.a, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&p[1], c->d.b.pos + r.b, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&q[0], c->d.b.rot + r.a, sizeof(float4), hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(&q[1], c->d.b.rot + r.b, sizeof(float4), hipMemcpyDeviceToHost));
+    });
+    const qf qa = asq(q[0]), qb = asq(q[1]);
+    const f3 ax{r.axa[0], r.axa[1], r.axa[2]};
+    if (r.kind == JK_REVOLUTE) return hinge_twist(qa, qb, ax) - r.prm[0];
+    const f3 wa = normalize_or(qrot(qa, ax), f3{0.0f, 0.0f, 1.0f});
+    const f3 pa = add(xyz(p[0]), qrot(qa, f3{r.la[0], r.la[1], r.la[2]}));
+    const f3 pb = add(xyz(p[1]), qrot(qb, f3{r.lb[0], r.lb[1], r.lb[2]}));
+    return dot3(sub(pb, pa), wa) - r.prm[0];
+  };
+  const float ca = coordinate(c->joints[joint_a]);
+  const float cb = coordinate(c->joints[joint_b]);
+  HostJoint& j = c->joints[idx];
+  j.gb[0] = joint_a;
+  j.gb[1] = joint_b;
+  j.prm[0] = ratio;
+  j.prm[1] = ca + ratio * cb;
+  if (out_index) *out_index = idx;
+  return OAVBD_OK;
 }
 
 int oavbd_add_motor_joint(oavbd_ctx* c, uint32_t a, uint32_t b, const float linear_target[3],

@@ -29,7 +29,7 @@ _FIELDS3 = ("shape_param", "pos", "vel", "angvel")
 # joint kinds (include/oavbd.h)
 JOINT_BALL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_FIXED = 0, 1, 2, 3
 JOINT_DISTANCE, JOINT_ROPE, JOINT_SPRING, JOINT_WHEEL = 4, 5, 6, 7
-JOINT_SIXDOF, JOINT_MOTOR = 9, 10
+JOINT_GEAR, JOINT_SIXDOF, JOINT_MOTOR = 8, 9, 10
 # SixDof axis configurations
 AXIS_FREE, AXIS_LOCKED, AXIS_LIMITED = 0, 1, 2
 # motor kinds and models of a Spring joint (oavbd_motor)
@@ -126,6 +126,13 @@ class SceneBuilder:
         self.joints.append((int(a), int(b), tuple(la), tuple(lb), JOINT_SIXDOF, (1.0, 0.0, 0.0), (1.0, 0.0, 0.0),
                             tuple(float(k) for k in kinds) + tuple(float(x) for x in lo) +
                             tuple(float(x) for x in hi)))
+        return len(self.joints) - 1
+
+    def gear(self, a, b, joint_a, joint_b, ratio):
+        """Gear joint: holds coord(joint_a) + ratio * coord(joint_b) at its
+        value at creation (Revolute: hinge twist, Prismatic: slide)."""
+        self.joints.append((int(a), int(b), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), JOINT_GEAR, (1.0, 0.0, 0.0),
+                            (1.0, 0.0, 0.0), (float(joint_a), float(joint_b), float(ratio))))
         return len(self.joints) - 1
 
     def motor(self, a, b, linear_target, angular_target, max_force, max_torque):

@@ -129,7 +129,7 @@ def test_joint_spec_validation(gpu_available):
     from ornis_amd.avbd import AvbdError
     for args in ((0, 0, (0, 0, 0), (0, 0, 0), 1, (0, 0, 1), (0, 0, 1)),       # self joint
                  (0, 5, (0, 0, 0), (0, 0, 0), 0),                          # body out of range
-                 (0, 1, (0, 0, 0), (0, 0, 0), 8),                          # kind not built (Gear)
+                 (0, 1, (0, 0, 0), (0, 0, 0), 8),                          # Gear needs its own entry
                  (0, 1, (0, 0, 0), (0, 0, 0), 11),                         # unknown kind
                  (0, 1, (0, 0, 0), (0, 0, 0), 1, (0, 0, 0), (0, 0, 1)),    # degenerate hinge axis
                  (0, 1, (0, 0, 0), (0, 0, 0), 2, (0, 0, 1), (1e-7, 0, 0))):  # degenerate slide axis
@@ -438,3 +438,56 @@ def test_sixdof_scenes_bitwise(gpu_available):
             prev = h
     g, o = _run(sb.build(), 200, every=10)
     assert np.isfinite(g.state()).all()
+
+
+def four_member_gears(n=6, seed=31):
+    """Pairs of mechanisms under gravity, released from horizontal: a
+    two-link pendulum (ground hinge, then a hinge between the links) and an
+    arm on a ground hinge carrying a slider on a limited slide.  The gear
+    couples the links' hinge to the slide, so its row reaches four distinct
+    dynamic bodies (its colouring links all of them)."""
+    rng = np.random.default_rng(seed)
+    b = scenes.SceneBuilder()
+    g = b.box((0.0, -1.0, 0.0), (30.0, 0.5, 30.0), 0.0)
+    R, P = scenes.JOINT_REVOLUTE, scenes.JOINT_PRISMATIC
+    for i in range(n):
+        x = -12.0 + 4.0 * i
+        l1 = b.box((x + 0.5, 5.5, 0.0), (0.5, 0.1, 0.1), 1.0)
+        l2 = b.box((x + 1.5, 5.5, 0.0), (0.5, 0.1, 0.1), 1.0)
+        b.joint(R, g, l1, (x, 5.5, 0.0), (-0.5, 0, 0), (0, 0, 1), (0, 0, 1))
+        j1 = b.joint(R, l1, l2, (0.5, 0, 0), (-0.5, 0, 0), (0, 0, 1), (0, 0, 1))
+        arm = b.box((x + 0.5, 3.0, 0.6), (0.5, 0.1, 0.1), 1.0)
+        slider = b.sphere((x + 1.0, 3.0, 0.6), 0.15, 1.0)
+        b.joint(R, g, arm, (x, 3.0, 0.6), (-0.5, 0, 0), (0, 0, 1), (0, 0, 1))
+        j3 = b.joint(P, arm, slider, (0.5, 0, 0), (0, 0, 0), (1, 0, 0), (1, 0, 0), limit=(-0.4, 0.4))
+        b.gear(l2, slider, j1, j3, float(rng.uniform(-2.0, 2.0)))
+    return b.build()
+
+
+def test_gear_scenes_bitwise(gpu_available):
+    """The reference's gear scenes (avbd_engine.rs:1360-1450, :824-890,
+    without the host teleport) and pendulum pairs whose gears reach four
+    bodies: every pose, pair, colour and joint dual bit-identical."""
+    import test_oracle_gear as og
+    sb, _, _ = og.hinge_pair_scene()
+    _run(sb.build(), 180, every=20)
+    sb, _ = og.slider_pair_scene()
+    _run(sb.build(), 240, every=20)
+    print("gear: four-member pendulum pairs", flush=True)
+    _run(four_member_gears(), 240, every=20)
+
+
+def test_gear_spec_validation(gpu_available):
+    from ornis_amd.avbd import AvbdError
+    b = scenes.SceneBuilder()
+    g = b.box((0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 0.0)
+    a = b.box((0.0, 1.0, 0.0), (0.1, 0.1, 0.1), 1.0)
+    c = b.box((1.0, 1.0, 0.0), (0.1, 0.1, 0.1), 1.0)
+    e = AvbdEngine(scene=b.build())
+    ja = e.add_joint(g, a, (0, 0, 0), (0, -1, 0), scenes.JOINT_REVOLUTE, (0, 0, 1), (0, 0, 1))
+    jball = e.add_joint(g, c, (1, 0, 0), (0, -1, 0), 0)
+    z = (0.0, 0.0, 0.0)
+    for prm in ((ja, 7, 1.0), (ja, ja, float("nan")), (ja, jball, 1.0)):
+        with pytest.raises(AvbdError):
+            e.add_joint(a, c, z, z, scenes.JOINT_GEAR, prm=prm)
+    assert e.add_joint(a, c, z, z, scenes.JOINT_GEAR, prm=(ja, ja, 1.0)) == 2
